@@ -1,0 +1,151 @@
+"""Synthetic structured meshes handed to the solver as unstructured connectivity.
+
+The reference reads CGNS meshes (src/Geom/Mesh/Mesh_Serial_ReadFromCGNS.cpp); its
+periodic test meshes (IV10_10, Uniform32_3D_Periodic) are uniform quad / hex grids,
+which are rebuilt here directly.  Periodic faces are stored once: the face geometry
+lives in the frame of its back cell (``face2cell[:, 0]``), and ``faceShift`` is the
+translation that carries a face point into the frame of the front cell -- the role
+of ``periodicInfo.TransCoord/TransCoordBack`` in src/CFV/FiniteVolume.hpp:283-317.
+
+Node order and face-normal conventions:
+  * Quad4 / Hex8 nodes follow the CGNS order.
+  * 2-D face normal = (t_y, -t_x) for the edge tangent t (FacialJacobianToNormVec,
+    src/Geom/Geometric.hpp:113-122); 3-D normal = d x/d xi  x  d x/d eta.
+    Faces are emitted so that this normal points from the back cell to the front cell.
+  * cell2face follows the element face order: Quad4 (y-, x+, y+, x-),
+    Hex8 (z-, y-, x+, y+, x-, z+).
+Cells are numbered lexicographically (x fastest); face ``d*N + c`` is the +d face
+of cell ``c``, so consecutive faces of one direction touch consecutive cells.
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass
+
+import numpy as np
+
+LINE2, QUAD4, HEX8, TRI3, TET4 = 1, 2, 3, 4, 5
+
+
+@dataclass
+class Mesh:
+    dim: int
+    coords: np.ndarray        # [nNode, 3] float64
+    cellType: np.ndarray      # [nCell] int32
+    cell2nodePtr: np.ndarray  # [nCell+1] int64
+    cell2node: np.ndarray     # int64
+    cell2facePtr: np.ndarray  # [nCell+1] int64
+    cell2face: np.ndarray     # int64
+    faceType: np.ndarray      # [nFace] int32
+    face2nodePtr: np.ndarray  # [nFace+1] int64
+    face2node: np.ndarray     # int64
+    face2cell: np.ndarray     # [nFace, 2] int64, -1 = boundary
+    faceShift: np.ndarray     # [nFace, 3] float64
+    shape: tuple = ()
+
+    @property
+    def nCell(self) -> int:
+        return int(self.cellType.shape[0])
+
+    @property
+    def nFace(self) -> int:
+        return int(self.faceType.shape[0])
+
+    @property
+    def nNode(self) -> int:
+        return int(self.coords.shape[0])
+
+
+def _jitter_nodes(X: np.ndarray, h: np.ndarray, jitter: float, seed: int) -> np.ndarray:
+    """Periodic-consistent node perturbation: node (n) and node (n + N_d) move together."""
+    if jitter == 0.0:
+        return X
+    rng = np.random.default_rng(seed)
+    sh = X.shape[:-1]
+    dim = len(sh)
+    core = tuple(s - 1 for s in sh)
+    d = rng.uniform(-jitter, jitter, size=core + (3,)) * np.concatenate([h, np.zeros(3 - dim)])
+    if dim == 2:
+        d[..., 2] = 0.0
+    idx = np.meshgrid(*[np.arange(s) % c for s, c in zip(sh, core)], indexing="ij")
+    return X + d[tuple(idx)]
+
+
+def periodic_quad_grid(nx: int, ny: int, Lx: float = 10.0, Ly: float = 10.0,
+                       x0: float = 0.0, y0: float = 0.0, jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """Doubly periodic nx x ny Quad4 grid on [x0, x0+Lx] x [y0, y0+Ly]."""
+    assert nx >= 2 and ny >= 2
+    hx, hy = Lx / nx, Ly / ny
+    I, J = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, np.zeros_like(I, dtype=float)], axis=-1).astype(np.float64)
+    X = _jitter_nodes(X, np.array([hx, hy]), jitter, seed)
+    nid = lambda i, j: i * (ny + 1) + j  # noqa: E731
+    coords = X.reshape(-1, 3)
+    N = nx * ny
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ci, cj = ci.T.ravel(), cj.T.ravel()          # cell c = i + nx*j
+    cell2node = np.stack([nid(ci, cj), nid(ci + 1, cj), nid(ci + 1, cj + 1), nid(ci, cj + 1)], 1)
+    c = np.arange(N)
+    # x-faces (+x of cell c), tangent (0,+1) -> normal +x
+    fx_nodes = np.stack([nid(ci + 1, cj), nid(ci + 1, cj + 1)], 1)
+    fx_R = ((ci + 1) % nx) + nx * cj
+    fx_shift = np.zeros((N, 3))
+    fx_shift[ci == nx - 1, 0] = -Lx
+    # y-faces (+y of cell c), tangent (-1,0) -> normal +y
+    fy_nodes = np.stack([nid(ci + 1, cj + 1), nid(ci, cj + 1)], 1)
+    fy_R = ci + nx * ((cj + 1) % ny)
+    fy_shift = np.zeros((N, 3))
+    fy_shift[cj == ny - 1, 1] = -Ly
+    face2node = np.concatenate([fx_nodes, fy_nodes])
+    face2cell = np.stack([np.concatenate([c, c]), np.concatenate([fx_R, fy_R])], 1)
+    faceShift = np.concatenate([fx_shift, fy_shift])
+    xm = ((ci - 1) % nx) + nx * cj
+    ym = ci + nx * ((cj - 1) % ny)
+    cell2face = np.stack([N + ym, c, N + c, xm], 1)   # y-, x+, y+, x-
+    return Mesh(2, coords, np.full(N, QUAD4, np.int32), np.arange(N + 1, dtype=np.int64) * 4,
+                cell2node.ravel().astype(np.int64), np.arange(N + 1, dtype=np.int64) * 4,
+                cell2face.ravel().astype(np.int64), np.full(2 * N, LINE2, np.int32),
+                np.arange(2 * N + 1, dtype=np.int64) * 2, face2node.ravel().astype(np.int64),
+                face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny))
+
+
+def periodic_hex_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.0, Lz: float = 1.0,
+                      x0: float = 0.0, y0: float = 0.0, z0: float = 0.0,
+                      jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """Triply periodic nx x ny x nz Hex8 grid on [x0,x0+Lx] x [y0,y0+Ly] x [z0,z0+Lz]."""
+    assert nx >= 2 and ny >= 2 and nz >= 2
+    hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, z0 + K * hz], axis=-1).astype(np.float64)
+    X = _jitter_nodes(X, np.array([hx, hy, hz]), jitter, seed)
+    coords = X.reshape(-1, 3)
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k  # noqa: E731
+    N = nx * ny * nz
+    c = np.arange(N)
+    ci = c % nx
+    cj = (c // nx) % ny
+    ck = c // (nx * ny)
+    cell2node = np.stack([nid(ci, cj, ck), nid(ci + 1, cj, ck), nid(ci + 1, cj + 1, ck), nid(ci, cj + 1, ck),
+                          nid(ci, cj, ck + 1), nid(ci + 1, cj, ck + 1), nid(ci + 1, cj + 1, ck + 1),
+                          nid(ci, cj + 1, ck + 1)], 1)
+    cid = lambda i, j, k: (i % nx) + nx * ((j % ny) + ny * (k % nz))  # noqa: E731
+    # +x face: a=(x1,y0,z0) b=(x1,y1,z0) c=(x1,y1,z1) d=(x1,y0,z1): (+y) x (+z) = +x
+    fx = np.stack([nid(ci + 1, cj, ck), nid(ci + 1, cj + 1, ck), nid(ci + 1, cj + 1, ck + 1), nid(ci + 1, cj, ck + 1)], 1)
+    # +y face: a=(x0,y1,z0) b=(x0,y1,z1) c=(x1,y1,z1) d=(x1,y1,z0): (+z) x (+x) = +y
+    fy = np.stack([nid(ci, cj + 1, ck), nid(ci, cj + 1, ck + 1), nid(ci + 1, cj + 1, ck + 1), nid(ci + 1, cj + 1, ck)], 1)
+    # +z face: a=(x0,y0,z1) b=(x1,y0,z1) c=(x1,y1,z1) d=(x0,y1,z1): (+x) x (+y) = +z
+    fz = np.stack([nid(ci, cj, ck + 1), nid(ci + 1, cj, ck + 1), nid(ci + 1, cj + 1, ck + 1), nid(ci, cj + 1, ck + 1)], 1)
+    sx = np.zeros((N, 3)); sx[ci == nx - 1, 0] = -Lx
+    sy = np.zeros((N, 3)); sy[cj == ny - 1, 1] = -Ly
+    sz = np.zeros((N, 3)); sz[ck == nz - 1, 2] = -Lz
+    face2node = np.concatenate([fx, fy, fz])
+    R = np.concatenate([cid(ci + 1, cj, ck), cid(ci, cj + 1, ck), cid(ci, cj, ck + 1)])
+    face2cell = np.stack([np.concatenate([c, c, c]), R], 1)
+    faceShift = np.concatenate([sx, sy, sz])
+    xm, ym, zm = cid(ci - 1, cj, ck), cid(ci, cj - 1, ck), cid(ci, cj, ck - 1)
+    cell2face = np.stack([2 * N + zm, N + ym, c, N + c, xm, 2 * N + c], 1)  # z-, y-, x+, y+, x-, z+
+    F = 3 * N
+    return Mesh(3, coords, np.full(N, HEX8, np.int32), np.arange(N + 1, dtype=np.int64) * 8,
+                cell2node.ravel().astype(np.int64), np.arange(N + 1, dtype=np.int64) * 6,
+                cell2face.ravel().astype(np.int64), np.full(F, QUAD4, np.int32),
+                np.arange(F + 1, dtype=np.int64) * 4, face2node.ravel().astype(np.int64),
+                face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny, nz))

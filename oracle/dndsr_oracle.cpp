@@ -1,0 +1,1745 @@
+// dndsr_oracle.cpp -- CPU oracle for the compact-FV residual / reconstruction / LU-SGS path.
+//
+// TEST INFRASTRUCTURE ONLY: a plain C++17 FP64 restatement of the reference
+// (CFDLAB-THU/DNDSR) algorithms that the HIP path is checked against.  It is
+// loaded only by tests/, __graft_entry__.smoke() and bench.py's cpu_baseline
+// leg; the product library never uses it.  Each function cites the reference
+// file:line it restates.  Loops keep the reference's order (faces in index
+// order, quadrature points in rule order, cell2face in element order) so that
+// summation order matches the reference's serial build.
+//
+// Pinning (tests/test_oracle_*.py):
+//   Riemann fluxes      : reference golden vectors test/cpp/Euler/test_RiemannSolvers.cpp:257-267,
+//                         consistency / symmetry / supersonic-upwind closed forms.
+//   metrics, VR setup,  : reference VR error goldens test/cpp/CFV/test_Reconstruction.cpp:531-545
+//   VR iteration          and test_Reconstruction3D.cpp:393-409 (uniform periodic grids),
+//                         constant / linear exactness, dense brute-force fixed point.
+//   EvaluateDt, RHS(p0),: reference pipeline golden test/cpp/Euler/test_EulerEvaluator.cpp:231-259
+//   LUSGSMatrixInit       (IV 10x10, rho-residual L1 and increment L2).
+//   RHS(high order)     : conservation (sum V rhs = 0), free-stream preservation, and the
+//                         p0 limit that the pipeline golden pins.
+//   UpdateSGS (GS)      : dense (D+L+U) brute force on a tiny mesh, FD check of the flux
+//                         Jacobian increment.
+//   SSP-RK3             : restated from src/Solver/ODE.hpp:1477-1520; pinned by the RHS pins
+//                         plus the closed-form RK3 combination (parity unpinned beyond that).
+#include "dndsr_oracle.h"
+
+#include <algorithm>
+#include <array>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+namespace orc
+{
+using real = double;
+using idx_t = int64_t;
+static constexpr real verySmallReal = 1e-200; // src/DNDS/Defines.hpp:199
+static constexpr real smallReal = 1e-10;      // src/DNDS/Defines.hpp:201
+static constexpr real pi = 3.14159265358979323846;
+
+static thread_local std::string g_err;
+
+struct Pt
+{
+    real x[3]{0, 0, 0};
+};
+
+// ---------------------------------------------------------------------------
+// Quadrature.  Gauss-Legendre tables exactly as the reference stores them
+// (15 significant digits), src/Geom/Quadratures/Line.hpp:25-42; tensor-product
+// point order src/Geom/QuadratureHub.hpp:318-373 (first parametric index fastest);
+// order -> scheme map src/Geom/QuadratureHub.hpp:39-96.
+// ---------------------------------------------------------------------------
+static const real GL1p[1] = {0}, GL1w[1] = {2};
+static const real GL2p[2] = {-0.577350269189626, 0.577350269189626}, GL2w[2] = {1, 1};
+static const real GL3p[3] = {-0.774596669241483, 0, 0.774596669241483},
+                  GL3w[3] = {0.555555555555555, 0.888888888888889, 0.555555555555555};
+static const real GL4p[4] = {-0.861136311594053, -0.339981043584856, 0.339981043584856, 0.861136311594053},
+                  GL4w[4] = {0.347854845137454, 0.652145154862546, 0.652145154862546, 0.347854845137454};
+
+static int lineNPoint(int order)
+{
+    if (order <= 1)
+        return 1;
+    if (order <= 3)
+        return 2;
+    if (order <= 5)
+        return 3;
+    if (order == 6)
+        return 4;
+    throw std::runtime_error("unsupported integration order");
+}
+static void glTable(int n, const real *&p, const real *&w)
+{
+    switch (n)
+    {
+    case 1: p = GL1p, w = GL1w; break;
+    case 2: p = GL2p, w = GL2w; break;
+    case 3: p = GL3p, w = GL3w; break;
+    case 4: p = GL4p, w = GL4w; break;
+    default: throw std::runtime_error("bad GL size");
+    }
+}
+struct QP
+{
+    real p[3];
+    real w;
+};
+static int elemParamDim(int t)
+{
+    switch (t)
+    {
+    case ORC_LINE2: return 1;
+    case ORC_QUAD4: return 2;
+    case ORC_HEX8: return 3;
+    default: throw std::runtime_error("element type not supported by the oracle");
+    }
+}
+static std::vector<QP> quadRule(int elemType, int order)
+{
+    int pd = elemParamDim(elemType);
+    int n = lineNPoint(order);
+    const real *p, *w;
+    glTable(n, p, w);
+    std::vector<QP> r;
+    int tot = pd == 1 ? n : (pd == 2 ? n * n : n * n * n);
+    for (int iG = 0; iG < tot; iG++)
+    {
+        QP q{{0, 0, 0}, 1};
+        int i = iG % n, j = (iG / n) % n, k = iG / (n * n);
+        q.p[0] = p[i];
+        q.w = w[i];
+        if (pd >= 2)
+            q.p[1] = p[j], q.w = w[i] * w[j];
+        if (pd == 3)
+            q.p[2] = p[k], q.w = w[i] * w[j] * w[k];
+        r.push_back(q);
+    }
+    return r;
+}
+static const int INT_ORDER_MAX = 6; // src/Geom/Quadratures/QuadratureBase.hpp:70
+
+// ---------------------------------------------------------------------------
+// Linear shape functions (Line2, Quad4, Hex8; CGNS node order).
+// ---------------------------------------------------------------------------
+static int elemNNode(int t)
+{
+    switch (t)
+    {
+    case ORC_LINE2: return 2;
+    case ORC_QUAD4: return 4;
+    case ORC_HEX8: return 8;
+    default: throw std::runtime_error("bad element");
+    }
+}
+static void shape(int t, const real *xi, real *N, real (*dN)[3])
+{
+    static const real q4[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
+    static const real h8[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1},
+                                  {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
+    switch (t)
+    {
+    case ORC_LINE2:
+        N[0] = 0.5 * (1 - xi[0]), N[1] = 0.5 * (1 + xi[0]);
+        dN[0][0] = -0.5, dN[1][0] = 0.5;
+        break;
+    case ORC_QUAD4:
+        for (int a = 0; a < 4; a++)
+        {
+            N[a] = 0.25 * (1 + q4[a][0] * xi[0]) * (1 + q4[a][1] * xi[1]);
+            dN[a][0] = 0.25 * q4[a][0] * (1 + q4[a][1] * xi[1]);
+            dN[a][1] = 0.25 * q4[a][1] * (1 + q4[a][0] * xi[0]);
+        }
+        break;
+    case ORC_HEX8:
+        for (int a = 0; a < 8; a++)
+        {
+            real fx = 1 + h8[a][0] * xi[0], fy = 1 + h8[a][1] * xi[1], fz = 1 + h8[a][2] * xi[2];
+            N[a] = 0.125 * fx * fy * fz;
+            dN[a][0] = 0.125 * h8[a][0] * fy * fz;
+            dN[a][1] = 0.125 * h8[a][1] * fx * fz;
+            dN[a][2] = 0.125 * h8[a][2] * fx * fy;
+        }
+        break;
+    default: throw std::runtime_error("bad element");
+    }
+}
+struct GeoPt
+{
+    real x[3];
+    real J[3][3]; // J[r][c] = d x_r / d xi_c
+};
+static GeoPt mapPoint(int t, const std::vector<Pt> &nodes, const real *xi)
+{
+    real N[8], dN[8][3];
+    shape(t, xi, N, dN);
+    GeoPt g{};
+    int nn = elemNNode(t), pd = elemParamDim(t);
+    for (int a = 0; a < nn; a++)
+        for (int r = 0; r < 3; r++)
+        {
+            g.x[r] += N[a] * nodes[a].x[r];
+            for (int c = 0; c < pd; c++)
+                g.J[r][c] += dN[a][c] * nodes[a].x[r];
+        }
+    return g;
+}
+// cell Jacobian determinant (CellJacobianDet), face Jacobian determinant (FaceJacobianDet)
+// and face normal (FacialJacobianToNormVec, src/Geom/Geometric.hpp:113-130)
+static real cellJDet(int dim, const GeoPt &g)
+{
+    if (dim == 2)
+        return g.J[0][0] * g.J[1][1] - g.J[0][1] * g.J[1][0];
+    return g.J[0][0] * (g.J[1][1] * g.J[2][2] - g.J[1][2] * g.J[2][1]) -
+           g.J[0][1] * (g.J[1][0] * g.J[2][2] - g.J[1][2] * g.J[2][0]) +
+           g.J[0][2] * (g.J[1][0] * g.J[2][1] - g.J[1][1] * g.J[2][0]);
+}
+static void faceNormVec(int dim, const GeoPt &g, real *n)
+{
+    if (dim == 2)
+    {
+        n[0] = g.J[1][0], n[1] = -g.J[0][0], n[2] = 0;
+        return;
+    }
+    n[0] = g.J[1][0] * g.J[2][1] - g.J[2][0] * g.J[1][1];
+    n[1] = g.J[2][0] * g.J[0][1] - g.J[0][0] * g.J[2][1];
+    n[2] = g.J[0][0] * g.J[1][1] - g.J[1][0] * g.J[0][1];
+}
+static real norm3(const real *v) { return std::sqrt(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]); }
+
+// ---------------------------------------------------------------------------
+// Taylor basis (src/Geom/BaseFunction.hpp:53-140, 284-438, 441-712): scaled
+// monomials about the cell centre, derivative rows in diffOperatorOrderList order.
+// ---------------------------------------------------------------------------
+static const int ord3D[20][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0}, {0, 2, 1}, {0, 1, 2}, {1, 0, 2}, {2, 0, 1}, {1, 1, 1}};
+static const int ord2D[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {3, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 3, 0}};
+static const int dFact[4][4] = {{1, 0, 0, 0}, {1, 1, 0, 0}, {1, 2, 2, 0}, {1, 3, 6, 6}};
+static const int nComb2D[10] = {1, 1, 1, 1, 2, 1, 1, 3, 3, 1}; // BaseFunction.hpp:216-217
+static const int nComb3D[20] = {1, 1, 1, 1, 1, 1, 1, 2, 2, 2, 1, 1, 1, 3, 3, 3, 3, 3, 3, 6};
+static const int factorials[10] = {1, 1, 2, 6, 24, 120, 720, 5040, 40320, 362880};
+
+static int nDofOf(int dim, int order)
+{
+    static const int d2[4] = {1, 3, 6, 10}, d3[4] = {1, 4, 10, 20};
+    return dim == 2 ? d2[order] : d3[order];
+}
+static real iPow(int p, real x)
+{
+    switch (p)
+    {
+    case 0: return 1.;
+    case 1: return x;
+    case 2: return x * x;
+    case 3: return x * x * x;
+    default: return 1e300;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Gas dynamics (src/Euler/Gas.hpp, src/DNDS/IdealGasPhysics.hpp:37-44)
+// Physics is always 3 velocity components (model NS / NS_3D, src/Euler/Euler.hpp).
+// ---------------------------------------------------------------------------
+static inline real sqr(real x) { return x * x; }
+static inline real signTol(real a, real tol) { return a > tol ? 1 : (a < -tol ? -1 : 0); } // Defines.hpp:543
+static inline real signP(real a) { return a >= 0 ? 1 : -1; }                                // Defines.hpp:549
+static inline void thermal(real E, real rho, real vSqr, real gamma, real &p, real &asqr, real &H)
+{
+    p = (gamma - 1) * (E - rho * 0.5 * vSqr);
+    asqr = gamma * p / rho;
+    H = (E + p) / rho;
+}
+static inline real dot3(const real *a, const real *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+
+// GasInviscidFlux_XY, Gas.hpp:390-397 (vg = 0 in every configuration here)
+static void fluxXY(const real *U, const real *velo, const real *n, real p, real *F)
+{
+    real vn = dot3(velo, n);
+    for (int i = 0; i < 5; i++)
+        F[i] = U[i] * vn;
+    for (int d = 0; d < 3; d++)
+        F[1 + d] += p * n[d];
+    F[4] += vn * p;
+}
+
+struct RoePre
+{
+    real veloLm[3], veloRm[3], asqrLm, asqrRm, pLm, pRm, HLm, HRm, vsqrLm, vsqrRm;
+    real veloRoe[3], sqrtRhoLm, sqrtRhoRm, vsqrRoe, HRoe, asqrRoe, rhoRoe, aRoe;
+};
+// ComputeRoePreamble, Gas.hpp:229-259
+static RoePre roePreamble(const real *ULm, const real *URm, real gamma)
+{
+    RoePre r;
+    for (int d = 0; d < 3; d++)
+        r.veloLm[d] = ULm[1 + d] / ULm[0], r.veloRm[d] = URm[1 + d] / URm[0];
+    r.vsqrLm = dot3(r.veloLm, r.veloLm);
+    r.vsqrRm = dot3(r.veloRm, r.veloRm);
+    thermal(ULm[4], ULm[0], r.vsqrLm, gamma, r.pLm, r.asqrLm, r.HLm);
+    thermal(URm[4], URm[0], r.vsqrRm, gamma, r.pRm, r.asqrRm, r.HRm);
+    r.sqrtRhoLm = std::sqrt(ULm[0]);
+    r.sqrtRhoRm = std::sqrt(URm[0]);
+    for (int d = 0; d < 3; d++)
+        r.veloRoe[d] = (r.sqrtRhoLm * r.veloLm[d] + r.sqrtRhoRm * r.veloRm[d]) / (r.sqrtRhoLm + r.sqrtRhoRm);
+    r.vsqrRoe = dot3(r.veloRoe, r.veloRoe);
+    r.HRoe = (r.sqrtRhoLm * r.HLm + r.sqrtRhoRm * r.HRm) / (r.sqrtRhoLm + r.sqrtRhoRm);
+    r.asqrRoe = (gamma - 1) * (r.HRoe - 0.5 * r.vsqrRoe);
+    r.rhoRoe = r.sqrtRhoLm * r.sqrtRhoRm;
+    if (!(r.asqrRoe > 0))
+        throw std::runtime_error("asqrRoe <= 0");
+    r.aRoe = std::sqrt(r.asqrRoe);
+    return r;
+}
+
+// Roe_EntropyFixer, Gas.hpp:878-1007
+static void roeEntropyFix(int eig, real aL, real aR, real aAve, real uL, real uR, real uAve,
+                          real VL, real VR, real VAve, real dLambda, real fixScale, real incFScale,
+                          real &lam0, real &lam123, real &lam4)
+{
+    (void)VL, (void)VR;
+    const real scaleHartenYee = 0.05 * fixScale, scaleLD = 0.2 * fixScale, scaleHFix = 0.25 * fixScale;
+    const real incFScaleA = 1.0;
+    if (eig == 0)
+    {
+        auto Flim = [=](real v, real lam, real lamL, real lamR) {
+            const real thresH = std::max({0., lam - lamL, lamR - lam}) * scaleHartenYee;
+            if (v < thresH)
+                return (sqr(v) / thresH + thresH) * 0.5;
+            return v;
+        };
+        lam0 = Flim(lam0 * incFScaleA, uAve - aAve, uL - aL, uR - aR);
+        lam123 = Flim(lam123 * incFScale, uAve, uL, uR);
+        lam4 = Flim(lam4 * incFScaleA, uAve + aAve, uL + aL, uR + aR);
+    }
+    else if (eig == 1)
+    {
+        const real aLm = aL * incFScaleA, aRm = aR * incFScaleA;
+        const real uLm = signTol(uL, aLm * smallReal) * std::max(std::abs(uL) * incFScale, aLm * scaleLD);
+        const real uRm = signTol(uR, aRm * smallReal) * std::max(std::abs(uR) * incFScale, aRm * scaleLD);
+        lam0 = std::max(std::abs(uLm - aLm), std::abs(uRm - aRm));
+        lam123 = std::max(std::abs(uLm), std::abs(uRm));
+        lam4 = std::max(std::abs(uLm + aLm), std::abs(uRm + aRm));
+    }
+    else if (eig == 3)
+    {
+        const real LDthreshold = std::abs(uAve) / scaleLD;
+        const real aRoeStar = std::min(LDthreshold, aAve * incFScaleA);
+        lam0 = std::abs(uAve * incFScale - aRoeStar);
+        lam4 = std::abs(uAve * incFScale + aRoeStar);
+    }
+    else if (eig == 4)
+    {
+        const real uStar = signTol(uAve, aAve * smallReal) * std::max(aAve * scaleLD, std::abs(uAve) * incFScale);
+        lam0 = std::abs(uStar - aAve * incFScaleA);
+        lam123 = std::abs(uStar);
+        lam4 = std::abs(uStar + aAve * incFScaleA);
+    }
+    else if (eig == 5)
+    {
+        const real veloLm0 = uL * incFScale, veloRm0 = uR * incFScale;
+        const real aLm = std::min(aL * incFScaleA, std::abs(veloLm0) / scaleLD);
+        const real aRm = std::min(aR * incFScaleA, std::abs(veloRm0) / scaleLD);
+        lam0 = std::max(std::abs(veloLm0 - aLm), std::abs(veloRm0 - aRm));
+        lam123 = std::max(std::abs(veloLm0), std::abs(veloRm0));
+        lam4 = std::max(std::abs(veloLm0 + aLm), std::abs(veloRm0 + aRm));
+    }
+    else if (eig == 6 || eig == 8)
+    {
+        lam0 = std::max(std::abs(uAve * incFScale - aAve * incFScaleA), dLambda * scaleHFix);
+        lam4 = std::max(std::abs(uAve * incFScale + aAve * incFScaleA), dLambda * scaleHFix);
+        lam123 = std::max(std::abs(uAve * incFScale), dLambda * scaleHFix);
+    }
+    else if (eig == 7)
+    {
+        lam0 *= std::abs(uAve * incFScale - aAve * incFScaleA);
+        lam4 *= std::abs(uAve * incFScale + aAve * incFScaleA);
+        lam123 *= std::abs(uAve * incFScale);
+    }
+    else
+        throw std::runtime_error("bad eigScheme");
+    if (eig == 7 || eig == 8)
+    {
+        const real th = scaleHartenYee * (VAve + aAve), thS = sqr(th);
+        if (lam0 < th)
+            lam0 = (sqr(lam0) + thS) / (2 * th);
+        if (lam4 < th)
+            lam4 = (sqr(lam4) + thS) / (2 * th);
+        if (lam123 < th)
+            lam123 = (sqr(lam123) + thS) / (2 * th);
+    }
+}
+
+// Roe dissipation |A| dU with Roe eigenvectors (Gas.hpp:1091-1125, 1209-1243)
+static void roeIncF(const real *incU, const real *n, const real *veloRoe, real vsqrRoe, real aRoe,
+                    real asqrRoe, real HRoe, real lam0, real lam123, real lam4, real gamma, real *incF)
+{
+    real veloRoeN = dot3(veloRoe, n);
+    real incU123N = incU[1] * n[0] + incU[2] * n[1] + incU[3] * n[2];
+    real a23V[3], a23VT[3];
+    for (int d = 0; d < 3; d++)
+        a23V[d] = incU[1 + d] - incU[0] * veloRoe[d];
+    real a23Vn = dot3(a23V, n);
+    for (int d = 0; d < 3; d++)
+        a23VT[d] = a23V[d] - n[d] * a23Vn;
+    real incU4b = incU[4] - dot3(a23VT, veloRoe);
+    real alpha1 = (gamma - 1) / asqrRoe * (incU[0] * (HRoe - veloRoeN * veloRoeN) + veloRoeN * incU123N - incU4b);
+    real alpha0 = (incU[0] * (veloRoeN + aRoe) - incU123N - aRoe * alpha1) / (2 * aRoe);
+    real alpha4 = incU[0] - (alpha0 + alpha1);
+    alpha0 *= lam0, alpha1 *= lam123, alpha4 *= lam4;
+    for (int d = 0; d < 3; d++)
+        a23VT[d] *= lam123;
+    incF[0] = alpha0 + alpha1 + alpha4;
+    incF[4] = (HRoe - veloRoeN * aRoe) * alpha0 + 0.5 * vsqrRoe * alpha1 + (HRoe + veloRoeN * aRoe) * alpha4 + dot3(a23VT, veloRoe);
+    for (int d = 0; d < 3; d++)
+        incF[1 + d] = (veloRoe[d] - aRoe * n[d]) * alpha0 + (veloRoe[d] + aRoe * n[d]) * alpha4 + veloRoe[d] * alpha1 + a23VT[d];
+}
+
+// RoeFlux_IdealGas_HartenYee<dim, eigScheme>, Gas.hpp:1038-1126 (vg = 0)
+static void roeFlux(int eig, const real *UL, const real *UR, const real *ULm, const real *URm,
+                    const real *n, real gamma, real *F, real dLambda, real fixScale, real incFScale,
+                    real &lam0, real &lam123, real &lam4)
+{
+    if (!(UL[0] > 0 && UR[0] > 0))
+        throw std::runtime_error("rho <= 0 in Roe flux");
+    real veloL[3], veloR[3];
+    for (int d = 0; d < 3; d++)
+        veloL[d] = UL[1 + d] / UL[0], veloR[d] = UR[1 + d] / UR[0];
+    real asqrL, asqrR, pL, pR, HL, HR;
+    thermal(UL[4], UL[0], dot3(veloL, veloL), gamma, pL, asqrL, HL);
+    thermal(UR[4], UR[0], dot3(veloR, veloR), gamma, pR, asqrR, HR);
+    RoePre rp = roePreamble(ULm, URm, gamma);
+    real FL[5], FR[5];
+    fluxXY(UL, veloL, n, pL, FL);
+    fluxXY(UR, veloR, n, pR, FR);
+    real veloRoeN = dot3(rp.veloRoe, n);
+    real veloRoe0 = veloRoeN;
+    lam0 = std::abs(veloRoe0 - rp.aRoe);
+    lam123 = std::abs(veloRoe0);
+    lam4 = std::abs(veloRoe0 + rp.aRoe);
+    real veloLm0 = dot3(rp.veloLm, n), veloRm0 = dot3(rp.veloRm, n);
+    if (eig == 2)
+    {
+        lam0 = lam123 = lam4 = std::max(std::abs(veloLm0) + std::sqrt(rp.asqrLm), std::abs(veloRm0) + std::sqrt(rp.asqrRm));
+        for (int i = 0; i < 5; i++)
+            F[i] = (FL[i] + FR[i]) * 0.5 - 0.5 * lam0 * (UR[i] - UL[i]);
+        return;
+    }
+    roeEntropyFix(eig, std::sqrt(rp.asqrLm), std::sqrt(rp.asqrRm), rp.aRoe, veloLm0, veloRm0, veloRoe0,
+                  norm3(rp.veloLm), norm3(rp.veloRm), norm3(rp.veloRoe), dLambda, fixScale, incFScale,
+                  lam0, lam123, lam4);
+    real incU[5], incF[5];
+    for (int i = 0; i < 5; i++)
+        incU[i] = UR[i] - UL[i];
+    roeIncF(incU, n, rp.veloRoe, rp.vsqrRoe, rp.aRoe, rp.asqrRoe, rp.HRoe, lam0, lam123, lam4, gamma, incF);
+    for (int i = 0; i < 5; i++)
+        F[i] = (FL[i] + FR[i]) * 0.5 - 0.5 * incF[i];
+}
+
+// HLLEPFlux_IdealGas<dim, type>, Gas.hpp:601-714 (vg = 0)
+static void hllepFlux(int type, const real *UL, const real *UR, const real *ULm, const real *URm,
+                      const real *n, real gamma, real *F, real dLambda, real fixScale,
+                      real &lam0, real &lam123, real &lam4)
+{
+    if (!(UL[0] > 0 && UR[0] > 0))
+        throw std::runtime_error("rho <= 0 in HLLEP flux");
+    real veloL[3], veloR[3];
+    for (int d = 0; d < 3; d++)
+        veloL[d] = UL[1 + d] / UL[0], veloR[d] = UR[1 + d] / UR[0];
+    real asqrL, asqrR, pL, pR, HL, HR;
+    thermal(UL[4], UL[0], dot3(veloL, veloL), gamma, pL, asqrL, HL);
+    thermal(UR[4], UR[0], dot3(veloR, veloR), gamma, pR, asqrR, HR);
+    RoePre rp = roePreamble(ULm, URm, gamma);
+    real FL[5], FR[5];
+    fluxXY(UL, veloL, n, pL, FL);
+    fluxXY(UR, veloR, n, pR, FR);
+    real veloRoeN = dot3(rp.veloRoe, n);
+    real veloRoe0 = veloRoeN;
+    lam0 = std::abs(veloRoe0 - rp.aRoe);
+    lam123 = std::abs(veloRoe0);
+    lam4 = std::abs(veloRoe0 + rp.aRoe);
+    real veloLm0 = dot3(rp.veloLm, n), veloRm0 = dot3(rp.veloRm, n);
+    real incU[5];
+    for (int i = 0; i < 5; i++)
+        incU[i] = UR[i] - UL[i];
+    real incU123N = incU[1] * n[0] + incU[2] * n[1] + incU[3] * n[2];
+    real a23V[3], a23VT[3];
+    for (int d = 0; d < 3; d++)
+        a23V[d] = incU[1 + d] - incU[0] * rp.veloRoe[d];
+    real a23Vn = dot3(a23V, n);
+    for (int d = 0; d < 3; d++)
+        a23VT[d] = a23V[d] - n[d] * a23Vn;
+    real incU4b = incU[4] - dot3(a23VT, rp.veloRoe);
+    real alpha1 = (gamma - 1) / rp.asqrRoe * (incU[0] * (rp.HRoe - veloRoeN * veloRoeN) + veloRoeN * incU123N - incU4b);
+    real alpha0 = (incU[0] * (veloRoeN + rp.aRoe) - incU123N - rp.aRoe * alpha1) / (2 * rp.aRoe);
+    real alpha4 = incU[0] - (alpha0 + alpha1);
+    real SL = std::min(veloRoe0 - rp.aRoe, veloLm0 - std::sqrt(asqrL));
+    real SR = std::max(veloRoe0 + rp.aRoe, veloRm0 + std::sqrt(asqrR));
+    real UU = std::abs(veloRoe0);
+    real dfix = rp.aRoe / (rp.aRoe + std::max(UU, dLambda * fixScale));
+    real SP = std::max(SR, 0.0), SM = std::min(SL, 0.0);
+    if (type != 1)
+    {
+        real ReV1[5] = {1, rp.veloRoe[0], rp.veloRoe[1], rp.veloRoe[2], rp.vsqrRoe * 0.5};
+        real div = SP - SM;
+        div += signP(div) * verySmallReal;
+        for (int i = 0; i < 5; i++)
+            F[i] = (SP * FL[i] - SM * FR[i]) / div + (SP * SM / div) * (UR[i] - UL[i] - dfix * ReV1[i] * alpha1);
+    }
+    else
+    {
+        real delta1 = dfix, delta2 = 0.0, delta3 = 0.0;
+        lam123 = ((SP + SM) * (veloRoe0)-2.0 * (1.0 - delta1) * (SP * SM)) / (SP - SM);
+        lam4 = ((SP + SM) * (veloRoe0 + rp.aRoe) - 2.0 * (1.0 - delta2) * (SP * SM)) / (SP - SM);
+        lam0 = ((SP + SM) * (veloRoe0 - rp.aRoe) - 2.0 * (1.0 - delta3) * (SP * SM)) / (SP - SM);
+        lam123 = std::abs(lam123), lam0 = std::abs(lam0), lam4 = std::abs(lam4);
+        real incF[5];
+        roeIncF(incU, n, rp.veloRoe, rp.vsqrRoe, rp.aRoe, rp.asqrRoe, rp.HRoe, lam0, lam123, lam4, gamma, incF);
+        for (int i = 0; i < 5; i++)
+            F[i] = (FL[i] + FR[i]) * 0.5 - 0.5 * incF[i];
+    }
+}
+
+// HLLCFlux_IdealGas_HartenYee, Gas.hpp:733-843 (vg = 0)
+static void hllcFlux(const real *UL, const real *UR, const real *ULm, const real *URm,
+                     const real *n, real gamma, real *F, real &lam0, real &lam123, real &lam4)
+{
+    if (!(UL[0] > 0 && UR[0] > 0))
+        throw std::runtime_error("rho <= 0 in HLLC flux");
+    real veloL[3], veloR[3];
+    for (int d = 0; d < 3; d++)
+        veloL[d] = UL[1 + d] / UL[0], veloR[d] = UR[1 + d] / UR[0];
+    real asqrL, asqrR, pL, pR, HL, HR;
+    thermal(UL[4], UL[0], dot3(veloL, veloL), gamma, pL, asqrL, HL);
+    thermal(UR[4], UR[0], dot3(veloR, veloR), gamma, pR, asqrR, HR);
+    RoePre rp = roePreamble(ULm, URm, gamma);
+    real FL[5], FR[5];
+    fluxXY(UL, veloL, n, pL, FL);
+    fluxXY(UR, veloR, n, pR, FR);
+    real veloRoe0 = dot3(rp.veloRoe, n);
+    lam0 = std::abs(veloRoe0 - rp.aRoe);
+    lam123 = std::abs(veloRoe0);
+    lam4 = std::abs(veloRoe0 + rp.aRoe);
+    real veloLm0 = dot3(rp.veloLm, n), veloRm0 = dot3(rp.veloRm, n);
+    auto HLLCq = [&](real p, real pS) {
+        real q = std::sqrt(1 + (gamma + 1) / 2 / gamma * (pS / p - 1));
+        if (pS <= p)
+            q = 1;
+        return q;
+    };
+    real pS = 0.5 * (rp.pLm + rp.pRm) - 0.5 * (veloLm0 - veloRm0) * rp.rhoRoe * rp.aRoe;
+    pS = std::max(0.0, pS);
+    real SL = veloLm0 - std::sqrt(rp.asqrLm) * HLLCq(rp.pLm, pS);
+    real SR = veloRm0 + std::sqrt(rp.asqrRm) * HLLCq(rp.pRm, pS);
+    if (0 <= SL)
+    {
+        std::copy(FL, FL + 5, F);
+        return;
+    }
+    if (SR <= 0)
+    {
+        std::copy(FR, FR + 5, F);
+        return;
+    }
+    real SS = 0;
+    real div = (UL[0] * (SL - veloLm0) - UR[0] * (SR - veloRm0));
+    if (std::abs(div) > verySmallReal)
+        SS = (pR - pL + (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) * (SL - veloLm0) -
+              (UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2]) * (SR - veloRm0)) /
+             div;
+    real DS[5] = {0, n[0], n[1], n[2], SS};
+    if (SS >= 0)
+    {
+        real dv = SL - SS;
+        if (std::abs(dv) < verySmallReal)
+            std::copy(FL, FL + 5, F);
+        else
+            for (int i = 0; i < 5; i++)
+                F[i] = ((UL[i] * SL - FL[i]) * SS + DS[i] * ((pL + UL[0] * (SL - veloLm0) * (SS - veloLm0)) * SL)) / dv;
+    }
+    else
+    {
+        real dv = SR - SS;
+        if (std::abs(dv) < verySmallReal)
+            std::copy(FR, FR + 5, F);
+        else
+            for (int i = 0; i < 5; i++)
+                F[i] = ((UR[i] * SR - FR[i]) * SS + DS[i] * ((pR + UR[0] * (SR - veloRm0) * (SS - veloRm0)) * SR)) / dv;
+    }
+}
+
+// InviscidFlux_IdealGas_Dispatcher, Gas.hpp:1434-1489
+static void dispatchFlux(int rsType, const real *UL, const real *UR, const real *ULm, const real *URm,
+                         const real *n, real gamma, real *F, real dLambda, real fixScale, real incFScale,
+                         real &lam0, real &lam123, real &lam4)
+{
+    if (rsType == 1)
+        roeFlux(0, UL, UR, ULm, URm, n, gamma, F, dLambda, fixScale, incFScale, lam0, lam123, lam4);
+    else if (rsType >= 11 && rsType <= 18)
+        roeFlux(rsType - 10, UL, UR, ULm, URm, n, gamma, F, dLambda, fixScale, incFScale, lam0, lam123, lam4);
+    else if (rsType == 3)
+        hllepFlux(0, UL, UR, ULm, URm, n, gamma, F, dLambda, fixScale, lam0, lam123, lam4);
+    else if (rsType == 21)
+        hllepFlux(1, UL, UR, ULm, URm, n, gamma, F, dLambda, fixScale, lam0, lam123, lam4);
+    else if (rsType == 2)
+        hllcFlux(UL, UR, ULm, URm, n, gamma, F, lam0, lam123, lam4);
+    else
+        throw std::runtime_error("the rs type is invalid");
+}
+
+// RoeFlux_IdealGas_HartenYee_Batch, Gas.hpp:1272-1407: eigenvalues from the cell-mean
+// Roe average on the mean normal nm, shared by all points of the face (vg = 0).
+static void roeFluxBatch(int eig, int nB, const real (*UL)[5], const real (*UR)[5], const real *ULm,
+                         const real *URm, const real (*n)[3], const real *nm, real gamma, real (*F)[5],
+                         real dLambda, real fixScale, real incFScale, real &lam0, real &lam123, real &lam4)
+{
+    for (int iB = 0; iB < nB; iB++)
+        if (!(UL[iB][0] > 0 && UR[iB][0] > 0))
+            throw std::runtime_error("rho <= 0 in batch Roe flux");
+    real veloLm[3], veloRm[3];
+    for (int d = 0; d < 3; d++)
+        veloLm[d] = ULm[1 + d] / ULm[0], veloRm[d] = URm[1 + d] / URm[0];
+    real asqrLm, asqrRm, pLm, pRm, HLm, HRm;
+    thermal(ULm[4], ULm[0], dot3(veloLm, veloLm), gamma, pLm, asqrLm, HLm);
+    thermal(URm[4], URm[0], dot3(veloRm, veloRm), gamma, pRm, asqrRm, HRm);
+    real sqrtRhoLm = std::sqrt(ULm[0]), sqrtRhoRm = std::sqrt(URm[0]);
+    real veloRoe[3];
+    for (int d = 0; d < 3; d++)
+        veloRoe[d] = (sqrtRhoLm * veloLm[d] + sqrtRhoRm * veloRm[d]) / (sqrtRhoLm + sqrtRhoRm);
+    real vsqrRoe = dot3(veloRoe, veloRoe);
+    real HRoe = (sqrtRhoLm * HLm + sqrtRhoRm * HRm) / (sqrtRhoLm + sqrtRhoRm);
+    real asqrRoe = (gamma - 1) * (HRoe - 0.5 * vsqrRoe);
+    real veloRoeN = dot3(veloRoe, nm);
+    real veloRoeRN = veloRoeN;
+    real veloLm0 = dot3(veloLm, nm), veloRm0 = dot3(veloRm, nm);
+    if (!(asqrRoe > 0))
+        throw std::runtime_error("asqrRoe <= 0");
+    real aRoe = std::sqrt(asqrRoe);
+    lam0 = std::abs(veloRoeRN - aRoe);
+    lam123 = std::abs(veloRoeRN);
+    lam4 = std::abs(veloRoeRN + aRoe);
+    std::vector<std::array<real, 5>> FL(nB), FR(nB);
+    for (int iB = 0; iB < nB; iB++)
+    {
+        real vL[3], vR[3], pL, pR, a2, H;
+        for (int d = 0; d < 3; d++)
+            vL[d] = UL[iB][1 + d] / UL[iB][0], vR[d] = UR[iB][1 + d] / UR[iB][0];
+        thermal(UL[iB][4], UL[iB][0], dot3(vL, vL), gamma, pL, a2, H);
+        thermal(UR[iB][4], UR[iB][0], dot3(vR, vR), gamma, pR, a2, H);
+        fluxXY(UL[iB], vL, n[iB], pL, FL[iB].data());
+        fluxXY(UR[iB], vR, n[iB], pR, FR[iB].data());
+    }
+    if (eig == 2)
+    {
+        lam0 = lam123 = lam4 = std::max(std::abs(veloLm0) + std::sqrt(asqrLm), std::abs(veloRm0) + std::sqrt(asqrRm));
+        for (int iB = 0; iB < nB; iB++)
+            for (int i = 0; i < 5; i++)
+                F[iB][i] = (FL[iB][i] + FR[iB][i]) * 0.5 - 0.5 * lam0 * (UR[iB][i] - UL[iB][i]);
+        return;
+    }
+    roeEntropyFix(eig, std::sqrt(asqrLm), std::sqrt(asqrRm), aRoe, veloLm0, veloRm0, veloRoeRN,
+                  norm3(veloLm), norm3(veloRm), norm3(veloRoe), dLambda, fixScale, incFScale, lam0, lam123, lam4);
+    for (int iB = 0; iB < nB; iB++)
+    {
+        // per-point decomposition: jump projected on the point normal, eigenvalues shared,
+        // veloRoeN taken on the mean normal as in the reference batch code
+        const real *nn = n[iB];
+        real incU[5];
+        for (int i = 0; i < 5; i++)
+            incU[i] = UR[iB][i] - UL[iB][i];
+        real incU123N = incU[1] * nn[0] + incU[2] * nn[1] + incU[3] * nn[2];
+        real a23V[3], a23VT[3];
+        for (int d = 0; d < 3; d++)
+            a23V[d] = incU[1 + d] - veloRoe[d] * incU[0];
+        real a23Vn = dot3(a23V, nn);
+        for (int d = 0; d < 3; d++)
+            a23VT[d] = a23V[d] - nn[d] * a23Vn;
+        real incU4b = incU[4] - dot3(veloRoe, a23VT);
+        real alpha1 = (gamma - 1) / asqrRoe * (incU[0] * (HRoe - veloRoeN * veloRoeN) + veloRoeN * incU123N - incU4b);
+        real alpha0 = (incU[0] * (veloRoeN + aRoe) - incU123N - aRoe * alpha1) / (2 * aRoe);
+        real alpha4 = incU[0] - (alpha0 + alpha1);
+        alpha0 *= lam0, alpha1 *= lam123, alpha4 *= lam4;
+        for (int d = 0; d < 3; d++)
+            a23VT[d] *= lam123;
+        real incF[5];
+        incF[0] = alpha0 + alpha1 + alpha4;
+        incF[4] = (HRoe - veloRoeN * aRoe) * alpha0 + 0.5 * vsqrRoe * alpha1 + (HRoe + veloRoeN * aRoe) * alpha4 + dot3(veloRoe, a23VT);
+        for (int d = 0; d < 3; d++)
+            incF[1 + d] = (-aRoe * nn[d] + veloRoe[d]) * alpha0 + (aRoe * nn[d] + veloRoe[d]) * alpha4 + a23VT[d] + veloRoe[d] * alpha1;
+        for (int i = 0; i < 5; i++)
+            F[iB][i] = (FL[iB][i] + FR[iB][i]) * 0.5 - 0.5 * incF[i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Mesh + FiniteVolume metrics + VR setup
+// ---------------------------------------------------------------------------
+struct VR
+{
+    int dim{2};
+    orc_vr_settings s{};
+    idx_t nCell{0}, nFace{0};
+    int maxOrder{1}, NDOF{3}, M{2};
+    std::vector<int32_t> cellType, faceType;
+    std::vector<std::vector<Pt>> cellNodes, faceNodes;
+    std::vector<std::vector<idx_t>> c2f;
+    std::vector<std::array<idx_t, 2>> f2c;
+    std::vector<Pt> fshift;
+    // cell metrics
+    std::vector<real> vol, smooth;
+    std::vector<Pt> bary, cent, hbox;
+    std::vector<std::vector<Pt>> cellQP;
+    std::vector<std::vector<real>> cellJD; // JDet + verySmall
+    std::vector<real> cellQw;              // rule weights (same rule for all cells of a type)
+    std::vector<std::vector<real>> moment; // [nCell][NDOF]
+    // face metrics
+    std::vector<real> area, faceL;
+    std::vector<Pt> faceCent, faceMeanNorm, faceBaryDiff;
+    std::vector<std::vector<Pt>> faceQP, faceN;
+    std::vector<std::vector<real>> faceJD;
+    std::vector<real> faceQw;
+    std::vector<std::vector<real>> faceWeight; // [nFace][maxOrder+1]
+    // VR coefficients
+    std::vector<real> AinvB; // [nCell][maxFC+1][M][M]
+    std::vector<real> Ainvb; // [nCell][maxFC][M]
+    int maxFC{0};
+    real globalVol{0};
+
+    bool cellIsBack(idx_t iCell, idx_t iFace) const { return f2c[iFace][0] == iCell; }
+    idx_t cellFaceOther(idx_t iCell, idx_t iFace) const { return f2c[iFace][0] == iCell ? f2c[iFace][1] : f2c[iFace][0]; }
+    // GetFaceQuadraturePPhysFromCell: the face point expressed in the frame of side if2c
+    Pt facePointFromSide(idx_t iFace, int if2c, const Pt &p) const
+    {
+        Pt r = p;
+        if (if2c == 1)
+            for (int d = 0; d < 3; d++)
+                r.x[d] += fshift[iFace].x[d];
+        return r;
+    }
+    // GetOtherCellBaryFromCell, FiniteVolume.hpp:319-337
+    Pt otherBaryFromCell(idx_t iCell, idx_t iCellOther, idx_t iFace) const
+    {
+        Pt r = bary[iCellOther];
+        real sgn = cellIsBack(iCell, iFace) ? -1.0 : 1.0;
+        for (int d = 0; d < 3; d++)
+            r.x[d] += sgn * fshift[iFace].x[d];
+        return r;
+    }
+
+    // FDiffBaseValue, VariationalReconstruction.hpp:317-369 (isotropic aligned h-box scaling)
+    void diffBase(idx_t iCell, const Pt &p, int nDiff, int flag, std::vector<real> &D) const
+    {
+        D.assign((size_t)nDiff * NDOF, 0.0);
+        Pt sc = hbox[iCell];
+        if (dim == 2)
+        {
+            real m = std::max(sc.x[0], sc.x[1]);
+            sc.x[0] = sc.x[1] = m;
+        }
+        else
+        {
+            real m = std::max({sc.x[0], sc.x[1], sc.x[2]});
+            sc.x[0] = sc.x[1] = sc.x[2] = m;
+        }
+        real xs[3];
+        for (int d = 0; d < 3; d++)
+            xs[d] = (p.x[d] - cent[iCell].x[d]) / sc.x[d];
+        const int(*ord)[3] = dim == 2 ? ord2D : ord3D;
+        for (int id = 0; id < nDiff; id++)
+            for (int ib = 0; ib < NDOF; ib++)
+            {
+                int px = ord[ib][0], py = ord[ib][1], pz = ord[ib][2];
+                int dx = ord[id][0], dy = ord[id][1], dz = ord[id][2];
+                int c = (dx <= px ? dFact[px][dx] : 0) * (dy <= py ? dFact[py][dy] : 0) * (dz <= pz ? dFact[pz][dz] : 0);
+                real v = c ? c * iPow(px - dx, xs[0]) * iPow(py - dy, xs[1]) * iPow(pz - dz, xs[2]) : 0.;
+                D[(size_t)id * NDOF + ib] = v / (iPow(dx, sc.x[0]) * iPow(dy, sc.x[1]) * iPow(dz, sc.x[2]));
+            }
+        if (flag == 0)
+            for (int ib = 0; ib < NDOF; ib++)
+                D[ib] -= moment[iCell][ib];
+    }
+
+    // FFaceFunctional + AccumulateDiffOrderContributions, VariationalReconstruction.hpp:32-93, 459-505
+    // DI: cnDiffs x nI (columns = basis 1..NDOF-1), DJ: cnDiffs x nJ; Conj += ...
+    void faceFunctional(idx_t iFace, const real *DI, int nI, const real *DJ, int nJ, int cnDiffs, real *Conj, real fac) const
+    {
+        const int *comb = dim == 2 ? nComb2D : nComb3D;
+        const int start2[4] = {0, 1, 3, 6}, cnt2[4] = {1, 2, 3, 4};
+        const int start3[4] = {0, 1, 4, 10}, cnt3[4] = {1, 3, 6, 10};
+        const int *st = dim == 2 ? start2 : start3, *ct = dim == 2 ? cnt2 : cnt3;
+        int maxO = 0;
+        for (int o = 0; o < 4; o++)
+            if (st[o] + ct[o] <= cnDiffs)
+                maxO = o;
+        real fL = faceL[iFace];
+        std::vector<real> tmp((size_t)nI * nJ, 0.0);
+        for (int o = maxO; o >= 0; o--)
+        {
+            real fL2 = fL * fL;
+            real pw = o == 0 ? 1.0 : (o == 1 ? fL2 : (o == 2 ? fL2 * fL2 : fL2 * fL2 * fL2));
+            real w = faceWeight[iFace][o];
+            real scale = w * w * pw;
+            for (int i = 0; i < nI; i++)
+                for (int j = 0; j < nJ; j++)
+                {
+                    real r = 0;
+                    for (int k = st[o]; k < st[o] + ct[o]; k++)
+                        r += DI[(size_t)k * nI + i] * DJ[(size_t)k * nJ + j] * comb[k];
+                    tmp[(size_t)i * nJ + j] += r * scale;
+                }
+        }
+        for (size_t q = 0; q < tmp.size(); q++)
+            Conj[q] += tmp[q] * fac;
+    }
+};
+
+static Pt toPt(const double *c)
+{
+    Pt p;
+    p.x[0] = c[0], p.x[1] = c[1], p.x[2] = c[2];
+    return p;
+}
+
+// Gauss-Jordan inverse with partial pivoting (stands in for Eigen's JacobiSVD
+// pseudo-inverse, HardEigen.cpp:13-23; A is SPD here so both give A^-1).
+static void invertMatrix(int n, std::vector<real> A, std::vector<real> &Ainv)
+{
+    Ainv.assign((size_t)n * n, 0.0);
+    for (int i = 0; i < n; i++)
+        Ainv[(size_t)i * n + i] = 1.0;
+    for (int c = 0; c < n; c++)
+    {
+        int piv = c;
+        real best = std::abs(A[(size_t)c * n + c]);
+        for (int r = c + 1; r < n; r++)
+            if (std::abs(A[(size_t)r * n + c]) > best)
+                best = std::abs(A[(size_t)r * n + c]), piv = r;
+        if (best == 0)
+            throw std::runtime_error("singular VR matrix");
+        if (piv != c)
+            for (int k = 0; k < n; k++)
+            {
+                std::swap(A[(size_t)c * n + k], A[(size_t)piv * n + k]);
+                std::swap(Ainv[(size_t)c * n + k], Ainv[(size_t)piv * n + k]);
+            }
+        real d = 1.0 / A[(size_t)c * n + c];
+        for (int k = 0; k < n; k++)
+            A[(size_t)c * n + k] *= d, Ainv[(size_t)c * n + k] *= d;
+        for (int r = 0; r < n; r++)
+        {
+            if (r == c)
+                continue;
+            real f = A[(size_t)r * n + c];
+            if (f == 0)
+                continue;
+            for (int k = 0; k < n; k++)
+                A[(size_t)r * n + k] -= f * A[(size_t)c * n + k], Ainv[(size_t)r * n + k] -= f * Ainv[(size_t)c * n + k];
+        }
+    }
+}
+
+static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
+{
+    v.dim = m->dim;
+    v.s = *s;
+    v.nCell = m->nCell;
+    v.nFace = m->nFace;
+    v.maxOrder = s->maxOrder;
+    v.NDOF = nDofOf(v.dim, s->maxOrder);
+    v.M = v.NDOF - 1;
+    int dim = v.dim;
+    v.cellType.assign(m->cellType, m->cellType + m->nCell);
+    v.faceType.assign(m->faceType, m->faceType + m->nFace);
+    v.cellNodes.resize(v.nCell);
+    v.c2f.resize(v.nCell);
+    v.maxFC = 0;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        for (idx_t k = m->cell2nodePtr[i]; k < m->cell2nodePtr[i + 1]; k++)
+            v.cellNodes[i].push_back(toPt(m->coords + 3 * m->cell2node[k]));
+        for (idx_t k = m->cell2facePtr[i]; k < m->cell2facePtr[i + 1]; k++)
+            v.c2f[i].push_back(m->cell2face[k]);
+        v.maxFC = std::max(v.maxFC, (int)v.c2f[i].size());
+    }
+    v.faceNodes.resize(v.nFace);
+    v.f2c.resize(v.nFace);
+    v.fshift.resize(v.nFace);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        for (idx_t k = m->face2nodePtr[f]; k < m->face2nodePtr[f + 1]; k++)
+            v.faceNodes[f].push_back(toPt(m->coords + 3 * m->face2node[k]));
+        v.f2c[f] = {m->face2cell[2 * f], m->face2cell[2 * f + 1]};
+        v.fshift[f] = toPt(m->faceShift + 3 * f);
+    }
+
+    // ---- cells: FiniteVolume.cpp:23-326
+    v.vol.resize(v.nCell);
+    v.bary.resize(v.nCell);
+    v.cent.resize(v.nCell);
+    v.hbox.resize(v.nCell);
+    v.cellQP.resize(v.nCell);
+    v.cellJD.resize(v.nCell);
+    v.globalVol = 0;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        int t = v.cellType[i];
+        auto qMax = quadRule(t, INT_ORDER_MAX);
+        real vv = 0;
+        for (auto &q : qMax)
+        {
+            GeoPt g = mapPoint(t, v.cellNodes[i], q.p);
+            vv += (1 * cellJDet(dim, g)) * q.w;
+        }
+        vv += verySmallReal;
+        v.vol[i] = vv;
+        v.globalVol += vv;
+        auto qc = quadRule(t, s->intOrder);
+        v.cellQw.clear();
+        Pt b;
+        for (auto &q : qc)
+        {
+            GeoPt g = mapPoint(t, v.cellNodes[i], q.p);
+            real jd = cellJDet(dim, g);
+            v.cellJD[i].push_back(jd + verySmallReal);
+            v.cellQP[i].push_back(toPt(g.x));
+            v.cellQw.push_back(q.w);
+            for (int d = 0; d < 3; d++)
+                b.x[d] += (g.x[d] * jd) * q.w;
+        }
+        for (int d = 0; d < 3; d++)
+            b.x[d] /= vv;
+        v.bary[i] = b;
+        auto q1 = quadRule(t, 1);
+        v.cent[i] = toPt(mapPoint(t, v.cellNodes[i], q1[0].p).x);
+        Pt h;
+        for (auto &nd : v.cellNodes[i])
+            for (int d = 0; d < 3; d++)
+                h.x[d] = std::max(h.x[d], std::abs(nd.x[d] - b.x[d]));
+        if (dim == 2)
+            h.x[2] = 1;
+        v.hbox[i] = h;
+    }
+    // ---- faces: FiniteVolume.cpp:353-575
+    v.area.resize(v.nFace);
+    v.faceCent.resize(v.nFace);
+    v.faceMeanNorm.resize(v.nFace);
+    v.faceQP.resize(v.nFace);
+    v.faceN.resize(v.nFace);
+    v.faceJD.resize(v.nFace);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        int t = v.faceType[f];
+        auto qf = quadRule(t, s->intOrder);
+        real a = 0;
+        Pt mn;
+        v.faceQw.clear();
+        for (auto &q : qf)
+        {
+            GeoPt g = mapPoint(t, v.faceNodes[f], q.p);
+            real nv[3];
+            faceNormVec(dim, g, nv);
+            real jd = norm3(nv);
+            a += (1 * jd) * q.w;
+            v.faceJD[f].push_back(jd + verySmallReal);
+            v.faceQP[f].push_back(toPt(g.x));
+            Pt nu;
+            for (int d = 0; d < 3; d++)
+                nu.x[d] = nv[d] / jd;
+            v.faceN[f].push_back(nu);
+            for (int d = 0; d < 3; d++)
+                mn.x[d] += (nu.x[d] * jd) * q.w;
+            v.faceQw.push_back(q.w);
+        }
+        v.area[f] = a + verySmallReal;
+        real nn = norm3(mn.x);
+        for (int d = 0; d < 3; d++)
+            mn.x[d] /= nn;
+        v.faceMeanNorm[f] = mn;
+        auto q1 = quadRule(t, 1);
+        v.faceCent[f] = toPt(mapPoint(t, v.faceNodes[f], q1[0].p).x);
+    }
+    // ---- cell smooth scale, FiniteVolume.cpp:581-631
+    {
+        std::vector<real> sc(v.nCell), cur(v.nCell), nw(v.nCell);
+        for (idx_t i = 0; i < v.nCell; i++)
+        {
+            real fs = 0;
+            for (auto f : v.c2f[i])
+                fs += v.area[f];
+            sc[i] = v.vol[i] / (fs + verySmallReal);
+            cur[i] = sc[i];
+        }
+        nw = cur;
+        for (int it = 1; it <= s->nIterCellSmoothScale; it++)
+        {
+            for (idx_t i = 0; i < v.nCell; i++)
+            {
+                real nAdj = 0, sum = 0;
+                for (auto f : v.c2f[i])
+                {
+                    idx_t o = v.cellFaceOther(i, f);
+                    if (o >= 0)
+                        nAdj += 1., sum += cur[o];
+                }
+                real w = 1;
+                sum += nAdj * w * cur[i];
+                sum /= nAdj * (1 + w);
+                nw[i] = std::min(cur[i], sum);
+            }
+            cur = nw;
+        }
+        v.smooth.resize(v.nCell);
+        for (idx_t i = 0; i < v.nCell; i++)
+            v.smooth[i] = cur[i] / sc[i];
+    }
+
+    // ---- base moments, VariationalReconstruction.cpp:62-108
+    v.moment.assign(v.nCell, std::vector<real>(v.NDOF, 0.0));
+    {
+        std::vector<real> D;
+        for (idx_t i = 0; i < v.nCell; i++)
+        {
+            std::vector<real> mm(v.NDOF, 0.0);
+            auto qMax = quadRule(v.cellType[i], INT_ORDER_MAX);
+            for (auto &q : qMax)
+            {
+                GeoPt g = mapPoint(v.cellType[i], v.cellNodes[i], q.p);
+                real jd = cellJDet(dim, g);
+                v.diffBase(i, toPt(g.x), 1, 1, D);
+                for (int k = 0; k < v.NDOF; k++)
+                    mm[k] += (D[k] * jd) * q.w;
+            }
+            for (int k = 0; k < v.NDOF; k++)
+                v.moment[i][k] = mm[k] / mm[0];
+        }
+    }
+    // ---- face weights and scales, VariationalReconstruction.cpp:160-427
+    v.faceWeight.assign(v.nFace, std::vector<real>(v.maxOrder + 1, 0.0));
+    v.faceL.resize(v.nFace);
+    v.faceBaryDiff.resize(v.nFace);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
+        Pt bd;
+        if (cR >= 0)
+        {
+            Pt ob = v.otherBaryFromCell(cL, cR, f);
+            for (int d = 0; d < 3; d++)
+                bd.x[d] = ob.x[d] - v.bary[cL].x[d];
+        }
+        else
+        {
+            real vCB[3];
+            for (int d = 0; d < 3; d++)
+                vCB[d] = v.faceCent[f].x[d] - v.bary[cL].x[d];
+            real pr = dot3(vCB, v.faceMeanNorm[f].x);
+            for (int d = 0; d < 3; d++)
+                bd.x[d] = 2 * pr * v.faceMeanNorm[f].x[d];
+        }
+        v.faceBaryDiff[f] = bd;
+        real l2 = 0;
+        for (int d = 0; d < dim; d++)
+            l2 += bd.x[d] * bd.x[d];
+        v.faceL[f] = std::sqrt(l2) * s->scaleMultiplier;
+        real wg = 1;
+        if (s->geomWeightScheme == 1)
+            wg = s->geomWeightBias + std::pow(std::pow(v.area[f], 1. / real(dim - 1)) / norm3(bd.x), s->geomWeightPower * 0.5);
+        std::vector<real> wd(v.maxOrder + 1);
+        if (s->dirWeightScheme == 0)
+            for (int p = 0; p <= v.maxOrder; p++)
+                wd[p] = 1. / factorials[p];
+        else
+        {
+            switch (v.maxOrder)
+            {
+            case 1: wd[0] = wd[1] = 1; break;
+            case 2: wd[0] = 1, wd[1] = 0.4643, wd[2] = 0.1559; break;
+            case 3: wd[0] = 1, wd[1] = .5295, wd[2] = wd[3] = .2117; break;
+            default: throw std::runtime_error("HQM_OPT needs maxOrder 1..3");
+            }
+        }
+        for (int p = 0; p <= v.maxOrder; p++)
+            v.faceWeight[f][p] = wd[p] * wg;
+    }
+
+    // ---- reconstruction coefficients, VariationalReconstruction.cpp:507-825
+    int M = v.M, NDOF = v.NDOF, NDIFF = v.NDOF;
+    v.AinvB.assign((size_t)v.nCell * (v.maxFC + 1) * M * M, 0.0);
+    v.Ainvb.assign((size_t)v.nCell * v.maxFC * M, 0.0);
+    std::vector<real> Dfull, DI((size_t)NDIFF * M), DJ((size_t)NDIFF * M);
+    auto extract = [&](const std::vector<real> &D, std::vector<real> &out, int nDiff) {
+        out.assign((size_t)nDiff * M, 0.0);
+        for (int k = 0; k < nDiff; k++)
+            for (int j = 0; j < M; j++)
+                out[(size_t)k * M + j] = D[(size_t)k * NDOF + 1 + j];
+    };
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        int nf = (int)v.c2f[i].size();
+        std::vector<real> A((size_t)M * M, 0.0);
+        for (int ic = 0; ic < nf; ic++)
+        {
+            idx_t f = v.c2f[i][ic];
+            int if2c = v.cellIsBack(i, f) ? 0 : 1;
+            for (size_t g = 0; g < v.faceQP[f].size(); g++)
+            {
+                v.diffBase(i, v.facePointFromSide(f, if2c, v.faceQP[f][g]), NDIFF, 0, Dfull);
+                extract(Dfull, DI, NDIFF);
+                v.faceFunctional(f, DI.data(), M, DI.data(), M, NDIFF, A.data(), v.faceJD[f][g] * v.faceQw[g]);
+            }
+        }
+        std::vector<std::vector<real>> Bs(nf), bs(nf);
+        for (int ic = 0; ic < nf; ic++)
+        {
+            idx_t f = v.c2f[i][ic];
+            idx_t o = v.cellFaceOther(i, f);
+            int if2c = v.cellIsBack(i, f) ? 0 : 1;
+            if (o < 0)
+                continue;
+            std::vector<real> B((size_t)M * M, 0.0);
+            for (size_t g = 0; g < v.faceQP[f].size(); g++)
+            {
+                v.diffBase(i, v.facePointFromSide(f, if2c, v.faceQP[f][g]), NDIFF, 0, Dfull);
+                extract(Dfull, DI, NDIFF);
+                v.diffBase(o, v.facePointFromSide(f, 1 - if2c, v.faceQP[f][g]), NDIFF, 0, Dfull);
+                extract(Dfull, DJ, NDIFF);
+                v.faceFunctional(f, DI.data(), M, DJ.data(), M, NDIFF, B.data(), v.faceJD[f][g] * v.faceQw[g]);
+            }
+            if (o == i)
+            {
+                for (size_t q = 0; q < A.size(); q++)
+                    A[q] -= B[q], B[q] = 0;
+            }
+            Bs[ic] = B;
+        }
+        for (int ic = 0; ic < nf; ic++)
+        {
+            idx_t f = v.c2f[i][ic];
+            int if2c = v.cellIsBack(i, f) ? 0 : 1;
+            std::vector<real> b(M, 0.0);
+            real one = 1.0;
+            for (size_t g = 0; g < v.faceQP[f].size(); g++)
+            {
+                v.diffBase(i, v.facePointFromSide(f, if2c, v.faceQP[f][g]), 1, 0, Dfull);
+                extract(Dfull, DI, 1);
+                v.faceFunctional(f, DI.data(), M, &one, 1, 1, b.data(), v.faceJD[f][g] * v.faceQw[g]);
+            }
+            bs[ic] = b;
+        }
+        std::vector<real> Ainv;
+        invertMatrix(M, A, Ainv);
+        real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
+        std::copy(Ainv.begin(), Ainv.end(), blk);
+        for (int ic = 0; ic < nf; ic++)
+        {
+            idx_t f = v.c2f[i][ic];
+            if (v.cellFaceOther(i, f) < 0)
+                continue;
+            real *bb = blk + (size_t)(1 + ic) * M * M;
+            for (int r = 0; r < M; r++)
+                for (int c = 0; c < M; c++)
+                {
+                    real acc = 0;
+                    for (int k = 0; k < M; k++)
+                        acc += Ainv[(size_t)r * M + k] * Bs[ic][(size_t)k * M + c];
+                    bb[(size_t)r * M + c] = acc;
+                }
+            real *bv = &v.Ainvb[((size_t)i * v.maxFC + ic) * M];
+            for (int r = 0; r < M; r++)
+            {
+                real acc = 0;
+                for (int k = 0; k < M; k++)
+                    acc += Ainv[(size_t)r * M + k] * bs[ic][k];
+                bv[r] = acc;
+            }
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Analytic fields (src/Euler/SpecialFields.hpp:41-75, EulerEvaluator.hxx:1039-1135,
+// test/cpp/CFV/test_Reconstruction.cpp:443-447, test_Reconstruction3D.cpp:321-325)
+// ---------------------------------------------------------------------------
+static void evalField(int id, const double *prm, const Pt &p, int nVars, real *out)
+{
+    for (int v = 0; v < nVars; v++)
+        out[v] = 0;
+    switch (id)
+    {
+    case 0: // constant: prm[0..nVars)
+        for (int v = 0; v < nVars; v++)
+            out[v] = prm[v];
+        return;
+    case 1: // linear a + b.x
+        out[0] = prm[0] + prm[1] * p.x[0] + prm[2] * p.x[1] + prm[3] * p.x[2];
+        return;
+    case 2: // sin(kx)cos(ky)[cos(kz)] with k = 2 pi / L; prm = {L, dim}
+    {
+        real k = 2.0 * pi / prm[0];
+        out[0] = std::sin(k * p.x[0]) * std::cos(k * p.x[1]) * (prm[1] == 3 ? std::cos(k * p.x[2]) : 1.0);
+        return;
+    }
+    case 3: // cos(kx)+cos(ky)[+cos(kz)]
+    {
+        real k = 2.0 * pi / prm[0];
+        out[0] = std::cos(k * p.x[0]) + std::cos(k * p.x[1]) + (prm[1] == 3 ? std::cos(k * p.x[2]) : 0.0);
+        return;
+    }
+    case 4: // quadratic x^2 + y^2 (+ z^2)
+        out[0] = p.x[0] * p.x[0] + p.x[1] * p.x[1] + prm[0] * p.x[2] * p.x[2];
+        return;
+    case 10: // isentropic vortex on [0,10]^2: prm = {chi, gamma, ubx, uby}
+    {
+        if (nVars < 5)
+            throw std::runtime_error("isentropic vortex needs 5 variables");
+        real chi = prm[0], gamma = prm[1], ubx = prm[2], uby = prm[3];
+        real xyc = 5;
+        real px = p.x[0], py = p.x[1];
+        real r = std::sqrt(sqr(px - xyc) + sqr(py - xyc));
+        real dT = -(gamma - 1) / (8 * gamma * sqr(pi)) * sqr(chi) * std::exp(1 - sqr(r));
+        real dux = chi / 2 / pi * std::exp((1 - sqr(r)) / 2) * -(py - xyc);
+        real duy = chi / 2 / pi * std::exp((1 - sqr(r)) / 2) * +(px - xyc);
+        real T = dT + 1, ux = dux + ubx, uy = duy + uby, S = 1;
+        real rho = std::pow(T / S, 1 / (gamma - 1));
+        real pr = T * rho;
+        real E = 0.5 * (sqr(ux) + sqr(uy)) * rho + pr / (gamma - 1);
+        out[0] = rho, out[1] = rho * ux, out[2] = rho * uy, out[3] = 0, out[4] = E;
+        return;
+    }
+    case 11: // Taylor-Green vortex: prm = {M0, gamma}
+    {
+        if (nVars < 5)
+            throw std::runtime_error("Taylor-Green vortex needs 5 variables");
+        real M0 = prm[0], gamma = prm[1];
+        real x = p.x[0], y = p.x[1], z = p.x[2];
+        real ux = std::sin(x) * std::cos(y) * std::cos(z);
+        real uy = -std::cos(x) * std::sin(y) * std::cos(z);
+        real pr = 1. / (gamma * sqr(M0)) + 1. / 16 * ((std::cos(2 * x) + std::cos(2 * y)) * (2 + std::cos(2 * z)));
+        real rho = gamma * sqr(M0) * pr;
+        real E = 0.5 * (sqr(ux) + sqr(uy)) * rho + pr / (gamma - 1);
+        out[0] = rho, out[1] = rho * ux, out[2] = rho * uy, out[3] = 0, out[4] = E;
+        return;
+    }
+    default: throw std::runtime_error("unknown field id");
+    }
+}
+
+} // namespace orc
+
+using namespace orc;
+
+struct orc_vr
+{
+    VR v;
+};
+
+extern "C" {
+
+const char *orc_last_error(void) { return g_err.c_str(); }
+
+#define ORC_TRY try {
+#define ORC_CATCH(ret)                     \
+    }                                      \
+    catch (const std::exception &e)        \
+    {                                      \
+        g_err = e.what();                  \
+        return ret;                        \
+    }
+
+orc_vr *orc_vr_create(const orc_mesh_desc *mesh, const orc_vr_settings *s)
+{
+    ORC_TRY
+    auto *h = new orc_vr;
+    buildVR(h->v, mesh, s);
+    return h;
+    ORC_CATCH(nullptr)
+}
+
+void orc_vr_destroy(orc_vr *h) { delete h; }
+
+int orc_vr_info(const orc_vr *h, int64_t *nCell, int64_t *nFace, int *M, int *maxFaceCell,
+                int *nGFace, int *nGCell, double *globalVol)
+{
+    const VR &v = h->v;
+    *nCell = v.nCell, *nFace = v.nFace, *M = v.M, *maxFaceCell = v.maxFC;
+    *nGFace = (int)v.faceQw.size(), *nGCell = (int)v.cellQw.size(), *globalVol = v.globalVol;
+    return 0;
+}
+
+int orc_get_cell_metrics(const orc_vr *h, double *vol, double *bary, double *cent, double *scale,
+                         double *smoothRatio, double *moment)
+{
+    const VR &v = h->v;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        vol[i] = v.vol[i];
+        for (int d = 0; d < 3; d++)
+            bary[3 * i + d] = v.bary[i].x[d], cent[3 * i + d] = v.cent[i].x[d];
+        real sc = v.dim == 2 ? std::max(v.hbox[i].x[0], v.hbox[i].x[1])
+                             : std::max({v.hbox[i].x[0], v.hbox[i].x[1], v.hbox[i].x[2]});
+        scale[i] = sc;
+        smoothRatio[i] = v.smooth[i];
+        for (int k = 0; k < v.M; k++)
+            moment[(size_t)i * v.M + k] = v.moment[i][1 + k];
+    }
+    return 0;
+}
+
+int orc_get_face_metrics(const orc_vr *h, double *area, double *meanNorm, double *pts, double *norms,
+                         double *wJDet, double *faceL, double *faceWeight)
+{
+    const VR &v = h->v;
+    size_t G = v.faceQw.size();
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        area[f] = v.area[f];
+        faceL[f] = v.faceL[f];
+        for (int d = 0; d < 3; d++)
+            meanNorm[3 * f + d] = v.faceMeanNorm[f].x[d];
+        for (size_t g = 0; g < G; g++)
+        {
+            for (int d = 0; d < 3; d++)
+                pts[(f * G + g) * 3 + d] = v.faceQP[f][g].x[d], norms[(f * G + g) * 3 + d] = v.faceN[f][g].x[d];
+            wJDet[f * G + g] = v.faceJD[f][g] * v.faceQw[g];
+        }
+        for (int o = 0; o <= v.maxOrder; o++)
+            faceWeight[f * (v.maxOrder + 1) + o] = v.faceWeight[f][o];
+    }
+    return 0;
+}
+
+int orc_get_cell_quad(const orc_vr *h, double *pts, double *wJDet)
+{
+    const VR &v = h->v;
+    size_t G = v.cellQw.size();
+    for (idx_t i = 0; i < v.nCell; i++)
+        for (size_t g = 0; g < G; g++)
+        {
+            for (int d = 0; d < 3; d++)
+                pts[(i * G + g) * 3 + d] = v.cellQP[i][g].x[d];
+            wJDet[i * G + g] = v.cellJD[i][g] * v.cellQw[g];
+        }
+    return 0;
+}
+
+int orc_get_vr_coeffs(const orc_vr *h, double *AinvB, double *Ainvb)
+{
+    const VR &v = h->v;
+    std::copy(v.AinvB.begin(), v.AinvB.end(), AinvB);
+    std::copy(v.Ainvb.begin(), v.Ainvb.end(), Ainvb);
+    return 0;
+}
+
+// cell averages by cell quadrature: EulerEvaluator.hxx:1046-1068, test_Reconstruction.cpp:246-259
+int orc_cell_average(const orc_vr *h, int fieldId, const double *params, int nVars, double *u)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    std::vector<real> val(nVars);
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        std::vector<real> acc(nVars, 0.0);
+        for (size_t g = 0; g < v.cellQw.size(); g++)
+        {
+            evalField(fieldId, params, v.cellQP[i][g], nVars, val.data());
+            for (int k = 0; k < nVars; k++)
+                acc[k] += (val[k] * v.cellJD[i][g]) * v.cellQw[g];
+        }
+        for (int k = 0; k < nVars; k++)
+            u[i * nVars + k] = acc[k] / v.vol[i];
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// L1 error of the reconstruction at cell quadrature points / domain volume,
+// test_Reconstruction.cpp:330-353
+double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, const double *u, const double *uRec)
+{
+    const VR &v = h->v;
+    int M = v.M;
+    std::vector<real> D;
+    real err = 0;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real ec = 0;
+        for (size_t g = 0; g < v.cellQw.size(); g++)
+        {
+            v.diffBase(i, v.cellQP[i][g], 1, 0, D);
+            real r = 0;
+            for (int k = 0; k < M; k++)
+                r += D[1 + k] * uRec[(size_t)i * M + k];
+            real ex;
+            evalField(fieldId, params, v.cellQP[i][g], 1, &ex);
+            ec += (std::abs(r + u[i] - ex) * v.cellJD[i][g]) * v.cellQw[g];
+        }
+        err += ec;
+    }
+    return err / v.globalVol;
+}
+
+// DoReconstructionIter (Jacobi branch, SORInstead = false, interior faces),
+// VariationalReconstruction_Reconstruction.hxx:485-615
+int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew)
+{
+    const VR &v = h->v;
+    int M = v.M;
+    size_t blkN = (size_t)M * nVars;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real relax = v.s.jacobiRelax;
+        real *out = uRecNew + i * blkN;
+        for (size_t q = 0; q < blkN; q++)
+            out[q] = uRec[i * blkN + q] * (1 - relax);
+        const real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
+        for (size_t ic = 0; ic < v.c2f[i].size(); ic++)
+        {
+            idx_t f = v.c2f[i][ic];
+            idx_t o = v.cellFaceOther(i, f);
+            if (o < 0)
+            {
+                g_err = "boundary faces are not supported by the oracle VR iteration";
+                return -1;
+            }
+            const real *B = blk + (1 + ic) * (size_t)M * M;
+            const real *bv = &v.Ainvb[((size_t)i * v.maxFC + ic) * M];
+            for (int var = 0; var < nVars; var++)
+            {
+                real du = u[o * nVars + var] - u[i * nVars + var];
+                for (int r = 0; r < M; r++)
+                {
+                    real acc = 0;
+                    for (int k = 0; k < M; k++)
+                        acc += B[(size_t)r * M + k] * uRec[o * blkN + (size_t)var * M + k];
+                    acc += bv[r] * du;
+                    out[(size_t)var * M + r] += relax * acc;
+                }
+            }
+        }
+    }
+    return 0;
+}
+
+// DoReconstructionIterDiff, VariationalReconstruction_Reconstruction.hxx:617-670
+int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double *uRecNew)
+{
+    const VR &v = h->v;
+    int M = v.M;
+    size_t blkN = (size_t)M * nVars;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real *out = uRecNew + i * blkN;
+        for (size_t q = 0; q < blkN; q++)
+            out[q] = uRecDiff[i * blkN + q];
+        const real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
+        for (size_t ic = 0; ic < v.c2f[i].size(); ic++)
+        {
+            idx_t o = v.cellFaceOther(i, v.c2f[i][ic]);
+            if (o < 0)
+                return -1;
+            const real *B = blk + (1 + ic) * (size_t)M * M;
+            for (int var = 0; var < nVars; var++)
+                for (int r = 0; r < M; r++)
+                {
+                    real acc = 0;
+                    for (int k = 0; k < M; k++)
+                        acc += B[(size_t)r * M + k] * uRecDiff[o * blkN + (size_t)var * M + k];
+                    out[(size_t)var * M + r] -= acc;
+                }
+        }
+    }
+    return 0;
+}
+
+int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const double *ULm,
+                const double *URm, const double *n, double gamma, double dLambda, double fixScale,
+                double incFScale, double *F, double *lam)
+{
+    (void)dim;
+    ORC_TRY
+    dispatchFlux(rsType, UL, UR, ULm, URm, n, gamma, F, dLambda, fixScale, incFScale, lam[0], lam[1], lam[2]);
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// EvaluateDt, EulerEvaluator_EvaluateDt.hxx:840-1048 (inviscid: lamVis = 0; interior faces)
+int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, double CFL,
+                double maxDt, int useLocalDt, double *dt, double *lambdaFace, double *deltaLambdaCell,
+                double *dtMin)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    real gamma = es->gamma;
+    std::vector<real> lamF(v.nFace), dLamF(v.nFace);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        const real *n = v.faceMeanNorm[f].x;
+        idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
+        const real *UL = u + cL * 5;
+        const real *UR = cR >= 0 ? u + cR * 5 : UL;
+        real vL[3], vR[3], pL, aL2, HL, pR, aR2, HR;
+        for (int d = 0; d < 3; d++)
+            vL[d] = UL[1 + d] / UL[0], vR[d] = UR[1 + d] / UR[0];
+        thermal(UL[4], UL[0], dot3(vL, vL), gamma, pL, aL2, HL);
+        thermal(UR[4], UR[0], dot3(vR, vR), gamma, pR, aR2, HR);
+        if (!(aL2 >= 0 && aR2 >= 0))
+            throw std::runtime_error("mean value violates PP");
+        real vgN = 0;
+        real veloNL = dot3(vL, n), veloNR = dot3(vR, n);
+        real veloNMean = 0.5 * (veloNL + veloNR);
+        {
+            real s[3] = {vL[0] + vR[0], vL[1] + vR[1], vL[2] + vR[2]};
+            veloNMean = 0.5 * dot3(s, n);
+        }
+        real aL = std::sqrt(aL2), aR = std::sqrt(aR2);
+        real lambdaConvection = std::max(aL + std::abs(veloNL - vgN), aR + std::abs(veloNR - vgN));
+        real lamVis = 0.0;
+        real volR = cR >= 0 ? v.vol[cR] : v.vol[cL];
+        real visc = lamVis * v.area[f] * (1. / v.vol[cL] + 1. / volR);
+        lamF[f] = lambdaConvection + visc;
+        lambdaFace[4 * f + 0] = lamF[f];                              // lambdaFace
+        lambdaFace[4 * f + 1] = std::abs(veloNMean - vgN) + visc;     // lambdaFaceC
+        lambdaFace[4 * f + 2] = visc;                                 // lambdaFaceVis
+        lambdaFace[4 * f + 3] = lambdaConvection;                     // lambdaFace0/123/4
+        real dvn = dot3(vR, n) - dot3(vL, n);
+        {
+            real dv[3] = {vR[0] - vL[0], vR[1] - vL[1], vR[2] - vL[2]};
+            dvn = dot3(dv, n);
+        }
+        dLamF[f] = std::max({std::abs(dvn + aR - aL), std::abs(dvn - aR + aL), std::abs(aL - aR)});
+    }
+    real dmin = 3e200;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real lc = 0;
+        for (auto f : v.c2f[i])
+            lc += lamF[f] * v.area[f];
+        dt[i] = std::min(CFL * v.vol[i] * v.smooth[i] / (lc + 1e-100), maxDt);
+        dmin = std::min(dmin, dt[i]);
+        real dl = 0;
+        for (auto f : v.c2f[i])
+            dl = std::max(dl, dLamF[f]);
+        deltaLambdaCell[i] = dl;
+    }
+    *dtMin = dmin;
+    if (!useLocalDt)
+        for (idx_t i = 0; i < v.nCell; i++)
+            dt[i] = dmin;
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// EvaluateRHS, EulerEvaluator_EvaluateRHS.hxx:55-818 with fluxFace
+// EulerEvaluator_EvaluateDt.hxx:1084-1511; inviscid, interior/periodic faces,
+// serial direct scatter (:550-562).
+int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                 const double *deltaLambdaCell, double *rhs)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    int M = v.M;
+    size_t G = v.faceQw.size();
+    std::fill(rhs, rhs + v.nCell * 5, 0.0);
+    std::vector<real> D;
+    std::vector<std::array<real, 5>> ULv(G), URv(G), fincC(G);
+    std::vector<std::array<real, 3>> nv(G);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
+        if (cR < 0)
+            throw std::runtime_error("boundary faces are not supported by the oracle RHS");
+        const real *ULm = u + cL * 5, *URm = u + cR * 5;
+        for (size_t g = 0; g < G; g++)
+        {
+            // ULxy = u[L] + (D0Bj(L) * uRec[L])^T, EvaluateRHS.hxx:235-243, 287-297
+            real UL[5], UR[5];
+            v.diffBase(cL, v.facePointFromSide(f, 0, v.faceQP[f][g]), 1, 0, D);
+            for (int var = 0; var < 5; var++)
+            {
+                real r = 0;
+                if (uRec)
+                    for (int k = 0; k < M; k++)
+                        r += D[1 + k] * uRec[((size_t)cL * 5 + var) * M + k];
+                UL[var] = ULm[var] + r;
+            }
+            v.diffBase(cR, v.facePointFromSide(f, 1, v.faceQP[f][g]), 1, 0, D);
+            for (int var = 0; var < 5; var++)
+            {
+                real r = 0;
+                if (uRec)
+                    for (int k = 0; k < M; k++)
+                        r += D[1 + k] * uRec[((size_t)cR * 5 + var) * M + k];
+                UR[var] = URm[var] + r;
+            }
+            for (int var = 0; var < 5; var++)
+                ULv[g][var] = UL[var], URv[g][var] = UR[var];
+            for (int d = 0; d < 3; d++)
+                nv[g][d] = v.faceN[f][g].x[d];
+        }
+        real dLambda = deltaLambdaCell ? std::max(deltaLambdaCell[cL], deltaLambdaCell[cR]) : 0.0;
+        real lam0, lam123, lam4;
+        int rs = es->rsType;
+        if (es->rsMeanValueEig != 0 && rs >= 11 && rs <= 15)
+        {
+            std::vector<std::array<real, 5>> Fb(G);
+            roeFluxBatch(rs - 10, (int)G, reinterpret_cast<const real(*)[5]>(ULv.data()),
+                         reinterpret_cast<const real(*)[5]>(URv.data()), ULm, URm,
+                         reinterpret_cast<const real(*)[3]>(nv.data()), v.faceMeanNorm[f].x, es->gamma,
+                         reinterpret_cast<real(*)[5]>(Fb.data()), dLambda, es->rsFixScale, es->rsIncFScale,
+                         lam0, lam123, lam4);
+            for (size_t g = 0; g < G; g++)
+                fincC[g] = Fb[g];
+        }
+        else
+            for (size_t g = 0; g < G; g++)
+                dispatchFlux(rs, ULv[g].data(), URv[g].data(), ULm, URm, nv[g].data(), es->gamma, fincC[g].data(),
+                             dLambda, es->rsFixScale, es->rsIncFScale, lam0, lam123, lam4);
+        real fluxEs[5] = {0, 0, 0, 0, 0};
+        for (size_t g = 0; g < G; g++)
+            for (int var = 0; var < 5; var++)
+                fluxEs[var] += ((-fincC[g][var]) * v.faceJD[f][g]) * v.faceQw[g];
+        for (int var = 0; var < 5; var++)
+        {
+            rhs[cL * 5 + var] += fluxEs[var] / v.vol[cL];
+            rhs[cR * 5 + var] += (-fluxEs[var]) / v.vol[cR];
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// LUSGSMatrixInit, scalar mode, EulerEvaluator.hxx:37-159 (ignoreSourceTerm)
+int orc_lusgs_init(const orc_vr *h, const double *dTau, double dt, double alphaDiag,
+                   const double *lambdaFace, double *jdiag)
+{
+    const VR &v = h->v;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real fp = 1.0 / dTau[i] + 1.0 / dt;
+        for (auto f : v.c2f[i])
+            fp += (0.5 * alphaDiag) * v.area[f] * lambdaFace[4 * f] / v.vol[i];
+        jdiag[i] = fp;
+    }
+    return 0;
+}
+
+// fluxJacobian0_Right_Times_du (scalar / non-Roe branch), EulerEvaluator.hpp:1100-1158,
+// with IdealGasUIncrement Gas.hpp:462-471 and GasInviscidFluxFacialIncrement Gas.hpp:495-513
+static void fluxJacobianTimesDu(const real *U, const real *n, const real *dU, real lambdaMain, real gamma, real *dF)
+{
+    real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    real p, H, asqr;
+    thermal(U[4], U[0], dot3(velo, velo), gamma, p, asqr, H);
+    real dVelo[3];
+    for (int d = 0; d < 3; d++)
+        dVelo[d] = (dU[1 + d] - U[1 + d] / U[0] * dU[0]) / U[0];
+    real dp = (gamma - 1) * (dU[4] - 0.5 * (dot3(dU + 1, velo) + dot3(U + 1, dVelo)));
+    real vn = dot3(velo, n), dvn = dot3(dVelo, n);
+    dF[0] = dot3(dU + 1, n);
+    for (int d = 0; d < 3; d++)
+        dF[1 + d] = dU[1 + d] * vn + U[1 + d] * dvn + n[d] * dp;
+    dF[4] = (dU[4] + dp) * vn + (U[4] + p) * dvn;
+    // vg = 0: F -= dU * vg.n is zero
+    for (int k = 0; k < 5; k++)
+        dF[k] += -1 * lambdaMain * dU[k];
+}
+
+// UpdateSGS, EulerEvaluator.hxx:550-693 (scalar JDiag, non-Roe Jacobian).  scanOrder
+// gives the scan position -> cell map (NULL = natural order); "already visited" is
+// decided by scan position, as the reference does with renumbered cell indices.
+int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                   const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                   const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero,
+                   const int64_t *scanOrder, double *sumInc)
+{
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    std::vector<idx_t> pos(N), ord(N);
+    for (idx_t s = 0; s < N; s++)
+    {
+        ord[s] = scanOrder ? scanOrder[s] : s;
+        pos[ord[s]] = s;
+    }
+    for (int k = 0; k < 5; k++)
+        sumInc[k] = 0;
+    for (idx_t sIt = 0; sIt < N; sIt++)
+    {
+        idx_t iScan = forward ? sIt : N - 1 - sIt;
+        idx_t i = ord[iScan];
+        real buf[5];
+        for (int k = 0; k < 5; k++)
+            buf[k] = rhs[i * 5 + k];
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            int atFace = v.cellIsBack(i, f) ? 0 : 1;
+            if (o < 0 || o == i)
+                continue;
+            real n[3];
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFace ? -1 : 1);
+            bool gsUseNew = gsUpdate && (forward ? (pos[o] < iScan) : (pos[o] > iScan));
+            if (uIncIsZero && !gsUseNew)
+                continue;
+            const real *duj = gsUseNew ? uIncNew + o * 5 : uInc + o * 5;
+            real dF[5];
+            fluxJacobianTimesDu(u + o * 5, n, duj, lambdaFace[4 * f], es->gamma, dF);
+            for (int k = 0; k < 5; k++)
+                buf[k] -= (0.5 * alphaDiag) * v.area[f] / v.vol[i] * dF[k];
+        }
+        for (int k = 0; k < 5; k++)
+        {
+            real old = uIncNew[i * 5 + k];
+            real nw = buf[k] / jdiag[i];
+            uIncNew[i * 5 + k] = nw;
+            sumInc[k] += std::abs(nw - old);
+        }
+    }
+    return 0;
+}
+
+// Explicit SSP-RK3 (src/Solver/ODE.hpp:1477-1520) around frhs = nRec x Jacobi VR + EvaluateRHS,
+// fdt = EvaluateDt (global dt from CFL unless dtFixed > 0); fincrement = plain add (the
+// positivity compression of EulerEvaluator.hpp:1795 is the identity for these smooth states).
+int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,
+               double CFL, int nRec, double dtFixed)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    size_t nR = (size_t)N * 5 * v.M;
+    std::vector<real> dtv(N), lam((size_t)v.nFace * 4), dlc(N), rhs(N * 5), xLast(N * 5), tmp(nR);
+    auto frhs = [&](const real *x) {
+        for (int r = 0; r < nRec; r++)
+        {
+            if (orc_vr_iter(h, 5, uRec, x, tmp.data()))
+                throw std::runtime_error(g_err);
+            std::copy(tmp.begin(), tmp.end(), uRec);
+        }
+        if (orc_eval_rhs(h, es, x, uRec, dlc.data(), rhs.data()))
+            throw std::runtime_error(g_err);
+    };
+    for (int step = 0; step < nStep; step++)
+    {
+        real dtMin;
+        if (orc_eval_dt(h, es, u, CFL, 1e100, 0, dtv.data(), lam.data(), dlc.data(), &dtMin))
+            throw std::runtime_error(g_err);
+        real dt = dtFixed > 0 ? dtFixed : dtMin;
+        std::copy(u, u + N * 5, xLast.begin());
+        frhs(u);
+        for (idx_t q = 0; q < N * 5; q++)
+            u[q] += rhs[q] * dt;
+        frhs(u);
+        for (idx_t q = 0; q < N * 5; q++)
+            u[q] = u[q] * 0.25 + xLast[q] * 0.75 + 0.25 * (rhs[q] * dt);
+        frhs(u);
+        for (idx_t q = 0; q < N * 5; q++)
+            u[q] = u[q] * (2. / 3.) + xLast[q] * (1. / 3.) + (2. / 3.) * (rhs[q] * dt);
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
+} // extern "C"

@@ -1,0 +1,118 @@
+/*
+ * dndsr_oracle.h -- C interface of the CPU oracle.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This is a plain C++17 FP64 restatement of the
+ * reference's compact-finite-volume algorithms (CFDLAB-THU/DNDSR), used as the
+ * checker for the HIP path.  Only tests/, __graft_entry__.smoke() and the
+ * cpu_baseline leg of bench.py may load it.  The product library never links
+ * or calls anything in oracle/.
+ *
+ * Array conventions follow the reference's row layout (ArrayDof<m,n> rows are
+ * contiguous column-major m x n blocks, src/DNDS/ArrayDerived/ArrayEigenMatrix.hpp:23,64):
+ *   u      [nCell][nVars]
+ *   uRec   [nCell][nVars][M]    (column-major M x nVars: element (k,v) at k + M*v)
+ *   AinvB  [nCell][nFaceCell+1][M][M] (block 0 = A^-1, block 1+ic2f = A^-1 B, row-major)
+ *   Ainvb  [nCell][nFaceCell][M]
+ */
+#ifndef DNDSR_ORACLE_H
+#define DNDSR_ORACLE_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* element types */
+enum { ORC_LINE2 = 1, ORC_QUAD4 = 2, ORC_HEX8 = 3, ORC_TRI3 = 4, ORC_TET4 = 5 };
+
+typedef struct
+{
+    int dim;                 /* geometric dimension, 2 or 3 */
+    int64_t nNode;
+    const double *coords;    /* [nNode][3] */
+    int64_t nCell;
+    const int32_t *cellType; /* [nCell] */
+    const int64_t *cell2nodePtr, *cell2node;
+    const int64_t *cell2facePtr, *cell2face; /* faces of each cell, element-local order */
+    int64_t nFace;
+    const int32_t *faceType; /* [nFace] */
+    const int64_t *face2nodePtr, *face2node;
+    const int64_t *face2cell;  /* [nFace][2]; -1 = boundary */
+    const double *faceShift;   /* [nFace][3]: translation from the back (L) cell frame to the front (R) cell frame */
+} orc_mesh_desc;
+
+typedef struct
+{
+    int maxOrder;
+    int intOrder;
+    int dirWeightScheme;  /* 0 Factorial, 1 HQM_OPT */
+    int geomWeightScheme; /* 0 GWNone, 1 HQM_SD */
+    double geomWeightBias;
+    double geomWeightPower;
+    double scaleMultiplier;
+    double jacobiRelax;
+    int nIterCellSmoothScale;
+} orc_vr_settings;
+
+typedef struct
+{
+    double gamma;
+    int rsType;         /* Gas::RiemannSolverType numbering: Roe=1, HLLC=2, HLLEP=3, HLLEP_V1=21, Roe_M1..M9=11..19 */
+    int rsMeanValueEig; /* batched mean-value eigen path for Roe_M1..Roe_M5 */
+    double rsFixScale;
+    double rsIncFScale;
+} orc_euler_settings;
+
+typedef struct orc_vr orc_vr;
+
+const char *orc_last_error(void);
+
+orc_vr *orc_vr_create(const orc_mesh_desc *mesh, const orc_vr_settings *s);
+void orc_vr_destroy(orc_vr *h);
+int orc_vr_info(const orc_vr *h, int64_t *nCell, int64_t *nFace, int *M, int *maxFaceCell,
+                int *nGFace, int *nGCell, double *globalVol);
+
+/* metric getters (copy out) */
+int orc_get_cell_metrics(const orc_vr *h, double *vol, double *bary, double *cent,
+                         double *scale, double *smoothRatio, double *moment /*[nCell][M]*/);
+int orc_get_face_metrics(const orc_vr *h, double *area, double *meanNorm /*[nFace][3]*/,
+                         double *pts /*[nFace][G][3]*/, double *norms /*[nFace][G][3]*/,
+                         double *wJDet /*[nFace][G]*/, double *faceL, double *faceWeight /*[nFace][maxOrder+1]*/);
+int orc_get_cell_quad(const orc_vr *h, double *pts /*[nCell][Gc][3]*/, double *wJDet /*[nCell][Gc]*/);
+int orc_get_vr_coeffs(const orc_vr *h, double *AinvB, double *Ainvb);
+
+/* fields */
+int orc_cell_average(const orc_vr *h, int fieldId, const double *params, int nVars, double *u);
+double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, const double *u,
+                        const double *uRec);
+
+/* VR iteration: one Jacobi sweep uRecNew = (1-w) uRec + w * sum(...) */
+int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew);
+/* uRecNew = uRecDiff - sum_f AinvB_f * uRecDiff_j (VR Jacobian action) */
+int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double *uRecNew);
+
+/* Riemann flux at one point; F gets the flux, lam[3] = lam0, lam123, lam4 */
+int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const double *ULm,
+                const double *URm, const double *n, double gamma, double dLambda,
+                double fixScale, double incFScale, double *F, double *lam);
+
+/* EulerEvaluator operators (nVars = 5, inviscid, periodic / interior faces) */
+int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, double CFL,
+                double maxDt, int useLocalDt, double *dt, double *lambdaFace /*[nFace][4]*/,
+                double *deltaLambdaCell, double *dtMin);
+int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
+                 const double *uRec, const double *deltaLambdaCell, double *rhs);
+int orc_lusgs_init(const orc_vr *h, const double *dTau, double dt, double alphaDiag,
+                   const double *lambdaFace, double *jdiag);
+int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDiag,
+                   const double *rhs, const double *u, const double *uInc, double *uIncNew,
+                   const double *jdiag, const double *lambdaFace, int forward, int gsUpdate,
+                   int uIncIsZero, const int64_t *scanOrder, double *sumInc);
+/* full explicit SSP-RK3 steps: nStep steps of (EvaluateDt once, 3 x (nRec VR + RHS)) */
+int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,
+               double CFL, int nRec, double dtFixed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

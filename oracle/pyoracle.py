@@ -1,0 +1,224 @@
+"""ctypes wrapper of the CPU oracle (oracle/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
+cpu_baseline leg of bench.py, as the checker of the HIP path.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB = os.path.join(_HERE, "liboracle.so")
+_lib = None
+
+dp = C.POINTER(C.c_double)
+i64p = C.POINTER(C.c_int64)
+i32p = C.POINTER(C.c_int32)
+
+
+class MeshDesc(C.Structure):
+    _fields_ = [("dim", C.c_int), ("nNode", C.c_int64), ("coords", dp), ("nCell", C.c_int64),
+                ("cellType", i32p), ("cell2nodePtr", i64p), ("cell2node", i64p),
+                ("cell2facePtr", i64p), ("cell2face", i64p), ("nFace", C.c_int64),
+                ("faceType", i32p), ("face2nodePtr", i64p), ("face2node", i64p),
+                ("face2cell", i64p), ("faceShift", dp)]
+
+
+class VRSettings(C.Structure):
+    _fields_ = [("maxOrder", C.c_int), ("intOrder", C.c_int), ("dirWeightScheme", C.c_int),
+                ("geomWeightScheme", C.c_int), ("geomWeightBias", C.c_double),
+                ("geomWeightPower", C.c_double), ("scaleMultiplier", C.c_double),
+                ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int)]
+
+
+class EulerSettings(C.Structure):
+    _fields_ = [("gamma", C.c_double), ("rsType", C.c_int), ("rsMeanValueEig", C.c_int),
+                ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double)]
+
+
+def build():
+    subprocess.run(["make", "-s", "-C", _HERE], check=True)
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB):
+            build()
+        _lib = C.CDLL(_LIB)
+        _lib.orc_vr_create.restype = C.c_void_p
+        _lib.orc_vr_create.argtypes = [C.POINTER(MeshDesc), C.POINTER(VRSettings)]
+        _lib.orc_vr_destroy.argtypes = [C.c_void_p]
+        _lib.orc_last_error.restype = C.c_char_p
+        _lib.orc_rec_error_l1.restype = C.c_double
+        for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
+                     "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_iter_diff",
+                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3",
+                     "orc_rec_error_l1", "orc_riemann"]:
+            getattr(_lib, name).argtypes = None
+    return _lib
+
+
+def P(a):
+    if a is None:
+        return None
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def _chk(rc):
+    if rc != 0:
+        raise RuntimeError("oracle: " + lib().orc_last_error().decode())
+
+
+def riemann(rsType, UL, UR, n, gamma=1.4, ULm=None, URm=None, dLambda=0.0, fixScale=1.0, incFScale=1.0):
+    UL = np.ascontiguousarray(UL, np.float64)
+    UR = np.ascontiguousarray(UR, np.float64)
+    ULm = UL if ULm is None else np.ascontiguousarray(ULm, np.float64)
+    URm = UR if URm is None else np.ascontiguousarray(URm, np.float64)
+    n = np.ascontiguousarray(n, np.float64)
+    F = np.zeros(5)
+    lam = np.zeros(3)
+    _chk(lib().orc_riemann(C.c_int(rsType), C.c_int(3), P(UL), P(UR), P(ULm), P(URm), P(n), C.c_double(gamma),
+                           C.c_double(dLambda), C.c_double(fixScale), C.c_double(incFScale), P(F), P(lam)))
+    return F, lam
+
+
+class OracleVR:
+    """VariationalReconstruction + EulerEvaluator restated on the CPU."""
+
+    def __init__(self, mesh, maxOrder=1, intOrder=3, dirWeightScheme=0, geomWeightScheme=0,
+                 geomWeightBias=0.0, geomWeightPower=0.5, jacobiRelax=1.0, nIterCellSmoothScale=15):
+        L = lib()
+        self.mesh = mesh
+        self._keep = [np.ascontiguousarray(x) for x in (
+            mesh.coords, mesh.cellType, mesh.cell2nodePtr, mesh.cell2node, mesh.cell2facePtr, mesh.cell2face,
+            mesh.faceType, mesh.face2nodePtr, mesh.face2node, mesh.face2cell, mesh.faceShift)]
+        k = self._keep
+        md = MeshDesc(mesh.dim, mesh.nNode, k[0].ctypes.data_as(dp), mesh.nCell, k[1].ctypes.data_as(i32p),
+                      k[2].ctypes.data_as(i64p), k[3].ctypes.data_as(i64p), k[4].ctypes.data_as(i64p),
+                      k[5].ctypes.data_as(i64p), mesh.nFace, k[6].ctypes.data_as(i32p), k[7].ctypes.data_as(i64p),
+                      k[8].ctypes.data_as(i64p), k[9].ctypes.data_as(i64p), k[10].ctypes.data_as(dp))
+        s = VRSettings(maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
+                       1.0, jacobiRelax, nIterCellSmoothScale)
+        self.settings = s
+        h = L.orc_vr_create(C.byref(md), C.byref(s))
+        if not h:
+            raise RuntimeError("oracle: " + L.orc_last_error().decode())
+        self.h = C.c_void_p(h)
+        nC, nF = C.c_int64(), C.c_int64()
+        M, mfc, gf, gc = C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        gv = C.c_double()
+        L.orc_vr_info(self.h, C.byref(nC), C.byref(nF), C.byref(M), C.byref(mfc), C.byref(gf), C.byref(gc),
+                      C.byref(gv))
+        self.nCell, self.nFace, self.M, self.maxFC = nC.value, nF.value, M.value, mfc.value
+        self.nGFace, self.nGCell, self.globalVol = gf.value, gc.value, gv.value
+        self.maxOrder = maxOrder
+
+    def __del__(self):
+        try:
+            lib().orc_vr_destroy(self.h)
+        except Exception:
+            pass
+
+    # ---- metrics
+    def cell_metrics(self):
+        N, M = self.nCell, self.M
+        vol, sc, sm = np.zeros(N), np.zeros(N), np.zeros(N)
+        bary, cent, mom = np.zeros((N, 3)), np.zeros((N, 3)), np.zeros((N, M))
+        _chk(lib().orc_get_cell_metrics(self.h, P(vol), P(bary), P(cent), P(sc), P(sm), P(mom)))
+        return dict(vol=vol, bary=bary, cent=cent, scale=sc, smooth=sm, moment=mom)
+
+    def face_metrics(self):
+        F, G = self.nFace, self.nGFace
+        area, fl = np.zeros(F), np.zeros(F)
+        mn, pts, nrm = np.zeros((F, 3)), np.zeros((F, G, 3)), np.zeros((F, G, 3))
+        wj, fw = np.zeros((F, G)), np.zeros((F, self.maxOrder + 1))
+        _chk(lib().orc_get_face_metrics(self.h, P(area), P(mn), P(pts), P(nrm), P(wj), P(fl), P(fw)))
+        return dict(area=area, meanNorm=mn, pts=pts, norms=nrm, wJDet=wj, faceL=fl, faceWeight=fw)
+
+    def cell_quad(self):
+        N, G = self.nCell, self.nGCell
+        pts, wj = np.zeros((N, G, 3)), np.zeros((N, G))
+        _chk(lib().orc_get_cell_quad(self.h, P(pts), P(wj)))
+        return pts, wj
+
+    def vr_coeffs(self):
+        N, M, K = self.nCell, self.M, self.maxFC
+        AinvB = np.zeros((N, K + 1, M, M))
+        Ainvb = np.zeros((N, K, M))
+        _chk(lib().orc_get_vr_coeffs(self.h, P(AinvB), P(Ainvb)))
+        return AinvB, Ainvb
+
+    # ---- fields
+    def cell_average(self, fieldId, params, nVars):
+        u = np.zeros((self.nCell, nVars))
+        prm = np.ascontiguousarray(params, np.float64)
+        _chk(lib().orc_cell_average(self.h, C.c_int(fieldId), P(prm), C.c_int(nVars), P(u)))
+        return u
+
+    def rec_error_l1(self, fieldId, params, u, uRec):
+        prm = np.ascontiguousarray(params, np.float64)
+        return lib().orc_rec_error_l1(self.h, C.c_int(fieldId), P(prm), P(np.ascontiguousarray(u)),
+                                      P(np.ascontiguousarray(uRec)))
+
+    # ---- VR
+    def vr_iter(self, uRec, u):
+        nVars = u.shape[1]
+        out = np.zeros_like(uRec)
+        _chk(lib().orc_vr_iter(self.h, C.c_int(nVars), P(np.ascontiguousarray(uRec)), P(np.ascontiguousarray(u)),
+                               P(out)))
+        return out
+
+    def vr_iter_diff(self, uRecDiff):
+        nVars = uRecDiff.shape[1]
+        out = np.zeros_like(uRecDiff)
+        _chk(lib().orc_vr_iter_diff(self.h, C.c_int(nVars), P(np.ascontiguousarray(uRecDiff)), P(out)))
+        return out
+
+    # ---- Euler
+    @staticmethod
+    def euler_settings(gamma=1.4, rsType=1, rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0):
+        return EulerSettings(gamma, rsType, rsMeanValueEig, rsFixScale, rsIncFScale)
+
+    def eval_dt(self, es, u, CFL, maxDt=1e100, useLocalDt=True):
+        N, F = self.nCell, self.nFace
+        dt, lam, dlc = np.zeros(N), np.zeros((F, 4)), np.zeros(N)
+        dmin = C.c_double()
+        _chk(lib().orc_eval_dt(self.h, C.byref(es), P(np.ascontiguousarray(u)), C.c_double(CFL), C.c_double(maxDt),
+                               C.c_int(int(useLocalDt)), P(dt), P(lam), P(dlc), C.byref(dmin)))
+        return dt, lam, dlc, dmin.value
+
+    def eval_rhs(self, es, u, uRec=None, deltaLambdaCell=None):
+        rhs = np.zeros((self.nCell, 5))
+        _chk(lib().orc_eval_rhs(self.h, C.byref(es), P(np.ascontiguousarray(u)),
+                                None if uRec is None else P(np.ascontiguousarray(uRec)),
+                                None if deltaLambdaCell is None else P(np.ascontiguousarray(deltaLambdaCell)), P(rhs)))
+        return rhs
+
+    def lusgs_init(self, dTau, dt, alphaDiag, lam):
+        jd = np.zeros(self.nCell)
+        _chk(lib().orc_lusgs_init(self.h, P(np.ascontiguousarray(dTau)), C.c_double(dt), C.c_double(alphaDiag),
+                                  P(np.ascontiguousarray(lam)), P(jd)))
+        return jd
+
+    def update_sgs(self, es, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lam, forward, gsUpdate, uIncIsZero,
+                   scanOrder=None):
+        out = np.array(uIncNew, dtype=np.float64, copy=True)
+        s = np.zeros(5)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        _chk(lib().orc_update_sgs(self.h, C.byref(es), C.c_double(alphaDiag), P(np.ascontiguousarray(rhs)),
+                                  P(np.ascontiguousarray(u)), P(np.ascontiguousarray(uInc)), P(out),
+                                  P(np.ascontiguousarray(jdiag)), P(np.ascontiguousarray(lam)), C.c_int(int(forward)),
+                                  C.c_int(int(gsUpdate)), C.c_int(int(uIncIsZero)), P(so), P(s)))
+        return out, s
+
+    def ssprk3(self, es, u, uRec, nStep, CFL, nRec=1, dtFixed=0.0):
+        u = np.array(u, dtype=np.float64, copy=True)
+        uRec = np.array(uRec, dtype=np.float64, copy=True)
+        _chk(lib().orc_ssprk3(self.h, C.byref(es), P(u), P(uRec), C.c_int(nStep), C.c_double(CFL), C.c_int(nRec),
+                              C.c_double(dtFixed)))
+        return u, uRec
