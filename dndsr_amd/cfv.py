@@ -1,0 +1,206 @@
+"""VariationalReconstruction on the device, with the reference's method names.
+
+Mirrors ``DNDS::CFV::VariationalReconstruction<dim>`` (src/CFV/VariationalReconstruction.hpp:108)
+and its Python binding ``CFV.VariationalReconstruction_{2,3}``
+(src/CFV/VariationalReconstruction_bind.hpp:18-144).  Field arrays are torch
+tensors on the GPU in the device layout of include/dndsr_amd.h (cell index
+fastest); torch only provides the memory and the stream, every operator is a
+HIP kernel of libdndsr_amd.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+
+RIEMANN_TYPES = {"Roe": 1, "HLLC": 2, "HLLEP": 3, "HLLEP_V1": 21, "Roe_M1": 11, "Roe_M2": 12, "Roe_M3": 13,
+                 "Roe_M4": 14, "Roe_M5": 15, "Roe_M6": 16, "Roe_M7": 17, "Roe_M8": 18}
+DIR_WEIGHT = {"Factorial": 0, "HQM_OPT": 1}
+GEOM_WEIGHT = {"GWNone": 0, "HQM_SD": 1}
+
+
+def _ptr(t):
+    return C.c_void_p(0 if t is None else t.data_ptr())
+
+
+class Context:
+    """One device context (C-ABI ``dndsr_ctx``) bound to torch's current stream."""
+
+    def __init__(self, device: int = 0):
+        if not torch.cuda.is_available():
+            raise RuntimeError("dndsr_amd needs a GPU (no CPU fallback)")
+        self.device = device
+        h = C.c_void_p()
+        check(lib().dndsr_create(C.byref(h), device))
+        self.h = h
+        self.torch_device = torch.device("cuda", device)
+        with torch.cuda.device(device):
+            stream = torch.cuda.current_stream(device)
+        check(lib().dndsr_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    def sync(self):
+        check(lib().dndsr_sync(self.h))
+
+    def close(self):
+        if getattr(self, "h", None) is not None and self.h.value:
+            lib().dndsr_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class VariationalReconstruction:
+    """``VariationalReconstruction<dim>``: metrics, VR coefficients and the VR iteration on the device."""
+
+    def __init__(self, mesh, settings: dict | None = None, device: int = 0, ctx: Context | None = None):
+        s = dict(maxOrder=1, intOrder=1, dirWeightScheme="Factorial", geomWeightScheme="GWNone",
+                 geomWeightBias=0.0, geomWeightPower=0.5, scaleMultiplier=1.0, jacobiRelax=1.0,
+                 nIterCellSmoothScale=15)
+        if settings:
+            s.update(settings)
+            if "functionalSettings" in settings:
+                s.update(settings["functionalSettings"])
+        if not isinstance(s["dirWeightScheme"], int):
+            s["dirWeightScheme"] = DIR_WEIGHT[s["dirWeightScheme"]]
+        if not isinstance(s["geomWeightScheme"], int):
+            s["geomWeightScheme"] = GEOM_WEIGHT[s["geomWeightScheme"]]
+        self.settings = s
+        self.mesh = mesh
+        self.dim = mesh.dim
+        self.ctx = ctx or Context(device)
+        self._constructed = False
+
+    # ---- construction (ConstructMetrics / ConstructBaseAndWeight / ConstructRecCoeff)
+    def ConstructMetrics(self):
+        if self._constructed:
+            return
+        m = self.mesh
+        keep = [np.ascontiguousarray(x) for x in (
+            m.coords, m.cellType, m.cell2nodePtr, m.cell2node, m.cell2facePtr, m.cell2face, m.faceType,
+            m.face2nodePtr, m.face2node, m.face2cell, m.faceShift)]
+        md = _lib.MeshDesc(m.dim, m.nNode, keep[0].ctypes.data_as(_lib.dp), m.nCell, keep[1].ctypes.data_as(_lib.i32p),
+                           keep[2].ctypes.data_as(_lib.i64p), keep[3].ctypes.data_as(_lib.i64p),
+                           keep[4].ctypes.data_as(_lib.i64p), keep[5].ctypes.data_as(_lib.i64p), m.nFace,
+                           keep[6].ctypes.data_as(_lib.i32p), keep[7].ctypes.data_as(_lib.i64p),
+                           keep[8].ctypes.data_as(_lib.i64p), keep[9].ctypes.data_as(_lib.i64p),
+                           keep[10].ctypes.data_as(_lib.dp))
+        s = self.settings
+        vs = _lib.VRSettings(s["maxOrder"], s["intOrder"], s["dirWeightScheme"], s["geomWeightScheme"],
+                             s["geomWeightBias"], s["geomWeightPower"], s["scaleMultiplier"], s["jacobiRelax"],
+                             s["nIterCellSmoothScale"])
+        check(lib().dndsr_vr_construct(self.ctx.h, C.byref(md), C.byref(vs)))
+        nC, nF = C.c_int64(), C.c_int64()
+        M, K, G, Gc, nCol = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
+        gv = C.c_double()
+        check(lib().dndsr_mesh_info(self.ctx.h, C.byref(nC), C.byref(nF), C.byref(M), C.byref(K), C.byref(G),
+                                    C.byref(Gc), C.byref(nCol), C.byref(gv)))
+        self.nCell, self.nFace, self.M, self.maxFC = nC.value, nF.value, M.value, K.value
+        self.nGFace, self.nGCell, self.nColor, self.globalVol = G.value, Gc.value, nCol.value, gv.value
+        self._constructed = True
+
+    def ConstructBaseAndWeight(self):
+        self.ConstructMetrics()
+
+    def ConstructRecCoeff(self):
+        self.ConstructMetrics()
+
+    def construct(self):
+        self.ConstructMetrics()
+        return self
+
+    # ---- arrays
+    def BuildUDof(self, nVars: int = 5, dtype=torch.float64):
+        return torch.zeros((nVars, self.nCell), dtype=dtype, device=self.ctx.torch_device)
+
+    def BuildURec(self, nVars: int = 5):
+        return torch.zeros((nVars, self.M, self.nCell), dtype=torch.float64, device=self.ctx.torch_device)
+
+    def to_device(self, rows: np.ndarray, out=None):
+        """Reference row layout [nCell, nComp] (uRec: [nCell, nVars, M]) -> device tensor."""
+        rows = np.ascontiguousarray(rows, dtype=np.float64)
+        nComp = int(np.prod(rows.shape[1:]))
+        if out is None:
+            shape = (rows.shape[1], rows.shape[2], self.nCell) if rows.ndim == 3 else (nComp, self.nCell)
+            out = torch.empty(shape, dtype=torch.float64, device=self.ctx.torch_device)
+        check(lib().dndsr_h2d_cellfield(self.ctx.h, rows.ctypes.data_as(C.c_void_p), _ptr(out), nComp))
+        return out
+
+    def to_host(self, dev) -> np.ndarray:
+        """Device tensor -> reference row layout."""
+        nComp = int(np.prod(dev.shape[:-1]))
+        rows = np.empty((self.nCell, nComp))
+        check(lib().dndsr_d2h_cellfield(self.ctx.h, _ptr(dev), rows.ctypes.data_as(C.c_void_p), nComp))
+        if dev.dim() == 3:
+            rows = rows.reshape(self.nCell, dev.shape[0], dev.shape[1])
+        return rows
+
+    # ---- metrics (host copies, reference row layout)
+    def cell_metrics(self):
+        N, M = self.nCell, self.M
+        vol, sc, sm = np.zeros(N), np.zeros(N), np.zeros(N)
+        bary, cent, mom = np.zeros((N, 3)), np.zeros((N, 3)), np.zeros((N, M))
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        check(lib().dndsr_get_cell_metrics(self.ctx.h, P(vol), P(bary), P(cent), P(sc), P(sm), P(mom)))
+        return dict(vol=vol, bary=bary, cent=cent, scale=sc, smooth=sm, moment=mom)
+
+    def face_metrics(self):
+        F, G = self.nFace, self.nGFace
+        area, fl = np.zeros(F), np.zeros(F)
+        mn, pts, nrm = np.zeros((F, 3)), np.zeros((F, G, 3)), np.zeros((F, G, 3))
+        wj, fw = np.zeros((F, G)), np.zeros((F, self.settings["maxOrder"] + 1))
+        P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
+        check(lib().dndsr_get_face_metrics(self.ctx.h, P(area), P(mn), P(pts), P(nrm), P(wj), P(fl), P(fw)))
+        return dict(area=area, meanNorm=mn, pts=pts, norms=nrm, wJDet=wj, faceL=fl, faceWeight=fw)
+
+    def cell_quad(self):
+        pts, wj = np.zeros((self.nCell, self.nGCell, 3)), np.zeros((self.nCell, self.nGCell))
+        check(lib().dndsr_get_cell_quad(self.ctx.h, pts.ctypes.data_as(C.c_void_p), wj.ctypes.data_as(C.c_void_p)))
+        return pts, wj
+
+    def get_matrixAAInvB(self):
+        """(matrixAAInvB [N, K+1, M, M], vectorAInvB [N, K, M]) in the reference's block order."""
+        N, M, K = self.nCell, self.M, self.maxFC
+        A, b = np.zeros((N, K + 1, M, M)), np.zeros((N, K, M))
+        check(lib().dndsr_get_vr_coeffs(self.ctx.h, A.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)))
+        return A, b
+
+    def set_matrixAAInvB(self, A, b):
+        A = np.ascontiguousarray(A, np.float64)
+        b = np.ascontiguousarray(b, np.float64)
+        check(lib().dndsr_set_vr_coeffs(self.ctx.h, A.ctypes.data_as(C.c_void_p), b.ctypes.data_as(C.c_void_p)))
+
+    def colors(self):
+        c = np.zeros(self.nCell, np.int32)
+        check(lib().dndsr_get_colors(self.ctx.h, c.ctypes.data_as(C.c_void_p)))
+        return c
+
+    def cell_average(self, f, nVars: int):
+        """Cell means of a point function f(pts[..., 3]) -> [..., nVars] by cell quadrature
+        (EulerEvaluator.hxx:1046-1068)."""
+        pts, wj = self.cell_quad()
+        vals = f(pts).reshape(self.nCell, self.nGCell, nVars)
+        vol = self.cell_metrics()["vol"]
+        return np.einsum("ngv,ng->nv", vals, wj) / vol[:, None]
+
+    # ---- reconstruction (VariationalReconstruction_Reconstruction.hxx:485-670)
+    def DoReconstructionIter(self, uRec, uRecNew, u, putIntoNew: bool = True, recordInc: bool = False,
+                             uRecIsZero: bool = False, relax: float = -1.0):
+        """Jacobi form.  putIntoNew=False leaves the new iterate in uRec (as the reference does)."""
+        nVars = u.shape[0]
+        mode = 0 if not recordInc else (2 if uRecIsZero else 1)
+        if recordInc and not putIntoNew:
+            raise ValueError("recordInc needs putIntoNew (the -RHS goes into uRecNew)")
+        check(lib().dndsr_vr_iter(self.ctx.h, nVars, _ptr(uRec), _ptr(uRecNew), _ptr(u), mode, float(relax)))
+        if not putIntoNew:
+            uRec.copy_(uRecNew)
+
+    def DoReconstructionIterDiff(self, uRecDiff, uRecNew):
+        check(lib().dndsr_vr_iter_diff(self.ctx.h, uRecDiff.shape[0], _ptr(uRecDiff), _ptr(uRecNew)))

@@ -1,0 +1,655 @@
+// capi.hip -- C-ABI of libdndsr_amd.so (declarations and reference citations in
+// include/dndsr_amd.h).  Host-side responsibilities: mesh upload into the device
+// SoA layout, multicolour ordering of the cells, buffer management and the
+// device-resident SSP-RK3 loop; every numerical operation runs in the HIP kernels
+// of setup.hip, vr.hip and euler.hip.
+#include <algorithm>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/dndsr_amd.h"
+#include "internal.hpp"
+
+using namespace dndsr;
+
+struct dndsr_ctx
+{
+    Ctx c;
+    hipStream_t own = nullptr;
+    // device-resident time-loop workspace
+    real *wXLast = nullptr, *wRhs = nullptr, *wURecNew = nullptr, *wDt = nullptr, *wLam = nullptr, *wDlc = nullptr,
+         *wDtMin = nullptr;
+    bool haveWork = false;
+};
+
+static thread_local std::string g_err;
+
+#define CAPI_BEGIN try {
+#define CAPI_END                                      \
+    return 0;                                         \
+    }                                                 \
+    catch (const std::exception &e)                   \
+    {                                                 \
+        g_err = e.what();                             \
+        return 1;                                     \
+    }                                                 \
+    catch (...)                                       \
+    {                                                 \
+        g_err = "unknown error";                      \
+        return 1;                                     \
+    }
+
+static void need(bool ok, const char *msg)
+{
+    if (!ok)
+        throw std::runtime_error(msg);
+}
+
+static void requireMesh(dndsr_ctx *ctx)
+{
+    need(ctx != nullptr, "null context");
+    need(ctx->c.haveMesh, "no mesh: call dndsr_vr_construct first");
+}
+
+template <typename T>
+static void upload(Ctx &c, T *dev, const std::vector<T> &h)
+{
+    DNDSR_HIP_CHECK(hipMemcpyAsync(dev, h.data(), sizeof(T) * h.size(), hipMemcpyHostToDevice, c.stream));
+}
+template <typename T>
+static std::vector<T> download(Ctx &c, const T *dev, size_t n)
+{
+    std::vector<T> h(n);
+    DNDSR_HIP_CHECK(hipMemcpyAsync(h.data(), dev, sizeof(T) * n, hipMemcpyDeviceToHost, c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    return h;
+}
+
+static void freeAll(Ctx &c)
+{
+    for (void *p : c.owned)
+        (void)hipFree(p);
+    c.owned.clear();
+    if (c.hostRed)
+        (void)hipHostFree(c.hostRed);
+    c.hostRed = nullptr;
+}
+
+extern "C" {
+
+const char *dndsr_last_error(void) { return g_err.c_str(); }
+
+int dndsr_create(dndsr_ctx **out, int device)
+{
+    CAPI_BEGIN
+    need(out != nullptr, "null output");
+    int n = 0;
+    DNDSR_HIP_CHECK(hipGetDeviceCount(&n));
+    need(device >= 0 && device < n, "invalid device index");
+    DNDSR_HIP_CHECK(hipSetDevice(device));
+    auto *ctx = new dndsr_ctx;
+    ctx->c.device = device;
+    DNDSR_HIP_CHECK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
+    ctx->c.stream = ctx->own;
+    DNDSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&ctx->c.hostRed), sizeof(real) * 16));
+    *out = ctx;
+    CAPI_END
+}
+
+int dndsr_destroy(dndsr_ctx *ctx)
+{
+    CAPI_BEGIN
+    if (!ctx)
+        return 0;
+    (void)hipSetDevice(ctx->c.device);
+    (void)hipStreamSynchronize(ctx->c.stream);
+    freeAll(ctx->c);
+    if (ctx->own)
+        (void)hipStreamDestroy(ctx->own);
+    delete ctx;
+    CAPI_END
+}
+
+int dndsr_sync(dndsr_ctx *ctx)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    DNDSR_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    CAPI_END
+}
+
+int dndsr_set_stream(dndsr_ctx *ctx, void *stream)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    ctx->c.stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own;
+    CAPI_END
+}
+
+int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr_settings *s)
+{
+    CAPI_BEGIN
+    need(ctx && md && s, "null argument");
+    need(!ctx->c.haveMesh, "context already holds a mesh");
+    need(md->dim == 2 || md->dim == 3, "dim must be 2 or 3");
+    need(s->maxOrder >= 1 && s->maxOrder <= 3, "maxOrder must be 1..3");
+    need(s->intOrder >= 0 && s->intOrder <= 6, "intOrder must be 0..6");
+    need(md->nCell > 1 && md->nCell < (int64_t(1) << 31) && md->nFace < (int64_t(1) << 31), "mesh size out of range");
+    Ctx &c = ctx->c;
+    DNDSR_HIP_CHECK(hipSetDevice(c.device));
+    DevMesh &m = c.m;
+    m.dim = md->dim;
+    m.P = s->maxOrder;
+    m.NDOF = nDof(m.dim, m.P);
+    m.M = m.NDOF - 1;
+    m.N = (idx)md->nCell;
+    m.F = (idx)md->nFace;
+    m.nNode = (idx)md->nNode;
+    m.s = VRSettingsDev{s->maxOrder, s->intOrder, s->dirWeightScheme, s->geomWeightScheme, s->geomWeightBias,
+                        s->geomWeightPower, s->scaleMultiplier, s->jacobiRelax, s->nIterCellSmoothScale};
+    const size_t N = m.N, F = m.F;
+    // element types: one cell type and one face type per mesh on this path
+    m.cellType = md->cellType[0];
+    m.faceType = md->faceType[0];
+    for (size_t i = 0; i < N; i++)
+        need(md->cellType[i] == m.cellType, "mixed cell types are not supported");
+    for (size_t f = 0; f < F; f++)
+        need(md->faceType[f] == m.faceType, "mixed face types are not supported");
+    if (m.dim == 2)
+        need(m.cellType == DNDSR_QUAD4 && m.faceType == DNDSR_LINE2, "2-D meshes must be Quad4 cells / Line2 faces");
+    else
+        need(m.cellType == DNDSR_HEX8 && m.faceType == DNDSR_QUAD4, "3-D meshes must be Hex8 cells / Quad4 faces");
+    const int nnc = m.dim == 2 ? 4 : 8, nnf = m.dim == 2 ? 2 : 4;
+    int K = 0;
+    for (size_t i = 0; i < N; i++)
+    {
+        need(md->cell2nodePtr[i + 1] - md->cell2nodePtr[i] == nnc, "bad cell node count");
+        K = std::max<int>(K, (int)(md->cell2facePtr[i + 1] - md->cell2facePtr[i]));
+    }
+    need(K <= kMaxFC, "too many faces per cell");
+    m.K = K;
+    int n1d = s->intOrder <= 1 ? 1 : s->intOrder <= 3 ? 2 : s->intOrder <= 5 ? 3 : 4;
+    m.G = m.dim == 2 ? n1d : n1d * n1d;
+    m.Gc = m.dim == 2 ? n1d * n1d : n1d * n1d * n1d;
+    need(m.G <= kMaxG, "too many face quadrature points");
+
+    std::vector<real> hc(3 * (size_t)m.nNode);
+    for (size_t a = 0; a < (size_t)m.nNode; a++)
+        for (int r = 0; r < 3; r++)
+            hc[r * m.nNode + a] = md->coords[3 * a + r];
+    std::vector<idx> hcn((size_t)nnc * N), hfn((size_t)nnf * F), hf2c(2 * F), hc2f((size_t)K * N, -1);
+    for (size_t i = 0; i < N; i++)
+        for (int a = 0; a < nnc; a++)
+        {
+            int64_t nd = md->cell2node[md->cell2nodePtr[i] + a];
+            need(nd >= 0 && nd < md->nNode, "cell2node out of range");
+            hcn[a * N + i] = (idx)nd;
+        }
+    std::vector<real> hsh(3 * F);
+    for (size_t f = 0; f < F; f++)
+    {
+        need(md->face2nodePtr[f + 1] - md->face2nodePtr[f] == nnf, "bad face node count");
+        for (int a = 0; a < nnf; a++)
+        {
+            int64_t nd = md->face2node[md->face2nodePtr[f] + a];
+            need(nd >= 0 && nd < md->nNode, "face2node out of range");
+            hfn[a * F + f] = (idx)nd;
+        }
+        int64_t L = md->face2cell[2 * f], R = md->face2cell[2 * f + 1];
+        need(L >= 0 && L < (int64_t)N && R < (int64_t)N, "face2cell out of range");
+        need(R >= 0, "boundary faces are not supported on this path yet (periodic / interior meshes only)");
+        hf2c[f] = (idx)L, hf2c[F + f] = (idx)R;
+        for (int r = 0; r < 3; r++)
+            hsh[r * F + f] = md->faceShift[3 * f + r];
+    }
+    for (size_t i = 0; i < N; i++)
+    {
+        int k = 0;
+        for (int64_t q = md->cell2facePtr[i]; q < md->cell2facePtr[i + 1]; q++, k++)
+        {
+            int64_t f = md->cell2face[q];
+            need(f >= 0 && f < (int64_t)F, "cell2face out of range");
+            need(hf2c[f] == (idx)i || hf2c[F + f] == (idx)i, "cell2face / face2cell mismatch");
+            hc2f[(size_t)k * N + i] = (idx)f;
+        }
+    }
+    // greedy multicolouring in cell order (smallest colour unused by face neighbours)
+    std::vector<int32_t> col(N, -1);
+    int nColor = 0;
+    for (size_t i = 0; i < N; i++)
+    {
+        uint64_t used = 0;
+        for (int k = 0; k < K; k++)
+        {
+            idx f = hc2f[(size_t)k * N + i];
+            if (f < 0)
+                continue;
+            idx o = hf2c[f] == (idx)i ? hf2c[F + f] : hf2c[f];
+            if (o >= 0 && o != (idx)i && col[o] >= 0 && col[o] < 64)
+                used |= (uint64_t(1) << col[o]);
+        }
+        int cc = 0;
+        while (cc < 64 && (used >> cc) & 1)
+            cc++;
+        need(cc < 64, "colouring needs more than 64 colours");
+        col[i] = cc;
+        nColor = std::max(nColor, cc + 1);
+    }
+    m.nColor = nColor;
+    m.colorPtrHost.assign(nColor + 1, 0);
+    for (size_t i = 0; i < N; i++)
+        m.colorPtrHost[col[i] + 1]++;
+    for (int q = 0; q < nColor; q++)
+        m.colorPtrHost[q + 1] += m.colorPtrHost[q];
+    std::vector<idx> cells(N), fill(m.colorPtrHost.begin(), m.colorPtrHost.end() - 1);
+    for (size_t i = 0; i < N; i++)
+        cells[fill[col[i]]++] = (idx)i;
+
+    // device allocations
+    m.coords = devAlloc<real>(c, hc.size());
+    m.cell2node = devAlloc<idx>(c, hcn.size());
+    m.face2node = devAlloc<idx>(c, hfn.size());
+    m.f2c = devAlloc<idx>(c, hf2c.size());
+    m.c2f = devAlloc<idx>(c, hc2f.size());
+    m.fshift = devAlloc<real>(c, hsh.size());
+    m.color = devAlloc<int32_t>(c, N);
+    m.colorCells = devAlloc<idx>(c, N);
+    upload(c, m.coords, hc);
+    upload(c, m.cell2node, hcn);
+    upload(c, m.face2node, hfn);
+    upload(c, m.f2c, hf2c);
+    upload(c, m.c2f, hc2f);
+    upload(c, m.fshift, hsh);
+    upload(c, m.color, col);
+    upload(c, m.colorCells, cells);
+    m.fpts = devAlloc<real>(c, (size_t)m.G * 3 * F);
+    m.fnorm = devAlloc<real>(c, (size_t)m.G * 3 * F);
+    m.fwj = devAlloc<real>(c, (size_t)m.G * F);
+    m.farea = devAlloc<real>(c, F);
+    m.fmeanN = devAlloc<real>(c, 3 * F);
+    m.fcent = devAlloc<real>(c, 3 * F);
+    m.faceL = devAlloc<real>(c, F);
+    m.faceW = devAlloc<real>(c, (size_t)(m.P + 1) * F);
+    m.vol = devAlloc<real>(c, N);
+    m.cent = devAlloc<real>(c, 3 * N);
+    m.bary = devAlloc<real>(c, 3 * N);
+    m.scale = devAlloc<real>(c, N);
+    m.smooth = devAlloc<real>(c, N);
+    m.moment = devAlloc<real>(c, (size_t)m.M * N);
+    m.cpts = devAlloc<real>(c, (size_t)m.Gc * 3 * N);
+    m.cwj = devAlloc<real>(c, (size_t)m.Gc * N);
+    m.AinvB = devAlloc<real>(c, (size_t)(K + 1) * m.M * m.M * N);
+    m.Ainvb = devAlloc<real>(c, (size_t)K * m.M * N);
+    c.faceFlux = devAlloc<real>(c, 5 * F);
+    c.faceLamTmp = devAlloc<real>(c, F);
+    size_t nPart = (size_t)gridFor(m.N, 256) + (size_t)nColor + 8;
+    c.red = devAlloc<real>(c, nPart * 5);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+
+    setupMetricsAndVR(c);
+    auto vol = download(c, m.vol, N);
+    m.globalVol = 0;
+    for (real v : vol)
+        m.globalVol += v;
+    c.haveMesh = true;
+    CAPI_END
+}
+
+int dndsr_mesh_info(dndsr_ctx *ctx, int64_t *nCell, int64_t *nFace, int *M, int *maxFaceCell, int *nGFace,
+                    int *nGCell, int *nColor, double *globalVol)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    const DevMesh &m = ctx->c.m;
+    if (nCell)
+        *nCell = m.N;
+    if (nFace)
+        *nFace = m.F;
+    if (M)
+        *M = m.M;
+    if (maxFaceCell)
+        *maxFaceCell = m.K;
+    if (nGFace)
+        *nGFace = m.G;
+    if (nGCell)
+        *nGCell = m.Gc;
+    if (nColor)
+        *nColor = m.nColor;
+    if (globalVol)
+        *globalVol = m.globalVol;
+    CAPI_END
+}
+
+int dndsr_get_cell_metrics(dndsr_ctx *ctx, double *vol, double *bary, double *cent, double *scale,
+                           double *smoothRatio, double *moment)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    const DevMesh &m = c.m;
+    const size_t N = m.N;
+    auto v = download(c, m.vol, N), b = download(c, m.bary, 3 * N), ce = download(c, m.cent, 3 * N);
+    auto sc = download(c, m.scale, N), sm = download(c, m.smooth, N), mo = download(c, m.moment, (size_t)m.M * N);
+    for (size_t i = 0; i < N; i++)
+    {
+        if (vol)
+            vol[i] = v[i];
+        if (scale)
+            scale[i] = sc[i];
+        if (smoothRatio)
+            smoothRatio[i] = sm[i];
+        for (int r = 0; r < 3; r++)
+        {
+            if (bary)
+                bary[3 * i + r] = b[r * N + i];
+            if (cent)
+                cent[3 * i + r] = ce[r * N + i];
+        }
+        if (moment)
+            for (int k = 0; k < m.M; k++)
+                moment[i * m.M + k] = mo[k * N + i];
+    }
+    CAPI_END
+}
+
+int dndsr_get_face_metrics(dndsr_ctx *ctx, double *area, double *meanNorm, double *pts, double *norms,
+                           double *wJDet, double *faceL, double *faceWeight)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    const DevMesh &m = c.m;
+    const size_t F = m.F, G = m.G;
+    auto a = download(c, m.farea, F), mn = download(c, m.fmeanN, 3 * F), p = download(c, m.fpts, G * 3 * F);
+    auto n = download(c, m.fnorm, G * 3 * F), w = download(c, m.fwj, G * F), l = download(c, m.faceL, F);
+    auto fw = download(c, m.faceW, (size_t)(m.P + 1) * F);
+    for (size_t f = 0; f < F; f++)
+    {
+        if (area)
+            area[f] = a[f];
+        if (faceL)
+            faceL[f] = l[f];
+        for (int r = 0; r < 3; r++)
+            if (meanNorm)
+                meanNorm[3 * f + r] = mn[r * F + f];
+        for (size_t g = 0; g < G; g++)
+        {
+            for (int r = 0; r < 3; r++)
+            {
+                if (pts)
+                    pts[(f * G + g) * 3 + r] = p[(g * 3 + r) * F + f];
+                if (norms)
+                    norms[(f * G + g) * 3 + r] = n[(g * 3 + r) * F + f];
+            }
+            if (wJDet)
+                wJDet[f * G + g] = w[g * F + f];
+        }
+        if (faceWeight)
+            for (int o = 0; o <= m.P; o++)
+                faceWeight[f * (m.P + 1) + o] = fw[o * F + f];
+    }
+    CAPI_END
+}
+
+int dndsr_get_cell_quad(dndsr_ctx *ctx, double *pts, double *wJDet)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    const DevMesh &m = c.m;
+    const size_t N = m.N, G = m.Gc;
+    auto p = download(c, m.cpts, G * 3 * N), w = download(c, m.cwj, G * N);
+    for (size_t i = 0; i < N; i++)
+        for (size_t g = 0; g < G; g++)
+        {
+            for (int r = 0; r < 3; r++)
+                pts[(i * G + g) * 3 + r] = p[(g * 3 + r) * N + i];
+            wJDet[i * G + g] = w[g * N + i];
+        }
+    CAPI_END
+}
+
+int dndsr_get_vr_coeffs(dndsr_ctx *ctx, double *AinvB, double *Ainvb)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    const DevMesh &m = c.m;
+    const size_t N = m.N, K = m.K, M = m.M;
+    auto B = download(c, m.AinvB, (K + 1) * M * M * N);
+    auto b = download(c, m.Ainvb, K * M * N);
+    for (size_t i = 0; i < N; i++)
+    {
+        for (size_t q = 0; q < (K + 1) * M * M; q++)
+            AinvB[i * (K + 1) * M * M + q] = B[q * N + i];
+        for (size_t q = 0; q < K * M; q++)
+            Ainvb[i * K * M + q] = b[q * N + i];
+    }
+    CAPI_END
+}
+
+int dndsr_set_vr_coeffs(dndsr_ctx *ctx, const double *AinvB, const double *Ainvb)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    const DevMesh &m = c.m;
+    const size_t N = m.N, K = m.K, M = m.M;
+    std::vector<real> B((K + 1) * M * M * N), b(K * M * N);
+    for (size_t i = 0; i < N; i++)
+    {
+        for (size_t q = 0; q < (K + 1) * M * M; q++)
+            B[q * N + i] = AinvB[i * (K + 1) * M * M + q];
+        for (size_t q = 0; q < K * M; q++)
+            b[q * N + i] = Ainvb[i * K * M + q];
+    }
+    upload(c, m.AinvB, B);
+    upload(c, m.Ainvb, b);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    CAPI_END
+}
+
+int dndsr_get_colors(dndsr_ctx *ctx, int32_t *color)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    auto h = download(ctx->c, ctx->c.m.color, ctx->c.m.N);
+    std::copy(h.begin(), h.end(), color);
+    CAPI_END
+}
+
+int dndsr_h2d_cellfield(dndsr_ctx *ctx, const double *hostRows, double *dev, int nComp)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(hostRows && dev && nComp > 0, "bad argument");
+    const size_t N = ctx->c.m.N;
+    std::vector<real> t((size_t)nComp * N);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < nComp; k++)
+            t[(size_t)k * N + i] = hostRows[i * nComp + k];
+    upload(ctx->c, dev, t);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    CAPI_END
+}
+
+int dndsr_d2h_cellfield(dndsr_ctx *ctx, const double *dev, double *hostRows, int nComp)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(hostRows && dev && nComp > 0, "bad argument");
+    const size_t N = ctx->c.m.N;
+    auto t = download(ctx->c, dev, (size_t)nComp * N);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < nComp; k++)
+            hostRows[i * nComp + k] = t[(size_t)k * N + i];
+    CAPI_END
+}
+
+int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew, const double *u, int mode,
+                  double relax)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(uRec && uRecNew && u, "null field");
+    need(uRec != uRecNew, "uRec and uRecNew must be different arrays (Jacobi form)");
+    need(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
+    launchVRIter(ctx->c, nVars, uRec, uRecNew, u, mode, relax < 0 ? ctx->c.m.s.jacobiRelax : relax);
+    CAPI_END
+}
+
+int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRecDiff, double *uRecNew)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(uRecDiff && uRecNew && uRecDiff != uRecNew, "bad fields");
+    launchVRIterDiff(ctx->c, nVars, uRecDiff, uRecNew);
+    CAPI_END
+}
+
+static GasParams toGas(const dndsr_euler_settings *es)
+{
+    need(es != nullptr, "null euler settings");
+    need(es->gamma > 1.0, "gamma must exceed 1");
+    return GasParams{es->gamma, es->rsType, es->rsMeanValueEig, es->rsFixScale, es->rsIncFScale};
+}
+
+int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, double CFL, double maxDt,
+                  int useLocalDt, double *dt, double *lambdaFace, double *deltaLambdaCell, double *dtMinDev,
+                  double *dtMinHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && dt && lambdaFace, "null field");
+    Ctx &c = ctx->c;
+    real *dmin = dtMinDev ? dtMinDev : c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 7);
+    launchEvalDt(c, toGas(es), u, CFL, maxDt, useLocalDt, dt, lambdaFace, deltaLambdaCell, dmin);
+    if (dtMinHost)
+    {
+        DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, dmin, sizeof(real), hipMemcpyDeviceToHost, c.stream));
+        DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        *dtMinHost = c.hostRed[0];
+    }
+    CAPI_END
+}
+
+int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                   const double *deltaLambdaCell, double *rhs)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && rhs, "null field");
+    launchEvalRHS(ctx->c, toGas(es), u, uRec, deltaLambdaCell, rhs);
+    CAPI_END
+}
+
+int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alphaDiag, const double *lambdaFace,
+                     double *jdiag)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(dTau && lambdaFace && jdiag, "null field");
+    launchLUSGSInit(ctx->c, dTau, dt, alphaDiag, lambdaFace, jdiag);
+    CAPI_END
+}
+
+static void fetchSum(Ctx &c, real *sumIncHost)
+{
+    if (!sumIncHost)
+        return;
+    real *dev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    // the final reduce already wrote there (see callers)
+    DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, dev, 5 * sizeof(real), hipMemcpyDeviceToHost, c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (int k = 0; k < 5; k++)
+        sumIncHost[k] = c.hostRed[k];
+}
+
+int dndsr_update_sgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
+                     const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                     const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && uInc && uIncNew && jdiag && lambdaFace, "null field");
+    need(uInc != uIncNew, "uInc and uIncNew must be different arrays");
+    Ctx &c = ctx->c;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    launchSGSSweep64(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
+                     sumDev);
+    fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
+int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
+                         const float *u, const float *uInc, float *uIncNew, const float *jdiag,
+                         const float *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && uInc && uIncNew && jdiag && lambdaFace, "null field");
+    need(uInc != uIncNew, "uInc and uIncNew must be different arrays");
+    Ctx &c = ctx->c;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    launchSGSSweep32(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
+                     sumDev);
+    fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
+int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, double *uRec, int nStep, double CFL,
+                 int nRec, double dtFixed)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && uRec && nStep >= 0 && nRec >= 0, "bad argument");
+    Ctx &c = ctx->c;
+    DevMesh &m = c.m;
+    GasParams gp = toGas(es);
+    const size_t N = m.N;
+    const size_t nU = 5 * N, nR = 5 * (size_t)m.M * N;
+    if (!ctx->haveWork)
+    {
+        ctx->wXLast = devAlloc<real>(c, nU);
+        ctx->wRhs = devAlloc<real>(c, nU);
+        ctx->wURecNew = devAlloc<real>(c, nR);
+        ctx->wDt = devAlloc<real>(c, N);
+        ctx->wLam = devAlloc<real>(c, 4 * (size_t)m.F);
+        ctx->wDlc = devAlloc<real>(c, N);
+        ctx->wDtMin = devAlloc<real>(c, 1);
+        ctx->haveWork = true;
+    }
+    real *rec = uRec, *recNew = ctx->wURecNew;
+    auto frhs = [&](const real *x) {
+        for (int r = 0; r < nRec; r++)
+        {
+            launchVRIter(c, 5, rec, recNew, x, 0, m.s.jacobiRelax);
+            std::swap(rec, recNew);
+        }
+        launchEvalRHS(c, gp, x, rec, ctx->wDlc, ctx->wRhs);
+    };
+    for (int step = 0; step < nStep; step++)
+    {
+        launchEvalDt(c, gp, u, CFL, 1e100, 0, ctx->wDt, ctx->wLam, ctx->wDlc, ctx->wDtMin);
+        const real *dtp = ctx->wDtMin;
+        if (dtFixed > 0)
+        {
+            c.hostRed[0] = dtFixed;
+            DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->wDtMin, c.hostRed, sizeof(real), hipMemcpyHostToDevice, c.stream));
+            DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        }
+        DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->wXLast, u, nU * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+        frhs(u);
+        launchAxpby(c, (idx)nU, 1.0, u, 0.0, nullptr, 1.0, ctx->wRhs, dtp, u);
+        frhs(u);
+        launchAxpby(c, (idx)nU, 0.25, u, 0.75, ctx->wXLast, 0.25, ctx->wRhs, dtp, u);
+        frhs(u);
+        launchAxpby(c, (idx)nU, 2. / 3., u, 1. / 3., ctx->wXLast, 2. / 3., ctx->wRhs, dtp, u);
+    }
+    if (rec != uRec) // odd number of VR sweeps in total: bring the result home
+        DNDSR_HIP_CHECK(hipMemcpyAsync(uRec, rec, nR * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+    CAPI_END
+}
+
+} // extern "C"

@@ -1,0 +1,99 @@
+// dev_common.hpp -- shared device helpers, tables and the device mesh/state layout.
+//
+// Device layout (structure of arrays, cell- or face-index fastest so that a
+// wavefront of consecutive cells/faces reads contiguous 512 B segments):
+//   u        [5][N]            conservative means
+//   uRec     [nVars][M][N]     reconstruction coefficients
+//   rhs      [5][N]
+//   AinvB    [K+1][M][M][N]    block 0 = A^-1, block 1+f = A^-1 B_f (K = max faces per cell)
+//   Ainvb    [K][M][N]
+//   face points / normals [G][3][F], w*JDet [G][F]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace dndsr
+{
+using real = double;
+using idx = int32_t;
+
+constexpr real kVerySmall = 1e-200; // src/DNDS/Defines.hpp:199
+constexpr real kSmall = 1e-10;      // src/DNDS/Defines.hpp:201
+constexpr int kMaxG = 16;           // max face/cell quadrature points handled per entity
+constexpr int kMaxFC = 6;           // max faces per cell (hexahedron)
+
+// Monomial exponent tables in the reference's derivative / basis order
+// (src/Geom/BaseFunction.hpp:53-90).  Same table for basis index and
+// derivative-operator index.
+__host__ __device__ inline void monoExp(int dim, int k, int &px, int &py, int &pz)
+{
+    // 3-D order: 1 x y z xx yy zz xy yz xz xxx yyy zzz xxy xyy yyz yzz xzz xxz xyz
+    const int8_t e3[20][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {2, 0, 0}, {0, 2, 0}, {0, 0, 2}, {1, 1, 0}, {0, 1, 1}, {1, 0, 1}, {3, 0, 0}, {0, 3, 0}, {0, 0, 3}, {2, 1, 0}, {1, 2, 0}, {0, 2, 1}, {0, 1, 2}, {1, 0, 2}, {2, 0, 1}, {1, 1, 1}};
+    // 2-D order: 1 x y xx xy yy xxx xxy xyy yyy
+    const int8_t e2[10][3] = {{0, 0, 0}, {1, 0, 0}, {0, 1, 0}, {2, 0, 0}, {1, 1, 0}, {0, 2, 0}, {3, 0, 0}, {2, 1, 0}, {1, 2, 0}, {0, 3, 0}};
+    if (dim == 2)
+        px = e2[k][0], py = e2[k][1], pz = e2[k][2];
+    else
+        px = e3[k][0], py = e3[k][1], pz = e3[k][2];
+}
+
+// number of polynomial DOFs for degree p (src/Geom/BaseFunction.hpp:233-273)
+__host__ __device__ constexpr int nDof(int dim, int p)
+{
+    return dim == 2 ? (p == 0 ? 1 : p == 1 ? 3 : p == 2 ? 6 : 10) : (p == 0 ? 1 : p == 1 ? 4 : p == 2 ? 10 : 20);
+}
+
+__host__ __device__ inline real ipow(int p, real x)
+{
+    return p == 0 ? 1.0 : (p == 1 ? x : (p == 2 ? x * x : (p == 3 ? x * x * x : 1e300)));
+}
+__host__ __device__ inline int dfact(int p, int d)
+{
+    // p! / (p-d)! for d <= p (dFactorials, BaseFunction.hpp:192-195)
+    if (d > p)
+        return 0;
+    int r = 1;
+    for (int q = 0; q < d; q++)
+        r *= (p - q);
+    return r;
+}
+
+// Derivative `id` of basis monomial `ib` at scaled point xs with isotropic scale s
+// (FPolynomial3D / FPolynomialFill, BaseFunction.hpp:222-227, 423-437).
+__host__ __device__ inline real basisDiff(int dim, int id, int ib, const real *xs, real s)
+{
+    int px, py, pz, dx, dy, dz;
+    monoExp(dim, ib, px, py, pz);
+    monoExp(dim, id, dx, dy, dz);
+    int c = dfact(px, dx) * dfact(py, dy) * dfact(pz, dz);
+    if (!c)
+        return 0.0;
+    real v = c * ipow(px - dx, xs[0]) * ipow(py - dy, xs[1]) * ipow(pz - dz, xs[2]);
+    real sz = dim == 2 ? 1.0 : s;
+    return v / (ipow(dx, s) * ipow(dy, s) * ipow(dz, sz));
+}
+
+// Row-0 basis values (j = 1..NDOF-1) minus the cell moment, i.e. the
+// zero-mean D0 row used by reconstruction (VariationalReconstruction.hpp:317-369).
+template <int DIM, int NDOF>
+__device__ inline void basisRow0(const real *xs, const real *moment /*[NDOF-1]*/, real *phi)
+{
+#pragma unroll
+    for (int j = 1; j < NDOF; j++)
+    {
+        int px, py, pz;
+        monoExp(DIM, j, px, py, pz);
+        phi[j - 1] = ipow(px, xs[0]) * ipow(py, xs[1]) * ipow(pz, xs[2]) - moment[j - 1];
+    }
+}
+
+struct GasParams
+{
+    real gamma;
+    int rsType;
+    int rsMeanValueEig;
+    real fixScale;
+    real incFScale;
+};
+
+} // namespace dndsr

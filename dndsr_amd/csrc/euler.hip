@@ -1,0 +1,528 @@
+// euler.hip -- EulerEvaluator hot path on the device: EvaluateDt, EvaluateRHS,
+// LUSGSMatrixInit, the multicolour symmetric Gauss-Seidel sweeps and the
+// SSP-RK3 stage combination.
+#include "gas.hpp"
+#include "internal.hpp"
+
+namespace dndsr
+{
+// ---------------------------------------------------------------------------
+// EvaluateDt (EulerEvaluator_EvaluateDt.hxx:840-1048), inviscid.
+// Face pass: spectral radius lambdaConvection = max(a_L + |v_L.n|, a_R + |v_R.n|)
+// and deltaLambdaFace; cell pass: dt = CFL V smooth / (sum lambda A),
+// deltaLambdaCell = max over faces; then a two-stage min for dtMin.
+// lambdaFace layout [4][F]: lambdaFace, lambdaFaceC, lambdaFaceVis, lambdaFace0(=123=4).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kDtFace(DevMesh m, real gamma, const real *__restrict__ u, real *__restrict__ lam,
+                                               real *__restrict__ dLamF)
+{
+    idx f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F)
+        return;
+    const size_t N = m.N, F = m.F;
+    idx cL = m.f2c[f], cR = m.f2c[F + f];
+    idx cR2 = cR >= 0 ? cR : cL;
+    real UL[5], UR[5], n[3];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        UL[v] = u[v * N + cL], UR[v] = u[v * N + cR2];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        n[r] = m.fmeanN[r * F + f];
+    real vL[3] = {UL[1] / UL[0], UL[2] / UL[0], UL[3] / UL[0]};
+    real vR[3] = {UR[1] / UR[0], UR[2] / UR[0], UR[3] / UR[0]};
+    real pL, aL2, HL, pR, aR2, HR;
+    thermal(UL[4], UL[0], dot3(vL, vL), gamma, pL, aL2, HL);
+    thermal(UR[4], UR[0], dot3(vR, vR), gamma, pR, aR2, HR);
+    real vnL = dot3(vL, n), vnR = dot3(vR, n);
+    real vs[3] = {vL[0] + vR[0], vL[1] + vR[1], vL[2] + vR[2]};
+    real vnMean = 0.5 * dot3(vs, n);
+    real aL = sqrt(aL2), aR = sqrt(aR2);
+    real lc = fmax(aL + fabs(vnL), aR + fabs(vnR));
+    real visc = 0.0; // inviscid: lamVis = 0
+    lam[f] = lc + visc;
+    lam[F + f] = fabs(vnMean) + visc;
+    lam[2 * F + f] = visc;
+    lam[3 * F + f] = lc;
+    real dv[3] = {vR[0] - vL[0], vR[1] - vL[1], vR[2] - vL[2]};
+    real dvn = dot3(dv, n);
+    dLamF[f] = max3(fabs(dvn + aR - aL), fabs(dvn - aR + aL), fabs(aL - aR));
+}
+
+__global__ void __launch_bounds__(256) kDtCell(DevMesh m, real CFL, real maxDt, const real *__restrict__ lam,
+                                               const real *__restrict__ dLamF, real *__restrict__ dt,
+                                               real *__restrict__ dlc, real *__restrict__ partMin)
+{
+    __shared__ real sm[256];
+    idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    real d = 3e200;
+    if (i < m.N)
+    {
+        real lc = 0, dl = 0;
+        for (int k = 0; k < m.K; k++)
+        {
+            idx f = m.c2f[(size_t)k * m.N + i];
+            if (f < 0)
+                continue;
+            lc += lam[f] * m.farea[f];
+            dl = fmax(dl, dLamF[f]);
+        }
+        d = fmin(CFL * m.vol[i] * m.smooth[i] / (lc + 1e-100), maxDt);
+        dt[i] = d;
+        if (dlc)
+            dlc[i] = dl;
+    }
+    sm[threadIdx.x] = d;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        partMin[blockIdx.x] = sm[0];
+}
+
+// single-block reductions over nPart partials (min or sum over `width` interleaved values)
+__global__ void __launch_bounds__(256) kReduceMin(const real *part, int nPart, real *out)
+{
+    __shared__ real sm[256];
+    real d = 3e200;
+    for (int q = threadIdx.x; q < nPart; q += 256)
+        d = fmin(d, part[q]);
+    sm[threadIdx.x] = d;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+            sm[threadIdx.x] = fmin(sm[threadIdx.x], sm[threadIdx.x + s]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+        out[0] = sm[0];
+}
+__global__ void kFillFromScalar(idx n, const real *s, real *out)
+{
+    idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = s[0];
+}
+
+void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,
+                  real *lambdaFace, real *deltaLambdaCell, real *dtMinDev)
+{
+    DevMesh &m = c.m;
+    kDtFace<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, gp.gamma, u, lambdaFace, c.faceLamTmp);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    int nb = gridFor(m.N, 256);
+    kDtCell<<<nb, 256, 0, c.stream>>>(m, CFL, maxDt, lambdaFace, c.faceLamTmp, dt, deltaLambdaCell, c.red);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    kReduceMin<<<1, 256, 0, c.stream>>>(c.red, nb, dtMinDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    if (!useLocalDt)
+    {
+        kFillFromScalar<<<nb, 256, 0, c.stream>>>(m.N, dtMinDev, dt);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
+}
+
+// ---------------------------------------------------------------------------
+// EvaluateRHS (EulerEvaluator_EvaluateRHS.hxx:171-662), inviscid, interior and
+// periodic faces.  Face pass: one thread per face evaluates the reconstructed
+// states U = u + D0phi(x_g) uRec on both sides at every face quadrature point
+// (basis recomputed from the cell centre, scale and moments), the Riemann flux,
+// and integrates -F w JDet.  Gather pass (the reference's face-buffer + cell
+// gather form, :640-662): rhs_i = sum over cell2face of +-F_f / V_i.  No atomics.
+// ---------------------------------------------------------------------------
+template <int DIM, int NDOF, int RS, bool BATCH>
+__global__ void __launch_bounds__(128) kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                               const real *__restrict__ uRec, const real *__restrict__ dlc,
+                                               real *__restrict__ fflux)
+{
+    constexpr int M = NDOF - 1;
+    idx f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F)
+        return;
+    const size_t N = m.N, F = m.F;
+    idx cL = m.f2c[f], cR = m.f2c[F + f];
+    real uL[5], uR[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        uL[v] = u[v * N + cL], uR[v] = u[v * N + cR];
+    real cenL[3], cenR[3], sL = 1, sR = 1, momL[M > 0 ? M : 1], momR[M > 0 ? M : 1], sh[3];
+    if (uRec)
+    {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cenL[r] = m.cent[r * N + cL], cenR[r] = m.cent[r * N + cR], sh[r] = m.fshift[r * F + f];
+        sL = m.scale[cL], sR = m.scale[cR];
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            momL[j] = m.moment[j * N + cL], momR[j] = m.moment[j * N + cR];
+    }
+    real dLambda = dlc ? fmax(dlc[cL], dlc[cR]) : 0.0;
+    RoeBatchFace bf;
+    if constexpr (BATCH)
+    {
+        real nm[3] = {m.fmeanN[f], m.fmeanN[F + f], m.fmeanN[2 * F + f]};
+        roeBatchSetup<RS>(uL, uR, nm, gp, dLambda, bf);
+    }
+    real acc[5] = {0, 0, 0, 0, 0};
+    for (int g = 0; g < m.G; g++)
+    {
+        real x[3], n[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            x[r] = m.fpts[((size_t)g * 3 + r) * F + f], n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+        real wj = m.fwj[(size_t)g * F + f];
+        real UL[5], UR[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            UL[v] = uL[v], UR[v] = uR[v];
+        if (uRec)
+        {
+            real xs[3], phi[M > 0 ? M : 1];
+            xs[0] = (x[0] - cenL[0]) / sL, xs[1] = (x[1] - cenL[1]) / sL;
+            xs[2] = DIM == 3 ? (x[2] - cenL[2]) / sL : (x[2] - cenL[2]);
+            basisRow0<DIM, NDOF>(xs, momL, phi);
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+            {
+                real s = 0;
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    s += phi[j] * uRec[((size_t)v * M + j) * N + cL];
+                UL[v] += s;
+            }
+            xs[0] = (x[0] + sh[0] - cenR[0]) / sR, xs[1] = (x[1] + sh[1] - cenR[1]) / sR;
+            xs[2] = DIM == 3 ? (x[2] + sh[2] - cenR[2]) / sR : (x[2] + sh[2] - cenR[2]);
+            basisRow0<DIM, NDOF>(xs, momR, phi);
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+            {
+                real s = 0;
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    s += phi[j] * uRec[((size_t)v * M + j) * N + cR];
+                UR[v] += s;
+            }
+        }
+        real Fp[5];
+        if constexpr (BATCH)
+            roeBatchPoint<RS>(UL, UR, n, gp, bf, Fp);
+        else
+        {
+            real l0, l1, l4;
+            riemannFlux<RS>(UL, UR, uL, uR, n, gp, dLambda, Fp, l0, l1, l4);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            acc[v] += (-Fp[v]) * wj;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        fflux[v * F + f] = acc[v];
+}
+
+__global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restrict__ fflux, real *__restrict__ rhs)
+{
+    idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N)
+        return;
+    const size_t N = m.N, F = m.F;
+    real vol = m.vol[i];
+    real acc[5] = {0, 0, 0, 0, 0};
+    for (int k = 0; k < m.K; k++)
+    {
+        idx f = m.c2f[(size_t)k * N + i];
+        if (f < 0)
+            continue;
+        bool back = m.f2c[f] == i;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            real fv = fflux[v * F + f];
+            acc[v] += (back ? fv : -fv) / vol;
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        rhs[v * N + i] = acc[v];
+}
+
+template <int DIM, int NDOF>
+static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
+{
+    int g = gridFor(c.m.F, 128);
+    hipStream_t s = c.stream;
+    bool batch = gp.rsMeanValueEig != 0 && gp.rsType >= RS_Roe_M1 && gp.rsType <= RS_Roe_M5;
+#define DNDSR_RHS_CASE(RSV)                                                                  \
+    case RSV:                                                                               \
+        kRHSFace<DIM, NDOF, RSV, false><<<g, 128, 0, s>>>(c.m, gp, u, uRec, dlc, ff);        \
+        break;
+#define DNDSR_RHS_BCASE(RSV)                                                                 \
+    case RSV:                                                                               \
+        kRHSFace<DIM, NDOF, RSV, true><<<g, 128, 0, s>>>(c.m, gp, u, uRec, dlc, ff);         \
+        break;
+    if (batch)
+    {
+        switch (gp.rsType)
+        {
+            DNDSR_RHS_BCASE(RS_Roe_M1)
+            DNDSR_RHS_BCASE(RS_Roe_M2)
+            DNDSR_RHS_BCASE(RS_Roe_M3)
+            DNDSR_RHS_BCASE(RS_Roe_M4)
+            DNDSR_RHS_BCASE(RS_Roe_M5)
+        default: throw std::runtime_error("invalid batched Riemann solver");
+        }
+    }
+    else
+    {
+        switch (gp.rsType)
+        {
+            DNDSR_RHS_CASE(RS_Roe)
+            DNDSR_RHS_CASE(RS_HLLC)
+            DNDSR_RHS_CASE(RS_HLLEP)
+            DNDSR_RHS_CASE(RS_HLLEP_V1)
+            DNDSR_RHS_CASE(RS_Roe_M1)
+            DNDSR_RHS_CASE(RS_Roe_M2)
+            DNDSR_RHS_CASE(RS_Roe_M3)
+            DNDSR_RHS_CASE(RS_Roe_M4)
+            DNDSR_RHS_CASE(RS_Roe_M5)
+            DNDSR_RHS_CASE(RS_Roe_M6)
+            DNDSR_RHS_CASE(RS_Roe_M7)
+            DNDSR_RHS_CASE(RS_Roe_M8)
+        default: throw std::runtime_error("the rs type is invalid");
+        }
+    }
+#undef DNDSR_RHS_CASE
+#undef DNDSR_RHS_BCASE
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs)
+{
+    DevMesh &m = c.m;
+    real *ff = c.faceFlux;
+    if (m.dim == 2)
+    {
+        if (!uRec || m.P == 1)
+            rhsDispatch<2, 3>(c, gp, u, uRec, dlc, ff);
+        else if (m.P == 2)
+            rhsDispatch<2, 6>(c, gp, u, uRec, dlc, ff);
+        else
+            rhsDispatch<2, 10>(c, gp, u, uRec, dlc, ff);
+    }
+    else
+    {
+        if (!uRec || m.P == 1)
+            rhsDispatch<3, 4>(c, gp, u, uRec, dlc, ff);
+        else if (m.P == 2)
+            rhsDispatch<3, 10>(c, gp, u, uRec, dlc, ff);
+        else
+            rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
+    }
+    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// LUSGSMatrixInit, scalar Jacobian (EulerEvaluator.hxx:37-159, ignoreSourceTerm):
+// D_i = 1/dTau_i + 1/dt + sum_f 0.5 alpha A_f lambda_f / V_i
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kLUSGSInit(DevMesh m, const real *dTau, real dt, real alphaDiag,
+                                                  const real *lam, real *jd)
+{
+    idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N)
+        return;
+    real fp = 1.0 / dTau[i] + 1.0 / dt;
+    for (int k = 0; k < m.K; k++)
+    {
+        idx f = m.c2f[(size_t)k * m.N + i];
+        if (f >= 0)
+            fp += (0.5 * alphaDiag) * m.farea[f] * lam[f] / m.vol[i];
+    }
+    jd[i] = fp;
+}
+void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag)
+{
+    kLUSGSInit<<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m, dTau, dt, alphaDiag, lambdaFace, jdiag);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+// ---------------------------------------------------------------------------
+// UpdateSGS (EulerEvaluator.hxx:550-693), scalar Jacobian, multicolour form.
+// Cells of one colour share no face, so one launch per colour updates them all
+// in parallel; a neighbour counts as already visited when its colour precedes
+// (forward) or follows (backward) the current one -- the serial reference sweep
+// over cells renumbered colour by colour.  Per-block |dx| partial sums are
+// written to distinct slots and reduced once at the end (deterministic).
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ rhs,
+                                                  const T *__restrict__ u, const T *__restrict__ uInc,
+                                                  T *__restrict__ uIncNew, const T *__restrict__ jd,
+                                                  const T *__restrict__ lam, int col, int forward, int gsUpdate,
+                                                  int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+{
+    __shared__ real sm[5][256];
+    idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    real loc[5] = {0, 0, 0, 0, 0};
+    if (t < end)
+    {
+        const size_t N = m.N, F = m.F;
+        idx i = m.colorCells[t];
+        const T gamma = (T)gammaD;
+        T buf[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            buf[v] = rhs[v * N + i];
+        T voli = (T)m.vol[i];
+        for (int k = 0; k < m.K; k++)
+        {
+            idx f = m.c2f[(size_t)k * N + i];
+            if (f < 0)
+                continue;
+            idx L = m.f2c[f], R = m.f2c[F + f];
+            bool back = L == i;
+            idx o = back ? R : L;
+            if (o < 0 || o == i)
+                continue;
+            int co = m.color[o];
+            bool useNew = gsUpdate && (forward ? co < col : co > col);
+            if (uIncIsZero && !useNew)
+                continue;
+            const T *du = useNew ? uIncNew : uInc;
+            T U[5], dU[5], n[3], dF[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                U[v] = u[v * N + o], dU[v] = du[v * N + o];
+            T sgn = back ? T(1) : T(-1);
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                n[r] = (T)m.fmeanN[r * F + f] * sgn;
+            fluxJacDu<T>(U, n, dU, lam[f], gamma, dF);
+            T coef = (T(0.5) * alphaDiag) * (T)m.farea[f] / voli;
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                buf[v] -= coef * dF[v];
+        }
+        T d = jd[i];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            T old = uIncNew[v * N + i];
+            T nw = buf[v] / d;
+            uIncNew[v * N + i] = nw;
+            loc[v] = (real)fabs(nw - old);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = loc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
+}
+
+__global__ void __launch_bounds__(256) kReduceSum5(const real *part, int nPart, real *out)
+{
+    __shared__ real sm[5][256];
+    real a[5] = {0, 0, 0, 0, 0};
+    for (int q = threadIdx.x; q < nPart; q += 256)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            a[v] += part[(size_t)q * 5 + v];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = a[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            out[v] = sm[v][0];
+}
+
+template <typename T>
+static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const T *u, const T *uInc, T *uIncNew,
+                     const T *jd, const T *lam, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev)
+{
+    DevMesh &m = c.m;
+    int nc = m.nColor;
+    size_t off = 0;
+    for (int s = 0; s < nc; s++)
+    {
+        int col = forward ? s : nc - 1 - s;
+        idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
+        if (e <= b)
+            continue;
+        int nb = gridFor(e - b, 256);
+        kSGSColour<T><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jd, lam, col, forward,
+                                               gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+        DNDSR_HIP_CHECK(hipGetLastError());
+        off += nb;
+    }
+    kReduceSum5<<<1, 256, 0, c.stream>>>(c.red, (int)off, sumIncDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
+                      real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
+                      int uIncIsZero, real *sumIncDev)
+{
+    sgsSweep<real>(c, gp, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
+                   sumIncDev);
+}
+void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
+                      const float *uInc, float *uIncNew, const float *jdiag, const float *lambdaFace, int forward,
+                      int gsUpdate, int uIncIsZero, real *sumIncDev)
+{
+    sgsSweep<float>(c, gp, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
+                    sumIncDev);
+}
+
+// out = a x + b y + c (z * dt); dt read from device memory (1 when null) so that a
+// time loop never waits on the host for dtMin.
+__global__ void __launch_bounds__(256) kAxpby(idx n, real a, const real *__restrict__ x, real b,
+                                              const real *__restrict__ y, real cc, const real *__restrict__ z,
+                                              const real *__restrict__ dtDev, real *__restrict__ out)
+{
+    idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n)
+        return;
+    real dt = dtDev ? dtDev[0] : 1.0;
+    real r = a * x[i];
+    if (y)
+        r += b * y[i];
+    if (z)
+        r += cc * (z[i] * dt);
+    out[i] = r;
+}
+void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
+                 const real *dtDev, real *out)
+{
+    kAxpby<<<gridFor(n, 256), 256, 0, c.stream>>>(n, a, x, b, y, cdt, z, dtDev, out);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+} // namespace dndsr

@@ -1,0 +1,135 @@
+// internal.hpp -- context, device mesh and launcher declarations shared by the
+// HIP translation units of libdndsr_amd.so (not part of the C-ABI).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "dev_common.hpp"
+
+#define DNDSR_HIP_CHECK(x)                                                                          \
+    do                                                                                              \
+    {                                                                                               \
+        hipError_t e_ = (x);                                                                        \
+        if (e_ != hipSuccess)                                                                       \
+            throw std::runtime_error(std::string(#x) + " failed: " + hipGetErrorString(e_));        \
+    } while (0)
+
+namespace dndsr
+{
+enum ElemType : int
+{
+    E_LINE2 = 1,
+    E_QUAD4 = 2,
+    E_HEX8 = 3,
+    E_TRI3 = 4,
+    E_TET4 = 5,
+};
+
+struct VRSettingsDev
+{
+    int maxOrder;
+    int intOrder;
+    int dirWeightScheme;  // 0 Factorial, 1 HQM_OPT
+    int geomWeightScheme; // 0 GWNone, 1 HQM_SD
+    real geomWeightBias;
+    real geomWeightPower;
+    real scaleMultiplier;
+    real jacobiRelax;
+    int nIterCellSmoothScale;
+};
+
+// Device-resident mesh, metrics and VR coefficients (all arrays SoA, see dev_common.hpp).
+struct DevMesh
+{
+    int dim = 0, P = 0, NDOF = 0, M = 0;
+    idx N = 0, F = 0, nNode = 0;
+    int K = 0;      // max faces per cell
+    int G = 0;      // face quadrature points
+    int Gc = 0;     // cell quadrature points
+    int cellType = 0, faceType = 0;
+    VRSettingsDev s{};
+    // connectivity
+    idx *f2c = nullptr;     // [2][F], -1 = boundary
+    idx *c2f = nullptr;     // [K][N], -1 = absent
+    idx *cell2node = nullptr; // [nnc][N]
+    idx *face2node = nullptr; // [nnf][F]
+    real *coords = nullptr;   // [3][nNode]
+    // geometry
+    real *fshift = nullptr;   // [3][F]
+    real *fpts = nullptr;     // [G][3][F]
+    real *fnorm = nullptr;    // [G][3][F]
+    real *fwj = nullptr;      // [G][F]   w * (JDet + tiny)
+    real *farea = nullptr;    // [F]
+    real *fmeanN = nullptr;   // [3][F]
+    real *fcent = nullptr;    // [3][F]
+    real *faceL = nullptr;    // [F]
+    real *faceW = nullptr;    // [P+1][F]
+    real *vol = nullptr, *cent = nullptr, *bary = nullptr, *scale = nullptr, *smooth = nullptr;
+    real *moment = nullptr;   // [M][N]
+    real *cpts = nullptr;     // [Gc][3][N]
+    real *cwj = nullptr;      // [Gc][N]
+    // VR
+    real *AinvB = nullptr;    // [(K+1)*M*M][N]
+    real *Ainvb = nullptr;    // [K*M][N]
+    // multicolour ordering of cells for the Gauss-Seidel sweeps
+    int nColor = 0;
+    std::vector<idx> colorPtrHost;
+    idx *colorCells = nullptr; // [N] cells grouped by colour
+    int32_t *color = nullptr;  // [N]
+    real globalVol = 0;
+};
+
+struct Ctx
+{
+    int device = 0;
+    hipStream_t stream = nullptr;
+    DevMesh m;
+    bool haveMesh = false;
+    // scratch owned by the context (grown on demand, freed at destroy)
+    real *faceFlux = nullptr;   // [5][F]
+    size_t faceFluxCap = 0;
+    real *red = nullptr;        // reduction partials
+    size_t redCap = 0;
+    real *hostRed = nullptr;    // pinned host mirror of reductions
+    real *faceLamTmp = nullptr; // [2][F] (lambdaFace, deltaLambdaFace)
+    size_t faceLamCap = 0;
+    std::vector<void *> owned;  // device allocations freed at destroy
+};
+
+// ---- launchers (setup.hip)
+void setupMetricsAndVR(Ctx &c);
+// ---- launchers (vr.hip)
+void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax);
+void launchVRIterDiff(Ctx &c, int nVars, const real *uRecDiff, real *uRecNew);
+// ---- launchers (euler.hip)
+void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,
+                  real *lambdaFace, real *deltaLambdaCell, real *dtMinDev);
+void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *deltaLambdaCell,
+                   real *rhs);
+void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag);
+void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
+                      real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
+                      int uIncIsZero, real *sumIncDev);
+void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
+                      const float *uInc, float *uIncNew, const float *jdiag, const float *lambdaFace, int forward,
+                      int gsUpdate, int uIncIsZero, real *sumIncDev);
+void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
+                 const real *dtDev, real *out);
+
+template <typename T>
+T *devAlloc(Ctx &c, size_t n)
+{
+    void *p = nullptr;
+    if (n == 0)
+        n = 1;
+    DNDSR_HIP_CHECK(hipMalloc(&p, n * sizeof(T)));
+    c.owned.push_back(p);
+    return static_cast<T *>(p);
+}
+
+inline int gridFor(idx n, int block) { return (int)((n + block - 1) / block); }
+
+} // namespace dndsr

@@ -1,0 +1,103 @@
+"""EulerEvaluator on the device, with the reference's method names.
+
+Mirrors ``DNDS::Euler::EulerEvaluator<model>`` for the inviscid NS / NS_3D models
+(src/Euler/EulerEvaluator.hpp:76): EvaluateDt (:359), EvaluateRHS (:399),
+LUSGSMatrixInit (:427), UpdateLUSGSForward/Backward (:481-502), UpdateSGS (:513),
+EvaluateNorm (:613).  Like the reference, the evaluator keeps lambdaFace* and
+deltaLambdaCell from the last EvaluateDt as members.
+"""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+import torch
+
+from . import _lib
+from ._lib import check, lib
+from .cfv import RIEMANN_TYPES, VariationalReconstruction, _ptr
+
+
+class EulerEvaluator:
+    nVars = 5
+
+    def __init__(self, vfv: VariationalReconstruction, settings: dict | None = None):
+        s = dict(gamma=1.4, riemannSolverType="Roe", rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0)
+        if settings:
+            s.update(settings)
+            if "idealGasProperty" in settings:
+                s["gamma"] = settings["idealGasProperty"].get("gamma", s["gamma"])
+        rs = s["riemannSolverType"]
+        self.settings = s
+        self.es = _lib.EulerSettings(float(s["gamma"]), RIEMANN_TYPES[rs] if isinstance(rs, str) else int(rs),
+                                     int(s["rsMeanValueEig"]), float(s["rsFixScale"]), float(s["rsIncFScale"]))
+        vfv.construct()
+        self.vfv = vfv
+        self.ctx = vfv.ctx
+        dev = self.ctx.torch_device
+        self.lambdaFace = torch.zeros((4, vfv.nFace), dtype=torch.float64, device=dev)
+        self.deltaLambdaCell = torch.zeros(vfv.nCell, dtype=torch.float64, device=dev)
+        self.dtMinDev = torch.zeros(1, dtype=torch.float64, device=dev)
+
+    # EvaluateDt(dt, u, uRec, CFL, dtMinall&, MaxDt, UseLocaldt, t)
+    def EvaluateDt(self, dt, u, uRec=None, CFL=1.0, MaxDt=1e100, UseLocaldt=True, t=0.0, sync_min=True):
+        dmin = C.c_double()
+        check(lib().dndsr_eval_dt(self.ctx.h, C.byref(self.es), _ptr(u), float(CFL), float(MaxDt), int(UseLocaldt),
+                                  _ptr(dt), _ptr(self.lambdaFace), _ptr(self.deltaLambdaCell), _ptr(self.dtMinDev),
+                                  C.byref(dmin) if sync_min else None))
+        return dmin.value if sync_min else None
+
+    # EvaluateRHS(rhs, JSource, u, uRecUnlim, uRec, uRecBeta, cellRHSAlpha, onlyOnHalfAlpha, t, flags)
+    def EvaluateRHS(self, rhs, JSource, u, uRecUnlim, uRec=None, uRecBeta=None, cellRHSAlpha=None,
+                    onlyOnHalfAlpha=False, t=0.0, flags=0):
+        rec = uRec if uRec is not None else uRecUnlim
+        check(lib().dndsr_eval_rhs(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(rec), _ptr(self.deltaLambdaCell),
+                                   _ptr(rhs)))
+
+    # LUSGSMatrixInit(JDiag, JSource, dTau, dt, alphaDiag, u, uRec, jacobianCode, t)
+    def LUSGSMatrixInit(self, JDiag, JSource, dTau, dt, alphaDiag, u=None, uRec=None, jacobianCode=0, t=0.0):
+        if jacobianCode != 0:
+            raise ValueError("jacobianCode must be 0")
+        check(lib().dndsr_lusgs_init(self.ctx.h, _ptr(dTau), float(dt), float(alphaDiag), _ptr(self.lambdaFace),
+                                     _ptr(JDiag)))
+
+    # UpdateSGS(alphaDiag, t, rhs, u, uInc, uIncNew, JDiag, forward, gsUpdate, sumInc&, uIncIsZero)
+    def UpdateSGS(self, alphaDiag, t, rhs, u, uInc, uIncNew, JDiag, forward, gsUpdate, uIncIsZero=False,
+                  lambdaFace=None):
+        s = np.zeros(5)
+        lam = self.lambdaFace if lambdaFace is None else lambdaFace
+        if rhs.dtype == torch.float32:
+            check(lib().dndsr_update_sgs_f32(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u),
+                                             _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(lam), int(forward),
+                                             int(gsUpdate), int(uIncIsZero), s.ctypes.data_as(_lib.dp)))
+        else:
+            check(lib().dndsr_update_sgs(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u),
+                                         _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(lam), int(forward),
+                                         int(gsUpdate), int(uIncIsZero), s.ctypes.data_as(_lib.dp)))
+        return s
+
+    # deprecated LU-SGS pair (EulerEvaluator.hxx:376-529): Forward writes D^-1 (r - L x_new),
+    # Backward adds D^-1 (-U x); with separate uInc they reduce to block Jacobi.
+    def UpdateLUSGSForward(self, alphaDiag, t, rhs, u, uInc, JDiag, uIncNew):
+        return self.UpdateSGS(alphaDiag, t, rhs, u, uInc, uIncNew, JDiag, True, True, True)
+
+    def UpdateLUSGSBackward(self, alphaDiag, t, rhs, u, uInc, JDiag, uIncNew):
+        tmp = uIncNew.clone()
+        return self.UpdateSGS(alphaDiag, t, rhs, u, tmp, uIncNew, JDiag, False, True, False)
+
+    # EvaluateNorm(res, rhs, P, volWise, average), EulerEvaluator.hxx:1394-1440
+    def EvaluateNorm(self, rhs, P=1, volWise=False, average=False):
+        vol = torch.as_tensor(self.vfv.cell_metrics()["vol"], device=rhs.device)
+        a = rhs.abs().double()
+        if P < 3:
+            w = vol if volWise else torch.ones_like(vol)
+            res = (a.pow(P) * w).sum(dim=1).pow(1.0 / P)
+            if average:
+                res = res / (w.sum() + 1e-200)
+            return res.cpu().numpy()
+        return a.max(dim=1).values.cpu().numpy()
+
+    def SSPRK3(self, u, uRec, nStep, CFL, nRec=1, dtFixed=0.0):
+        """Device-resident explicit SSP-RK3 loop (ODE.hpp:1437-1535)."""
+        check(lib().dndsr_ssprk3(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(uRec), int(nStep), float(CFL), int(nRec),
+                                 float(dtFixed)))

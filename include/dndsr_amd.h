@@ -1,0 +1,164 @@
+/*
+ * dndsr_amd.h -- C-ABI of the MI355X (gfx950) compact-finite-volume hot path.
+ *
+ * The reference (CFDLAB-THU/DNDSR) exposes this path as C++ templates:
+ *   VariationalReconstruction<dim>   src/CFV/VariationalReconstruction.hpp:108
+ *   EulerEvaluator<model>            src/Euler/EulerEvaluator.hpp:76
+ * Each entry point below replaces one of their member functions; the cited
+ * line is the declaration it stands in for.  All entry points:
+ *   - return 0 on success, nonzero on error (message via dndsr_last_error());
+ *     no C++ exception crosses the boundary;
+ *   - take DEVICE pointers for every field array (caller-owned, never
+ *     reallocated) and enqueue their kernels on the context's stream; results
+ *     are ready after dndsr_sync() unless a host output is requested;
+ *   - are not thread-safe per context.
+ *
+ * Device field layout (structure of arrays, cell index fastest):
+ *   u, rhs, uInc      [5][nCell]        (rho, rho u, rho v, rho w, E)
+ *   uRec              [nVars][M][nCell] (M = NDOF(maxOrder) - 1 modes)
+ *   dt, deltaLambdaCell, jdiag          [nCell]
+ *   lambdaFace        [4][nFace]        (lambdaFace, lambdaFaceC, lambdaFaceVis, lambdaFace0/123/4)
+ * dndsr_h2d_cellfield / dndsr_d2h_cellfield convert from / to the reference's
+ * row layout (ArrayDof rows of column-major m x n blocks,
+ * src/DNDS/ArrayDerived/ArrayEigenMatrix.hpp:23,64): a host row of nComp values
+ * per cell, with uRec's (k, var) entry at component var*M + k.
+ */
+#ifndef DNDSR_AMD_H
+#define DNDSR_AMD_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct dndsr_ctx dndsr_ctx;
+
+/* element type codes of dndsr_mesh_desc */
+enum { DNDSR_LINE2 = 1, DNDSR_QUAD4 = 2, DNDSR_HEX8 = 3, DNDSR_TRI3 = 4, DNDSR_TET4 = 5 };
+
+/* Mesh as handed over by the caller (host memory).  Periodic faces are stored once,
+ * in the frame of face2cell[f][0]; faceShift[f] carries a face point into the frame of
+ * face2cell[f][1] (UnstructuredMesh::periodicInfo, src/Geom/Mesh/Mesh.hpp:37). */
+typedef struct
+{
+    int dim;
+    int64_t nNode;
+    const double *coords;    /* [nNode][3] */
+    int64_t nCell;
+    const int32_t *cellType; /* [nCell] */
+    const int64_t *cell2nodePtr, *cell2node;
+    const int64_t *cell2facePtr, *cell2face;
+    int64_t nFace;
+    const int32_t *faceType; /* [nFace] */
+    const int64_t *face2nodePtr, *face2node;
+    const int64_t *face2cell; /* [nFace][2], -1 = boundary */
+    const double *faceShift;  /* [nFace][3] */
+} dndsr_mesh_desc;
+
+/* VRSettings subset used on this path (src/CFV/VRSettings.hpp:37-212) */
+typedef struct
+{
+    int maxOrder;
+    int intOrder;
+    int dirWeightScheme;  /* 0 Factorial, 1 HQM_OPT */
+    int geomWeightScheme; /* 0 GWNone, 1 HQM_SD */
+    double geomWeightBias;
+    double geomWeightPower;
+    double scaleMultiplier;
+    double jacobiRelax;
+    int nIterCellSmoothScale;
+} dndsr_vr_settings;
+
+/* EulerEvaluatorSettings subset (src/Euler/EulerEvaluatorSettings.hpp:53-80) */
+typedef struct
+{
+    double gamma;
+    int rsType;         /* Gas::RiemannSolverType: Roe=1, HLLC=2, HLLEP=3, Roe_M1..M8=11..18, HLLEP_V1=21 */
+    int rsMeanValueEig; /* cell-mean eigenvalues shared by a face's points (Roe_M1..M5) */
+    double rsFixScale;
+    double rsIncFScale;
+} dndsr_euler_settings;
+
+/* ---- context ---------------------------------------------------------- */
+int dndsr_create(dndsr_ctx **ctx, int device);
+int dndsr_destroy(dndsr_ctx *ctx);
+const char *dndsr_last_error(void);
+int dndsr_sync(dndsr_ctx *ctx);
+/* enqueue on the caller's stream (e.g. the framework's current stream) instead of the
+ * context's own; pass NULL to return to the context stream */
+int dndsr_set_stream(dndsr_ctx *ctx, void *hipStream);
+
+/* ---- setup ------------------------------------------------------------ */
+/* ConstructMetrics + ConstructBaseAndWeight + ConstructRecCoeff, computed on the device.
+ * Replaces VariationalReconstruction<dim>::ConstructMetrics / ConstructBaseAndWeight /
+ * ConstructRecCoeff, src/CFV/VariationalReconstruction.hpp:282-289. */
+int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *mesh, const dndsr_vr_settings *s);
+int dndsr_mesh_info(dndsr_ctx *ctx, int64_t *nCell, int64_t *nFace, int *M, int *maxFaceCell, int *nGFace,
+                    int *nGCell, int *nColor, double *globalVol);
+/* host copies of metrics in row layout:
+ *   vol[N], bary[N][3], cent[N][3], scale[N], smoothRatio[N], moment[N][M]        */
+int dndsr_get_cell_metrics(dndsr_ctx *ctx, double *vol, double *bary, double *cent, double *scale,
+                           double *smoothRatio, double *moment);
+/*   area[F], meanNorm[F][3], pts[F][G][3], norms[F][G][3], wJDet[F][G], faceL[F], faceWeight[F][P+1] */
+int dndsr_get_face_metrics(dndsr_ctx *ctx, double *area, double *meanNorm, double *pts, double *norms,
+                           double *wJDet, double *faceL, double *faceWeight);
+/*   pts[N][Gc][3], wJDet[N][Gc] (cell quadrature) */
+int dndsr_get_cell_quad(dndsr_ctx *ctx, double *pts, double *wJDet);
+/* matrixAAInvB / vectorAInvB in the reference's block order
+ * (src/CFV/VariationalReconstruction.hpp:169-197): AinvB[N][K+1][M][M] (block 0 = A^-1),
+ * Ainvb[N][K][M].  get = download; set = upload coefficients computed elsewhere
+ * (e.g. by the reference's own ConstructRecCoeff). */
+int dndsr_get_vr_coeffs(dndsr_ctx *ctx, double *AinvB, double *Ainvb);
+int dndsr_set_vr_coeffs(dndsr_ctx *ctx, const double *AinvB, const double *Ainvb);
+/* multicolour ordering used by the Gauss-Seidel sweeps: color[N] */
+int dndsr_get_colors(dndsr_ctx *ctx, int32_t *color);
+
+/* ---- layout helpers ---------------------------------------------------- */
+int dndsr_h2d_cellfield(dndsr_ctx *ctx, const double *hostRows, double *dev, int nComp);
+int dndsr_d2h_cellfield(dndsr_ctx *ctx, const double *dev, double *hostRows, int nComp);
+
+/* ---- operators ---------------------------------------------------------- */
+/* DoReconstructionIter, Jacobi form (SORInstead = false):
+ *   mode 0: uRecNew = (1-relax) uRec + relax sum_f (A^-1 B_f uRec_j + A^-1 b_f (u_j - u_i))
+ *   mode 1: recordInc (uRecNew = uRec - sum_f ...), mode 2: recordInc with uRecIsZero.
+ * relax < 0 uses settings.jacobiRelax.
+ * Replaces VariationalReconstruction<dim>::DoReconstructionIter,
+ * src/CFV/VariationalReconstruction.hpp:984-992. */
+int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew, const double *u, int mode,
+                  double relax);
+/* DoReconstructionIterDiff: uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j,
+ * src/CFV/VariationalReconstruction.hpp:1012-1018. */
+int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRecDiff, double *uRecNew);
+
+/* EvaluateDt, src/Euler/EulerEvaluator.hpp:359-366.  dtMinHost may be NULL (then no
+ * host synchronisation happens; the minimum stays in dtMinDev if non-NULL). */
+int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, double CFL, double maxDt,
+                  int useLocalDt, double *dt, double *lambdaFace, double *deltaLambdaCell, double *dtMinDev,
+                  double *dtMinHost);
+/* EvaluateRHS, src/Euler/EulerEvaluator.hpp:399-409 (inviscid; uRec may be NULL for the
+ * first-order residual; deltaLambdaCell feeds HLLE+ and may be NULL = 0). */
+int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                   const double *deltaLambdaCell, double *rhs);
+/* LUSGSMatrixInit (scalar Jacobian), src/Euler/EulerEvaluator.hpp:427-434. */
+int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alphaDiag, const double *lambdaFace,
+                     double *jdiag);
+/* UpdateSGS, src/Euler/EulerEvaluator.hpp:513-522, in multicolour order.  sumInc (5
+ * values) is returned on the host when sumIncHost is non-NULL (synchronises). */
+int dndsr_update_sgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
+                     const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                     const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
+/* FP32 sweep for the mixed-precision path: every array is float (same layouts). */
+int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
+                         const float *u, const float *uInc, float *uIncNew, const float *jdiag,
+                         const float *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
+/* Explicit SSP-RK3 time loop kept on the device (ExplicitSSPRK3TimeStepAsImplicitDualTimeStep::Step,
+ * src/Solver/ODE.hpp:1477-1520): per step EvaluateDt once (global dt = CFL-limited minimum, or
+ * dtFixed > 0), then three stages of nRec Jacobi VR iterations + EvaluateRHS + combination.
+ * u and uRec are updated in place. */
+int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, double *uRec, int nStep, double CFL,
+                 int nRec, double dtFixed);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

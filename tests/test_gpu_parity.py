@@ -1,0 +1,238 @@
+"""HIP path vs CPU oracle on identical meshes, settings and seeded inputs (FP64).
+
+Tolerances: relative L2 difference <= 1e-12 for every FP64 operator output given
+identical inputs (north_star bound for the FP64 path).  The VR coefficients come
+out of an SPD inverse, so their comparison is scaled by the matrix conditioning
+(<= 1e-10); the VR iteration itself is compared with the oracle's coefficients
+uploaded through dndsr_set_vr_coeffs so the kernel is checked at 1e-12.
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import RS, colour_scan_order, random_rec, random_state, rel_l2, small_mesh, vr_settings
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+CASES = [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 1), ("3d", 2)]
+
+
+def _pair(kind, order, hqm=False, jitter=0.15):
+    import pyoracle as po
+    from dndsr_amd.cfv import VariationalReconstruction
+    m = small_mesh(kind, jitter=jitter)
+    s = vr_settings(order, hqm=hqm)
+    o = po.OracleVR(m, **s)
+    g = VariationalReconstruction(m, s).construct()
+    return m, o, g
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[f"{k}-p{p}" for k, p in CASES])
+def pair(request, require_gpu):
+    kind, order = request.param
+    print(f"\n[parity] mesh={kind} p={order}", flush=True)
+    return _pair(kind, order)
+
+
+def test_cell_metrics(pair):
+    _, o, g = pair
+    a, b = o.cell_metrics(), g.cell_metrics()
+    for k in ("vol", "bary", "cent", "scale", "smooth", "moment"):
+        assert rel_l2(b[k], a[k]) <= TOL, k
+
+
+def test_face_metrics(pair):
+    _, o, g = pair
+    a, b = o.face_metrics(), g.face_metrics()
+    for k in ("area", "meanNorm", "pts", "norms", "wJDet", "faceL", "faceWeight"):
+        assert rel_l2(b[k], a[k]) <= TOL, k
+    pa, wa = o.cell_quad()
+    pb, wb = g.cell_quad()
+    assert rel_l2(pb, pa) <= TOL and rel_l2(wb, wa) <= TOL
+
+
+def test_vr_coefficients(pair):
+    _, o, g = pair
+    Ao, bo = o.vr_coeffs()
+    Ag, bg = g.get_matrixAAInvB()
+    assert rel_l2(Ag, Ao) <= 1e-10
+    assert rel_l2(bg, bo) <= 1e-10
+
+
+@pytest.mark.parametrize("nVars", [1, 5])
+def test_vr_iter_modes(pair, nVars):
+    import torch
+    m, o, g = pair
+    Ao, bo = o.vr_coeffs()
+    g.set_matrixAAInvB(Ao, bo)
+    N, M = o.nCell, o.M
+    u = random_state(N, seed=3)[:, :nVars].copy()
+    uRec = random_rec(N, M, nVars, seed=4)
+    du, dr = g.to_device(u), g.to_device(uRec)
+    out = torch.zeros_like(dr)
+    # Jacobi sweep
+    g.DoReconstructionIter(dr, out, du)
+    assert rel_l2(g.to_host(out), o.vr_iter(uRec, u)) <= TOL
+    # relaxed Jacobi against the closed-form blend of two oracle evaluations
+    g.DoReconstructionIter(dr, out, du, relax=0.7)
+    ref = 0.3 * uRec + 0.7 * o.vr_iter(uRec, u)
+    assert rel_l2(g.to_host(out), ref) <= 1e-12
+    # recordInc: uRec - sum(...) = 2 uRec - Jacobi(uRec)
+    g.DoReconstructionIter(dr, out, du, recordInc=True)
+    assert rel_l2(g.to_host(out), 2 * uRec - o.vr_iter(uRec, u)) <= 1e-11
+    # recordInc + uRecIsZero = -Jacobi(0)
+    g.DoReconstructionIter(dr, out, du, recordInc=True, uRecIsZero=True)
+    assert rel_l2(g.to_host(out), -o.vr_iter(np.zeros_like(uRec), u)) <= TOL
+    # Jacobian action
+    g.DoReconstructionIterDiff(dr, out)
+    assert rel_l2(g.to_host(out), o.vr_iter_diff(uRec)) <= TOL
+
+
+def _euler_pair(g, rsType, batch=0):
+    import pyoracle as po
+    from dndsr_amd.euler import EulerEvaluator
+    es = po.OracleVR.euler_settings(1.4, RS[rsType], batch, 1.0, 1.0)
+    ev = EulerEvaluator(g, dict(gamma=1.4, riemannSolverType=rsType, rsMeanValueEig=batch))
+    return es, ev
+
+
+def test_eval_dt(pair):
+    m, o, g = pair
+    es, ev = _euler_pair(g, "Roe")
+    u = random_state(o.nCell, seed=5)
+    du = g.to_device(u)
+    dt = g.BuildUDof(1)[0]
+    dtmin = ev.EvaluateDt(dt, du, None, CFL=0.7, MaxDt=1e100, UseLocaldt=True)
+    dto, lamo, dlco, dmino = o.eval_dt(es, u, 0.7, 1e100, True)
+    assert rel_l2(dt.cpu().numpy(), dto) <= TOL
+    assert rel_l2(ev.lambdaFace.cpu().numpy().T, lamo) <= TOL
+    assert rel_l2(ev.deltaLambdaCell.cpu().numpy(), dlco) <= TOL
+    assert abs(dtmin - dmino) <= TOL * dmino
+    ev.EvaluateDt(dt, du, None, CFL=0.7, UseLocaldt=False)
+    assert np.all(dt.cpu().numpy() == dmino)
+
+
+@pytest.mark.parametrize("rs,batch", [("Roe", 0), ("HLLEP", 0), ("HLLC", 0), ("Roe_M1", 1), ("Roe_M1", 0),
+                                      ("Roe_M2", 0), ("Roe_M3", 1), ("Roe_M4", 0), ("Roe_M5", 1), ("Roe_M6", 0),
+                                      ("Roe_M7", 0), ("Roe_M8", 0), ("HLLEP_V1", 0)])
+def test_eval_rhs(pair, rs, batch):
+    m, o, g = pair
+    es, ev = _euler_pair(g, rs, batch)
+    N, M = o.nCell, o.M
+    u = random_state(N, seed=11)
+    uRec = random_rec(N, M, 5, seed=12)
+    du, dr = g.to_device(u), g.to_device(uRec)
+    dt = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dt, du, None, CFL=1.0)
+    _, _, dlco, _ = o.eval_dt(es, u, 1.0, 1e100, True)
+    rhs = g.BuildUDof(5)
+    ev.EvaluateRHS(rhs, None, du, dr, dr)
+    ref = o.eval_rhs(es, u, uRec, dlco)
+    assert rel_l2(g.to_host(rhs), ref) <= TOL
+    # first-order residual (uRec = 0)
+    ev.EvaluateRHS(rhs, None, du, None, None)
+    assert rel_l2(g.to_host(rhs), o.eval_rhs(es, u, None, dlco)) <= TOL
+
+
+def test_lusgs(pair):
+    import torch
+    m, o, g = pair
+    es, ev = _euler_pair(g, "Roe")
+    N = o.nCell
+    u = random_state(N, seed=21)
+    du = g.to_device(u)
+    dt = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dt, du, None, CFL=5.0)
+    dto, lamo, _, _ = o.eval_dt(es, u, 5.0, 1e100, True)
+    jd = g.BuildUDof(1)[0]
+    ev.LUSGSMatrixInit(jd, None, dt, 0.3, 1.0, du, None)
+    jdo = o.lusgs_init(dto, 0.3, 1.0, lamo)
+    assert rel_l2(jd.cpu().numpy(), jdo) <= TOL
+    rhs = np.random.default_rng(22).uniform(-1, 1, (N, 5))
+    drhs = g.to_device(rhs)
+    order = colour_scan_order(g.colors())
+    x0 = np.random.default_rng(23).uniform(-0.1, 0.1, (N, 5))
+    # symmetric GS step: forward from zero (LU-SGS) then backward
+    dInc = g.BuildUDof(5)
+    dNew = g.BuildUDof(5)
+    s1 = ev.UpdateSGS(1.0, 0.0, drhs, du, dInc, dNew, jd, True, True, True)
+    r1, so1 = o.update_sgs(es, 1.0, rhs, u, np.zeros_like(u), np.zeros_like(u), jdo, lamo, True, True, True, order)
+    assert rel_l2(g.to_host(dNew), r1) <= TOL
+    assert rel_l2(s1, so1) <= TOL
+    dInc.copy_(dNew)
+    s2 = ev.UpdateSGS(1.0, 0.0, drhs, du, dInc, dNew, jd, False, True, False)
+    r2, so2 = o.update_sgs(es, 1.0, rhs, u, r1, r1, jdo, lamo, False, True, False, order)
+    assert rel_l2(g.to_host(dNew), r2) <= TOL
+    assert rel_l2(s2, so2) <= TOL
+    # Jacobi sweep from a nonzero increment
+    dInc2 = g.to_device(x0)
+    dNew2 = g.to_device(x0)
+    ev.UpdateSGS(1.0, 0.0, drhs, du, dInc2, dNew2, jd, True, False, False)
+    rj, _ = o.update_sgs(es, 1.0, rhs, u, x0, x0, jdo, lamo, True, False, False, order)
+    assert rel_l2(g.to_host(dNew2), rj) <= TOL
+    # FP32 sweep of the mixed path against the FP64 sweep
+    f32 = lambda t: t.to(torch.float32).contiguous()  # noqa: E731
+    z32 = torch.zeros((5, N), dtype=torch.float32, device=du.device)
+    n32 = torch.zeros_like(z32)
+    lam32 = f32(ev.lambdaFace)
+    ev.UpdateSGS(1.0, 0.0, f32(drhs), f32(du), z32, n32, f32(jd), True, True, True, lambdaFace=lam32)
+    assert rel_l2(n32.double().cpu().numpy().T, r1) <= 1e-5
+
+
+def test_colouring_is_proper(pair):
+    m, o, g = pair
+    col = g.colors()
+    L, R = m.face2cell[:, 0], m.face2cell[:, 1]
+    assert np.all(col[L] != col[R])
+
+
+@pytest.mark.parametrize("rs", ["Roe", "HLLEP"])
+def test_ssprk3_steps(require_gpu, rs):
+    import pyoracle as po
+    from dndsr_amd import fields
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    m = mm.periodic_quad_grid(12, 12, 10.0, 10.0)
+    s = vr_settings(1, intOrder=3)
+    o = po.OracleVR(m, **s)
+    g = VariationalReconstruction(m, s).construct()
+    ub = fields.iv_background_for_mach(0.5)
+    u0 = g.cell_average(lambda p: fields.isentropic_vortex(p, 5.0, 1.4, ub), 5)
+    es = po.OracleVR.euler_settings(1.4, RS[rs], 0, 1.0, 1.0)
+    ev = EulerEvaluator(g, dict(riemannSolverType=rs))
+    du, dr = g.to_device(u0), g.BuildURec(5)
+    ev.SSPRK3(du, dr, 4, 0.5, 1)
+    uo, ro = o.ssprk3(es, u0, np.zeros((o.nCell, 5, o.M)), 4, 0.5, 1)
+    assert rel_l2(g.to_host(du), uo) <= 1e-11
+    assert rel_l2(g.to_host(dr), ro) <= 1e-10
+
+
+def test_iv_pipeline_golden_on_gpu(require_gpu):
+    """Reference golden (test/cpp/Euler/test_EulerEvaluator.cpp:231-259) through the HIP path:
+    IV 10x10, p1, intOrder 3, Roe_M1 batched, EvaluateDt(CFL 100, local) -> first-order
+    EvaluateRHS -> LUSGSMatrixInit(dt 1e100) -> block-Jacobi increment."""
+    from dndsr_amd import fields
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    m = mm.periodic_quad_grid(10, 10, 10.0, 10.0)
+    g = VariationalReconstruction(m, dict(maxOrder=1, intOrder=3)).construct()
+    ev = EulerEvaluator(g, dict(riemannSolverType="Roe_M1", rsMeanValueEig=1))
+    u = g.cell_average(lambda p: fields.isentropic_vortex(p, 10.0828, 1.4, (1.0, 1.0)), 5)
+    du = g.to_device(u)
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, du, None, CFL=100.0, MaxDt=1e100, UseLocaldt=True)
+    rhs = g.BuildUDof(5)
+    ev.EvaluateRHS(rhs, None, du, None, None)
+    res = ev.EvaluateNorm(rhs, 1, True)
+    jd = g.BuildUDof(1)[0]
+    ev.LUSGSMatrixInit(jd, None, dTau, 1e100, 1.0, du, None)
+    x0, x1 = g.BuildUDof(5), g.BuildUDof(5)
+    ev.UpdateSGS(1.0, 0.0, rhs, du, x0, x1, jd, True, False, False)
+    inc = np.linalg.norm(ev.EvaluateNorm(x1, 2, False))
+    assert abs(res[0] - 1.0524823780e+01) <= 1e-6 * 1.0524823780e+01
+    assert abs(inc - 3.1557730986e+00) <= 1e-6 * 3.1557730986e+00
