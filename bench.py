@@ -1,0 +1,183 @@
+"""Benchmark of the compact-FV hot path on one MI355X.
+
+Workload (the reference-path configuration the metric is quoted on): 3-D Euler
+Taylor-Green vortex, Mach 0.1, periodic [0, 2 pi]^3 with 64^3 = 262144 uniform
+hexahedra, 3rd-order VR (p2, 9 modes per variable, intOrder 5), HLLE+ flux, FP64,
+explicit SSP-RK3 at CFL 0.5.  One step = EvaluateDt + 3 x (one Jacobi VR sweep +
+EvaluateRHS + stage combination), all on the device (dndsr_ssprk3).
+
+value = cell-updates per second per RHS evaluation = nCell * 3 * steps / time.
+The roofline entry is the dominant kernel: algorithmic bytes per launch divided
+by its average launch time, measured with HIP events on its stream during the
+timed region.  cpu_baseline times the CPU oracle (OpenMP on the host cores) on a
+bounded 32^3 sample of the same discretisation.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HBM_PEAK_GBS = 8000.0  # MI355X HBM3E datasheet peak
+METRIC = "cell-updates/s per RHS eval; time-to-1e-8 residual, mixed vs FP64 GPU"
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+def alg_bytes(dim, M, nV, n_fc, f_c, G_f):
+    """Algorithmic HBM bytes per cell and launch: each array counted once at its stored
+    precision, basis values recomputed (not tabulated), 4-byte indices.
+    VR Jacobi sweep: 8 [2 M nV + nV + n_fc (M^2 + M)] + 4 n_fc
+    EvaluateRHS:     8 [nV (2 + M) + 1 + 2 d + M] + f_c [8 + 8 G_f (2 d + 1)]"""
+    b_vr = 8 * (2 * M * nV + nV + n_fc * (M * M + M)) + 4 * n_fc
+    b_rhs = 8 * (nV * (2 + M) + 1 + 2 * dim + M) + f_c * (8 + 8 * G_f * (2 * dim + 1))
+    return b_vr, b_rhs
+
+
+def read_traffic(kernel_tag):
+    """HBM bytes per launch measured with rocprofv3 --pmc (FETCH_SIZE + WRITE_SIZE),
+    recorded in profiles/traffic_<kernel_tag>.txt as `bytes_per_launch: X`."""
+    p = os.path.join(HERE, "profiles", f"traffic_{kernel_tag}.txt")
+    if not os.path.exists(p):
+        return None
+    for line in open(p):
+        if line.startswith("bytes_per_launch:"):
+            return float(line.split(":")[1])
+    return None
+
+
+def cpu_baseline(n=32, evals_min=3, budget_s=12.0):
+    """Oracle frhs (Jacobi VR sweep + EvaluateRHS, OpenMP face buffer + cell gather)
+    on the TGV sample mesh n^3, p2, HLLE+."""
+    sys.path.insert(0, os.path.join(HERE, "oracle"))
+    import pyoracle as po
+    from dndsr_amd import mesh as mm
+    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    threads = min(threads, 16)
+    po.set_threads(threads, rhsGather=True)
+    m = mm.periodic_hex_grid(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi)
+    o = po.OracleVR(m, maxOrder=2, intOrder=5)
+    u = o.cell_average(11, [0.1, 1.4], 5)
+    es = o.euler_settings(1.4, 3, 0, 1.0, 1.0)
+    rec = np.zeros((o.nCell, 5, o.M))
+    _, _, dlc, _ = o.eval_dt(es, u, 0.5, 1e100, False)
+    rec = o.vr_iter(rec, u)
+    o.eval_rhs(es, u, rec, dlc)  # warm-up
+    t0 = time.perf_counter()
+    k = 0
+    while k < evals_min or (time.perf_counter() - t0 < budget_s and k < 200):
+        rec = o.vr_iter(rec, u)
+        o.eval_rhs(es, u, rec, dlc)
+        k += 1
+    dt = time.perf_counter() - t0
+    return {"value": o.nCell * k / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
+            "sample": f"TGV {n}^3 hex p2 HLLE+ FP64: {k} x (Jacobi VR sweep + EvaluateRHS), oracle C++ "
+                      f"with OpenMP face-buffer/cell-gather on {threads} threads, {dt:.1f} s"}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=64, help="cells per direction")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
+              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+    from dndsr_amd import _lib, fields
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+
+    n = args.n
+    L = 2 * np.pi
+    t_setup = time.perf_counter()
+    m = mm.periodic_hex_grid(n, n, n, L, L, L)
+    vfv = VariationalReconstruction(m, dict(maxOrder=2, intOrder=5)).construct()
+    ev = EulerEvaluator(vfv, dict(gamma=1.4, riemannSolverType="HLLEP"))
+    u0 = vfv.cell_average(lambda p: fields.taylor_green(p, 0.1, 1.4), 5)
+    u = vfv.to_device(u0)
+    uRec = vfv.BuildURec(5)
+    torch.cuda.synchronize()
+    t_setup = time.perf_counter() - t_setup
+    N = vfv.nCell
+    CFL = 0.5
+
+    ev.SSPRK3(u, uRec, args.warmup, CFL, 1)
+    torch.cuda.synchronize()
+    L_ = _lib.lib()
+    _lib.check(L_.dndsr_get_timing(vfv.ctx.h, None, None, 1))
+    _lib.check(L_.dndsr_set_timing(vfv.ctx.h, 1))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    ev.SSPRK3(u, uRec, args.steps, CFL, 1)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    _lib.check(L_.dndsr_set_timing(vfv.ctx.h, 0))
+    ms = np.zeros(8)
+    cnt = np.zeros(8, np.int64)
+    _lib.check(L_.dndsr_get_timing(vfv.ctx.h, ms.ctypes.data_as(_lib.dp), cnt.ctypes.data_as(_lib.i64p), 1))
+    elapsed = t1 - t0
+    ms_per_step = elapsed * 1e3 / max(args.steps, 1)
+    value = N * 3 * args.steps / elapsed
+
+    un = vfv.to_host(u)
+    if not np.all(np.isfinite(un)):
+        raise RuntimeError("non-finite state after the timed steps")
+
+    b_vr, b_rhs = alg_bytes(3, vfv.M, 5, 6, 3, vfv.nGFace)
+    kern = {}
+    names = _lib.KERNEL_IDS
+    for k, nm in enumerate(names):
+        if cnt[k]:
+            kern[nm] = {"ms_total": float(ms[k]), "launches": int(cnt[k]), "avg_ms": float(ms[k] / cnt[k])}
+    vr_avg = kern["vr_iter"]["avg_ms"]
+    rhs_avg = kern["rhs_face"]["avg_ms"] + kern["rhs_gather"]["avg_ms"]
+    roof = {
+        "vr_iter": {"bound": "hbm", "achieved": N * b_vr / (vr_avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                    "unit": "GB/s", "bytes_per_cell": b_vr, "avg_ms": vr_avg,
+                    "traffic": read_traffic("vr_iter_tgv64_p2")},
+        "eval_rhs": {"bound": "hbm", "achieved": N * b_rhs / (rhs_avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "bytes_per_cell": b_rhs, "avg_ms": rhs_avg,
+                     "traffic": read_traffic("rhs_tgv64_p2")},
+    }
+    for r in roof.values():
+        r["frac"] = r["achieved"] / r["peak"]
+    dom = "vr_iter" if kern["vr_iter"]["ms_total"] >= kern["rhs_face"]["ms_total"] + kern["rhs_gather"]["ms_total"] \
+        else "eval_rhs"
+    droof = roof[dom]
+    roofline = {"bound": "hbm", "achieved": round(droof["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                "frac": round(droof["frac"], 4), "traffic": droof["traffic"], "kernel": dom,
+                "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5)}
+
+    cpu = None
+    if not args.no_cpu_baseline:
+        cpu = cpu_baseline()
+
+    out = {
+        "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(ms_per_step, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "f64", "data": "synthetic",
+        "config": {"workload": f"3-D Euler Taylor-Green vortex M0.1, periodic [0,2pi]^3, {n}^3 hex = {N} cells, "
+                               f"p2 VR (9 modes), intOrder 5, HLLE+, FP64, SSP-RK3 CFL {CFL}, "
+                               f"1 Jacobi VR sweep per RHS", "cells": N, "rhs_evals_per_step": 3},
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": kern,
+        "roofline_all": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
+                         for k, r in roof.items()},
+        "setup_s": round(t_setup, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -65,7 +65,11 @@ SIGNATURES = {
     "dndsr_update_sgs_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                              C.c_int, vp],
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
+    "dndsr_set_timing": [vp, C.c_int],
+    "dndsr_get_timing": [vp, dp, i64p, C.c_int],
 }
+
+KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init"]
 
 _lib = None
 

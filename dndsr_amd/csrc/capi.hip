@@ -104,6 +104,10 @@ int dndsr_destroy(dndsr_ctx *ctx)
         return 0;
     (void)hipSetDevice(ctx->c.device);
     (void)hipStreamSynchronize(ctx->c.stream);
+    for (auto &r : ctx->c.recs)
+        (void)hipEventDestroy(r.a), (void)hipEventDestroy(r.b);
+    for (auto e : ctx->c.evPool)
+        (void)hipEventDestroy(e);
     freeAll(ctx->c);
     if (ctx->own)
         (void)hipStreamDestroy(ctx->own);
@@ -595,6 +599,42 @@ int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float a
     launchSGSSweep32(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
                      sumDev);
     fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
+int dndsr_set_timing(dndsr_ctx *ctx, int enable)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    ctx->c.timing = enable != 0;
+    CAPI_END
+}
+
+int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    Ctx &c = ctx->c;
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (auto &r : c.recs)
+    {
+        float ms = 0;
+        DNDSR_HIP_CHECK(hipEventElapsedTime(&ms, r.a, r.b));
+        c.kernMs[r.id] += ms;
+        c.kernCount[r.id] += 1;
+        c.evPool.push_back(r.a);
+        c.evPool.push_back(r.b);
+    }
+    c.recs.clear();
+    for (int k = 0; k < K_NUM; k++)
+    {
+        if (msTotal)
+            msTotal[k] = c.kernMs[k];
+        if (launches)
+            launches[k] = c.kernCount[k];
+        if (reset)
+            c.kernMs[k] = 0, c.kernCount[k] = 0;
+    }
     CAPI_END
 }
 

@@ -113,11 +113,17 @@ void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real max
                   real *lambdaFace, real *deltaLambdaCell, real *dtMinDev)
 {
     DevMesh &m = c.m;
-    kDtFace<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, gp.gamma, u, lambdaFace, c.faceLamTmp);
-    DNDSR_HIP_CHECK(hipGetLastError());
+    {
+        KernelTimer kt(c, K_DT_FACE);
+        kDtFace<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, gp.gamma, u, lambdaFace, c.faceLamTmp);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
     int nb = gridFor(m.N, 256);
-    kDtCell<<<nb, 256, 0, c.stream>>>(m, CFL, maxDt, lambdaFace, c.faceLamTmp, dt, deltaLambdaCell, c.red);
-    DNDSR_HIP_CHECK(hipGetLastError());
+    {
+        KernelTimer kt(c, K_DT_CELL);
+        kDtCell<<<nb, 256, 0, c.stream>>>(m, CFL, maxDt, lambdaFace, c.faceLamTmp, dt, deltaLambdaCell, c.red);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
     kReduceMin<<<1, 256, 0, c.stream>>>(c.red, nb, dtMinDev);
     DNDSR_HIP_CHECK(hipGetLastError());
     if (!useLocalDt)
@@ -305,6 +311,7 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
 {
     DevMesh &m = c.m;
     real *ff = c.faceFlux;
+    KernelTimer kt(c, K_RHS_FACE);
     if (m.dim == 2)
     {
         if (!uRec || m.P == 1)
@@ -323,6 +330,7 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
         else
             rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
     }
+    KernelTimer kt2(c, K_RHS_GATHER);
     kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
@@ -348,6 +356,7 @@ __global__ void __launch_bounds__(256) kLUSGSInit(DevMesh m, const real *dTau, r
 }
 void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag)
 {
+    KernelTimer kt(c, K_LUSGS_INIT);
     kLUSGSInit<<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m, dTau, dt, alphaDiag, lambdaFace, jdiag);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
@@ -477,6 +486,7 @@ static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, con
         if (e <= b)
             continue;
         int nb = gridFor(e - b, 256);
+        KernelTimer kt(c, K_SGS);
         kSGSColour<T><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jd, lam, col, forward,
                                                gsUpdate, uIncIsZero, b, e, c.red + off * 5);
         DNDSR_HIP_CHECK(hipGetLastError());
@@ -521,6 +531,7 @@ __global__ void __launch_bounds__(256) kAxpby(idx n, real a, const real *__restr
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
                  const real *dtDev, real *out)
 {
+    KernelTimer kt(c, K_AXPBY);
     kAxpby<<<gridFor(n, 256), 256, 0, c.stream>>>(n, a, x, b, y, cdt, z, dtDev, out);
     DNDSR_HIP_CHECK(hipGetLastError());
 }

@@ -97,6 +97,65 @@ struct Ctx
     real *faceLamTmp = nullptr; // [2][F] (lambdaFace, deltaLambdaFace)
     size_t faceLamCap = 0;
     std::vector<void *> owned;  // device allocations freed at destroy
+    // optional per-kernel timing with HIP events on the launch stream
+    bool timing = false;
+    struct TimedRec
+    {
+        int id;
+        hipEvent_t a, b;
+    };
+    std::vector<TimedRec> recs;
+    std::vector<hipEvent_t> evPool;
+    double kernMs[16] = {0};
+    long long kernCount[16] = {0};
+};
+
+// kernel ids reported by dndsr_get_timing
+enum KernelId : int
+{
+    K_VR_ITER = 0,
+    K_RHS_FACE = 1,
+    K_RHS_GATHER = 2,
+    K_DT_FACE = 3,
+    K_DT_CELL = 4,
+    K_AXPBY = 5,
+    K_SGS = 6,
+    K_LUSGS_INIT = 7,
+    K_NUM = 8,
+};
+
+// Records a HIP event pair around the launches in its scope when timing is on.
+struct KernelTimer
+{
+    Ctx &c;
+    int id;
+    hipEvent_t b = nullptr;
+    KernelTimer(Ctx &cc, int kid) : c(cc), id(kid)
+    {
+        if (!c.timing)
+            return;
+        hipEvent_t a = take();
+        b = take();
+        DNDSR_HIP_CHECK(hipEventRecord(a, c.stream));
+        c.recs.push_back({id, a, b});
+    }
+    ~KernelTimer()
+    {
+        if (b)
+            (void)hipEventRecord(b, c.stream);
+    }
+    hipEvent_t take()
+    {
+        if (!c.evPool.empty())
+        {
+            hipEvent_t e = c.evPool.back();
+            c.evPool.pop_back();
+            return e;
+        }
+        hipEvent_t e;
+        DNDSR_HIP_CHECK(hipEventCreate(&e));
+        return e;
+    }
 };
 
 // ---- launchers (setup.hip)

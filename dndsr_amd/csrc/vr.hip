@@ -172,6 +172,7 @@ static void vrIterDiffM(Ctx &c, int nVars, const real *x, real *y)
 
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
 {
+    KernelTimer kt(c, K_VR_ITER);
     switch (c.m.M)
     {
     case 2: vrIterM<2>(c, nVars, uRec, uRecNew, u, mode, relax); break;
@@ -185,6 +186,7 @@ void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real
 
 void launchVRIterDiff(Ctx &c, int nVars, const real *x, real *y)
 {
+    KernelTimer kt(c, K_VR_ITER);
     switch (c.m.M)
     {
     case 2: vrIterDiffM<2>(c, nVars, x, y); break;

@@ -158,6 +158,18 @@ int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float a
 int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, double *uRec, int nStep, double CFL,
                  int nRec, double dtFixed);
 
+/* ---- instrumentation (no counterpart in the reference, whose timers are the host-side
+ * PerformanceTimer slots of src/DNDS/Profiling.hpp:22-49) ------------------------------
+ * When enabled, every kernel launch is bracketed by a HIP event pair on its own stream.
+ * dndsr_get_timing synchronises, adds the elapsed times of all launches recorded since the
+ * last call per kernel id, and returns the running totals (ms) and launch counts:
+ *   0 VR iteration, 1 RHS face pass, 2 RHS cell gather, 3 EvaluateDt face pass,
+ *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init.
+ * reset != 0 clears the totals after reading them. */
+#define DNDSR_NUM_KERNEL_IDS 8
+int dndsr_set_timing(dndsr_ctx *ctx, int enable);
+int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset);
+
 #ifdef __cplusplus
 }
 #endif

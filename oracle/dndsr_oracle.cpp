@@ -42,6 +42,9 @@ static constexpr real smallReal = 1e-10;      // src/DNDS/Defines.hpp:201
 static constexpr real pi = 3.14159265358979323846;
 
 static thread_local std::string g_err;
+static int g_threads = 1;     // OpenMP threads used by the parallel loops
+static int g_rhsGather = 0;   // 0: serial face-order scatter (EvaluateRHS.hxx:550-562);
+                              // 1: face buffer + cell gather (the OpenMP variant, :153-159, 640-662)
 
 struct Pt
 {
@@ -997,8 +1000,10 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
 
     // ---- base moments, VariationalReconstruction.cpp:62-108
     v.moment.assign(v.nCell, std::vector<real>(v.NDOF, 0.0));
+#pragma omp parallel num_threads(g_threads)
     {
         std::vector<real> D;
+#pragma omp for schedule(static)
         for (idx_t i = 0; i < v.nCell; i++)
         {
             std::vector<real> mm(v.NDOF, 0.0);
@@ -1068,15 +1073,17 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
     int M = v.M, NDOF = v.NDOF, NDIFF = v.NDOF;
     v.AinvB.assign((size_t)v.nCell * (v.maxFC + 1) * M * M, 0.0);
     v.Ainvb.assign((size_t)v.nCell * v.maxFC * M, 0.0);
-    std::vector<real> Dfull, DI((size_t)NDIFF * M), DJ((size_t)NDIFF * M);
     auto extract = [&](const std::vector<real> &D, std::vector<real> &out, int nDiff) {
         out.assign((size_t)nDiff * M, 0.0);
         for (int k = 0; k < nDiff; k++)
             for (int j = 0; j < M; j++)
                 out[(size_t)k * M + j] = D[(size_t)k * NDOF + 1 + j];
     };
+    std::string firstErr;
+#pragma omp parallel for schedule(dynamic, 64) num_threads(g_threads)
     for (idx_t i = 0; i < v.nCell; i++)
     {
+        std::vector<real> Dfull, DI((size_t)NDIFF * M), DJ((size_t)NDIFF * M);
         int nf = (int)v.c2f[i].size();
         std::vector<real> A((size_t)M * M, 0.0);
         for (int ic = 0; ic < nf; ic++)
@@ -1129,7 +1136,19 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
             bs[ic] = b;
         }
         std::vector<real> Ainv;
-        invertMatrix(M, A, Ainv);
+        bool ok = true;
+        try
+        {
+            invertMatrix(M, A, Ainv);
+        }
+        catch (const std::exception &e)
+        {
+            ok = false;
+#pragma omp critical
+            firstErr = e.what();
+        }
+        if (!ok)
+            continue;
         real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
         std::copy(Ainv.begin(), Ainv.end(), blk);
         for (int ic = 0; ic < nf; ic++)
@@ -1156,6 +1175,8 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
             }
         }
     }
+    if (!firstErr.empty())
+        throw std::runtime_error(firstErr);
 }
 
 // ---------------------------------------------------------------------------
@@ -1238,6 +1259,13 @@ struct orc_vr
 extern "C" {
 
 const char *orc_last_error(void) { return g_err.c_str(); }
+
+int orc_set_threads(int nThreads, int rhsGather)
+{
+    g_threads = nThreads > 0 ? nThreads : 1;
+    g_rhsGather = rhsGather;
+    return 0;
+}
 
 #define ORC_TRY try {
 #define ORC_CATCH(ret)                     \
@@ -1387,6 +1415,13 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
     const VR &v = h->v;
     int M = v.M;
     size_t blkN = (size_t)M * nVars;
+    for (idx_t f = 0; f < v.nFace; f++)
+        if (v.f2c[f][1] < 0)
+        {
+            g_err = "boundary faces are not supported by the oracle VR iteration";
+            return -1;
+        }
+#pragma omp parallel for schedule(static) num_threads(g_threads)
     for (idx_t i = 0; i < v.nCell; i++)
     {
         real relax = v.s.jacobiRelax;
@@ -1398,11 +1433,6 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
         {
             idx_t f = v.c2f[i][ic];
             idx_t o = v.cellFaceOther(i, f);
-            if (o < 0)
-            {
-                g_err = "boundary faces are not supported by the oracle VR iteration";
-                return -1;
-            }
             const real *B = blk + (1 + ic) * (size_t)M * M;
             const real *bv = &v.Ainvb[((size_t)i * v.maxFC + ic) * M];
             for (int var = 0; var < nVars; var++)
@@ -1542,14 +1572,22 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
     int M = v.M;
     size_t G = v.faceQw.size();
     std::fill(rhs, rhs + v.nCell * 5, 0.0);
+    for (idx_t f = 0; f < v.nFace; f++)
+        if (v.f2c[f][1] < 0)
+            throw std::runtime_error("boundary faces are not supported by the oracle RHS");
+    std::vector<std::array<real, 5>> fbuf(v.nFace);
+    std::string firstErr;
+#pragma omp parallel num_threads(g_threads)
+    {
     std::vector<real> D;
     std::vector<std::array<real, 5>> ULv(G), URv(G), fincC(G);
     std::vector<std::array<real, 3>> nv(G);
+#pragma omp for schedule(static)
     for (idx_t f = 0; f < v.nFace; f++)
     {
+      try
+      {
         idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
-        if (cR < 0)
-            throw std::runtime_error("boundary faces are not supported by the oracle RHS");
         const real *ULm = u + cL * 5, *URm = u + cR * 5;
         for (size_t g = 0; g < G; g++)
         {
@@ -1601,10 +1639,41 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
             for (int var = 0; var < 5; var++)
                 fluxEs[var] += ((-fincC[g][var]) * v.faceJD[f][g]) * v.faceQw[g];
         for (int var = 0; var < 5; var++)
+            fbuf[f][var] = fluxEs[var];
+      }
+      catch (const std::exception &e)
+      {
+#pragma omp critical
+        firstErr = e.what();
+      }
+    }
+    }
+    if (!firstErr.empty())
+        throw std::runtime_error(firstErr);
+    if (!g_rhsGather)
+    {
+        // serial build: direct scatter in face order, EvaluateRHS.hxx:550-562
+        for (idx_t f = 0; f < v.nFace; f++)
         {
-            rhs[cL * 5 + var] += fluxEs[var] / v.vol[cL];
-            rhs[cR * 5 + var] += (-fluxEs[var]) / v.vol[cR];
+            idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
+            for (int var = 0; var < 5; var++)
+            {
+                rhs[cL * 5 + var] += fbuf[f][var] / v.vol[cL];
+                rhs[cR * 5 + var] += (-fbuf[f][var]) / v.vol[cR];
+            }
         }
+    }
+    else
+    {
+        // OpenMP build: per-cell gather over cell2face, EvaluateRHS.hxx:640-662
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+        for (idx_t i = 0; i < v.nCell; i++)
+            for (idx_t f : v.c2f[i])
+            {
+                real sgn = v.cellIsBack(i, f) ? 1.0 : -1.0;
+                for (int var = 0; var < 5; var++)
+                    rhs[i * 5 + var] += (fbuf[f][var] * sgn) / v.vol[i];
+            }
     }
     return 0;
     ORC_CATCH(-1)

@@ -66,6 +66,9 @@ typedef struct
 typedef struct orc_vr orc_vr;
 
 const char *orc_last_error(void);
+/* OpenMP threads for the parallel loops; rhsGather selects the reference's OpenMP
+ * face-buffer + cell-gather scatter instead of the serial face-order scatter */
+int orc_set_threads(int nThreads, int rhsGather);
 
 orc_vr *orc_vr_create(const orc_mesh_desc *mesh, const orc_vr_settings *s);
 void orc_vr_destroy(orc_vr *h);

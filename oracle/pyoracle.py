@@ -63,6 +63,10 @@ def lib():
     return _lib
 
 
+def set_threads(n, rhsGather=False):
+    lib().orc_set_threads(C.c_int(n), C.c_int(int(rhsGather)))
+
+
 def P(a):
     if a is None:
         return None
