@@ -87,6 +87,17 @@ __device__ inline void basisRow0(const real *xs, const real *moment /*[NDOF-1]*/
     }
 }
 
+// MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each
+// with its own L2.  Remap so that XCD x processes one contiguous range of tiles:
+// neighbouring tiles (and the cells / faces they share) then stay in one L2.
+constexpr int kNumXCD = 8;
+__device__ inline int xcdTile(int b, int nb)
+{
+    const int per = nb / kNumXCD, rem = nb % kNumXCD;
+    const int x = b % kNumXCD, k = b / kNumXCD;
+    return x * per + (x < rem ? x : rem) + k;
+}
+
 struct GasParams
 {
     real gamma;

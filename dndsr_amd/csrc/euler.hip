@@ -141,94 +141,195 @@ void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real max
 // and integrates -F w JDet.  Gather pass (the reference's face-buffer + cell
 // gather form, :640-662): rhs_i = sum over cell2face of +-F_f / V_i.  No atomics.
 // ---------------------------------------------------------------------------
-template <int DIM, int NDOF, int RS, bool BATCH>
-__global__ void __launch_bounds__(128) kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u,
-                                               const real *__restrict__ uRec, const real *__restrict__ dlc,
-                                               real *__restrict__ fflux)
+// Face tile kernel: 64 consecutive faces per block, one wavefront per face
+// quadrature point (lane = face, wave = point).  The two cells' means and
+// reconstruction coefficients are staged once into LDS with coalesced loads
+// (consecutive faces of one direction touch consecutive cells); wave 0 also
+// forms the face's Roe average of the means (it depends on the means only) and
+// parks it in LDS.  Every lane then evaluates its point, and the G point fluxes
+// of a face are summed in point order through LDS (the staging area is reused
+// after a barrier).
+constexpr int kTileF = 64;
+constexpr int kRoeRows = 20; // RoeAve fields
+constexpr int kFaceRows = kRoeRows + 5; // + batch vRN, lam0, lam123, lam4, and dLambda
+
+__device__ inline void roeStore(real *lds, int row0, int lane, const RoeAve &r)
+{
+    const real v[kRoeRows] = {r.veloLm[0], r.veloLm[1], r.veloLm[2], r.veloRm[0], r.veloRm[1], r.veloRm[2],
+                              r.aLm, r.aRm, r.pLm, r.pRm, r.veloRoe[0], r.veloRoe[1], r.veloRoe[2],
+                              r.vsqrRoe, r.HRoe, r.asqrRoe, r.rhoRoe, r.aRoe, 0, 0};
+#pragma unroll
+    for (int k = 0; k < kRoeRows; k++)
+        lds[(row0 + k) * kTileF + lane] = v[k];
+}
+__device__ inline void roeLoad(const real *lds, int row0, int lane, RoeAve &r)
+{
+    real v[kRoeRows];
+#pragma unroll
+    for (int k = 0; k < kRoeRows; k++)
+        v[k] = lds[(row0 + k) * kTileF + lane];
+    r.veloLm[0] = v[0], r.veloLm[1] = v[1], r.veloLm[2] = v[2];
+    r.veloRm[0] = v[3], r.veloRm[1] = v[4], r.veloRm[2] = v[5];
+    r.aLm = v[6], r.aRm = v[7], r.pLm = v[8], r.pRm = v[9];
+    r.veloRoe[0] = v[10], r.veloRoe[1] = v[11], r.veloRoe[2] = v[12];
+    r.vsqrRoe = v[13], r.HRoe = v[14], r.asqrRoe = v[15], r.rhoRoe = v[16], r.aRoe = v[17];
+}
+
+template <int DIM, int NDOF, int G, int RS, bool BATCH>
+__global__ void __launch_bounds__(kTileF *G) __attribute__((amdgpu_waves_per_eu(5)))
+kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__restrict__ uRec,
+         const real *__restrict__ dlc, real *__restrict__ fflux)
 {
     constexpr int M = NDOF - 1;
-    idx f = blockIdx.x * blockDim.x + threadIdx.x;
-    if (f >= m.F)
-        return;
+    extern __shared__ real lds[]; // rows of kTileF: [2][5] means, [2][5][M] coefficients, face constants
+    const int lane = threadIdx.x % kTileF, g = threadIdx.x / kTileF;
     const size_t N = m.N, F = m.F;
-    idx cL = m.f2c[f], cR = m.f2c[F + f];
-    real uL[5], uR[5];
-#pragma unroll
-    for (int v = 0; v < 5; v++)
-        uL[v] = u[v * N + cL], uR[v] = u[v * N + cR];
-    real cenL[3], cenR[3], sL = 1, sR = 1, momL[M > 0 ? M : 1], momR[M > 0 ? M : 1], sh[3];
-    if (uRec)
+    const int tile = xcdTile(blockIdx.x, gridDim.x);
+    const idx f = tile * kTileF + lane;
+    const bool valid = f < m.F;
+    idx cL = 0, cR = 0;
+    if (valid)
+        cL = m.f2c[f], cR = m.f2c[F + f];
+    const int rows = uRec ? 10 + 10 * M : 10;
+    const int RF = rows; // first face-constant row
+    for (int q = g; q < rows; q += G)
     {
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-            cenL[r] = m.cent[r * N + cL], cenR[r] = m.cent[r * N + cR], sh[r] = m.fshift[r * F + f];
-        sL = m.scale[cL], sR = m.scale[cR];
-#pragma unroll
-        for (int j = 0; j < M; j++)
-            momL[j] = m.moment[j * N + cL], momR[j] = m.moment[j * N + cR];
+        real val = 0;
+        if (valid)
+        {
+            if (q < 10)
+                val = u[(size_t)(q % 5) * N + (q < 5 ? cL : cR)];
+            else
+            {
+                int r = q - 10, side = r / (5 * M), vj = r % (5 * M);
+                val = uRec[(size_t)vj * N + (side ? cR : cL)];
+            }
+        }
+        lds[q * kTileF + lane] = val;
     }
-    real dLambda = dlc ? fmax(dlc[cL], dlc[cR]) : 0.0;
-    RoeBatchFace bf;
-    if constexpr (BATCH)
+    if (g == 0)
     {
-        real nm[3] = {m.fmeanN[f], m.fmeanN[F + f], m.fmeanN[2 * F + f]};
-        roeBatchSetup<RS>(uL, uR, nm, gp, dLambda, bf);
+        // face constants from the cell means
+        real uLm[5], uRm[5];
+        RoeAve ra;
+        real dLambda = 0;
+        if (valid)
+        {
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                uLm[v] = u[(size_t)v * N + cL], uRm[v] = u[(size_t)v * N + cR];
+            dLambda = dlc ? fmax(dlc[cL], dlc[cR]) : 0.0;
+        }
+        else
+        {
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                uLm[v] = uRm[v] = v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0);
+        }
+        if constexpr (BATCH)
+        {
+            RoeBatchFace bf;
+            real nm[3] = {m.fmeanN[valid ? f : 0], m.fmeanN[F + (valid ? f : 0)], m.fmeanN[2 * F + (valid ? f : 0)]};
+            roeBatchSetup<RS>(uLm, uRm, nm, gp, dLambda, bf);
+            roeStore(lds, RF, lane, bf.r);
+            lds[(RF + kRoeRows + 0) * kTileF + lane] = bf.vRN;
+            lds[(RF + kRoeRows + 1) * kTileF + lane] = bf.lam0;
+            lds[(RF + kRoeRows + 2) * kTileF + lane] = bf.lam123;
+            lds[(RF + kRoeRows + 3) * kTileF + lane] = bf.lam4;
+        }
+        else
+        {
+            roeAverage(uLm, uRm, gp.gamma, ra);
+            roeStore(lds, RF, lane, ra);
+        }
+        lds[(RF + kRoeRows + 4) * kTileF + lane] = dLambda;
     }
-    real acc[5] = {0, 0, 0, 0, 0};
-    for (int g = 0; g < m.G; g++)
+    __syncthreads();
+    real Fp[5] = {0, 0, 0, 0, 0};
+    if (valid)
     {
+        real UL[5], UR[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            UL[v] = lds[v * kTileF + lane], UR[v] = lds[(5 + v) * kTileF + lane];
         real x[3], n[3];
 #pragma unroll
         for (int r = 0; r < 3; r++)
             x[r] = m.fpts[((size_t)g * 3 + r) * F + f], n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
-        real wj = m.fwj[(size_t)g * F + f];
-        real UL[5], UR[5];
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-            UL[v] = uL[v], UR[v] = uR[v];
+        const real wj = m.fwj[(size_t)g * F + f];
         if (uRec)
         {
-            real xs[3], phi[M > 0 ? M : 1];
-            xs[0] = (x[0] - cenL[0]) / sL, xs[1] = (x[1] - cenL[1]) / sL;
-            xs[2] = DIM == 3 ? (x[2] - cenL[2]) / sL : (x[2] - cenL[2]);
-            basisRow0<DIM, NDOF>(xs, momL, phi);
 #pragma unroll
-            for (int v = 0; v < 5; v++)
+            for (int side = 0; side < 2; side++)
             {
-                real s = 0;
+                const idx c = side ? cR : cL;
+                real xs[3], mom[M], phi[M];
+                const real sc = m.scale[c];
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                {
+                    real p = x[r] + (side ? m.fshift[(size_t)r * F + f] : 0.0);
+                    xs[r] = (p - m.cent[(size_t)r * N + c]);
+                    if (DIM == 3 || r < 2)
+                        xs[r] /= sc;
+                }
 #pragma unroll
                 for (int j = 0; j < M; j++)
-                    s += phi[j] * uRec[((size_t)v * M + j) * N + cL];
-                UL[v] += s;
-            }
-            xs[0] = (x[0] + sh[0] - cenR[0]) / sR, xs[1] = (x[1] + sh[1] - cenR[1]) / sR;
-            xs[2] = DIM == 3 ? (x[2] + sh[2] - cenR[2]) / sR : (x[2] + sh[2] - cenR[2]);
-            basisRow0<DIM, NDOF>(xs, momR, phi);
+                    mom[j] = m.moment[(size_t)j * N + c];
+                basisRow0<DIM, NDOF>(xs, mom, phi);
+                const real *rec = lds + (size_t)(10 + side * 5 * M) * kTileF + lane;
+                real *U = side ? UR : UL;
 #pragma unroll
-            for (int v = 0; v < 5; v++)
-            {
-                real s = 0;
+                for (int v = 0; v < 5; v++)
+                {
+                    real s = 0;
 #pragma unroll
-                for (int j = 0; j < M; j++)
-                    s += phi[j] * uRec[((size_t)v * M + j) * N + cR];
-                UR[v] += s;
+                    for (int j = 0; j < M; j++)
+                        s += phi[j] * rec[(v * M + j) * kTileF];
+                    U[v] += s;
+                }
             }
         }
-        real Fp[5];
+        const real dLambda = lds[(RF + kRoeRows + 4) * kTileF + lane];
+        RoeAve ra;
+        roeLoad(lds, RF, lane, ra);
+        real Fl[5];
         if constexpr (BATCH)
-            roeBatchPoint<RS>(UL, UR, n, gp, bf, Fp);
+        {
+            RoeBatchFace bf;
+            bf.r = ra;
+            bf.vRN = lds[(RF + kRoeRows + 0) * kTileF + lane];
+            bf.lam0 = lds[(RF + kRoeRows + 1) * kTileF + lane];
+            bf.lam123 = lds[(RF + kRoeRows + 2) * kTileF + lane];
+            bf.lam4 = lds[(RF + kRoeRows + 3) * kTileF + lane];
+            roeBatchPoint<RS>(UL, UR, n, gp, bf, Fl);
+        }
         else
         {
             real l0, l1, l4;
-            riemannFlux<RS>(UL, UR, uL, uR, n, gp, dLambda, Fp, l0, l1, l4);
+            riemannFluxR<RS>(UL, UR, ra, n, gp, dLambda, Fl, l0, l1, l4);
         }
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            acc[v] += (-Fp[v]) * wj;
+            Fp[v] = (-Fl[v]) * wj;
     }
+    __syncthreads(); // all lanes are done with the staged data
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        fflux[v * F + f] = acc[v];
+        lds[(g * 5 + v) * kTileF + lane] = Fp[v];
+    __syncthreads();
+    for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
+    {
+        const int v = t / kTileF, l = t % kTileF;
+        const idx fo = tile * kTileF + l;
+        if (fo >= m.F)
+            continue;
+        real acc = 0;
+#pragma unroll
+        for (int q = 0; q < G; q++)
+            acc += lds[(q * 5 + v) * kTileF + l];
+        fflux[(size_t)v * F + fo] = acc;
+    }
 }
 
 __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restrict__ fflux, real *__restrict__ rhs)
@@ -257,19 +358,33 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
         rhs[v * N + i] = acc[v];
 }
 
-template <int DIM, int NDOF>
-static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
+template <int DIM, int NDOF, int G, int RS, bool BATCH>
+static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
 {
-    int g = gridFor(c.m.F, 128);
-    hipStream_t s = c.stream;
+    constexpr int M = NDOF - 1;
+    int rows = (uRec ? 10 + 10 * M : 10) + kFaceRows;
+    if (rows < 5 * G)
+        rows = 5 * G;
+    size_t smem = (size_t)rows * kTileF * sizeof(real);
+    auto kern = kRHSFace<DIM, NDOF, G, RS, BATCH>;
+    if (smem > 65536)
+        DNDSR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+    kern<<<gridFor(c.m.F, kTileF), kTileF * G, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+template <int DIM, int NDOF, int G>
+static void rhsDispatchRS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
+{
     bool batch = gp.rsMeanValueEig != 0 && gp.rsType >= RS_Roe_M1 && gp.rsType <= RS_Roe_M5;
-#define DNDSR_RHS_CASE(RSV)                                                                  \
-    case RSV:                                                                               \
-        kRHSFace<DIM, NDOF, RSV, false><<<g, 128, 0, s>>>(c.m, gp, u, uRec, dlc, ff);        \
+#define DNDSR_RHS_CASE(RSV)                                                  \
+    case RSV:                                                               \
+        rhsLaunch<DIM, NDOF, G, RSV, false>(c, gp, u, uRec, dlc, ff);       \
         break;
-#define DNDSR_RHS_BCASE(RSV)                                                                 \
-    case RSV:                                                                               \
-        kRHSFace<DIM, NDOF, RSV, true><<<g, 128, 0, s>>>(c.m, gp, u, uRec, dlc, ff);         \
+#define DNDSR_RHS_BCASE(RSV)                                                 \
+    case RSV:                                                               \
+        rhsLaunch<DIM, NDOF, G, RSV, true>(c, gp, u, uRec, dlc, ff);        \
         break;
     if (batch)
     {
@@ -304,7 +419,20 @@ static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *
     }
 #undef DNDSR_RHS_CASE
 #undef DNDSR_RHS_BCASE
-    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+// face quadrature sizes: 2-D Line rules of 2 / 3 points (intOrder 2..5), 3-D Quad rules
+// of 4 / 9 points (intOrder 2..5); the tile kernel is specialised on the point count.
+template <int DIM, int NDOF>
+static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
+{
+    constexpr int G1 = DIM == 2 ? 2 : 4, G2 = DIM == 2 ? 3 : 9;
+    if (c.m.G == G1)
+        rhsDispatchRS<DIM, NDOF, G1>(c, gp, u, uRec, dlc, ff);
+    else if (c.m.G == G2)
+        rhsDispatchRS<DIM, NDOF, G2>(c, gp, u, uRec, dlc, ff);
+    else
+        throw std::runtime_error("EvaluateRHS supports intOrder 2..5 (2/3 face points in 2-D, 4/9 in 3-D)");
 }
 
 void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs)

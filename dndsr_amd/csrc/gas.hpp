@@ -191,17 +191,17 @@ __host__ __device__ inline void sideFlux(const real *U, const real *n, real gamm
 
 // Full Riemann flux at one point with its own Roe average of the mean states.
 // Returns lam0/123/4 (after entropy fix) like the reference.
+// Point part of the Riemann flux, given the face's Roe average of the mean states
+// (the average depends only on ULm / URm, so a face computes it once).
 template <int RS>
-__host__ __device__ inline void riemannFlux(const real *UL, const real *UR, const real *ULm, const real *URm,
-                                            const real *n, const GasParams &gp, real dLambda, real *F,
-                                            real &lam0, real &lam123, real &lam4)
+__host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, const RoeAve &r, const real *n,
+                                             const GasParams &gp, real dLambda, real *F, real &lam0, real &lam123,
+                                             real &lam4)
 {
     const real gamma = gp.gamma;
     real FL[5], FR[5], pL, pR, aL, aR;
     sideFlux(UL, n, gamma, FL, pL, aL);
     sideFlux(UR, n, gamma, FR, pR, aR);
-    RoeAve r;
-    roeAverage(ULm, URm, gamma, r);
     real vRN = dot3(r.veloRoe, n);
     lam0 = fabs(vRN - r.aRoe);
     lam123 = fabs(vRN);
@@ -315,6 +315,16 @@ __host__ __device__ inline void riemannFlux(const real *UL, const real *UR, cons
             }
         }
     }
+}
+
+template <int RS>
+__host__ __device__ inline void riemannFlux(const real *UL, const real *UR, const real *ULm, const real *URm,
+                                            const real *n, const GasParams &gp, real dLambda, real *F,
+                                            real &lam0, real &lam123, real &lam4)
+{
+    RoeAve r;
+    roeAverage(ULm, URm, gp.gamma, r);
+    riemannFluxR<RS>(UL, UR, r, n, gp, dLambda, F, lam0, lam123, lam4);
 }
 
 // Batched Roe flux: eigenvalues from the cell-mean Roe average on the mean face

@@ -2,157 +2,138 @@
 //
 // DoReconstructionIter, Jacobi branch (SORInstead = false) and the recordInc
 // modes, src/CFV/VariationalReconstruction_Reconstruction.hxx:485-615, and
-// DoReconstructionIterDiff, :617-670.  One thread per cell: the cell's
-// (K x M x M) A^-1 B blocks stream in coalesced (cell index fastest), the
-// neighbour's M x nVars block is held in registers while the M rows are applied.
+// DoReconstructionIterDiff, :617-670.
 #include "internal.hpp"
 
 namespace dndsr
 {
-// mode 0: uRecNew = (1-w) uRec + w sum_f (A^-1B_f uRec_j + A^-1b_f (u_j - u_i))
-// mode 1: uRecNew = uRec - sum_f (...)            (recordInc: minus the block-Jacobi residual)
-// mode 2: uRecNew = - sum_f A^-1b_f (u_j - u_i)   (recordInc with uRecIsZero)
-template <int M, int NV>
-__global__ void __launch_bounds__(256) kVRIter(DevMesh m, const real *__restrict__ uRec, real *__restrict__ uRecNew,
-                                               const real *__restrict__ u, int mode, real relax)
+// Tile kernel: 64 consecutive cells per block, one wavefront per row r of the
+// M x M blocks (lane = cell).  For each face slot k the neighbours' M x nVars
+// coefficient blocks and means are staged once into LDS (coalesced), then every
+// row-wave applies its row of A^-1 B_k, streamed from HBM exactly once.
+// Modes (DIFF = false):
+//   0: uRecNew = (1-w) uRec + w sum_f (A^-1B_f uRec_j + A^-1b_f (u_j - u_i))
+//   1: uRecNew = uRec - sum_f (...)            (recordInc: minus the block-Jacobi residual)
+//   2: uRecNew = - sum_f A^-1b_f (u_j - u_i)   (recordInc with uRecIsZero)
+// DIFF = true: uRecNew = x - sum_f A^-1B_f x_j (DoReconstructionIterDiff).
+constexpr int kTileC = 64;
+
+template <int M>
+struct VRRows
 {
-    idx i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.N)
-        return;
+    static constexpr int RW = M <= 16 ? M : (M + 1) / 2; // row-waves per block
+    static constexpr int RPT = (M + RW - 1) / RW;        // rows per thread
+};
+
+template <int M, int NV, bool DIFF>
+__global__ void __launch_bounds__(kTileC *VRRows<M>::RW) __attribute__((amdgpu_waves_per_eu(5))) kVRIter(DevMesh m, const real *__restrict__ uRec,
+                                                                  real *__restrict__ uRecNew,
+                                                                  const real *__restrict__ u, int mode, real relax)
+{
+    constexpr int RW = VRRows<M>::RW, RPT = VRRows<M>::RPT;
+    constexpr int NROW = NV * M + (DIFF ? 0 : NV);
+    __shared__ real sNb[NROW][kTileC];
+    const int lane = threadIdx.x % kTileC, w = threadIdx.x / kTileC;
+    const int tile = xcdTile(blockIdx.x, gridDim.x);
     const size_t N = m.N;
-    real acc[NV][M];
-    real cf = mode == 0 ? (1 - relax) : (mode == 1 ? 1.0 : 0.0);
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-#pragma unroll
-        for (int r = 0; r < M; r++)
-            acc[v][r] = mode == 2 ? 0.0 : uRec[((size_t)v * M + r) * N + i] * cf;
+    const idx i = tile * kTileC + lane;
+    const bool valid = i < m.N;
+    real acc[RPT][NV];
     real ui[NV];
+    const real cf = DIFF ? 1.0 : (mode == 0 ? (1 - relax) : (mode == 1 ? 1.0 : 0.0));
 #pragma unroll
-    for (int v = 0; v < NV; v++)
-        ui[v] = u[(size_t)v * N + i];
-    for (int k = 0; k < m.K; k++)
+    for (int t = 0; t < RPT; t++)
     {
-        idx f = m.c2f[(size_t)k * N + i];
-        if (f < 0)
-            continue;
-        idx L = m.f2c[f], R = m.f2c[(size_t)m.F + f];
-        idx o = L == i ? R : L;
-        real du[NV];
+        const int r = w + t * RW;
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            du[v] = u[(size_t)v * N + o] - ui[v];
+            acc[t][v] = (valid && r < M && (DIFF || mode != 2)) ? uRec[((size_t)v * M + r) * N + i] * cf : 0.0;
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        ui[v] = (!DIFF && valid) ? u[(size_t)v * N + i] : 0.0;
+    const bool needRec = DIFF || mode != 2;
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = valid ? m.c2f[(size_t)k * N + i] : -1;
+        idx o = 0;
+        if (f >= 0)
+        {
+            const idx L = m.f2c[f], R = m.f2c[(size_t)m.F + f];
+            o = L == i ? R : L;
+        }
+        __syncthreads(); // previous slot's LDS reads are done
+        for (int q = w; q < NROW; q += RW)
+        {
+            real val = 0;
+            if (f >= 0)
+            {
+                if (q < NV * M)
+                    val = needRec ? uRec[(size_t)q * N + o] : 0.0;
+                else
+                    val = u[(size_t)(q - NV * M) * N + o];
+            }
+            sNb[q][lane] = val;
+        }
+        __syncthreads();
+        if (f < 0)
+            continue;
         const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N + i;
         const real *bv = m.Ainvb + ((size_t)k * M) * N + i;
-        if (mode == 2)
+#pragma unroll
+        for (int t = 0; t < RPT; t++)
         {
-#pragma unroll
-            for (int r = 0; r < M; r++)
-            {
-                real b = bv[(size_t)r * N];
-#pragma unroll
-                for (int v = 0; v < NV; v++)
-                    acc[v][r] -= b * du[v];
-            }
-            continue;
-        }
-        real ro[NV][M];
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-#pragma unroll
-            for (int q = 0; q < M; q++)
-                ro[v][q] = uRec[((size_t)v * M + q) * N + o];
-#pragma unroll
-        for (int r = 0; r < M; r++)
-        {
+            const int r = w + t * RW;
+            if (r >= M)
+                continue;
             real brow[M];
+            if (needRec)
 #pragma unroll
-            for (int q = 0; q < M; q++)
-                brow[q] = B[((size_t)r * M + q) * N];
-            real b = bv[(size_t)r * N];
+                for (int q = 0; q < M; q++)
+                    brow[q] = B[((size_t)r * M + q) * N];
+            const real b = DIFF ? 0.0 : bv[(size_t)r * N];
 #pragma unroll
             for (int v = 0; v < NV; v++)
             {
                 real s = 0;
+                if (needRec)
 #pragma unroll
-                for (int q = 0; q < M; q++)
-                    s += brow[q] * ro[v][q];
-                s += b * du[v];
-                if (mode == 0)
-                    acc[v][r] += relax * s;
+                    for (int q = 0; q < M; q++)
+                        s += brow[q] * sNb[v * M + q][lane];
+                if constexpr (DIFF)
+                    acc[t][v] -= s;
                 else
-                    acc[v][r] -= s;
+                {
+                    s += b * (sNb[NV * M + v][lane] - ui[v]);
+                    if (mode == 0)
+                        acc[t][v] += relax * s;
+                    else
+                        acc[t][v] -= s;
+                }
             }
         }
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-#pragma unroll
-        for (int r = 0; r < M; r++)
-            uRecNew[((size_t)v * M + r) * N + i] = acc[v][r];
-}
-
-// uRecNew = x - sum_f A^-1B_f x_j
-template <int M, int NV>
-__global__ void __launch_bounds__(256) kVRIterDiff(DevMesh m, const real *__restrict__ x, real *__restrict__ y)
-{
-    idx i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.N)
+    if (!valid)
         return;
-    const size_t N = m.N;
-    real acc[NV][M];
 #pragma unroll
-    for (int v = 0; v < NV; v++)
-#pragma unroll
-        for (int r = 0; r < M; r++)
-            acc[v][r] = x[((size_t)v * M + r) * N + i];
-    for (int k = 0; k < m.K; k++)
+    for (int t = 0; t < RPT; t++)
     {
-        idx f = m.c2f[(size_t)k * N + i];
-        if (f < 0)
-            continue;
-        idx L = m.f2c[f], R = m.f2c[(size_t)m.F + f];
-        idx o = L == i ? R : L;
-        const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N + i;
-        real ro[NV][M];
-#pragma unroll
-        for (int v = 0; v < NV; v++)
-#pragma unroll
-            for (int q = 0; q < M; q++)
-                ro[v][q] = x[((size_t)v * M + q) * N + o];
-#pragma unroll
-        for (int r = 0; r < M; r++)
-        {
-            real brow[M];
-#pragma unroll
-            for (int q = 0; q < M; q++)
-                brow[q] = B[((size_t)r * M + q) * N];
+        const int r = w + t * RW;
+        if (r < M)
 #pragma unroll
             for (int v = 0; v < NV; v++)
-            {
-                real s = 0;
-#pragma unroll
-                for (int q = 0; q < M; q++)
-                    s += brow[q] * ro[v][q];
-                acc[v][r] -= s;
-            }
-        }
+                uRecNew[((size_t)v * M + r) * N + i] = acc[t][v];
     }
-#pragma unroll
-    for (int v = 0; v < NV; v++)
-#pragma unroll
-        for (int r = 0; r < M; r++)
-            y[((size_t)v * M + r) * N + i] = acc[v][r];
 }
 
 template <int M>
 static void vrIterM(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
 {
-    int g = gridFor(c.m.N, 256);
+    int g = gridFor(c.m.N, kTileC), bs = kTileC * VRRows<M>::RW;
     if (nVars == 5)
-        kVRIter<M, 5><<<g, 256, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
+        kVRIter<M, 5, false><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
     else if (nVars == 1)
-        kVRIter<M, 1><<<g, 256, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
+        kVRIter<M, 1, false><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
     else
         throw std::runtime_error("nVars must be 1 or 5");
     DNDSR_HIP_CHECK(hipGetLastError());
@@ -160,11 +141,11 @@ static void vrIterM(Ctx &c, int nVars, const real *uRec, real *uRecNew, const re
 template <int M>
 static void vrIterDiffM(Ctx &c, int nVars, const real *x, real *y)
 {
-    int g = gridFor(c.m.N, 256);
+    int g = gridFor(c.m.N, kTileC), bs = kTileC * VRRows<M>::RW;
     if (nVars == 5)
-        kVRIterDiff<M, 5><<<g, 256, 0, c.stream>>>(c.m, x, y);
+        kVRIter<M, 5, true><<<g, bs, 0, c.stream>>>(c.m, x, y, nullptr, 0, 1.0);
     else if (nVars == 1)
-        kVRIterDiff<M, 1><<<g, 256, 0, c.stream>>>(c.m, x, y);
+        kVRIter<M, 1, true><<<g, bs, 0, c.stream>>>(c.m, x, y, nullptr, 0, 1.0);
     else
         throw std::runtime_error("nVars must be 1 or 5");
     DNDSR_HIP_CHECK(hipGetLastError());
