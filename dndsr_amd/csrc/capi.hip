@@ -218,6 +218,15 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
             hc2f[(size_t)k * N + i] = (idx)f;
         }
     }
+    // neighbour across each face slot (the VR sweep's only index)
+    std::vector<idx> hc2c((size_t)K * N, -1);
+    for (size_t i = 0; i < N; i++)
+        for (int k = 0; k < K; k++)
+        {
+            idx f = hc2f[(size_t)k * N + i];
+            if (f >= 0)
+                hc2c[(size_t)k * N + i] = hf2c[f] == (idx)i ? hf2c[F + f] : hf2c[f];
+        }
     // greedy multicolouring in cell order (smallest colour unused by face neighbours)
     std::vector<int32_t> col(N, -1);
     int nColor = 0;
@@ -256,6 +265,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.face2node = devAlloc<idx>(c, hfn.size());
     m.f2c = devAlloc<idx>(c, hf2c.size());
     m.c2f = devAlloc<idx>(c, hc2f.size());
+    m.c2c = devAlloc<idx>(c, hc2c.size());
     m.fshift = devAlloc<real>(c, hsh.size());
     m.color = devAlloc<int32_t>(c, N);
     m.colorCells = devAlloc<idx>(c, N);
@@ -264,6 +274,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     upload(c, m.face2node, hfn);
     upload(c, m.f2c, hf2c);
     upload(c, m.c2f, hc2f);
+    upload(c, m.c2c, hc2c);
     upload(c, m.fshift, hsh);
     upload(c, m.color, col);
     upload(c, m.colorCells, cells);

@@ -87,6 +87,47 @@ __device__ inline void basisRow0(const real *xs, const real *moment /*[NDOF-1]*/
     }
 }
 
+// Element i of a row whose base address is wave-uniform.  The 64-bit base stays
+// in SGPRs and each lane supplies one 32-bit byte offset (global_load saddr form),
+// so a lane needs one VGPR per distinct index instead of two per load address.
+// Every device row is below 4 GiB (entities per row < 2^29 doubles).
+template <class T>
+__device__ __forceinline__ T ldu(const T *__restrict__ base, uint32_t i)
+{
+    return *reinterpret_cast<const T *>(reinterpret_cast<const char *>(base) + (uint32_t)(i * (uint32_t)sizeof(T)));
+}
+template <class T>
+__device__ __forceinline__ void stu(T *__restrict__ base, uint32_t i, T v)
+{
+    *reinterpret_cast<T *>(reinterpret_cast<char *>(base) + (uint32_t)(i * (uint32_t)sizeof(T))) = v;
+}
+// Raw buffer access to one device row (CDNA buffer descriptor: 48-bit base, 32-bit
+// size in bytes, word 3 = 0x00020000 on gfx9-family parts).  The descriptor lives in
+// SGPRs, each lane supplies a 32-bit byte offset and the row's scalar offset goes in
+// soffset, so streaming many rows per lane costs no VGPRs for addresses.  A read at
+// an offset at or beyond the row size returns 0 (hardware range check).
+using Rsrc = __amdgpu_buffer_rsrc_t;
+__device__ __forceinline__ Rsrc rowRsrc(const void *base, uint64_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<void *>(base), (short)0,
+                                             (int)(uint32_t)(bytes < 0xFFFFFFFFull ? bytes : 0xFFFFFFFFull), 0x00020000);
+}
+__device__ __forceinline__ double bld(Rsrc r, uint32_t voff, uint32_t soff = 0)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
+}
+__device__ __forceinline__ int32_t bldi(Rsrc r, uint32_t voff, uint32_t soff = 0)
+{
+    return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);
+}
+constexpr uint32_t kNoRow = 0xFFFFFFF0u; // byte offset that reads as 0
+
+// wavefront index inside the block, as a scalar (uniform) value
+__device__ __forceinline__ int waveIndex(int waveSize = 64)
+{
+    return __builtin_amdgcn_readfirstlane((int)threadIdx.x / waveSize);
+}
+
 // MI355X dispatches the workgroups of a launch round-robin over its 8 XCDs, each
 // with its own L2.  Remap so that XCD x processes one contiguous range of tiles:
 // neighbouring tiles (and the cells / faces they share) then stay in one L2.

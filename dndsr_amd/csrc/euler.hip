@@ -142,13 +142,15 @@ void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real max
 // gather form, :640-662): rhs_i = sum over cell2face of +-F_f / V_i.  No atomics.
 // ---------------------------------------------------------------------------
 // Face tile kernel: 64 consecutive faces per block, one wavefront per face
-// quadrature point (lane = face, wave = point).  The two cells' means and
-// reconstruction coefficients are staged once into LDS with coalesced loads
-// (consecutive faces of one direction touch consecutive cells); wave 0 also
-// forms the face's Roe average of the means (it depends on the means only) and
-// parks it in LDS.  Every lane then evaluates its point, and the G point fluxes
-// of a face are summed in point order through LDS (the staging area is reused
-// after a barrier).
+// quadrature point (lane = face, wave = point).  Everything a point needs from
+// the two cells -- means, reconstruction coefficients, centre, 1/scale, basis
+// moments -- and the face's periodic shift is staged once into LDS rows of 64
+// doubles by waves 1..G-1 (coalesced row loads; all loads of a lane are issued
+// before the first wait), while wave 0 forms the face constants from the means
+// (Roe average, batched-Roe eigenvalues, dLambda) and parks them in LDS.  Each
+// lane's own point geometry is loaded into registers before the barrier.  After
+// it every lane evaluates the basis on both sides, reconstructs, evaluates the
+// flux and the G point fluxes of a face are summed in point order through LDS.
 constexpr int kTileF = 64;
 constexpr int kRoeRows = 20; // RoeAve fields
 constexpr int kFaceRows = kRoeRows + 5; // + batch vRN, lam0, lam123, lam4, and dLambda
@@ -157,7 +159,7 @@ __device__ inline void roeStore(real *lds, int row0, int lane, const RoeAve &r)
 {
     const real v[kRoeRows] = {r.veloLm[0], r.veloLm[1], r.veloLm[2], r.veloRm[0], r.veloRm[1], r.veloRm[2],
                               r.aLm, r.aRm, r.pLm, r.pRm, r.veloRoe[0], r.veloRoe[1], r.veloRoe[2],
-                              r.vsqrRoe, r.HRoe, r.asqrRoe, r.rhoRoe, r.aRoe, 0, 0};
+                              r.vsqrRoe, r.HRoe, r.asqrRoe, r.rhoRoe, r.aRoe, r.g1a2, r.inv2a};
 #pragma unroll
     for (int k = 0; k < kRoeRows; k++)
         lds[(row0 + k) * kTileF + lane] = v[k];
@@ -173,6 +175,55 @@ __device__ inline void roeLoad(const real *lds, int row0, int lane, RoeAve &r)
     r.aLm = v[6], r.aRm = v[7], r.pLm = v[8], r.pRm = v[9];
     r.veloRoe[0] = v[10], r.veloRoe[1] = v[11], r.veloRoe[2] = v[12];
     r.vsqrRoe = v[13], r.HRoe = v[14], r.asqrRoe = v[15], r.rhoRoe = v[16], r.aRoe = v[17];
+    r.g1a2 = v[18], r.inv2a = v[19];
+}
+
+// LDS row map of the face tile (rows of kTileF doubles)
+template <int NDOF>
+struct RHSRows
+{
+    static constexpr int M = NDOF - 1;
+    static constexpr int MEAN = 0;                 // [side][5]
+    static constexpr int REC = 10;                 // [side][5][M]
+    static constexpr int MET = REC + 10 * M;       // [side][cent 3, 1/scale, moment M]
+    static constexpr int METW = 4 + M;
+    static constexpr int SHIFT = MET + 2 * METW;   // [3]
+    static constexpr int STAGED = SHIFT + 3;       // rows loaded from global memory
+    static constexpr int FC = STAGED;              // face constants
+    static constexpr int TOTAL = FC + kFaceRows;
+};
+
+// Staged row q of the face tile: its device row (uniform per wave) and which
+// per-lane index (0: face, 1: left cell, 2: right cell) selects the element.
+template <int NDOF>
+__device__ inline const real *rhsStageRow(const DevMesh &m, const real *u, const real *uRec, int q, int &which)
+{
+    using R = RHSRows<NDOF>;
+    constexpr int M = R::M;
+    const size_t N = m.N, F = m.F;
+    if (q < R::REC)
+    {
+        which = q < 5 ? 1 : 2;
+        return u + (size_t)(q % 5) * N;
+    }
+    if (q < R::MET)
+    {
+        const int r = q - R::REC, side = r / (5 * M), vj = r % (5 * M);
+        which = 1 + side;
+        return uRec + (size_t)vj * N;
+    }
+    if (q < R::SHIFT)
+    {
+        const int r = q - R::MET, side = r / R::METW, k = r % R::METW;
+        which = 1 + side;
+        if (k < 3)
+            return m.cent + (size_t)k * N;
+        if (k == 3)
+            return m.scale;
+        return m.moment + (size_t)(k - 4) * N;
+    }
+    which = 0;
+    return m.fshift + (size_t)(q - R::SHIFT) * F;
 }
 
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
@@ -180,47 +231,80 @@ __global__ void __launch_bounds__(kTileF *G) __attribute__((amdgpu_waves_per_eu(
 kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__restrict__ uRec,
          const real *__restrict__ dlc, real *__restrict__ fflux)
 {
-    constexpr int M = NDOF - 1;
-    extern __shared__ real lds[]; // rows of kTileF: [2][5] means, [2][5][M] coefficients, face constants
-    const int lane = threadIdx.x % kTileF, g = threadIdx.x / kTileF;
+    using R = RHSRows<NDOF>;
+    constexpr int M = R::M;
+    // waves that stage rows: all but wave 0 when there are enough waves
+    constexpr int S0 = G >= 4 ? 1 : 0, NS = G - S0;
+    constexpr int PER = (R::STAGED + NS - 1) / NS;
+    extern __shared__ real lds[];
+    const int lane = threadIdx.x % kTileF, g = waveIndex(kTileF);
     const size_t N = m.N, F = m.F;
+    const uint64_t rowN = (uint64_t)N * 8, rowF = (uint64_t)F * 8;
     const int tile = xcdTile(blockIdx.x, gridDim.x);
     const idx f = tile * kTileF + lane;
     const bool valid = f < m.F;
-    idx cL = 0, cR = 0;
-    if (valid)
-        cL = m.f2c[f], cR = m.f2c[F + f];
-    const int rows = uRec ? 10 + 10 * M : 10;
-    const int RF = rows; // first face-constant row
-    for (int q = g; q < rows; q += G)
+    const uint32_t fo = (uint32_t)(valid ? f : 0) * 8u;
+    const idx cL = bldi(rowRsrc(m.f2c, (uint64_t)F * 4), fo / 2);
+    const idx cR = bldi(rowRsrc(m.f2c + F, (uint64_t)F * 4), fo / 2);
+    const uint32_t oL = (uint32_t)cL * 8u, oR = (uint32_t)cR * 8u;
+    auto offOf = [&](int which) { return which == 0 ? fo : (which == 1 ? oL : oR); };
+
+    // this lane's point geometry (registers, used after the barrier)
+    real x[3], n[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
     {
-        real val = 0;
-        if (valid)
+        x[r] = bld(rowRsrc(m.fpts + ((size_t)g * 3 + r) * F, rowF), fo);
+        n[r] = bld(rowRsrc(m.fnorm + ((size_t)g * 3 + r) * F, rowF), fo);
+    }
+    const real wj = bld(rowRsrc(m.fwj + (size_t)g * F, rowF), fo);
+
+    if (g >= S0)
+    {
+        // chunks of CH rows per lane: the loads of a chunk are in flight together
+        constexpr int CH = PER < 12 ? PER : 12;
+#pragma unroll
+        for (int t0 = 0; t0 < PER; t0 += CH)
         {
-            if (q < 10)
-                val = u[(size_t)(q % 5) * N + (q < 5 ? cL : cR)];
-            else
+            real val[CH];
+#pragma unroll
+            for (int tt = 0; tt < CH; tt++)
             {
-                int r = q - 10, side = r / (5 * M), vj = r % (5 * M);
-                val = uRec[(size_t)vj * N + (side ? cR : cL)];
+                const int qq = (g - S0) + (t0 + tt) * NS;
+                const int q = qq < R::STAGED ? qq : R::STAGED - 1;
+                int which;
+                const real *row = rhsStageRow<NDOF>(m, u, uRec, q, which);
+                // without coefficients (first-order residual) the rows read as 0
+                const bool skip = q >= R::REC && q < R::MET && !uRec;
+                val[tt] = bld(rowRsrc(skip ? m.cent : row, which ? rowN : rowF), skip ? kNoRow : offOf(which));
+            }
+#pragma unroll
+            for (int tt = 0; tt < CH; tt++)
+            {
+                const int q = (g - S0) + (t0 + tt) * NS;
+                const bool isScale = q >= R::MET && q < R::SHIFT && (q - R::MET) % R::METW == 3;
+                if (t0 + tt < PER && q < R::STAGED)
+                    lds[q * kTileF + lane] = isScale ? 1.0 / val[tt] : val[tt];
             }
         }
-        lds[q * kTileF + lane] = val;
     }
     if (g == 0)
     {
         // face constants from the cell means
         real uLm[5], uRm[5];
-        RoeAve ra;
         real dLambda = 0;
-        if (valid)
-        {
 #pragma unroll
-            for (int v = 0; v < 5; v++)
-                uLm[v] = u[(size_t)v * N + cL], uRm[v] = u[(size_t)v * N + cR];
-            dLambda = dlc ? fmax(dlc[cL], dlc[cR]) : 0.0;
+        for (int v = 0; v < 5; v++)
+        {
+            const Rsrc rr = rowRsrc(u + (size_t)v * N, rowN);
+            uLm[v] = bld(rr, oL), uRm[v] = bld(rr, oR);
         }
-        else
+        if (dlc)
+        {
+            const Rsrc rr = rowRsrc(dlc, rowN);
+            dLambda = fmax(bld(rr, oL), bld(rr, oR));
+        }
+        if (!valid)
         {
 #pragma unroll
             for (int v = 0; v < 5; v++)
@@ -229,20 +313,24 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
         if constexpr (BATCH)
         {
             RoeBatchFace bf;
-            real nm[3] = {m.fmeanN[valid ? f : 0], m.fmeanN[F + (valid ? f : 0)], m.fmeanN[2 * F + (valid ? f : 0)]};
+            real nm[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                nm[r] = bld(rowRsrc(m.fmeanN + (size_t)r * F, rowF), fo);
             roeBatchSetup<RS>(uLm, uRm, nm, gp, dLambda, bf);
-            roeStore(lds, RF, lane, bf.r);
-            lds[(RF + kRoeRows + 0) * kTileF + lane] = bf.vRN;
-            lds[(RF + kRoeRows + 1) * kTileF + lane] = bf.lam0;
-            lds[(RF + kRoeRows + 2) * kTileF + lane] = bf.lam123;
-            lds[(RF + kRoeRows + 3) * kTileF + lane] = bf.lam4;
+            roeStore(lds, R::FC, lane, bf.r);
+            lds[(R::FC + kRoeRows + 0) * kTileF + lane] = bf.vRN;
+            lds[(R::FC + kRoeRows + 1) * kTileF + lane] = bf.lam0;
+            lds[(R::FC + kRoeRows + 2) * kTileF + lane] = bf.lam123;
+            lds[(R::FC + kRoeRows + 3) * kTileF + lane] = bf.lam4;
         }
         else
         {
+            RoeAve ra;
             roeAverage(uLm, uRm, gp.gamma, ra);
-            roeStore(lds, RF, lane, ra);
+            roeStore(lds, R::FC, lane, ra);
         }
-        lds[(RF + kRoeRows + 4) * kTileF + lane] = dLambda;
+        lds[(R::FC + kRoeRows + 4) * kTileF + lane] = dLambda;
     }
     __syncthreads();
     real Fp[5] = {0, 0, 0, 0, 0};
@@ -251,33 +339,28 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
         real UL[5], UR[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            UL[v] = lds[v * kTileF + lane], UR[v] = lds[(5 + v) * kTileF + lane];
-        real x[3], n[3];
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-            x[r] = m.fpts[((size_t)g * 3 + r) * F + f], n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
-        const real wj = m.fwj[(size_t)g * F + f];
+            UL[v] = lds[(R::MEAN + v) * kTileF + lane], UR[v] = lds[(R::MEAN + 5 + v) * kTileF + lane];
         if (uRec)
         {
 #pragma unroll
             for (int side = 0; side < 2; side++)
             {
-                const idx c = side ? cR : cL;
-                real xs[3], mom[M], phi[M];
-                const real sc = m.scale[c];
+                const real *met = lds + (size_t)(R::MET + side * R::METW) * kTileF + lane;
+                const real isc = met[3 * kTileF];
+                real xs[3], mom[M > 0 ? M : 1], phi[M > 0 ? M : 1];
 #pragma unroll
                 for (int r = 0; r < 3; r++)
                 {
-                    real p = x[r] + (side ? m.fshift[(size_t)r * F + f] : 0.0);
-                    xs[r] = (p - m.cent[(size_t)r * N + c]);
+                    real p = x[r] + (side ? lds[(R::SHIFT + r) * kTileF + lane] : 0.0);
+                    xs[r] = p - met[r * kTileF];
                     if (DIM == 3 || r < 2)
-                        xs[r] /= sc;
+                        xs[r] *= isc;
                 }
 #pragma unroll
                 for (int j = 0; j < M; j++)
-                    mom[j] = m.moment[(size_t)j * N + c];
+                    mom[j] = met[(4 + j) * kTileF];
                 basisRow0<DIM, NDOF>(xs, mom, phi);
-                const real *rec = lds + (size_t)(10 + side * 5 * M) * kTileF + lane;
+                const real *rec = lds + (size_t)(R::REC + side * 5 * M) * kTileF + lane;
                 real *U = side ? UR : UL;
 #pragma unroll
                 for (int v = 0; v < 5; v++)
@@ -290,18 +373,18 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
                 }
             }
         }
-        const real dLambda = lds[(RF + kRoeRows + 4) * kTileF + lane];
+        const real dLambda = lds[(R::FC + kRoeRows + 4) * kTileF + lane];
         RoeAve ra;
-        roeLoad(lds, RF, lane, ra);
+        roeLoad(lds, R::FC, lane, ra);
         real Fl[5];
         if constexpr (BATCH)
         {
             RoeBatchFace bf;
             bf.r = ra;
-            bf.vRN = lds[(RF + kRoeRows + 0) * kTileF + lane];
-            bf.lam0 = lds[(RF + kRoeRows + 1) * kTileF + lane];
-            bf.lam123 = lds[(RF + kRoeRows + 2) * kTileF + lane];
-            bf.lam4 = lds[(RF + kRoeRows + 3) * kTileF + lane];
+            bf.vRN = lds[(R::FC + kRoeRows + 0) * kTileF + lane];
+            bf.lam0 = lds[(R::FC + kRoeRows + 1) * kTileF + lane];
+            bf.lam123 = lds[(R::FC + kRoeRows + 2) * kTileF + lane];
+            bf.lam4 = lds[(R::FC + kRoeRows + 3) * kTileF + lane];
             roeBatchPoint<RS>(UL, UR, n, gp, bf, Fl);
         }
         else
@@ -321,14 +404,14 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
     for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
     {
         const int v = t / kTileF, l = t % kTileF;
-        const idx fo = tile * kTileF + l;
-        if (fo >= m.F)
+        const idx fout = tile * kTileF + l;
+        if (fout >= m.F)
             continue;
         real acc = 0;
 #pragma unroll
         for (int q = 0; q < G; q++)
             acc += lds[(q * 5 + v) * kTileF + l];
-        fflux[(size_t)v * F + fo] = acc;
+        fflux[(size_t)v * F + fout] = acc;
     }
 }
 
@@ -361,8 +444,7 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
 static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
 {
-    constexpr int M = NDOF - 1;
-    int rows = (uRec ? 10 + 10 * M : 10) + kFaceRows;
+    int rows = RHSRows<NDOF>::TOTAL;
     if (rows < 5 * G)
         rows = 5 * G;
     size_t smem = (size_t)rows * kTileF * sizeof(real);

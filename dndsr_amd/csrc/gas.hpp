@@ -33,12 +33,19 @@ __host__ __device__ inline real signTol(real a, real tol) { return a > tol ? 1 :
 __host__ __device__ inline real signP(real a) { return a >= 0 ? 1 : -1; }
 __host__ __device__ inline real max3(real a, real b, real c) { return fmax(a, fmax(b, c)); }
 
-// IdealGasThermal (src/DNDS/IdealGasPhysics.hpp:37-44)
-__host__ __device__ inline void thermal(real E, real rho, real vSqr, real gamma, real &p, real &asqr, real &H)
+// IdealGasThermal (src/DNDS/IdealGasPhysics.hpp:37-44).  Divisions by rho are
+// taken as products with one reciprocal (FP64 division costs ~10 VALU ops on
+// CDNA; the product differs from the quotient by at most an ulp).
+__host__ __device__ inline void thermalR(real E, real rho, real rinv, real vSqr, real gamma, real &p, real &asqr,
+                                         real &H)
 {
     p = (gamma - 1) * (E - rho * 0.5 * vSqr);
-    asqr = gamma * p / rho;
-    H = (E + p) / rho;
+    asqr = gamma * p * rinv;
+    H = (E + p) * rinv;
+}
+__host__ __device__ inline void thermal(real E, real rho, real vSqr, real gamma, real &p, real &asqr, real &H)
+{
+    thermalR(E, rho, 1.0 / rho, vSqr, gamma, p, asqr, H);
 }
 
 // Face-normal physical flux (GasInviscidFlux_XY, Gas.hpp:390-397)
@@ -59,26 +66,32 @@ struct RoeAve
 {
     real veloLm[3], veloRm[3], aLm, aRm, pLm, pRm;
     real veloRoe[3], vsqrRoe, HRoe, asqrRoe, rhoRoe, aRoe;
+    real g1a2;  // (gamma - 1) / asqrRoe
+    real inv2a; // 1 / (2 aRoe)
 };
 __host__ __device__ inline void roeAverage(const real *ULm, const real *URm, real gamma, RoeAve &r)
 {
     real HLm, HRm, aLm2, aRm2;
+    const real rinvL = 1.0 / ULm[0], rinvR = 1.0 / URm[0];
 #pragma unroll
     for (int d = 0; d < 3; d++)
-        r.veloLm[d] = ULm[1 + d] / ULm[0], r.veloRm[d] = URm[1 + d] / URm[0];
-    thermal(ULm[4], ULm[0], dot3(r.veloLm, r.veloLm), gamma, r.pLm, aLm2, HLm);
-    thermal(URm[4], URm[0], dot3(r.veloRm, r.veloRm), gamma, r.pRm, aRm2, HRm);
+        r.veloLm[d] = ULm[1 + d] * rinvL, r.veloRm[d] = URm[1 + d] * rinvR;
+    thermalR(ULm[4], ULm[0], rinvL, dot3(r.veloLm, r.veloLm), gamma, r.pLm, aLm2, HLm);
+    thermalR(URm[4], URm[0], rinvR, dot3(r.veloRm, r.veloRm), gamma, r.pRm, aRm2, HRm);
     r.aLm = sqrt(aLm2);
     r.aRm = sqrt(aRm2);
     real sL = sqrt(ULm[0]), sR = sqrt(URm[0]);
+    const real sInv = 1.0 / (sL + sR);
 #pragma unroll
     for (int d = 0; d < 3; d++)
-        r.veloRoe[d] = (sL * r.veloLm[d] + sR * r.veloRm[d]) / (sL + sR);
+        r.veloRoe[d] = (sL * r.veloLm[d] + sR * r.veloRm[d]) * sInv;
     r.vsqrRoe = dot3(r.veloRoe, r.veloRoe);
-    r.HRoe = (sL * HLm + sR * HRm) / (sL + sR);
+    r.HRoe = (sL * HLm + sR * HRm) * sInv;
     r.asqrRoe = (gamma - 1) * (r.HRoe - 0.5 * r.vsqrRoe);
     r.rhoRoe = sL * sR;
     r.aRoe = sqrt(r.asqrRoe); // NaN when asqrRoe <= 0; the NaN flag of the caller catches it
+    r.g1a2 = (gamma - 1) / r.asqrRoe;
+    r.inv2a = 0.5 / r.aRoe;
 }
 
 // Entropy fixes (Roe_EntropyFixer, Gas.hpp:878-1007)
@@ -165,8 +178,8 @@ __host__ __device__ inline void roeDissipation(const real *dU, const real *n, co
     for (int d = 0; d < 3; d++)
         a23[d] -= n[d] * a23n;
     real dU4b = dU[4] - dot3(a23, r.veloRoe);
-    real al1 = (gamma - 1) / r.asqrRoe * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
-    real al0 = (dU[0] * (vRN + r.aRoe) - dUn - r.aRoe * al1) / (2 * r.aRoe);
+    real al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
+    real al0 = (dU[0] * (vRN + r.aRoe) - dUn - r.aRoe * al1) * r.inv2a;
     real al4 = dU[0] - (al0 + al1);
     al0 *= lam0, al1 *= lam123, al4 *= lam4;
 #pragma unroll
@@ -182,9 +195,10 @@ __host__ __device__ inline void roeDissipation(const real *dU, const real *n, co
 // Point states -> physical fluxes, pressures, velocities
 __host__ __device__ inline void sideFlux(const real *U, const real *n, real gamma, real *F, real &p, real &a)
 {
-    real v[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    const real rinv = 1.0 / U[0];
+    real v[3] = {U[1] * rinv, U[2] * rinv, U[3] * rinv};
     real a2, H;
-    thermal(U[4], U[0], dot3(v, v), gamma, p, a2, H);
+    thermalR(U[4], U[0], rinv, dot3(v, v), gamma, p, a2, H);
     a = sqrt(a2);
     physFlux(U, v, n, p, F);
 }
@@ -242,7 +256,7 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
         for (int d = 0; d < 3; d++)
             a23[d] -= n[d] * a23n;
         real dU4b = dU[4] - dot3(a23, r.veloRoe);
-        real al1 = (gamma - 1) / r.asqrRoe * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
+        real al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
         real SL = fmin(vRN - r.aRoe, vLm0 - aL);
         real SR = fmax(vRN + r.aRoe, vRm0 + aR);
         real dfix = r.aRoe / (r.aRoe + fmax(fabs(vRN), dLambda * gp.fixScale));
@@ -252,15 +266,17 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
             real rev[5] = {1, r.veloRoe[0], r.veloRoe[1], r.veloRoe[2], r.vsqrRoe * 0.5};
             real div = SP - SM;
             div += signP(div) * kVerySmall;
+            const real dinv = 1.0 / div, spsm = SP * SM * dinv, dal = dfix * al1;
 #pragma unroll
             for (int i = 0; i < 5; i++)
-                F[i] = (SP * FL[i] - SM * FR[i]) / div + (SP * SM / div) * (dU[i] - dfix * rev[i] * al1);
+                F[i] = (SP * FL[i] - SM * FR[i]) * dinv + spsm * (dU[i] - rev[i] * dal);
         }
         else
         {
-            lam123 = fabs(((SP + SM) * vRN - 2.0 * (1.0 - dfix) * (SP * SM)) / (SP - SM));
-            lam4 = fabs(((SP + SM) * (vRN + r.aRoe) - 2.0 * (SP * SM)) / (SP - SM));
-            lam0 = fabs(((SP + SM) * (vRN - r.aRoe) - 2.0 * (SP * SM)) / (SP - SM));
+            const real dinv = 1.0 / (SP - SM);
+            lam123 = fabs(((SP + SM) * vRN - 2.0 * (1.0 - dfix) * (SP * SM)) * dinv);
+            lam4 = fabs(((SP + SM) * (vRN + r.aRoe) - 2.0 * (SP * SM)) * dinv);
+            lam0 = fabs(((SP + SM) * (vRN - r.aRoe) - 2.0 * (SP * SM)) * dinv);
             real incF[5];
             roeDissipation(dU, n, r, vRN, lam0, lam123, lam4, gamma, incF);
 #pragma unroll

@@ -54,6 +54,7 @@ struct DevMesh
     // connectivity
     idx *f2c = nullptr;     // [2][F], -1 = boundary
     idx *c2f = nullptr;     // [K][N], -1 = absent
+    idx *c2c = nullptr;     // [K][N] cell across face slot k, -1 = absent
     idx *cell2node = nullptr; // [nnc][N]
     idx *face2node = nullptr; // [nnf][F]
     real *coords = nullptr;   // [3][nNode]

@@ -10,7 +10,11 @@ namespace dndsr
 // Tile kernel: 64 consecutive cells per block, one wavefront per row r of the
 // M x M blocks (lane = cell).  For each face slot k the neighbours' M x nVars
 // coefficient blocks and means are staged once into LDS (coalesced), then every
-// row-wave applies its row of A^-1 B_k, streamed from HBM exactly once.
+// row-wave applies its row of A^-1 B_k, streamed from HBM exactly once.  The
+// loop is software-pipelined over the face slots: the neighbour rows and the
+// matrix row of slot k+1 are loaded into registers while slot k is applied, and
+// the LDS stage is double-buffered, so one barrier per slot suffices and two
+// slots' worth of loads are in flight per wave.
 // Modes (DIFF = false):
 //   0: uRecNew = (1-w) uRec + w sum_f (A^-1B_f uRec_j + A^-1b_f (u_j - u_i))
 //   1: uRecNew = uRec - sum_f (...)            (recordInc: minus the block-Jacobi residual)
@@ -25,94 +29,146 @@ struct VRRows
     static constexpr int RPT = (M + RW - 1) / RW;        // rows per thread
 };
 
-template <int M, int NV, bool DIFF>
-__global__ void __launch_bounds__(kTileC *VRRows<M>::RW) __attribute__((amdgpu_waves_per_eu(5))) kVRIter(DevMesh m, const real *__restrict__ uRec,
-                                                                  real *__restrict__ uRecNew,
-                                                                  const real *__restrict__ u, int mode, real relax)
+// KIND: 0 = Jacobi iteration (mode 0), 3 = recordInc (mode 1), 1 = recordInc with
+// uRecIsZero (mode 2), 2 = IterDiff.  The staged mean rows hold u_j - u_i.
+template <int M, int NV, int KIND>
+__global__ void __launch_bounds__(kTileC *VRRows<M>::RW) __attribute__((amdgpu_waves_per_eu(5)))
+kVRIter(DevMesh m, const real *__restrict__ uRec, real *__restrict__ uRecNew, const real *__restrict__ u, int mode,
+        real relax)
 {
+    constexpr bool DIFF = KIND == 2, REC = KIND != 1, RELAX = KIND == 0;
     constexpr int RW = VRRows<M>::RW, RPT = VRRows<M>::RPT;
-    constexpr int NROW = NV * M + (DIFF ? 0 : NV);
-    __shared__ real sNb[NROW][kTileC];
-    const int lane = threadIdx.x % kTileC, w = threadIdx.x / kTileC;
+    constexpr int NROW = (REC ? NV * M : 0) + (DIFF ? 0 : NV); // staged neighbour rows
+    constexpr int R0 = REC ? NV * M : 0;                       // first mean row
+    constexpr int PER = (NROW + RW - 1) / RW;                  // staged rows per lane
+    __shared__ real sNb[2][PER * RW][kTileC]; // padded so that every lane stores PER rows
+    const int lane = threadIdx.x % kTileC, w = waveIndex(kTileC);
     const int tile = xcdTile(blockIdx.x, gridDim.x);
     const size_t N = m.N;
+    const int K = m.K;
     const idx i = tile * kTileC + lane;
     const bool valid = i < m.N;
+    const uint32_t ioff = (uint32_t)i * 8u; // reads past the row (tail lanes) return 0
+    const uint64_t rowB = (uint64_t)N * 8;
+    auto row = [&](const real *base) { return rowRsrc(base, rowB); };
     real acc[RPT][NV];
-    real ui[NV];
-    const real cf = DIFF ? 1.0 : (mode == 0 ? (1 - relax) : (mode == 1 ? 1.0 : 0.0));
+    const real cf = RELAX ? (1 - relax) : 1.0;
 #pragma unroll
     for (int t = 0; t < RPT; t++)
     {
-        const int r = w + t * RW;
+        const int r = w + t * RW < M ? w + t * RW : M - 1;
 #pragma unroll
         for (int v = 0; v < NV; v++)
-            acc[t][v] = (valid && r < M && (DIFF || mode != 2)) ? uRec[((size_t)v * M + r) * N + i] * cf : 0.0;
+            acc[t][v] = REC ? bld(row(uRec + ((size_t)v * M + r) * N), ioff) * cf : 0.0;
     }
+    // own mean of the variable this lane stages in the mean rows (q = R0 + v)
+    real uOwn[PER];
 #pragma unroll
-    for (int v = 0; v < NV; v++)
-        ui[v] = (!DIFF && valid) ? u[(size_t)v * N + i] : 0.0;
-    const bool needRec = DIFF || mode != 2;
-    for (int k = 0; k < m.K; k++)
+    for (int t = 0; t < PER; t++)
     {
-        const idx f = valid ? m.c2f[(size_t)k * N + i] : -1;
-        idx o = 0;
-        if (f >= 0)
+        const int q = w + t * RW;
+        uOwn[t] = (!DIFF && q >= R0 && q < NROW) ? bld(row(u + (size_t)(q - R0) * N), ioff) : 0.0;
+    }
+
+    // All loads are raw buffer loads with the row descriptor in SGPRs; absent
+    // neighbours (o < 0) read 0 through an out-of-range offset, so no load is
+    // predicated and the compiler keeps them in flight together.
+    auto loadNb = [&](idx o, real *nb) {
+        const uint32_t ooff = o >= 0 ? (uint32_t)o * 8u : kNoRow;
+#pragma unroll
+        for (int t = 0; t < PER; t++)
         {
-            const idx L = m.f2c[f], R = m.f2c[(size_t)m.F + f];
-            o = L == i ? R : L;
+            const int q = w + t * RW < NROW ? w + t * RW : NROW - 1;
+            const real *base = q < R0 ? uRec + (size_t)q * N : u + (size_t)(q - R0) * N;
+            nb[t] = bld(row(base), ooff);
         }
-        __syncthreads(); // previous slot's LDS reads are done
-        for (int q = w; q < NROW; q += RW)
+    };
+    auto storeNb = [&](int buf, const real *nb) {
+#pragma unroll
+        for (int t = 0; t < PER; t++)
+            sNb[buf][w + t * RW][lane] = nb[t] - uOwn[t];
+    };
+    // matrix rows of slot k (AinvB / Ainvb hold K slots for every cell)
+    auto loadB = [&](int k, real (*br)[M], real *bb) {
+        const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N;
+        const real *bv = m.Ainvb + ((size_t)k * M) * N;
+#pragma unroll
+        for (int t = 0; t < RPT; t++)
         {
-            real val = 0;
-            if (f >= 0)
-            {
-                if (q < NV * M)
-                    val = needRec ? uRec[(size_t)q * N + o] : 0.0;
-                else
-                    val = u[(size_t)(q - NV * M) * N + o];
-            }
-            sNb[q][lane] = val;
+            const int r = w + t * RW < M ? w + t * RW : M - 1;
+            if constexpr (REC)
+#pragma unroll
+                for (int q = 0; q < M; q++)
+                    br[t][q] = bld(row(B + ((size_t)r * M + q) * N), ioff);
+            bb[t] = DIFF ? 0.0 : bld(row(bv + (size_t)r * N), ioff);
         }
-        __syncthreads();
-        if (f < 0)
-            continue;
-        const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N + i;
-        const real *bv = m.Ainvb + ((size_t)k * M) * N + i;
+    };
+    auto nbIndex = [&](int k) -> idx {
+        return valid ? bldi(rowRsrc(m.c2c + (size_t)k * N, (uint64_t)N * 4), (uint32_t)i * 4u) : -1;
+    };
+
+    real brow[RPT][M], bb[RPT];
+    auto apply = [&](int buf, idx oc) {
+        if (oc < 0)
+            return;
 #pragma unroll
         for (int t = 0; t < RPT; t++)
         {
             const int r = w + t * RW;
             if (r >= M)
                 continue;
-            real brow[M];
-            if (needRec)
-#pragma unroll
-                for (int q = 0; q < M; q++)
-                    brow[q] = B[((size_t)r * M + q) * N];
-            const real b = DIFF ? 0.0 : bv[(size_t)r * N];
 #pragma unroll
             for (int v = 0; v < NV; v++)
             {
                 real s = 0;
-                if (needRec)
+                if constexpr (REC)
 #pragma unroll
                     for (int q = 0; q < M; q++)
-                        s += brow[q] * sNb[v * M + q][lane];
+                        s += brow[t][q] * sNb[buf][v * M + q][lane];
                 if constexpr (DIFF)
                     acc[t][v] -= s;
                 else
                 {
-                    s += b * (sNb[NV * M + v][lane] - ui[v]);
-                    if (mode == 0)
+                    s += bb[t] * sNb[buf][R0 + v][lane];
+                    if constexpr (RELAX)
                         acc[t][v] += relax * s;
                     else
                         acc[t][v] -= s;
                 }
             }
         }
+    };
+
+    // Slot pipeline.  Iteration k: issue the neighbour rows of slot k+1, barrier,
+    // apply slot k (its matrix rows were issued during iteration k-1), issue the
+    // matrix rows of slot k+1 into the registers just freed, then park the
+    // neighbour rows of slot k+1 in the other LDS buffer.  The neighbour index of
+    // slot k+2 is fetched one iteration ahead.  The last slot is peeled so that
+    // the loop body is straight-line code (exact waits on the in-order vmcnt).
+    idx o = nbIndex(0);
+    idx on = K > 1 ? nbIndex(1) : -1;
+    {
+        real nb[PER];
+        loadNb(o, nb);
+        loadB(0, brow, bb);
+        storeNb(0, nb);
     }
+    for (int k = 0; k + 1 < K; k++)
+    {
+        const int buf = k & 1;
+        const int k2 = k + 2 < K ? k + 2 : K - 1;
+        const idx onn = nbIndex(k2);
+        real nbN[PER];
+        loadNb(on, nbN);
+        __syncthreads(); // sNb[buf] complete; sNb[buf ^ 1] no longer read
+        apply(buf, o);
+        loadB(k + 1, brow, bb);
+        storeNb(buf ^ 1, nbN);
+        o = on;
+        on = onn;
+    }
+    __syncthreads();
+    apply((K - 1) & 1, o);
     if (!valid)
         return;
 #pragma unroll
@@ -122,33 +178,36 @@ __global__ void __launch_bounds__(kTileC *VRRows<M>::RW) __attribute__((amdgpu_w
         if (r < M)
 #pragma unroll
             for (int v = 0; v < NV; v++)
-                uRecNew[((size_t)v * M + r) * N + i] = acc[t][v];
+                stu(uRecNew + ((size_t)v * M + r) * N, (uint32_t)i, acc[t][v]);
     }
 }
 
-template <int M>
-static void vrIterM(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
+template <int M, int KIND>
+static void vrLaunch(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
 {
     int g = gridFor(c.m.N, kTileC), bs = kTileC * VRRows<M>::RW;
     if (nVars == 5)
-        kVRIter<M, 5, false><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
+        kVRIter<M, 5, KIND><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
     else if (nVars == 1)
-        kVRIter<M, 1, false><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
+        kVRIter<M, 1, KIND><<<g, bs, 0, c.stream>>>(c.m, uRec, uRecNew, u, mode, relax);
     else
         throw std::runtime_error("nVars must be 1 or 5");
     DNDSR_HIP_CHECK(hipGetLastError());
+}
+template <int M>
+static void vrIterM(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
+{
+    if (mode == 2)
+        vrLaunch<M, 1>(c, nVars, uRec, uRecNew, u, mode, relax);
+    else if (mode == 1)
+        vrLaunch<M, 3>(c, nVars, uRec, uRecNew, u, mode, relax);
+    else
+        vrLaunch<M, 0>(c, nVars, uRec, uRecNew, u, mode, relax);
 }
 template <int M>
 static void vrIterDiffM(Ctx &c, int nVars, const real *x, real *y)
 {
-    int g = gridFor(c.m.N, kTileC), bs = kTileC * VRRows<M>::RW;
-    if (nVars == 5)
-        kVRIter<M, 5, true><<<g, bs, 0, c.stream>>>(c.m, x, y, nullptr, 0, 1.0);
-    else if (nVars == 1)
-        kVRIter<M, 1, true><<<g, bs, 0, c.stream>>>(c.m, x, y, nullptr, 0, 1.0);
-    else
-        throw std::runtime_error("nVars must be 1 or 5");
-    DNDSR_HIP_CHECK(hipGetLastError());
+    vrLaunch<M, 2>(c, nVars, x, y, nullptr, 0, 1.0);
 }
 
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
