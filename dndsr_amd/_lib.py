@@ -37,6 +37,13 @@ class EulerSettings(C.Structure):
                 ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double)]
 
 
+class ImplicitSettings(C.Structure):
+    _fields_ = [("dt", C.c_double), ("CFL", C.c_double), ("nRec", C.c_int), ("sgsIter", C.c_int),
+                ("gsUpdate", C.c_int), ("maxIter", C.c_int), ("minIter", C.c_int), ("tol", C.c_double),
+                ("fp32Sweeps", C.c_int), ("incrementRelax", C.c_double), ("refRho", C.c_double),
+                ("refP", C.c_double)]
+
+
 # Every symbol of include/dndsr_amd.h with its argument types.
 SIGNATURES = {
     "dndsr_create": [C.POINTER(vp), C.c_int],
@@ -65,11 +72,14 @@ SIGNATURES = {
     "dndsr_update_sgs_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                              C.c_int, vp],
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
+    "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
+                                  C.POINTER(C.c_int), dp, dp, dp],
     "dndsr_set_timing": [vp, C.c_int],
     "dndsr_get_timing": [vp, dp, i64p, C.c_int],
 }
 
-KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init"]
+KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init",
+              "dual_residual", "increment", "to_f32"]
 
 _lib = None
 

@@ -4,6 +4,7 @@
 // device-resident SSP-RK3 loop; every numerical operation runs in the HIP kernels
 // of setup.hip, vr.hip and euler.hip.
 #include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <string>
 #include <vector>
@@ -21,6 +22,12 @@ struct dndsr_ctx
     real *wXLast = nullptr, *wRhs = nullptr, *wURecNew = nullptr, *wDt = nullptr, *wLam = nullptr, *wDlc = nullptr,
          *wDtMin = nullptr;
     bool haveWork = false;
+    // implicit dual-time workspace
+    real *iXLast = nullptr, *iRhsF = nullptr, *iRhs = nullptr, *iDTau = nullptr, *iJd = nullptr, *iInc = nullptr,
+         *iTmp = nullptr, *iRes = nullptr;
+    float *fRhs = nullptr, *fU = nullptr, *fJd = nullptr, *fLam = nullptr, *fInc = nullptr, *fTmp = nullptr;
+    int *iFlag = nullptr;
+    bool haveImplicit = false;
 };
 
 static thread_local std::string g_err;
@@ -700,6 +707,139 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
     }
     if (rec != uRec) // odd number of VR sweeps in total: bring the result home
         DNDSR_HIP_CHECK(hipMemcpyAsync(uRec, rec, nR * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+    CAPI_END
+}
+
+int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
+                              double *u, double *uRec, int *nIter, double *res, double *resRel, double *history)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && is && u && uRec, "null argument");
+    need(is->dt > 0 && is->CFL > 0 && is->nRec >= 0 && is->sgsIter >= 0 && is->maxIter >= 1 && is->minIter >= 0,
+         "bad implicit settings");
+    Ctx &c = ctx->c;
+    DevMesh &m = c.m;
+    GasParams gp = toGas(es);
+    const size_t N = m.N, F = m.F;
+    const size_t nU = 5 * N, nR = 5 * (size_t)m.M * N;
+    if (!ctx->haveWork)
+    {
+        ctx->wXLast = devAlloc<real>(c, nU);
+        ctx->wRhs = devAlloc<real>(c, nU);
+        ctx->wURecNew = devAlloc<real>(c, nR);
+        ctx->wDt = devAlloc<real>(c, N);
+        ctx->wLam = devAlloc<real>(c, 4 * F);
+        ctx->wDlc = devAlloc<real>(c, N);
+        ctx->wDtMin = devAlloc<real>(c, 1);
+        ctx->haveWork = true;
+    }
+    if (!ctx->haveImplicit)
+    {
+        ctx->iXLast = devAlloc<real>(c, nU);
+        ctx->iRhsF = devAlloc<real>(c, nU);
+        ctx->iRhs = devAlloc<real>(c, nU);
+        ctx->iDTau = devAlloc<real>(c, N);
+        ctx->iJd = devAlloc<real>(c, N);
+        ctx->iInc = devAlloc<real>(c, nU);
+        ctx->iTmp = devAlloc<real>(c, nU);
+        ctx->iRes = devAlloc<real>(c, 8);
+        ctx->fRhs = devAlloc<float>(c, nU);
+        ctx->fU = devAlloc<float>(c, nU);
+        ctx->fJd = devAlloc<float>(c, N);
+        ctx->fLam = devAlloc<float>(c, 4 * F);
+        ctx->fInc = devAlloc<float>(c, nU);
+        ctx->fTmp = devAlloc<float>(c, nU);
+        ctx->iFlag = devAlloc<int>(c, 1);
+        ctx->haveImplicit = true;
+    }
+    const real gamma = es->gamma;
+    const real rhoEps = kSmall * is->refRho * 1e-1, pEps = kSmall * is->refP * 1e-1;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(m.N, 256) + m.nColor + 6);
+    DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iFlag, 0, sizeof(int), c.stream));
+    DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->iXLast, u, nU * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+    real *rec = uRec, *recNew = ctx->wURecNew;
+    real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
+    int it = 0;
+    for (it = 1; it <= is->maxIter; it++)
+    {
+        // fdtau: local pseudo time step (alphaDiag = 1)
+        launchEvalDt(c, gp, u, is->CFL, 1e100, 1, ctx->iDTau, ctx->wLam, ctx->wDlc, ctx->wDtMin);
+        // frhs: nRec Jacobi VR sweeps + EvaluateRHS
+        for (int r = 0; r < is->nRec; r++)
+        {
+            launchVRIter(c, 5, rec, recNew, u, 0, m.s.jacobiRelax);
+            std::swap(rec, recNew);
+        }
+        launchEvalRHS(c, gp, u, rec, ctx->wDlc, ctx->iRhsF);
+        launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
+        // fsolve: LUSGSMatrixInit + (1 + sgsIter) symmetric sweeps from a zero increment
+        launchLUSGSInit(c, ctx->iDTau, is->dt, 1.0, ctx->wLam, ctx->iJd);
+        if (is->fp32Sweeps)
+        {
+            launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
+            launchToF32(c, nU, u, ctx->fU);
+            launchToF32(c, N, ctx->iJd, ctx->fJd);
+            launchToF32(c, F, ctx->wLam, ctx->fLam); // lambdaFace row only
+            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->fInc, 0, nU * sizeof(float), c.stream));
+            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->fTmp, 0, nU * sizeof(float), c.stream));
+            float *inc = ctx->fInc, *tmp = ctx->fTmp;
+            for (int s = 0; s <= is->sgsIter; s++)
+                for (int dir = 1; dir >= 0; dir--)
+                {
+                    launchSGSSweep32(c, gp, 1.0f, ctx->fRhs, ctx->fU, inc, tmp, ctx->fJd, ctx->fLam, dir,
+                                     is->gsUpdate, 0, sumDev);
+                    std::swap(inc, tmp);
+                }
+            launchAddIncrement32(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, ctx->iFlag);
+        }
+        else
+        {
+            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iInc, 0, nU * sizeof(real), c.stream));
+            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iTmp, 0, nU * sizeof(real), c.stream));
+            real *inc = ctx->iInc, *tmp = ctx->iTmp;
+            for (int s = 0; s <= is->sgsIter; s++)
+                for (int dir = 1; dir >= 0; dir--)
+                {
+                    launchSGSSweep64(c, gp, 1.0, ctx->iRhs, u, inc, tmp, ctx->iJd, ctx->wLam, dir, is->gsUpdate, 0,
+                                     sumDev);
+                    std::swap(inc, tmp);
+                }
+            launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, ctx->iFlag);
+        }
+        // fstop: residual of this iteration relative to its running maximum (density component)
+        DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, ctx->iRes, 5 * sizeof(real), hipMemcpyDeviceToHost, c.stream));
+        DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        for (int v = 0; v < 5; v++)
+        {
+            rr[v] = c.hostRed[v];
+            need(std::isfinite(rr[v]), "non-finite residual in the implicit step");
+            resBase[v] = std::max(resBase[v], rr[v]);
+            rel[v] = rr[v] / (resBase[v] + kVerySmall);
+        }
+        if (history)
+            history[it - 1] = rel[0];
+        if (it >= is->minIter && rel[0] < is->tol)
+            break;
+    }
+    if (it > is->maxIter)
+        it = is->maxIter;
+    if (rec != uRec)
+        DNDSR_HIP_CHECK(hipMemcpyAsync(uRec, rec, nR * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+    int flag = 0;
+    DNDSR_HIP_CHECK(hipMemcpyAsync(&flag, ctx->iFlag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    need(flag == 0, "an increment needed the positivity limiter (EvaluateCellRHSAlpha / CompressInc), which this "
+                    "path does not apply; lower the pseudo-time CFL");
+    if (nIter)
+        *nIter = it;
+    for (int v = 0; v < 5; v++)
+    {
+        if (res)
+            res[v] = rr[v];
+        if (resRel)
+            resRel[v] = rel[v];
+    }
     CAPI_END
 }
 

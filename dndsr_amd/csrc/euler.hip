@@ -746,4 +746,123 @@ void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, re
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Implicit dual-time step pieces (ImplicitEulerDualTimeStep::Step, ODE.hpp:80-98).
+// Unsteady residual rhs = xLast (1/dt) + x (-1/dt) + frhs(x), in the reference's
+// operation order, fused with the per-component L1 norm (EvaluateNorm with P = 1,
+// not volume-weighted, EulerEvaluator.hxx:1394-1424).  The partial sums of each
+// block land in distinct slots and are reduced once (deterministic).
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kDualResidual(idx N, real rdt, const real *__restrict__ xLast,
+                                                     const real *__restrict__ x, const real *__restrict__ rhsF,
+                                                     real *__restrict__ rhs, real *__restrict__ part)
+{
+    __shared__ real sm[5][256];
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    real loc[5] = {0, 0, 0, 0, 0};
+    if (i < N)
+    {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            const size_t k = (size_t)v * N + i;
+            real r = xLast[k] * rdt;
+            r += x[k] * (-rdt);
+            r += rhsF[k];
+            rhs[k] = r;
+            loc[v] = fabs(r);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = loc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
+}
+void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs, real *resDev)
+{
+    KernelTimer kt(c, K_DUAL_RES);
+    const int nb = gridFor(c.m.N, 256);
+    kDualResidual<<<nb, 256, 0, c.stream>>>(c.m.N, 1.0 / dt, xLast, x, rhsF, rhs, c.red);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    kReduceSum5<<<1, 256, 0, c.stream>>>(c.red, nb, resDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+// u += du after the positivity tests of the increment (EvaluateCellRHSAlpha,
+// EulerEvaluator.hxx:1964-2020, with relax = incrementPPRelax) and AddFixedIncrement /
+// CompressInc (EulerEvaluator.hpp:1645-1830).  When neither density nor pressure
+// would fall below its relaxed bound, both reduce to the plain sum; a cell that
+// would need the limiter sets *flag instead (the caller reports it as an error).
+template <typename T>
+__global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real relax, real rhoEps, real pEps,
+                                                     real *__restrict__ u, const T *__restrict__ du,
+                                                     int *__restrict__ flag)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+    real U[5], D[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        U[v] = u[(size_t)v * N + i], D[v] = (real)du[(size_t)v * N + i];
+    bool limited = false;
+    const real relaxedRho = rhoEps * relax + U[0] * (1 - relax);
+    if (D[0] < 0 && (U[0] - relaxedRho) / (-D[0] - kSmall * D[0]) < 1.0)
+        limited = true;
+    real Un[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        Un[v] = U[v] + D[v];
+    const real pNew = (Un[4] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) / Un[0]) * (gamma - 1);
+    const real pOld = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]) * (gamma - 1);
+    real relaxedP = pEps;
+    if (pNew < pOld)
+        relaxedP = pEps + (pOld - pEps) * (1 - relax);
+    if (pNew < relaxedP || !(Un[0] > 0))
+        limited = true;
+    if (limited)
+        atomicOr(flag, 1);
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        u[(size_t)v * N + i] = U[v] + D[v];
+}
+void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du, int *flagDev)
+{
+    KernelTimer kt(c, K_INCREMENT);
+    kAddIncrement<real><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, flagDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
+                          int *flagDev)
+{
+    KernelTimer kt(c, K_INCREMENT);
+    kAddIncrement<float><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, flagDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) kToF32(size_t n, const real *__restrict__ in, float *__restrict__ out)
+{
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        out[i] = (float)in[i];
+}
+void launchToF32(Ctx &c, size_t n, const real *in, float *out)
+{
+    KernelTimer kt(c, K_CONVERT);
+    kToF32<<<(unsigned)((n + 255) / 256), 256, 0, c.stream>>>(n, in, out);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
 } // namespace dndsr

@@ -122,7 +122,10 @@ enum KernelId : int
     K_AXPBY = 5,
     K_SGS = 6,
     K_LUSGS_INIT = 7,
-    K_NUM = 8,
+    K_DUAL_RES = 8,
+    K_INCREMENT = 9,
+    K_CONVERT = 10,
+    K_NUM = 11,
 };
 
 // Records a HIP event pair around the launches in its scope when timing is on.
@@ -178,6 +181,13 @@ void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float 
                       int gsUpdate, int uIncIsZero, real *sumIncDev);
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
                  const real *dtDev, real *out);
+void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs,
+                        real *resDev);
+void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du,
+                          int *flagDev);
+void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
+                          int *flagDev);
+void launchToF32(Ctx &c, size_t n, const real *in, float *out);
 
 template <typename T>
 T *devAlloc(Ctx &c, size_t n)

@@ -97,6 +97,23 @@ class EulerEvaluator:
             return res.cpu().numpy()
         return a.max(dim=1).values.cpu().numpy()
 
+    def ImplicitEulerStep(self, u, uRec, dt, CFL=10.0, nRec=1, sgsIter=0, jacobiCode=1, maxIter=20, minIter=5,
+                          tol=1e-10, fp32Sweeps=False, incrementPPRelax=0.9, refRho=1.0, refP=1.0):
+        """One implicit-Euler dual-time step on the device (ImplicitEulerDualTimeStep::Step,
+        ODE.hpp:80-98, with EulerSolver's fdtau / frhs / fsolve / fincrement / fstop,
+        EulerSolver.hxx:645-830, 1206).  Defaults follow EulerSolver's config defaults
+        (EulerSolver.hpp:171-760).  fp32Sweeps runs the LU-SGS sweeps in FP32 while the
+        residual, reconstruction and update stay FP64.  Returns (nIter, res, resRel, history)."""
+        s = _lib.ImplicitSettings(float(dt), float(CFL), int(nRec), int(sgsIter), int(jacobiCode), int(maxIter),
+                                  int(minIter), float(tol), int(bool(fp32Sweeps)), float(incrementPPRelax),
+                                  float(refRho), float(refP))
+        n = C.c_int()
+        res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(int(maxIter), 1))
+        check(lib().dndsr_implicit_euler_step(self.ctx.h, C.byref(self.es), C.byref(s), _ptr(u), _ptr(uRec),
+                                              C.byref(n), res.ctypes.data_as(_lib.dp), rel.ctypes.data_as(_lib.dp),
+                                              hist.ctypes.data_as(_lib.dp)))
+        return n.value, res, rel, hist[:n.value]
+
     def SSPRK3(self, u, uRec, nStep, CFL, nRec=1, dtFixed=0.0):
         """Device-resident explicit SSP-RK3 loop (ODE.hpp:1437-1535)."""
         check(lib().dndsr_ssprk3(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(uRec), int(nStep), float(CFL), int(nRec),

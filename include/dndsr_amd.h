@@ -158,15 +158,47 @@ int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float a
 int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, double *uRec, int nStep, double CFL,
                  int nRec, double dtFixed);
 
+/* Settings of the implicit dual-time step (EulerSolver config: timeMarchControl,
+ * implicitCFLControl, convergenceControl, linearSolverControl; src/Euler/EulerSolver.hpp:171-760). */
+typedef struct
+{
+    double dt;             /* physical time step (dtImplicit); 1e100 marches to a steady state */
+    double CFL;            /* pseudo-time CFL of the local dTau (implicitCFLControl.CFL) */
+    int nRec;              /* Jacobi VR sweeps per residual evaluation (nInternalRecStep) */
+    int sgsIter;           /* symmetric sweeps after the first one (linearSolverControl.sgsIter) */
+    int gsUpdate;          /* jacobiCode: 1 Gauss-Seidel, 0 Jacobi */
+    int maxIter;           /* nTimeStepInternal */
+    int minIter;           /* nTimeStepInternalMin */
+    double tol;            /* rhsThresholdInternal on the relative density residual */
+    int fp32Sweeps;        /* 1: LU-SGS sweeps in FP32 (residual, VR and update stay FP64) */
+    double incrementRelax; /* incrementPPRelax */
+    double refRho, refP;   /* reference density / pressure of the positivity epsilons (refUPrim) */
+} dndsr_implicit_settings;
+
+/* One implicit-Euler dual-time step, entirely on the device
+ * (ImplicitEulerDualTimeStep::Step, src/Solver/ODE.hpp:80-98, with the EulerSolver
+ * callbacks fdtau / frhs / fsolve / fincrement / fstop, src/Euler/EulerSolver.hxx:645-830,
+ * 1206; solveLinear / doPrecondition :1407-1658).  Per inner iteration: local dTau from
+ * EvaluateDt, nRec Jacobi VR sweeps, EvaluateRHS, the unsteady residual and its L1 norm,
+ * LUSGSMatrixInit, (1 + sgsIter) symmetric multicolour sweeps (FP64 or FP32) and the
+ * increment.  Stops when the density residual relative to its running maximum is below
+ * tol (after minIter iterations).  u and uRec are updated in place.  Outputs (host, may
+ * be NULL): nIter, res[5] (L1 residual of the last iteration), resRel[5], history[maxIter]
+ * (relative density residual per iteration).  Returns an error if a cell would need the
+ * positivity limiter of the increment. */
+int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
+                              double *u, double *uRec, int *nIter, double *res, double *resRel, double *history);
+
 /* ---- instrumentation (no counterpart in the reference, whose timers are the host-side
  * PerformanceTimer slots of src/DNDS/Profiling.hpp:22-49) ------------------------------
  * When enabled, every kernel launch is bracketed by a HIP event pair on its own stream.
  * dndsr_get_timing synchronises, adds the elapsed times of all launches recorded since the
  * last call per kernel id, and returns the running totals (ms) and launch counts:
  *   0 VR iteration, 1 RHS face pass, 2 RHS cell gather, 3 EvaluateDt face pass,
- *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init.
+ *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init,
+ *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion.
  * reset != 0 clears the totals after reading them. */
-#define DNDSR_NUM_KERNEL_IDS 8
+#define DNDSR_NUM_KERNEL_IDS 11
 int dndsr_set_timing(dndsr_ctx *ctx, int enable);
 int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset);
 

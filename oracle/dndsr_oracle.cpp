@@ -1811,4 +1811,107 @@ int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double 
     ORC_CATCH(-1)
 }
 
+// One implicit-Euler dual-time step (ImplicitEulerDualTimeStep::Step, src/Solver/ODE.hpp:80-98)
+// with the EulerSolver callbacks (src/Euler/EulerSolver.hxx): fdtau :674-694 (local dTau,
+// alphaDiag = 1, no dTau smoothing), frhs :645-672 / frhsOuter :147-235 (Jacobi VR sweeps,
+// recLinearScheme 0), fsolve :740-822 (LUSGSMatrixInit, zero increment, solveLinear
+// :1407-1500 -> doPrecondition :1605-1641: forward then backward UpdateSGS, repeated
+// 1 + sgsIter times), fincrement :696-738 (increment alpha = 1 when no positivity limit
+// is needed, then AddFixedIncrement), fstop -> functor_fstop (EulerSolver_Init.hxx:490-530:
+// L1 norm, running-maximum base, density component).
+int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
+                            double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
+                            double *resRel, double *history)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    size_t nU = (size_t)N * 5, nR = nU * v.M;
+    std::vector<real> dTau(N), lam((size_t)v.nFace * 4), dlc(N), rhsF(nU), rhs(nU), xLast(u, u + nU), tmp(nR),
+        jd(N), inc(nU), incNew(nU);
+    const real gamma = es->gamma;
+    const real rhoEps = smallReal * is->refRho * 1e-1, pEps = smallReal * is->refP * 1e-1;
+    real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
+    int it;
+    for (it = 1; it <= is->maxIter; it++)
+    {
+        real dtMin;
+        if (orc_eval_dt(h, es, u, is->CFL, 1e100, 1, dTau.data(), lam.data(), dlc.data(), &dtMin))
+            throw std::runtime_error(g_err);
+        for (int r = 0; r < is->nRec; r++)
+        {
+            if (orc_vr_iter(h, 5, uRec, u, tmp.data()))
+                throw std::runtime_error(g_err);
+            std::copy(tmp.begin(), tmp.end(), uRec);
+        }
+        if (orc_eval_rhs(h, es, u, uRec, dlc.data(), rhsF.data()))
+            throw std::runtime_error(g_err);
+        // crhs = xLast / dt - x / dt + frhs
+        for (size_t q = 0; q < nU; q++)
+        {
+            real r = xLast[q] * (1.0 / is->dt);
+            r += u[q] * (-1.0 / is->dt);
+            r += rhsF[q];
+            rhs[q] = r;
+        }
+        if (orc_lusgs_init(h, dTau.data(), is->dt, 1.0, lam.data(), jd.data()))
+            throw std::runtime_error(g_err);
+        std::fill(inc.begin(), inc.end(), 0.0);
+        real sumInc[5];
+        for (int s = 0; s <= is->sgsIter; s++)
+            for (int dir = 1; dir >= 0; dir--)
+            {
+                if (orc_update_sgs(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jd.data(), lam.data(), dir,
+                                   is->gsUpdate, 0, scanOrder, sumInc))
+                    throw std::runtime_error(g_err);
+                std::swap(inc, incNew);
+            }
+        // EvaluateCellRHSAlpha (EulerEvaluator.hxx:1964-2020): a limited cell is an error here
+        for (idx_t i = 0; i < N; i++)
+        {
+            const real *U = u + i * 5, *D = inc.data() + i * 5;
+            real relaxedRho = rhoEps * is->incrementRelax + U[0] * (1 - is->incrementRelax);
+            if (D[0] < 0 && std::min(1.0, (U[0] - relaxedRho) / (-D[0] - smallReal * D[0])) < 1.0)
+                throw std::runtime_error("positivity limiter needed (density)");
+            real Un[5];
+            for (int k = 0; k < 5; k++)
+                Un[k] = U[k] + D[k];
+            real pNew = (Un[4] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) / Un[0]) * (gamma - 1);
+            real pOld = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]) * (gamma - 1);
+            real relaxedP = pEps;
+            if (pNew < pOld)
+                relaxedP = pEps + (pOld - pEps) * (1 - is->incrementRelax);
+            if (pNew < relaxedP)
+                throw std::runtime_error("positivity limiter needed (pressure)");
+        }
+        for (size_t q = 0; q < nU; q++)
+            u[q] += inc[q];
+        for (int k = 0; k < 5; k++)
+            rr[k] = 0;
+        for (idx_t i = 0; i < N; i++)
+            for (int k = 0; k < 5; k++)
+                rr[k] += std::abs(rhs[i * 5 + k]);
+        for (int k = 0; k < 5; k++)
+        {
+            resBase[k] = std::max(resBase[k], rr[k]);
+            rel[k] = rr[k] / (resBase[k] + verySmallReal);
+        }
+        if (history)
+            history[it - 1] = rel[0];
+        if (it >= is->minIter && rel[0] < is->tol)
+            break;
+    }
+    if (nIter)
+        *nIter = std::min(it, is->maxIter);
+    for (int k = 0; k < 5; k++)
+    {
+        if (res)
+            res[k] = rr[k];
+        if (resRel)
+            resRel[k] = rel[k];
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
 } // extern "C"

@@ -115,6 +115,19 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
 int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,
                double CFL, int nRec, double dtFixed);
 
+/* implicit-Euler dual-time step (ImplicitEulerDualTimeStep::Step, src/Solver/ODE.hpp:80-98) */
+typedef struct
+{
+    double dt, CFL;
+    int nRec, sgsIter, gsUpdate, maxIter, minIter;
+    double tol;
+    int fp32Sweeps; /* ignored by the oracle (FP64 only) */
+    double incrementRelax, refRho, refP;
+} orc_implicit_settings;
+int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
+                            double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
+                            double *resRel, double *history);
+
 #ifdef __cplusplus
 }
 #endif

@@ -40,6 +40,13 @@ class EulerSettings(C.Structure):
                 ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double)]
 
 
+class ImplicitSettings(C.Structure):
+    _fields_ = [("dt", C.c_double), ("CFL", C.c_double), ("nRec", C.c_int), ("sgsIter", C.c_int),
+                ("gsUpdate", C.c_int), ("maxIter", C.c_int), ("minIter", C.c_int), ("tol", C.c_double),
+                ("fp32Sweeps", C.c_int), ("incrementRelax", C.c_double), ("refRho", C.c_double),
+                ("refP", C.c_double)]
+
+
 def build():
     subprocess.run(["make", "-s", "-C", _HERE], check=True)
 
@@ -57,7 +64,7 @@ def lib():
         _lib.orc_rec_error_l1.restype = C.c_double
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_iter_diff",
-                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3",
+                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step",
                      "orc_rec_error_l1", "orc_riemann"]:
             getattr(_lib, name).argtypes = None
     return _lib
@@ -226,3 +233,16 @@ class OracleVR:
         _chk(lib().orc_ssprk3(self.h, C.byref(es), P(u), P(uRec), C.c_int(nStep), C.c_double(CFL), C.c_int(nRec),
                               C.c_double(dtFixed)))
         return u, uRec
+
+    def implicit_euler_step(self, es, u, uRec, dt, CFL=10.0, nRec=1, sgsIter=0, gsUpdate=1, maxIter=20, minIter=5,
+                            tol=1e-10, incrementRelax=0.9, refRho=1.0, refP=1.0, scanOrder=None):
+        """Implicit-Euler dual-time step; returns (u, uRec, nIter, res, resRel, history)."""
+        u = np.array(u, dtype=np.float64, copy=True)
+        uRec = np.array(uRec, dtype=np.float64, copy=True)
+        st = ImplicitSettings(dt, CFL, nRec, sgsIter, gsUpdate, maxIter, minIter, tol, 0, incrementRelax, refRho, refP)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        n = C.c_int()
+        res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(maxIter, 1))
+        _chk(lib().orc_implicit_euler_step(self.h, C.byref(es), C.byref(st), P(u), P(uRec), P(so), C.byref(n), P(res),
+                                           P(rel), P(hist)))
+        return u, uRec, n.value, res, rel, hist[:n.value]
