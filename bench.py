@@ -68,7 +68,7 @@ def cpu_baseline(n=32, evals_min=3, budget_s=12.0):
     o.eval_rhs(es, u, rec, dlc)  # warm-up
     t0 = time.perf_counter()
     k = 0
-    while k < evals_min or (time.perf_counter() - t0 < budget_s and k < 200):
+    while k < evals_min or (time.perf_counter() - t0 < budget_s and k < 5000):
         rec = o.vr_iter(rec, u)
         o.eval_rhs(es, u, rec, dlc)
         k += 1
@@ -134,8 +134,9 @@ def main():
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     _lib.check(L_.dndsr_set_timing(vfv.ctx.h, 0))
-    ms = np.zeros(8)
-    cnt = np.zeros(8, np.int64)
+    nk = len(_lib.KERNEL_IDS)  # == DNDSR_NUM_KERNEL_IDS: dndsr_get_timing fills that many entries
+    ms = np.zeros(nk)
+    cnt = np.zeros(nk, np.int64)
     _lib.check(L_.dndsr_get_timing(vfv.ctx.h, ms.ctypes.data_as(_lib.dp), cnt.ctypes.data_as(_lib.i64p), 1))
     elapsed = t1 - t0
     ms_per_step = elapsed * 1e3 / max(args.steps, 1)

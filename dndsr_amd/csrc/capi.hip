@@ -297,6 +297,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.cent = devAlloc<real>(c, 3 * N);
     m.bary = devAlloc<real>(c, 3 * N);
     m.scale = devAlloc<real>(c, N);
+    m.iscale = devAlloc<real>(c, N);
     m.smooth = devAlloc<real>(c, N);
     m.moment = devAlloc<real>(c, (size_t)m.M * N);
     m.cpts = devAlloc<real>(c, (size_t)m.Gc * 3 * N);

@@ -116,6 +116,11 @@ __device__ __forceinline__ double bld(Rsrc r, uint32_t voff, uint32_t soff = 0)
 {
     return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 0));
 }
+// streaming read (non-temporal hint: the line is not kept in preference to reused data)
+__device__ __forceinline__ double bldnt(Rsrc r, uint32_t voff, uint32_t soff = 0)
+{
+    return __builtin_bit_cast(double, __builtin_amdgcn_raw_buffer_load_b64(r, voff, soff, 2));
+}
 __device__ __forceinline__ int32_t bldi(Rsrc r, uint32_t voff, uint32_t soff = 0)
 {
     return (int32_t)__builtin_amdgcn_raw_buffer_load_b32(r, voff, soff, 0);

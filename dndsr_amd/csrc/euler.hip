@@ -193,37 +193,91 @@ struct RHSRows
     static constexpr int TOTAL = FC + kFaceRows;
 };
 
-// Staged row q of the face tile: its device row (uniform per wave) and which
-// per-lane index (0: face, 1: left cell, 2: right cell) selects the element.
+// Staging of the face tile.  The row -> (array, row of the array, side) map is
+// resolved at compile time: one dispatch on the staging wave's index, then every
+// row is a constant, so a row costs a few scalar address operations and one
+// buffer load.  Loads go out in chunks of CH rows that are in flight together.
 template <int NDOF>
-__device__ inline const real *rhsStageRow(const DevMesh &m, const real *u, const real *uRec, int q, int &which)
+struct RHSStageCtx
+{
+    const DevMesh *m;
+    const real *u, *uRec;
+    real *lds;
+    int lane;
+    uint32_t fo, oL, oR; // byte offsets of the face and its two cells
+};
+
+template <int NDOF, int Q>
+__device__ __forceinline__ real rhsLoadRow(const RHSStageCtx<NDOF> &c)
 {
     using R = RHSRows<NDOF>;
     constexpr int M = R::M;
-    const size_t N = m.N, F = m.F;
-    if (q < R::REC)
+    const size_t N = c.m->N, F = c.m->F;
+    const uint64_t rowN = (uint64_t)N * 8;
+    if constexpr (Q < R::REC)
+        return bld(rowRsrc(c.u + (size_t)(Q % 5) * N, rowN), Q < 5 ? c.oL : c.oR);
+    else if constexpr (Q < R::MET)
     {
-        which = q < 5 ? 1 : 2;
-        return u + (size_t)(q % 5) * N;
+        constexpr int r = Q - R::REC, side = r / (5 * M), vj = r % (5 * M);
+        // without coefficients (first-order residual) the rows read as 0
+        return c.uRec ? bld(rowRsrc(c.uRec + (size_t)vj * N, rowN), side ? c.oR : c.oL)
+                      : bld(rowRsrc(c.u, rowN), kNoRow);
     }
-    if (q < R::MET)
+    else if constexpr (Q < R::SHIFT)
     {
-        const int r = q - R::REC, side = r / (5 * M), vj = r % (5 * M);
-        which = 1 + side;
-        return uRec + (size_t)vj * N;
+        constexpr int r = Q - R::MET, side = r / R::METW, k = r % R::METW;
+        const real *base = k < 3 ? c.m->cent + (size_t)k * N
+                                 : (k == 3 ? c.m->iscale : c.m->moment + (size_t)(k - 4) * N);
+        return bld(rowRsrc(base, rowN), side ? c.oR : c.oL);
     }
-    if (q < R::SHIFT)
+    else
+        return bld(rowRsrc(c.m->fshift + (size_t)(Q - R::SHIFT) * F, (uint64_t)F * 8), c.fo);
+}
+
+template <int NDOF, int W, int NS, int T0, int I, int CH>
+__device__ __forceinline__ void rhsLoadSeq(const RHSStageCtx<NDOF> &c, real *val)
+{
+    if constexpr (I < CH)
     {
-        const int r = q - R::MET, side = r / R::METW, k = r % R::METW;
-        which = 1 + side;
-        if (k < 3)
-            return m.cent + (size_t)k * N;
-        if (k == 3)
-            return m.scale;
-        return m.moment + (size_t)(k - 4) * N;
+        constexpr int Q = W + (T0 + I) * NS;
+        if constexpr (Q < RHSRows<NDOF>::STAGED)
+            val[I] = rhsLoadRow<NDOF, Q>(c);
+        rhsLoadSeq<NDOF, W, NS, T0, I + 1, CH>(c, val);
     }
-    which = 0;
-    return m.fshift + (size_t)(q - R::SHIFT) * F;
+}
+template <int NDOF, int W, int NS, int T0, int I, int CH>
+__device__ __forceinline__ void rhsStoreSeq(const RHSStageCtx<NDOF> &c, const real *val)
+{
+    if constexpr (I < CH)
+    {
+        constexpr int Q = W + (T0 + I) * NS;
+        if constexpr (Q < RHSRows<NDOF>::STAGED)
+            c.lds[Q * kTileF + c.lane] = val[I];
+        rhsStoreSeq<NDOF, W, NS, T0, I + 1, CH>(c, val);
+    }
+}
+template <int NDOF, int W, int NS, int T0>
+__device__ __forceinline__ void rhsStageChunks(const RHSStageCtx<NDOF> &c)
+{
+    constexpr int PER = (RHSRows<NDOF>::STAGED + NS - 1) / NS, CH = 12;
+    if constexpr (T0 < PER)
+    {
+        real val[CH];
+        rhsLoadSeq<NDOF, W, NS, T0, 0, CH>(c, val);
+        rhsStoreSeq<NDOF, W, NS, T0, 0, CH>(c, val);
+        rhsStageChunks<NDOF, W, NS, T0 + CH>(c);
+    }
+}
+template <int NDOF, int NS, int W>
+__device__ __forceinline__ void rhsStageDispatch(int w, const RHSStageCtx<NDOF> &c)
+{
+    if constexpr (W < NS)
+    {
+        if (w == W)
+            rhsStageChunks<NDOF, W, NS, 0>(c);
+        else
+            rhsStageDispatch<NDOF, NS, W + 1>(w, c);
+    }
 }
 
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
@@ -235,7 +289,6 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
     constexpr int M = R::M;
     // waves that stage rows: all but wave 0 when there are enough waves
     constexpr int S0 = G >= 4 ? 1 : 0, NS = G - S0;
-    constexpr int PER = (R::STAGED + NS - 1) / NS;
     extern __shared__ real lds[];
     const int lane = threadIdx.x % kTileF, g = waveIndex(kTileF);
     const size_t N = m.N, F = m.F;
@@ -247,46 +300,21 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
     const idx cL = bldi(rowRsrc(m.f2c, (uint64_t)F * 4), fo / 2);
     const idx cR = bldi(rowRsrc(m.f2c + F, (uint64_t)F * 4), fo / 2);
     const uint32_t oL = (uint32_t)cL * 8u, oR = (uint32_t)cR * 8u;
-    auto offOf = [&](int which) { return which == 0 ? fo : (which == 1 ? oL : oR); };
 
-    // this lane's point geometry (registers, used after the barrier)
+    // this lane's point geometry (registers, used after the barrier; read once, streamed)
     real x[3], n[3];
 #pragma unroll
     for (int r = 0; r < 3; r++)
     {
-        x[r] = bld(rowRsrc(m.fpts + ((size_t)g * 3 + r) * F, rowF), fo);
-        n[r] = bld(rowRsrc(m.fnorm + ((size_t)g * 3 + r) * F, rowF), fo);
+        x[r] = bldnt(rowRsrc(m.fpts + ((size_t)g * 3 + r) * F, rowF), fo);
+        n[r] = bldnt(rowRsrc(m.fnorm + ((size_t)g * 3 + r) * F, rowF), fo);
     }
-    const real wj = bld(rowRsrc(m.fwj + (size_t)g * F, rowF), fo);
+    const real wj = bldnt(rowRsrc(m.fwj + (size_t)g * F, rowF), fo);
 
     if (g >= S0)
     {
-        // chunks of CH rows per lane: the loads of a chunk are in flight together
-        constexpr int CH = PER < 12 ? PER : 12;
-#pragma unroll
-        for (int t0 = 0; t0 < PER; t0 += CH)
-        {
-            real val[CH];
-#pragma unroll
-            for (int tt = 0; tt < CH; tt++)
-            {
-                const int qq = (g - S0) + (t0 + tt) * NS;
-                const int q = qq < R::STAGED ? qq : R::STAGED - 1;
-                int which;
-                const real *row = rhsStageRow<NDOF>(m, u, uRec, q, which);
-                // without coefficients (first-order residual) the rows read as 0
-                const bool skip = q >= R::REC && q < R::MET && !uRec;
-                val[tt] = bld(rowRsrc(skip ? m.cent : row, which ? rowN : rowF), skip ? kNoRow : offOf(which));
-            }
-#pragma unroll
-            for (int tt = 0; tt < CH; tt++)
-            {
-                const int q = (g - S0) + (t0 + tt) * NS;
-                const bool isScale = q >= R::MET && q < R::SHIFT && (q - R::MET) % R::METW == 3;
-                if (t0 + tt < PER && q < R::STAGED)
-                    lds[q * kTileF + lane] = isScale ? 1.0 / val[tt] : val[tt];
-            }
-        }
+        const RHSStageCtx<NDOF> sc{&m, u, uRec, lds, lane, fo, oL, oR};
+        rhsStageDispatch<NDOF, NS, 0>(g - S0, sc);
     }
     if (g == 0)
     {

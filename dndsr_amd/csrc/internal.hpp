@@ -69,6 +69,7 @@ struct DevMesh
     real *faceL = nullptr;    // [F]
     real *faceW = nullptr;    // [P+1][F]
     real *vol = nullptr, *cent = nullptr, *bary = nullptr, *scale = nullptr, *smooth = nullptr;
+    real *iscale = nullptr;   // [N] 1 / scale
     real *moment = nullptr;   // [M][N]
     real *cpts = nullptr;     // [Gc][3][N]
     real *cwj = nullptr;      // [Gc][N]

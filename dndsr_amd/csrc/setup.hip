@@ -165,6 +165,7 @@ __global__ void __launch_bounds__(128) kCellMetrics(DevMesh m)
             h[r] = fmax(h[r], fabs(X[a][r] - b[r]));
     real s = DIM == 2 ? fmax(h[0], h[1]) : fmax(h[0], fmax(h[1], h[2]));
     m.scale[i] = s;
+    m.iscale[i] = 1.0 / s;
     // basis moments with the highest rule (VariationalReconstruction.cpp:86-108)
     real mm[NDOF];
     for (int k = 0; k < NDOF; k++)

@@ -88,7 +88,8 @@ kVRIter(DevMesh m, const real *__restrict__ uRec, real *__restrict__ uRecNew, co
         for (int t = 0; t < PER; t++)
             sNb[buf][w + t * RW][lane] = nb[t] - uOwn[t];
     };
-    // matrix rows of slot k (AinvB / Ainvb hold K slots for every cell)
+    // matrix rows of slot k (AinvB / Ainvb hold K slots for every cell), streamed once
+    // with the non-temporal hint so that they do not evict the neighbours' rows from L2
     auto loadB = [&](int k, real (*br)[M], real *bb) {
         const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N;
         const real *bv = m.Ainvb + ((size_t)k * M) * N;
@@ -99,8 +100,8 @@ kVRIter(DevMesh m, const real *__restrict__ uRec, real *__restrict__ uRecNew, co
             if constexpr (REC)
 #pragma unroll
                 for (int q = 0; q < M; q++)
-                    br[t][q] = bld(row(B + ((size_t)r * M + q) * N), ioff);
-            bb[t] = DIFF ? 0.0 : bld(row(bv + (size_t)r * N), ioff);
+                    br[t][q] = bldnt(row(B + ((size_t)r * M + q) * N), ioff);
+            bb[t] = DIFF ? 0.0 : bldnt(row(bv + (size_t)r * N), ioff);
         }
     };
     auto nbIndex = [&](int k) -> idx {

@@ -33,3 +33,11 @@ def test_error_path_without_context():
     L = _lib.lib()
     assert L.dndsr_sync(None) != 0
     assert b"null context" in L.dndsr_last_error()
+
+
+def test_kernel_id_count_matches_header():
+    """dndsr_get_timing writes DNDSR_NUM_KERNEL_IDS entries: the Python list sizes the buffers."""
+    from dndsr_amd import _lib
+    hdr = open(os.path.join(ROOT, "include", "dndsr_amd.h")).read()
+    n = int(re.search(r"#define\s+DNDSR_NUM_KERNEL_IDS\s+(\d+)", hdr).group(1))
+    assert len(_lib.KERNEL_IDS) == n
