@@ -78,6 +78,42 @@ def cpu_baseline(n=32, evals_min=3, budget_s=12.0):
                       f"with OpenMP face-buffer/cell-gather on {threads} threads, {dt:.1f} s"}
 
 
+def implicit_time_to_tol(vfv, ev, u0_dev, dt_mult=20.0, CFL=10.0, tol=1e-8, max_iter=400):
+    """Second half of the metric: wall time for one implicit-Euler dual-time step
+    (ImplicitEulerDualTimeStep::Step) to drive the relative density residual below tol,
+    with FP64 LU-SGS sweeps and with FP32 sweeps (residual, reconstruction and update in
+    FP64).  Same initial state and physical step for both; reports iterations, time and
+    the relative L2 difference of the two converged states."""
+    import torch
+    dtv = torch.zeros(vfv.nCell, dtype=torch.float64, device=u0_dev.device)
+    dtE = ev.EvaluateDt(dtv, u0_dev, None, 1.0, 1e100, False)
+    dt = dt_mult * dtE
+    out = {}
+    states = {}
+    for name, fp32 in (("fp64", False), ("mixed", True)):
+        best = None
+        for rep in range(2):  # first run warms allocations and kernels
+            u = u0_dev.clone()
+            rec = vfv.BuildURec(5)
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            n, res, rel, hist = ev.ImplicitEulerStep(u, rec, dt, CFL=CFL, maxIter=max_iter, tol=tol, fp32Sweeps=fp32,
+                                                     refRho=1.0, refP=1.0)
+            torch.cuda.synchronize()
+            ms = (time.perf_counter() - t0) * 1e3
+            if best is None or ms < best[0]:
+                best = (ms, n, float(rel[0]))
+        states[name] = vfv.to_host(u)
+        out[name] = {"ms": round(best[0], 3), "iterations": best[1], "resRel_rho": best[2],
+                     "ms_per_iteration": round(best[0] / best[1], 4)}
+    a, b = states["mixed"], states["fp64"]
+    out["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["speedup_mixed"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    out["settings"] = {"dt": dt, "dt_over_explicit_CFL1": dt_mult, "pseudo_CFL": CFL, "tol_rel_rho": tol,
+                       "nRec": 1, "sgsIter": 0, "jacobiCode": 1, "max_iter": max_iter}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,6 +121,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit time-to-tolerance measurement")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -171,6 +208,11 @@ def main():
                 "frac": round(droof["frac"], 4), "traffic": droof["traffic"], "kernel": dom,
                 "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5)}
 
+    implicit = None
+    if not args.no_implicit:
+        u0 = vfv.to_device(vfv.cell_average(lambda p: fields.taylor_green(p, 0.1, 1.4), 5))
+        implicit = implicit_time_to_tol(vfv, ev, u0)
+
     cpu = None
     if not args.no_cpu_baseline:
         cpu = cpu_baseline()
@@ -184,6 +226,7 @@ def main():
                                f"1 Jacobi VR sweep per RHS", "cells": N, "rhs_evals_per_step": 3},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "implicit": implicit,
         "kernels": kern,
         "roofline_all": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                          for k, r in roof.items()},

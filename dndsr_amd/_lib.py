@@ -74,12 +74,14 @@ SIGNATURES = {
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
     "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
                                   C.POINTER(C.c_int), dp, dp, dp],
+    "dndsr_eval_rhs_vjp": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp, vp, vp],
+    "dndsr_vr_iter_vjp": [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp],
     "dndsr_set_timing": [vp, C.c_int],
     "dndsr_get_timing": [vp, dp, i64p, C.c_int],
 }
 
 KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init",
-              "dual_residual", "increment", "to_f32"]
+              "dual_residual", "increment", "to_f32", "adjoint"]
 
 _lib = None
 

@@ -28,7 +28,9 @@ def _ptr(t):
 
 
 class Context:
-    """One device context (C-ABI ``dndsr_ctx``) bound to torch's current stream."""
+    """One device context (C-ABI ``dndsr_ctx``) bound to torch's current stream (also when
+    that is the default stream, handle 0), so library kernels and torch operations on the
+    same tensors are ordered without explicit synchronisation."""
 
     def __init__(self, device: int = 0):
         if not torch.cuda.is_available():

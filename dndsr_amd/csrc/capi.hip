@@ -134,7 +134,7 @@ int dndsr_set_stream(dndsr_ctx *ctx, void *stream)
 {
     CAPI_BEGIN
     need(ctx != nullptr, "null context");
-    ctx->c.stream = stream ? reinterpret_cast<hipStream_t>(stream) : ctx->own;
+    ctx->c.stream = reinterpret_cast<hipStream_t>(stream); // NULL: the device's default (null) stream
     CAPI_END
 }
 
@@ -225,14 +225,22 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
             hc2f[(size_t)k * N + i] = (idx)f;
         }
     }
-    // neighbour across each face slot (the VR sweep's only index)
+    // neighbour across each face slot (the VR sweep's only index) and the slot of the
+    // same face in that neighbour (the transposed sweep gathers through it)
     std::vector<idx> hc2c((size_t)K * N, -1);
+    std::vector<int32_t> hc2s((size_t)K * N, -1);
     for (size_t i = 0; i < N; i++)
         for (int k = 0; k < K; k++)
         {
             idx f = hc2f[(size_t)k * N + i];
-            if (f >= 0)
-                hc2c[(size_t)k * N + i] = hf2c[f] == (idx)i ? hf2c[F + f] : hf2c[f];
+            if (f < 0)
+                continue;
+            idx o = hf2c[f] == (idx)i ? hf2c[F + f] : hf2c[f];
+            hc2c[(size_t)k * N + i] = o;
+            for (int k2 = 0; k2 < K; k2++)
+                if (hc2f[(size_t)k2 * N + o] == f)
+                    hc2s[(size_t)k * N + i] = k2;
+            need(hc2s[(size_t)k * N + i] >= 0, "face missing from its neighbour's cell2face");
         }
     // greedy multicolouring in cell order (smallest colour unused by face neighbours)
     std::vector<int32_t> col(N, -1);
@@ -273,6 +281,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.f2c = devAlloc<idx>(c, hf2c.size());
     m.c2f = devAlloc<idx>(c, hc2f.size());
     m.c2c = devAlloc<idx>(c, hc2c.size());
+    m.c2s = devAlloc<int32_t>(c, hc2s.size());
     m.fshift = devAlloc<real>(c, hsh.size());
     m.color = devAlloc<int32_t>(c, N);
     m.colorCells = devAlloc<idx>(c, N);
@@ -282,6 +291,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     upload(c, m.f2c, hf2c);
     upload(c, m.c2f, hc2f);
     upload(c, m.c2c, hc2c);
+    upload(c, m.c2s, hc2s);
     upload(c, m.fshift, hsh);
     upload(c, m.color, col);
     upload(c, m.colorCells, cells);
@@ -841,6 +851,34 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         if (resRel)
             resRel[v] = rel[v];
     }
+    CAPI_END
+}
+
+int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                       const double *deltaLambdaCell, const double *rhsBar, double *uBar, double *uRecBar)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && u && rhsBar && uBar, "null field");
+    need(!uRecBar || uRec, "uRecBar needs uRec");
+    Ctx &c = ctx->c;
+    if (!c.adjPoint)
+        c.adjPoint = devAlloc<real>(c, (size_t)20 * c.m.G * c.m.F);
+    launchEvalRHSVJP(c, toGas(es), u, uRec, deltaLambdaCell, rhsBar, uBar, uRec ? uRecBar : nullptr);
+    CAPI_END
+}
+
+int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRecNewBar, double *uRecBar,
+                      double *uBar)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(uRecNewBar != nullptr, "null field");
+    need(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
+    need(mode == 3 || uBar != nullptr, "uBar is required for modes 0-2");
+    Ctx &c = ctx->c;
+    launchVRIterVJP(c, nVars, mode, relax < 0 ? c.m.s.jacobiRelax : relax, uRecNewBar, uRecBar,
+                    mode == 3 ? nullptr : uBar);
     CAPI_END
 }
 

@@ -27,31 +27,38 @@ enum RSKind : int
     RS_HLLEP_V1 = 21,
 };
 
-__host__ __device__ inline real sq(real x) { return x * x; }
-__host__ __device__ inline real dot3(const real *a, const real *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
-__host__ __device__ inline real signTol(real a, real tol) { return a > tol ? 1 : (a < -tol ? -1 : 0); }
-__host__ __device__ inline real signP(real a) { return a >= 0 ? 1 : -1; }
-__host__ __device__ inline real max3(real a, real b, real c) { return fmax(a, fmax(b, c)); }
+template <typename T>
+__host__ __device__ inline T sq(T x) { return x * x; }
+template <typename A, typename B>
+__host__ __device__ inline A dot3(const A *a, const B *b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
+template <typename T>
+__host__ __device__ inline real signTol(T a, T tol) { return a > tol ? 1 : (a < -tol ? -1 : 0); }
+template <typename T>
+__host__ __device__ inline real signP(T a) { return a >= 0 ? 1 : -1; }
+template <typename T>
+__host__ __device__ inline T max3(T a, T b, T c) { return fmax(a, fmax(b, c)); }
 
 // IdealGasThermal (src/DNDS/IdealGasPhysics.hpp:37-44).  Divisions by rho are
 // taken as products with one reciprocal (FP64 division costs ~10 VALU ops on
 // CDNA; the product differs from the quotient by at most an ulp).
-__host__ __device__ inline void thermalR(real E, real rho, real rinv, real vSqr, real gamma, real &p, real &asqr,
-                                         real &H)
+template <typename T>
+__host__ __device__ inline void thermalR(T E, T rho, T rinv, T vSqr, real gamma, T &p, T &asqr, T &H)
 {
     p = (gamma - 1) * (E - rho * 0.5 * vSqr);
     asqr = gamma * p * rinv;
     H = (E + p) * rinv;
 }
-__host__ __device__ inline void thermal(real E, real rho, real vSqr, real gamma, real &p, real &asqr, real &H)
+template <typename T>
+__host__ __device__ inline void thermal(T E, T rho, T vSqr, real gamma, T &p, T &asqr, T &H)
 {
-    thermalR(E, rho, 1.0 / rho, vSqr, gamma, p, asqr, H);
+    thermalR(E, rho, T(1.0) / rho, vSqr, gamma, p, asqr, H);
 }
 
 // Face-normal physical flux (GasInviscidFlux_XY, Gas.hpp:390-397)
-__host__ __device__ inline void physFlux(const real *U, const real *velo, const real *n, real p, real *F)
+template <typename T>
+__host__ __device__ inline void physFlux(const T *U, const T *velo, const real *n, T p, T *F)
 {
-    real vn = dot3(velo, n);
+    T vn = dot3(velo, n);
 #pragma unroll
     for (int i = 0; i < 5; i++)
         F[i] = U[i] * vn;
@@ -62,17 +69,19 @@ __host__ __device__ inline void physFlux(const real *U, const real *velo, const 
 }
 
 // Roe average of two mean states (ComputeRoePreamble, Gas.hpp:229-259)
-struct RoeAve
+template <typename T>
+struct RoeAveT
 {
-    real veloLm[3], veloRm[3], aLm, aRm, pLm, pRm;
-    real veloRoe[3], vsqrRoe, HRoe, asqrRoe, rhoRoe, aRoe;
-    real g1a2;  // (gamma - 1) / asqrRoe
-    real inv2a; // 1 / (2 aRoe)
+    T veloLm[3], veloRm[3], aLm, aRm, pLm, pRm;
+    T veloRoe[3], vsqrRoe, HRoe, asqrRoe, rhoRoe, aRoe;
+    T g1a2;  // (gamma - 1) / asqrRoe
+    T inv2a; // 1 / (2 aRoe)
 };
-__host__ __device__ inline void roeAverage(const real *ULm, const real *URm, real gamma, RoeAve &r)
+template <typename T>
+__host__ __device__ inline void roeAverage(const T *ULm, const T *URm, real gamma, RoeAveT<T> &r)
 {
-    real HLm, HRm, aLm2, aRm2;
-    const real rinvL = 1.0 / ULm[0], rinvR = 1.0 / URm[0];
+    T HLm, HRm, aLm2, aRm2;
+    const T rinvL = T(1.0) / ULm[0], rinvR = T(1.0) / URm[0];
 #pragma unroll
     for (int d = 0; d < 3; d++)
         r.veloLm[d] = ULm[1 + d] * rinvL, r.veloRm[d] = URm[1 + d] * rinvR;
@@ -80,8 +89,8 @@ __host__ __device__ inline void roeAverage(const real *ULm, const real *URm, rea
     thermalR(URm[4], URm[0], rinvR, dot3(r.veloRm, r.veloRm), gamma, r.pRm, aRm2, HRm);
     r.aLm = sqrt(aLm2);
     r.aRm = sqrt(aRm2);
-    real sL = sqrt(ULm[0]), sR = sqrt(URm[0]);
-    const real sInv = 1.0 / (sL + sR);
+    T sL = sqrt(ULm[0]), sR = sqrt(URm[0]);
+    const T sInv = T(1.0) / (sL + sR);
 #pragma unroll
     for (int d = 0; d < 3; d++)
         r.veloRoe[d] = (sL * r.veloLm[d] + sR * r.veloRm[d]) * sInv;
@@ -95,15 +104,15 @@ __host__ __device__ inline void roeAverage(const real *ULm, const real *URm, rea
 }
 
 // Entropy fixes (Roe_EntropyFixer, Gas.hpp:878-1007)
-template <int EIG>
-__host__ __device__ inline void roeFix(real aL, real aR, real aAve, real uL, real uR, real uAve, real VAve,
-                                       real dLambda, real fixScale, real incFScale, real &lam0, real &lam123, real &lam4)
+template <int EIG, typename T>
+__host__ __device__ inline void roeFix(T aL, T aR, T aAve, T uL, T uR, T uAve, T VAve, real dLambda, real fixScale,
+                                       real incFScale, T &lam0, T &lam123, T &lam4)
 {
     const real sHY = 0.05 * fixScale, sLD = 0.2 * fixScale, sHF = 0.25 * fixScale;
     if constexpr (EIG == 0)
     {
-        auto Flim = [=](real v, real lam, real lamL, real lamR) {
-            real th = max3(0., lam - lamL, lamR - lam) * sHY;
+        auto Flim = [=](T v, T lam, T lamL, T lamR) {
+            T th = max3(T(0.), lam - lamL, lamR - lam) * sHY;
             return v < th ? (sq(v) / th + th) * 0.5 : v;
         };
         lam0 = Flim(lam0, uAve - aAve, uL - aL, uR - aR);
@@ -112,29 +121,29 @@ __host__ __device__ inline void roeFix(real aL, real aR, real aAve, real uL, rea
     }
     else if constexpr (EIG == 1)
     {
-        real uLm = signTol(uL, aL * kSmall) * fmax(fabs(uL) * incFScale, aL * sLD);
-        real uRm = signTol(uR, aR * kSmall) * fmax(fabs(uR) * incFScale, aR * sLD);
+        T uLm = signTol(uL, aL * kSmall) * fmax(fabs(uL) * incFScale, aL * sLD);
+        T uRm = signTol(uR, aR * kSmall) * fmax(fabs(uR) * incFScale, aR * sLD);
         lam0 = fmax(fabs(uLm - aL), fabs(uRm - aR));
         lam123 = fmax(fabs(uLm), fabs(uRm));
         lam4 = fmax(fabs(uLm + aL), fabs(uRm + aR));
     }
     else if constexpr (EIG == 3)
     {
-        real aStar = fmin(fabs(uAve) / sLD, aAve);
+        T aStar = fmin(fabs(uAve) / sLD, aAve);
         lam0 = fabs(uAve * incFScale - aStar);
         lam4 = fabs(uAve * incFScale + aStar);
     }
     else if constexpr (EIG == 4)
     {
-        real uStar = signTol(uAve, aAve * kSmall) * fmax(aAve * sLD, fabs(uAve) * incFScale);
+        T uStar = signTol(uAve, aAve * kSmall) * fmax(aAve * sLD, fabs(uAve) * incFScale);
         lam0 = fabs(uStar - aAve);
         lam123 = fabs(uStar);
         lam4 = fabs(uStar + aAve);
     }
     else if constexpr (EIG == 5)
     {
-        real vL = uL * incFScale, vR = uR * incFScale;
-        real aLm = fmin(aL, fabs(vL) / sLD), aRm = fmin(aR, fabs(vR) / sLD);
+        T vL = uL * incFScale, vR = uR * incFScale;
+        T aLm = fmin(aL, fabs(vL) / sLD), aRm = fmin(aR, fabs(vR) / sLD);
         lam0 = fmax(fabs(vL - aLm), fabs(vR - aRm));
         lam123 = fmax(fabs(vL), fabs(vR));
         lam4 = fmax(fabs(vL + aLm), fabs(vR + aRm));
@@ -153,7 +162,7 @@ __host__ __device__ inline void roeFix(real aL, real aR, real aAve, real uL, rea
     }
     if constexpr (EIG == 7 || EIG == 8)
     {
-        real th = sHY * (VAve + aAve), th2 = sq(th);
+        T th = sHY * (VAve + aAve), th2 = sq(th);
         if (lam0 < th)
             lam0 = (sq(lam0) + th2) / (2 * th);
         if (lam4 < th)
@@ -165,22 +174,23 @@ __host__ __device__ inline void roeFix(real aL, real aR, real aAve, real uL, rea
 
 // Roe-eigenvector decomposition of a jump, |A| dU (Gas.hpp:1091-1125, 1209-1243).
 // vRN is the Roe normal velocity used in the eigenvectors; n the point normal.
-__host__ __device__ inline void roeDissipation(const real *dU, const real *n, const RoeAve &r, real vRN,
-                                               real lam0, real lam123, real lam4, real gamma, real *incF)
+template <typename T>
+__host__ __device__ inline void roeDissipation(const T *dU, const real *n, const RoeAveT<T> &r, T vRN, T lam0, T lam123,
+                                               T lam4, real gamma, T *incF)
 {
-    real dUn = dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2];
-    real a23[3];
+    T dUn = dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2];
+    T a23[3];
 #pragma unroll
     for (int d = 0; d < 3; d++)
         a23[d] = dU[1 + d] - dU[0] * r.veloRoe[d];
-    real a23n = dot3(a23, n);
+    T a23n = dot3(a23, n);
 #pragma unroll
     for (int d = 0; d < 3; d++)
         a23[d] -= n[d] * a23n;
-    real dU4b = dU[4] - dot3(a23, r.veloRoe);
-    real al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
-    real al0 = (dU[0] * (vRN + r.aRoe) - dUn - r.aRoe * al1) * r.inv2a;
-    real al4 = dU[0] - (al0 + al1);
+    T dU4b = dU[4] - dot3(a23, r.veloRoe);
+    T al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
+    T al0 = (dU[0] * (vRN + r.aRoe) - dUn - r.aRoe * al1) * r.inv2a;
+    T al4 = dU[0] - (al0 + al1);
     al0 *= lam0, al1 *= lam123, al4 *= lam4;
 #pragma unroll
     for (int d = 0; d < 3; d++)
@@ -193,11 +203,12 @@ __host__ __device__ inline void roeDissipation(const real *dU, const real *n, co
 }
 
 // Point states -> physical fluxes, pressures, velocities
-__host__ __device__ inline void sideFlux(const real *U, const real *n, real gamma, real *F, real &p, real &a)
+template <typename T>
+__host__ __device__ inline void sideFlux(const T *U, const real *n, real gamma, T *F, T &p, T &a)
 {
-    const real rinv = 1.0 / U[0];
-    real v[3] = {U[1] * rinv, U[2] * rinv, U[3] * rinv};
-    real a2, H;
+    const T rinv = T(1.0) / U[0];
+    T v[3] = {U[1] * rinv, U[2] * rinv, U[3] * rinv};
+    T a2, H;
     thermalR(U[4], U[0], rinv, dot3(v, v), gamma, p, a2, H);
     a = sqrt(a2);
     physFlux(U, v, n, p, F);
@@ -207,21 +218,20 @@ __host__ __device__ inline void sideFlux(const real *U, const real *n, real gamm
 // Returns lam0/123/4 (after entropy fix) like the reference.
 // Point part of the Riemann flux, given the face's Roe average of the mean states
 // (the average depends only on ULm / URm, so a face computes it once).
-template <int RS>
-__host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, const RoeAve &r, const real *n,
-                                             const GasParams &gp, real dLambda, real *F, real &lam0, real &lam123,
-                                             real &lam4)
+template <int RS, typename T>
+__host__ __device__ inline void riemannFluxR(const T *UL, const T *UR, const RoeAveT<T> &r, const real *n,
+                                             const GasParams &gp, real dLambda, T *F, T &lam0, T &lam123, T &lam4)
 {
     const real gamma = gp.gamma;
-    real FL[5], FR[5], pL, pR, aL, aR;
+    T FL[5], FR[5], pL, pR, aL, aR;
     sideFlux(UL, n, gamma, FL, pL, aL);
     sideFlux(UR, n, gamma, FR, pR, aR);
-    real vRN = dot3(r.veloRoe, n);
+    T vRN = dot3(r.veloRoe, n);
     lam0 = fabs(vRN - r.aRoe);
     lam123 = fabs(vRN);
     lam4 = fabs(vRN + r.aRoe);
-    real vLm0 = dot3(r.veloLm, n), vRm0 = dot3(r.veloRm, n);
-    real dU[5];
+    T vLm0 = dot3(r.veloLm, n), vRm0 = dot3(r.veloRm, n);
+    T dU[5];
 #pragma unroll
     for (int i = 0; i < 5; i++)
         dU[i] = UR[i] - UL[i];
@@ -237,7 +247,7 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
         constexpr int EIG = RS == RS_Roe ? 0 : RS - 10;
         roeFix<EIG>(r.aLm, r.aRm, r.aRoe, vLm0, vRm0, vRN, sqrt(r.vsqrRoe), dLambda, gp.fixScale, gp.incFScale,
                     lam0, lam123, lam4);
-        real incF[5];
+        T incF[5];
         roeDissipation(dU, n, r, vRN, lam0, lam123, lam4, gamma, incF);
 #pragma unroll
         for (int i = 0; i < 5; i++)
@@ -246,38 +256,38 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
     else if constexpr (RS == RS_HLLEP || RS == RS_HLLEP_V1)
     {
         // HLLEPFlux_IdealGas (Gas.hpp:601-714)
-        real dUn = dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2];
-        real a23[3];
+        T dUn = dU[1] * n[0] + dU[2] * n[1] + dU[3] * n[2];
+        T a23[3];
 #pragma unroll
         for (int d = 0; d < 3; d++)
             a23[d] = dU[1 + d] - dU[0] * r.veloRoe[d];
-        real a23n = dot3(a23, n);
+        T a23n = dot3(a23, n);
 #pragma unroll
         for (int d = 0; d < 3; d++)
             a23[d] -= n[d] * a23n;
-        real dU4b = dU[4] - dot3(a23, r.veloRoe);
-        real al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
-        real SL = fmin(vRN - r.aRoe, vLm0 - aL);
-        real SR = fmax(vRN + r.aRoe, vRm0 + aR);
-        real dfix = r.aRoe / (r.aRoe + fmax(fabs(vRN), dLambda * gp.fixScale));
-        real SP = fmax(SR, 0.0), SM = fmin(SL, 0.0);
+        T dU4b = dU[4] - dot3(a23, r.veloRoe);
+        T al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
+        T SL = fmin(vRN - r.aRoe, vLm0 - aL);
+        T SR = fmax(vRN + r.aRoe, vRm0 + aR);
+        T dfix = r.aRoe / (r.aRoe + fmax(fabs(vRN), dLambda * gp.fixScale));
+        T SP = fmax(SR, T(0.0)), SM = fmin(SL, T(0.0));
         if constexpr (RS == RS_HLLEP)
         {
-            real rev[5] = {1, r.veloRoe[0], r.veloRoe[1], r.veloRoe[2], r.vsqrRoe * 0.5};
-            real div = SP - SM;
+            T rev[5] = {T(1), r.veloRoe[0], r.veloRoe[1], r.veloRoe[2], r.vsqrRoe * 0.5};
+            T div = SP - SM;
             div += signP(div) * kVerySmall;
-            const real dinv = 1.0 / div, spsm = SP * SM * dinv, dal = dfix * al1;
+            const T dinv = T(1.0) / div, spsm = SP * SM * dinv, dal = dfix * al1;
 #pragma unroll
             for (int i = 0; i < 5; i++)
                 F[i] = (SP * FL[i] - SM * FR[i]) * dinv + spsm * (dU[i] - rev[i] * dal);
         }
         else
         {
-            const real dinv = 1.0 / (SP - SM);
+            const T dinv = T(1.0) / (SP - SM);
             lam123 = fabs(((SP + SM) * vRN - 2.0 * (1.0 - dfix) * (SP * SM)) * dinv);
             lam4 = fabs(((SP + SM) * (vRN + r.aRoe) - 2.0 * (SP * SM)) * dinv);
             lam0 = fabs(((SP + SM) * (vRN - r.aRoe) - 2.0 * (SP * SM)) * dinv);
-            real incF[5];
+            T incF[5];
             roeDissipation(dU, n, r, vRN, lam0, lam123, lam4, gamma, incF);
 #pragma unroll
             for (int i = 0; i < 5; i++)
@@ -287,10 +297,10 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
     else if constexpr (RS == RS_HLLC)
     {
         // HLLCFlux_IdealGas_HartenYee (Gas.hpp:733-843)
-        auto q = [&](real p, real pS) { return pS <= p ? 1.0 : sqrt(1 + (gamma + 1) / 2 / gamma * (pS / p - 1)); };
-        real pS = fmax(0.0, 0.5 * (r.pLm + r.pRm) - 0.5 * (vLm0 - vRm0) * r.rhoRoe * r.aRoe);
-        real SL = vLm0 - r.aLm * q(r.pLm, pS);
-        real SR = vRm0 + r.aRm * q(r.pRm, pS);
+        auto q = [&](T p, T pS) { return pS <= p ? T(1.0) : sqrt(T(1) + (gamma + 1) / 2 / gamma * (pS / p - T(1))); };
+        T pS = fmax(T(0.0), 0.5 * (r.pLm + r.pRm) - 0.5 * (vLm0 - vRm0) * r.rhoRoe * r.aRoe);
+        T SL = vLm0 - r.aLm * q(r.pLm, pS);
+        T SR = vRm0 + r.aRm * q(r.pRm, pS);
         if (0 <= SL)
         {
 #pragma unroll
@@ -305,18 +315,18 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
         }
         else
         {
-            real SS = 0;
-            real div = UL[0] * (SL - vLm0) - UR[0] * (SR - vRm0);
+            T SS = 0;
+            T div = UL[0] * (SL - vLm0) - UR[0] * (SR - vRm0);
             if (fabs(div) > kVerySmall)
                 SS = (pR - pL + (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) * (SL - vLm0) -
                       (UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2]) * (SR - vRm0)) /
                      div;
-            real DS[5] = {0, n[0], n[1], n[2], SS};
+            T DS[5] = {T(0), T(n[0]), T(n[1]), T(n[2]), SS};
             bool left = SS >= 0;
-            const real *U = left ? UL : UR;
-            const real *Fs = left ? FL : FR;
-            real S = left ? SL : SR, v0 = left ? vLm0 : vRm0, p = left ? pL : pR;
-            real dv = S - SS;
+            const T *U = left ? UL : UR;
+            const T *Fs = left ? FL : FR;
+            T S = left ? SL : SR, v0 = left ? vLm0 : vRm0, p = left ? pL : pR;
+            T dv = S - SS;
             if (fabs(dv) < kVerySmall)
             {
 #pragma unroll
@@ -333,12 +343,11 @@ __host__ __device__ inline void riemannFluxR(const real *UL, const real *UR, con
     }
 }
 
-template <int RS>
-__host__ __device__ inline void riemannFlux(const real *UL, const real *UR, const real *ULm, const real *URm,
-                                            const real *n, const GasParams &gp, real dLambda, real *F,
-                                            real &lam0, real &lam123, real &lam4)
+template <int RS, typename T>
+__host__ __device__ inline void riemannFlux(const T *UL, const T *UR, const T *ULm, const T *URm, const real *n,
+                                            const GasParams &gp, real dLambda, T *F, T &lam0, T &lam123, T &lam4)
 {
-    RoeAve r;
+    RoeAveT<T> r;
     roeAverage(ULm, URm, gp.gamma, r);
     riemannFluxR<RS>(UL, UR, r, n, gp, dLambda, F, lam0, lam123, lam4);
 }
@@ -346,33 +355,34 @@ __host__ __device__ inline void riemannFlux(const real *UL, const real *UR, cons
 // Batched Roe flux: eigenvalues from the cell-mean Roe average on the mean face
 // normal, shared across the face points (RoeFlux_IdealGas_HartenYee_Batch,
 // Gas.hpp:1272-1407).  `r`, `vRNm` and the fixed lam are computed once per face.
-struct RoeBatchFace
+template <typename T>
+struct RoeBatchFaceT
 {
-    RoeAve r;
-    real vRN; // Roe velocity on the mean normal
-    real lam0, lam123, lam4;
+    RoeAveT<T> r;
+    T vRN; // Roe velocity on the mean normal
+    T lam0, lam123, lam4;
 };
-template <int RS>
-__host__ __device__ inline void roeBatchSetup(const real *ULm, const real *URm, const real *nm, const GasParams &gp,
-                                              real dLambda, RoeBatchFace &b)
+template <int RS, typename T>
+__host__ __device__ inline void roeBatchSetup(const T *ULm, const T *URm, const real *nm, const GasParams &gp,
+                                              real dLambda, RoeBatchFaceT<T> &b)
 {
     roeAverage(ULm, URm, gp.gamma, b.r);
     b.vRN = dot3(b.r.veloRoe, nm);
     b.lam0 = fabs(b.vRN - b.r.aRoe);
     b.lam123 = fabs(b.vRN);
     b.lam4 = fabs(b.vRN + b.r.aRoe);
-    real vLm0 = dot3(b.r.veloLm, nm), vRm0 = dot3(b.r.veloRm, nm);
+    T vLm0 = dot3(b.r.veloLm, nm), vRm0 = dot3(b.r.veloRm, nm);
     if constexpr (RS == RS_Roe_M2)
         b.lam0 = b.lam123 = b.lam4 = fmax(fabs(vLm0) + b.r.aLm, fabs(vRm0) + b.r.aRm);
     else
         roeFix<RS - 10>(b.r.aLm, b.r.aRm, b.r.aRoe, vLm0, vRm0, b.vRN, sqrt(b.r.vsqrRoe), dLambda, gp.fixScale,
                         gp.incFScale, b.lam0, b.lam123, b.lam4);
 }
-template <int RS>
-__host__ __device__ inline void roeBatchPoint(const real *UL, const real *UR, const real *n, const GasParams &gp,
-                                              const RoeBatchFace &b, real *F)
+template <int RS, typename T>
+__host__ __device__ inline void roeBatchPoint(const T *UL, const T *UR, const real *n, const GasParams &gp,
+                                              const RoeBatchFaceT<T> &b, T *F)
 {
-    real FL[5], FR[5], pL, pR, aL, aR, dU[5];
+    T FL[5], FR[5], pL, pR, aL, aR, dU[5];
     sideFlux(UL, n, gp.gamma, FL, pL, aL);
     sideFlux(UR, n, gp.gamma, FR, pR, aR);
 #pragma unroll
@@ -386,13 +396,16 @@ __host__ __device__ inline void roeBatchPoint(const real *UL, const real *UR, co
     }
     else
     {
-        real incF[5];
+        T incF[5];
         roeDissipation(dU, n, b.r, b.vRN, b.lam0, b.lam123, b.lam4, gp.gamma, incF);
 #pragma unroll
         for (int i = 0; i < 5; i++)
             F[i] = (FL[i] + FR[i]) * 0.5 - 0.5 * incF[i];
     }
 }
+
+using RoeAve = RoeAveT<real>;
+using RoeBatchFace = RoeBatchFaceT<real>;
 
 // Scalar-Jacobian flux increment dF = A(U) dU . n - lambda dU
 // (fluxJacobian0_Right_Times_du, EulerEvaluator.hpp:1100-1158; IdealGasUIncrement

@@ -55,6 +55,7 @@ struct DevMesh
     idx *f2c = nullptr;     // [2][F], -1 = boundary
     idx *c2f = nullptr;     // [K][N], -1 = absent
     idx *c2c = nullptr;     // [K][N] cell across face slot k, -1 = absent
+    int32_t *c2s = nullptr; // [K][N] slot of the same face in that neighbour cell
     idx *cell2node = nullptr; // [nnc][N]
     idx *face2node = nullptr; // [nnf][F]
     real *coords = nullptr;   // [3][nNode]
@@ -97,6 +98,7 @@ struct Ctx
     size_t redCap = 0;
     real *hostRed = nullptr;    // pinned host mirror of reductions
     real *faceLamTmp = nullptr; // [2][F] (lambdaFace, deltaLambdaFace)
+    real *adjPoint = nullptr;   // [20][G F] point sensitivities of the RHS adjoint
     size_t faceLamCap = 0;
     std::vector<void *> owned;  // device allocations freed at destroy
     // optional per-kernel timing with HIP events on the launch stream
@@ -126,7 +128,8 @@ enum KernelId : int
     K_DUAL_RES = 8,
     K_INCREMENT = 9,
     K_CONVERT = 10,
-    K_NUM = 11,
+    K_ADJOINT = 11,
+    K_NUM = 12,
 };
 
 // Records a HIP event pair around the launches in its scope when timing is on.
@@ -189,6 +192,10 @@ void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
                           int *flagDev);
 void launchToF32(Ctx &c, size_t n, const real *in, float *out);
+// ---- launchers (adjoint.hip)
+void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
+                      const real *rbar, real *ubar, real *urecbar);
+void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *ybar, real *xbar, real *ubar);
 
 template <typename T>
 T *devAlloc(Ctx &c, size_t n)

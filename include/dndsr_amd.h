@@ -85,7 +85,8 @@ int dndsr_destroy(dndsr_ctx *ctx);
 const char *dndsr_last_error(void);
 int dndsr_sync(dndsr_ctx *ctx);
 /* enqueue on the caller's stream (e.g. the framework's current stream) instead of the
- * context's own; pass NULL to return to the context stream */
+ * context's own one, so that the library's work is ordered with the caller's; NULL selects
+ * the device's default (null) stream */
 int dndsr_set_stream(dndsr_ctx *ctx, void *hipStream);
 
 /* ---- setup ------------------------------------------------------------ */
@@ -189,6 +190,20 @@ typedef struct
 int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
                               double *u, double *uRec, int *nIter, double *res, double *resRel, double *history);
 
+/* ---- discrete adjoint (no counterpart in the reference; checked against central finite
+ * differences of the forward operators) ----------------------------------------------
+ * Vector-Jacobian product of EvaluateRHS (dndsr_eval_rhs) with deltaLambdaCell held fixed:
+ *   uBar = (d rhs / d u)^T rhsBar, uRecBar = (d rhs / d uRec)^T rhsBar (overwritten; uRecBar may
+ *   be NULL, or must be when uRec is NULL).  Riemann solvers: all of dndsr_eval_rhs's;
+ *   maxOrder 1 and 2. */
+int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                       const double *deltaLambdaCell, const double *rhsBar, double *uBar, double *uRecBar);
+/* Transpose of one VR sweep (dndsr_vr_iter modes 0-2, mode 3 = dndsr_vr_iter_diff):
+ *   uRecBar = (d uRecNew / d uRec)^T uRecNewBar, uBar = (d uRecNew / d u)^T uRecNewBar
+ * (uBar unused for mode 3; uRecBar may be NULL).  relax < 0 uses settings.jacobiRelax. */
+int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRecNewBar, double *uRecBar,
+                      double *uBar);
+
 /* ---- instrumentation (no counterpart in the reference, whose timers are the host-side
  * PerformanceTimer slots of src/DNDS/Profiling.hpp:22-49) ------------------------------
  * When enabled, every kernel launch is bracketed by a HIP event pair on its own stream.
@@ -196,9 +211,9 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
  * last call per kernel id, and returns the running totals (ms) and launch counts:
  *   0 VR iteration, 1 RHS face pass, 2 RHS cell gather, 3 EvaluateDt face pass,
  *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init,
- *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion.
+ *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion, 11 adjoint kernels.
  * reset != 0 clears the totals after reading them. */
-#define DNDSR_NUM_KERNEL_IDS 11
+#define DNDSR_NUM_KERNEL_IDS 12
 int dndsr_set_timing(dndsr_ctx *ctx, int enable);
 int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset);
 
