@@ -285,6 +285,23 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.fshift = devAlloc<real>(c, hsh.size());
     m.color = devAlloc<int32_t>(c, N);
     m.colorCells = devAlloc<idx>(c, N);
+    {
+        std::vector<uint8_t> hfl((size_t)K * N, 0);
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < K; k++)
+            {
+                idx f = hc2f[(size_t)k * N + i];
+                if (f < 0)
+                    continue;
+                idx o = hc2c[(size_t)k * N + i];
+                uint8_t fl = hf2c[f] == (idx)i ? 0 : 1;
+                if (o >= 0 && col[o] < col[i])
+                    fl |= 2;
+                hfl[(size_t)k * N + i] = fl;
+            }
+        m.c2flag = devAlloc<uint8_t>(c, hfl.size());
+        upload(c, m.c2flag, hfl);
+    }
     upload(c, m.coords, hc);
     upload(c, m.cell2node, hcn);
     upload(c, m.face2node, hfn);
@@ -321,6 +338,13 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
 
     setupMetricsAndVR(c);
+    // FP32 copies of the face and cell geometry read by the FP32 Gauss-Seidel sweeps
+    m.fareaF = devAlloc<float>(c, F);
+    m.fmeanNF = devAlloc<float>(c, 3 * F);
+    m.volF = devAlloc<float>(c, N);
+    launchToF32(c, F, m.farea, m.fareaF);
+    launchToF32(c, 3 * F, m.fmeanN, m.fmeanNF);
+    launchToF32(c, N, m.vol, m.volF);
     auto vol = download(c, m.vol, N);
     m.globalVol = 0;
     for (real v : vol)

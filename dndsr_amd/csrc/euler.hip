@@ -608,6 +608,26 @@ void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const re
 // written to distinct slots and reduced once at the end (deterministic).
 // ---------------------------------------------------------------------------
 template <typename T>
+struct SGSGeom;
+template <>
+struct SGSGeom<real>
+{
+    static __device__ const real *area(const DevMesh &m) { return m.farea; }
+    static __device__ const real *norm(const DevMesh &m) { return m.fmeanN; }
+    static __device__ const real *vol(const DevMesh &m) { return m.vol; }
+};
+template <>
+struct SGSGeom<float>
+{
+    static __device__ const float *area(const DevMesh &m) { return m.fareaF; }
+    static __device__ const float *norm(const DevMesh &m) { return m.fmeanNF; }
+    static __device__ const float *vol(const DevMesh &m) { return m.volF; }
+};
+
+// One thread per cell of the colour.  Per face slot the face index, the neighbour
+// (c2c) and the slot flags (side, colour order) are independent loads; the face's
+// geometry and the neighbour's state follow in one round.
+template <typename T>
 __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ rhs,
                                                   const T *__restrict__ u, const T *__restrict__ uInc,
                                                   T *__restrict__ uIncNew, const T *__restrict__ jd,
@@ -620,25 +640,25 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
     if (t < end)
     {
         const size_t N = m.N, F = m.F;
-        idx i = m.colorCells[t];
+        const idx i = m.colorCells[t];
         const T gamma = (T)gammaD;
+        const T *__restrict__ farea = SGSGeom<T>::area(m);
+        const T *__restrict__ fnorm = SGSGeom<T>::norm(m);
         T buf[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
             buf[v] = rhs[v * N + i];
-        T voli = (T)m.vol[i];
+        const T voli = SGSGeom<T>::vol(m)[i];
         for (int k = 0; k < m.K; k++)
         {
-            idx f = m.c2f[(size_t)k * N + i];
-            if (f < 0)
+            const idx f = m.c2f[(size_t)k * N + i];
+            const idx o = m.c2c[(size_t)k * N + i];
+            const uint8_t fl = m.c2flag[(size_t)k * N + i];
+            if (f < 0 || o < 0 || o == i)
                 continue;
-            idx L = m.f2c[f], R = m.f2c[F + f];
-            bool back = L == i;
-            idx o = back ? R : L;
-            if (o < 0 || o == i)
-                continue;
-            int co = m.color[o];
-            bool useNew = gsUpdate && (forward ? co < col : co > col);
+            // the neighbour precedes the cell in this sweep: forward = lower colour
+            const bool earlier = forward ? (fl & 2) != 0 : (fl & 2) == 0;
+            const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
             const T *du = useNew ? uIncNew : uInc;
@@ -646,12 +666,12 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
 #pragma unroll
             for (int v = 0; v < 5; v++)
                 U[v] = u[v * N + o], dU[v] = du[v * N + o];
-            T sgn = back ? T(1) : T(-1);
+            const T sgn = (fl & 1) ? T(-1) : T(1);
 #pragma unroll
             for (int r = 0; r < 3; r++)
-                n[r] = (T)m.fmeanN[r * F + f] * sgn;
+                n[r] = fnorm[r * F + f] * sgn;
             fluxJacDu<T>(U, n, dU, lam[f], gamma, dF);
-            T coef = (T(0.5) * alphaDiag) * (T)m.farea[f] / voli;
+            T coef = (T(0.5) * alphaDiag) * farea[f] / voli;
 #pragma unroll
             for (int v = 0; v < 5; v++)
                 buf[v] -= coef * dF[v];

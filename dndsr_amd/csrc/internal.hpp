@@ -56,6 +56,9 @@ struct DevMesh
     idx *c2f = nullptr;     // [K][N], -1 = absent
     idx *c2c = nullptr;     // [K][N] cell across face slot k, -1 = absent
     int32_t *c2s = nullptr; // [K][N] slot of the same face in that neighbour cell
+    uint8_t *c2flag = nullptr; // [K][N] bit 0: the cell is the face's front (R) side;
+                               //        bit 1: the neighbour's colour precedes the cell's
+    float *fareaF = nullptr, *fmeanNF = nullptr, *volF = nullptr; // FP32 copies for the FP32 sweeps
     idx *cell2node = nullptr; // [nnc][N]
     idx *face2node = nullptr; // [nnf][F]
     real *coords = nullptr;   // [3][nNode]
