@@ -53,6 +53,25 @@ def build(verbose: bool = False, jobs: int = 4) -> str:
     return LIB
 
 
+MIRROR_TEST_SRC = os.path.join(HERE, "..", "tests", "cpp", "host_mirror_main.cpp")
+MIRROR_TEST_BIN = os.path.join(HERE, "_bin", "host_mirror_test")
+
+
+def build_host_mirror_test() -> str:
+    """Compile the C++17 host-mirror test driver (include/dndsr_amd.hpp) against the library."""
+    os.makedirs(os.path.dirname(MIRROR_TEST_BIN), exist_ok=True)
+    rocm = os.path.dirname(os.path.dirname(os.path.realpath(hipcc())))
+    cmd = [shutil.which("g++") or "g++", "-O2", "-std=c++17", "-Wall", "-D__HIP_PLATFORM_AMD__",
+           "-I", os.path.join(HERE, "..", "include"), "-I", os.path.join(rocm, "include"), MIRROR_TEST_SRC,
+           "-o", MIRROR_TEST_BIN + ".tmp", "-L", HERE, "-ldndsr_amd", "-L", os.path.join(rocm, "lib"), "-lamdhip64",
+           "-Wl,-rpath,$ORIGIN/..", "-Wl,-rpath," + os.path.join(rocm, "lib")]
+    r = subprocess.run(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if r.returncode != 0:
+        raise RuntimeError("host mirror test driver failed to build:\n" + r.stdout)
+    os.replace(MIRROR_TEST_BIN + ".tmp", MIRROR_TEST_BIN)
+    return MIRROR_TEST_BIN
+
+
 def _wait(procs):
     while procs:
         src, p = procs.pop(0)
@@ -63,3 +82,4 @@ def _wait(procs):
 
 if __name__ == "__main__":
     print(build(verbose="-v" in sys.argv))
+    print(build_host_mirror_test())

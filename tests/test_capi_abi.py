@@ -41,3 +41,13 @@ def test_kernel_id_count_matches_header():
     hdr = open(os.path.join(ROOT, "include", "dndsr_amd.h")).read()
     n = int(re.search(r"#define\s+DNDSR_NUM_KERNEL_IDS\s+(\d+)", hdr).group(1))
     assert len(_lib.KERNEL_IDS) == n
+
+
+def test_cpp_host_mirror_driver_is_built():
+    """The C++ host-mirror driver links against the library and runs its argument check
+    without touching the GPU."""
+    import subprocess
+    from dndsr_amd import build as b
+    assert os.path.exists(b.MIRROR_TEST_BIN)
+    r = subprocess.run([b.MIRROR_TEST_BIN], capture_output=True, text=True, timeout=60)
+    assert r.returncode == 2 and "usage" in r.stderr
