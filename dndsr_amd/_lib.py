@@ -41,7 +41,7 @@ class ImplicitSettings(C.Structure):
     _fields_ = [("dt", C.c_double), ("CFL", C.c_double), ("nRec", C.c_int), ("sgsIter", C.c_int),
                 ("gsUpdate", C.c_int), ("maxIter", C.c_int), ("minIter", C.c_int), ("tol", C.c_double),
                 ("fp32Sweeps", C.c_int), ("incrementRelax", C.c_double), ("refRho", C.c_double),
-                ("refP", C.c_double)]
+                ("refP", C.c_double), ("gmresCode", C.c_int), ("nGmresSpace", C.c_int), ("nGmresIter", C.c_int)]
 
 
 # Every symbol of include/dndsr_amd.h with its argument types.
@@ -74,6 +74,9 @@ SIGNATURES = {
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
     "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
                                   C.POINTER(C.c_int), dp, dp, dp],
+    "dndsr_lusgs_matvec": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp],
+    "dndsr_gmres_lusgs": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
+                          C.c_int, C.c_int, dp],
     "dndsr_eval_rhs_vjp": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp, vp, vp],
     "dndsr_vr_iter_vjp": [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp],
     "dndsr_set_timing": [vp, C.c_int],
@@ -81,7 +84,7 @@ SIGNATURES = {
 }
 
 KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init",
-              "dual_residual", "increment", "to_f32", "adjoint"]
+              "dual_residual", "increment", "to_f32", "adjoint", "linsolve"]
 
 _lib = None
 

@@ -28,7 +28,45 @@ struct dndsr_ctx
     float *fRhs = nullptr, *fU = nullptr, *fJd = nullptr, *fLam = nullptr, *fInc = nullptr, *fTmp = nullptr;
     int *iFlag = nullptr;
     bool haveImplicit = false;
+    // linear-solver workspace (GMRES Krylov basis for up to linK vectors + temporaries)
+    LinWork lin;
+    int linK = -1;
+    bool linF32 = false;
 };
+
+// workspace for GMRES(k) and the LU-SGS preconditioner; FP32 buffers only when needed
+static LinWork &linWork(dndsr_ctx *ctx, int k, bool fp32)
+{
+    Ctx &c = ctx->c;
+    const size_t N = c.m.N, nU = 5 * N;
+    LinWork &w = ctx->lin;
+    if (ctx->linK < 0)
+    {
+        w.dT = devAlloc<real>(c, nU);
+        w.Vt = devAlloc<real>(c, nU);
+        w.MLb = devAlloc<real>(c, nU);
+        w.part = devAlloc<real>(c, (size_t)256 * 64);
+        w.scal = devAlloc<real>(c, 8);
+        w.coef = devAlloc<real>(c, 64);
+        ctx->linK = 0;
+    }
+    if (k > ctx->linK)
+    {
+        if (k > 60)
+            throw std::runtime_error("GMRES subspace size must be at most 60");
+        w.V = devAlloc<real>(c, (size_t)(k + 1) * nU);
+        w.h = devAlloc<real>(c, (size_t)(k + 1) * k);
+        ctx->linK = k;
+    }
+    if (fp32 && !ctx->linF32)
+    {
+        w.fR = devAlloc<float>(c, nU);
+        w.fX = devAlloc<float>(c, nU);
+        w.fT = devAlloc<float>(c, nU);
+        ctx->linF32 = true;
+    }
+    return w;
+}
 
 static thread_local std::string g_err;
 
@@ -753,6 +791,8 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     need(es && is && u && uRec, "null argument");
     need(is->dt > 0 && is->CFL > 0 && is->nRec >= 0 && is->sgsIter >= 0 && is->maxIter >= 1 && is->minIter >= 0,
          "bad implicit settings");
+    need(is->gmresCode >= 0 && is->gmresCode <= 2 && (is->gmresCode == 0 || (is->nGmresSpace >= 1 && is->nGmresIter >= 0)),
+         "bad GMRES settings");
     Ctx &c = ctx->c;
     DevMesh &m = c.m;
     GasParams gp = toGas(es);
@@ -780,9 +820,12 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         ctx->iTmp = devAlloc<real>(c, nU);
         ctx->iRes = devAlloc<real>(c, 8);
         ctx->fRhs = devAlloc<float>(c, nU);
-        ctx->fU = devAlloc<float>(c, nU);
-        ctx->fJd = devAlloc<float>(c, N);
-        ctx->fLam = devAlloc<float>(c, 4 * F);
+        if (!ctx->fU)
+            ctx->fU = devAlloc<float>(c, nU);
+        if (!ctx->fJd)
+            ctx->fJd = devAlloc<float>(c, N);
+        if (!ctx->fLam)
+            ctx->fLam = devAlloc<float>(c, 4 * F);
         ctx->fInc = devAlloc<float>(c, nU);
         ctx->fTmp = devAlloc<float>(c, nU);
         ctx->iFlag = devAlloc<int>(c, 1);
@@ -810,7 +853,28 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
         // fsolve: LUSGSMatrixInit + (1 + sgsIter) symmetric sweeps from a zero increment
         launchLUSGSInit(c, ctx->iDTau, is->dt, 1.0, ctx->wLam, ctx->iJd);
-        if (is->fp32Sweeps)
+        if (is->gmresCode != 0)
+        {
+            // solveLinear with gmresCode 1 (GMRES from zero) or 2 (LU-SGS result as the start),
+            // EulerSolver.hxx:1407-1578; the preconditioner sweeps in FP32 when fp32Sweeps
+            LinWork &w = linWork(ctx, is->nGmresSpace, is->fp32Sweeps != 0);
+            w.u = u, w.jd = ctx->iJd, w.lam = ctx->wLam, w.alphaDiag = 1.0;
+            w.sgsIter = is->sgsIter, w.gsUpdate = is->gsUpdate, w.fp32 = is->fp32Sweeps != 0;
+            if (w.fp32)
+            {
+                launchToF32(c, nU, u, ctx->fU);
+                launchToF32(c, N, ctx->iJd, ctx->fJd);
+                launchToF32(c, F, ctx->wLam, ctx->fLam);
+                w.fU = ctx->fU, w.fJd = ctx->fJd, w.fLam = ctx->fLam;
+            }
+            if (is->gmresCode == 2)
+                applyLUSGSPrecond(c, gp, w, ctx->iRhs, ctx->iInc);
+            else
+                DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iInc, 0, nU * sizeof(real), c.stream));
+            gmresSolve(c, gp, w, ctx->iRhs, ctx->iInc, is->nGmresSpace, is->nGmresIter, nullptr);
+            launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, ctx->iInc, ctx->iFlag);
+        }
+        else if (is->fp32Sweeps)
         {
             launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
             launchToF32(c, nU, u, ctx->fU);
@@ -875,6 +939,51 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         if (resRel)
             resRel[v] = rel[v];
     }
+    CAPI_END
+}
+
+int dndsr_lusgs_matvec(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *u,
+                       const double *x, const double *jdiag, const double *lambdaFace, double *Ax)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && u && x && jdiag && lambdaFace && Ax, "null field");
+    need(x != Ax, "x and Ax must be different arrays");
+    launchLUSGSMatVec(ctx->c, toGas(es), alphaDiag, u, x, jdiag, lambdaFace, Ax);
+    CAPI_END
+}
+
+int dndsr_gmres_lusgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *u,
+                      const double *jdiag, const double *lambdaFace, const double *b, double *x, int nGmresSpace,
+                      int nGmresIter, int sgsIter, int gsUpdate, int fp32Precond, double *residual)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && u && jdiag && lambdaFace && b && x, "null field");
+    need(nGmresSpace >= 1 && nGmresIter >= 0 && sgsIter >= 0, "bad GMRES settings");
+    Ctx &c = ctx->c;
+    const size_t N = c.m.N, F = c.m.F, nU = 5 * N;
+    LinWork &w = linWork(ctx, nGmresSpace, fp32Precond != 0);
+    w.u = u, w.jd = jdiag, w.lam = lambdaFace, w.alphaDiag = alphaDiag;
+    w.sgsIter = sgsIter, w.gsUpdate = gsUpdate, w.fp32 = fp32Precond != 0;
+    if (w.fp32)
+    {
+        if (!ctx->fU)
+            ctx->fU = devAlloc<float>(c, nU);
+        if (!ctx->fJd)
+            ctx->fJd = devAlloc<float>(c, N);
+        if (!ctx->fLam)
+            ctx->fLam = devAlloc<float>(c, 4 * F);
+        launchToF32(c, nU, u, ctx->fU);
+        launchToF32(c, N, jdiag, ctx->fJd);
+        launchToF32(c, F, lambdaFace, ctx->fLam);
+        w.fU = ctx->fU, w.fJd = ctx->fJd, w.fLam = ctx->fLam;
+    }
+    double r[2] = {0, 0};
+    gmresSolve(c, toGas(es), w, b, x, nGmresSpace, nGmresIter, r);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    if (residual)
+        residual[0] = r[0], residual[1] = r[1];
     CAPI_END
 }
 

@@ -132,7 +132,8 @@ enum KernelId : int
     K_INCREMENT = 9,
     K_CONVERT = 10,
     K_ADJOINT = 11,
-    K_NUM = 12,
+    K_LINSOLVE = 12,
+    K_NUM = 13,
 };
 
 // Records a HIP event pair around the launches in its scope when timing is on.
@@ -195,6 +196,24 @@ void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
                           int *flagDev);
 void launchToF32(Ctx &c, size_t n, const real *in, float *out);
+// ---- linear solver (linsolve.hip)
+struct LinWork
+{
+    // operator: LU-SGS matrix at the state u (FP64) and its FP32 copies for FP32 sweeps
+    const real *u = nullptr, *jd = nullptr, *lam = nullptr;
+    const float *fU = nullptr, *fJd = nullptr, *fLam = nullptr;
+    real alphaDiag = 1.0;
+    int sgsIter = 0, gsUpdate = 1, fp32 = 0;
+    // scratch: FP32 sweep vectors, FP64 sweep vector, Krylov basis [(k+1)][5N], temporaries
+    float *fR = nullptr, *fX = nullptr, *fT = nullptr;
+    real *dT = nullptr, *V = nullptr, *Vt = nullptr, *MLb = nullptr, *h = nullptr, *part = nullptr,
+         *scal = nullptr, *coef = nullptr;
+};
+void launchLUSGSMatVec(Ctx &c, const GasParams &gp, real alphaDiag, const real *u, const real *x, const real *jd,
+                       const real *lam, real *Ax);
+void applyLUSGSPrecond(Ctx &c, const GasParams &gp, const LinWork &w, const real *r, real *out);
+int gmresSolve(Ctx &c, const GasParams &gp, const LinWork &w, const real *b, real *x, int kSub, int nRestart,
+               double *residualOut);
 // ---- launchers (adjoint.hip)
 void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
                       const real *rbar, real *ubar, real *urecbar);

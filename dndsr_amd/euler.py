@@ -97,16 +97,36 @@ class EulerEvaluator:
             return res.cpu().numpy()
         return a.max(dim=1).values.cpu().numpy()
 
+    # LUSGSMatrixVec(alphaDiag, t, u, uInc, JDiag, AuInc), EulerEvaluator.hpp:449-455
+    def LUSGSMatrixVec(self, alphaDiag, t, u, uInc, JDiag, AuInc, lambdaFace=None):
+        lam = self.lambdaFace if lambdaFace is None else lambdaFace
+        check(lib().dndsr_lusgs_matvec(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(u), _ptr(uInc),
+                                       _ptr(JDiag), _ptr(lam), _ptr(AuInc)))
+
+    def GMRES(self, alphaDiag, u, JDiag, b, x, nGmresSpace=10, nGmresIter=2, sgsIter=0, jacobiCode=1,
+              fp32Precond=False, lambdaFace=None):
+        """GMRES_LeftPreconditioned::solve (Linear.hpp:47-131) on the LU-SGS matrix with the
+        LU-SGS preconditioner (FP32 sweeps when fp32Precond); x: initial guess -> solution.
+        Returns (last preconditioned residual norm, |M b|)."""
+        lam = self.lambdaFace if lambdaFace is None else lambdaFace
+        r = np.zeros(2)
+        check(lib().dndsr_gmres_lusgs(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(u), _ptr(JDiag),
+                                      _ptr(lam), _ptr(b), _ptr(x), int(nGmresSpace), int(nGmresIter), int(sgsIter),
+                                      int(jacobiCode), int(bool(fp32Precond)), r.ctypes.data_as(_lib.dp)))
+        return r[0], r[1]
+
     def ImplicitEulerStep(self, u, uRec, dt, CFL=10.0, nRec=1, sgsIter=0, jacobiCode=1, maxIter=20, minIter=5,
-                          tol=1e-10, fp32Sweeps=False, incrementPPRelax=0.9, refRho=1.0, refP=1.0):
+                          tol=1e-10, fp32Sweeps=False, incrementPPRelax=0.9, refRho=1.0, refP=1.0, gmresCode=0,
+                          nGmresSpace=10, nGmresIter=2):
         """One implicit-Euler dual-time step on the device (ImplicitEulerDualTimeStep::Step,
         ODE.hpp:80-98, with EulerSolver's fdtau / frhs / fsolve / fincrement / fstop,
         EulerSolver.hxx:645-830, 1206).  Defaults follow EulerSolver's config defaults
-        (EulerSolver.hpp:171-760).  fp32Sweeps runs the LU-SGS sweeps in FP32 while the
-        residual, reconstruction and update stay FP64.  Returns (nIter, res, resRel, history)."""
+        (EulerSolver.hpp:171-760).  fp32Sweeps runs the LU-SGS sweeps (the solver, or the GMRES
+        preconditioner when gmresCode is 1 or 2) in FP32 while the residual, reconstruction,
+        Krylov vectors and update stay FP64.  Returns (nIter, res, resRel, history)."""
         s = _lib.ImplicitSettings(float(dt), float(CFL), int(nRec), int(sgsIter), int(jacobiCode), int(maxIter),
                                   int(minIter), float(tol), int(bool(fp32Sweeps)), float(incrementPPRelax),
-                                  float(refRho), float(refP))
+                                  float(refRho), float(refP), int(gmresCode), int(nGmresSpace), int(nGmresIter))
         n = C.c_int()
         res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(int(maxIter), 1))
         check(lib().dndsr_implicit_euler_step(self.ctx.h, C.byref(self.es), C.byref(s), _ptr(u), _ptr(uRec),

@@ -174,6 +174,9 @@ typedef struct
     int fp32Sweeps;        /* 1: LU-SGS sweeps in FP32 (residual, VR and update stay FP64) */
     double incrementRelax; /* incrementPPRelax */
     double refRho, refP;   /* reference density / pressure of the positivity epsilons (refUPrim) */
+    int gmresCode;         /* linearSolverControl.gmresCode: 0 LU-SGS, 1 GMRES, 2 LU-SGS then GMRES */
+    int nGmresSpace;       /* Krylov subspace size (nGmresSpace) */
+    int nGmresIter;        /* restart cycles (nGmresIter) */
 } dndsr_implicit_settings;
 
 /* One implicit-Euler dual-time step, entirely on the device
@@ -189,6 +192,18 @@ typedef struct
  * positivity limiter of the increment. */
 int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
                               double *u, double *uRec, int *nIter, double *res, double *resRel, double *history);
+
+/* LUSGSMatrixVec (scalar Jacobian), src/Euler/EulerEvaluator.hpp:449-455:
+ *   Ax = D x + sum_f (alphaDiag/2) A_f / V fluxJacobian0(u_j) x_j, the matrix UpdateSGS sweeps. */
+int dndsr_lusgs_matvec(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *u,
+                       const double *x, const double *jdiag, const double *lambdaFace, double *Ax);
+/* GMRES_LeftPreconditioned<...>::solve (src/Solver/Linear.hpp:47-131) on the LU-SGS matrix with
+ * the LU-SGS preconditioner ((1 + sgsIter) symmetric sweeps from zero, FP32 when fp32Precond):
+ * x holds the initial guess and receives the solution; nGmresSpace vectors, nGmresIter restart
+ * cycles.  residual (may be NULL) receives the last preconditioned residual norm and |M b|. */
+int dndsr_gmres_lusgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *u,
+                      const double *jdiag, const double *lambdaFace, const double *b, double *x, int nGmresSpace,
+                      int nGmresIter, int sgsIter, int gsUpdate, int fp32Precond, double *residual);
 
 /* ---- discrete adjoint (no counterpart in the reference; checked against central finite
  * differences of the forward operators) ----------------------------------------------
@@ -211,9 +226,10 @@ int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const d
  * last call per kernel id, and returns the running totals (ms) and launch counts:
  *   0 VR iteration, 1 RHS face pass, 2 RHS cell gather, 3 EvaluateDt face pass,
  *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init,
- *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion, 11 adjoint kernels.
+ *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion, 11 adjoint kernels,
+ *   12 linear-solver kernels (LUSGSMatrixVec, GMRES vector operations).
  * reset != 0 clears the totals after reading them. */
-#define DNDSR_NUM_KERNEL_IDS 12
+#define DNDSR_NUM_KERNEL_IDS 13
 int dndsr_set_timing(dndsr_ctx *ctx, int enable);
 int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset);
 

@@ -1769,6 +1769,36 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
     return 0;
 }
 
+// LUSGSMatrixVec (EulerEvaluator.hxx:175-275), scalar Jacobian: the matrix whose
+// Gauss-Seidel sweeps UpdateSGS performs, Ax_i = D_i x_i - buf_i with
+// buf_i = -sum_f (alpha/2) A_f / V_i * fluxJacobian0(u_j) x_j (lambda term included).
+int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
+                     const double *x, const double *jdiag, const double *lambdaFace, double *Ax)
+{
+    const VR &v = h->v;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real buf[5] = {0, 0, 0, 0, 0};
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            int atFace = v.cellIsBack(i, f) ? 0 : 1;
+            if (o < 0)
+                continue;
+            real n[3];
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFace ? -1 : 1);
+            real dF[5];
+            fluxJacobianTimesDu(u + o * 5, n, x + o * 5, lambdaFace[4 * f], es->gamma, dF);
+            for (int k = 0; k < 5; k++)
+                buf[k] -= (0.5 * alphaDiag) * v.area[f] / v.vol[i] * dF[k];
+        }
+        for (int k = 0; k < 5; k++)
+            Ax[i * 5 + k] = jdiag[i] * x[i * 5 + k] - buf[k];
+    }
+    return 0;
+}
+
 // Explicit SSP-RK3 (src/Solver/ODE.hpp:1477-1520) around frhs = nRec x Jacobi VR + EvaluateRHS,
 // fdt = EvaluateDt (global dt from CFL unless dtFixed > 0); fincrement = plain add (the
 // positivity compression of EulerEvaluator.hpp:1795 is the identity for these smooth states).

@@ -111,6 +111,9 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
                    const double *rhs, const double *u, const double *uInc, double *uIncNew,
                    const double *jdiag, const double *lambdaFace, int forward, int gsUpdate,
                    int uIncIsZero, const int64_t *scanOrder, double *sumInc);
+/* LUSGSMatrixVec (scalar Jacobian): Ax = D x + sum_f (alpha/2) A_f / V fluxJacobian0(u_j) x_j */
+int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
+                     const double *x, const double *jdiag, const double *lambdaFace, double *Ax);
 /* full explicit SSP-RK3 steps: nStep steps of (EvaluateDt once, 3 x (nRec VR + RHS)) */
 int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,
                double CFL, int nRec, double dtFixed);

@@ -64,7 +64,7 @@ def lib():
         _lib.orc_rec_error_l1.restype = C.c_double
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_iter_diff",
-                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step",
+                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann"]:
             getattr(_lib, name).argtypes = None
     return _lib
@@ -246,3 +246,11 @@ class OracleVR:
         _chk(lib().orc_implicit_euler_step(self.h, C.byref(es), C.byref(st), P(u), P(uRec), P(so), C.byref(n), P(res),
                                            P(rel), P(hist)))
         return u, uRec, n.value, res, rel, hist[:n.value]
+
+    def lusgs_matvec(self, es, alphaDiag, u, x, jdiag, lam):
+        """LUSGSMatrixVec: the scalar-Jacobian LU-SGS matrix applied to x."""
+        out = np.zeros((self.nCell, 5))
+        _chk(lib().orc_lusgs_matvec(self.h, C.byref(es), C.c_double(alphaDiag), P(np.ascontiguousarray(u)),
+                                    P(np.ascontiguousarray(x)), P(np.ascontiguousarray(jdiag)),
+                                    P(np.ascontiguousarray(lam)), P(out)))
+        return out

@@ -85,3 +85,23 @@ def test_first_iteration_matches_operator_sequence():
     x2, _ = o.update_sgs(es, 1.0, rhs, u0, x1, z, jd, lam, False, True, False, scanOrder=order)
     np.testing.assert_array_equal(rec1, rec)
     np.testing.assert_allclose(u1, u0 + x2, rtol=0, atol=1e-14 * np.abs(u0).max())
+
+
+def test_sgs_converges_to_the_lusgs_matrix_solution():
+    """LUSGSMatrixVec is the operator UpdateSGS inverts: repeated symmetric Gauss-Seidel
+    sweeps converge to x with lusgs_matvec(x) = b (pins the oracle's matvec)."""
+    from helpers import random_state
+    m = mm.periodic_quad_grid(6, 5, 6.0, 5.0, jitter=0.1, seed=5)
+    o = po.OracleVR(m, maxOrder=1, intOrder=1)
+    es = o.euler_settings(1.4, 1)
+    N = o.nCell
+    u = random_state(N, 6)
+    dt, lam, _, _ = o.eval_dt(es, u, 1.0)
+    jd = o.lusgs_init(dt, 1e100, 1.0, lam)
+    b = np.random.default_rng(3).uniform(-1, 1, (N, 5))
+    x = np.zeros((N, 5))
+    z = np.zeros_like(x)
+    for _ in range(200):
+        x1, _ = o.update_sgs(es, 1.0, b, u, x, z, jd, lam, True, True, False)
+        x, _ = o.update_sgs(es, 1.0, b, u, x1, z, jd, lam, False, True, False)
+    assert np.abs(o.lusgs_matvec(es, 1.0, u, x, jd, lam) - b).max() < 1e-13
