@@ -63,8 +63,14 @@ def test_gmres_solves_the_lusgs_system(require_gpu, fp32):
     xTrue = np.linalg.solve(A, b.ravel()).reshape(N, 5)
     x = torch.zeros_like(du)
     res, resB = ev.GMRES(1.0, du, jdd, g.to_device(b), x, nGmresSpace=30, nGmresIter=6, sgsIter=1, fp32Precond=fp32)
-    assert res <= 1e-9 * resB
-    assert rel_l2(g.to_host(x), xTrue) <= 1e-8
+    # Left preconditioning minimises |M (A x - b)|; an FP32 preconditioner is linear only to
+    # FP32 rounding, so that residual (and the solution) floor near single precision.  In the
+    # implicit loop the FP64 outer residual removes this floor (test below).
+    tol = 1e-6 if fp32 else 1e-9
+    assert res <= tol * resB
+    xs = g.to_host(x)
+    assert rel_l2(xs, xTrue) <= (1e-5 if fp32 else 1e-8)
+    assert np.linalg.norm(A @ xs.ravel() - b.ravel()) <= (1e-5 if fp32 else 1e-8) * np.linalg.norm(b)
 
 
 @pytest.mark.parametrize("gmresCode,fp32", [(1, False), (2, True)])
