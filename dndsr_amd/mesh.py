@@ -14,10 +14,11 @@ Node order and face-normal conventions:
     Faces are emitted so that this normal points from the back cell to the front cell.
   * cell2face follows the element face order: Quad4 (y-, x+, y+, x-),
     Hex8 (z-, y-, x+, y+, x-, z+).
-Cells are numbered lexicographically (x fastest); face ``dim*c + d`` is the +d face
-of cell ``c`` (faces numbered by owner cell), so a run of consecutive faces covers
-all faces of a run of consecutive cells and each cell's data is fetched once per
-run rather than once per direction.
+Cells are numbered lexicographically (x fastest).  Faces are numbered by blocks of
+64 consecutive owner cells: within a block, the 64 +x faces, then the 64 +y faces
+(then +z), so a run of 64 faces is one direction over 64 consecutive cells (one
+coalesced row per wavefront) and the faces of a block of cells are processed back
+to back (the cells' data is reused from cache across the directions).
 """
 from __future__ import annotations
 
@@ -28,10 +29,18 @@ import numpy as np
 LINE2, QUAD4, HEX8, TRI3, TET4 = 1, 2, 3, 4, 5
 
 
-def _by_owner(dim, N, face2node, face2cell, faceShift, cell2face):
-    """Renumber direction-major faces (d N + c) to owner-major (dim c + d)."""
+FACE_BLOCK = 64
+
+
+def _by_cell_block(dim, N, face2node, face2cell, faceShift, cell2face, B=FACE_BLOCK):
+    """Renumber direction-major faces (d N + c) block-direction-major: cell c lies in block
+    k = c // B of size r (B, or the remainder for the last block); its +d face becomes
+    k B dim + d r + c % B."""
     f = np.arange(dim * N)
-    new = dim * (f % N) + f // N
+    c, d = f % N, f // N
+    k = c // B
+    r = np.minimum(B, N - k * B)
+    new = k * B * dim + d * r + (c - k * B)
     inv = np.empty_like(new)
     inv[new] = f
     return face2node[inv], face2cell[inv], faceShift[inv], new[cell2face]
@@ -112,7 +121,7 @@ def periodic_quad_grid(nx: int, ny: int, Lx: float = 10.0, Ly: float = 10.0,
     xm = ((ci - 1) % nx) + nx * cj
     ym = ci + nx * ((cj - 1) % ny)
     cell2face = np.stack([N + ym, c, N + c, xm], 1)   # y-, x+, y+, x-
-    face2node, face2cell, faceShift, cell2face = _by_owner(2, N, face2node, face2cell, faceShift, cell2face)
+    face2node, face2cell, faceShift, cell2face = _by_cell_block(2, N, face2node, face2cell, faceShift, cell2face)
     return Mesh(2, coords, np.full(N, QUAD4, np.int32), np.arange(N + 1, dtype=np.int64) * 4,
                 cell2node.ravel().astype(np.int64), np.arange(N + 1, dtype=np.int64) * 4,
                 cell2face.ravel().astype(np.int64), np.full(2 * N, LINE2, np.int32),
@@ -155,7 +164,7 @@ def periodic_hex_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.
     faceShift = np.concatenate([sx, sy, sz])
     xm, ym, zm = cid(ci - 1, cj, ck), cid(ci, cj - 1, ck), cid(ci, cj, ck - 1)
     cell2face = np.stack([2 * N + zm, N + ym, c, N + c, xm, 2 * N + c], 1)  # z-, y-, x+, y+, x-, z+
-    face2node, face2cell, faceShift, cell2face = _by_owner(3, N, face2node, face2cell, faceShift, cell2face)
+    face2node, face2cell, faceShift, cell2face = _by_cell_block(3, N, face2node, face2cell, faceShift, cell2face)
     F = 3 * N
     return Mesh(3, coords, np.full(N, HEX8, np.int32), np.arange(N + 1, dtype=np.int64) * 8,
                 cell2node.ravel().astype(np.int64), np.arange(N + 1, dtype=np.int64) * 6,
