@@ -28,12 +28,17 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 def alg_bytes(dim, M, nV, n_fc, f_c, G_f):
-    """Algorithmic HBM bytes per cell and launch: each array counted once at its stored
-    precision, basis values recomputed (not tabulated), 4-byte indices.
+    """Algorithmic HBM bytes per cell and launch: each array the kernels read or write
+    counted once at its stored precision, 4-byte indices, basis values recomputed (not
+    tabulated); the face-flux intermediate of EvaluateRHS is not counted.
     VR Jacobi sweep: 8 [2 M nV + nV + n_fc (M^2 + M)] + 4 n_fc
-    EvaluateRHS:     8 [nV (2 + M) + 1 + 2 d + M] + f_c [8 + 8 G_f (2 d + 1)]"""
+        (uRec in/out, u, A^-1 B blocks and A^-1 b per face slot, neighbour index)
+    EvaluateRHS:     8 [nV (2 + M) + M + d + 3] + f_c [8 + 8 (G_f (2 d + 1) + 3)]
+        per cell: u, rhs out, uRec, moments, centre, 1/scale, volume, deltaLambda;
+        per face (f_c faces per cell): the two cell indices, point coordinates, unit
+        normals and w JDet at G_f points, the periodic shift"""
     b_vr = 8 * (2 * M * nV + nV + n_fc * (M * M + M)) + 4 * n_fc
-    b_rhs = 8 * (nV * (2 + M) + 1 + 2 * dim + M) + f_c * (8 + 8 * G_f * (2 * dim + 1))
+    b_rhs = 8 * (nV * (2 + M) + M + dim + 3) + f_c * (8 + 8 * (G_f * (2 * dim + 1) + 3))
     return b_vr, b_rhs
 
 

@@ -19,8 +19,7 @@ struct dndsr_ctx
     Ctx c;
     hipStream_t own = nullptr;
     // device-resident time-loop workspace
-    real *wXLast = nullptr, *wRhs = nullptr, *wURecNew = nullptr, *wDt = nullptr, *wLam = nullptr, *wDlc = nullptr,
-         *wDtMin = nullptr;
+    real *wXLast = nullptr, *wURecNew = nullptr, *wLam = nullptr, *wDlc = nullptr, *wDtMin = nullptr;
     bool haveWork = false;
     // implicit dual-time workspace
     real *iXLast = nullptr, *iRhsF = nullptr, *iRhs = nullptr, *iDTau = nullptr, *iJd = nullptr, *iInc = nullptr,
@@ -33,6 +32,21 @@ struct dndsr_ctx
     int linK = -1;
     bool linF32 = false;
 };
+
+// workspace of the device time loops (SSP-RK3 and the implicit step share it)
+static void timeLoopWork(dndsr_ctx *ctx)
+{
+    if (ctx->haveWork)
+        return;
+    Ctx &c = ctx->c;
+    const size_t N = c.m.N;
+    ctx->wXLast = devAlloc<real>(c, 5 * N);
+    ctx->wURecNew = devAlloc<real>(c, 5 * (size_t)c.m.M * N);
+    ctx->wLam = devAlloc<real>(c, 4 * (size_t)c.m.F);
+    ctx->wDlc = devAlloc<real>(c, N);
+    ctx->wDtMin = devAlloc<real>(c, 1);
+    ctx->haveWork = true;
+}
 
 // workspace for GMRES(k) and the LU-SGS preconditioner; FP32 buffers only when needed
 static LinWork &linWork(dndsr_ctx *ctx, int k, bool fp32)
@@ -740,29 +754,21 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
     GasParams gp = toGas(es);
     const size_t N = m.N;
     const size_t nU = 5 * N, nR = 5 * (size_t)m.M * N;
-    if (!ctx->haveWork)
-    {
-        ctx->wXLast = devAlloc<real>(c, nU);
-        ctx->wRhs = devAlloc<real>(c, nU);
-        ctx->wURecNew = devAlloc<real>(c, nR);
-        ctx->wDt = devAlloc<real>(c, N);
-        ctx->wLam = devAlloc<real>(c, 4 * (size_t)m.F);
-        ctx->wDlc = devAlloc<real>(c, N);
-        ctx->wDtMin = devAlloc<real>(c, 1);
-        ctx->haveWork = true;
-    }
+    timeLoopWork(ctx);
     real *rec = uRec, *recNew = ctx->wURecNew;
-    auto frhs = [&](const real *x) {
+    // one RK stage: VR sweeps at u, RHS(u), and u = a u + b xLast + cdt dt RHS in the gather pass
+    auto stage = [&](real a, real b, real cdt, const real *dtp, real *save) {
         for (int r = 0; r < nRec; r++)
         {
-            launchVRIter(c, 5, rec, recNew, x, 0, m.s.jacobiRelax);
+            launchVRIter(c, 5, rec, recNew, u, 0, m.s.jacobiRelax);
             std::swap(rec, recNew);
         }
-        launchEvalRHS(c, gp, x, rec, ctx->wDlc, ctx->wRhs);
+        StageUpdate st{a, b, cdt, b != 0.0 ? ctx->wXLast : nullptr, dtp, u, save};
+        launchEvalRHS(c, gp, u, rec, ctx->wDlc, nullptr, &st);
     };
     for (int step = 0; step < nStep; step++)
     {
-        launchEvalDt(c, gp, u, CFL, 1e100, 0, ctx->wDt, ctx->wLam, ctx->wDlc, ctx->wDtMin);
+        launchEvalDt(c, gp, u, CFL, 1e100, 0, nullptr, ctx->wLam, ctx->wDlc, ctx->wDtMin); // global dt only
         const real *dtp = ctx->wDtMin;
         if (dtFixed > 0)
         {
@@ -770,16 +776,12 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
             DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->wDtMin, c.hostRed, sizeof(real), hipMemcpyHostToDevice, c.stream));
             DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
         }
-        DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->wXLast, u, nU * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
-        frhs(u);
-        launchAxpby(c, (idx)nU, 1.0, u, 0.0, nullptr, 1.0, ctx->wRhs, dtp, u);
-        frhs(u);
-        launchAxpby(c, (idx)nU, 0.25, u, 0.75, ctx->wXLast, 0.25, ctx->wRhs, dtp, u);
-        frhs(u);
-        launchAxpby(c, (idx)nU, 2. / 3., u, 1. / 3., ctx->wXLast, 2. / 3., ctx->wRhs, dtp, u);
+        stage(1.0, 0.0, 1.0, dtp, ctx->wXLast); // xLast = u, then u1 = u + dt R(u)
+        stage(0.25, 0.75, 0.25, dtp, nullptr);
+        stage(2. / 3., 1. / 3., 2. / 3., dtp, nullptr);
     }
     if (rec != uRec) // odd number of VR sweeps in total: bring the result home
-        DNDSR_HIP_CHECK(hipMemcpyAsync(uRec, rec, nR * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+        launchCopy(c, nR, rec, uRec);
     CAPI_END
 }
 
@@ -798,17 +800,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     GasParams gp = toGas(es);
     const size_t N = m.N, F = m.F;
     const size_t nU = 5 * N, nR = 5 * (size_t)m.M * N;
-    if (!ctx->haveWork)
-    {
-        ctx->wXLast = devAlloc<real>(c, nU);
-        ctx->wRhs = devAlloc<real>(c, nU);
-        ctx->wURecNew = devAlloc<real>(c, nR);
-        ctx->wDt = devAlloc<real>(c, N);
-        ctx->wLam = devAlloc<real>(c, 4 * F);
-        ctx->wDlc = devAlloc<real>(c, N);
-        ctx->wDtMin = devAlloc<real>(c, 1);
-        ctx->haveWork = true;
-    }
+    timeLoopWork(ctx);
     if (!ctx->haveImplicit)
     {
         ctx->iXLast = devAlloc<real>(c, nU);
@@ -835,7 +827,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     const real rhoEps = kSmall * is->refRho * 1e-1, pEps = kSmall * is->refP * 1e-1;
     real *sumDev = c.red + 5 * ((size_t)gridFor(m.N, 256) + m.nColor + 6);
     DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iFlag, 0, sizeof(int), c.stream));
-    DNDSR_HIP_CHECK(hipMemcpyAsync(ctx->iXLast, u, nU * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+    launchCopy(c, nU, u, ctx->iXLast);
     real *rec = uRec, *recNew = ctx->wURecNew;
     real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
     int it = 0;
@@ -924,7 +916,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     if (it > is->maxIter)
         it = is->maxIter;
     if (rec != uRec)
-        DNDSR_HIP_CHECK(hipMemcpyAsync(uRec, rec, nR * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+        launchCopy(c, nR, rec, uRec);
     int flag = 0;
     DNDSR_HIP_CHECK(hipMemcpyAsync(&flag, ctx->iFlag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));

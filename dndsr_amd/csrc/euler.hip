@@ -68,7 +68,8 @@ __global__ void __launch_bounds__(256) kDtCell(DevMesh m, real CFL, real maxDt, 
             dl = fmax(dl, dLamF[f]);
         }
         d = fmin(CFL * m.vol[i] * m.smooth[i] / (lc + 1e-100), maxDt);
-        dt[i] = d;
+        if (dt)
+            dt[i] = d;
         if (dlc)
             dlc[i] = dl;
     }
@@ -126,7 +127,7 @@ void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real max
     }
     kReduceMin<<<1, 256, 0, c.stream>>>(c.red, nb, dtMinDev);
     DNDSR_HIP_CHECK(hipGetLastError());
-    if (!useLocalDt)
+    if (!useLocalDt && dt)
     {
         kFillFromScalar<<<nb, 256, 0, c.stream>>>(m.N, dtMinDev, dt);
         DNDSR_HIP_CHECK(hipGetLastError());
@@ -443,7 +444,8 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
     }
 }
 
-__global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restrict__ fflux, real *__restrict__ rhs)
+__global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restrict__ fflux, real *__restrict__ rhs,
+                                                  StageUpdate st)
 {
     idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.N)
@@ -464,9 +466,343 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
             acc[v] += (back ? fv : -fv) / vol;
         }
     }
+    if (rhs)
+    {
 #pragma unroll
-    for (int v = 0; v < 5; v++)
-        rhs[v * N + i] = acc[v];
+        for (int v = 0; v < 5; v++)
+            rhs[v * N + i] = acc[v];
+    }
+    if (st.u) // same arithmetic as kAxpby(a, u, b, y, cdt, rhs, dt)
+    {
+        const real dt = st.dtDev ? st.dtDev[0] : 1.0;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            real xi = st.u[v * N + i];
+            if (st.save)
+                st.save[v * N + i] = xi;
+            real r = st.a * xi;
+            if (st.y)
+                r += st.b * st.y[v * N + i];
+            r += st.cdt * (acc[v] * dt);
+            st.u[v * N + i] = r;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Multi-point face tile kernel: NW = ceil(G / PPW) wavefronts per 64-face tile,
+// each lane evaluating PPW quadrature points of its face.  One LDS read of a
+// staged row (reconstruction coefficient, centre, moment, mean, face constant)
+// feeds PPW point evaluations, and the PPW independent point computations give
+// each wave instruction-level parallelism in place of wave count.  Every wave
+// stages rows straight from global memory into LDS (buffer_load ... lds: no
+// VGPRs, no LDS store instructions, all rows of a wave in flight at once); wave 0
+// also forms the face constants.  Point fluxes go through LDS and are summed in
+// point order, as in the one-point-per-wave kernel above.
+// ---------------------------------------------------------------------------
+typedef __attribute__((address_space(3))) void *LdsPtr;
+
+// Byte offsets of the direct-to-LDS row loads: lane l carries dword (l & 1) of
+// the faces (l >> 1) and 32 + (l >> 1) of the tile, so a row of 64 doubles is
+// two dword loads.
+struct LdsRowOffsets
+{
+    uint32_t L[2], R[2], F[2];
+};
+
+template <int NDOF, int Q>
+__device__ __forceinline__ void rhsLdsRow(const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                          const LdsRowOffsets &o)
+{
+    using R = RHSRows<NDOF>;
+    constexpr int M = R::M;
+    const size_t N = m.N, F = m.F;
+    const uint64_t rowN = (uint64_t)N * 8;
+    real *dst = lds + Q * kTileF;
+    if constexpr (Q < R::REC)
+    {
+        const Rsrc rs = rowRsrc(u + (size_t)(Q % 5) * N, rowN);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, Q < 5 ? o.L[h] : o.R[h], 0, 0, 0);
+    }
+    else if constexpr (Q < R::MET)
+    {
+        constexpr int r = Q - R::REC, side = r / (5 * M), vj = r % (5 * M);
+        if (uRec) // without coefficients these rows are never read
+        {
+            const Rsrc rs = rowRsrc(uRec + (size_t)vj * N, rowN);
+#pragma unroll
+            for (int h = 0; h < 2; h++)
+                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, side ? o.R[h] : o.L[h], 0, 0,
+                                                         0);
+        }
+    }
+    else if constexpr (Q < R::SHIFT)
+    {
+        constexpr int r = Q - R::MET, side = r / R::METW, k = r % R::METW;
+        const real *base = k < 3 ? m.cent + (size_t)k * N : (k == 3 ? m.iscale : m.moment + (size_t)(k - 4) * N);
+        const Rsrc rs = rowRsrc(base, rowN);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, side ? o.R[h] : o.L[h], 0, 0, 0);
+    }
+    else
+    {
+        const Rsrc rs = rowRsrc(m.fshift + (size_t)(Q - R::SHIFT) * F, (uint64_t)F * 8);
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, o.F[h], 0, 0, 0);
+    }
+}
+template <int NDOF, int W, int NS, int T>
+__device__ __forceinline__ void rhsLdsSeq(const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                          const LdsRowOffsets &o)
+{
+    constexpr int PER = (RHSRows<NDOF>::STAGED + NS - 1) / NS;
+    if constexpr (T < PER)
+    {
+        constexpr int Q = W + T * NS;
+        if constexpr (Q < RHSRows<NDOF>::STAGED)
+            rhsLdsRow<NDOF, Q>(m, u, uRec, lds, o);
+        rhsLdsSeq<NDOF, W, NS, T + 1>(m, u, uRec, lds, o);
+    }
+}
+template <int NDOF, int NS, int W>
+__device__ __forceinline__ void rhsLdsDispatch(int w, const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                               const LdsRowOffsets &o)
+{
+    if constexpr (W < NS)
+    {
+        if (w == W)
+            rhsLdsSeq<NDOF, W, NS, 0>(m, u, uRec, lds, o);
+        else
+            rhsLdsDispatch<NDOF, NS, W + 1>(w, m, u, uRec, lds, o);
+    }
+}
+
+template <int DIM, int NDOF, int G, int RS, bool BATCH, int PPW>
+__global__ void __launch_bounds__(kTileF *((G + PPW - 1) / PPW)) __attribute__((amdgpu_waves_per_eu(2)))
+kRHSFaceW(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__restrict__ uRec,
+          const real *__restrict__ dlc, real *__restrict__ fflux)
+{
+    using R = RHSRows<NDOF>;
+    constexpr int M = R::M;
+    constexpr int NW = (G + PPW - 1) / PPW;
+    extern __shared__ real lds[];
+    const int lane = threadIdx.x % kTileF, g = waveIndex(kTileF);
+    const size_t N = m.N, F = m.F;
+    const uint64_t rowN = (uint64_t)N * 8, rowF = (uint64_t)F * 8;
+    const int tile = xcdTile(blockIdx.x, gridDim.x);
+    const idx f = tile * kTileF + lane;
+    const bool valid = f < m.F;
+    const uint32_t fo = (uint32_t)(valid ? f : 0) * 8u;
+    const idx cL = bldi(rowRsrc(m.f2c, (uint64_t)F * 4), fo / 2);
+    const idx cR = bldi(rowRsrc(m.f2c + F, (uint64_t)F * 4), fo / 2);
+    const uint32_t oL = (uint32_t)cL * 8u, oR = (uint32_t)cR * 8u;
+
+    {
+        LdsRowOffsets o;
+        const int hl = lane >> 1;
+        const uint32_t par = (uint32_t)(lane & 1) * 4u;
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+        {
+            o.L[h] = (uint32_t)__shfl((int)oL, 32 * h + hl) + par;
+            o.R[h] = (uint32_t)__shfl((int)oR, 32 * h + hl) + par;
+            const idx f2 = tile * kTileF + 32 * h + hl;
+            o.F[h] = (uint32_t)(f2 < m.F ? f2 : 0) * 8u + par;
+        }
+        rhsLdsDispatch<NDOF, NW, 0>(g, m, u, uRec, lds, o);
+    }
+    // this lane's point geometry (registers, used after the barrier; read once, streamed)
+    real x[PPW][3], n[PPW][3], wj[PPW];
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        const int q = g * PPW + p < G ? g * PPW + p : G - 1; // the last wave may hold fewer points
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+        {
+            x[p][r] = bldnt(rowRsrc(m.fpts + ((size_t)q * 3 + r) * F, rowF), fo);
+            n[p][r] = bldnt(rowRsrc(m.fnorm + ((size_t)q * 3 + r) * F, rowF), fo);
+        }
+        wj[p] = bldnt(rowRsrc(m.fwj + (size_t)q * F, rowF), fo);
+    }
+    if (g == 0)
+    {
+        // face constants from the cell means
+        real uLm[5], uRm[5];
+        real dLambda = 0;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            const Rsrc rr = rowRsrc(u + (size_t)v * N, rowN);
+            uLm[v] = bld(rr, oL), uRm[v] = bld(rr, oR);
+        }
+        if (dlc)
+        {
+            const Rsrc rr = rowRsrc(dlc, rowN);
+            dLambda = fmax(bld(rr, oL), bld(rr, oR));
+        }
+        if (!valid)
+        {
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                uLm[v] = uRm[v] = v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0);
+        }
+        if constexpr (BATCH)
+        {
+            RoeBatchFace bf;
+            real nm[3];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                nm[r] = bld(rowRsrc(m.fmeanN + (size_t)r * F, rowF), fo);
+            roeBatchSetup<RS>(uLm, uRm, nm, gp, dLambda, bf);
+            roeStore(lds, R::FC, lane, bf.r);
+            lds[(R::FC + kRoeRows + 0) * kTileF + lane] = bf.vRN;
+            lds[(R::FC + kRoeRows + 1) * kTileF + lane] = bf.lam0;
+            lds[(R::FC + kRoeRows + 2) * kTileF + lane] = bf.lam123;
+            lds[(R::FC + kRoeRows + 3) * kTileF + lane] = bf.lam4;
+        }
+        else
+        {
+            RoeAve ra;
+            roeAverage(uLm, uRm, gp.gamma, ra);
+            roeStore(lds, R::FC, lane, ra);
+        }
+        lds[(R::FC + kRoeRows + 4) * kTileF + lane] = dLambda;
+    }
+    __builtin_amdgcn_s_waitcnt(0); // this wave's direct-to-LDS rows have landed
+    __syncthreads();
+    real Fp[PPW][5];
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            Fp[p][v] = 0;
+    if (valid)
+    {
+        real U[PPW][2][5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            const real ul = lds[(R::MEAN + v) * kTileF + lane], ur = lds[(R::MEAN + 5 + v) * kTileF + lane];
+#pragma unroll
+            for (int p = 0; p < PPW; p++)
+                U[p][0][v] = ul, U[p][1][v] = ur;
+        }
+        if (uRec)
+        {
+#pragma unroll
+            for (int side = 0; side < 2; side++)
+            {
+                const real *met = lds + (size_t)(R::MET + side * R::METW) * kTileF + lane;
+                const real isc = met[3 * kTileF];
+                real cen[3], sh[3], mom[M > 0 ? M : 1], phi[PPW][M > 0 ? M : 1];
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    cen[r] = met[r * kTileF], sh[r] = side ? lds[(R::SHIFT + r) * kTileF + lane] : 0.0;
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    mom[j] = met[(4 + j) * kTileF];
+#pragma unroll
+                for (int p = 0; p < PPW; p++)
+                {
+                    real xs[3];
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                    {
+                        real pt = x[p][r] + sh[r];
+                        xs[r] = pt - cen[r];
+                        if (DIM == 3 || r < 2)
+                            xs[r] *= isc;
+                    }
+                    basisRow0<DIM, NDOF>(xs, mom, phi[p]);
+                }
+                const real *rec = lds + (size_t)(R::REC + side * 5 * M) * kTileF + lane;
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                {
+                    real sacc[PPW];
+#pragma unroll
+                    for (int p = 0; p < PPW; p++)
+                        sacc[p] = 0;
+#pragma unroll
+                    for (int j = 0; j < M; j++)
+                    {
+                        const real c = rec[(v * M + j) * kTileF];
+#pragma unroll
+                        for (int p = 0; p < PPW; p++)
+                            sacc[p] += phi[p][j] * c;
+                    }
+#pragma unroll
+                    for (int p = 0; p < PPW; p++)
+                        U[p][side][v] += sacc[p];
+                }
+            }
+        }
+        const real dLambda = lds[(R::FC + kRoeRows + 4) * kTileF + lane];
+        RoeAve ra;
+        roeLoad(lds, R::FC, lane, ra);
+#pragma unroll
+        for (int p = 0; p < PPW; p++)
+        {
+            if (g * PPW + p >= G)
+                continue;
+            real Fl[5];
+            if constexpr (BATCH)
+            {
+                RoeBatchFace bf;
+                bf.r = ra;
+                bf.vRN = lds[(R::FC + kRoeRows + 0) * kTileF + lane];
+                bf.lam0 = lds[(R::FC + kRoeRows + 1) * kTileF + lane];
+                bf.lam123 = lds[(R::FC + kRoeRows + 2) * kTileF + lane];
+                bf.lam4 = lds[(R::FC + kRoeRows + 3) * kTileF + lane];
+                roeBatchPoint<RS>(U[p][0], U[p][1], n[p], gp, bf, Fl);
+            }
+            else
+            {
+                real l0, l1, l4;
+                riemannFluxR<RS>(U[p][0], U[p][1], ra, n[p], gp, dLambda, Fl, l0, l1, l4);
+            }
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                Fp[p][v] = (-Fl[v]) * wj[p];
+        }
+    }
+    __syncthreads(); // all lanes are done with the staged data
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        const int q = g * PPW + p;
+        if (q < G)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                lds[(q * 5 + v) * kTileF + lane] = Fp[p][v];
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
+    {
+        const int v = t / kTileF, l = t % kTileF;
+        const idx fout = tile * kTileF + l;
+        if (fout >= m.F)
+            continue;
+        real acc = 0;
+#pragma unroll
+        for (int q = 0; q < G; q++)
+            acc += lds[(q * 5 + v) * kTileF + l];
+        fflux[(size_t)v * F + fout] = acc;
+    }
+}
+
+// points per lane of the face tile kernel for a face rule of G points (1: one
+// wavefront per point)
+template <int DIM, int G>
+constexpr int rhsPointsPerLane()
+{
+    return DIM == 3 && G == 9 ? 3 : 1;
 }
 
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
@@ -476,11 +812,24 @@ static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uR
     if (rows < 5 * G)
         rows = 5 * G;
     size_t smem = (size_t)rows * kTileF * sizeof(real);
-    auto kern = kRHSFace<DIM, NDOF, G, RS, BATCH>;
-    if (smem > 65536)
-        DNDSR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                            hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-    kern<<<gridFor(c.m.F, kTileF), kTileF * G, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+    constexpr int PPW = rhsPointsPerLane<DIM, G>();
+    if constexpr (PPW > 1)
+    {
+        constexpr int NW = (G + PPW - 1) / PPW;
+        auto kern = kRHSFaceW<DIM, NDOF, G, RS, BATCH, PPW>;
+        if (smem > 65536)
+            DNDSR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        kern<<<gridFor(c.m.F, kTileF), kTileF * NW, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+    }
+    else
+    {
+        auto kern = kRHSFace<DIM, NDOF, G, RS, BATCH>;
+        if (smem > 65536)
+            DNDSR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
+        kern<<<gridFor(c.m.F, kTileF), kTileF * G, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+    }
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -545,7 +894,8 @@ static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *
         throw std::runtime_error("EvaluateRHS supports intOrder 2..5 (2/3 face points in 2-D, 4/9 in 3-D)");
 }
 
-void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs)
+void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs,
+                   const StageUpdate *stage)
 {
     DevMesh &m = c.m;
     real *ff = c.faceFlux;
@@ -569,7 +919,7 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
             rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
     }
     KernelTimer kt2(c, K_RHS_GATHER);
-    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs);
+    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{});
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -773,13 +1123,17 @@ void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float 
 // time loop never waits on the host for dtMin.
 __global__ void __launch_bounds__(256) kAxpby(idx n, real a, const real *__restrict__ x, real b,
                                               const real *__restrict__ y, real cc, const real *__restrict__ z,
-                                              const real *__restrict__ dtDev, real *__restrict__ out)
+                                              const real *__restrict__ dtDev, real *__restrict__ out,
+                                              real *__restrict__ save)
 {
     idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n)
         return;
     real dt = dtDev ? dtDev[0] : 1.0;
-    real r = a * x[i];
+    real xi = x[i];
+    if (save)
+        save[i] = xi;
+    real r = a * xi;
     if (y)
         r += b * y[i];
     if (z)
@@ -787,10 +1141,48 @@ __global__ void __launch_bounds__(256) kAxpby(idx n, real a, const real *__restr
     out[i] = r;
 }
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
-                 const real *dtDev, real *out)
+                 const real *dtDev, real *out, real *save)
 {
     KernelTimer kt(c, K_AXPBY);
-    kAxpby<<<gridFor(n, 256), 256, 0, c.stream>>>(n, a, x, b, y, cdt, z, dtDev, out);
+    kAxpby<<<gridFor(n, 256), 256, 0, c.stream>>>(n, a, x, b, y, cdt, z, dtDev, out, save);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+// Device-to-device copy of n doubles as one kernel on the library's stream (the runtime's
+// blit path splits one large copy into several dispatches).  16-byte lanes where both
+// pointers allow it.
+__global__ void __launch_bounds__(256) kCopy2(size_t n2, const double2 *__restrict__ src, double2 *__restrict__ dst)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n2; i += stride)
+        dst[i] = src[i];
+}
+__global__ void __launch_bounds__(256) kCopy1(size_t n, const real *__restrict__ src, real *__restrict__ dst)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride)
+        dst[i] = src[i];
+}
+void launchCopy(Ctx &c, size_t n, const real *src, real *dst)
+{
+    if (n == 0 || src == dst)
+        return;
+    KernelTimer kt(c, K_AXPBY);
+    const bool vec = ((reinterpret_cast<uintptr_t>(src) | reinterpret_cast<uintptr_t>(dst)) & 15) == 0;
+    const size_t work = vec ? n / 2 : n;
+    const unsigned nb = (unsigned)std::min<size_t>((work + 255) / 256, 8192);
+    if (vec)
+    {
+        kCopy2<<<nb, 256, 0, c.stream>>>(n / 2, reinterpret_cast<const double2 *>(src),
+                                         reinterpret_cast<double2 *>(dst));
+        DNDSR_HIP_CHECK(hipGetLastError());
+        if (n & 1)
+            kCopy1<<<1, 64, 0, c.stream>>>(1, src + n - 1, dst + n - 1);
+    }
+    else
+        kCopy1<<<nb, 256, 0, c.stream>>>(n, src, dst);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 

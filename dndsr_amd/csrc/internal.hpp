@@ -178,8 +178,16 @@ void launchVRIterDiff(Ctx &c, int nVars, const real *uRecDiff, real *uRecNew);
 // ---- launchers (euler.hip)
 void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,
                   real *lambdaFace, real *deltaLambdaCell, real *dtMinDev);
+// Optional stage combination fused into the RHS gather: u = a u + b y + cdt dt RHS (y, dtDev
+// optional; save receives u first).  The face pass reads u before the gather starts.
+struct StageUpdate
+{
+    real a = 0, b = 0, cdt = 0;
+    const real *y = nullptr, *dtDev = nullptr;
+    real *u = nullptr, *save = nullptr;
+};
 void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *deltaLambdaCell,
-                   real *rhs);
+                   real *rhs, const StageUpdate *stage = nullptr);
 void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag);
 void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
@@ -187,8 +195,11 @@ void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *r
 void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
                       const float *uInc, float *uIncNew, const float *jdiag, const float *lambdaFace, int forward,
                       int gsUpdate, int uIncIsZero, real *sumIncDev);
+// out = a x + b y + cdt dt z (y, z, dtDev optional); save, when given, receives x first
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
-                 const real *dtDev, real *out);
+                 const real *dtDev, real *out, real *save = nullptr);
+// dst[0:n) = src[0:n) on the library's stream
+void launchCopy(Ctx &c, size_t n, const real *src, real *dst);
 void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs,
                         real *resDev);
 void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du,

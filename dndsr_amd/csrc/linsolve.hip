@@ -237,7 +237,7 @@ void applyLUSGSPrecond(Ctx &c, const GasParams &gp, const LinWork &w, const real
                 std::swap(x, t);
             }
         if (x != out) // an odd number of sweeps leaves the result in the scratch array
-            DNDSR_HIP_CHECK(hipMemcpyAsync(out, x, nU * sizeof(real), hipMemcpyDeviceToDevice, c.stream));
+            launchCopy(c, nU, x, out);
     }
 }
 

@@ -7,102 +7,10 @@
 // with the defaults of the path (isotropic cell-aligned h-box basis scaling,
 // BaryDiff functional scale, Factorial / HQM_OPT direction weights, GWNone /
 // HQM_SD geometric weights, no Green-Gauss blending, svdTolerance 0).
-#include "internal.hpp"
+#include "geom.hpp"
 
 namespace dndsr
 {
-// Gauss-Legendre tables as the reference stores them (15 digits),
-// src/Geom/Quadratures/Line.hpp:25-42.
-__constant__ real cGLp[4][4] = {{0, 0, 0, 0},
-                                {-0.577350269189626, 0.577350269189626, 0, 0},
-                                {-0.774596669241483, 0, 0.774596669241483, 0},
-                                {-0.861136311594053, -0.339981043584856, 0.339981043584856, 0.861136311594053}};
-__constant__ real cGLw[4][4] = {{2, 0, 0, 0},
-                                {1, 1, 0, 0},
-                                {0.555555555555555, 0.888888888888889, 0.555555555555555, 0},
-                                {0.347854845137454, 0.652145154862546, 0.652145154862546, 0.347854845137454}};
-
-// points per direction for an integration order (QuadratureHub.hpp:39-96)
-__host__ __device__ inline int glN(int order) { return order <= 1 ? 1 : order <= 3 ? 2 : order <= 5 ? 3 : 4; }
-__host__ __device__ inline int paramDim(int t) { return t == E_LINE2 ? 1 : t == E_QUAD4 ? 2 : 3; }
-__host__ __device__ inline int nodeCount(int t) { return t == E_LINE2 ? 2 : t == E_QUAD4 ? 4 : 8; }
-__host__ __device__ inline int ruleSize(int t, int order)
-{
-    int n = glN(order), pd = paramDim(t);
-    return pd == 1 ? n : pd == 2 ? n * n : n * n * n;
-}
-// tensor Gauss-Legendre point iG (first parametric index fastest, QuadratureHub.hpp:318-373)
-__device__ inline void rulePoint(int t, int order, int iG, real *xi, real &w)
-{
-    int n = glN(order), pd = paramDim(t);
-    int i = iG % n, j = (iG / n) % n, k = iG / (n * n);
-    xi[0] = cGLp[n - 1][i], xi[1] = 0, xi[2] = 0;
-    w = cGLw[n - 1][i];
-    if (pd >= 2)
-        xi[1] = cGLp[n - 1][j], w = cGLw[n - 1][i] * cGLw[n - 1][j];
-    if (pd == 3)
-        xi[2] = cGLp[n - 1][k], w = cGLw[n - 1][i] * cGLw[n - 1][j] * cGLw[n - 1][k];
-}
-
-// Linear element map: x(xi) and J = dx/dxi (CGNS node order)
-__device__ inline void mapLinear(int t, const real (*X)[3], const real *xi, real *x, real (*J)[3])
-{
-    for (int r = 0; r < 3; r++)
-    {
-        x[r] = 0;
-        for (int c = 0; c < 3; c++)
-            J[r][c] = 0;
-    }
-    if (t == E_LINE2)
-    {
-        real N0 = 0.5 * (1 - xi[0]), N1 = 0.5 * (1 + xi[0]);
-        for (int r = 0; r < 3; r++)
-            x[r] = N0 * X[0][r] + N1 * X[1][r], J[r][0] = -0.5 * X[0][r] + 0.5 * X[1][r];
-    }
-    else if (t == E_QUAD4)
-    {
-        const real q[4][2] = {{-1, -1}, {1, -1}, {1, 1}, {-1, 1}};
-        for (int a = 0; a < 4; a++)
-        {
-            real N = 0.25 * (1 + q[a][0] * xi[0]) * (1 + q[a][1] * xi[1]);
-            real d0 = 0.25 * q[a][0] * (1 + q[a][1] * xi[1]);
-            real d1 = 0.25 * q[a][1] * (1 + q[a][0] * xi[0]);
-            for (int r = 0; r < 3; r++)
-                x[r] += N * X[a][r], J[r][0] += d0 * X[a][r], J[r][1] += d1 * X[a][r];
-        }
-    }
-    else
-    {
-        const real h[8][3] = {{-1, -1, -1}, {1, -1, -1}, {1, 1, -1}, {-1, 1, -1}, {-1, -1, 1}, {1, -1, 1}, {1, 1, 1}, {-1, 1, 1}};
-        for (int a = 0; a < 8; a++)
-        {
-            real fx = 1 + h[a][0] * xi[0], fy = 1 + h[a][1] * xi[1], fz = 1 + h[a][2] * xi[2];
-            real N = 0.125 * fx * fy * fz;
-            real d0 = 0.125 * h[a][0] * fy * fz, d1 = 0.125 * h[a][1] * fx * fz, d2 = 0.125 * h[a][2] * fx * fy;
-            for (int r = 0; r < 3; r++)
-                x[r] += N * X[a][r], J[r][0] += d0 * X[a][r], J[r][1] += d1 * X[a][r], J[r][2] += d2 * X[a][r];
-        }
-    }
-}
-__device__ inline real cellDet(int dim, const real (*J)[3])
-{
-    if (dim == 2)
-        return J[0][0] * J[1][1] - J[0][1] * J[1][0];
-    return J[0][0] * (J[1][1] * J[2][2] - J[1][2] * J[2][1]) - J[0][1] * (J[1][0] * J[2][2] - J[1][2] * J[2][0]) +
-           J[0][2] * (J[1][0] * J[2][1] - J[1][1] * J[2][0]);
-}
-__device__ inline void faceNormal(int dim, const real (*J)[3], real *n)
-{
-    if (dim == 2)
-    {
-        n[0] = J[1][0], n[1] = -J[0][0], n[2] = 0;
-        return;
-    }
-    n[0] = J[1][0] * J[2][1] - J[2][0] * J[1][1];
-    n[1] = J[2][0] * J[0][1] - J[0][0] * J[2][1];
-    n[2] = J[0][0] * J[1][1] - J[1][0] * J[0][1];
-}
-
 // ---------------------------------------------------------------------------
 // cell metrics: volume, quadrature points, w*JDet, barycentre, centre, scale, moments
 // ---------------------------------------------------------------------------
@@ -208,21 +116,18 @@ __global__ void __launch_bounds__(128) kFaceMetrics(DevMesh m)
             X[a][r] = m.coords[(size_t)r * m.nNode + nd];
     }
     real area = 0, mn[3] = {0, 0, 0};
+    const int n1d = glN(m.s.intOrder);
     for (int g = 0; g < m.G; g++)
     {
-        real xi[3], w, x[3], J[3][3], n[3];
-        rulePoint(t, m.s.intOrder, g, xi, w);
-        mapLinear(t, X, xi, x, J);
-        faceNormal(DIM, J, n);
-        real jd = sqrt(n[0] * n[0] + n[1] * n[1] + n[2] * n[2]);
+        real x[3], nu[3], wj, jd, w;
+        facePoint(DIM, n1d, g, X, x, nu, wj, jd, w);
         area += (1 * jd) * w;
-        m.fwj[(size_t)g * m.F + f] = (jd + kVerySmall) * w;
+        m.fwj[(size_t)g * m.F + f] = wj;
         for (int r = 0; r < 3; r++)
         {
-            real nu = n[r] / jd;
             m.fpts[((size_t)g * 3 + r) * m.F + f] = x[r];
-            m.fnorm[((size_t)g * 3 + r) * m.F + f] = nu;
-            mn[r] += (nu * jd) * w;
+            m.fnorm[((size_t)g * 3 + r) * m.F + f] = nu[r];
+            mn[r] += (nu[r] * jd) * w;
         }
     }
     m.farea[f] = area + kVerySmall;
