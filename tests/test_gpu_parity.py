@@ -17,7 +17,7 @@ pytestmark = pytest.mark.gpu
 
 TOL = 1e-12
 
-CASES = [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 1), ("3d", 2)]
+CASES = [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 1), ("3d", 2), ("3d", 3)]
 
 
 def _pair(kind, order, hqm=False, jitter=0.15):
