@@ -14,6 +14,10 @@ Node order and face-normal conventions:
     Faces are emitted so that this normal points from the back cell to the front cell.
   * cell2face follows the element face order: Quad4 (y-, x+, y+, x-),
     Hex8 (z-, y-, x+, y+, x-, z+).
+Boundary faces (``box_quad_grid``, ``box_hex_grid``): a direction that is not
+periodic ends in boundary faces with ``face2cell[f] = (cell, -1)``, the normal
+pointing out of the domain, and a zone index ``faceZone[f]`` (the mesh's face zone,
+UnstructuredMesh::GetFaceZone) per side.
 Cells are numbered lexicographically (x fastest).  Faces are numbered by blocks of
 64 consecutive owner cells: within a block, the 64 +x faces, then the 64 +y faces
 (then +z), so a run of 64 faces is one direction over 64 consecutive cells (one
@@ -61,6 +65,8 @@ class Mesh:
     face2cell: np.ndarray     # [nFace, 2] int64, -1 = boundary
     faceShift: np.ndarray     # [nFace, 3] float64
     shape: tuple = ()
+    faceZone: np.ndarray | None = None  # [nFace] int32: boundary zone of faces with face2cell[:, 1] = -1
+    nBCZone: int = 0
 
     @property
     def nCell(self) -> int:
@@ -171,3 +177,199 @@ def periodic_hex_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.
                 cell2face.ravel().astype(np.int64), np.full(F, QUAD4, np.int32),
                 np.arange(F + 1, dtype=np.int64) * 4, face2node.ravel().astype(np.int64),
                 face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny, nz))
+
+
+def _jitter_nodes_box(X: np.ndarray, h: np.ndarray, jitter: float, seed: int, periodic) -> np.ndarray:
+    """Node perturbation for partly periodic grids: periodic directions move node n and
+    n + N_d together; in a bounded direction the nodes on its two end planes keep that
+    coordinate (the domain keeps its shape)."""
+    if jitter == 0.0:
+        return X
+    rng = np.random.default_rng(seed)
+    sh = X.shape[:-1]
+    dim = len(sh)
+    d = rng.uniform(-jitter, jitter, size=sh + (3,)) * np.concatenate([h, np.zeros(3 - dim)])
+    if dim == 2:
+        d[..., 2] = 0.0
+    for a in range(dim):
+        if periodic[a]:
+            src = [slice(None)] * dim
+            dst = [slice(None)] * dim
+            src[a], dst[a] = 0, sh[a] - 1
+            d[tuple(dst)] = d[tuple(src)]
+        else:
+            for end in (0, sh[a] - 1):
+                sl = [slice(None)] * dim
+                sl[a] = end
+                d[tuple(sl) + (a,)] = 0.0
+    return X + d
+
+
+def _zone_of(zones, key):
+    if zones is None or key not in zones:
+        raise ValueError(f"a bounded direction needs a zone for side {key!r}")
+    return int(zones[key])
+
+
+def box_quad_grid(nx: int, ny: int, Lx: float = 1.0, Ly: float = 1.0, x0: float = 0.0, y0: float = 0.0,
+                  periodic=(False, False), zones: dict | None = None, jitter: float = 0.0, seed: int = 0,
+                  node_map=None) -> Mesh:
+    """nx x ny Quad4 grid on [x0, x0+Lx] x [y0, y0+Ly]; each direction periodic or bounded.
+    zones: {'xmin', 'xmax', 'ymin', 'ymax'} -> zone index for the bounded sides.
+    node_map: optional f(coords[nNode, 3]) -> coords applied after the jitter."""
+    assert nx >= 2 and ny >= 2
+    px, py = bool(periodic[0]), bool(periodic[1])
+    hx, hy = Lx / nx, Ly / ny
+    I, J = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, np.zeros_like(I, dtype=float)], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([hx, hy]), jitter, seed, (px, py))
+    coords = X.reshape(-1, 3)
+    if node_map is not None:
+        coords = np.ascontiguousarray(node_map(coords.copy()), np.float64)
+    nid = lambda i, j: i * (ny + 1) + j  # noqa: E731
+    N = nx * ny
+    ci, cj = np.meshgrid(np.arange(nx), np.arange(ny), indexing="ij")
+    ci, cj = ci.T.ravel(), cj.T.ravel()
+    c = np.arange(N)
+    fx_nodes = np.stack([nid(ci + 1, cj), nid(ci + 1, cj + 1)], 1)
+    fx_R = ((ci + 1) % nx) + nx * cj
+    fx_shift = np.zeros((N, 3))
+    fy_nodes = np.stack([nid(ci + 1, cj + 1), nid(ci, cj + 1)], 1)
+    fy_R = ci + nx * ((cj + 1) % ny)
+    fy_shift = np.zeros((N, 3))
+    zone = np.full(2 * N, -1, np.int32)
+    if px:
+        fx_shift[ci == nx - 1, 0] = -Lx
+    else:
+        fx_R = np.where(ci == nx - 1, -1, fx_R)
+        zone[:N][ci == nx - 1] = _zone_of(zones, "xmax")
+    if py:
+        fy_shift[cj == ny - 1, 1] = -Ly
+    else:
+        fy_R = np.where(cj == ny - 1, -1, fy_R)
+        zone[N:][cj == ny - 1] = _zone_of(zones, "ymax")
+    face2node = np.concatenate([fx_nodes, fy_nodes])
+    face2cell = np.stack([np.concatenate([c, c]), np.concatenate([fx_R, fy_R])], 1)
+    faceShift = np.concatenate([fx_shift, fy_shift])
+    xm = ((ci - 1) % nx) + nx * cj
+    ym = ci + nx * ((cj - 1) % ny)
+    cell2face = np.stack([N + ym, c, N + c, xm], 1)   # y-, x+, y+, x-
+    face2node, face2cell, faceShift, cell2face = _by_cell_block(2, N, face2node, face2cell, faceShift, cell2face)
+    zone = zone[np.argsort(_block_perm(2, N))]
+    extra_n, extra_c, extra_z = [], [], []
+    F = 2 * N
+    if not px:  # x- boundary faces, normal -x
+        cells = c[ci == 0]
+        extra_n.append(np.stack([nid(0 * cells, cells // nx + 1), nid(0 * cells, cells // nx)], 1))
+        extra_c.append(cells)
+        extra_z.append(np.full(len(cells), _zone_of(zones, "xmin"), np.int32))
+        cell2face[cells, 3] = F + np.arange(len(cells))
+        F += len(cells)
+    if not py:  # y- boundary faces, normal -y
+        cells = c[cj == 0]
+        extra_n.append(np.stack([nid(cells % nx, 0 * cells), nid(cells % nx + 1, 0 * cells)], 1))
+        extra_c.append(cells)
+        extra_z.append(np.full(len(cells), _zone_of(zones, "ymin"), np.int32))
+        cell2face[cells, 0] = F + np.arange(len(cells))
+        F += len(cells)
+    if extra_n:
+        en = np.concatenate(extra_n)
+        ec = np.concatenate(extra_c)
+        face2node = np.concatenate([face2node, en])
+        face2cell = np.concatenate([face2cell, np.stack([ec, -np.ones_like(ec)], 1)])
+        faceShift = np.concatenate([faceShift, np.zeros((len(ec), 3))])
+        zone = np.concatenate([zone] + extra_z)
+    nz = int(zone.max()) + 1 if (zone >= 0).any() else 0
+    return Mesh(2, coords, np.full(N, QUAD4, np.int32), np.arange(N + 1, dtype=np.int64) * 4,
+                np.stack([nid(ci, cj), nid(ci + 1, cj), nid(ci + 1, cj + 1), nid(ci, cj + 1)], 1).ravel().astype(np.int64),
+                np.arange(N + 1, dtype=np.int64) * 4, cell2face.ravel().astype(np.int64),
+                np.full(F, LINE2, np.int32), np.arange(F + 1, dtype=np.int64) * 2, face2node.ravel().astype(np.int64),
+                face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny),
+                None if nz == 0 else zone.astype(np.int32), nz)
+
+
+def _block_perm(dim, N, B=FACE_BLOCK):
+    """New index of direction-major face d N + c under the block-direction-major numbering."""
+    f = np.arange(dim * N)
+    c, d = f % N, f // N
+    k = c // B
+    r = np.minimum(B, N - k * B)
+    return k * B * dim + d * r + (c - k * B)
+
+
+def box_hex_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.0, Lz: float = 1.0,
+                 x0: float = 0.0, y0: float = 0.0, z0: float = 0.0, periodic=(False, False, False),
+                 zones: dict | None = None, jitter: float = 0.0, seed: int = 0, node_map=None) -> Mesh:
+    """nx x ny x nz Hex8 grid; each direction periodic or bounded.  zones: {'xmin', 'xmax',
+    'ymin', 'ymax', 'zmin', 'zmax'} -> zone index for the bounded sides."""
+    assert nx >= 2 and ny >= 2 and nz >= 2
+    per = [bool(p) for p in periodic]
+    hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, z0 + K * hz], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([hx, hy, hz]), jitter, seed, per)
+    coords = X.reshape(-1, 3)
+    if node_map is not None:
+        coords = np.ascontiguousarray(node_map(coords.copy()), np.float64)
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k  # noqa: E731
+    N = nx * ny * nz
+    c = np.arange(N)
+    ci, cj, ck = c % nx, (c // nx) % ny, c // (nx * ny)
+    cell2node = np.stack([nid(ci, cj, ck), nid(ci + 1, cj, ck), nid(ci + 1, cj + 1, ck), nid(ci, cj + 1, ck),
+                          nid(ci, cj, ck + 1), nid(ci + 1, cj, ck + 1), nid(ci + 1, cj + 1, ck + 1),
+                          nid(ci, cj + 1, ck + 1)], 1)
+    cid = lambda i, j, k: (i % nx) + nx * ((j % ny) + ny * (k % nz))  # noqa: E731
+    fx = np.stack([nid(ci + 1, cj, ck), nid(ci + 1, cj + 1, ck), nid(ci + 1, cj + 1, ck + 1), nid(ci + 1, cj, ck + 1)], 1)
+    fy = np.stack([nid(ci, cj + 1, ck), nid(ci, cj + 1, ck + 1), nid(ci + 1, cj + 1, ck + 1), nid(ci + 1, cj + 1, ck)], 1)
+    fz = np.stack([nid(ci, cj, ck + 1), nid(ci + 1, cj, ck + 1), nid(ci + 1, cj + 1, ck + 1), nid(ci, cj + 1, ck + 1)], 1)
+    idxs = (ci, cj, ck)
+    ns = (nx, ny, nz)
+    Ls = (Lx, Ly, Lz)
+    R = [cid(ci + 1, cj, ck), cid(ci, cj + 1, ck), cid(ci, cj, ck + 1)]
+    sh = [np.zeros((N, 3)) for _ in range(3)]
+    zone = np.full(3 * N, -1, np.int32)
+    for a, key in enumerate(("xmax", "ymax", "zmax")):
+        last = idxs[a] == ns[a] - 1
+        if per[a]:
+            sh[a][last, a] = -Ls[a]
+        else:
+            R[a] = np.where(last, -1, R[a])
+            zone[a * N:(a + 1) * N][last] = _zone_of(zones, key)
+    face2node = np.concatenate([fx, fy, fz])
+    face2cell = np.stack([np.concatenate([c, c, c]), np.concatenate(R)], 1)
+    faceShift = np.concatenate(sh)
+    xm, ym, zm = cid(ci - 1, cj, ck), cid(ci, cj - 1, ck), cid(ci, cj, ck - 1)
+    cell2face = np.stack([2 * N + zm, N + ym, c, N + c, xm, 2 * N + c], 1)  # z-, y-, x+, y+, x-, z+
+    face2node, face2cell, faceShift, cell2face = _by_cell_block(3, N, face2node, face2cell, faceShift, cell2face)
+    zone = zone[np.argsort(_block_perm(3, N))]
+    F = 3 * N
+    extra_n, extra_c, extra_z = [], [], []
+    # minus-side boundary faces with outward normals; (axis, zone key, cell2face slot, node builder)
+    minus = [
+        (0, "xmin", 4, lambda i, j, k: np.stack([nid(i, j, k), nid(i, j, k + 1), nid(i, j + 1, k + 1), nid(i, j + 1, k)], 1)),
+        (1, "ymin", 1, lambda i, j, k: np.stack([nid(i, j, k), nid(i + 1, j, k), nid(i + 1, j, k + 1), nid(i, j, k + 1)], 1)),
+        (2, "zmin", 0, lambda i, j, k: np.stack([nid(i, j, k), nid(i, j + 1, k), nid(i + 1, j + 1, k), nid(i + 1, j, k)], 1)),
+    ]
+    for a, key, slot, nodes in minus:
+        if per[a]:
+            continue
+        sel = idxs[a] == 0
+        cells = c[sel]
+        extra_n.append(nodes(ci[sel], cj[sel], ck[sel]))
+        extra_c.append(cells)
+        extra_z.append(np.full(len(cells), _zone_of(zones, key), np.int32))
+        cell2face[cells, slot] = F + np.arange(len(cells))
+        F += len(cells)
+    if extra_n:
+        ec = np.concatenate(extra_c)
+        face2node = np.concatenate([face2node] + extra_n)
+        face2cell = np.concatenate([face2cell, np.stack([ec, -np.ones_like(ec)], 1)])
+        faceShift = np.concatenate([faceShift, np.zeros((len(ec), 3))])
+        zone = np.concatenate([zone] + extra_z)
+    nzone = int(zone.max()) + 1 if (zone >= 0).any() else 0
+    return Mesh(3, coords, np.full(N, HEX8, np.int32), np.arange(N + 1, dtype=np.int64) * 8,
+                cell2node.ravel().astype(np.int64), np.arange(N + 1, dtype=np.int64) * 6,
+                cell2face.ravel().astype(np.int64), np.full(F, QUAD4, np.int32),
+                np.arange(F + 1, dtype=np.int64) * 4, face2node.ravel().astype(np.int64),
+                face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny, nz),
+                None if nzone == 0 else zone.astype(np.int32), nzone)

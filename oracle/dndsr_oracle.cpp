@@ -709,6 +709,14 @@ struct VR
     std::vector<real> Ainvb; // [nCell][maxFC][M]
     int maxFC{0};
     real globalVol{0};
+    // boundary zones (faces with f2c[f][1] < 0) and their conditions
+    std::vector<int32_t> faceZone;
+    int nBCZone{0};
+    std::vector<int32_t> bcType;              // EulerBCType per zone (empty: no Euler BC table set)
+    std::vector<std::array<real, 5>> bcValue; // per zone
+    real bcGamma{1.4};
+    int dirNVars{0};
+    std::vector<real> dirValue;               // [nFace][G][dirNVars] boundary point values
 
     bool cellIsBack(idx_t iCell, idx_t iFace) const { return f2c[iFace][0] == iCell; }
     idx_t cellFaceOther(idx_t iCell, idx_t iFace) const { return f2c[iFace][0] == iCell ? f2c[iFace][1] : f2c[iFace][0]; }
@@ -876,6 +884,15 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
         v.f2c[f] = {m->face2cell[2 * f], m->face2cell[2 * f + 1]};
         v.fshift[f] = toPt(m->faceShift + 3 * f);
     }
+    v.nBCZone = m->nBCZone;
+    v.faceZone.assign(v.nFace, -1);
+    for (idx_t f = 0; f < v.nFace; f++)
+        if (v.f2c[f][1] < 0)
+        {
+            if (!m->faceZone || m->faceZone[f] < 0 || m->faceZone[f] >= m->nBCZone)
+                throw std::runtime_error("boundary face without a valid zone");
+            v.faceZone[f] = m->faceZone[f];
+        }
 
     // ---- cells: FiniteVolume.cpp:23-326
     v.vol.resize(v.nCell);
@@ -1065,6 +1082,11 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
             default: throw std::runtime_error("HQM_OPT needs maxOrder 1..3");
             }
         }
+        // boundary faces: id2faceDircWeight(zone, order) * bcWeight, VariationalReconstruction.cpp:438-441
+        if (cR < 0)
+            for (int p = 0; p <= v.maxOrder; p++)
+                wd[p] *= (s->zoneDirWeight ? s->zoneDirWeight[(size_t)v.faceZone[f] * (v.maxOrder + 1) + p] : 1.0) *
+                         s->bcWeight;
         for (int p = 0; p <= v.maxOrder; p++)
             v.faceWeight[f][p] = wd[p] * wg;
     }
@@ -1179,6 +1201,181 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
         throw std::runtime_error(firstErr);
 }
 
+
+// ---------------------------------------------------------------------------
+// Boundary conditions
+// ---------------------------------------------------------------------------
+// IdealGasThermalConservative2Primitive / Primitive2Conservative, Gas.hpp:279-317
+// (three velocity components: the NS models carry rho w also in 2-D)
+static void cons2prim(const real *U, real *prim, real gamma)
+{
+    for (int k = 0; k < 5; k++)
+        prim[k] = U[k] / U[0];
+    real vSqr = sqr(U[1] / U[0]) + sqr(U[2] / U[0]) + sqr(U[3] / U[0]);
+    real rho = U[0], E = U[4];
+    prim[0] = rho;
+    prim[4] = (gamma - 1) * (E - rho * 0.5 * vSqr);
+}
+static void prim2cons(const real *prim, real *U, real gamma)
+{
+    for (int k = 0; k < 5; k++)
+        U[k] = prim[k] * prim[0];
+    real vSqr = sqr(prim[1]) + sqr(prim[2]) + sqr(prim[3]);
+    real rho = prim[0], p = prim[4];
+    U[0] = rho;
+    U[4] = p / (gamma - 1) + (rho * 0.5 * vSqr);
+}
+
+// generateBoundaryValue (EulerEvaluator_EvaluateDt.hxx:1743-1838) for the inviscid types,
+// static frame, no CL driver / anchors: generateBV_FarField :1870-1950 (BCFar, BCOutP),
+// generateBV_InviscidWall :2199-2215 (BCWallInvis, BCSym), generateBV_ViscousWall :2241-2270
+// (BCWall, velocity reversed), generateBV_Outflow :2346-2353, generateBV_Inflow :2377-2390.
+static void bcGhost(int type, const real *UL, const real *n, const real *far, real gamma, real *UR)
+{
+    switch (type)
+    {
+    case ORC_BC_FAR:
+    case ORC_BC_OUTP:
+    {
+        real un = (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) / UL[0];
+        real p, asqr, H;
+        thermal(UL[4], UL[0], sqr(UL[1] / UL[0]) + sqr(UL[2] / UL[0]) + sqr(UL[3] / UL[0]), gamma, p, asqr, H);
+        if (!(asqr >= 0))
+            throw std::runtime_error("far-field ghost: negative a^2");
+        real a = std::sqrt(asqr);
+        const real vgN = 0;
+        if (un - vgN - a > 0) // supersonic outflow
+            for (int k = 0; k < 5; k++)
+                UR[k] = UL[k];
+        else if (un - vgN > 0) // subsonic outflow: far pressure
+        {
+            real fp[5], lp[5];
+            cons2prim(far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            lp[4] = fp[4];
+            prim2cons(lp, UR, gamma);
+        }
+        else if (un - vgN + a > 0) // subsonic inflow: interior pressure
+        {
+            real fp[5], lp[5];
+            cons2prim(far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            fp[4] = lp[4];
+            prim2cons(fp, UR, gamma);
+        }
+        else // supersonic inflow
+            for (int k = 0; k < 5; k++)
+                UR[k] = far[k];
+        break;
+    }
+    case ORC_BC_WALL_INVIS:
+    case ORC_BC_SYM:
+    {
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        real vn = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] -= 2 * vn * n[d];
+        break;
+    }
+    case ORC_BC_WALL:
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] *= -1;
+        break;
+    case ORC_BC_OUT:
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        break;
+    case ORC_BC_IN:
+        for (int k = 0; k < 5; k++)
+            UR[k] = far[k];
+        break;
+    default:
+        throw std::runtime_error("boundary type not supported on this path");
+    }
+}
+
+// CompressRecPart, EulerEvaluator.hpp:1541-1631: the reconstructed point state, or the
+// cell mean where it would carry negative density or internal energy
+static void compressRecPart(const real *umean, const real *inc, real *ret)
+{
+    for (int k = 0; k < 5; k++)
+        ret[k] = umean[k] + inc[k];
+    if (ret[0] < 0)
+        for (int k = 0; k < 5; k++)
+            ret[k] = umean[k];
+    real eK = (sqr(ret[1]) + sqr(ret[2]) + sqr(ret[3])) * 0.5 / (verySmallReal + ret[0]);
+    if (ret[4] - eK < 0)
+        for (int k = 0; k < 5; k++)
+            ret[k] = umean[k];
+}
+
+// Boundary term of the VR sweep for boundary face f of cell i (GetBoundaryRHS,
+// VariationalReconstruction_Reconstruction.hxx:19-96, intOrderVRBC = intOrder):
+//   BCC += sum_g FFaceFunctional(D0phi(x_g), uB(x_g) - u_i) JDet w_g
+// with uB the ghost of the compressed point state (Euler BC table, nVars = 5; the
+// solver's FBoundary, EulerSolver.hxx:163-175) or the given point value.  With
+// uRecDiff the linearised form (GetBoundaryRHSDiff :98-176, FBoundaryDiff
+// EulerSolver.hxx:176-193): uB(uBL + dU) - uB(uBL), no subtraction of u_i.
+static void boundaryBCC(const VR &v, idx_t i, idx_t f, int nVars, const real *uRec, const real *u,
+                        const real *uRecDiff, std::vector<real> &BCC)
+{
+    const int M = v.M;
+    const size_t blkN = (size_t)M * nVars, G = v.faceQw.size();
+    const int zone = v.faceZone[f];
+    const bool euler = !v.bcType.empty() && nVars == 5;
+    if (!euler && !(v.dirNVars == nVars && !v.dirValue.empty()))
+        throw std::runtime_error("boundary values are not set for this number of variables");
+    std::vector<real> D, inc(nVars), phi(M);
+    for (size_t g = 0; g < G; g++)
+    {
+        v.diffBase(i, v.faceQP[f][g], 1, 0, D);
+        for (int k = 0; k < M; k++)
+            phi[k] = D[1 + k];
+        real uBL[5], dU[5], uB[5];
+        for (int var = 0; var < nVars; var++)
+        {
+            real r = 0, rd = 0;
+            for (int k = 0; k < M; k++)
+            {
+                r += phi[k] * uRec[i * blkN + (size_t)var * M + k];
+                if (uRecDiff)
+                    rd += phi[k] * uRecDiff[i * blkN + (size_t)var * M + k];
+            }
+            if (var < 5)
+                uBL[var] = r, dU[var] = rd;
+            inc[var] = r;
+        }
+        if (euler)
+        {
+            const real *n = v.faceN[f][g].x;
+            const real *far = v.bcValue[zone].data();
+            real ULf[5];
+            compressRecPart(u + i * 5, uBL, ULf);
+            bcGhost(v.bcType[zone], ULf, n, far, v.bcGamma, uB);
+            if (uRecDiff)
+            {
+                real inc2[5], ULp[5], uBp[5];
+                for (int k = 0; k < 5; k++)
+                    inc2[k] = uBL[k] + dU[k];
+                compressRecPart(u + i * 5, inc2, ULp);
+                bcGhost(v.bcType[zone], ULp, n, far, v.bcGamma, uBp);
+                for (int k = 0; k < 5; k++)
+                    inc[k] = uBp[k] - uB[k];
+            }
+            else
+                for (int k = 0; k < 5; k++)
+                    inc[k] = uB[k] - u[i * 5 + k];
+        }
+        else
+            for (int var = 0; var < nVars; var++)
+                inc[var] = uRecDiff ? 0.0 : v.dirValue[((size_t)f * G + g) * nVars + var] - u[i * nVars + var];
+        v.faceFunctional(f, phi.data(), M, inc.data(), nVars, 1, BCC.data(), v.faceJD[f][g] * v.faceQw[g]);
+    }
+}
+
 // ---------------------------------------------------------------------------
 // Analytic fields (src/Euler/SpecialFields.hpp:41-75, EulerEvaluator.hxx:1039-1135,
 // test/cpp/CFV/test_Reconstruction.cpp:443-447, test_Reconstruction3D.cpp:321-325)
@@ -1210,6 +1407,9 @@ static void evalField(int id, const double *prm, const Pt &p, int nVars, real *o
     }
     case 4: // quadratic x^2 + y^2 (+ z^2)
         out[0] = p.x[0] * p.x[0] + p.x[1] * p.x[1] + prm[0] * p.x[2] * p.x[2];
+        return;
+    case 5: // cubic x^3 + x y^2 (test_Reconstruction.cpp:504-507) (+ z^3)
+        out[0] = p.x[0] * p.x[0] * p.x[0] + p.x[0] * p.x[1] * p.x[1] + prm[0] * p.x[2] * p.x[2] * p.x[2];
         return;
     case 10: // isentropic vortex on [0,10]^2: prm = {chi, gamma, ubx, uby}
     {
@@ -1408,19 +1608,29 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
     return err / v.globalVol;
 }
 
-// DoReconstructionIter (Jacobi branch, SORInstead = false, interior faces),
-// VariationalReconstruction_Reconstruction.hxx:485-615
+// DoReconstructionIter (Jacobi branch, SORInstead = false),
+// VariationalReconstruction_Reconstruction.hxx:485-615; boundary faces add A^-1 BCC
+static void addAinvBCC(const VR &v, idx_t i, int nVars, const std::vector<real> &BCC, real scale, real *out)
+{
+    const int M = v.M;
+    const real *Ainv = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
+    for (int var = 0; var < nVars; var++)
+        for (int r = 0; r < M; r++)
+        {
+            real acc = 0;
+            for (int k = 0; k < M; k++)
+                acc += Ainv[(size_t)r * M + k] * BCC[(size_t)k * nVars + var];
+            out[(size_t)var * M + r] += scale * acc;
+        }
+}
+
 int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew)
 {
+    ORC_TRY
     const VR &v = h->v;
     int M = v.M;
     size_t blkN = (size_t)M * nVars;
-    for (idx_t f = 0; f < v.nFace; f++)
-        if (v.f2c[f][1] < 0)
-        {
-            g_err = "boundary faces are not supported by the oracle VR iteration";
-            return -1;
-        }
+    std::string firstErr;
 #pragma omp parallel for schedule(static) num_threads(g_threads)
     for (idx_t i = 0; i < v.nCell; i++)
     {
@@ -1433,6 +1643,21 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
         {
             idx_t f = v.c2f[i][ic];
             idx_t o = v.cellFaceOther(i, f);
+            if (o < 0)
+            {
+                std::vector<real> BCC(blkN, 0.0);
+                try
+                {
+                    boundaryBCC(v, i, f, nVars, uRec, u, nullptr, BCC);
+                }
+                catch (const std::exception &e)
+                {
+#pragma omp critical
+                    firstErr = e.what();
+                }
+                addAinvBCC(v, i, nVars, BCC, relax, out);
+                continue;
+            }
             const real *B = blk + (1 + ic) * (size_t)M * M;
             const real *bv = &v.Ainvb[((size_t)i * v.maxFC + ic) * M];
             for (int var = 0; var < nVars; var++)
@@ -1449,12 +1674,17 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
             }
         }
     }
+    if (!firstErr.empty())
+        throw std::runtime_error(firstErr);
     return 0;
+    ORC_CATCH(-1)
 }
 
 // DoReconstructionIterDiff, VariationalReconstruction_Reconstruction.hxx:617-670
-int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double *uRecNew)
+int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const double *u, const double *uRecDiff,
+                     double *uRecNew)
 {
+    ORC_TRY
     const VR &v = h->v;
     int M = v.M;
     size_t blkN = (size_t)M * nVars;
@@ -1466,9 +1696,17 @@ int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double 
         const real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
         for (size_t ic = 0; ic < v.c2f[i].size(); ic++)
         {
-            idx_t o = v.cellFaceOther(i, v.c2f[i][ic]);
+            idx_t f = v.c2f[i][ic];
+            idx_t o = v.cellFaceOther(i, f);
             if (o < 0)
-                return -1;
+            {
+                if (!uRec || !u)
+                    throw std::runtime_error("the boundary term of the VR Jacobian needs uRec and u");
+                std::vector<real> BCC(blkN, 0.0);
+                boundaryBCC(v, i, f, nVars, uRec, u, uRecDiff, BCC);
+                addAinvBCC(v, i, nVars, BCC, -1.0, out);
+                continue;
+            }
             const real *B = blk + (1 + ic) * (size_t)M * M;
             for (int var = 0; var < nVars; var++)
                 for (int r = 0; r < M; r++)
@@ -1481,6 +1719,41 @@ int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double 
         }
     }
     return 0;
+    ORC_CATCH(-1)
+}
+
+int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma)
+{
+    ORC_TRY
+    VR &v = h->v;
+    if (nZone != v.nBCZone)
+        throw std::runtime_error("zone count differs from the mesh's");
+    v.bcType.assign(type, type + nZone);
+    v.bcValue.assign(nZone, {0, 0, 0, 0, 0});
+    for (int z = 0; z < nZone; z++)
+        for (int k = 0; k < 5; k++)
+            v.bcValue[z][k] = value ? value[z * 5 + k] : 0.0;
+    v.bcGamma = gamma;
+    return 0;
+    ORC_CATCH(-1)
+}
+
+int orc_set_vr_dirichlet(orc_vr *h, int nVars, const double *value)
+{
+    ORC_TRY
+    VR &v = h->v;
+    v.dirNVars = nVars;
+    v.dirValue.assign(value, value + (size_t)v.nFace * v.faceQw.size() * nVars);
+    return 0;
+    ORC_CATCH(-1)
+}
+
+int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double *UR)
+{
+    ORC_TRY
+    bcGhost(type, UL, n, zoneValue, gamma, UR);
+    return 0;
+    ORC_CATCH(-1)
 }
 
 int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const double *ULm,
@@ -1494,7 +1767,8 @@ int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const d
     ORC_CATCH(-1)
 }
 
-// EvaluateDt, EulerEvaluator_EvaluateDt.hxx:840-1048 (inviscid: lamVis = 0; interior faces)
+// EvaluateDt, EulerEvaluator_EvaluateDt.hxx:840-1048 (inviscid: lamVis = 0; a boundary face
+// takes UR = UL, :880-899)
 int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, double CFL,
                 double maxDt, int useLocalDt, double *dt, double *lambdaFace, double *deltaLambdaCell,
                 double *dtMin)
@@ -1562,8 +1836,8 @@ int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, 
 }
 
 // EvaluateRHS, EulerEvaluator_EvaluateRHS.hxx:55-818 with fluxFace
-// EulerEvaluator_EvaluateDt.hxx:1084-1511; inviscid, interior/periodic faces,
-// serial direct scatter (:550-562).
+// EulerEvaluator_EvaluateDt.hxx:1084-1511; inviscid; interior, periodic and boundary
+// faces (ghost states from the boundary-condition table); serial direct scatter (:550-562).
 int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
                  const double *deltaLambdaCell, double *rhs)
 {
@@ -1573,8 +1847,8 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
     size_t G = v.faceQw.size();
     std::fill(rhs, rhs + v.nCell * 5, 0.0);
     for (idx_t f = 0; f < v.nFace; f++)
-        if (v.f2c[f][1] < 0)
-            throw std::runtime_error("boundary faces are not supported by the oracle RHS");
+        if (v.f2c[f][1] < 0 && v.bcType.empty())
+            throw std::runtime_error("boundary faces need the boundary-condition table (orc_set_bc)");
     std::vector<std::array<real, 5>> fbuf(v.nFace);
     std::string firstErr;
 #pragma omp parallel num_threads(g_threads)
@@ -1588,7 +1862,15 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
       try
       {
         idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
-        const real *ULm = u + cL * 5, *URm = u + cR * 5;
+        const bool bnd = cR < 0;
+        const int zone = bnd ? v.faceZone[f] : -1;
+        const real *ULm = u + cL * 5, *URm = bnd ? nullptr : u + cR * 5;
+        real URmB[5];
+        if (bnd) // URMeanXy = ghost of the mean at the face centre, EvaluateRHS.hxx:194-207
+        {
+            bcGhost(v.bcType[zone], ULm, v.faceMeanNorm[f].x, v.bcValue[zone].data(), es->gamma, URmB);
+            URm = URmB;
+        }
         for (size_t g = 0; g < G; g++)
         {
             // ULxy = u[L] + (D0Bj(L) * uRec[L])^T, EvaluateRHS.hxx:235-243, 287-297
@@ -1602,21 +1884,28 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
                         r += D[1 + k] * uRec[((size_t)cL * 5 + var) * M + k];
                 UL[var] = ULm[var] + r;
             }
-            v.diffBase(cR, v.facePointFromSide(f, 1, v.faceQP[f][g]), 1, 0, D);
-            for (int var = 0; var < 5; var++)
+            if (bnd) // URxy = generateBoundaryValue(ULxy, ...), EvaluateRHS.hxx:336-345
+                bcGhost(v.bcType[zone], UL, v.faceN[f][g].x, v.bcValue[zone].data(), es->gamma, UR);
+            else
             {
-                real r = 0;
-                if (uRec)
-                    for (int k = 0; k < M; k++)
-                        r += D[1 + k] * uRec[((size_t)cR * 5 + var) * M + k];
-                UR[var] = URm[var] + r;
+                v.diffBase(cR, v.facePointFromSide(f, 1, v.faceQP[f][g]), 1, 0, D);
+                for (int var = 0; var < 5; var++)
+                {
+                    real r = 0;
+                    if (uRec)
+                        for (int k = 0; k < M; k++)
+                            r += D[1 + k] * uRec[((size_t)cR * 5 + var) * M + k];
+                    UR[var] = URm[var] + r;
+                }
             }
             for (int var = 0; var < 5; var++)
                 ULv[g][var] = UL[var], URv[g][var] = UR[var];
             for (int d = 0; d < 3; d++)
                 nv[g][d] = v.faceN[f][g].x[d];
         }
-        real dLambda = deltaLambdaCell ? std::max(deltaLambdaCell[cL], deltaLambdaCell[cR]) : 0.0;
+        real dLambda = 0.0; // fluxFace, EvaluateDt.hxx:1158-1160
+        if (deltaLambdaCell)
+            dLambda = bnd ? deltaLambdaCell[cL] : std::max(deltaLambdaCell[cL], deltaLambdaCell[cR]);
         real lam0, lam123, lam4;
         int rs = es->rsType;
         if (es->rsMeanValueEig != 0 && rs >= 11 && rs <= 15)
@@ -1659,7 +1948,8 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
             for (int var = 0; var < 5; var++)
             {
                 rhs[cL * 5 + var] += fbuf[f][var] / v.vol[cL];
-                rhs[cR * 5 + var] += (-fbuf[f][var]) / v.vol[cR];
+                if (cR >= 0)
+                    rhs[cR * 5 + var] += (-fbuf[f][var]) / v.vol[cR];
             }
         }
     }

@@ -39,6 +39,11 @@ typedef struct
     const int64_t *face2nodePtr, *face2node;
     const int64_t *face2cell;  /* [nFace][2]; -1 = boundary */
     const double *faceShift;   /* [nFace][3]: translation from the back (L) cell frame to the front (R) cell frame */
+    /* boundary zone (0 .. nBCZone-1) of every face with face2cell[f][1] = -1 (the mesh's
+     * face zone id, UnstructuredMesh::GetFaceZone); entries of interior faces are ignored.
+     * NULL when the mesh has no boundary face. */
+    const int32_t *faceZone;
+    int32_t nBCZone;
 } orc_mesh_desc;
 
 typedef struct
@@ -52,6 +57,11 @@ typedef struct
     double scaleMultiplier;
     double jacobiRelax;
     int nIterCellSmoothScale;
+    /* boundary faces: face weight of order p multiplied by zoneDirWeight[zone][p] * bcWeight
+     * (the id2faceDircWeight callback of ConstructBaseAndWeight,
+     * VariationalReconstruction.cpp:438-441); NULL = 1 for every zone and order */
+    double bcWeight;
+    const double *zoneDirWeight; /* [nBCZone][maxOrder+1] */
 } orc_vr_settings;
 
 typedef struct
@@ -91,8 +101,22 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
 
 /* VR iteration: one Jacobi sweep uRecNew = (1-w) uRec + w * sum(...) */
 int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew);
-/* uRecNew = uRecDiff - sum_f AinvB_f * uRecDiff_j (VR Jacobian action) */
-int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRecDiff, double *uRecNew);
+/* uRecNew = uRecDiff - sum_f AinvB_f * uRecDiff_j - A^-1 BCdiff (VR Jacobian action); the
+ * boundary term needs the base state uRec, u (may be NULL on meshes without boundary faces) */
+int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const double *u, const double *uRecDiff,
+                     double *uRecNew);
+
+/* Boundary conditions (EulerBCType codes, src/Euler/EulerBC.hpp:35-48): per zone a type and a
+ * value vector of 5 conservative variables (far-field / inflow state; unused otherwise).
+ * Used by the VR sweep for nVars = 5 (with gamma), by EvaluateRHS and by EvaluateDt. */
+enum { ORC_BC_FAR = 1, ORC_BC_WALL = 2, ORC_BC_WALL_INVIS = 3, ORC_BC_OUT = 5, ORC_BC_OUTP = 6, ORC_BC_IN = 7,
+       ORC_BC_SYM = 9 };
+int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma);
+/* VR sweep boundary values given as data (the TFBoundary callback of the reference tests):
+ * value[f][g][var] at boundary face f, quadrature point g; used when no Euler BC applies. */
+int orc_set_vr_dirichlet(orc_vr *h, int nVars, const double *value);
+/* ghost state of one boundary point (generateBoundaryValue, EulerEvaluator_EvaluateDt.hxx:1743-1838) */
+int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double *UR);
 
 /* Riemann flux at one point; F gets the flux, lam[3] = lam0, lam123, lam4 */
 int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const double *ULm,

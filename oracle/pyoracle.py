@@ -25,14 +25,15 @@ class MeshDesc(C.Structure):
                 ("cellType", i32p), ("cell2nodePtr", i64p), ("cell2node", i64p),
                 ("cell2facePtr", i64p), ("cell2face", i64p), ("nFace", C.c_int64),
                 ("faceType", i32p), ("face2nodePtr", i64p), ("face2node", i64p),
-                ("face2cell", i64p), ("faceShift", dp)]
+                ("face2cell", i64p), ("faceShift", dp), ("faceZone", i32p), ("nBCZone", C.c_int32)]
 
 
 class VRSettings(C.Structure):
     _fields_ = [("maxOrder", C.c_int), ("intOrder", C.c_int), ("dirWeightScheme", C.c_int),
                 ("geomWeightScheme", C.c_int), ("geomWeightBias", C.c_double),
                 ("geomWeightPower", C.c_double), ("scaleMultiplier", C.c_double),
-                ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int)]
+                ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int), ("bcWeight", C.c_double),
+                ("zoneDirWeight", dp)]
 
 
 class EulerSettings(C.Structure):
@@ -65,7 +66,7 @@ def lib():
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_iter_diff",
                      "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
-                     "orc_rec_error_l1", "orc_riemann"]:
+                     "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost"]:
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -98,23 +99,41 @@ def riemann(rsType, UL, UR, n, gamma=1.4, ULm=None, URm=None, dLambda=0.0, fixSc
     return F, lam
 
 
+def bc_ghost(bcType, UL, n, zoneValue=None, gamma=1.4):
+    """Ghost state of one boundary point (generateBoundaryValue)."""
+    UL = np.ascontiguousarray(UL, np.float64)
+    n = np.ascontiguousarray(n, np.float64)
+    zv = np.zeros(5) if zoneValue is None else np.ascontiguousarray(zoneValue, np.float64)
+    UR = np.zeros(5)
+    _chk(lib().orc_bc_ghost(C.c_int(bcType), P(UL), P(n), P(zv), C.c_double(gamma), P(UR)))
+    return UR
+
+
 class OracleVR:
     """VariationalReconstruction + EulerEvaluator restated on the CPU."""
 
     def __init__(self, mesh, maxOrder=1, intOrder=3, dirWeightScheme=0, geomWeightScheme=0,
-                 geomWeightBias=0.0, geomWeightPower=0.5, jacobiRelax=1.0, nIterCellSmoothScale=15):
+                 geomWeightBias=0.0, geomWeightPower=0.5, jacobiRelax=1.0, nIterCellSmoothScale=15,
+                 bcWeight=1.0, zoneDirWeight=None):
         L = lib()
         self.mesh = mesh
         self._keep = [np.ascontiguousarray(x) for x in (
             mesh.coords, mesh.cellType, mesh.cell2nodePtr, mesh.cell2node, mesh.cell2facePtr, mesh.cell2face,
             mesh.faceType, mesh.face2nodePtr, mesh.face2node, mesh.face2cell, mesh.faceShift)]
         k = self._keep
+        fz = getattr(mesh, "faceZone", None)
+        nz = int(getattr(mesh, "nBCZone", 0))
+        fz = None if fz is None else np.ascontiguousarray(fz, np.int32)
+        zw = None if zoneDirWeight is None else np.ascontiguousarray(zoneDirWeight, np.float64)
+        self._keep += [fz, zw]
         md = MeshDesc(mesh.dim, mesh.nNode, k[0].ctypes.data_as(dp), mesh.nCell, k[1].ctypes.data_as(i32p),
                       k[2].ctypes.data_as(i64p), k[3].ctypes.data_as(i64p), k[4].ctypes.data_as(i64p),
                       k[5].ctypes.data_as(i64p), mesh.nFace, k[6].ctypes.data_as(i32p), k[7].ctypes.data_as(i64p),
-                      k[8].ctypes.data_as(i64p), k[9].ctypes.data_as(i64p), k[10].ctypes.data_as(dp))
+                      k[8].ctypes.data_as(i64p), k[9].ctypes.data_as(i64p), k[10].ctypes.data_as(dp),
+                      None if fz is None else fz.ctypes.data_as(i32p), nz)
         s = VRSettings(maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
-                       1.0, jacobiRelax, nIterCellSmoothScale)
+                       1.0, jacobiRelax, nIterCellSmoothScale, bcWeight,
+                       None if zw is None else zw.ctypes.data_as(dp))
         self.settings = s
         h = L.orc_vr_create(C.byref(md), C.byref(s))
         if not h:
@@ -184,11 +203,26 @@ class OracleVR:
                                P(out)))
         return out
 
-    def vr_iter_diff(self, uRecDiff):
+    def vr_iter_diff(self, uRecDiff, uRec=None, u=None):
         nVars = uRecDiff.shape[1]
         out = np.zeros_like(uRecDiff)
-        _chk(lib().orc_vr_iter_diff(self.h, C.c_int(nVars), P(np.ascontiguousarray(uRecDiff)), P(out)))
+        _chk(lib().orc_vr_iter_diff(self.h, C.c_int(nVars), None if uRec is None else P(np.ascontiguousarray(uRec)),
+                                    None if u is None else P(np.ascontiguousarray(u)),
+                                    P(np.ascontiguousarray(uRecDiff)), P(out)))
         return out
+
+    # ---- boundary conditions
+    def set_bc(self, types, values=None, gamma=1.4):
+        """Per-zone EulerBCType codes and 5-vectors (far-field / inflow states)."""
+        t = np.ascontiguousarray(types, np.int32)
+        v = None if values is None else np.ascontiguousarray(values, np.float64).reshape(len(t), 5)
+        self._bc = (t, v)
+        _chk(lib().orc_set_bc(self.h, C.c_int(len(t)), P(t), P(v), C.c_double(gamma)))
+
+    def set_vr_dirichlet(self, values):
+        """Boundary point values [nFace, nGFace, nVars] for the VR sweep (boundary faces read)."""
+        v = np.ascontiguousarray(values, np.float64)
+        _chk(lib().orc_set_vr_dirichlet(self.h, C.c_int(v.shape[2]), P(v)))
 
     # ---- Euler
     @staticmethod
