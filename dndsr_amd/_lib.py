@@ -22,14 +22,15 @@ class MeshDesc(C.Structure):
                 ("cellType", i32p), ("cell2nodePtr", i64p), ("cell2node", i64p),
                 ("cell2facePtr", i64p), ("cell2face", i64p), ("nFace", C.c_int64),
                 ("faceType", i32p), ("face2nodePtr", i64p), ("face2node", i64p),
-                ("face2cell", i64p), ("faceShift", dp)]
+                ("face2cell", i64p), ("faceShift", dp), ("faceZone", i32p), ("nBCZone", C.c_int32)]
 
 
 class VRSettings(C.Structure):
     _fields_ = [("maxOrder", C.c_int), ("intOrder", C.c_int), ("dirWeightScheme", C.c_int),
                 ("geomWeightScheme", C.c_int), ("geomWeightBias", C.c_double),
                 ("geomWeightPower", C.c_double), ("scaleMultiplier", C.c_double),
-                ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int)]
+                ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int), ("bcWeight", C.c_double),
+                ("zoneDirWeight", dp)]
 
 
 class EulerSettings(C.Structure):
@@ -63,7 +64,9 @@ SIGNATURES = {
     "dndsr_h2d_cellfield": [vp, vp, vp, C.c_int],
     "dndsr_d2h_cellfield": [vp, vp, vp, C.c_int],
     "dndsr_vr_iter": [vp, C.c_int, vp, vp, vp, C.c_int, C.c_double],
-    "dndsr_vr_iter_diff": [vp, C.c_int, vp, vp],
+    "dndsr_vr_iter_diff": [vp, C.c_int, vp, vp, vp, vp],
+    "dndsr_set_bc": [vp, C.c_int, i32p, dp, C.c_double],
+    "dndsr_set_vr_dirichlet": [vp, C.c_int, vp],
     "dndsr_eval_dt": [vp, C.POINTER(EulerSettings), vp, C.c_double, C.c_double, C.c_int, vp, vp, vp, vp, vp],
     "dndsr_eval_rhs": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],
     "dndsr_lusgs_init": [vp, vp, C.c_double, C.c_double, vp, vp],

@@ -65,11 +65,14 @@ class VariationalReconstruction:
     def __init__(self, mesh, settings: dict | None = None, device: int = 0, ctx: Context | None = None):
         s = dict(maxOrder=1, intOrder=1, dirWeightScheme="Factorial", geomWeightScheme="GWNone",
                  geomWeightBias=0.0, geomWeightPower=0.5, scaleMultiplier=1.0, jacobiRelax=1.0,
-                 nIterCellSmoothScale=15)
+                 nIterCellSmoothScale=15, bcWeight=1.0, zoneDirWeight=None)
         if settings:
             s.update(settings)
             if "functionalSettings" in settings:
                 s.update(settings["functionalSettings"])
+        if s.get("bcTypes") is not None and s.get("zoneDirWeight") is None:
+            from . import bc as _bc
+            s["zoneDirWeight"] = _bc.euler_dir_weights(s["bcTypes"], s["maxOrder"])
         if not isinstance(s["dirWeightScheme"], int):
             s["dirWeightScheme"] = DIR_WEIGHT[s["dirWeightScheme"]]
         if not isinstance(s["geomWeightScheme"], int):
@@ -79,6 +82,7 @@ class VariationalReconstruction:
         self.dim = mesh.dim
         self.ctx = ctx or Context(device)
         self._constructed = False
+        self._dirichlet = None
 
     # ---- construction (ConstructMetrics / ConstructBaseAndWeight / ConstructRecCoeff)
     def ConstructMetrics(self):
@@ -88,16 +92,23 @@ class VariationalReconstruction:
         keep = [np.ascontiguousarray(x) for x in (
             m.coords, m.cellType, m.cell2nodePtr, m.cell2node, m.cell2facePtr, m.cell2face, m.faceType,
             m.face2nodePtr, m.face2node, m.face2cell, m.faceShift)]
+        fz = getattr(m, "faceZone", None)
+        fz = None if fz is None else np.ascontiguousarray(fz, np.int32)
+        zw = self.settings.get("zoneDirWeight")
+        zw = None if zw is None else np.ascontiguousarray(zw, np.float64)
+        keep += [fz, zw]
         md = _lib.MeshDesc(m.dim, m.nNode, keep[0].ctypes.data_as(_lib.dp), m.nCell, keep[1].ctypes.data_as(_lib.i32p),
                            keep[2].ctypes.data_as(_lib.i64p), keep[3].ctypes.data_as(_lib.i64p),
                            keep[4].ctypes.data_as(_lib.i64p), keep[5].ctypes.data_as(_lib.i64p), m.nFace,
                            keep[6].ctypes.data_as(_lib.i32p), keep[7].ctypes.data_as(_lib.i64p),
                            keep[8].ctypes.data_as(_lib.i64p), keep[9].ctypes.data_as(_lib.i64p),
-                           keep[10].ctypes.data_as(_lib.dp))
+                           keep[10].ctypes.data_as(_lib.dp), None if fz is None else fz.ctypes.data_as(_lib.i32p),
+                           int(getattr(m, "nBCZone", 0)))
         s = self.settings
         vs = _lib.VRSettings(s["maxOrder"], s["intOrder"], s["dirWeightScheme"], s["geomWeightScheme"],
                              s["geomWeightBias"], s["geomWeightPower"], s["scaleMultiplier"], s["jacobiRelax"],
-                             s["nIterCellSmoothScale"])
+                             s["nIterCellSmoothScale"], s["bcWeight"],
+                             None if zw is None else zw.ctypes.data_as(_lib.dp))
         check(lib().dndsr_vr_construct(self.ctx.h, C.byref(md), C.byref(vs)))
         nC, nF = C.c_int64(), C.c_int64()
         M, K, G, Gc, nCol = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -108,7 +119,18 @@ class VariationalReconstruction:
         self.nGFace, self.nGCell, self.nColor, self.globalVol = G.value, Gc.value, nCol.value, gv.value
         self._constructed = True
 
-    def ConstructBaseAndWeight(self):
+    def ConstructBaseAndWeight(self, id2faceDircWeight=None):
+        """id2faceDircWeight(zone, order) multiplies the weights of boundary faces
+        (VariationalReconstruction.cpp:438-441); it must come before the device build."""
+        if id2faceDircWeight is not None:
+            nz = int(getattr(self.mesh, "nBCZone", 0))
+            P = self.settings["maxOrder"]
+            w = np.array([[id2faceDircWeight(z, p) for p in range(P + 1)] for z in range(nz)], np.float64)
+            if self._constructed:
+                old = self.settings.get("zoneDirWeight")
+                if old is None or not np.array_equal(np.asarray(old), w):
+                    raise RuntimeError("boundary weights must be set before the VR coefficients are built")
+            self.settings["zoneDirWeight"] = w
         self.ConstructMetrics()
 
     def ConstructRecCoeff(self):
@@ -204,5 +226,18 @@ class VariationalReconstruction:
         if not putIntoNew:
             uRec.copy_(uRecNew)
 
-    def DoReconstructionIterDiff(self, uRecDiff, uRecNew):
-        check(lib().dndsr_vr_iter_diff(self.ctx.h, uRecDiff.shape[0], _ptr(uRecDiff), _ptr(uRecNew)))
+    def DoReconstructionIterDiff(self, uRec, uRecDiff, uRecNew, u=None):
+        """uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j - A^-1 BCdiff; uRec and u feed the
+        boundary term (None on meshes without boundary faces)."""
+        check(lib().dndsr_vr_iter_diff(self.ctx.h, uRecDiff.shape[0], _ptr(uRec), _ptr(u), _ptr(uRecDiff),
+                                       _ptr(uRecNew)))
+
+    def set_vr_dirichlet(self, values):
+        """Boundary point values for the VR sweep when the Euler boundary table does not apply
+        (any nVars): device tensor [nVars, nGFace, nFace] (boundary faces read); kept by reference."""
+        if values is None:
+            check(lib().dndsr_set_vr_dirichlet(self.ctx.h, 0, None))
+            self._dirichlet = None
+            return
+        self._dirichlet = values.contiguous()
+        check(lib().dndsr_set_vr_dirichlet(self.ctx.h, int(values.shape[0]), _ptr(self._dirichlet)))

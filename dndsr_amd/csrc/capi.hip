@@ -10,6 +10,7 @@
 #include <vector>
 
 #include "../../include/dndsr_amd.h"
+#include "gas.hpp"
 #include "internal.hpp"
 
 using namespace dndsr;
@@ -210,7 +211,10 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.F = (idx)md->nFace;
     m.nNode = (idx)md->nNode;
     m.s = VRSettingsDev{s->maxOrder, s->intOrder, s->dirWeightScheme, s->geomWeightScheme, s->geomWeightBias,
-                        s->geomWeightPower, s->scaleMultiplier, s->jacobiRelax, s->nIterCellSmoothScale};
+                        s->geomWeightPower, s->scaleMultiplier, s->jacobiRelax, s->nIterCellSmoothScale,
+                        s->bcWeight};
+    need(md->nBCZone >= 0 && md->nBCZone < 4096, "nBCZone out of range");
+    m.nBCZone = md->nBCZone;
     const size_t N = m.N, F = m.F;
     // element types: one cell type and one face type per mesh on this path
     m.cellType = md->cellType[0];
@@ -250,6 +254,8 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
             hcn[a * N + i] = (idx)nd;
         }
     std::vector<real> hsh(3 * F);
+    std::vector<int32_t> hzone(F, -1);
+    std::vector<idx> hbnd;
     for (size_t f = 0; f < F; f++)
     {
         need(md->face2nodePtr[f + 1] - md->face2nodePtr[f] == nnf, "bad face node count");
@@ -261,7 +267,13 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
         }
         int64_t L = md->face2cell[2 * f], R = md->face2cell[2 * f + 1];
         need(L >= 0 && L < (int64_t)N && R < (int64_t)N, "face2cell out of range");
-        need(R >= 0, "boundary faces are not supported on this path yet (periodic / interior meshes only)");
+        if (R < 0)
+        {
+            need(R == -1 && md->faceZone && md->faceZone[f] >= 0 && md->faceZone[f] < md->nBCZone,
+                 "a boundary face needs a zone in 0 .. nBCZone-1");
+            hzone[f] = md->faceZone[f];
+            hbnd.push_back((idx)f);
+        }
         hf2c[f] = (idx)L, hf2c[F + f] = (idx)R;
         for (int r = 0; r < 3; r++)
             hsh[r * F + f] = md->faceShift[3 * f + r];
@@ -289,6 +301,8 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
                 continue;
             idx o = hf2c[f] == (idx)i ? hf2c[F + f] : hf2c[f];
             hc2c[(size_t)k * N + i] = o;
+            if (o < 0) // boundary face: no neighbour, no reverse slot
+                continue;
             for (int k2 = 0; k2 < K; k2++)
                 if (hc2f[(size_t)k2 * N + o] == f)
                     hc2s[(size_t)k * N + i] = k2;
@@ -364,6 +378,31 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     upload(c, m.fshift, hsh);
     upload(c, m.color, col);
     upload(c, m.colorCells, cells);
+    // boundary faces, the cells that own them, per-zone VR direction weights
+    m.nBnd = (idx)hbnd.size();
+    m.fzone = devAlloc<int32_t>(c, F);
+    upload(c, m.fzone, hzone);
+    if (m.nBnd > 0)
+    {
+        m.bndFace = devAlloc<idx>(c, hbnd.size());
+        upload(c, m.bndFace, hbnd);
+        std::vector<idx> bcells;
+        for (idx f : hbnd)
+            bcells.push_back(hf2c[f]);
+        std::sort(bcells.begin(), bcells.end());
+        bcells.erase(std::unique(bcells.begin(), bcells.end()), bcells.end());
+        m.nBndCell = (idx)bcells.size();
+        m.bndCell = devAlloc<idx>(c, bcells.size());
+        upload(c, m.bndCell, bcells);
+    }
+    if (m.nBCZone > 0)
+    {
+        std::vector<real> zw((size_t)m.nBCZone * (m.P + 1), 1.0);
+        if (s->zoneDirWeight)
+            std::copy(s->zoneDirWeight, s->zoneDirWeight + zw.size(), zw.begin());
+        m.zoneW = devAlloc<real>(c, zw.size());
+        upload(c, m.zoneW, zw);
+    }
     m.fpts = devAlloc<real>(c, (size_t)m.G * 3 * F);
     m.fnorm = devAlloc<real>(c, (size_t)m.G * 3 * F);
     m.fwj = devAlloc<real>(c, (size_t)m.G * F);
@@ -608,12 +647,58 @@ int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew
     CAPI_END
 }
 
-int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRecDiff, double *uRecNew)
+int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRec, const double *u, const double *uRecDiff,
+                       double *uRecNew)
 {
     CAPI_BEGIN
     requireMesh(ctx);
     need(uRecDiff && uRecNew && uRecDiff != uRecNew, "bad fields");
-    launchVRIterDiff(ctx->c, nVars, uRecDiff, uRecNew);
+    launchVRIterDiff(ctx->c, nVars, uRec, u, uRecDiff, uRecNew);
+    CAPI_END
+}
+
+int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    Ctx &c = ctx->c;
+    DevMesh &m = c.m;
+    need(nZone == m.nBCZone, "the zone count differs from the mesh's nBCZone");
+    need(nZone > 0 && type, "no boundary zones");
+    need(gamma > 1.0, "gamma must exceed 1");
+    std::vector<int32_t> t(type, type + nZone);
+    std::vector<real> v((size_t)nZone * 5, 0.0);
+    for (int z = 0; z < nZone; z++)
+    {
+        need(t[z] == BC_FAR || t[z] == BC_WALL || t[z] == BC_WALL_INVIS || t[z] == BC_OUT || t[z] == BC_OUTP ||
+                 t[z] == BC_IN || t[z] == BC_SYM,
+             "boundary type not supported on this path (BCFar, BCWall, BCWallInvis, BCOut, BCOutP, BCIn, BCSym)");
+        if (t[z] == BC_FAR || t[z] == BC_OUTP || t[z] == BC_IN)
+        {
+            need(value != nullptr, "far-field / inflow zones need a value");
+            for (int k = 0; k < 5; k++)
+                v[(size_t)z * 5 + k] = value[(size_t)z * 5 + k];
+            need(v[(size_t)z * 5] > 0, "boundary state with nonpositive density");
+        }
+    }
+    if (!m.bcType)
+    {
+        m.bcType = devAlloc<int32_t>(c, nZone);
+        m.bcVal = devAlloc<real>(c, (size_t)nZone * 5);
+    }
+    upload(c, m.bcType, t);
+    upload(c, m.bcVal, v);
+    m.bcGamma = gamma;
+    CAPI_END
+}
+
+int dndsr_set_vr_dirichlet(dndsr_ctx *ctx, int nVars, const double *value)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(value == nullptr || (nVars >= 1 && nVars <= 5), "nVars must be 1..5");
+    ctx->c.m.dirVal = value;
+    ctx->c.m.dirNV = value ? nVars : 0;
     CAPI_END
 }
 
@@ -984,6 +1069,7 @@ int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dou
 {
     CAPI_BEGIN
     requireMesh(ctx);
+    need(ctx->c.m.nBnd == 0, "the discrete adjoint covers periodic / interior meshes only");
     need(es && u && rhsBar && uBar, "null field");
     need(!uRecBar || uRec, "uRecBar needs uRec");
     Ctx &c = ctx->c;
@@ -998,6 +1084,7 @@ int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const d
 {
     CAPI_BEGIN
     requireMesh(ctx);
+    need(ctx->c.m.nBnd == 0, "the discrete adjoint covers periodic / interior meshes only");
     need(uRecNewBar != nullptr, "null field");
     need(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
     need(mode == 3 || uBar != nullptr, "uBar is required for modes 0-2");

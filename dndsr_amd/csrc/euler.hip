@@ -805,6 +805,103 @@ constexpr int rhsPointsPerLane()
     return DIM == 3 && G == 9 ? 3 : 1;
 }
 
+// Boundary faces (EvaluateRHS.hxx:194-207, 336-345): one thread per boundary face,
+// run after the tile kernel and overwriting the tile kernel's flux of that face.  The
+// Roe average takes the cell mean and its ghost at the face centre (mean normal); every
+// point takes the reconstructed state and its ghost at the point normal; dLambda is the
+// owner cell's.  Point fluxes are summed in point order as in the tile kernels.
+template <int DIM, int NDOF, int G, int RS, bool BATCH>
+__global__ void __launch_bounds__(64) kRHSBoundary(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                                   const real *__restrict__ uRec, const real *__restrict__ dlc,
+                                                   real *__restrict__ fflux)
+{
+    constexpr int M = NDOF - 1;
+    const idx t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nBnd)
+        return;
+    const idx f = m.bndFace[t];
+    const size_t N = m.N, F = m.F;
+    const idx cL = m.f2c[f];
+    const int zone = m.fzone[f];
+    const int type = m.bcType[zone];
+    const real *far = m.bcVal + (size_t)zone * 5;
+    real ULm[5], URm[5], nm[3];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        ULm[v] = u[(size_t)v * N + cL];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        nm[r] = m.fmeanN[(size_t)r * F + f];
+    bcGhost(type, ULm, nm, far, gp.gamma, URm);
+    const real dLambda = dlc ? dlc[cL] : 0.0;
+    RoeBatchFace bf;
+    RoeAve ra;
+    if constexpr (BATCH)
+        roeBatchSetup<RS>(ULm, URm, nm, gp, dLambda, bf);
+    else
+        roeAverage(ULm, URm, gp.gamma, ra);
+    real mom[M > 0 ? M : 1], cen[3], isc = 0;
+    if (uRec)
+    {
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            mom[j] = m.moment[(size_t)j * N + cL];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cen[r] = m.cent[(size_t)r * N + cL];
+        isc = m.iscale[cL];
+    }
+    real acc[5] = {0, 0, 0, 0, 0};
+    for (int g = 0; g < G; g++)
+    {
+        real UL[5], UR[5], n[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            UL[v] = ULm[v];
+        if (uRec)
+        {
+            real xs[3], phi[M > 0 ? M : 1];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+            {
+                real p = m.fpts[((size_t)g * 3 + r) * F + f] + 0.0;
+                xs[r] = p - cen[r];
+                if (DIM == 3 || r < 2)
+                    xs[r] *= isc;
+            }
+            basisRow0<DIM, NDOF>(xs, mom, phi);
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+            {
+                real sacc = 0;
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    sacc += phi[j] * uRec[((size_t)v * M + j) * N + cL];
+                UL[v] += sacc;
+            }
+        }
+        bcGhost(type, UL, n, far, gp.gamma, UR);
+        real Fl[5];
+        if constexpr (BATCH)
+            roeBatchPoint<RS>(UL, UR, n, gp, bf, Fl);
+        else
+        {
+            real l0, l1, l4;
+            riemannFluxR<RS>(UL, UR, ra, n, gp, dLambda, Fl, l0, l1, l4);
+        }
+        const real wj = m.fwj[(size_t)g * F + f];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            acc[v] += (-Fl[v]) * wj;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        fflux[(size_t)v * F + f] = acc[v];
+}
+
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
 static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
 {
@@ -831,6 +928,11 @@ static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uR
         kern<<<gridFor(c.m.F, kTileF), kTileF * G, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
     }
     DNDSR_HIP_CHECK(hipGetLastError());
+    if (c.m.nBnd > 0)
+    {
+        kRHSBoundary<DIM, NDOF, G, RS, BATCH><<<gridFor(c.m.nBnd, 64), 64, 0, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
 }
 
 template <int DIM, int NDOF, int G>
@@ -898,6 +1000,8 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
                    const StageUpdate *stage)
 {
     DevMesh &m = c.m;
+    if (m.nBnd > 0 && !m.bcType)
+        throw std::runtime_error("the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
     real *ff = c.faceFlux;
     KernelTimer kt(c, K_RHS_FACE);
     if (m.dim == 2)

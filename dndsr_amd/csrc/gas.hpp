@@ -432,4 +432,108 @@ __host__ __device__ inline void fluxJacDu(const T *U, const T *n, const T *dU, T
         dF[k] += -lambdaMain * dU[k];
 }
 
+// ---------------------------------------------------------------------------
+// Boundary ghost states (generateBoundaryValue, EulerEvaluator_EvaluateDt.hxx:1743-1838),
+// static frame: generateBV_FarField :1870-1950 (BCFar, BCOutP without anchors),
+// generateBV_InviscidWall :2199-2215 (BCWallInvis, BCSym), generateBV_ViscousWall
+// :2241-2270 (BCWall: velocity reversed), generateBV_Outflow :2346-2353, generateBV_Inflow
+// :2377-2390.  Codes are EulerBCType (src/Euler/EulerBC.hpp:35-48).
+// ---------------------------------------------------------------------------
+constexpr int BC_FAR = 1, BC_WALL = 2, BC_WALL_INVIS = 3, BC_OUT = 5, BC_OUTP = 6, BC_IN = 7, BC_SYM = 9;
+
+// IdealGasThermalConservative2Primitive / Primitive2Conservative (Gas.hpp:279-317)
+__host__ __device__ inline void cons2prim(const real *U, real *prim, real gamma)
+{
+    for (int k = 0; k < 5; k++)
+        prim[k] = U[k] / U[0];
+    const real vSqr = sq(U[1] / U[0]) + sq(U[2] / U[0]) + sq(U[3] / U[0]);
+    prim[0] = U[0];
+    prim[4] = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
+}
+__host__ __device__ inline void prim2cons(const real *prim, real *U, real gamma)
+{
+    for (int k = 0; k < 5; k++)
+        U[k] = prim[k] * prim[0];
+    const real vSqr = sq(prim[1]) + sq(prim[2]) + sq(prim[3]);
+    U[0] = prim[0];
+    U[4] = prim[4] / (gamma - 1) + (prim[0] * 0.5 * vSqr);
+}
+
+__host__ __device__ inline void bcGhost(int type, const real *UL, const real *n, const real *far, real gamma, real *UR)
+{
+    switch (type)
+    {
+    case BC_FAR:
+    case BC_OUTP:
+    {
+        const real un = (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) / UL[0];
+        real p, asqr, H;
+        thermal(UL[4], UL[0], sq(UL[1] / UL[0]) + sq(UL[2] / UL[0]) + sq(UL[3] / UL[0]), gamma, p, asqr, H);
+        const real a = sqrt(asqr);
+        if (un - a > 0) // supersonic outflow
+            for (int k = 0; k < 5; k++)
+                UR[k] = UL[k];
+        else if (un > 0) // subsonic outflow: exterior pressure
+        {
+            real fp[5], lp[5];
+            cons2prim(far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            lp[4] = fp[4];
+            prim2cons(lp, UR, gamma);
+        }
+        else if (un + a > 0) // subsonic inflow: interior pressure
+        {
+            real fp[5], lp[5];
+            cons2prim(far, fp, gamma);
+            cons2prim(UL, lp, gamma);
+            fp[4] = lp[4];
+            prim2cons(fp, UR, gamma);
+        }
+        else // supersonic inflow
+            for (int k = 0; k < 5; k++)
+                UR[k] = far[k];
+        return;
+    }
+    case BC_WALL_INVIS:
+    case BC_SYM:
+    {
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        const real vn = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] -= 2 * vn * n[d];
+        return;
+    }
+    case BC_WALL:
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] *= -1;
+        return;
+    case BC_IN:
+        for (int k = 0; k < 5; k++)
+            UR[k] = far[k];
+        return;
+    default: // BC_OUT
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        return;
+    }
+}
+
+// CompressRecPart (EulerEvaluator.hpp:1541-1631): the reconstructed point state, or the
+// cell mean where it would carry negative density or internal energy
+__host__ __device__ inline void compressRecPart(const real *umean, const real *inc, real *ret)
+{
+    for (int k = 0; k < 5; k++)
+        ret[k] = umean[k] + inc[k];
+    if (ret[0] < 0)
+        for (int k = 0; k < 5; k++)
+            ret[k] = umean[k];
+    const real eK = (sq(ret[1]) + sq(ret[2]) + sq(ret[3])) * 0.5 / (kVerySmall + ret[0]);
+    if (ret[4] - eK < 0)
+        for (int k = 0; k < 5; k++)
+            ret[k] = umean[k];
+}
+
 } // namespace dndsr

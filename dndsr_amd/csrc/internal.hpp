@@ -39,6 +39,7 @@ struct VRSettingsDev
     real scaleMultiplier;
     real jacobiRelax;
     int nIterCellSmoothScale;
+    real bcWeight;
 };
 
 // Device-resident mesh, metrics and VR coefficients (all arrays SoA, see dev_common.hpp).
@@ -86,6 +87,19 @@ struct DevMesh
     idx *colorCells = nullptr; // [N] cells grouped by colour
     int32_t *color = nullptr;  // [N]
     real globalVol = 0;
+    // boundary faces (face2cell[f][1] = -1) and their zones
+    int nBCZone = 0;
+    idx nBnd = 0, nBndCell = 0;
+    int32_t *fzone = nullptr;   // [F] zone of a boundary face, -1 elsewhere
+    idx *bndFace = nullptr;     // [nBnd]
+    idx *bndCell = nullptr;     // [nBndCell] cells with at least one boundary face
+    real *zoneW = nullptr;      // [nBCZone][P+1] boundary direction weights (setup only)
+    // boundary-condition table (dndsr_set_bc) and VR boundary point values (dndsr_set_vr_dirichlet)
+    int32_t *bcType = nullptr;  // [nBCZone]
+    real *bcVal = nullptr;      // [nBCZone][5]
+    real bcGamma = 1.4;
+    int dirNV = 0;
+    const real *dirVal = nullptr; // [dirNV][G][F]
 };
 
 struct Ctx
@@ -174,7 +188,7 @@ struct KernelTimer
 void setupMetricsAndVR(Ctx &c);
 // ---- launchers (vr.hip)
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax);
-void launchVRIterDiff(Ctx &c, int nVars, const real *uRecDiff, real *uRecNew);
+void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *uRecDiff, real *uRecNew);
 // ---- launchers (euler.hip)
 void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,
                   real *lambdaFace, real *deltaLambdaCell, real *dtMinDev);

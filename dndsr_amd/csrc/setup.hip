@@ -228,6 +228,9 @@ __global__ void kFaceWeight(DevMesh m)
             else
                 wd = p == 0 ? 1 : (p == 1 ? .5295 : .2117);
         }
+        // boundary faces: id2faceDircWeight(zone, order) * bcWeight, VariationalReconstruction.cpp:438-441
+        if (cR < 0)
+            wd *= (m.zoneW ? m.zoneW[(size_t)m.fzone[f] * (m.P + 1) + p] : 1.0) * m.s.bcWeight;
         m.faceW[(size_t)p * m.F + f] = wd * wg;
     }
 }

@@ -3,6 +3,7 @@
 // DoReconstructionIter, Jacobi branch (SORInstead = false) and the recordInc
 // modes, src/CFV/VariationalReconstruction_Reconstruction.hxx:485-615, and
 // DoReconstructionIterDiff, :617-670.
+#include "gas.hpp"
 #include "internal.hpp"
 
 namespace dndsr
@@ -211,6 +212,175 @@ static void vrIterDiffM(Ctx &c, int nVars, const real *x, real *y)
     vrLaunch<M, 2>(c, nVars, x, y, nullptr, 0, 1.0);
 }
 
+
+// ---------------------------------------------------------------------------
+// Boundary term of the sweep (GetBoundaryRHS / GetBoundaryRHSDiff,
+// VariationalReconstruction_Reconstruction.hxx:19-176, intOrderVRBC = intOrder):
+// one thread per cell that owns boundary faces, run after the tile kernel (whose
+// boundary slots contribute nothing).  Per boundary face of the cell
+//   BCC = sum_g D0phi(x_g)^T w_0^2 (uB(x_g) - u_i) JDet w_g,   uRecNew += scale A^-1 BCC
+// with uB the ghost of the compressed point state u_i + D0phi uRec (Euler table, nVars = 5,
+// the solver's FBoundary, EulerSolver.hxx:163-175) or the given point value.
+// MODE 0: Jacobi (scale = relax); 1: recordInc (scale = -1); 2: the Jacobian action
+// (uB(uBL + D0phi x) - uB(uBL) in place of uB - u_i, scale = -1).
+// ---------------------------------------------------------------------------
+template <int DIM, int NDOF, int NV, int MODE>
+__global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restrict__ uRec, const real *__restrict__ u,
+                                                  const real *__restrict__ x, real *__restrict__ uRecNew, real scale)
+{
+    constexpr int M = NDOF - 1;
+    const idx t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nBndCell)
+        return;
+    const idx i = m.bndCell[t];
+    const size_t N = m.N, F = m.F;
+    real mom[M], cen[3], um[NV];
+#pragma unroll
+    for (int j = 0; j < M; j++)
+        mom[j] = m.moment[(size_t)j * N + i];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        cen[r] = m.cent[(size_t)r * N + i];
+    const real isc = m.iscale[i];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        um[v] = u ? u[(size_t)v * N + i] : 0.0;
+    const bool euler = NV == 5 && m.bcType != nullptr;
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + i];
+        if (f < 0 || m.c2c[(size_t)k * N + i] >= 0)
+            continue;
+        const int zone = m.fzone[f];
+        const real w0 = m.faceW[f]; // order-0 weight; faceL^0 = 1
+        const real w2 = w0 * w0;
+        real BCC[M][NV];
+#pragma unroll
+        for (int j = 0; j < M; j++)
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                BCC[j][v] = 0;
+        for (int g = 0; g < m.G; g++)
+        {
+            real xs[3], phi[M];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+            {
+                xs[r] = m.fpts[((size_t)g * 3 + r) * F + f] - cen[r];
+                if (DIM == 3 || r < 2)
+                    xs[r] *= isc;
+            }
+            basisRow0<DIM, NDOF>(xs, mom, phi);
+            real inc[NV];
+            if (euler)
+            {
+                real uBL[5], n[3], ULf[5], uB[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                {
+                    real r = 0;
+#pragma unroll
+                    for (int j = 0; j < M; j++)
+                        r += phi[j] * uRec[((size_t)v * M + j) * N + i];
+                    uBL[v] = r;
+                }
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+                const real *far = m.bcVal + (size_t)zone * 5;
+                const int type = m.bcType[zone];
+                compressRecPart(um, uBL, ULf);
+                bcGhost(type, ULf, n, far, m.bcGamma, uB);
+                if constexpr (MODE == 2)
+                {
+                    real in2[5], ULp[5], uBp[5];
+#pragma unroll
+                    for (int v = 0; v < 5; v++)
+                    {
+                        real r = 0;
+#pragma unroll
+                        for (int j = 0; j < M; j++)
+                            r += phi[j] * x[((size_t)v * M + j) * N + i];
+                        in2[v] = uBL[v] + r;
+                    }
+                    compressRecPart(um, in2, ULp);
+                    bcGhost(type, ULp, n, far, m.bcGamma, uBp);
+#pragma unroll
+                    for (int v = 0; v < NV; v++)
+                        inc[v] = uBp[v] - uB[v];
+                }
+                else
+#pragma unroll
+                    for (int v = 0; v < NV; v++)
+                        inc[v] = uB[v] - um[v];
+            }
+            else
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    inc[v] = MODE == 2 ? 0.0 : m.dirVal[((size_t)v * m.G + g) * F + f] - um[v];
+            const real fac = m.fwj[(size_t)g * F + f];
+#pragma unroll
+            for (int j = 0; j < M; j++)
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    BCC[j][v] += ((phi[j] * inc[v]) * w2) * fac;
+        }
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int r = 0; r < M; r++)
+            {
+                real acc = 0;
+#pragma unroll
+                for (int c2 = 0; c2 < M; c2++)
+                    acc += m.AinvB[((size_t)r * M + c2) * N + i] * BCC[c2][v];
+                uRecNew[((size_t)v * M + r) * N + i] += scale * acc;
+            }
+    }
+}
+
+template <int MODE, int DIM, int NDOF>
+static void vrBoundaryN(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
+{
+    const int g = gridFor(c.m.nBndCell, 64);
+    if (nVars == 5)
+        kVRBoundary<DIM, NDOF, 5, MODE><<<g, 64, 0, c.stream>>>(c.m, uRec, u, x, uRecNew, scale);
+    else
+        kVRBoundary<DIM, NDOF, 1, MODE><<<g, 64, 0, c.stream>>>(c.m, uRec, u, x, uRecNew, scale);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+template <int MODE>
+static void vrBoundary(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
+{
+    const DevMesh &m = c.m;
+    if (m.nBnd == 0)
+        return;
+    const bool euler = nVars == 5 && m.bcType != nullptr;
+    if (!euler && !(m.dirVal && m.dirNV == nVars))
+        throw std::runtime_error("the mesh has boundary faces: set the boundary conditions (dndsr_set_bc, nVars 5) "
+                                 "or boundary point values (dndsr_set_vr_dirichlet) for this nVars");
+    if (MODE == 2 && euler && (!uRec || !u))
+        throw std::runtime_error("the boundary term of the VR Jacobian needs uRec and u");
+    if (m.dim == 2)
+    {
+        if (m.P == 1)
+            vrBoundaryN<MODE, 2, 3>(c, nVars, uRec, u, x, uRecNew, scale);
+        else if (m.P == 2)
+            vrBoundaryN<MODE, 2, 6>(c, nVars, uRec, u, x, uRecNew, scale);
+        else
+            vrBoundaryN<MODE, 2, 10>(c, nVars, uRec, u, x, uRecNew, scale);
+    }
+    else
+    {
+        if (m.P == 1)
+            vrBoundaryN<MODE, 3, 4>(c, nVars, uRec, u, x, uRecNew, scale);
+        else if (m.P == 2)
+            vrBoundaryN<MODE, 3, 10>(c, nVars, uRec, u, x, uRecNew, scale);
+        else
+            vrBoundaryN<MODE, 3, 20>(c, nVars, uRec, u, x, uRecNew, scale);
+    }
+}
+
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
 {
     KernelTimer kt(c, K_VR_ITER);
@@ -223,9 +393,14 @@ void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real
     case 19: vrIterM<19>(c, nVars, uRec, uRecNew, u, mode, relax); break;
     default: throw std::runtime_error("unsupported number of reconstruction modes");
     }
+    // boundary faces: recordInc with uRecIsZero takes no boundary term (:577-581)
+    if (mode == 0)
+        vrBoundary<0>(c, nVars, uRec, u, nullptr, uRecNew, relax);
+    else if (mode == 1)
+        vrBoundary<1>(c, nVars, uRec, u, nullptr, uRecNew, -1.0);
 }
 
-void launchVRIterDiff(Ctx &c, int nVars, const real *x, real *y)
+void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *y)
 {
     KernelTimer kt(c, K_VR_ITER);
     switch (c.m.M)
@@ -237,6 +412,7 @@ void launchVRIterDiff(Ctx &c, int nVars, const real *x, real *y)
     case 19: vrIterDiffM<19>(c, nVars, x, y); break;
     default: throw std::runtime_error("unsupported number of reconstruction modes");
     }
+    vrBoundary<2>(c, nVars, uRec, u, x, y, -1.0);
 }
 
 } // namespace dndsr

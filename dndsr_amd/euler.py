@@ -31,9 +31,24 @@ class EulerEvaluator:
         self.settings = s
         self.es = _lib.EulerSettings(float(s["gamma"]), RIEMANN_TYPES[rs] if isinstance(rs, str) else int(rs),
                                      int(s["rsMeanValueEig"]), float(s["rsFixScale"]), float(s["rsIncFScale"]))
+        bcTypes = s.get("bcTypes")
+        if bcTypes is not None:
+            # boundary weights of the VR functional follow the boundary types, as the
+            # reference's constructor sets them (EulerEvaluator.hpp:138-152)
+            from . import bc as _bc
+            codes = [_bc.bc_code(t) for t in bcTypes]
+            w = _bc.euler_dir_weights(codes, vfv.settings["maxOrder"])
+            vfv.ConstructBaseAndWeight(lambda z, p: w[z, p])
         vfv.construct()
         self.vfv = vfv
         self.ctx = vfv.ctx
+        if bcTypes is not None:
+            t = np.ascontiguousarray(codes, np.int32)
+            vals = s.get("bcValues")
+            v = None if vals is None else np.ascontiguousarray(vals, np.float64).reshape(len(t), 5)
+            self._bc = (t, v)
+            check(lib().dndsr_set_bc(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p),
+                                     None if v is None else v.ctypes.data_as(_lib.dp), float(s["gamma"])))
         dev = self.ctx.torch_device
         self.lambdaFace = torch.zeros((4, vfv.nFace), dtype=torch.float64, device=dev)
         self.deltaLambdaCell = torch.zeros(vfv.nCell, dtype=torch.float64, device=dev)

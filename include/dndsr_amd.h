@@ -38,7 +38,10 @@ enum { DNDSR_LINE2 = 1, DNDSR_QUAD4 = 2, DNDSR_HEX8 = 3, DNDSR_TRI3 = 4, DNDSR_T
 
 /* Mesh as handed over by the caller (host memory).  Periodic faces are stored once,
  * in the frame of face2cell[f][0]; faceShift[f] carries a face point into the frame of
- * face2cell[f][1] (UnstructuredMesh::periodicInfo, src/Geom/Mesh/Mesh.hpp:37). */
+ * face2cell[f][1] (UnstructuredMesh::periodicInfo, src/Geom/Mesh/Mesh.hpp:37).
+ * A boundary face has face2cell[f][1] = -1, its normal pointing out of the domain, and
+ * a boundary zone faceZone[f] in 0 .. nBCZone-1 (the face zone id of
+ * UnstructuredMesh::GetFaceZone); faceZone may be NULL when there is no boundary face. */
 typedef struct
 {
     int dim;
@@ -53,6 +56,8 @@ typedef struct
     const int64_t *face2nodePtr, *face2node;
     const int64_t *face2cell; /* [nFace][2], -1 = boundary */
     const double *faceShift;  /* [nFace][3] */
+    const int32_t *faceZone;  /* [nFace]: zone of each boundary face (entries of other faces ignored) */
+    int32_t nBCZone;
 } dndsr_mesh_desc;
 
 /* VRSettings subset used on this path (src/CFV/VRSettings.hpp:37-212) */
@@ -67,6 +72,13 @@ typedef struct
     double scaleMultiplier;
     double jacobiRelax;
     int nIterCellSmoothScale;
+    /* boundary faces: the face weight of derivative order p is multiplied by
+     * zoneDirWeight[zone][p] * bcWeight (the id2faceDircWeight callback of
+     * ConstructBaseAndWeight, VariationalReconstruction.cpp:438-441; EulerEvaluator
+     * passes 1 for the value and 0 for derivatives, 0 everywhere for BCOut);
+     * NULL = 1 for every zone and order */
+    double bcWeight;
+    const double *zoneDirWeight; /* host, [nBCZone][maxOrder+1] */
 } dndsr_vr_settings;
 
 /* EulerEvaluatorSettings subset (src/Euler/EulerEvaluatorSettings.hpp:53-80) */
@@ -127,9 +139,29 @@ int dndsr_d2h_cellfield(dndsr_ctx *ctx, const double *dev, double *hostRows, int
  * src/CFV/VariationalReconstruction.hpp:984-992. */
 int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew, const double *u, int mode,
                   double relax);
-/* DoReconstructionIterDiff: uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j,
- * src/CFV/VariationalReconstruction.hpp:1012-1018. */
-int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRecDiff, double *uRecNew);
+/* DoReconstructionIterDiff: uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j - A^-1 BCdiff,
+ * src/CFV/VariationalReconstruction.hpp:1012-1018.  The boundary term (GetBoundaryRHSDiff,
+ * VariationalReconstruction_Reconstruction.hxx:98-176) is the difference of the boundary
+ * values at the point states of uRec + uRecDiff and of uRec, so uRec and u are needed on
+ * meshes with boundary faces (they may be NULL otherwise). */
+int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRec, const double *u, const double *uRecDiff,
+                       double *uRecNew);
+
+/* ---- boundary conditions ----------------------------------------------------------
+ * Boundary zones of the Euler path (BoundaryHandler / EulerBCType, src/Euler/EulerBC.hpp:35-48):
+ * type[z] is an EulerBCType code (BCFar 1, BCWall 2, BCWallInvis 3, BCOut 5, BCOutP 6, BCIn 7,
+ * BCSym 9; the others are rejected) and value[z][5] the conservative far-field / inflow state
+ * (host; may be NULL when no zone needs one).  The ghost states follow generateBoundaryValue
+ * (EulerEvaluator_EvaluateDt.hxx:1743-2390): EvaluateRHS takes them at every face point and,
+ * for the Roe average, at the face centre; the VR sweep with nVars = 5 takes them at the
+ * compressed point state (EulerSolver.hxx:163-175) with the given gamma. */
+int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma);
+/* Boundary values of the VR sweep given as data, for any nVars (the TFBoundary callback of
+ * DoReconstructionIter, VariationalReconstruction.hpp:893-901, evaluated by the caller):
+ * value is a device array [nVars][nGFace][nFace] of which the boundary faces are read; the
+ * library keeps the pointer.  Used when the Euler table does not apply (nVars != 5 or no
+ * dndsr_set_bc).  NULL clears it. */
+int dndsr_set_vr_dirichlet(dndsr_ctx *ctx, int nVars, const double *value);
 
 /* EvaluateDt, src/Euler/EulerEvaluator.hpp:359-366.  dtMinHost may be NULL (then no
  * host synchronisation happens; the minimum stays in dtMinDev if non-NULL). */

@@ -15,6 +15,7 @@
 #include <array>
 #include <cmath>
 #include <cstdint>
+#include <functional>
 #include <stdexcept>
 #include <string>
 #include <utility>
@@ -154,8 +155,10 @@ struct Mesh
     std::vector<double> coords; // [nNode][3]
     std::vector<int32_t> cellType, faceType;
     std::vector<int64_t> cell2nodePtr, cell2node, cell2facePtr, cell2face, face2nodePtr, face2node;
-    std::vector<int64_t> face2cell; // [nFace][2]
+    std::vector<int64_t> face2cell; // [nFace][2], -1 = boundary
     std::vector<double> faceShift;  // [nFace][3]
+    std::vector<int32_t> faceZone;  // [nFace]: zone of each boundary face (empty: no boundary face)
+    int32_t nBCZone = 0;
     dndsr_mesh_desc desc() const
     {
         dndsr_mesh_desc d{};
@@ -171,6 +174,8 @@ struct Mesh
         d.face2nodePtr = face2nodePtr.data(), d.face2node = face2node.data();
         d.face2cell = face2cell.data();
         d.faceShift = faceShift.data();
+        d.faceZone = faceZone.empty() ? nullptr : faceZone.data();
+        d.nBCZone = nBCZone;
         return d;
     }
 };
@@ -182,10 +187,13 @@ struct VRSettings
     int dirWeightScheme = 0, geomWeightScheme = 0; // Factorial, GWNone
     double geomWeightBias = 0, geomWeightPower = 0.5, scaleMultiplier = 1, jacobiRelax = 1;
     int nIterCellSmoothScale = 15;
+    double bcWeight = 1;
+    std::vector<double> zoneDirWeight; // [nBCZone][maxOrder+1]; empty = 1
     dndsr_vr_settings c() const
     {
         return {maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
-                scaleMultiplier, jacobiRelax, nIterCellSmoothScale};
+                scaleMultiplier, jacobiRelax, nIterCellSmoothScale, bcWeight,
+                zoneDirWeight.empty() ? nullptr : zoneDirWeight.data()};
     }
 };
 
@@ -201,18 +209,34 @@ class VariationalReconstruction
             throw std::runtime_error("dndsr_amd: mesh dimension does not match VariationalReconstruction<dim>");
     }
     // ConstructMetrics / ConstructBaseAndWeight / ConstructRecCoeff (VariationalReconstruction.hpp:282-289):
-    // the device builds all three at the first call
-    void ConstructMetrics() { construct(); }
-    void ConstructBaseAndWeight() { construct(); }
+    // the device builds all three at ConstructRecCoeff (or at the first use).
+    // id2faceDircWeight(zone, order) multiplies the weights of boundary faces, as the
+    // reference's tFGetBoundaryWeight does (VariationalReconstruction.cpp:438-441).
+    void ConstructMetrics() {}
+    void ConstructBaseAndWeight(const std::function<double(int, int)> &id2faceDircWeight = nullptr)
+    {
+        if (!id2faceDircWeight)
+            return;
+        std::vector<double> w((size_t)mesh_.nBCZone * (s_.maxOrder + 1));
+        for (int z = 0; z < mesh_.nBCZone; z++)
+            for (int p = 0; p <= s_.maxOrder; p++)
+                w[(size_t)z * (s_.maxOrder + 1) + p] = id2faceDircWeight(z, p);
+        if (constructed_ && w != s_.zoneDirWeight)
+            throw std::runtime_error("dndsr_amd: boundary weights must be set before ConstructRecCoeff");
+        s_.zoneDirWeight = w;
+    }
     void ConstructRecCoeff() { construct(); }
+    bool Constructed() const { return constructed_; }
 
-    index NumCell() const { return nCell_; }
-    index NumFace() const { return nFace_; }
-    int NumRecModes() const { return M_; }
+    index NumCell() { construct(); return nCell_; }
+    index NumFace() { construct(); return nFace_; }
+    int NumRecModes() { construct(); return M_; }
+    int NumBCZone() const { return mesh_.nBCZone; }
+    int MaxOrder() const { return s_.maxOrder; }
     Context &context() const { return ctx_; }
 
-    void BuildUDof(ArrayDOF &u, int nVars) { u = ArrayDOF(ctx_, nCell_, nVars, 1); }
-    void BuildURec(ArrayREC &uRec, int nVars) { uRec = ArrayREC(ctx_, nCell_, nVars, M_); }
+    void BuildUDof(ArrayDOF &u, int nVars) { construct(); u = ArrayDOF(ctx_, nCell_, nVars, 1); }
+    void BuildURec(ArrayREC &uRec, int nVars) { construct(); uRec = ArrayREC(ctx_, nCell_, nVars, M_); }
 
     // DoReconstructionIter (VariationalReconstruction.hpp:984-992), Jacobi form:
     // putIntoNew = false leaves the new iterate in uRec; recordInc writes -residual to uRecNew.
@@ -226,10 +250,12 @@ class VariationalReconstruction
         if (!putIntoNew)
             std::swap(uRec, uRecNew);
     }
-    // DoReconstructionIterDiff (VariationalReconstruction.hpp:1012-1018)
-    void DoReconstructionIterDiff(const ArrayREC &uRecDiff, ArrayREC &uRecNew)
+    // DoReconstructionIterDiff(uRec, uRecDiff, uRecNew, u) (VariationalReconstruction.hpp:1012-1018);
+    // uRec and u feed the boundary term and may be empty on meshes without boundary faces
+    void DoReconstructionIterDiff(const ArrayREC &uRec, const ArrayREC &uRecDiff, ArrayREC &uRecNew, const ArrayDOF &u)
     {
-        check(dndsr_vr_iter_diff(ctx_.handle(), uRecDiff.nVars(), uRecDiff.data(), uRecNew.data()));
+        check(dndsr_vr_iter_diff(ctx_.handle(), uRecDiff.nVars(), uRec.data(), u.data(), uRecDiff.data(),
+                                 uRecNew.data()));
     }
 
   private:
@@ -272,6 +298,10 @@ struct EulerEvaluatorSettings
     int riemannSolverType = 1; // Gas::RiemannSolverType numbering (Roe = 1, HLLC = 2, HLLEP = 3, Roe_M1.. = 11..)
     int rsMeanValueEig = 0;
     double rsFixScale = 1, rsIncFScale = 1;
+    // boundary zones (BoundaryHandler, src/Euler/EulerBC.hpp): EulerBCType code per zone and
+    // its 5 conservative values (far field / inflow); empty on periodic meshes
+    std::vector<int32_t> bcType;
+    std::vector<double> bcValue; // [nZone][5]
     dndsr_euler_settings c() const { return {gamma, riemannSolverType, rsMeanValueEig, rsFixScale, rsIncFScale}; }
 };
 
@@ -285,7 +315,23 @@ class EulerEvaluator
     EulerEvaluator(VariationalReconstruction<dim> &vfv, const EulerEvaluatorSettings &s)
         : settings(s), vfv_(vfv), es_(s.c())
     {
-        vfv.ConstructMetrics();
+        if (!s.bcType.empty())
+        {
+            if ((int)s.bcType.size() != vfv.NumBCZone())
+                throw std::runtime_error("dndsr_amd: one boundary type per mesh zone is needed");
+            // boundary weights of the VR functional by boundary type (EulerEvaluator.hpp:138-152):
+            // value only (Dirichlet type) except BCOut, which takes none
+            const std::vector<int32_t> types = s.bcType;
+            vfv.ConstructBaseAndWeight([types](int zone, int order) -> double {
+                if (types[zone] == 5 /* BCOut */)
+                    return 0.;
+                return order ? 0. : 1.;
+            });
+        }
+        vfv.ConstructRecCoeff();
+        if (!s.bcType.empty())
+            check(dndsr_set_bc(vfv.context().handle(), (int)s.bcType.size(), s.bcType.data(),
+                               s.bcValue.empty() ? nullptr : s.bcValue.data(), s.gamma));
         lambdaFace = DeviceBuffer<double>(4 * (size_t)vfv.NumFace());
         lambdaFace.zero();
         deltaLambdaCell = DeviceBuffer<double>((size_t)vfv.NumCell());

@@ -77,7 +77,7 @@ def test_vr_vjp_dot_product(require_gpu, kind, order, mode):
     yb = torch.randn((5, M, N), generator=gen, dtype=torch.float64).cuda()
     y = torch.empty_like(x)
     if mode == 3:
-        g.DoReconstructionIterDiff(x, y)
+        g.DoReconstructionIterDiff(None, x, y)
     else:
         g.DoReconstructionIter(x, y, u, putIntoNew=True, recordInc=mode >= 1, uRecIsZero=mode == 2, relax=0.8)
     xb, ub = adjoint.vr_vjp(g, yb, mode, 0.8)

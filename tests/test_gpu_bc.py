@@ -1,0 +1,221 @@
+"""Boundary conditions on the device vs the CPU oracle (FP64, relative L2 <= 1e-12 unless
+noted): meshes with far-field, inflow, back-pressure outflow, outflow, slip-wall, wall and
+symmetry zones; the VR coefficients with per-zone boundary weights, the VR sweep with the
+Euler ghost states or given point values, its Jacobian action, EvaluateDt, EvaluateRHS
+for several Riemann solvers, LU-SGS, SSP-RK3 and the implicit step."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from helpers import RS, colour_scan_order, random_rec, random_state, rel_l2, vr_settings
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+GAMMA = 1.4
+
+
+def _cons(rho, v, p):
+    v = np.asarray(v, float)
+    return np.array([rho, *(rho * v), p / (GAMMA - 1) + 0.5 * rho * v @ v])
+
+
+def _case(kind):
+    from dndsr_amd import bc
+    from dndsr_amd import mesh as mm
+    if kind == "2d":
+        m = mm.box_quad_grid(7, 6, 3.5, 3.0, zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=0.12, seed=2)
+        types = [bc.BC_FAR, bc.BC_OUT, bc.BC_WALL_INVIS, bc.BC_SYM]
+    else:
+        m = mm.box_hex_grid(5, 4, 3, 1.0, 0.8, 0.6, periodic=(False, True, False),
+                            zones=dict(xmin=0, xmax=1, zmin=2, zmax=3), jitter=0.1, seed=3)
+        types = [bc.BC_IN, bc.BC_OUTP, bc.BC_WALL, bc.BC_FAR]
+    vals = np.stack([_cons(1.0, [0.3, 0.1, 0.0], 1 / GAMMA), _cons(1.1, [0.4, 0.0, 0.05], 0.8),
+                     _cons(0.9, [0.2, -0.1, 0.0], 0.75), _cons(1.05, [-0.1, 0.2, 0.1], 0.9)])
+    return m, types, vals
+
+
+CASES = [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 1), ("3d", 2)]
+
+
+@pytest.fixture(scope="module", params=CASES, ids=[f"{k}-p{p}" for k, p in CASES])
+def pair(request, require_gpu):
+    import pyoracle as po
+    from dndsr_amd import bc
+    from dndsr_amd.cfv import VariationalReconstruction
+    kind, order = request.param
+    print(f"\n[bc parity] mesh={kind} p={order}", flush=True)
+    m, types, vals = _case(kind)
+    s = vr_settings(order)
+    zw = bc.euler_dir_weights(types, order)
+    o = po.OracleVR(m, **s, zoneDirWeight=zw)
+    o.set_bc(types, vals, GAMMA)
+    g = VariationalReconstruction(m, dict(s, zoneDirWeight=zw)).construct()
+    return m, o, g, types, vals
+
+
+def _ev(g, types, vals, rs="Roe", batch=0):
+    import pyoracle as po
+    from dndsr_amd.euler import EulerEvaluator
+    es = po.OracleVR.euler_settings(GAMMA, RS[rs], batch, 1.0, 1.0)
+    ev = EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType=rs, rsMeanValueEig=batch, bcTypes=types,
+                                bcValues=vals))
+    return es, ev
+
+
+def test_metrics_and_vr_coefficients(pair):
+    m, o, g, _, _ = pair
+    a, b = o.face_metrics(), g.face_metrics()
+    for k in ("area", "meanNorm", "pts", "norms", "wJDet", "faceL", "faceWeight"):
+        assert rel_l2(b[k], a[k]) <= TOL, k
+    bnd = m.face2cell[:, 1] < 0
+    assert np.any(b["faceWeight"][bnd, 1:] == 0)     # value-only boundary weights reached the device
+    Ao, bo = o.vr_coeffs()
+    Ag, bg = g.get_matrixAAInvB()
+    assert rel_l2(Ag, Ao) <= 1e-10 and rel_l2(bg, bo) <= 1e-10
+
+
+def test_vr_iter_euler_boundary(pair):
+    import torch
+    m, o, g, types, vals = pair
+    _, ev = _ev(g, types, vals)
+    Ao, bo = o.vr_coeffs()
+    g.set_matrixAAInvB(Ao, bo)
+    N, M = o.nCell, o.M
+    u = random_state(N, seed=3)
+    uRec = random_rec(N, M, 5, seed=4)
+    du, dr = g.to_device(u), g.to_device(uRec)
+    out = torch.zeros_like(dr)
+    g.DoReconstructionIter(dr, out, du)
+    ref = o.vr_iter(uRec, u)
+    assert rel_l2(g.to_host(out), ref) <= TOL
+    g.DoReconstructionIter(dr, out, du, relax=0.7)
+    assert rel_l2(g.to_host(out), 0.3 * uRec + 0.7 * ref) <= TOL
+    g.DoReconstructionIter(dr, out, du, recordInc=True)
+    assert rel_l2(g.to_host(out), uRec - ref) <= 1e-11
+    x = random_rec(N, M, 5, seed=6, amp=1e-3)
+    g.DoReconstructionIterDiff(dr, g.to_device(x), out, du)
+    assert rel_l2(g.to_host(out), o.vr_iter_diff(x, uRec, u)) <= 1e-11
+
+
+def test_vr_iter_point_values(pair):
+    """nVars = 1 with boundary values given as data (the reference tests' TFBoundary)."""
+    import torch
+    m, o, g, _, _ = pair
+    Ao, bo = o.vr_coeffs()
+    g.set_matrixAAInvB(Ao, bo)
+    N, M, F, G = o.nCell, o.M, o.nFace, o.nGFace
+    val = np.random.default_rng(8).uniform(-1, 1, (F, G, 1))
+    o.set_vr_dirichlet(val)
+    dval = torch.from_numpy(np.ascontiguousarray(val.transpose(2, 1, 0))).to(g.ctx.torch_device)
+    g.set_vr_dirichlet(dval)
+    u = np.random.default_rng(9).uniform(-1, 1, (N, 1))
+    uRec = random_rec(N, M, 1, seed=10)
+    out = torch.zeros((1, M, N), dtype=torch.float64, device=g.ctx.torch_device)
+    g.DoReconstructionIter(g.to_device(uRec), out, g.to_device(u))
+    assert rel_l2(g.to_host(out), o.vr_iter(uRec, u)) <= TOL
+    g.set_vr_dirichlet(None)
+
+
+def test_eval_dt(pair):
+    m, o, g, types, vals = pair
+    es, ev = _ev(g, types, vals)
+    u = random_state(o.nCell, seed=5)
+    du = g.to_device(u)
+    dt = g.BuildUDof(1)[0]
+    dtmin = ev.EvaluateDt(dt, du, None, CFL=0.7, UseLocaldt=True)
+    dto, lamo, dlco, dmino = o.eval_dt(es, u, 0.7, 1e100, True)
+    assert rel_l2(dt.cpu().numpy(), dto) <= TOL
+    assert rel_l2(ev.lambdaFace.cpu().numpy().T, lamo) <= TOL
+    assert rel_l2(ev.deltaLambdaCell.cpu().numpy(), dlco) <= TOL
+    assert abs(dtmin - dmino) <= TOL * dmino
+
+
+@pytest.mark.parametrize("rs,batch", [("Roe", 0), ("HLLEP", 0), ("HLLC", 0), ("Roe_M1", 1), ("Roe_M3", 0),
+                                      ("HLLEP_V1", 0)])
+def test_eval_rhs(pair, rs, batch):
+    m, o, g, types, vals = pair
+    es, ev = _ev(g, types, vals, rs, batch)
+    N, M = o.nCell, o.M
+    u = random_state(N, seed=11)
+    uRec = random_rec(N, M, 5, seed=12)
+    du, dr = g.to_device(u), g.to_device(uRec)
+    dt = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dt, du, None, CFL=1.0)
+    _, _, dlco, _ = o.eval_dt(es, u, 1.0, 1e100, True)
+    rhs = g.BuildUDof(5)
+    ev.EvaluateRHS(rhs, None, du, dr, dr)
+    assert rel_l2(g.to_host(rhs), o.eval_rhs(es, u, uRec, dlco)) <= TOL
+    ev.EvaluateRHS(rhs, None, du, None, None)
+    assert rel_l2(g.to_host(rhs), o.eval_rhs(es, u, None, dlco)) <= TOL
+
+
+def test_lusgs_and_matvec(pair):
+    m, o, g, types, vals = pair
+    es, ev = _ev(g, types, vals)
+    N = o.nCell
+    u = random_state(N, seed=21)
+    du = g.to_device(u)
+    dt = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dt, du, None, CFL=5.0)
+    dto, lamo, _, _ = o.eval_dt(es, u, 5.0, 1e100, True)
+    jd = g.BuildUDof(1)[0]
+    ev.LUSGSMatrixInit(jd, None, dt, 0.3, 1.0, du, None)
+    jdo = o.lusgs_init(dto, 0.3, 1.0, lamo)
+    assert rel_l2(jd.cpu().numpy(), jdo) <= TOL
+    rhs = np.random.default_rng(22).uniform(-1, 1, (N, 5))
+    drhs = g.to_device(rhs)
+    order = colour_scan_order(g.colors())
+    dInc, dNew = g.BuildUDof(5), g.BuildUDof(5)
+    ev.UpdateSGS(1.0, 0.0, drhs, du, dInc, dNew, jd, True, True, True)
+    r1, _ = o.update_sgs(es, 1.0, rhs, u, np.zeros_like(u), np.zeros_like(u), jdo, lamo, True, True, True, order)
+    assert rel_l2(g.to_host(dNew), r1) <= TOL
+    x = np.random.default_rng(24).uniform(-0.1, 0.1, (N, 5))
+    Ax = g.BuildUDof(5)
+    ev.LUSGSMatrixVec(1.0, 0.0, du, g.to_device(x), jd, Ax)
+    assert rel_l2(g.to_host(Ax), o.lusgs_matvec(es, 1.0, u, x, jdo, lamo)) <= TOL
+
+
+def _flow(o, g, kind):
+    """Smooth positive state: far-field base flow with a bump in density and pressure."""
+    from dndsr_amd.cfv import VariationalReconstruction  # noqa: F401
+    pts, wj = g.cell_quad()
+    rho = 1.0 + 0.1 * np.exp(-8 * ((pts[..., 0] - 0.5 * pts[..., 0].max()) ** 2))
+    v = np.array([0.3, 0.05, 0.0])
+    p = 1 / GAMMA * (1 + 0.05 * np.sin(3 * pts[..., 1]))
+    U = np.stack([rho, rho * v[0], rho * v[1], rho * v[2], p / (GAMMA - 1) + 0.5 * rho * (v @ v)], -1)
+    vol = g.cell_metrics()["vol"]
+    return np.einsum("ngv,ng->nv", U, wj) / vol[:, None]
+
+
+@pytest.mark.parametrize("kind,rs", [("2d", "Roe"), ("2d", "HLLEP"), ("3d", "HLLEP")])
+def test_ssprk3_and_implicit_steps(require_gpu, kind, rs):
+    import pyoracle as po
+    from dndsr_amd import bc
+    from dndsr_amd.cfv import VariationalReconstruction
+    print(f"\n[bc steps] {kind} {rs}", flush=True)
+    m, types, vals = _case(kind)
+    order = 1 if kind == "2d" else 2
+    s = vr_settings(order, intOrder=3 if kind == "2d" else 4)
+    zw = bc.euler_dir_weights(types, order)
+    o = po.OracleVR(m, **s, zoneDirWeight=zw)
+    o.set_bc(types, vals, GAMMA)
+    g = VariationalReconstruction(m, dict(s, bcTypes=types)).construct()
+    es, ev = _ev(g, types, vals, rs)
+    u0 = _flow(o, g, kind)
+    du, dr = g.to_device(u0), g.BuildURec(5)
+    ev.SSPRK3(du, dr, 3, 0.4, 1)
+    uo, ro = o.ssprk3(es, u0, np.zeros((o.nCell, 5, o.M)), 3, 0.4, 1)
+    assert rel_l2(g.to_host(du), uo) <= 1e-11
+    assert rel_l2(g.to_host(dr), ro) <= 1e-10
+    dt = 5 * o.eval_dt(es, u0, 0.5, 1e100, False)[3]
+    ordr = colour_scan_order(g.colors())
+    for nIt in (1, 3):
+        du, dr = g.to_device(u0), g.BuildURec(5)
+        n, res, rel, hist = ev.ImplicitEulerStep(du, dr, dt, CFL=10.0, maxIter=nIt, minIter=nIt, tol=0.0)
+        ui, ri, ni, resi, reli, histi = o.implicit_euler_step(es, u0, np.zeros((o.nCell, 5, o.M)), dt, CFL=10.0,
+                                                              maxIter=nIt, minIter=nIt, tol=0.0, scanOrder=ordr)
+        assert n == ni == nIt
+        assert rel_l2(g.to_host(du), ui) <= 1e-11
+        assert rel_l2(g.to_host(dr), ri) <= 1e-10

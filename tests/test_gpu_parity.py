@@ -87,7 +87,7 @@ def test_vr_iter_modes(pair, nVars):
     g.DoReconstructionIter(dr, out, du, recordInc=True, uRecIsZero=True)
     assert rel_l2(g.to_host(out), -o.vr_iter(np.zeros_like(uRec), u)) <= TOL
     # Jacobian action
-    g.DoReconstructionIterDiff(dr, out)
+    g.DoReconstructionIterDiff(None, dr, out)
     assert rel_l2(g.to_host(out), o.vr_iter_diff(uRec)) <= TOL
 
 
