@@ -373,3 +373,23 @@ def box_hex_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.0, Lz
                 np.arange(F + 1, dtype=np.int64) * 4, face2node.ravel().astype(np.int64),
                 face2cell.astype(np.int64), faceShift.astype(np.float64), (nx, ny, nz),
                 None if nzone == 0 else zone.astype(np.int32), nzone)
+
+
+def bump_channel_hex(nx: int, ny: int, nz: int, L: float = 3.0, H: float = 1.0, W: float = 0.5,
+                     amp: float = 0.1, jitter: float = 0.0, seed: int = 0, stretch: float = 1.0) -> Mesh:
+    """Channel [0, L] x [0, H] x [0, W] with a sinusoidal bump of height amp * H on the lower
+    wall over x in [L/3, 2L/3] (y_wall = amp H sin^2(3 pi x / L - pi)); the grid lines follow
+    the wall, y' = y + y_wall(x) (1 - y / H), and stretch > 1 clusters them toward the lower
+    wall (y -> H (stretch^(y/H) - 1) / (stretch - 1)).  Spanwise (z) periodic.  Zones: 0 inlet
+    (x = 0), 1 outlet (x = L), 2 lower wall, 3 upper boundary."""
+    def bump(P):
+        x, y = P[:, 0], P[:, 1]
+        if stretch != 1.0:
+            y = H * (stretch ** (y / H) - 1) / (stretch - 1)
+        s = np.where((x > L / 3) & (x < 2 * L / 3), np.sin(3 * np.pi * x / L - np.pi) ** 2, 0.0)
+        Q = P.copy()
+        Q[:, 1] = y + amp * H * s * (1 - y / H)
+        return Q
+
+    return box_hex_grid(nx, ny, nz, L, H, W, periodic=(False, False, True),
+                        zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)
