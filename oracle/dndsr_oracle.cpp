@@ -1767,6 +1767,91 @@ int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const d
     ORC_CATCH(-1)
 }
 
+
+// ---------------------------------------------------------------------------
+// Laminar viscous terms
+// ---------------------------------------------------------------------------
+static constexpr real veryLargeReal = 3e200; // src/DNDS/Defines.hpp:195
+
+static real cpOf(const orc_euler_settings *es) { return es->Rgas * es->gamma / (es->gamma - 1); }
+
+// muEff, EulerEvaluator.hpp:751-776 (no turbulence)
+static real muEffOf(const orc_euler_settings *es, const real *U, real T)
+{
+    switch (es->muModel)
+    {
+    case 0: return es->muGas;
+    case 1:
+    {
+        real TRel = T / es->TRef;
+        return es->muGas * TRel * std::sqrt(TRel) * (es->TRef + es->CSutherland) / (T + es->CSutherland);
+    }
+    case 2: return es->muGas * U[0];
+    default: throw std::runtime_error("no such muModel");
+    }
+}
+
+// GradientCons2Prim_IdealGas, Gas.hpp:1678-1698; G[d][var] = d U_var / d x_d
+static void gradCons2Prim(const real *U, const real (*G)[5], real (*Gp)[5], real gamma)
+{
+    real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    for (int d = 0; d < 3; d++)
+        for (int k = 0; k < 5; k++)
+            Gp[d][k] = G[d][k];
+    for (int d = 0; d < 3; d++)
+        for (int j = 0; j < 3; j++)
+            Gp[d][1 + j] = (1.0 / sqr(U[0])) * (U[0] * G[d][1 + j] - G[d][0] * U[1 + j]);
+    for (int d = 0; d < 3; d++)
+    {
+        real a = 0, b = 0;
+        for (int j = 0; j < 3; j++)
+            a += G[d][1 + j] * velo[j], b += Gp[d][1 + j] * U[1 + j];
+        Gp[d][4] = (gamma - 1) * (G[d][4] - 0.5 * (a + b));
+    }
+}
+
+// ViscousFlux_IdealGas, Gas.hpp:1595-1655 (laminar: no QCR); Gp[d][1 + j] = d u_j / d x_d
+static void viscousFlux(const real *U, const real (*Gp)[5], const real *n, bool adiabatic, real gamma, real mu, real k,
+                        real Cp, real *F)
+{
+    const real lambda = -2. / 3.;
+    real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    real vSqr = velo[0] * velo[0] + velo[1] * velo[1] + velo[2] * velo[2];
+    real p = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
+    real GradT[3];
+    for (int d = 0; d < 3; d++)
+        GradT[d] = (gamma / ((gamma - 1) * Cp * U[0] * U[0])) * (U[0] * Gp[d][4] - p * Gp[d][0]);
+    if (adiabatic)
+    {
+        real tn = dot3(GradT, n);
+        for (int d = 0; d < 3; d++)
+            GradT[d] -= tn * n[d];
+    }
+    real tr = Gp[0][1] + Gp[1][2] + Gp[2][3];
+    real S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            S[i][j] = (Gp[i][1 + j] + Gp[j][1 + i]) * mu + (i == j ? lambda * mu * tr : 0.0);
+    F[0] = 0;
+    for (int i = 0; i < 3; i++)
+        F[1 + i] = S[i][0] * n[0] + S[i][1] * n[1] + S[i][2] * n[2];
+    real e = 0;
+    for (int i = 0; i < 3; i++)
+        e += (S[i][0] * velo[0] + S[i][1] * velo[1] + S[i][2] * velo[2] + k * GradT[i]) * n[i];
+    F[4] = e;
+}
+
+// lamVis of EvaluateDt, EulerEvaluator_EvaluateDt.hxx:964-979 (no turbulence)
+static real lamVisOf(const orc_euler_settings *es, const real *uMean, real pMean)
+{
+    if (!es->viscous)
+        return 0.0;
+    real gamma = es->gamma, Cp = cpOf(es);
+    real T = pMean / ((gamma - 1) / gamma * Cp * uMean[0]);
+    real muf = muEffOf(es, uMean, T);
+    return muf / uMean[0] * std::max(4. / 3., gamma / es->prGas);
+}
+
 // EvaluateDt, EulerEvaluator_EvaluateDt.hxx:840-1048 (inviscid: lamVis = 0; a boundary face
 // takes UR = UL, :880-899)
 int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, double CFL,
@@ -1799,7 +1884,10 @@ int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, 
         }
         real aL = std::sqrt(aL2), aR = std::sqrt(aR2);
         real lambdaConvection = std::max(aL + std::abs(veloNL - vgN), aR + std::abs(veloNR - vgN));
-        real lamVis = 0.0;
+        real uMean[5];
+        for (int k = 0; k < 5; k++)
+            uMean[k] = cR >= 0 ? (UL[k] + UR[k]) * 0.5 : UL[k];
+        real lamVis = lamVisOf(es, uMean, (pL + pR) * 0.5);
         real volR = cR >= 0 ? v.vol[cR] : v.vol[cL];
         real visc = lamVis * v.area[f] * (1. / v.vol[cL] + 1. / volR);
         lamF[f] = lambdaConvection + visc;
@@ -1856,6 +1944,9 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
     std::vector<real> D;
     std::vector<std::array<real, 5>> ULv(G), URv(G), fincC(G);
     std::vector<std::array<real, 3>> nv(G);
+    std::vector<std::array<std::array<real, 5>, 3>> GLv(G), GRv(G); // point gradients d U / d x_d
+    const bool vis = es->viscous != 0;
+    const int nDiff = vis ? v.dim + 1 : 1;
 #pragma omp for schedule(static)
     for (idx_t f = 0; f < v.nFace; f++)
     {
@@ -1875,7 +1966,7 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
         {
             // ULxy = u[L] + (D0Bj(L) * uRec[L])^T, EvaluateRHS.hxx:235-243, 287-297
             real UL[5], UR[5];
-            v.diffBase(cL, v.facePointFromSide(f, 0, v.faceQP[f][g]), 1, 0, D);
+            v.diffBase(cL, v.facePointFromSide(f, 0, v.faceQP[f][g]), nDiff, 0, D);
             for (int var = 0; var < 5; var++)
             {
                 real r = 0;
@@ -1884,11 +1975,24 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
                         r += D[1 + k] * uRec[((size_t)cL * 5 + var) * M + k];
                 UL[var] = ULm[var] + r;
             }
-            if (bnd) // URxy = generateBoundaryValue(ULxy, ...), EvaluateRHS.hxx:336-345
+            // GradULxy = D1phi(L) uRec[L], EvaluateRHS.hxx:257-273 (zero rows beyond the mesh dimension)
+            for (int d = 0; d < 3; d++)
+                for (int var = 0; var < 5; var++)
+                {
+                    real r = 0;
+                    if (vis && uRec && d < v.dim)
+                        for (int k = 0; k < M; k++)
+                            r += D[(size_t)(1 + d) * v.NDOF + 1 + k] * uRec[((size_t)cL * 5 + var) * M + k];
+                    GLv[g][d][var] = r;
+                }
+            if (bnd) // URxy = generateBoundaryValue(ULxy, ...), GradURxy = GradULxy, EvaluateRHS.hxx:336-355
+            {
                 bcGhost(v.bcType[zone], UL, v.faceN[f][g].x, v.bcValue[zone].data(), es->gamma, UR);
+                GRv[g] = GLv[g];
+            }
             else
             {
-                v.diffBase(cR, v.facePointFromSide(f, 1, v.faceQP[f][g]), 1, 0, D);
+                v.diffBase(cR, v.facePointFromSide(f, 1, v.faceQP[f][g]), nDiff, 0, D);
                 for (int var = 0; var < 5; var++)
                 {
                     real r = 0;
@@ -1897,6 +2001,15 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
                             r += D[1 + k] * uRec[((size_t)cR * 5 + var) * M + k];
                     UR[var] = URm[var] + r;
                 }
+                for (int d = 0; d < 3; d++)
+                    for (int var = 0; var < 5; var++)
+                    {
+                        real r = 0;
+                        if (vis && uRec && d < v.dim)
+                            for (int k = 0; k < M; k++)
+                                r += D[(size_t)(1 + d) * v.NDOF + 1 + k] * uRec[((size_t)cR * 5 + var) * M + k];
+                        GRv[g][d][var] = r;
+                    }
             }
             for (int var = 0; var < 5; var++)
                 ULv[g][var] = UL[var], URv[g][var] = UR[var];
@@ -1923,6 +2036,55 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
             for (size_t g = 0; g < G; g++)
                 dispatchFlux(rs, ULv[g].data(), URv[g].data(), ULm, URm, nv[g].data(), es->gamma, fincC[g].data(),
                              dLambda, es->rsFixScale, es->rsIncFScale, lam0, lam123, lam4);
+        if (vis)
+        {
+            // viscous flux, EvaluateRHS.hxx:346-424 and fluxFace EvaluateDt.hxx:1163-1215, finc -= visFlux (:1460)
+            const int bt = bnd ? v.bcType[zone] : 0;
+            const bool noGrad = bnd && (bt == ORC_BC_WALL_INVIS || bt == ORC_BC_OUT || bt == ORC_BC_FAR || bt == ORC_BC_SYM);
+            real minVol = v.vol[cL];
+            Pt dB;
+            if (!bnd)
+            {
+                minVol = std::min(minVol, v.vol[cR]);
+                Pt ob = v.otherBaryFromCell(cL, cR, f);
+                for (int d = 0; d < 3; d++)
+                    dB.x[d] = ob.x[d] - v.bary[cL].x[d];
+            }
+            else
+                for (int d = 0; d < 3; d++)
+                    dB.x[d] = v.faceCent[f].x[d] - v.bary[cL].x[d];
+            const real gamma = es->gamma, Cp = cpOf(es);
+            for (size_t g = 0; g < G; g++)
+            {
+                const real *n = nv[g].data();
+                real distBaryPerp = std::abs(dot3(dB.x, n)) * (bnd ? 2.0 : 1.0);
+                real distGRP = minVol / v.area[f] * 2;
+                distGRP = std::max(std::min(distBaryPerp, distGRP * 2), distGRP * 0.25);
+                if (bnd && (bt == ORC_BC_WALL_INVIS || bt == ORC_BC_SYM))
+                    distGRP += veryLargeReal;
+                real Um[5], Gm[3][5], Gp[3][5];
+                for (int k = 0; k < 5; k++)
+                    Um[k] = 0.5 * (ULv[g][k] + URv[g][k]);
+                for (int d = 0; d < 3; d++)
+                    for (int k = 0; k < 5; k++)
+                        Gm[d][k] = noGrad ? 0.0
+                                          : (GRv[g][d][k] + GLv[g][d][k]) * 0.5 +
+                                                (1.0 / distGRP) * (n[d] * (URv[g][k] - ULv[g][k]));
+                gradCons2Prim(Um, Gm, Gp, gamma);
+                real pM, a2M, HM;
+                thermal(Um[4], Um[0], sqr(Um[1] / Um[0]) + sqr(Um[2] / Um[0]) + sqr(Um[3] / Um[0]), gamma, pM, a2M, HM);
+                real T = pM / ((gamma - 1) / gamma * Cp * Um[0]);
+                real mu = muEffOf(es, Um, T);
+                real kc = Cp * 0.0 / 0.9 + Cp * mu / es->prGas;
+                real Fv[5];
+                viscousFlux(Um, Gp, n, bnd && bt == ORC_BC_WALL, gamma, mu, kc, Cp, Fv);
+                if (bnd && (bt == ORC_BC_WALL_INVIS || bt == ORC_BC_SYM))
+                    for (int k = 0; k < 5; k++)
+                        Fv[k] = 0.0;
+                for (int k = 0; k < 5; k++)
+                    fincC[g][k] -= Fv[k];
+            }
+        }
         real fluxEs[5] = {0, 0, 0, 0, 0};
         for (size_t g = 0; g < G; g++)
             for (int var = 0; var < 5; var++)

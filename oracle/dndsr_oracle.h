@@ -71,6 +71,11 @@ typedef struct
     int rsMeanValueEig; /* batched mean-value eigen path for Roe_M1..Roe_M5 */
     double rsFixScale;
     double rsIncFScale;
+    /* laminar viscous terms (idealGasProperty, EulerEvaluatorSettings.hpp:288-326);
+     * viscous = 0: Euler.  muModel 0 constant, 1 Sutherland, 2 constant nu */
+    int viscous;
+    int muModel;
+    double Rgas, muGas, prGas, TRef, CSutherland;
 } orc_euler_settings;
 
 typedef struct orc_vr orc_vr;

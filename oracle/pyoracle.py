@@ -38,7 +38,9 @@ class VRSettings(C.Structure):
 
 class EulerSettings(C.Structure):
     _fields_ = [("gamma", C.c_double), ("rsType", C.c_int), ("rsMeanValueEig", C.c_int),
-                ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double)]
+                ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double), ("viscous", C.c_int),
+                ("muModel", C.c_int), ("Rgas", C.c_double), ("muGas", C.c_double), ("prGas", C.c_double),
+                ("TRef", C.c_double), ("CSutherland", C.c_double)]
 
 
 class ImplicitSettings(C.Structure):
@@ -226,8 +228,10 @@ class OracleVR:
 
     # ---- Euler
     @staticmethod
-    def euler_settings(gamma=1.4, rsType=1, rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0):
-        return EulerSettings(gamma, rsType, rsMeanValueEig, rsFixScale, rsIncFScale)
+    def euler_settings(gamma=1.4, rsType=1, rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0, viscous=0,
+                       muModel=1, Rgas=1.0, muGas=1.0, prGas=0.72, TRef=273.15, CSutherland=110.4):
+        return EulerSettings(gamma, rsType, rsMeanValueEig, rsFixScale, rsIncFScale, viscous, muModel, Rgas, muGas,
+                             prGas, TRef, CSutherland)
 
     def eval_dt(self, es, u, CFL, maxDt=1e100, useLocalDt=True):
         N, F = self.nCell, self.nFace
