@@ -35,7 +35,9 @@ class VRSettings(C.Structure):
 
 class EulerSettings(C.Structure):
     _fields_ = [("gamma", C.c_double), ("rsType", C.c_int), ("rsMeanValueEig", C.c_int),
-                ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double)]
+                ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double), ("viscous", C.c_int),
+                ("muModel", C.c_int), ("Rgas", C.c_double), ("muGas", C.c_double), ("prGas", C.c_double),
+                ("TRef", C.c_double), ("CSutherland", C.c_double)]
 
 
 class ImplicitSettings(C.Structure):

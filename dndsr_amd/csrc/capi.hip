@@ -706,7 +706,14 @@ static GasParams toGas(const dndsr_euler_settings *es)
 {
     need(es != nullptr, "null euler settings");
     need(es->gamma > 1.0, "gamma must exceed 1");
-    return GasParams{es->gamma, es->rsType, es->rsMeanValueEig, es->rsFixScale, es->rsIncFScale};
+    if (es->viscous)
+    {
+        need(es->muModel >= 0 && es->muModel <= 2, "muModel must be 0 (constant), 1 (Sutherland) or 2 (constant nu)");
+        need(es->Rgas > 0 && es->prGas > 0 && es->muGas >= 0, "viscous gas needs Rgas > 0, prGas > 0, muGas >= 0");
+        need(es->muModel != 1 || (es->TRef > 0 && es->CSutherland >= 0), "Sutherland's law needs TRef > 0, CSutherland >= 0");
+    }
+    return GasParams{es->gamma, es->rsType, es->rsMeanValueEig, es->rsFixScale, es->rsIncFScale, es->viscous ? 1 : 0,
+                     es->muModel, es->Rgas, es->muGas, es->prGas, es->TRef, es->CSutherland};
 }
 
 int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, double CFL, double maxDt,
@@ -1071,6 +1078,7 @@ int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dou
     requireMesh(ctx);
     need(ctx->c.m.nBnd == 0, "the discrete adjoint covers periodic / interior meshes only");
     need(es && u && rhsBar && uBar, "null field");
+    need(!es->viscous, "the discrete adjoint covers the inviscid residual only");
     need(!uRecBar || uRec, "uRecBar needs uRec");
     Ctx &c = ctx->c;
     if (!c.adjPoint)

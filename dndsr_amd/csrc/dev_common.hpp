@@ -151,6 +151,10 @@ struct GasParams
     int rsMeanValueEig;
     real fixScale;
     real incFScale;
+    // laminar viscosity (idealGasProperty, src/Euler/EulerEvaluatorSettings.hpp:280-330)
+    int viscous;  // 0: inviscid residual and time step
+    int muModel;  // 0 constant mu, 1 Sutherland, 2 constant nu
+    real Rgas, muGas, prGas, TRef, CSutherland;
 };
 
 } // namespace dndsr

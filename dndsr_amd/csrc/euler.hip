@@ -13,9 +13,10 @@ namespace dndsr
 // deltaLambdaCell = max over faces; then a two-stage min for dtMin.
 // lambdaFace layout [4][F]: lambdaFace, lambdaFaceC, lambdaFaceVis, lambdaFace0(=123=4).
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(256) kDtFace(DevMesh m, real gamma, const real *__restrict__ u, real *__restrict__ lam,
-                                               real *__restrict__ dLamF)
+__global__ void __launch_bounds__(256) kDtFace(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                               real *__restrict__ lam, real *__restrict__ dLamF)
 {
+    const real gamma = gp.gamma;
     idx f = blockIdx.x * blockDim.x + threadIdx.x;
     if (f >= m.F)
         return;
@@ -40,6 +41,17 @@ __global__ void __launch_bounds__(256) kDtFace(DevMesh m, real gamma, const real
     real aL = sqrt(aL2), aR = sqrt(aR2);
     real lc = fmax(aL + fabs(vnL), aR + fabs(vnR));
     real visc = 0.0; // inviscid: lamVis = 0
+    if (gp.viscous)
+    {
+        // lamVis at the mean of the two cell means (a boundary face has UR = UL),
+        // times A (1 / V_L + 1 / V_R), EvaluateDt.hxx:964-992
+        real uMean[5];
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            uMean[k] = (UL[k] + UR[k]) * 0.5;
+        const real lamVis = lamVisFace(gp, uMean, (pL + pR) * 0.5);
+        visc = lamVis * m.farea[f] * (1. / m.vol[cL] + 1. / m.vol[cR2]);
+    }
     lam[f] = lc + visc;
     lam[F + f] = fabs(vnMean) + visc;
     lam[2 * F + f] = visc;
@@ -116,7 +128,7 @@ void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real max
     DevMesh &m = c.m;
     {
         KernelTimer kt(c, K_DT_FACE);
-        kDtFace<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, gp.gamma, u, lambdaFace, c.faceLamTmp);
+        kDtFace<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, gp, u, lambdaFace, c.faceLamTmp);
         DNDSR_HIP_CHECK(hipGetLastError());
     }
     int nb = gridFor(m.N, 256);
@@ -902,6 +914,160 @@ __global__ void __launch_bounds__(64) kRHSBoundary(DevMesh m, GasParams gp, cons
         fflux[(size_t)v * F + f] = acc[v];
 }
 
+// ---------------------------------------------------------------------------
+// Laminar viscous face fluxes (EvaluateRHS.hxx:257-424 with fluxFace,
+// EvaluateDt.hxx:1163-1215; finc -= visFlux at :1460).  One thread per face, after
+// the inviscid face and boundary passes: at every face point both sides' values and
+// gradients D1phi uRec, the GRP mean gradient
+//   GradUMean = (GradL + GradR) / 2 + n (UR - UL) / distGRP,
+//   distGRP = max(min(distBaryPerp, 2 d0), d0 / 4), d0 = 2 minVol / area,
+// its primitive form at the mean state, and the Navier-Stokes flux; the face flux
+// gains sum_g F_vis w_g (JDet + tiny).  A boundary point takes the ghost state and
+// GradR = GradL; slip walls, symmetry planes, far field and outflow carry no viscous
+// flux (their mean gradient is zeroed, :427-435, or the flux is, :1200-1204).
+// ---------------------------------------------------------------------------
+// Value and gradient of the reconstruction of cell c at point p (in the cell's frame).
+template <int DIM, int NDOF>
+__device__ inline void pointValueGrad(const DevMesh &m, idx c, const real *p, const real *Um,
+                                      const real *__restrict__ uRec, real *U, real (*Gr)[5])
+{
+    constexpr int M = NDOF - 1;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        U[v] = Um[v];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            Gr[d][v] = 0;
+    }
+    if (!uRec)
+        return;
+    const size_t N = m.N;
+    const real s = m.scale[c];
+    real xs[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        xs[r] = p[r] - m.cent[(size_t)r * N + c];
+    xs[0] /= s, xs[1] /= s;
+    if (DIM == 3)
+        xs[2] /= s;
+    real phi[M], dphi[DIM][M];
+#pragma unroll
+    for (int j = 0; j < M; j++)
+    {
+        phi[j] = basisDiff(DIM, 0, j + 1, xs, s) - m.moment[(size_t)j * N + c];
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            dphi[d][j] = basisDiff(DIM, 1 + d, j + 1, xs, s);
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        real a = 0, g[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            g[d] = 0;
+#pragma unroll
+        for (int j = 0; j < M; j++)
+        {
+            const real r = uRec[((size_t)v * M + j) * N + c];
+            a += phi[j] * r;
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                g[d] += dphi[d][j] * r;
+        }
+        U[v] += a;
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            Gr[d][v] = g[d];
+    }
+}
+
+template <int DIM, int NDOF>
+__global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                                   const real *__restrict__ uRec, real *__restrict__ fflux)
+{
+    const idx f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F)
+        return;
+    const size_t N = m.N, F = m.F;
+    const idx cL = m.f2c[f], cR = m.f2c[F + f];
+    const bool bnd = cR < 0;
+    const int type = bnd ? m.bcType[m.fzone[f]] : 0;
+    if (bnd && (type == BC_WALL_INVIS || type == BC_SYM || type == BC_FAR || type == BC_OUT))
+        return;
+    const real *far = bnd ? m.bcVal + (size_t)m.fzone[f] * 5 : nullptr;
+    real ULm[5], URm[5], dB[3], sh[3];
+    real minVol = m.vol[cL];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        ULm[v] = u[(size_t)v * N + cL], URm[v] = bnd ? 0.0 : u[(size_t)v * N + cR];
+    if (!bnd)
+    {
+        minVol = fmin(minVol, m.vol[cR]);
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+        {
+            sh[r] = m.fshift[(size_t)r * F + f];
+            dB[r] = (m.bary[(size_t)r * N + cR] - sh[r]) - m.bary[(size_t)r * N + cL];
+        }
+    }
+    else
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            sh[r] = 0, dB[r] = m.fcent[(size_t)r * F + f] - m.bary[(size_t)r * N + cL];
+    const real d0 = minVol / m.farea[f] * 2;
+    const real gamma = gp.gamma, Cp = cpGas(gp);
+    real acc[5] = {0, 0, 0, 0, 0};
+    for (int g = 0; g < m.G; g++)
+    {
+        real x[3], n[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            x[r] = m.fpts[((size_t)g * 3 + r) * F + f], n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+        real UL[5], UR[5], GL[3][5], GR[3][5];
+        pointValueGrad<DIM, NDOF>(m, cL, x, ULm, uRec, UL, GL);
+        if (bnd)
+        {
+            bcGhost(type, UL, n, far, gamma, UR);
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    GR[d][v] = GL[d][v];
+        }
+        else
+        {
+            real xR[3] = {x[0] + sh[0], x[1] + sh[1], x[2] + sh[2]};
+            pointValueGrad<DIM, NDOF>(m, cR, xR, URm, uRec, UR, GR);
+        }
+        const real distBaryPerp = fabs(dot3(dB, n)) * (bnd ? 2.0 : 1.0);
+        const real distGRP = fmax(fmin(distBaryPerp, d0 * 2), d0 * 0.25);
+        real Um[5], Gm[3][5], Gp[3][5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            Um[v] = 0.5 * (UL[v] + UR[v]);
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                Gm[d][v] = (GR[d][v] + GL[d][v]) * 0.5 + (1.0 / distGRP) * (n[d] * (UR[v] - UL[v]));
+        gradCons2Prim(Um, Gm, Gp, gamma);
+        real pM, a2M, HM;
+        thermal(Um[4], Um[0], sq(Um[1] / Um[0]) + sq(Um[2] / Um[0]) + sq(Um[3] / Um[0]), gamma, pM, a2M, HM);
+        const real mu = muEff(gp, Um, gasTemperature(gp, pM, Um[0]));
+        real Fv[5];
+        viscousFlux(Um, Gp, n, bnd && type == BC_WALL, gamma, mu, Cp * mu / gp.prGas, Cp, Fv);
+        const real wj = m.fwj[(size_t)g * F + f];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            acc[v] += Fv[v] * wj;
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        fflux[(size_t)v * F + f] += acc[v];
+}
+
 template <int DIM, int NDOF, int G, int RS, bool BATCH>
 static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
 {
@@ -1021,6 +1187,32 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
             rhsDispatch<3, 10>(c, gp, u, uRec, dlc, ff);
         else
             rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
+    }
+    if (gp.viscous)
+    {
+        if (!(gp.prGas > 0 && gp.Rgas > 0 && gp.muGas >= 0))
+            throw std::runtime_error("viscous residual needs prGas > 0, Rgas > 0 and muGas >= 0");
+        const int nb = gridFor(m.F, 128);
+        const int P = uRec ? m.P : 1;
+        if (m.dim == 2)
+        {
+            if (P == 1)
+                kRHSViscous<2, 3><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+            else if (P == 2)
+                kRHSViscous<2, 6><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+            else
+                kRHSViscous<2, 10><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+        }
+        else
+        {
+            if (P == 1)
+                kRHSViscous<3, 4><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+            else if (P == 2)
+                kRHSViscous<3, 10><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+            else
+                kRHSViscous<3, 20><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+        }
+        DNDSR_HIP_CHECK(hipGetLastError());
     }
     KernelTimer kt2(c, K_RHS_GATHER);
     kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{});

@@ -536,4 +536,91 @@ __host__ __device__ inline void compressRecPart(const real *umean, const real *i
             ret[k] = umean[k];
 }
 
+// ---- laminar viscous terms
+// CpGas = Rgas gamma / (gamma - 1), src/Euler/EulerEvaluatorSettings.hpp:297
+__host__ __device__ inline real cpGas(const GasParams &gp) { return gp.Rgas * gp.gamma / (gp.gamma - 1); }
+
+// muEff, src/Euler/EulerEvaluator.hpp:751-776 (laminar part): constant mu, Sutherland's
+// law at temperature T, or constant kinematic viscosity (mu = nu rho)
+__host__ __device__ inline real muEff(const GasParams &gp, const real *U, real T)
+{
+    if (gp.muModel == 1)
+    {
+        const real TRel = T / gp.TRef;
+        return gp.muGas * TRel * sqrt(TRel) * (gp.TRef + gp.CSutherland) / (T + gp.CSutherland);
+    }
+    if (gp.muModel == 2)
+        return gp.muGas * U[0];
+    return gp.muGas;
+}
+
+// T = p / ((gamma - 1) / gamma Cp rho), EulerEvaluator_EvaluateDt.hxx:969, :1178
+__host__ __device__ inline real gasTemperature(const GasParams &gp, real p, real rho)
+{
+    return p / ((gp.gamma - 1) / gp.gamma * cpGas(gp) * rho);
+}
+
+// lamVis of EvaluateDt, EulerEvaluator_EvaluateDt.hxx:964-979: mu / rho max(4/3, gamma / Pr)
+// at the face mean state; 0 for an inviscid run
+__host__ __device__ inline real lamVisFace(const GasParams &gp, const real *uMean, real pMean)
+{
+    if (!gp.viscous)
+        return 0.0;
+    const real mu = muEff(gp, uMean, gasTemperature(gp, pMean, uMean[0]));
+    return mu / uMean[0] * fmax(4. / 3., gp.gamma / gp.prGas);
+}
+
+// GradientCons2Prim_IdealGas, Gas.hpp:1678-1698: G[d][k] = d U_k / d x_d (conservative)
+// -> Gp[d][k] = d (rho, u, v, w, p)_k / d x_d
+__host__ __device__ inline void gradCons2Prim(const real *U, const real (*G)[5], real (*Gp)[5], real gamma)
+{
+    const real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    const real r2 = 1.0 / sq(U[0]);
+    for (int d = 0; d < 3; d++)
+    {
+        Gp[d][0] = G[d][0];
+        for (int j = 0; j < 3; j++)
+            Gp[d][1 + j] = r2 * (U[0] * G[d][1 + j] - G[d][0] * U[1 + j]);
+        real a = 0, b = 0;
+        for (int j = 0; j < 3; j++)
+            a += G[d][1 + j] * velo[j], b += Gp[d][1 + j] * U[1 + j];
+        Gp[d][4] = (gamma - 1) * (G[d][4] - 0.5 * (a + b));
+    }
+}
+
+// ViscousFlux_IdealGas, Gas.hpp:1595-1655, laminar (no QCR): stress
+// tau = mu (grad u + grad u^T) + lambda mu div u I with lambda = -2/3, heat flux k grad T,
+// adiabatic walls drop the normal part of grad T.  Gp[d][1 + j] = d u_j / d x_d.
+__host__ __device__ inline void viscousFlux(const real *U, const real (*Gp)[5], const real *n, bool adiabatic,
+                                            real gamma, real mu, real k, real Cp, real *F)
+{
+    const real lambda = -2. / 3.;
+    const real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    const real vSqr = velo[0] * velo[0] + velo[1] * velo[1] + velo[2] * velo[2];
+    const real p = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
+    const real c = gamma / ((gamma - 1) * Cp * U[0] * U[0]);
+    real GradT[3];
+    for (int d = 0; d < 3; d++)
+        GradT[d] = c * (U[0] * Gp[d][4] - p * Gp[d][0]);
+    if (adiabatic)
+    {
+        const real tn = dot3(GradT, n);
+        for (int d = 0; d < 3; d++)
+            GradT[d] -= tn * n[d];
+    }
+    const real tr = Gp[0][1] + Gp[1][2] + Gp[2][3];
+    real S[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++)
+            S[i][j] = (Gp[i][1 + j] + Gp[j][1 + i]) * mu + (i == j ? lambda * mu * tr : 0.0);
+    F[0] = 0;
+    real e = 0;
+    for (int i = 0; i < 3; i++)
+    {
+        F[1 + i] = S[i][0] * n[0] + S[i][1] * n[1] + S[i][2] * n[2];
+        e += (S[i][0] * velo[0] + S[i][1] * velo[1] + S[i][2] * velo[2] + k * GradT[i]) * n[i];
+    }
+    F[4] = e;
+}
+
 } // namespace dndsr

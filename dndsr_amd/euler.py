@@ -1,10 +1,16 @@
 """EulerEvaluator on the device, with the reference's method names.
 
-Mirrors ``DNDS::Euler::EulerEvaluator<model>`` for the inviscid NS / NS_3D models
+Mirrors ``DNDS::Euler::EulerEvaluator<model>`` for the NS / NS_3D models
 (src/Euler/EulerEvaluator.hpp:76): EvaluateDt (:359), EvaluateRHS (:399),
 LUSGSMatrixInit (:427), UpdateLUSGSForward/Backward (:481-502), UpdateSGS (:513),
 EvaluateNorm (:613).  Like the reference, the evaluator keeps lambdaFace* and
 deltaLambdaCell from the last EvaluateDt as members.
+
+The laminar viscous terms are on with ``settings["viscous"] = True``; the gas then
+follows ``settings["idealGasProperty"]`` (gamma, Rgas, muGas, prGas, TRef, CSutherland,
+muModel: 0 constant, 1 Sutherland, 2 constant nu; defaults as
+src/Euler/EulerEvaluatorSettings.hpp:291-300).  Without it the residual is the
+inviscid one (the reference's RHS_Ignore_Viscosity flag, EvaluateRHS.hxx:78).
 """
 from __future__ import annotations
 
@@ -22,15 +28,23 @@ class EulerEvaluator:
     nVars = 5
 
     def __init__(self, vfv: VariationalReconstruction, settings: dict | None = None):
-        s = dict(gamma=1.4, riemannSolverType="Roe", rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0)
+        s = dict(gamma=1.4, riemannSolverType="Roe", rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0,
+                 viscous=False)
+        gas = dict(Rgas=1.0, muGas=1.0, prGas=0.72, TRef=273.15, CSutherland=110.4, muModel=1)
         if settings:
             s.update(settings)
             if "idealGasProperty" in settings:
-                s["gamma"] = settings["idealGasProperty"].get("gamma", s["gamma"])
+                igp = settings["idealGasProperty"]
+                s["gamma"] = igp.get("gamma", s["gamma"])
+                gas.update({k: igp[k] for k in gas if k in igp})
+        s["idealGasProperty"] = dict(gas, gamma=s["gamma"])
         rs = s["riemannSolverType"]
         self.settings = s
         self.es = _lib.EulerSettings(float(s["gamma"]), RIEMANN_TYPES[rs] if isinstance(rs, str) else int(rs),
-                                     int(s["rsMeanValueEig"]), float(s["rsFixScale"]), float(s["rsIncFScale"]))
+                                     int(s["rsMeanValueEig"]), float(s["rsFixScale"]), float(s["rsIncFScale"]),
+                                     int(bool(s["viscous"])), int(gas["muModel"]), float(gas["Rgas"]),
+                                     float(gas["muGas"]), float(gas["prGas"]), float(gas["TRef"]),
+                                     float(gas["CSutherland"]))
         bcTypes = s.get("bcTypes")
         if bcTypes is not None:
             # boundary weights of the VR functional follow the boundary types, as the

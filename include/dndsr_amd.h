@@ -89,6 +89,17 @@ typedef struct
     int rsMeanValueEig; /* cell-mean eigenvalues shared by a face's points (Roe_M1..M5) */
     double rsFixScale;
     double rsIncFScale;
+    /* laminar Navier-Stokes terms (idealGasProperty, EulerEvaluatorSettings.hpp:280-330;
+     * models NS / NS_3D): 0 = Euler.  muModel: 0 constant mu = muGas, 1 Sutherland's law
+     * mu = muGas (T/TRef)^1.5 (TRef + CSutherland) / (T + CSutherland), 2 constant
+     * nu (mu = muGas rho); conductivity Cp mu / prGas with Cp = Rgas gamma / (gamma - 1). */
+    int viscous;
+    int muModel;
+    double Rgas;
+    double muGas;
+    double prGas;
+    double TRef;
+    double CSutherland;
 } dndsr_euler_settings;
 
 /* ---- context ---------------------------------------------------------- */
@@ -168,8 +179,9 @@ int dndsr_set_vr_dirichlet(dndsr_ctx *ctx, int nVars, const double *value);
 int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, double CFL, double maxDt,
                   int useLocalDt, double *dt, double *lambdaFace, double *deltaLambdaCell, double *dtMinDev,
                   double *dtMinHost);
-/* EvaluateRHS, src/Euler/EulerEvaluator.hpp:399-409 (inviscid; uRec may be NULL for the
- * first-order residual; deltaLambdaCell feeds HLLE+ and may be NULL = 0). */
+/* EvaluateRHS, src/Euler/EulerEvaluator.hpp:399-409 (inviscid, plus the laminar viscous
+ * fluxes when es->viscous; uRec may be NULL for the first-order residual; deltaLambdaCell
+ * feeds HLLE+ and may be NULL = 0). */
 int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                    const double *deltaLambdaCell, double *rhs);
 /* LUSGSMatrixInit (scalar Jacobian), src/Euler/EulerEvaluator.hpp:427-434. */

@@ -302,7 +302,20 @@ struct EulerEvaluatorSettings
     // its 5 conservative values (far field / inflow); empty on periodic meshes
     std::vector<int32_t> bcType;
     std::vector<double> bcValue; // [nZone][5]
-    dndsr_euler_settings c() const { return {gamma, riemannSolverType, rsMeanValueEig, rsFixScale, rsIncFScale}; }
+    // laminar viscous terms (false: the RHS_Ignore_Viscosity residual, EvaluateRHS.hxx:78)
+    // and IdealGasProperty (src/Euler/EulerEvaluatorSettings.hpp:291-300)
+    bool viscous = false;
+    struct IdealGasProperty
+    {
+        double Rgas = 1, muGas = 1, prGas = 0.72, TRef = 273.15, CSutherland = 110.4;
+        int muModel = 1; // 0 constant, 1 Sutherland, 2 constant nu
+    } idealGasProperty;
+    dndsr_euler_settings c() const
+    {
+        const IdealGasProperty &g = idealGasProperty;
+        return {gamma,   riemannSolverType, rsMeanValueEig, rsFixScale, rsIncFScale, viscous ? 1 : 0,
+                g.muModel, g.Rgas,          g.muGas,        g.prGas,    g.TRef,      g.CSutherland};
+    }
 };
 
 template <EulerModel model>
