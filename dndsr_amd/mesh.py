@@ -393,3 +393,182 @@ def bump_channel_hex(nx: int, ny: int, nz: int, L: float = 3.0, H: float = 1.0, 
 
     return box_hex_grid(nx, ny, nz, L, H, W, periodic=(False, False, True),
                         zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)
+
+
+# ---------------------------------------------------------------------------- simplex meshes
+# Element face tables with outward normals for positively oriented cells:
+# Tri3 edges (src/Geom/Elements/Tri3.hpp:83-85), Tet4 faces (Tet4.hpp:92-95).
+TRI3_FACES = np.array([[0, 1], [1, 2], [2, 0]])
+TET4_FACES = np.array([[0, 2, 1], [0, 1, 3], [1, 2, 3], [2, 0, 3]])
+
+
+def simplex_mesh(dim: int, coords: np.ndarray, cells: np.ndarray, canon: np.ndarray | None = None,
+                 bnd_zone=None, shape: tuple = ()) -> Mesh:
+    """Tri3 (dim 2) or Tet4 (dim 3) mesh from cells [N, dim + 1] of node ids.
+
+    Cells are reoriented to a positive Jacobian.  Faces are numbered by their first
+    occurrence in (cell, element face) order; that cell is the face's back cell and the
+    face keeps its node order, so the normal points out of it.  canon[n] is the node id
+    after periodic identification: two cell faces with the same canonical node set are
+    one face, and faceShift is the translation from the back cell's copy of the face to
+    the front cell's.  A face seen once is a boundary face; bnd_zone(face2node [nb, dim])
+    returns its zone."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    cells = np.array(cells, np.int64)
+    N, nNode = len(cells), len(coords)
+    P = coords[cells]
+    if dim == 2:
+        e1, e2 = P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]
+        det = e1[:, 0] * e2[:, 1] - e1[:, 1] * e2[:, 0]
+    else:
+        det = np.einsum("ij,ij->i", np.cross(P[:, 1] - P[:, 0], P[:, 2] - P[:, 0]), P[:, 3] - P[:, 0])
+    if np.any(det == 0):
+        raise ValueError("degenerate cell")
+    neg = det < 0
+    cells[neg, 1], cells[neg, 2] = cells[neg, 2].copy(), cells[neg, 1].copy()
+    FT = TRI3_FACES if dim == 2 else TET4_FACES
+    nf = len(FT)
+    occ = cells[:, FT].reshape(N * nf, dim)              # occurrence o = c nf + k
+    canon = np.arange(nNode, dtype=np.int64) if canon is None else np.asarray(canon, np.int64)
+    cn = np.sort(canon[occ], axis=1)
+    base = int(canon.max()) + 1
+    if base ** dim >= 2 ** 62:
+        raise ValueError("too many nodes for the face key")
+    key = cn[:, 0].copy()
+    for a in range(1, dim):
+        key = key * base + cn[:, a]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    start = np.r_[True, ks[1:] != ks[:-1]]
+    gid = np.cumsum(start) - 1
+    counts = np.bincount(gid)
+    if counts.max() > 2:
+        raise ValueError("a face is shared by more than two cells")
+    pos = np.flatnonzero(start)
+    first = order[pos]
+    second = np.where(counts == 2, order[np.minimum(pos + 1, len(order) - 1)], -1)
+    fo = np.argsort(first, kind="stable")
+    first, second = first[fo], second[fo]
+    F = len(first)
+    face_of_group = np.empty(F, np.int64)
+    face_of_group[fo] = np.arange(F)
+    occ_face = np.empty(N * nf, np.int64)
+    occ_face[order] = face_of_group[gid]
+    cell2face = occ_face.reshape(N, nf)
+    face2node = occ[first]
+    has2 = second >= 0
+    face2cell = np.stack([first // nf, np.where(has2, second // nf, -1)], 1)
+    faceShift = np.zeros((F, 3))
+    if has2.any():
+        # translation between the two copies: the node of smallest canonical id in each
+        a, b = occ[first[has2]], occ[second[has2]]
+        ia = np.argmin(canon[a], axis=1)
+        ib = np.argmin(canon[b], axis=1)
+        r = np.arange(len(a))
+        faceShift[has2] = coords[b[r, ib]] - coords[a[r, ia]]
+    zone = None
+    nzone = 0
+    if (~has2).any():
+        if bnd_zone is None:
+            raise ValueError("the mesh has boundary faces: bnd_zone is required")
+        zone = np.full(F, -1, np.int32)
+        zone[~has2] = np.asarray(bnd_zone(face2node[~has2]), np.int32)
+        nzone = int(zone.max()) + 1
+    ct, ft = (TRI3, LINE2) if dim == 2 else (TET4, TRI3)
+    return Mesh(dim, coords, np.full(N, ct, np.int32), np.arange(N + 1, dtype=np.int64) * (dim + 1),
+                cells.ravel(), np.arange(N + 1, dtype=np.int64) * nf, cell2face.ravel(),
+                np.full(F, ft, np.int32), np.arange(F + 1, dtype=np.int64) * dim, face2node.ravel().astype(np.int64),
+                face2cell.astype(np.int64), faceShift, shape, zone, nzone)
+
+
+def _lattice_zone(lat, ns, per, zones):
+    """Zone of a boundary face from the lattice indices lat[node] = (i, j, k) of its nodes."""
+    keys = (("xmin", "xmax"), ("ymin", "ymax"), ("zmin", "zmax"))
+
+    def f(fn):
+        L = lat[fn]                                   # [nb, dim, 3]
+        z = np.full(len(fn), -1, np.int32)
+        for a in range(len(ns)):
+            if per[a]:
+                continue
+            for end, key in ((0, keys[a][0]), (ns[a], keys[a][1])):
+                on = np.all(L[:, :, a] == end, axis=1)
+                z[on] = _zone_of(zones, key)
+        if np.any(z < 0):
+            raise ValueError("boundary face off the box sides")
+        return z
+    return f
+
+
+def tri_grid(nx: int, ny: int, Lx: float = 1.0, Ly: float = 1.0, x0: float = 0.0, y0: float = 0.0,
+             periodic=(False, False), zones: dict | None = None, jitter: float = 0.0, seed: int = 0,
+             node_map=None) -> Mesh:
+    """nx x ny lattice on [x0, x0+Lx] x [y0, y0+Ly], each square split into two Tri3 cells
+    along its (i, j)-(i+1, j+1) diagonal; directions periodic or bounded as box_quad_grid."""
+    assert nx >= 2 and ny >= 2
+    per = [bool(p) for p in periodic]
+    hx, hy = Lx / nx, Ly / ny
+    I, J = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, np.zeros_like(I, dtype=float)], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([hx, hy]), jitter, seed, per)
+    coords = X.reshape(-1, 3)
+    if node_map is not None:
+        coords = np.ascontiguousarray(node_map(coords.copy()), np.float64)
+    nid = lambda i, j: i * (ny + 1) + j  # noqa: E731
+    lat = np.stack([I.ravel(), J.ravel(), 0 * I.ravel()], 1)
+    canon = nid(I % nx if per[0] else I, J % ny if per[1] else J).ravel()
+    c = np.arange(nx * ny)
+    ci, cj = c % nx, c // nx
+    a, b, cc, d = nid(ci, cj), nid(ci + 1, cj), nid(ci + 1, cj + 1), nid(ci, cj + 1)
+    cells = np.stack([np.stack([a, b, cc], 1), np.stack([a, cc, d], 1)], 1).reshape(-1, 3)
+    return simplex_mesh(2, coords, cells, canon, _lattice_zone(lat, (nx, ny), per, zones), (nx, ny))
+
+
+def tet_grid(nx: int, ny: int, nz: int, Lx: float = 1.0, Ly: float = 1.0, Lz: float = 1.0,
+             x0: float = 0.0, y0: float = 0.0, z0: float = 0.0, periodic=(False, False, False),
+             zones: dict | None = None, jitter: float = 0.0, seed: int = 0, node_map=None) -> Mesh:
+    """nx x ny x nz lattice, each cube split into six Tet4 cells around its (0,0,0)-(1,1,1)
+    diagonal (the Kuhn subdivision: one tetrahedron per ordering of the axes, so that
+    neighbouring cubes share face triangulations); directions periodic or bounded as
+    box_hex_grid.  With jittered nodes this is the jittered-lattice Delaunay-type
+    tetrahedralisation of the box."""
+    assert nx >= 2 and ny >= 2 and nz >= 2
+    per = [bool(p) for p in periodic]
+    hx, hy, hz = Lx / nx, Ly / ny, Lz / nz
+    I, J, K = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), np.arange(nz + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, z0 + K * hz], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([hx, hy, hz]), jitter, seed, per)
+    coords = X.reshape(-1, 3)
+    if node_map is not None:
+        coords = np.ascontiguousarray(node_map(coords.copy()), np.float64)
+    nid = lambda i, j, k: (i * (ny + 1) + j) * (nz + 1) + k  # noqa: E731
+    lat = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    canon = nid(I % nx if per[0] else I, J % ny if per[1] else J, K % nz if per[2] else K).ravel()
+    c = np.arange(nx * ny * nz)
+    ci, cj, ck = c % nx, (c // nx) % ny, c // (nx * ny)
+    tets = []
+    for p in ((0, 1, 2), (0, 2, 1), (1, 0, 2), (1, 2, 0), (2, 0, 1), (2, 1, 0)):
+        o = np.zeros((3, len(c)), np.int64)
+        path = [nid(ci, cj, ck)]
+        for ax in p:
+            o[ax] += 1
+            path.append(nid(ci + o[0], cj + o[1], ck + o[2]))
+        tets.append(np.stack(path, 1))
+    cells = np.stack(tets, 1).reshape(-1, 4)
+    return simplex_mesh(3, coords, cells, canon, _lattice_zone(lat, (nx, ny, nz), per, zones), (nx, ny, nz))
+
+
+def bump_channel_tet(nx: int, ny: int, nz: int, L: float = 3.0, H: float = 1.0, W: float = 0.5,
+                     amp: float = 0.1, jitter: float = 0.0, seed: int = 0) -> Mesh:
+    """The bump channel of bump_channel_hex (same wall-following grid lines, zones 0 inlet,
+    1 outlet, 2 lower wall, 3 upper boundary, spanwise periodic) on tetrahedra: each
+    hexahedron of the nx x ny x nz grid split into six Tet4 cells (6 nx ny nz cells)."""
+    def bump(P):
+        x, y = P[:, 0], P[:, 1]
+        s = np.where((x > L / 3) & (x < 2 * L / 3), np.sin(3 * np.pi * x / L - np.pi) ** 2, 0.0)
+        Q = P.copy()
+        Q[:, 1] = y + amp * H * s * (1 - y / H)
+        return Q
+
+    return tet_grid(nx, ny, nz, L, H, W, periodic=(False, False, True),
+                    zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)

@@ -92,18 +92,130 @@ struct QP
     real p[3];
     real w;
 };
+
+// Hammer rules on the reference triangle (0,0)-(1,0)-(0,1) and tetrahedron
+// (0,0,0)-(1,0,0)-(0,1,0)-(0,0,1), digits as the reference stores them
+// (src/Geom/Quadratures/Tri.hpp:25-66, Tet.hpp:27-92); rows xi, eta, (zeta,) w.
+static const real kTri1[3][1] = {{1. / 3.}, {1. / 3.}, {1. / 2.}};
+static const real kTri3[3][3] = {{2. / 3., 1. / 6., 1. / 6.}, {1. / 6., 2. / 3., 1. / 6.}, {1. / 6., 1. / 6., 1. / 6.}};
+static const real kTri6[3][6] = {
+    {0.09157621350977102, 0.8168475729804585, 0.09157621350977052, 0.445948490915965, 0.10810301816807, 0.445948490915965},
+    {0.09157621350977102, 0.09157621350977052, 0.8168475729804585, 0.445948490915965, 0.445948490915965, 0.10810301816807},
+    {0.054975871827661, 0.054975871827661, 0.054975871827661, 0.1116907948390057, 0.1116907948390057, 0.1116907948390057}};
+static const real kTri7[3][7] = {
+    {0.3333333333333335, 0.470142064105115, 0.05971587178977, 0.470142064105115, 0.1012865073234565, 0.7974269853530875,
+     0.1012865073234565},
+    {0.3333333333333335, 0.470142064105115, 0.470142064105115, 0.05971587178977, 0.1012865073234565, 0.1012865073234565,
+     0.7974269853530875},
+    {0.1125, 0.066197076394253, 0.066197076394253, 0.066197076394253, 0.0629695902724135, 0.0629695902724135,
+     0.0629695902724135}};
+static const real kTri12[3][12] = {
+    {0.2492867451709105, 0.501426509658179, 0.2492867451709105, 0.063089014491502, 0.8738219710169954, 0.063089014491502,
+     0.3103524510337845, 0.05314504984481699, 0.6365024991213986, 0.05314504984481699, 0.6365024991213986,
+     0.3103524510337845},
+    {0.2492867451709105, 0.2492867451709105, 0.501426509658179, 0.063089014491502, 0.063089014491502, 0.8738219710169954,
+     0.05314504984481699, 0.3103524510337845, 0.05314504984481699, 0.6365024991213986, 0.3103524510337845,
+     0.6365024991213986},
+    {0.05839313786318975, 0.05839313786318975, 0.05839313786318975, 0.0254224531851035, 0.0254224531851035,
+     0.0254224531851035, 0.04142553780918675, 0.04142553780918675, 0.04142553780918675, 0.04142553780918675,
+     0.04142553780918675, 0.04142553780918675}};
+static const real kTet1[4][1] = {{0.25}, {0.25}, {0.25}, {0.1666666666666667}};
+static const real kTet4[4][4] = {{0.1381966011250105, 0.1381966011250105, 0.5854101966249685, 0.1381966011250105},
+                                 {0.1381966011250105, 0.5854101966249684, 0.1381966011250105, 0.1381966011250105},
+                                 {0.5854101966249684, 0.1381966011250105, 0.1381966011250105, 0.1381966011250105},
+                                 {0.04166666666666666, 0.04166666666666666, 0.04166666666666666, 0.04166666666666666}};
+static const real kTet8[4][8] = {
+    {0.1069940147705369, 0.1069940147705369, 0.6790179556883893, 0.1069940147705369, 0.3280585716625127,
+     0.3280585716625127, 0.01582428501246191, 0.3280585716625127},
+    {0.1069940147705369, 0.6790179556883893, 0.1069940147705369, 0.1069940147705369, 0.3280585716625127,
+     0.01582428501246191, 0.3280585716625127, 0.3280585716625127},
+    {0.6790179556883893, 0.1069940147705369, 0.1069940147705369, 0.1069940147705369, 0.01582428501246191,
+     0.3280585716625127, 0.3280585716625127, 0.3280585716625127},
+    {0.01859314997209119, 0.01859314997209119, 0.01859314997209119, 0.01859314997209119, 0.02307351669457548,
+     0.02307351669457548, 0.02307351669457548, 0.02307351669457548}};
+static const real kTet14[4][14] = {
+    {0.09273525031089125, 0.09273525031089125, 0.7217942490673263, 0.09273525031089125, 0.3108859192633008,
+     0.3108859192633008, 0.06734224221009777, 0.3108859192633008, 0.04550370412564952, 0.4544962958743505,
+     0.04550370412564952, 0.04550370412564952, 0.4544962958743505, 0.4544962958743505},
+    {0.09273525031089125, 0.7217942490673263, 0.09273525031089125, 0.09273525031089125, 0.3108859192633008,
+     0.06734224221009777, 0.3108859192633008, 0.3108859192633008, 0.4544962958743505, 0.04550370412564952,
+     0.04550370412564952, 0.4544962958743505, 0.04550370412564952, 0.4544962958743505},
+    {0.7217942490673263, 0.09273525031089125, 0.09273525031089125, 0.09273525031089125, 0.06734224221009777,
+     0.3108859192633008, 0.3108859192633008, 0.3108859192633008, 0.4544962958743505, 0.4544962958743505,
+     0.4544962958743505, 0.04550370412564952, 0.04550370412564952, 0.04550370412564952},
+    {0.01224884051939367, 0.01224884051939367, 0.01224884051939367, 0.01224884051939367, 0.01878132095300264,
+     0.01878132095300264, 0.01878132095300264, 0.01878132095300264, 0.0070910034628469, 0.0070910034628469,
+     0.0070910034628469, 0.0070910034628469, 0.0070910034628469, 0.0070910034628469}};
+static const real kTet24[4][24] = {
+    {0.2146028712591523, 0.2146028712591523, 0.3561913862225431, 0.2146028712591523, 0.3223378901422754,
+     0.3223378901422754, 0.0329863295731736, 0.3223378901422754, 0.04067395853461131, 0.04067395853461131,
+     0.8779781243961661, 0.04067395853461131, 0.6030056647916491, 0.6030056647916491, 0.06366100187501755,
+     0.2696723314583158, 0.06366100187501755, 0.06366100187501755, 0.2696723314583158, 0.06366100187501755,
+     0.06366100187501755, 0.06366100187501755, 0.2696723314583158, 0.6030056647916491},
+    {0.2146028712591523, 0.3561913862225431, 0.2146028712591523, 0.2146028712591523, 0.3223378901422754,
+     0.0329863295731736, 0.3223378901422754, 0.3223378901422754, 0.04067395853461131, 0.8779781243961661,
+     0.04067395853461131, 0.04067395853461131, 0.06366100187501755, 0.06366100187501755, 0.06366100187501755,
+     0.6030056647916491, 0.2696723314583158, 0.6030056647916491, 0.06366100187501755, 0.2696723314583158,
+     0.06366100187501755, 0.6030056647916491, 0.06366100187501755, 0.2696723314583158},
+    {0.3561913862225431, 0.2146028712591523, 0.2146028712591523, 0.2146028712591523, 0.0329863295731736,
+     0.3223378901422754, 0.3223378901422754, 0.3223378901422754, 0.8779781243961661, 0.04067395853461131,
+     0.04067395853461131, 0.04067395853461131, 0.2696723314583158, 0.06366100187501755, 0.6030056647916491,
+     0.06366100187501755, 0.6030056647916491, 0.06366100187501755, 0.6030056647916491, 0.06366100187501755,
+     0.2696723314583158, 0.2696723314583158, 0.06366100187501755, 0.06366100187501755},
+    {0.006653791709694545, 0.006653791709694545, 0.006653791709694545, 0.006653791709694545, 0.009226196923942479,
+     0.009226196923942479, 0.009226196923942479, 0.009226196923942479, 0.001679535175886773, 0.001679535175886773,
+     0.001679535175886773, 0.001679535175886773, 0.008035714285714285, 0.008035714285714285, 0.008035714285714285,
+     0.008035714285714285, 0.008035714285714285, 0.008035714285714285, 0.008035714285714285, 0.008035714285714285,
+     0.008035714285714285, 0.008035714285714285, 0.008035714285714285, 0.008035714285714285}};
+
+// Hammer rule for an integration order (GetQuadratureScheme, src/Geom/QuadratureHub.hpp:57-108)
+template <size_t R, size_t n>
+static std::vector<QP> hammer(const real (&t)[R][n])
+{
+    std::vector<QP> r(n);
+    for (size_t i = 0; i < n; i++)
+    {
+        r[i] = QP{{t[0][i], t[1][i], R == 4 ? t[2][i] : 0.0}, t[R - 1][i]};
+    }
+    return r;
+}
+static std::vector<QP> simplexRule(int elemType, int order)
+{
+    if (elemType == ORC_TRI3)
+    {
+        if (order <= 1) return hammer(kTri1);
+        if (order == 2) return hammer(kTri3);
+        if (order <= 4) return hammer(kTri6);
+        if (order == 5) return hammer(kTri7);
+        if (order == 6) return hammer(kTri12);
+    }
+    else
+    {
+        if (order <= 1) return hammer(kTet1);
+        if (order == 2) return hammer(kTet4);
+        if (order == 3) return hammer(kTet8);
+        if (order <= 5) return hammer(kTet14);
+        if (order == 6) return hammer(kTet24);
+    }
+    throw std::runtime_error("unsupported integration order");
+}
+
 static int elemParamDim(int t)
 {
     switch (t)
     {
     case ORC_LINE2: return 1;
     case ORC_QUAD4: return 2;
+    case ORC_TRI3: return 2;
+    case ORC_TET4: return 3;
     case ORC_HEX8: return 3;
     default: throw std::runtime_error("element type not supported by the oracle");
     }
 }
 static std::vector<QP> quadRule(int elemType, int order)
 {
+    if (elemType == ORC_TRI3 || elemType == ORC_TET4)
+        return simplexRule(elemType, order);
     int pd = elemParamDim(elemType);
     int n = lineNPoint(order);
     const real *p, *w;
@@ -127,7 +239,8 @@ static std::vector<QP> quadRule(int elemType, int order)
 static const int INT_ORDER_MAX = 6; // src/Geom/Quadratures/QuadratureBase.hpp:70
 
 // ---------------------------------------------------------------------------
-// Linear shape functions (Line2, Quad4, Hex8; CGNS node order).
+// Linear shape functions (Line2, Quad4, Hex8, Tri3, Tet4; CGNS node order;
+// simplices: src/Geom/Elements/Tri3.hpp:20-50, Tet4.hpp:20-56).
 // ---------------------------------------------------------------------------
 static int elemNNode(int t)
 {
@@ -136,6 +249,8 @@ static int elemNNode(int t)
     case ORC_LINE2: return 2;
     case ORC_QUAD4: return 4;
     case ORC_HEX8: return 8;
+    case ORC_TRI3: return 3;
+    case ORC_TET4: return 4;
     default: throw std::runtime_error("bad element");
     }
 }
@@ -167,6 +282,16 @@ static void shape(int t, const real *xi, real *N, real (*dN)[3])
             dN[a][1] = 0.125 * h8[a][1] * fx * fz;
             dN[a][2] = 0.125 * h8[a][2] * fx * fy;
         }
+        break;
+    case ORC_TRI3:
+        N[0] = -xi[1] - xi[0] + 1, N[1] = xi[0], N[2] = xi[1];
+        dN[0][0] = -1, dN[0][1] = -1, dN[1][0] = 1, dN[1][1] = 0, dN[2][0] = 0, dN[2][1] = 1;
+        break;
+    case ORC_TET4:
+        N[0] = -xi[1] - xi[0] - xi[2] + 1, N[1] = xi[0], N[2] = xi[1], N[3] = xi[2];
+        for (int a = 0; a < 4; a++)
+            for (int c = 0; c < 3; c++)
+                dN[a][c] = a == 0 ? -1.0 : (a == c + 1 ? 1.0 : 0.0);
         break;
     default: throw std::runtime_error("bad element");
     }
@@ -863,6 +988,13 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
     int dim = v.dim;
     v.cellType.assign(m->cellType, m->cellType + m->nCell);
     v.faceType.assign(m->faceType, m->faceType + m->nFace);
+    // one rule per entity kind: a single cell type and a single face type per mesh
+    for (auto t : v.cellType)
+        if (t != v.cellType[0])
+            throw std::runtime_error("mixed cell types are not supported");
+    for (auto t : v.faceType)
+        if (t != v.faceType[0])
+            throw std::runtime_error("mixed face types are not supported");
     v.cellNodes.resize(v.nCell);
     v.c2f.resize(v.nCell);
     v.maxFC = 0;

@@ -1,0 +1,210 @@
+"""CPU pins of the oracle on Tri3 / Tet4 meshes (Hammer rules, src/Geom/Quadratures/Tri.hpp,
+Tet.hpp; rule choice QuadratureHub.hpp:57-108; shape functions Elements/Tri3.hpp, Tet4.hpp).
+
+Pinned by what the mathematics fixes:
+* every rule integrates the monomials up to its degree exactly on the unit right
+  simplex: int x^a y^b = a! b! / (a + b + 2)!, int x^a y^b z^c = a! b! c! / (a + b + c + 3)!
+  (cell rules and the face rule on a tetrahedron's face);
+* cell metrics close: volumes sum to the box volume; per cell the divergence theorem
+  holds with the face rules (sum_f int n dS = 0, sum_f int x.n dS = dim V, and
+  sum_f int x_d^2 / 2 n_d dS = V bary_d), which a wrong normal, area, point or weight breaks;
+* the VR sweep with Dirichlet boundary values and value-only boundary weights
+  reproduces every polynomial of degree <= p on jittered triangle and tetrahedron
+  meshes; EvaluateRHS preserves a uniform flow and conserves on periodic simplex
+  meshes; the viscous residual of a shear layer converges at second order on triangles.
+"""
+from __future__ import annotations
+
+from math import factorial
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from dndsr_amd import bc
+from dndsr_amd import mesh as mm
+from helpers import RS
+
+GAMMA = 1.4
+
+
+def _unit_simplex(dim):
+    if dim == 2:
+        coords = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0]], float)
+    else:
+        coords = np.array([[0, 0, 0], [1, 0, 0], [0, 1, 0], [0, 0, 1]], float)
+    return mm.simplex_mesh(dim, coords, np.arange(dim + 1)[None], bnd_zone=lambda fn: np.zeros(len(fn)))
+
+
+# degree of exactness of the rule chosen for each integration order
+TRI_DEG = {1: 1, 2: 2, 3: 4, 4: 4, 5: 5, 6: 6}
+TET_DEG = {1: 1, 2: 2, 3: 3, 4: 5, 5: 5, 6: 6}
+
+
+@pytest.mark.parametrize("dim", [2, 3])
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
+def test_cell_rules_exact_on_unit_simplex(dim, order):
+    o = po.OracleVR(_unit_simplex(dim), maxOrder=1, intOrder=order)
+    pts, wj = o.cell_quad()
+    x, w = pts[0], wj[0]
+    deg = (TRI_DEG if dim == 2 else TET_DEG)[order]
+    for a in range(deg + 1):
+        for b in range(deg + 1 - a):
+            for c in range(deg + 1 - a - b if dim == 3 else 1):
+                f = x[:, 0] ** a * x[:, 1] ** b * x[:, 2] ** c
+                exact = factorial(a) * factorial(b) * factorial(c) / factorial(a + b + c + dim)
+                assert abs((f * w).sum() - exact) <= 1e-14, (a, b, c)
+
+
+@pytest.mark.parametrize("order", [1, 2, 3, 4, 5, 6])
+def test_face_rules_exact_on_tet_face(order):
+    o = po.OracleVR(_unit_simplex(3), maxOrder=1, intOrder=order)
+    fm = o.face_metrics()
+    z0 = np.flatnonzero(np.all(np.abs(fm["pts"][:, :, 2]) < 1e-15, axis=1))
+    assert len(z0) == 1                                   # the face z = 0
+    f = z0[0]
+    x, w = fm["pts"][f], fm["wJDet"][f]
+    np.testing.assert_allclose(fm["norms"][f], np.tile([0, 0, -1.0], (len(w), 1)), atol=1e-15)
+    for a in range(TRI_DEG[order] + 1):
+        for b in range(TRI_DEG[order] + 1 - a):
+            exact = factorial(a) * factorial(b) / factorial(a + b + 2)
+            assert abs((x[:, 0] ** a * x[:, 1] ** b * w).sum() - exact) <= 1e-14, (a, b)
+
+
+def _meshes():
+    return {
+        "tri": mm.tri_grid(5, 4, 2.0, 1.5, zones=dict(xmin=0, xmax=0, ymin=0, ymax=0), jitter=0.15, seed=3),
+        "tet": mm.tet_grid(3, 3, 2, 1.0, 1.2, 0.8, zones=dict(xmin=0, xmax=0, ymin=0, ymax=0, zmin=0, zmax=0),
+                           jitter=0.12, seed=4),
+    }
+
+
+@pytest.mark.parametrize("kind", ["tri", "tet"])
+def test_metrics_close_by_divergence_theorem(kind):
+    m = _meshes()[kind]
+    o = po.OracleVR(m, maxOrder=1, intOrder=3)
+    cm, fm = o.cell_metrics(), o.face_metrics()
+    box = 2.0 * 1.5 if kind == "tri" else 1.0 * 1.2 * 0.8
+    assert abs(cm["vol"].sum() - box) <= 1e-13
+    dim = m.dim
+    N = m.nCell
+    sn = np.zeros((N, 3))
+    sx = np.zeros(N)
+    sb = np.zeros((N, 3))
+    for f in range(m.nFace):
+        w, x, n = fm["wJDet"][f], fm["pts"][f], fm["norms"][f]
+        for side, c in enumerate(m.face2cell[f]):
+            if c < 0:
+                continue
+            s = 1.0 if side == 0 else -1.0
+            xs = x + (m.faceShift[f] if side == 1 else 0.0)
+            sn[c] += s * (w[:, None] * n).sum(0)
+            sx[c] += s * (w * (xs * n).sum(1)).sum()
+            sb[c] += s * (w[:, None] * 0.5 * xs**2 * n).sum(0)
+    assert np.abs(sn).max() <= 1e-14
+    np.testing.assert_allclose(sx, dim * cm["vol"], rtol=1e-13)
+    np.testing.assert_allclose(sb[:, :dim], cm["vol"][:, None] * cm["bary"][:, :dim], rtol=1e-12, atol=1e-15)
+
+
+FIELDS = {
+    1: (lambda P: 0.5 + P[..., 0] + 2 * P[..., 1] - 0.5 * P[..., 2], [0.5, 1.0, 2.0, -0.5], 1),
+    4: (lambda P: P[..., 0] ** 2 + P[..., 1] ** 2 + P[..., 2] ** 2, [1.0], 2),
+    5: (lambda P: P[..., 0] ** 3 + P[..., 0] * P[..., 1] ** 2 + P[..., 2] ** 3, [1.0], 3),
+}
+
+
+@pytest.mark.parametrize("kind,order", [("tri", 1), ("tri", 2), ("tri", 3), ("tet", 1), ("tet", 2)])
+def test_vr_dirichlet_polynomial_exactness(kind, order):
+    m = _meshes()[kind]
+    o = po.OracleVR(m, maxOrder=order, intOrder=max(order + 2, 3),
+                    zoneDirWeight=bc.euler_dir_weights([bc.BC_FAR], order))
+    pts = o.face_metrics()["pts"]
+    for fid, (f, prm, deg) in FIELDS.items():
+        if deg > order:
+            continue
+        o.set_vr_dirichlet(f(pts)[..., None])
+        u = o.cell_average(fid, prm, 1)
+        rec = np.zeros((o.nCell, 1, o.M))
+        for _ in range(4000):
+            new = o.vr_iter(rec, u)
+            d = np.abs(new - rec).max()
+            rec = new
+            if d < 1e-15:
+                break
+        assert o.rec_error_l1(fid, prm, u, rec) < 1e-9, fid
+
+
+def _cons(rho, v, p):
+    v = np.asarray(v, float)
+    return np.array([rho, *(rho * v), p / (GAMMA - 1) + 0.5 * rho * v @ v])
+
+
+@pytest.mark.parametrize("kind", ["tri", "tet"])
+@pytest.mark.parametrize("rs", ["Roe", "HLLEP"])
+def test_free_stream_and_conservation(kind, rs):
+    m = _meshes()[kind]
+    U = _cons(1.0, [0.4, -0.2, 0.1 if kind == "tet" else 0.0], 1 / GAMMA)
+    o = po.OracleVR(m, maxOrder=2, intOrder=4, zoneDirWeight=bc.euler_dir_weights([bc.BC_FAR], 2))
+    o.set_bc([bc.BC_FAR], U[None], GAMMA)
+    es = o.euler_settings(GAMMA, RS[rs])
+    u = np.tile(U, (o.nCell, 1))
+    rec = o.vr_iter(np.zeros((o.nCell, 5, o.M)), u)
+    assert np.abs(rec).max() < 1e-13
+    assert np.abs(o.eval_rhs(es, u, rec, o.eval_dt(es, u, 1.0)[2])).max() < 1e-12
+    # conservation on a periodic mesh
+    if kind == "tri":
+        mp = mm.tri_grid(5, 4, 2.0, 1.5, periodic=(True, True), jitter=0.15, seed=3)
+    else:
+        mp = mm.tet_grid(3, 3, 2, periodic=(True, True, True), jitter=0.12, seed=4)
+    op = po.OracleVR(mp, maxOrder=2, intOrder=4)
+    rng = np.random.default_rng(2)
+    n = op.nCell
+    up = np.stack([_cons(rng.uniform(0.9, 1.1), rng.uniform(-0.2, 0.2, 3) * [1, 1, kind == "tet"],
+                         rng.uniform(0.7, 0.9)) for _ in range(n)])
+    recp = rng.uniform(-0.01, 0.01, (n, 5, op.M))
+    for visc in (0, 1):
+        es = op.euler_settings(GAMMA, RS[rs], viscous=visc, muModel=0, muGas=0.05)
+        r = op.eval_rhs(es, up, recp, op.eval_dt(es, up, 1.0)[2])
+        vol = op.cell_metrics()["vol"]
+        assert np.abs((r * vol[:, None]).sum(0)).max() <= 1e-13 * np.abs(r * vol[:, None]).sum()
+
+
+def test_viscous_shear_converges_on_triangles():
+    """p1 on triangles: the residual of one cell carries an O(h) truncation error that
+    alternates between the two triangles of a square (the two orientations); the sum
+    over a square -- the flux balance around it, the diagonal face cancelling --
+    converges at second order to the exact term."""
+    U0, mu, K = 0.3, 0.01, 2 * np.pi
+
+    def state(P):
+        y = P[..., 1]
+        one = np.ones_like(y)
+        u = U0 * np.sin(K * y)
+        return np.stack([one, u, 0 * y, 0 * y, one / (GAMMA - 1) + 0.5 * u * u], -1)
+
+    def exact(P):
+        y = P[..., 1]
+        z = 0 * y
+        return np.stack([z, -mu * U0 * K * K * np.sin(K * y), z, z, mu * U0**2 * K * K * np.cos(2 * K * y)], -1)
+
+    errs = []
+    for n in (8, 16, 32):
+        o = po.OracleVR(mm.tri_grid(n, n, periodic=(True, True)), maxOrder=1, intOrder=4)
+        pts, wj = o.cell_quad()
+        vol = wj.sum(1)
+        u = np.einsum("ngk,ng->nk", state(pts), wj) / vol[:, None]
+        rec = np.zeros((o.nCell, 5, o.M))
+        for _ in range(20000):
+            new = o.vr_iter(rec, u)
+            d = np.abs(new - rec).max()
+            rec = new
+            if d < 1e-15:
+                break
+        esv = o.euler_settings(GAMMA, 1, viscous=1, muModel=0, muGas=mu)
+        esi = o.euler_settings(GAMMA, 1)
+        r = (o.eval_rhs(esv, u, rec, None) - o.eval_rhs(esi, u, rec, None)) * vol[:, None]
+        ex = np.einsum("ngk,ng->nk", exact(pts), wj)
+        sq_r, sq_ex = r.reshape(-1, 2, 5).sum(1), ex.reshape(-1, 2, 5).sum(1)   # the two triangles of a square
+        errs.append(np.abs(sq_r[:, 1] - sq_ex[:, 1]).max() / np.abs(sq_ex[:, 1]).max())
+    assert errs[-1] < 1e-2
+    assert all(np.log2(a / b) > 1.5 for a, b in zip(errs[:-1], errs[1:])), errs
