@@ -11,7 +11,7 @@
 
 #include "../../include/dndsr_amd.h"
 #include "gas.hpp"
-#include "internal.hpp"
+#include "geom.hpp"
 
 using namespace dndsr;
 
@@ -224,10 +224,13 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     for (size_t f = 0; f < F; f++)
         need(md->faceType[f] == m.faceType, "mixed face types are not supported");
     if (m.dim == 2)
-        need(m.cellType == DNDSR_QUAD4 && m.faceType == DNDSR_LINE2, "2-D meshes must be Quad4 cells / Line2 faces");
+        need((m.cellType == DNDSR_QUAD4 || m.cellType == DNDSR_TRI3) && m.faceType == DNDSR_LINE2,
+             "2-D meshes must be Quad4 or Tri3 cells / Line2 faces");
     else
-        need(m.cellType == DNDSR_HEX8 && m.faceType == DNDSR_QUAD4, "3-D meshes must be Hex8 cells / Quad4 faces");
-    const int nnc = m.dim == 2 ? 4 : 8, nnf = m.dim == 2 ? 2 : 4;
+        need((m.cellType == DNDSR_HEX8 && m.faceType == DNDSR_QUAD4) ||
+                 (m.cellType == DNDSR_TET4 && m.faceType == DNDSR_TRI3),
+             "3-D meshes must be Hex8 cells / Quad4 faces or Tet4 cells / Tri3 faces");
+    const int nnc = nodeCount(m.cellType), nnf = nodeCount(m.faceType);
     int K = 0;
     for (size_t i = 0; i < N; i++)
     {
@@ -236,9 +239,9 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     }
     need(K <= kMaxFC, "too many faces per cell");
     m.K = K;
-    int n1d = s->intOrder <= 1 ? 1 : s->intOrder <= 3 ? 2 : s->intOrder <= 5 ? 3 : 4;
-    m.G = m.dim == 2 ? n1d : n1d * n1d;
-    m.Gc = m.dim == 2 ? n1d * n1d : n1d * n1d * n1d;
+    need(s->intOrder >= 0 && s->intOrder <= 6, "intOrder must be 0..6");
+    m.G = ruleSize(m.faceType, s->intOrder);
+    m.Gc = ruleSize(m.cellType, s->intOrder);
     need(m.G <= kMaxG, "too many face quadrature points");
 
     std::vector<real> hc(3 * (size_t)m.nNode);

@@ -1149,7 +1149,8 @@ static void rhsDispatchRS(Ctx &c, const GasParams &gp, const real *u, const real
 }
 
 // face quadrature sizes: 2-D Line rules of 2 / 3 points (intOrder 2..5), 3-D Quad rules
-// of 4 / 9 points (intOrder 2..5); the tile kernel is specialised on the point count.
+// of 4 / 9 points (intOrder 2..5), Tri rules of 6 points (intOrder 3..4); the tile
+// kernel is specialised on the point count.
 template <int DIM, int NDOF>
 static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *ff)
 {
@@ -1158,8 +1159,11 @@ static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *
         rhsDispatchRS<DIM, NDOF, G1>(c, gp, u, uRec, dlc, ff);
     else if (c.m.G == G2)
         rhsDispatchRS<DIM, NDOF, G2>(c, gp, u, uRec, dlc, ff);
+    else if (DIM == 3 && c.m.G == 6)
+        rhsDispatchRS<DIM, NDOF, 6>(c, gp, u, uRec, dlc, ff);
     else
-        throw std::runtime_error("EvaluateRHS supports intOrder 2..5 (2/3 face points in 2-D, 4/9 in 3-D)");
+        throw std::runtime_error("EvaluateRHS supports 2/3 face points in 2-D (intOrder 2..5), 4/9 on quadrilateral "
+                                 "faces (intOrder 2..5) and 6 on triangular faces (intOrder 3..4)");
 }
 
 void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs,

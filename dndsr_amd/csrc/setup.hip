@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(128) kCellMetrics(DevMesh m)
     // parametric centre (order-1 rule) (FiniteVolume.cpp:218-241)
     real cc[3];
     {
-        real xi[3] = {0, 0, 0}, J[3][3];
+        real xi[3], J[3][3];
+        centreParam(t, xi);
         mapLinear(t, X, xi, cc, J);
         for (int r = 0; r < 3; r++)
             m.cent[(size_t)r * m.N + i] = cc[r];
@@ -116,11 +117,10 @@ __global__ void __launch_bounds__(128) kFaceMetrics(DevMesh m)
             X[a][r] = m.coords[(size_t)r * m.nNode + nd];
     }
     real area = 0, mn[3] = {0, 0, 0};
-    const int n1d = glN(m.s.intOrder);
     for (int g = 0; g < m.G; g++)
     {
         real x[3], nu[3], wj, jd, w;
-        facePoint(DIM, n1d, g, X, x, nu, wj, jd, w);
+        facePoint(t, m.s.intOrder, g, X, x, nu, wj, jd, w);
         area += (1 * jd) * w;
         m.fwj[(size_t)g * m.F + f] = wj;
         for (int r = 0; r < 3; r++)
@@ -132,7 +132,8 @@ __global__ void __launch_bounds__(128) kFaceMetrics(DevMesh m)
     }
     m.farea[f] = area + kVerySmall;
     real l = sqrt(mn[0] * mn[0] + mn[1] * mn[1] + mn[2] * mn[2]);
-    real xi[3] = {0, 0, 0}, x[3], J[3][3];
+    real xi[3], x[3], J[3][3];
+    centreParam(t, xi);
     mapLinear(t, X, xi, x, J);
     for (int r = 0; r < 3; r++)
         m.fmeanN[(size_t)r * m.F + f] = mn[r] / l, m.fcent[(size_t)r * m.F + f] = x[r];
