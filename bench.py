@@ -133,6 +133,78 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     return out
 
 
+def implicit_tet_channel(shape=(180, 60, 30), Re=1e3, Re_target=1e4, tol=1e-8, CFL=1000.0, sgsIter=3,
+                         max_iter=20000, fixed_iter=1000, time_to_tol=True):
+    """The laminar bump channel on tetrahedra: channel [0,3] x [0,1] x [0,0.5], sinusoidal
+    bump of 0.1 of the height on the lower wall, a jittered (10 % of the edge) isotropic
+    lattice of shape[0] x shape[1] x shape[2] cubes split into six tetrahedra each
+    (6 * 180 * 60 * 30 = 1 944 000 cells), Mach 0.5, p1 VR (intOrder 3: 6-point face and
+    8-point cell rules), Roe, laminar viscous fluxes (constant mu = rho U H / Re, Pr 0.72),
+    no-slip adiabatic lower wall, free stream through the Riemann flux at inlet, outlet
+    and top, spanwise periodic; implicit LU-SGS pseudo-time loop with an infinite
+    physical step.  At Re = Re_target (1e4) the steady iteration does not settle (the
+    residual stalls near 1e-2 of its start and grows again, a separated flow that does
+    not stay steady at this resolution), so time-to-tolerance is measured at Re (1e3)
+    on the same mesh, and Re_target is reported as ms per iteration and the residual
+    after fixed_iter iterations.  Each precision: FP64 sweeps, FP32 sweeps (residual,
+    reconstruction and update in FP64)."""
+    import torch
+    from dndsr_amd import bc
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    gamma = 1.4
+    far = np.array([1.0, 0.5, 0.0, 0.0, 1 / gamma / (gamma - 1) + 0.5 * 0.25])  # Mach 0.5, a = 1, H = 1
+    types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL, bc.BC_IN]
+    m = mm.bump_channel_tet(*shape, jitter=0.1, seed=1)
+    g = VariationalReconstruction(m, dict(maxOrder=1, intOrder=3, bcTypes=types)).construct()
+    u0 = g.to_device(np.tile(far, (g.nCell, 1)))
+    out = {"case": (f"laminar NS bump channel, {shape[0]}x{shape[1]}x{shape[2]} cubes x 6 = {g.nCell} jittered "
+                    f"tetrahedra, Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far boundaries"),
+           "settings": {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
+                        "physical_dt": "infinite (steady)"}}
+
+    def evaluator(re):
+        gas = dict(Rgas=1.0, muGas=0.5 / re, prGas=0.72, muModel=0)
+        return EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
+                                      bcValues=np.stack([far] * len(types)), viscous=True, idealGasProperty=gas))
+
+    def run(ev, fp32, maxIter, tol_, minIter=5):
+        u = u0.clone()
+        rec = g.BuildURec(5)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        n, res, rel, hist = ev.ImplicitEulerStep(u, rec, 1e100, CFL=CFL, sgsIter=sgsIter, maxIter=maxIter,
+                                                 minIter=minIter, tol=tol_, fp32Sweeps=fp32)
+        torch.cuda.synchronize()
+        return (time.perf_counter() - t0) * 1e3, n, float(rel[0]), u
+
+    ev = evaluator(Re)
+    states = {}
+    conv = {"Re": Re}
+    for name, fp32 in ((("fp64", False), ("mixed", True)) if time_to_tol else ()):
+        run(ev, fp32, 20, 0.0, 20)  # warm-up: kernels, allocations
+        print(f"[bench] tet channel Re {Re:g} {name} ...", file=sys.stderr, flush=True)
+        ms, n, rel, u = run(ev, fp32, max_iter, tol)
+        states[name] = g.to_host(u)
+        conv[name] = {"ms": round(ms, 1), "iterations": n, "resRel_rho": rel, "ms_per_iteration": round(ms / n, 4),
+                      "converged": bool(rel < tol)}
+    if time_to_tol:
+        a, b = states["mixed"], states["fp64"]
+        conv["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+        conv["speedup_mixed"] = round(conv["fp64"]["ms"] / conv["mixed"]["ms"], 3)
+        out["time_to_tol"] = conv
+    ev = evaluator(Re_target)
+    fixed = {"Re": Re_target, "iterations": fixed_iter}
+    for name, fp32 in (("fp64", False), ("mixed", True)):
+        print(f"[bench] tet channel Re {Re_target:g} {name}, {fixed_iter} iterations ...", file=sys.stderr, flush=True)
+        ms, n, rel, u = run(ev, fp32, fixed_iter, 0.0, fixed_iter)
+        fixed[name] = {"ms": round(ms, 1), "ms_per_iteration": round(ms / n, 4), "resRel_rho": rel}
+    fixed["speedup_mixed"] = round(fixed["fp64"]["ms"] / fixed["mixed"]["ms"], 3)
+    out["fixed_iterations"] = fixed
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +212,10 @@ def main():
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--n", type=int, default=64, help="cells per direction")
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit time-to-tolerance measurement")
+    ap.add_argument("--no-implicit", action="store_true", help="skip the implicit time-to-tolerance measurements")
+    ap.add_argument("--tet-iterations", type=int, default=0, metavar="K",
+                    help="only run K implicit iterations of the laminar tetrahedral channel (Re 1e4) with FP64 and "
+                         "with FP32 sweeps (a short run for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -153,6 +228,10 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8, "launches": 3}))
+        return
+    if args.tet_iterations:
+        r = implicit_tet_channel(time_to_tol=False, fixed_iter=args.tet_iterations)
+        print(json.dumps(r), flush=True)
         return
     if args.gpus != 1:
         print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
@@ -227,9 +306,10 @@ def main():
                 "frac": round(droof["frac"], 4), "traffic": droof["traffic"], "kernel": dom,
                 "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5)}
 
-    implicit = None
+    implicit = implicit_tet = None
     if not args.no_implicit:
         implicit = implicit_bump_time_to_tol()
+        implicit_tet = implicit_tet_channel()
 
     cpu = None
     if not args.no_cpu_baseline:
@@ -245,6 +325,7 @@ def main():
         "roofline": roofline,
         "cpu_baseline": cpu,
         "implicit": implicit,
+        "implicit_laminar_tet": implicit_tet,
         "kernels": kern,
         "roofline_all": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                          for k, r in roof.items()},
