@@ -926,62 +926,101 @@ __global__ void __launch_bounds__(64) kRHSBoundary(DevMesh m, GasParams gp, cons
 // GradR = GradL; slip walls, symmetry planes, far field and outflow carry no viscous
 // flux (their mean gradient is zeroed, :427-435, or the flux is, :1200-1204).
 // ---------------------------------------------------------------------------
-// Value and gradient of the reconstruction of cell c at point p (in the cell's frame).
+// One side of a face for the viscous pass: the cell's mean, centre, scale and basis
+// moments, read once per face; with a linear basis (M <= 3) also its reconstruction
+// coefficients, kept in registers across the face's points.  eval() returns the
+// reconstruction's value and gradient at a point p in the cell's frame.
 template <int DIM, int NDOF>
-__device__ inline void pointValueGrad(const DevMesh &m, idx c, const real *p, const real *Um,
-                                      const real *__restrict__ uRec, real *U, real (*Gr)[5])
+struct SideRec
 {
-    constexpr int M = NDOF - 1;
-#pragma unroll
-    for (int v = 0; v < 5; v++)
+    static constexpr int M = NDOF - 1;
+    static constexpr bool REG = M <= 3;
+    idx c;
+    real um[5], cen[3], s, mom[M];
+    real rec[REG ? 5 : 1][REG ? M : 1];
+
+    __device__ __forceinline__ void load(const DevMesh &m, idx cc, const real *__restrict__ u,
+                                         const real *__restrict__ uRec)
     {
-        U[v] = Um[v];
+        const size_t N = m.N;
+        c = cc;
 #pragma unroll
-        for (int d = 0; d < 3; d++)
-            Gr[d][v] = 0;
+        for (int v = 0; v < 5; v++)
+            um[v] = u[(size_t)v * N + c];
+        if (!uRec)
+            return;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cen[r] = m.cent[(size_t)r * N + c];
+        s = m.scale[c];
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            mom[j] = m.moment[(size_t)j * N + c];
+        if constexpr (REG)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    rec[v][j] = uRec[((size_t)v * M + j) * N + c];
     }
-    if (!uRec)
-        return;
-    const size_t N = m.N;
-    const real s = m.scale[c];
-    real xs[3];
-#pragma unroll
-    for (int r = 0; r < 3; r++)
-        xs[r] = p[r] - m.cent[(size_t)r * N + c];
-    xs[0] /= s, xs[1] /= s;
-    if (DIM == 3)
-        xs[2] /= s;
-    real phi[M], dphi[DIM][M];
-#pragma unroll
-    for (int j = 0; j < M; j++)
+
+    __device__ __forceinline__ void eval(const DevMesh &m, const real *p, const real *__restrict__ uRec, real *U,
+                                         real (*Gr)[5]) const
     {
-        phi[j] = basisDiff(DIM, 0, j + 1, xs, s) - m.moment[(size_t)j * N + c];
 #pragma unroll
-        for (int d = 0; d < DIM; d++)
-            dphi[d][j] = basisDiff(DIM, 1 + d, j + 1, xs, s);
-    }
+        for (int v = 0; v < 5; v++)
+        {
+            U[v] = um[v];
 #pragma unroll
-    for (int v = 0; v < 5; v++)
-    {
-        real a = 0, g[DIM];
+            for (int d = 0; d < 3; d++)
+                Gr[d][v] = 0;
+        }
+        if (!uRec)
+            return;
+        real xs[3];
 #pragma unroll
-        for (int d = 0; d < DIM; d++)
-            g[d] = 0;
+        for (int r = 0; r < 3; r++)
+            xs[r] = p[r] - cen[r];
+        xs[0] /= s, xs[1] /= s;
+        if (DIM == 3)
+            xs[2] /= s;
+        real phi[M], dphi[DIM][M];
 #pragma unroll
         for (int j = 0; j < M; j++)
         {
-            const real r = uRec[((size_t)v * M + j) * N + c];
-            a += phi[j] * r;
+            phi[j] = basisDiff(DIM, 0, j + 1, xs, s) - mom[j];
 #pragma unroll
             for (int d = 0; d < DIM; d++)
-                g[d] += dphi[d][j] * r;
+                dphi[d][j] = basisDiff(DIM, 1 + d, j + 1, xs, s);
         }
-        U[v] += a;
+        const size_t N = m.N;
 #pragma unroll
-        for (int d = 0; d < DIM; d++)
-            Gr[d][v] = g[d];
+        for (int v = 0; v < 5; v++)
+        {
+            real a = 0, g[DIM];
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                g[d] = 0;
+#pragma unroll
+            for (int j = 0; j < M; j++)
+            {
+                real r;
+                if constexpr (REG)
+                    r = rec[v][j];
+                else
+                    r = uRec[((size_t)v * M + j) * N + c];
+                a += phi[j] * r;
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    g[d] += dphi[d][j] * r;
+            }
+            U[v] += a;
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                Gr[d][v] = g[d];
+        }
     }
-}
+};
 
 template <int DIM, int NDOF>
 __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, const real *__restrict__ u,
@@ -997,11 +1036,12 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
     if (bnd && (type == BC_WALL_INVIS || type == BC_SYM || type == BC_FAR || type == BC_OUT))
         return;
     const real *far = bnd ? m.bcVal + (size_t)m.fzone[f] * 5 : nullptr;
-    real ULm[5], URm[5], dB[3], sh[3];
+    real dB[3], sh[3];
     real minVol = m.vol[cL];
-#pragma unroll
-    for (int v = 0; v < 5; v++)
-        ULm[v] = u[(size_t)v * N + cL], URm[v] = bnd ? 0.0 : u[(size_t)v * N + cR];
+    SideRec<DIM, NDOF> L, R;
+    L.load(m, cL, u, uRec);
+    if (!bnd)
+        R.load(m, cR, u, uRec);
     if (!bnd)
     {
         minVol = fmin(minVol, m.vol[cR]);
@@ -1026,7 +1066,7 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
         for (int r = 0; r < 3; r++)
             x[r] = m.fpts[((size_t)g * 3 + r) * F + f], n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
         real UL[5], UR[5], GL[3][5], GR[3][5];
-        pointValueGrad<DIM, NDOF>(m, cL, x, ULm, uRec, UL, GL);
+        L.eval(m, x, uRec, UL, GL);
         if (bnd)
         {
             bcGhost(type, UL, n, far, gamma, UR);
@@ -1039,7 +1079,7 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
         else
         {
             real xR[3] = {x[0] + sh[0], x[1] + sh[1], x[2] + sh[2]};
-            pointValueGrad<DIM, NDOF>(m, cR, xR, URm, uRec, UR, GR);
+            R.eval(m, xR, uRec, UR, GR);
         }
         const real distBaryPerp = fabs(dot3(dB, n)) * (bnd ? 2.0 : 1.0);
         const real distGRP = fmax(fmin(distBaryPerp, d0 * 2), d0 * 0.25);

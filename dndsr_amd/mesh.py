@@ -572,3 +572,45 @@ def bump_channel_tet(nx: int, ny: int, nz: int, L: float = 3.0, H: float = 1.0, 
 
     return tet_grid(nx, ny, nz, L, H, W, periodic=(False, False, True),
                     zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)
+
+
+def permute_cells(m: Mesh, order: np.ndarray) -> Mesh:
+    """The same mesh with cell order[k] renumbered k.  Faces are reordered by their back
+    cell's new index (stable), keeping each face's orientation, node order, shift and
+    zone; cell2face keeps each cell's element face order.  A field u of the original
+    mesh corresponds to u[order] on the new one."""
+    order = np.asarray(order, np.int64)
+    N = m.nCell
+    if order.shape != (N,) or not np.array_equal(np.sort(order), np.arange(N)):
+        raise ValueError("order must be a permutation of the cells")
+    new_of = np.empty(N, np.int64)
+    new_of[order] = np.arange(N)
+    f2c = m.face2cell
+    f2c_new = np.where(f2c >= 0, new_of[np.maximum(f2c, 0)], -1)
+    forder = np.argsort(f2c_new[:, 0], kind="stable")
+    fnew_of = np.empty(m.nFace, np.int64)
+    fnew_of[forder] = np.arange(m.nFace)
+    cptr = m.cell2nodePtr
+    nper = np.diff(cptr)
+    fptr = m.cell2facePtr
+    fper = np.diff(fptr)
+    if np.any(nper != nper[0]) or np.any(fper != fper[0]):
+        raise ValueError("permute_cells expects one cell type")
+    c2n = m.cell2node.reshape(N, nper[0])[order].ravel()
+    c2f = fnew_of[m.cell2face.reshape(N, fper[0])[order]].ravel()
+    fn = np.diff(m.face2nodePtr)
+    if np.any(fn != fn[0]):
+        raise ValueError("permute_cells expects one face type")
+    f2n = m.face2node.reshape(m.nFace, fn[0])[forder].ravel()
+    return Mesh(m.dim, m.coords, m.cellType[order], m.cell2nodePtr.copy(), c2n, m.cell2facePtr.copy(), c2f,
+                m.faceType[forder], m.face2nodePtr.copy(), f2n, f2c_new[forder], m.faceShift[forder], m.shape,
+                None if m.faceZone is None else m.faceZone[forder], m.nBCZone)
+
+
+def colour_sorted(m: Mesh, colors: np.ndarray) -> Mesh:
+    """The mesh with its cells grouped by colour (stable within a colour), colors being
+    the library's multicolouring of m (VariationalReconstruction.colors()).  The library
+    colours greedily in cell order, so on the new mesh it finds the same colour classes,
+    now contiguous index ranges: each Gauss-Seidel colour pass then reads its own cells'
+    rows coalesced instead of every few elements."""
+    return permute_cells(m, np.argsort(np.asarray(colors), kind="stable"))

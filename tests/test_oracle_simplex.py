@@ -208,3 +208,26 @@ def test_viscous_shear_converges_on_triangles():
         errs.append(np.abs(sq_r[:, 1] - sq_ex[:, 1]).max() / np.abs(sq_ex[:, 1]).max())
     assert errs[-1] < 1e-2
     assert all(np.log2(a / b) > 1.5 for a, b in zip(errs[:-1], errs[1:])), errs
+
+
+def test_permute_cells_is_a_relabelling():
+    """EvaluateRHS (viscous, with boundary faces) and the VR sweep commute with a cell
+    renumbering: the permuted mesh gives the permuted residual."""
+    m = mm.bump_channel_tet(6, 3, 2, jitter=0.1, seed=1)
+    rng = np.random.default_rng(0)
+    order = rng.permutation(m.nCell)
+    m2 = mm.permute_cells(m, order)
+    types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL, bc.BC_IN]
+    far = _cons(1.0, [0.5, 0.0, 0.0], 1 / GAMMA)
+    os_ = []
+    for mesh in (m, m2):
+        o = po.OracleVR(mesh, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+        o.set_bc(types, np.stack([far] * 4), GAMMA)
+        os_.append(o)
+    u = far + rng.uniform(-0.05, 0.05, (m.nCell, 5))
+    rec = rng.uniform(-0.01, 0.01, (m.nCell, 5, os_[0].M))
+    es = os_[0].euler_settings(GAMMA, 1, viscous=1, muModel=0, muGas=0.01)
+    r1 = os_[0].eval_rhs(es, u, os_[0].vr_iter(rec, u), None)
+    r2 = os_[1].eval_rhs(es, u[order], os_[1].vr_iter(rec[order], u[order]), None)
+    assert np.abs(r1[order] - r2).max() <= 1e-13 * np.abs(r1).max()
+    assert not np.array_equal(m2.face2cell, m.face2cell)
