@@ -26,6 +26,9 @@ struct dndsr_ctx
     real *iXLast = nullptr, *iRhsF = nullptr, *iRhs = nullptr, *iDTau = nullptr, *iJd = nullptr, *iInc = nullptr,
          *iTmp = nullptr, *iRes = nullptr;
     float *fRhs = nullptr, *fU = nullptr, *fJd = nullptr, *fLam = nullptr, *fInc = nullptr, *fTmp = nullptr;
+    // cell-major sweep state and per-cell-slot face data of the packed sweeps
+    real *iSlot = nullptr, *iUA = nullptr;
+    float *fSlot = nullptr, *fUA = nullptr;
     int *iFlag = nullptr;
     bool haveImplicit = false;
     // linear-solver workspace (GMRES Krylov basis for up to linK vectors + temporaries)
@@ -920,7 +923,6 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     }
     const real gamma = es->gamma;
     const real rhoEps = kSmall * is->refRho * 1e-1, pEps = kSmall * is->refP * 1e-1;
-    real *sumDev = c.red + 5 * ((size_t)gridFor(m.N, 256) + m.nColor + 6);
     DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iFlag, 0, sizeof(int), c.stream));
     launchCopy(c, nU, u, ctx->iXLast);
     real *rec = uRec, *recNew = ctx->wURecNew;
@@ -963,35 +965,25 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         }
         else if (is->fp32Sweeps)
         {
+            // sweeps with the neighbour state cell-major (the sweep's gathers), FP32
+            if (!ctx->fSlot)
+                ctx->fSlot = devAlloc<float>(c, (size_t)m.K * 5 * N), ctx->fUA = devAlloc<float>(c, nU);
             launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
-            launchToF32(c, nU, u, ctx->fU);
             launchToF32(c, N, ctx->iJd, ctx->fJd);
             launchToF32(c, F, ctx->wLam, ctx->fLam); // lambdaFace row only
-            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->fInc, 0, nU * sizeof(float), c.stream));
-            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->fTmp, 0, nU * sizeof(float), c.stream));
-            float *inc = ctx->fInc, *tmp = ctx->fTmp;
-            for (int s = 0; s <= is->sgsIter; s++)
-                for (int dir = 1; dir >= 0; dir--)
-                {
-                    launchSGSSweep32(c, gp, 1.0f, ctx->fRhs, ctx->fU, inc, tmp, ctx->fJd, ctx->fLam, dir,
-                                     is->gsUpdate, 0, sumDev);
-                    std::swap(inc, tmp);
-                }
-            launchAddIncrement32(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, ctx->iFlag);
+            float *inc = nullptr;
+            launchSGSPacked32(c, gp, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fInc, ctx->fTmp,
+                              is->sgsIter, is->gsUpdate, &inc);
+            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iFlag);
         }
         else
         {
-            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iInc, 0, nU * sizeof(real), c.stream));
-            DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iTmp, 0, nU * sizeof(real), c.stream));
-            real *inc = ctx->iInc, *tmp = ctx->iTmp;
-            for (int s = 0; s <= is->sgsIter; s++)
-                for (int dir = 1; dir >= 0; dir--)
-                {
-                    launchSGSSweep64(c, gp, 1.0, ctx->iRhs, u, inc, tmp, ctx->iJd, ctx->wLam, dir, is->gsUpdate, 0,
-                                     sumDev);
-                    std::swap(inc, tmp);
-                }
-            launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, ctx->iFlag);
+            if (!ctx->iSlot)
+                ctx->iSlot = devAlloc<real>(c, (size_t)m.K * 5 * N), ctx->iUA = devAlloc<real>(c, nU);
+            real *inc = nullptr;
+            launchSGSPacked64(c, gp, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iInc, ctx->iTmp,
+                              is->sgsIter, is->gsUpdate, &inc);
+            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iFlag);
         }
         // fstop: residual of this iteration relative to its running maximum (density component)
         DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, ctx->iRes, 5 * sizeof(real), hipMemcpyDeviceToHost, c.stream));

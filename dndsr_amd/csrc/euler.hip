@@ -1582,10 +1582,145 @@ void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const
 
 // u += du after the positivity tests of the increment (EvaluateCellRHSAlpha,
 // EulerEvaluator.hxx:1964-2020, with relax = incrementPPRelax) and AddFixedIncrement /
+// ---------------------------------------------------------------------------
+// Packed Gauss-Seidel sweep of the device implicit loop.  Same arithmetic as
+// kSGSColour, other layout: the state the sweep gathers from neighbours (u, uInc,
+// uIncNew) is stored cell-major, [N][5], so one neighbour costs one or two cache
+// lines instead of ten, and the face data a cell needs per slot (signed mean normal,
+// area, lambdaFace) is laid out per cell and slot, [K][5][N], by kSGSSlots once per
+// iteration, so the cell reads it coalesced instead of through the face index.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict__ lam, T *__restrict__ slot)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N)
+        return;
+    const size_t N = m.N, F = m.F;
+    const T *__restrict__ farea = SGSGeom<T>::area(m);
+    const T *__restrict__ fnorm = SGSGeom<T>::norm(m);
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + i];
+        T q[5] = {0, 0, 0, 0, 0};
+        if (f >= 0)
+        {
+            const T sgn = (m.c2flag[(size_t)k * N + i] & 1) ? T(-1) : T(1);
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                q[r] = fnorm[r * F + f] * sgn;
+            q[3] = farea[f];
+            q[4] = lam[f];
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            slot[((size_t)k * 5 + v) * N + i] = q[v];
+    }
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag,
+                                                        const T *__restrict__ rhs, const T *__restrict__ uA,
+                                                        const T *__restrict__ incA, T *__restrict__ incNewA,
+                                                        const T *__restrict__ jd, const T *__restrict__ slot,
+                                                        int forward, int gsUpdate, idx begin, idx end)
+{
+    const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= end)
+        return;
+    const size_t N = m.N;
+    const idx i = m.colorCells[t];
+    const T gamma = (T)gammaD;
+    T buf[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        buf[v] = rhs[v * N + i];
+    const T voli = SGSGeom<T>::vol(m)[i];
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx o = m.c2c[(size_t)k * N + i];
+        const uint8_t fl = m.c2flag[(size_t)k * N + i];
+        if (o < 0 || o == i)
+            continue;
+        const bool earlier = forward ? (fl & 2) != 0 : (fl & 2) == 0;
+        const T *du = (gsUpdate && earlier) ? incNewA : incA;
+        T U[5], dU[5], n[3], dF[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            U[v] = uA[(size_t)o * 5 + v], dU[v] = du[(size_t)o * 5 + v];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            n[r] = slot[((size_t)k * 5 + r) * N + i];
+        const T area = slot[((size_t)k * 5 + 3) * N + i], lamf = slot[((size_t)k * 5 + 4) * N + i];
+        fluxJacDu<T>(U, n, dU, lamf, gamma, dF);
+        T coef = (T(0.5) * alphaDiag) * area / voli;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            buf[v] -= coef * dF[v];
+    }
+    const T d = jd[i];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        incNewA[(size_t)i * 5 + v] = buf[v] / d;
+}
+
+template <typename T>
+__global__ void __launch_bounds__(256) kToCellMajor(idx N, const real *__restrict__ soa, T *__restrict__ aos)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= N)
+        return;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        aos[(size_t)i * 5 + v] = (T)soa[(size_t)v * N + i];
+}
+
+template <typename T>
+void launchSGSPackedT(Ctx &c, const GasParams &gp, const T *rhs, const real *u, const T *lam, const T *jd, T *uA,
+                      T *slot, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
+{
+    DevMesh &m = c.m;
+    const int nb = gridFor(m.N, 256);
+    kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, u, uA);
+    kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, slot);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    const size_t nU = 5 * (size_t)m.N;
+    DNDSR_HIP_CHECK(hipMemsetAsync(inc, 0, nU * sizeof(T), c.stream));
+    DNDSR_HIP_CHECK(hipMemsetAsync(tmp, 0, nU * sizeof(T), c.stream));
+    for (int s = 0; s <= sgsIter; s++)
+        for (int dir = 1; dir >= 0; dir--)
+        {
+            for (int q = 0; q < m.nColor; q++)
+            {
+                const int col = dir ? q : m.nColor - 1 - q;
+                const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
+                if (e <= b)
+                    continue;
+                KernelTimer kt(c, K_SGS);
+                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, T(1), rhs, uA, inc, tmp,
+                                                                                jd, slot, dir, gsUpdate, b, e);
+                DNDSR_HIP_CHECK(hipGetLastError());
+            }
+            std::swap(inc, tmp);
+        }
+    *result = inc;
+}
+void launchSGSPacked64(Ctx &c, const GasParams &gp, const real *rhs, const real *u, const real *lam, const real *jd,
+                       real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate, real **result)
+{
+    launchSGSPackedT<real>(c, gp, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+}
+void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real *u, const float *lam,
+                       const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter, int gsUpdate,
+                       float **result)
+{
+    launchSGSPackedT<float>(c, gp, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+}
+
 // CompressInc (EulerEvaluator.hpp:1645-1830).  When neither density nor pressure
 // would fall below its relaxed bound, both reduce to the plain sum; a cell that
 // would need the limiter sets *flag instead (the caller reports it as an error).
-template <typename T>
+template <typename T, bool CELLMAJOR = false>
 __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real relax, real rhoEps, real pEps,
                                                      real *__restrict__ u, const T *__restrict__ du,
                                                      int *__restrict__ flag)
@@ -1596,7 +1731,7 @@ __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real rel
     real U[5], D[5];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        U[v] = u[(size_t)v * N + i], D[v] = (real)du[(size_t)v * N + i];
+        U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? (size_t)i * 5 + v : (size_t)v * N + i];
     bool limited = false;
     const real relaxedRho = rhoEps * relax + U[0] * (1 - relax);
     if (D[0] < 0 && (U[0] - relaxedRho) / (-D[0] - kSmall * D[0]) < 1.0)
@@ -1622,6 +1757,18 @@ void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps
 {
     KernelTimer kt(c, K_INCREMENT);
     kAddIncrement<real><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, flagDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
+                                 const float *du32, int *flagDev)
+{
+    KernelTimer kt(c, K_INCREMENT);
+    if (du64)
+        kAddIncrement<real, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
+                                                                             du64, flagDev);
+    else
+        kAddIncrement<float, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
+                                                                              du32, flagDev);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,

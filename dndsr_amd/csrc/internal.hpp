@@ -221,6 +221,17 @@ void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
                           int *flagDev);
 void launchToF32(Ctx &c, size_t n, const real *in, float *out);
+// (1 + sgsIter) forward/backward multicolour sweeps from a zero increment with the state
+// gathered from neighbours stored cell-major: uA <- u as [N][5], slot <- per-cell-slot face
+// data; inc / tmp are [N][5] scratch; *result points at the buffer holding the increment.
+void launchSGSPacked64(Ctx &c, const GasParams &gp, const real *rhs, const real *u, const real *lam, const real *jd,
+                       real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate, real **result);
+void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real *u, const float *lam,
+                       const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter, int gsUpdate,
+                       float **result);
+// u += du with du cell-major ([N][5], FP64 or FP32: exactly one of du64 / du32 non-null)
+void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
+                                 const float *du32, int *flagDev);
 // ---- linear solver (linsolve.hip)
 struct LinWork
 {
