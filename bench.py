@@ -135,22 +135,20 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     return out
 
 
-def implicit_tet_channel(shape=(180, 60, 30), Re=1e3, Re_target=1e4, tol=1e-8, CFL=1000.0, sgsIter=3,
-                         max_iter=20000, fixed_iter=1000, time_to_tol=True):
-    """The laminar bump channel on tetrahedra: channel [0,3] x [0,1] x [0,0.5], sinusoidal
-    bump of 0.1 of the height on the lower wall, a jittered (10 % of the edge) isotropic
-    lattice of shape[0] x shape[1] x shape[2] cubes split into six tetrahedra each
-    (6 * 180 * 60 * 30 = 1 944 000 cells), Mach 0.5, p1 VR (intOrder 3: 6-point face and
-    8-point cell rules), Roe, laminar viscous fluxes (constant mu = rho U H / Re, Pr 0.72),
-    no-slip adiabatic lower wall, free stream through the Riemann flux at inlet, outlet
-    and top, spanwise periodic; implicit LU-SGS pseudo-time loop with an infinite
-    physical step.  At Re = Re_target (1e4) the steady iteration does not settle (the
-    residual stalls near 1e-2 of its start and grows again, a separated flow that does
-    not stay steady at this resolution), so time-to-tolerance is measured at Re (1e3)
-    on the same mesh, and Re_target is reported as ms per iteration and the residual
-    after fixed_iter iterations.  The cells are grouped by Gauss-Seidel colour
-    (mesh.colour_sorted) before the timed runs.  Each precision: FP64 sweeps, FP32 sweeps (residual,
-    reconstruction and update in FP64)."""
+def implicit_laminar_tet(n=69, Re=1e4, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=20000, fixed_iter=0):
+    """Second half of the metric on the laminar configuration: steady laminar
+    Navier-Stokes flow in the unit box with a sinusoidal bump of 0.1 of the height on
+    the lower wall (y_w = 0.1 sin^2(pi x)), an n^3 lattice (n = 69: 70^3 vertices,
+    jittered uniformly by +-10 % of the edge, PCG64 seed 20261202) split into six
+    tetrahedra per cube (1 971 054 cells), Re = 1e4 on the height and the free-stream
+    speed, Mach 0.5, p1 VR (intOrder 3: 6-point face and 8-point cell rules), Roe,
+    laminar viscous fluxes (constant mu, Pr 0.72), no-slip adiabatic lower wall, free
+    stream through the Riemann flux at inlet, outlet and top, spanwise periodic; the
+    implicit LU-SGS pseudo-time loop (infinite physical step) to a density residual
+    below tol of its running maximum, with FP64 sweeps and with FP32 sweeps (residual,
+    reconstruction and update in FP64).  Cells are grouped by Gauss-Seidel colour
+    (mesh.colour_sorted) first.  fixed_iter > 0 runs that many iterations per precision
+    instead (short runs for kernel profiles)."""
     import torch
     from dndsr_amd import bc
     from dndsr_amd import mesh as mm
@@ -159,54 +157,44 @@ def implicit_tet_channel(shape=(180, 60, 30), Re=1e3, Re_target=1e4, tol=1e-8, C
     gamma = 1.4
     far = np.array([1.0, 0.5, 0.0, 0.0, 1 / gamma / (gamma - 1) + 0.5 * 0.25])  # Mach 0.5, a = 1, H = 1
     types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL, bc.BC_IN]
-    m = mm.bump_channel_tet(*shape, jitter=0.1, seed=1)
+    m = mm.bump_box_tet(n)
     vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
     m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
     g = VariationalReconstruction(m, vs).construct()
+    ev = EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
+                                bcValues=np.stack([far] * len(types)), viscous=True,
+                                idealGasProperty=dict(Rgas=1.0, muGas=0.5 / Re, prGas=0.72, muModel=0)))
     u0 = g.to_device(np.tile(far, (g.nCell, 1)))
-    out = {"case": (f"laminar NS bump channel, {shape[0]}x{shape[1]}x{shape[2]} cubes x 6 = {g.nCell} jittered "
-                    f"tetrahedra, Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far boundaries"),
-           "settings": {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
-                        "physical_dt": "infinite (steady)"}}
 
-    def evaluator(re):
-        gas = dict(Rgas=1.0, muGas=0.5 / re, prGas=0.72, muModel=0)
-        return EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
-                                      bcValues=np.stack([far] * len(types)), viscous=True, idealGasProperty=gas))
-
-    def run(ev, fp32, maxIter, tol_, minIter=5):
+    def run(fp32, maxIter, tol_, minIter=5):
         u = u0.clone()
         rec = g.BuildURec(5)
         torch.cuda.synchronize()
         t0 = time.perf_counter()
-        n, res, rel, hist = ev.ImplicitEulerStep(u, rec, 1e100, CFL=CFL, sgsIter=sgsIter, maxIter=maxIter,
-                                                 minIter=minIter, tol=tol_, fp32Sweeps=fp32)
+        n_, res, rel, hist = ev.ImplicitEulerStep(u, rec, 1e100, CFL=CFL, sgsIter=sgsIter, maxIter=maxIter,
+                                                  minIter=minIter, tol=tol_, fp32Sweeps=fp32)
         torch.cuda.synchronize()
-        return (time.perf_counter() - t0) * 1e3, n, float(rel[0]), u
+        return (time.perf_counter() - t0) * 1e3, n_, float(rel[0]), u
 
-    ev = evaluator(Re)
+    out = {"case": (f"steady laminar NS, unit box with a 0.1H sin^2 bump, {n}^3 cubes x 6 = {g.nCell} jittered "
+                    f"tetrahedra, Re {Re:g}, Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far "
+                    f"boundaries, cells grouped by colour"),
+           "settings": {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
+                        "physical_dt": "infinite (steady)"}}
     states = {}
-    conv = {"Re": Re}
-    for name, fp32 in ((("fp64", False), ("mixed", True)) if time_to_tol else ()):
-        run(ev, fp32, 20, 0.0, 20)  # warm-up: kernels, allocations
-        print(f"[bench] tet channel Re {Re:g} {name} ...", file=sys.stderr, flush=True)
-        ms, n, rel, u = run(ev, fp32, max_iter, tol)
-        states[name] = g.to_host(u)
-        conv[name] = {"ms": round(ms, 1), "iterations": n, "resRel_rho": rel, "ms_per_iteration": round(ms / n, 4),
-                      "converged": bool(rel < tol)}
-    if time_to_tol:
-        a, b = states["mixed"], states["fp64"]
-        conv["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
-        conv["speedup_mixed"] = round(conv["fp64"]["ms"] / conv["mixed"]["ms"], 3)
-        out["time_to_tol"] = conv
-    ev = evaluator(Re_target)
-    fixed = {"Re": Re_target, "iterations": fixed_iter}
     for name, fp32 in (("fp64", False), ("mixed", True)):
-        print(f"[bench] tet channel Re {Re_target:g} {name}, {fixed_iter} iterations ...", file=sys.stderr, flush=True)
-        ms, n, rel, u = run(ev, fp32, fixed_iter, 0.0, fixed_iter)
-        fixed[name] = {"ms": round(ms, 1), "ms_per_iteration": round(ms / n, 4), "resRel_rho": rel}
-    fixed["speedup_mixed"] = round(fixed["fp64"]["ms"] / fixed["mixed"]["ms"], 3)
-    out["fixed_iterations"] = fixed
+        if fixed_iter:
+            ms, n_, rel, u = run(fp32, fixed_iter, 0.0, fixed_iter)
+        else:
+            run(fp32, 20, 0.0, 20)  # warm-up: kernels, allocations
+            print(f"[bench] laminar tet box {name} ...", file=sys.stderr, flush=True)
+            ms, n_, rel, u = run(fp32, max_iter, tol)
+        states[name] = g.to_host(u)
+        out[name] = {"ms": round(ms, 1), "iterations": n_, "resRel_rho": rel, "ms_per_iteration": round(ms / n_, 4),
+                     "converged": bool(rel < tol)}
+    a, b = states["mixed"], states["fp64"]
+    out["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
+    out["speedup_mixed"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
     return out
 
 
@@ -219,8 +207,8 @@ def main():
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-implicit", action="store_true", help="skip the implicit time-to-tolerance measurements")
     ap.add_argument("--tet-iterations", type=int, default=0, metavar="K",
-                    help="only run K implicit iterations of the laminar tetrahedral channel (Re 1e4) with FP64 and "
-                         "with FP32 sweeps (a short run for kernel profiles)")
+                    help="only run K implicit iterations of the laminar tetrahedral case with FP64 and with FP32 "
+                         "sweeps (a short run for kernel profiles)")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -235,7 +223,7 @@ def main():
         print(json.dumps({"copy_bytes": n * 8, "launches": 3}))
         return
     if args.tet_iterations:
-        r = implicit_tet_channel(time_to_tol=False, fixed_iter=args.tet_iterations)
+        r = implicit_laminar_tet(fixed_iter=args.tet_iterations)
         print(json.dumps(r), flush=True)
         return
     if args.gpus != 1:
@@ -314,7 +302,7 @@ def main():
     implicit = implicit_tet = None
     if not args.no_implicit:
         implicit = implicit_bump_time_to_tol()
-        implicit_tet = implicit_tet_channel()
+        implicit_tet = implicit_laminar_tet()
 
     cpu = None
     if not args.no_cpu_baseline:

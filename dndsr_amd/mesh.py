@@ -614,3 +614,19 @@ def colour_sorted(m: Mesh, colors: np.ndarray) -> Mesh:
     now contiguous index ranges: each Gauss-Seidel colour pass then reads its own cells'
     rows coalesced instead of every few elements."""
     return permute_cells(m, np.argsort(np.asarray(colors), kind="stable"))
+
+
+def bump_box_tet(n: int = 69, amp: float = 0.1, jitter: float = 0.1, seed: int = 20261202) -> Mesh:
+    """Unit box [0,1]^3 on an n^3 lattice (n + 1 points per direction, vertices jittered
+    uniformly by +-jitter of the edge with numpy's PCG64 seeded by seed, boundary
+    vertices only within their side), each cube split into six Tet4 cells (6 n^3 cells;
+    n = 69: 1 971 054), the lower wall raised by a bump y_w = amp sin^2(pi x) with the
+    grid lines following it (y' = y + y_w (1 - y)).  z periodic.  Zones: 0 inlet (x = 0),
+    1 outlet (x = 1), 2 lower wall, 3 upper boundary."""
+    def bump(P):
+        Q = P.copy()
+        Q[:, 1] = P[:, 1] + amp * np.sin(np.pi * P[:, 0]) ** 2 * (1 - P[:, 1])
+        return Q
+
+    return tet_grid(n, n, n, 1.0, 1.0, 1.0, periodic=(False, False, True),
+                    zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)

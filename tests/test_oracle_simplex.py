@@ -231,3 +231,23 @@ def test_permute_cells_is_a_relabelling():
     r2 = os_[1].eval_rhs(es, u[order], os_[1].vr_iter(rec[order], u[order]), None)
     assert np.abs(r1[order] - r2).max() <= 1e-13 * np.abs(r1).max()
     assert not np.array_equal(m2.face2cell, m.face2cell)
+
+
+def test_bump_box_tet_mesh():
+    """Unit box with the lower wall raised by 0.1 sin^2(pi x): six tetrahedra per cube,
+    volume 1 - 0.1 int sin^2 = 0.95 up to the chord error of the linear faces, every
+    boundary face in its zone (inlet/outlet x = 0 / 1, wall, top y = 1), z periodic."""
+    n = 8
+    m = mm.bump_box_tet(n, jitter=0.1)
+    assert m.nCell == 6 * n**3
+    o = po.OracleVR(m, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights([bc.BC_FAR] * 4, 1))
+    assert abs(o.cell_metrics()["vol"].sum() - 0.95) < 2e-3
+    fm = o.face_metrics()
+    bnd = m.face2cell[:, 1] < 0
+    cen = fm["pts"].mean(1)
+    z = m.faceZone
+    assert np.all(np.abs(cen[bnd & (z == 0), 0]) < 1e-12) and np.all(np.abs(cen[bnd & (z == 1), 0] - 1) < 1e-12)
+    assert np.all(np.abs(cen[bnd & (z == 3), 1] - 1) < 1e-12)
+    wall = bnd & (z == 2)
+    assert np.all(np.abs(cen[wall, 1] - 0.1 * np.sin(np.pi * cen[wall, 0]) ** 2) < 0.02)
+    assert set(np.unique(z[bnd])) == {0, 1, 2, 3}
