@@ -219,46 +219,66 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     need(md->nBCZone >= 0 && md->nBCZone < 4096, "nBCZone out of range");
     m.nBCZone = md->nBCZone;
     const size_t N = m.N, F = m.F;
-    // element types: one cell type and one face type per mesh on this path
+    // element types: one face type per mesh; cells of one type, or (2-D) Tri3 and Quad4 mixed
     m.cellType = md->cellType[0];
     m.faceType = md->faceType[0];
+    bool mixed = false;
     for (size_t i = 0; i < N; i++)
-        need(md->cellType[i] == m.cellType, "mixed cell types are not supported");
+        mixed |= md->cellType[i] != m.cellType;
     for (size_t f = 0; f < F; f++)
         need(md->faceType[f] == m.faceType, "mixed face types are not supported");
     if (m.dim == 2)
-        need((m.cellType == DNDSR_QUAD4 || m.cellType == DNDSR_TRI3) && m.faceType == DNDSR_LINE2,
-             "2-D meshes must be Quad4 or Tri3 cells / Line2 faces");
+    {
+        for (size_t i = 0; i < N; i++)
+            need(md->cellType[i] == DNDSR_QUAD4 || md->cellType[i] == DNDSR_TRI3,
+                 "2-D meshes must be Quad4 and/or Tri3 cells / Line2 faces");
+        need(m.faceType == DNDSR_LINE2, "2-D meshes must have Line2 faces");
+    }
     else
+    {
+        need(!mixed, "3-D meshes take one cell type");
         need((m.cellType == DNDSR_HEX8 && m.faceType == DNDSR_QUAD4) ||
                  (m.cellType == DNDSR_TET4 && m.faceType == DNDSR_TRI3),
              "3-D meshes must be Hex8 cells / Quad4 faces or Tet4 cells / Tri3 faces");
-    const int nnc = nodeCount(m.cellType), nnf = nodeCount(m.faceType);
+    }
+    const int nnc = mixed ? 4 : nodeCount(m.cellType), nnf = nodeCount(m.faceType);
+    std::vector<int8_t> hct;
+    if (mixed)
+    {
+        hct.resize(N);
+        for (size_t i = 0; i < N; i++)
+            hct[i] = (int8_t)md->cellType[i];
+        m.cellType = 0;
+    }
     int K = 0;
     for (size_t i = 0; i < N; i++)
     {
-        need(md->cell2nodePtr[i + 1] - md->cell2nodePtr[i] == nnc, "bad cell node count");
         K = std::max<int>(K, (int)(md->cell2facePtr[i + 1] - md->cell2facePtr[i]));
     }
     need(K <= kMaxFC, "too many faces per cell");
     m.K = K;
     need(s->intOrder >= 0 && s->intOrder <= 6, "intOrder must be 0..6");
     m.G = ruleSize(m.faceType, s->intOrder);
-    m.Gc = ruleSize(m.cellType, s->intOrder);
+    m.Gc = mixed ? std::max(ruleSize(E_QUAD4, s->intOrder), ruleSize(E_TRI3, s->intOrder))
+                 : ruleSize(m.cellType, s->intOrder);
     need(m.G <= kMaxG, "too many face quadrature points");
 
     std::vector<real> hc(3 * (size_t)m.nNode);
     for (size_t a = 0; a < (size_t)m.nNode; a++)
         for (int r = 0; r < 3; r++)
             hc[r * m.nNode + a] = md->coords[3 * a + r];
-    std::vector<idx> hcn((size_t)nnc * N), hfn((size_t)nnf * F), hf2c(2 * F), hc2f((size_t)K * N, -1);
+    std::vector<idx> hcn((size_t)nnc * N, 0), hfn((size_t)nnf * F), hf2c(2 * F), hc2f((size_t)K * N, -1);
     for (size_t i = 0; i < N; i++)
-        for (int a = 0; a < nnc; a++)
+    {
+        const int nn = nodeCount(md->cellType[i]);
+        need(md->cell2nodePtr[i + 1] - md->cell2nodePtr[i] == nn, "bad cell node count");
+        for (int a = 0; a < nn; a++)
         {
             int64_t nd = md->cell2node[md->cell2nodePtr[i] + a];
             need(nd >= 0 && nd < md->nNode, "cell2node out of range");
             hcn[a * N + i] = (idx)nd;
         }
+    }
     std::vector<real> hsh(3 * F);
     std::vector<int32_t> hzone(F, -1);
     std::vector<idx> hbnd;
@@ -376,6 +396,11 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     }
     upload(c, m.coords, hc);
     upload(c, m.cell2node, hcn);
+    if (mixed)
+    {
+        m.ctype = devAlloc<int8_t>(c, N);
+        upload(c, m.ctype, hct);
+    }
     upload(c, m.face2node, hfn);
     upload(c, m.f2c, hf2c);
     upload(c, m.c2f, hc2f);

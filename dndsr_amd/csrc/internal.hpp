@@ -50,7 +50,8 @@ struct DevMesh
     int K = 0;      // max faces per cell
     int G = 0;      // face quadrature points
     int Gc = 0;     // cell quadrature points
-    int cellType = 0, faceType = 0;
+    int cellType = 0, faceType = 0; // cellType: the type of every cell, or 0 on a mixed mesh
+    int8_t *ctype = nullptr;         // [N] per-cell type on a mixed (Tri3 + Quad4) mesh
     VRSettingsDev s{};
     // connectivity
     idx *f2c = nullptr;     // [2][F], -1 = boundary
@@ -60,7 +61,7 @@ struct DevMesh
     uint8_t *c2flag = nullptr; // [K][N] bit 0: the cell is the face's front (R) side;
                                //        bit 1: the neighbour's colour precedes the cell's
     float *fareaF = nullptr, *fmeanNF = nullptr, *volF = nullptr; // FP32 copies for the FP32 sweeps
-    idx *cell2node = nullptr; // [nnc][N]
+    idx *cell2node = nullptr; // [nnc][N] (nnc = the most nodes of a cell type present)
     idx *face2node = nullptr; // [nnf][F]
     real *coords = nullptr;   // [3][nNode]
     // geometry

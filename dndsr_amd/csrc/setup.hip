@@ -20,7 +20,7 @@ __global__ void __launch_bounds__(128) kCellMetrics(DevMesh m)
     idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.N)
         return;
-    const int t = m.cellType;
+    const int t = m.ctype ? (int)m.ctype[i] : m.cellType;
     const int nn = nodeCount(t);
     real X[8][3];
     for (int a = 0; a < nn; a++)
@@ -43,7 +43,14 @@ __global__ void __launch_bounds__(128) kCellMetrics(DevMesh m)
     m.vol[i] = v;
     // integration-order points, w*JDet and barycentre (FiniteVolume.cpp:84-216)
     real b[3] = {0, 0, 0};
-    for (int g = 0; g < m.Gc; g++)
+    const int gc = ruleSize(t, m.s.intOrder);
+    for (int g = gc; g < m.Gc; g++) // a mixed mesh's smaller rules are padded with zero points and weights
+    {
+        m.cwj[(size_t)g * m.N + i] = 0;
+        for (int r = 0; r < 3; r++)
+            m.cpts[((size_t)g * 3 + r) * m.N + i] = 0;
+    }
+    for (int g = 0; g < gc; g++)
     {
         real xi[3], w, x[3], J[3][3];
         rulePoint(t, m.s.intOrder, g, xi, w);

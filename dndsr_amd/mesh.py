@@ -630,3 +630,104 @@ def bump_box_tet(n: int = 69, amp: float = 0.1, jitter: float = 0.1, seed: int =
 
     return tet_grid(n, n, n, 1.0, 1.0, 1.0, periodic=(False, False, True),
                     zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)
+
+
+QUAD4_FACES = np.array([[0, 1], [1, 2], [2, 3], [3, 0]])
+
+
+def mixed2d_mesh(coords: np.ndarray, cells: list, canon: np.ndarray | None = None, bnd_zone=None,
+                 shape: tuple = ()) -> Mesh:
+    """2-D mesh of Tri3 and Quad4 cells (cells: a list of node-id sequences of length 3
+    or 4, counter-clockwise).  Faces (Line2) are matched as in simplex_mesh: numbered by
+    first occurrence in (cell, element face) order, the back cell's edge orientation
+    kept (normal out of it), periodic copies identified through canon, a face seen
+    once is a boundary face with zone bnd_zone(face2node [nb, 2])."""
+    coords = np.ascontiguousarray(coords, np.float64)
+    nNode = len(coords)
+    ncn = np.array([len(c) for c in cells], np.int64)
+    if np.any((ncn != 3) & (ncn != 4)):
+        raise ValueError("cells must have 3 or 4 nodes")
+    N = len(cells)
+    c2nPtr = np.concatenate([[0], np.cumsum(ncn)])
+    c2n = np.concatenate([np.asarray(c, np.int64) for c in cells])
+    # signed area (shoelace) must be positive
+    x, y = coords[c2n, 0], coords[c2n, 1]
+    nxt = np.arange(len(c2n)) + 1
+    nxt[c2nPtr[1:] - 1] = c2nPtr[:-1]
+    area2 = np.add.reduceat(x * y[nxt] - x[nxt] * y, c2nPtr[:-1])
+    if np.any(area2 <= 0):
+        raise ValueError("cells must be counter-clockwise and non-degenerate")
+    # one occurrence per cell edge: node k -> node k+1 (Tri3 / Quad4 face tables)
+    occ = np.stack([c2n, c2n[nxt]], 1)
+    occ_cell = np.repeat(np.arange(N), ncn)
+    canon = np.arange(nNode, dtype=np.int64) if canon is None else np.asarray(canon, np.int64)
+    cn = np.sort(canon[occ], axis=1)
+    base = int(canon.max()) + 1
+    key = cn[:, 0] * base + cn[:, 1]
+    order = np.argsort(key, kind="stable")
+    ks = key[order]
+    start = np.r_[True, ks[1:] != ks[:-1]]
+    gid = np.cumsum(start) - 1
+    counts = np.bincount(gid)
+    if counts.max() > 2:
+        raise ValueError("a face is shared by more than two cells")
+    pos = np.flatnonzero(start)
+    first = order[pos]
+    second = np.where(counts == 2, order[np.minimum(pos + 1, len(order) - 1)], -1)
+    fo = np.argsort(first, kind="stable")
+    first, second = first[fo], second[fo]
+    F = len(first)
+    face_of_group = np.empty(F, np.int64)
+    face_of_group[fo] = np.arange(F)
+    occ_face = np.empty(len(occ), np.int64)
+    occ_face[order] = face_of_group[gid]
+    face2node = occ[first]
+    has2 = second >= 0
+    face2cell = np.stack([occ_cell[first], np.where(has2, occ_cell[np.maximum(second, 0)], -1)], 1)
+    faceShift = np.zeros((F, 3))
+    if has2.any():
+        a, b = occ[first[has2]], occ[second[has2]]
+        ia, ib = np.argmin(canon[a], axis=1), np.argmin(canon[b], axis=1)
+        r = np.arange(len(a))
+        faceShift[has2] = coords[b[r, ib]] - coords[a[r, ia]]
+    zone, nzone = None, 0
+    if (~has2).any():
+        if bnd_zone is None:
+            raise ValueError("the mesh has boundary faces: bnd_zone is required")
+        zone = np.full(F, -1, np.int32)
+        zone[~has2] = np.asarray(bnd_zone(face2node[~has2]), np.int32)
+        nzone = int(zone.max()) + 1
+    ctype = np.where(ncn == 3, TRI3, QUAD4).astype(np.int32)
+    return Mesh(2, coords, ctype, c2nPtr, c2n, c2nPtr.copy(), occ_face, np.full(F, LINE2, np.int32),
+                np.arange(F + 1, dtype=np.int64) * 2, face2node.ravel().astype(np.int64), face2cell.astype(np.int64),
+                faceShift, shape, zone, nzone)
+
+
+def mixed_grid(nx: int, ny: int, Lx: float = 1.0, Ly: float = 1.0, x0: float = 0.0, y0: float = 0.0,
+               periodic=(False, False), zones: dict | None = None, jitter: float = 0.0, seed: int = 0,
+               node_map=None, split=None) -> Mesh:
+    """nx x ny lattice of Quad4 cells in which the squares (i, j) with split(i, j) true
+    are cut into two Tri3 cells along their (i, j)-(i+1, j+1) diagonal (default: the
+    right half of the lattice); directions periodic or bounded as box_quad_grid."""
+    assert nx >= 2 and ny >= 2
+    per = [bool(p) for p in periodic]
+    hx, hy = Lx / nx, Ly / ny
+    I, J = np.meshgrid(np.arange(nx + 1), np.arange(ny + 1), indexing="ij")
+    X = np.stack([x0 + I * hx, y0 + J * hy, np.zeros_like(I, dtype=float)], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([hx, hy]), jitter, seed, per)
+    coords = X.reshape(-1, 3)
+    if node_map is not None:
+        coords = np.ascontiguousarray(node_map(coords.copy()), np.float64)
+    nid = lambda i, j: i * (ny + 1) + j  # noqa: E731
+    lat = np.stack([I.ravel(), J.ravel(), 0 * I.ravel()], 1)
+    canon = nid(I % nx if per[0] else I, J % ny if per[1] else J).ravel()
+    split = split if split is not None else (lambda i, j: i >= nx // 2)
+    cells = []
+    for j in range(ny):
+        for i in range(nx):
+            a, b, c, d = nid(i, j), nid(i + 1, j), nid(i + 1, j + 1), nid(i, j + 1)
+            if split(i, j):
+                cells += [(a, b, c), (a, c, d)]
+            else:
+                cells.append((a, b, c, d))
+    return mixed2d_mesh(coords, cells, canon, _lattice_zone(lat, (nx, ny), per, zones), (nx, ny))

@@ -820,7 +820,7 @@ struct VR
     std::vector<Pt> bary, cent, hbox;
     std::vector<std::vector<Pt>> cellQP;
     std::vector<std::vector<real>> cellJD; // JDet + verySmall
-    std::vector<real> cellQw;              // rule weights (same rule for all cells of a type)
+    std::vector<std::vector<real>> cellQw; // rule weights per cell (its type's rule)
     std::vector<std::vector<real>> moment; // [nCell][NDOF]
     // face metrics
     std::vector<real> area, faceL;
@@ -988,10 +988,10 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
     int dim = v.dim;
     v.cellType.assign(m->cellType, m->cellType + m->nCell);
     v.faceType.assign(m->faceType, m->faceType + m->nFace);
-    // one rule per entity kind: a single cell type and a single face type per mesh
+    // one face rule per mesh (a single face type); cells may mix types of one dimension
     for (auto t : v.cellType)
-        if (t != v.cellType[0])
-            throw std::runtime_error("mixed cell types are not supported");
+        if (elemParamDim(t) != elemParamDim(v.cellType[0]))
+            throw std::runtime_error("cells of different dimensions");
     for (auto t : v.faceType)
         if (t != v.faceType[0])
             throw std::runtime_error("mixed face types are not supported");
@@ -1033,6 +1033,7 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
     v.hbox.resize(v.nCell);
     v.cellQP.resize(v.nCell);
     v.cellJD.resize(v.nCell);
+    v.cellQw.resize(v.nCell);
     v.globalVol = 0;
     for (idx_t i = 0; i < v.nCell; i++)
     {
@@ -1048,7 +1049,7 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
         v.vol[i] = vv;
         v.globalVol += vv;
         auto qc = quadRule(t, s->intOrder);
-        v.cellQw.clear();
+        v.cellQw[i].clear();
         Pt b;
         for (auto &q : qc)
         {
@@ -1056,7 +1057,7 @@ static void buildVR(VR &v, const orc_mesh_desc *m, const orc_vr_settings *s)
             real jd = cellJDet(dim, g);
             v.cellJD[i].push_back(jd + verySmallReal);
             v.cellQP[i].push_back(toPt(g.x));
-            v.cellQw.push_back(q.w);
+            v.cellQw[i].push_back(q.w);
             for (int d = 0; d < 3; d++)
                 b.x[d] += (g.x[d] * jd) * q.w;
         }
@@ -1624,7 +1625,10 @@ int orc_vr_info(const orc_vr *h, int64_t *nCell, int64_t *nFace, int *M, int *ma
 {
     const VR &v = h->v;
     *nCell = v.nCell, *nFace = v.nFace, *M = v.M, *maxFaceCell = v.maxFC;
-    *nGFace = (int)v.faceQw.size(), *nGCell = (int)v.cellQw.size(), *globalVol = v.globalVol;
+    size_t gc = 0;
+    for (auto &w : v.cellQw)
+        gc = std::max(gc, w.size());
+    *nGFace = (int)v.faceQw.size(), *nGCell = (int)gc, *globalVol = v.globalVol;
     return 0;
 }
 
@@ -1673,13 +1677,17 @@ int orc_get_face_metrics(const orc_vr *h, double *area, double *meanNorm, double
 int orc_get_cell_quad(const orc_vr *h, double *pts, double *wJDet)
 {
     const VR &v = h->v;
-    size_t G = v.cellQw.size();
+    size_t G = 0;
+    for (auto &w : v.cellQw)
+        G = std::max(G, w.size());
+    // a cell with a smaller rule than the largest is padded with zero points and weights
     for (idx_t i = 0; i < v.nCell; i++)
         for (size_t g = 0; g < G; g++)
         {
+            const bool in = g < v.cellQw[i].size();
             for (int d = 0; d < 3; d++)
-                pts[(i * G + g) * 3 + d] = v.cellQP[i][g].x[d];
-            wJDet[i * G + g] = v.cellJD[i][g] * v.cellQw[g];
+                pts[(i * G + g) * 3 + d] = in ? v.cellQP[i][g].x[d] : 0.0;
+            wJDet[i * G + g] = in ? v.cellJD[i][g] * v.cellQw[i][g] : 0.0;
         }
     return 0;
 }
@@ -1701,11 +1709,11 @@ int orc_cell_average(const orc_vr *h, int fieldId, const double *params, int nVa
     for (idx_t i = 0; i < v.nCell; i++)
     {
         std::vector<real> acc(nVars, 0.0);
-        for (size_t g = 0; g < v.cellQw.size(); g++)
+        for (size_t g = 0; g < v.cellQw[i].size(); g++)
         {
             evalField(fieldId, params, v.cellQP[i][g], nVars, val.data());
             for (int k = 0; k < nVars; k++)
-                acc[k] += (val[k] * v.cellJD[i][g]) * v.cellQw[g];
+                acc[k] += (val[k] * v.cellJD[i][g]) * v.cellQw[i][g];
         }
         for (int k = 0; k < nVars; k++)
             u[i * nVars + k] = acc[k] / v.vol[i];
@@ -1725,7 +1733,7 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
     for (idx_t i = 0; i < v.nCell; i++)
     {
         real ec = 0;
-        for (size_t g = 0; g < v.cellQw.size(); g++)
+        for (size_t g = 0; g < v.cellQw[i].size(); g++)
         {
             v.diffBase(i, v.cellQP[i][g], 1, 0, D);
             real r = 0;
@@ -1733,7 +1741,7 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
                 r += D[1 + k] * uRec[(size_t)i * M + k];
             real ex;
             evalField(fieldId, params, v.cellQP[i][g], 1, &ex);
-            ec += (std::abs(r + u[i] - ex) * v.cellJD[i][g]) * v.cellQw[g];
+            ec += (std::abs(r + u[i] - ex) * v.cellJD[i][g]) * v.cellQw[i][g];
         }
         err += ec;
     }

@@ -1,4 +1,4 @@
-"""Tri3 / Tet4 meshes on the device vs the CPU oracle (FP64, relative L2 <= 1e-12 unless
+"""Tri3 / Tet4 and mixed Tri3 + Quad4 meshes on the device vs the CPU oracle (FP64, relative L2 <= 1e-12 unless
 noted): metrics and VR coefficients, the VR sweep, EvaluateDt, EvaluateRHS (inviscid and
 viscous, several Riemann solvers), LU-SGS, SSP-RK3 and implicit steps on periodic and
 bounded triangle and tetrahedron meshes (jittered lattices)."""
@@ -30,8 +30,13 @@ def _mesh(kind):
         return mm.tri_grid(7, 6, 3.5, 3.0, periodic=(True, True), jitter=0.15, seed=2), None, None
     if kind == "tet-periodic":
         return mm.tet_grid(4, 3, 3, 1.0, 0.8, 0.9, periodic=(True, True, True), jitter=0.12, seed=3), None, None
+    if kind == "mixed-periodic":
+        return mm.mixed_grid(8, 6, 3.5, 3.0, periodic=(True, True), jitter=0.15, seed=2), None, None
     if kind == "tri":
         m = mm.tri_grid(7, 6, 3.5, 3.0, zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=0.12, seed=2)
+        return m, [bc.BC_FAR, bc.BC_OUT, bc.BC_WALL, bc.BC_SYM], vals
+    if kind == "mixed":
+        m = mm.mixed_grid(8, 6, 3.5, 3.0, zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=0.12, seed=2)
         return m, [bc.BC_FAR, bc.BC_OUT, bc.BC_WALL, bc.BC_SYM], vals
     m = mm.tet_grid(4, 3, 3, 1.0, 0.8, 0.6, periodic=(False, True, False), zones=dict(xmin=0, xmax=1, zmin=2, zmax=3),
                     jitter=0.1, seed=3)
@@ -39,7 +44,7 @@ def _mesh(kind):
 
 
 CASES = [("tri-periodic", 1, 3), ("tri-periodic", 3, 5), ("tri", 2, 4), ("tet-periodic", 1, 3), ("tet-periodic", 2, 4),
-         ("tet", 1, 3), ("tet", 2, 3)]
+         ("tet", 1, 3), ("tet", 2, 3), ("mixed-periodic", 1, 3), ("mixed-periodic", 3, 5), ("mixed", 2, 4)]
 
 
 @pytest.fixture(scope="module", params=CASES, ids=[f"{k}-p{p}-i{i}" for k, p, i in CASES])
@@ -146,7 +151,8 @@ def test_lusgs(pair):
     assert rel_l2(g.to_host(dNew), r1) <= TOL
 
 
-@pytest.mark.parametrize("kind,order,viscous", [("tri", 1, False), ("tet", 1, True), ("tet-periodic", 2, False)])
+@pytest.mark.parametrize("kind,order,viscous", [("tri", 1, False), ("tet", 1, True), ("tet-periodic", 2, False),
+                                                ("mixed", 2, True)])
 def test_time_loops(require_gpu, kind, order, viscous):
     import pyoracle as po
     from dndsr_amd import bc

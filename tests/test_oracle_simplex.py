@@ -1,4 +1,4 @@
-"""CPU pins of the oracle on Tri3 / Tet4 meshes (Hammer rules, src/Geom/Quadratures/Tri.hpp,
+"""CPU pins of the oracle on Tri3 / Tet4 and mixed Tri3 + Quad4 meshes (Hammer rules, src/Geom/Quadratures/Tri.hpp,
 Tet.hpp; rule choice QuadratureHub.hpp:57-108; shape functions Elements/Tri3.hpp, Tet4.hpp).
 
 Pinned by what the mathematics fixes:
@@ -76,15 +76,16 @@ def _meshes():
         "tri": mm.tri_grid(5, 4, 2.0, 1.5, zones=dict(xmin=0, xmax=0, ymin=0, ymax=0), jitter=0.15, seed=3),
         "tet": mm.tet_grid(3, 3, 2, 1.0, 1.2, 0.8, zones=dict(xmin=0, xmax=0, ymin=0, ymax=0, zmin=0, zmax=0),
                            jitter=0.12, seed=4),
+        "mixed": mm.mixed_grid(5, 4, 2.0, 1.5, zones=dict(xmin=0, xmax=0, ymin=0, ymax=0), jitter=0.15, seed=3),
     }
 
 
-@pytest.mark.parametrize("kind", ["tri", "tet"])
+@pytest.mark.parametrize("kind", ["tri", "tet", "mixed"])
 def test_metrics_close_by_divergence_theorem(kind):
     m = _meshes()[kind]
     o = po.OracleVR(m, maxOrder=1, intOrder=3)
     cm, fm = o.cell_metrics(), o.face_metrics()
-    box = 2.0 * 1.5 if kind == "tri" else 1.0 * 1.2 * 0.8
+    box = 1.0 * 1.2 * 0.8 if kind == "tet" else 2.0 * 1.5
     assert abs(cm["vol"].sum() - box) <= 1e-13
     dim = m.dim
     N = m.nCell
@@ -113,7 +114,8 @@ FIELDS = {
 }
 
 
-@pytest.mark.parametrize("kind,order", [("tri", 1), ("tri", 2), ("tri", 3), ("tet", 1), ("tet", 2)])
+@pytest.mark.parametrize("kind,order", [("tri", 1), ("tri", 2), ("tri", 3), ("tet", 1), ("tet", 2), ("mixed", 1),
+                                        ("mixed", 2), ("mixed", 3)])
 def test_vr_dirichlet_polynomial_exactness(kind, order):
     m = _meshes()[kind]
     o = po.OracleVR(m, maxOrder=order, intOrder=max(order + 2, 3),
@@ -139,7 +141,7 @@ def _cons(rho, v, p):
     return np.array([rho, *(rho * v), p / (GAMMA - 1) + 0.5 * rho * v @ v])
 
 
-@pytest.mark.parametrize("kind", ["tri", "tet"])
+@pytest.mark.parametrize("kind", ["tri", "tet", "mixed"])
 @pytest.mark.parametrize("rs", ["Roe", "HLLEP"])
 def test_free_stream_and_conservation(kind, rs):
     m = _meshes()[kind]
@@ -154,6 +156,8 @@ def test_free_stream_and_conservation(kind, rs):
     # conservation on a periodic mesh
     if kind == "tri":
         mp = mm.tri_grid(5, 4, 2.0, 1.5, periodic=(True, True), jitter=0.15, seed=3)
+    elif kind == "mixed":
+        mp = mm.mixed_grid(6, 4, 2.0, 1.5, periodic=(True, True), jitter=0.15, seed=3)
     else:
         mp = mm.tet_grid(3, 3, 2, periodic=(True, True, True), jitter=0.12, seed=4)
     op = po.OracleVR(mp, maxOrder=2, intOrder=4)
