@@ -31,12 +31,15 @@ def rhs_vjp(ev, u, uRec, deltaLambdaCell, rhsBar, need_rec=True):
     return uBar, recBar
 
 
-def vr_vjp(vfv, recNewBar, mode=0, relax=-1.0):
-    """Transpose of one VR sweep: returns (uRecBar, uBar) (uBar is None for mode 3 = IterDiff)."""
+def vr_vjp(vfv, recNewBar, mode=0, relax=-1.0, uRec=None, u=None):
+    """Transpose of one VR sweep: returns (uRecBar, uBar) (uBar is None for mode 3 = IterDiff).
+    uRec, u: the state the forward sweep read (needed on meshes with boundary faces)."""
     nVars = recNewBar.shape[0]
     recBar = torch.empty_like(recNewBar)
     uBar = None if mode == 3 else torch.empty((nVars, vfv.nCell), dtype=recNewBar.dtype, device=recNewBar.device)
-    check(lib().dndsr_vr_iter_vjp(vfv.ctx.h, int(nVars), int(mode), float(relax), _ptr(recNewBar.contiguous()),
+    check(lib().dndsr_vr_iter_vjp(vfv.ctx.h, int(nVars), int(mode), float(relax),
+                                  None if uRec is None else _ptr(uRec.contiguous()),
+                                  None if u is None else _ptr(u.contiguous()), _ptr(recNewBar.contiguous()),
                                   _ptr(recBar), _ptr(uBar)))
     return recBar, uBar
 
@@ -72,11 +75,14 @@ class VRIterFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, uRec, u, vfv, relax=-1.0):
         ctx.vfv, ctx.relax = vfv, relax
-        return _vr_sweep(vfv, uRec.contiguous(), u.contiguous(), relax)
+        uRec, u = uRec.contiguous(), u.contiguous()
+        ctx.save_for_backward(uRec, u)
+        return _vr_sweep(vfv, uRec, u, relax)
 
     @staticmethod
     def backward(ctx, recNewBar):
-        recBar, uBar = vr_vjp(ctx.vfv, recNewBar, 0, ctx.relax)
+        uRec, u = ctx.saved_tensors
+        recBar, uBar = vr_vjp(ctx.vfv, recNewBar, 0, ctx.relax, uRec, u)
         return recBar, uBar, None, None
 
 
@@ -101,8 +107,8 @@ class ResidualFunction(torch.autograd.Function):
         for _ in range(ctx.nRec):
             recs.append(_vr_sweep(ev.vfv, recs[-1], u, -1.0))
         uBar, recBar = rhs_vjp(ev, u, recs[-1], dlc, rhsBar)
-        for _ in range(ctx.nRec):
-            recBar, ub = vr_vjp(ev.vfv, recBar, 0, -1.0)
+        for k in range(ctx.nRec - 1, -1, -1):
+            recBar, ub = vr_vjp(ev.vfv, recBar, 0, -1.0, recs[k], u)
             uBar = uBar + ub
         return uBar, recBar, None, None, None
 

@@ -3,8 +3,10 @@
 //
 // The reference has no adjoint; these kernels differentiate exactly the discrete
 // operators of vr.hip / euler.hip:
-//   EvaluateRHS:  rhs_i = sum_{f in i} s_if / V_i sum_g (-w_g F(U_L(x_g), U_R(x_g), uL, uR, n_g))
-//                 with U_side(x) = u_side + phi_side(x) . uRec_side and deltaLambdaCell frozen;
+//   EvaluateRHS:  rhs_i = sum_{f in i} s_if / V_i sum_g w_g (-F(U_L, U_R, uL, uR, n_g) + Fv(U_L, U_R, G_L, G_R, n_g))
+//                 with U_side(x) = u_side + phi_side(x) . uRec_side, G_side(x) = D1phi_side(x) . uRec_side,
+//                 deltaLambdaCell frozen; a boundary face takes U_R = ghost(U_L), u_R = ghost(u_L) (at
+//                 the mean normal), G_R = G_L;
 //   VR sweep:     y_i = c0 x_i + c1 sum_k [A^-1B_{i,k} x_j + A^-1b_{i,k} (u_j - u_i)].
 // Flux derivatives come from the templated Riemann solvers evaluated on dual
 // numbers (one pass per input direction), so every branch of the flux is
@@ -39,6 +41,79 @@ __device__ inline void cellBasisAt(const DevMesh &m, idx c, const real *x, const
     basisRow0<DIM, NDOF>(xs, mom, phi);
 }
 
+// zero-mean basis and its gradient rows of cell c at x (+ shift), as the viscous pass
+// evaluates them (scaled coordinates divided by the cell scale)
+template <int DIM, int NDOF>
+__device__ inline void cellBasisGradAt(const DevMesh &m, idx c, const real *x, const real *shift, real *phi,
+                                       real (*dphi)[NDOF > 1 ? NDOF - 1 : 1])
+{
+    constexpr int M = NDOF - 1;
+    const size_t N = m.N;
+    const real s = m.scale[c];
+    real xs[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        xs[r] = (x[r] + (shift ? shift[r] : 0.0)) - m.cent[(size_t)r * N + c];
+    xs[0] /= s, xs[1] /= s;
+    if (DIM == 3)
+        xs[2] /= s;
+#pragma unroll
+    for (int j = 0; j < M; j++)
+    {
+        phi[j] = basisDiff(DIM, 0, j + 1, xs, s) - m.moment[(size_t)j * N + c];
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            dphi[d][j] = basisDiff(DIM, 1 + d, j + 1, xs, s);
+    }
+}
+
+// value and gradient of cell c's reconstruction at x (+ shift), as the viscous pass
+template <int DIM, int NDOF>
+__device__ inline void cellValGradAt(const DevMesh &m, idx c, const real *x, const real *shift, const real *u,
+                                     const real *uRec, real *U, real (*G)[5])
+{
+    constexpr int M = NDOF - 1;
+    const size_t N = m.N;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        U[v] = u[(size_t)v * N + c];
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            G[d][v] = 0;
+    }
+    if (!uRec)
+        return;
+    real phi[M > 0 ? M : 1], dphi[DIM][M > 0 ? M : 1];
+    cellBasisGradAt<DIM, NDOF>(m, c, x, shift, phi, dphi);
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        real a = 0, g[DIM];
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            g[d] = 0;
+#pragma unroll
+        for (int j = 0; j < M; j++)
+        {
+            const real r = uRec[(size_t)(v * M + j) * N + c];
+            a += phi[j] * r;
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                g[d] += dphi[d][j] * r;
+        }
+        U[v] += a;
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            G[d][v] = g[d];
+    }
+}
+
+// Point-sensitivity rows (of the buffer pbuf[row][G F]): 0-4 d/dU_L, 5-9 d/dU_R, 10-14
+// d/du_L (cell mean), 15-19 d/du_R, and with viscous fluxes 20-34 d/dG_L[d][v]
+// (row 20 + 5 d + v), 35-49 d/dG_R.
+constexpr int kAdjRowsInv = 20, kAdjRowsVis = 50;
+
 // One thread per (face, point), point-major: t = g F + f.  Writes the 20 sensitivities
 // dF.Fbar / d(UL, UR, uLmean, uRmean) of the point to pbuf[20][G F].
 template <int DIM, int NDOF, int RS, bool BATCH>
@@ -54,6 +129,8 @@ __global__ void __launch_bounds__(128) kRHSPointAdj(DevMesh m, GasParams gp, con
     const idx f = (idx)(t % F);
     const int g = (int)(t / F);
     const idx cL = m.f2c[f], cR = m.f2c[F + f];
+    if (cR < 0) // boundary faces: kRHSBndPointAdj
+        return;
     real x[3], n[3], sh[3];
 #pragma unroll
     for (int r = 0; r < 3; r++)
@@ -129,11 +206,171 @@ __global__ void __launch_bounds__(128) kRHSPointAdj(DevMesh m, GasParams gp, con
     }
 }
 
+// Viscous part of an interior face point: adds d(Fv . Fvbar)/dU_L,R to rows 0-9 and
+// writes the gradient rows 20-49.  Fvbar = -Fbar (the face flux gains +Fv w).
+template <int DIM, int NDOF>
+__global__ void __launch_bounds__(128) kRHSPointAdjVis(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                                       const real *__restrict__ uRec, const real *__restrict__ rbar,
+                                                       real *__restrict__ pbuf)
+{
+    const size_t N = m.N, F = m.F, GF = (size_t)m.G * F;
+    const size_t t = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= GF)
+        return;
+    const idx f = (idx)(t % F);
+    const int g = (int)(t / F);
+    const idx cL = m.f2c[f], cR = m.f2c[F + f];
+    if (cR < 0)
+        return;
+    real x[3], n[3], sh[3], dB[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+        x[r] = m.fpts[((size_t)g * 3 + r) * F + f];
+        n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+        sh[r] = m.fshift[(size_t)r * F + f];
+        dB[r] = (m.bary[(size_t)r * N + cR] - sh[r]) - m.bary[(size_t)r * N + cL];
+    }
+    const real d0 = fmin(m.vol[cL], m.vol[cR]) / m.farea[f] * 2;
+    const real wj = m.fwj[(size_t)g * F + f];
+    real Fvb[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        Fvb[v] = wj * (rbar[(size_t)v * N + cL] / m.vol[cL] - rbar[(size_t)v * N + cR] / m.vol[cR]);
+    real UL[5], UR[5], GL[3][5], GR[3][5];
+    cellValGradAt<DIM, NDOF>(m, cL, x, nullptr, u, uRec, UL, GL);
+    cellValGradAt<DIM, NDOF>(m, cR, x, sh, u, uRec, UR, GR);
+#pragma unroll 1
+    for (int d = 0; d < 40; d++)
+    {
+        Dual DL[5], DR[5], DGL[3][5], DGR[3][5], DF[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            DL[v] = Dual(UL[v], d == v ? 1.0 : 0.0);
+            DR[v] = Dual(UR[v], d == 5 + v ? 1.0 : 0.0);
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+            {
+                DGL[q][v] = Dual(GL[q][v], d == 10 + 5 * q + v ? 1.0 : 0.0);
+                DGR[q][v] = Dual(GR[q][v], d == 25 + 5 * q + v ? 1.0 : 0.0);
+            }
+        }
+        viscousPointFlux(gp, DL, DR, DGL, DGR, n, dB, d0, false, false, DF);
+        real sv = 0;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            sv += Fvb[v] * DF[v].d;
+        const int row = d < 10 ? d : 20 + (d - 10);
+        if (d < 10)
+            pbuf[(size_t)row * GF + t] += sv;
+        else
+            pbuf[(size_t)row * GF + t] = sv;
+    }
+}
+
+// One thread per (boundary face, point): the sensitivities of the inviscid flux against
+// the ghost states (and of the viscous flux, vis) to the interior point value, cell mean
+// and gradient; the front-side rows are zero.
+template <int DIM, int NDOF, int RS, bool BATCH>
+__global__ void __launch_bounds__(128) kRHSBndPointAdj(DevMesh m, GasParams gp, const real *__restrict__ u,
+                                                       const real *__restrict__ uRec, const real *__restrict__ dlc,
+                                                       const real *__restrict__ rbar, real *__restrict__ pbuf, int vis)
+{
+    const size_t N = m.N, F = m.F, GF = (size_t)m.G * F;
+    const size_t tb = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (tb >= (size_t)m.nBnd * m.G)
+        return;
+    const idx f = m.bndFace[tb % m.nBnd];
+    const int g = (int)(tb / m.nBnd);
+    const size_t t = (size_t)g * F + f;
+    const idx cL = m.f2c[f];
+    const int zone = m.fzone[f], type = m.bcType[zone];
+    const real *far = m.bcVal + (size_t)zone * 5;
+    real x[3], n[3], nm[3], dB[3];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+        x[r] = m.fpts[((size_t)g * 3 + r) * F + f];
+        n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+        nm[r] = m.fmeanN[(size_t)r * F + f];
+        dB[r] = m.fcent[(size_t)r * F + f] - m.bary[(size_t)r * N + cL];
+    }
+    const real d0 = m.vol[cL] / m.farea[f] * 2;
+    const real wj = m.fwj[(size_t)g * F + f];
+    const real dLambda = dlc ? dlc[cL] : 0.0;
+    real Fbar[5], uLm[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        Fbar[v] = -wj * (rbar[(size_t)v * N + cL] / m.vol[cL]);
+        uLm[v] = u[(size_t)v * N + cL];
+    }
+    real UL[5], GL[3][5];
+    cellValGradAt<DIM, NDOF>(m, cL, x, nullptr, u, uRec, UL, GL);
+    const bool visFace =
+        vis && !(type == BC_WALL_INVIS || type == BC_SYM || type == BC_FAR || type == BC_OUT);
+    const int nDir = vis ? 25 : 10;
+#pragma unroll 1
+    for (int d = 0; d < nDir; d++)
+    {
+        Dual DL[5], DLm[5], DR[5], DRm[5], DF[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            DL[v] = Dual(UL[v], d == v ? 1.0 : 0.0);
+            DLm[v] = Dual(uLm[v], d == 5 + v ? 1.0 : 0.0);
+        }
+        bcGhost(type, DLm, nm, far, gp.gamma, DRm);
+        bcGhost(type, DL, n, far, gp.gamma, DR);
+        if constexpr (BATCH)
+        {
+            RoeBatchFaceT<Dual> bf;
+            roeBatchSetup<RS>(DLm, DRm, nm, gp, dLambda, bf);
+            roeBatchPoint<RS>(DL, DR, n, gp, bf, DF);
+        }
+        else
+        {
+            Dual l0, l1, l4;
+            riemannFlux<RS>(DL, DR, DLm, DRm, n, gp, dLambda, DF, l0, l1, l4);
+        }
+        real sv = 0;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            sv += Fbar[v] * DF[v].d;
+        if (visFace)
+        {
+            Dual DGL[3][5], DV[5];
+#pragma unroll
+            for (int q = 0; q < 3; q++)
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                    DGL[q][v] = Dual(GL[q][v], d == 10 + 5 * q + v ? 1.0 : 0.0);
+            viscousPointFlux(gp, DL, DR, DGL, DGL, n, dB, d0, true, type == BC_WALL, DV);
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sv -= Fbar[v] * DV[v].d;
+        }
+        const int row = d < 5 ? d : (d < 10 ? 10 + (d - 5) : 20 + (d - 10));
+        pbuf[(size_t)row * GF + t] = sv;
+    }
+    // the front side has no state of its own
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        pbuf[(size_t)(5 + v) * GF + t] = 0;
+        pbuf[(size_t)(15 + v) * GF + t] = 0;
+    }
+    if (vis)
+        for (int q = 0; q < 15; q++)
+            pbuf[(size_t)(35 + q) * GF + t] = 0;
+}
+
 // One thread per cell: collect the point sensitivities of the cell's faces into
 // uBar (N x 5) and uRecBar (5 M x N), evaluating the cell's basis at each point.
 template <int DIM, int NDOF>
 __global__ void __launch_bounds__(128) kRHSGatherAdj(DevMesh m, const real *__restrict__ pbuf,
-                                                     real *__restrict__ ubar, real *__restrict__ urecbar)
+                                                     real *__restrict__ ubar, real *__restrict__ urecbar, int vis)
 {
     constexpr int M = NDOF - 1;
     const idx i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -152,7 +389,7 @@ __global__ void __launch_bounds__(128) kRHSGatherAdj(DevMesh m, const real *__re
         if (f < 0)
             continue;
         const bool back = m.f2c[f] == i;
-        const int rowU = back ? 0 : 5, rowM = back ? 10 : 15;
+        const int rowU = back ? 0 : 5, rowM = back ? 10 : 15, rowG = back ? 20 : 35;
         real sh[3];
 #pragma unroll
         for (int r = 0; r < 3; r++)
@@ -179,6 +416,21 @@ __global__ void __launch_bounds__(128) kRHSGatherAdj(DevMesh m, const real *__re
 #pragma unroll
                     for (int j = 0; j < M; j++)
                         ar[v][j] += phi[j] * gU[v];
+                if (vis)
+                {
+                    real phi2[M > 0 ? M : 1], dphi[DIM][M > 0 ? M : 1];
+                    cellBasisGradAt<DIM, NDOF>(m, i, x, back ? nullptr : sh, phi2, dphi);
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+#pragma unroll
+                        for (int v = 0; v < 5; v++)
+                        {
+                            const real gg = pbuf[(size_t)(rowG + 5 * d + v) * GF + t];
+#pragma unroll
+                            for (int j = 0; j < M; j++)
+                                ar[v][j] += dphi[d][j] * gg;
+                        }
+                }
             }
         }
     }
@@ -198,10 +450,25 @@ static void rhsAdjLaunch(Ctx &c, const GasParams &gp, const real *u, const real 
                          const real *rbar, real *ubar, real *urecbar)
 {
     const size_t GF = (size_t)c.m.G * c.m.F;
+    const int vis = gp.viscous ? 1 : 0;
     kRHSPointAdj<DIM, NDOF, RS, BATCH><<<(unsigned)((GF + 127) / 128), 128, 0, c.stream>>>(c.m, gp, u, uRec, dlc,
                                                                                           rbar, c.adjPoint);
     DNDSR_HIP_CHECK(hipGetLastError());
-    kRHSGatherAdj<DIM, NDOF><<<gridFor(c.m.N, 128), 128, 0, c.stream>>>(c.m, c.adjPoint, ubar, urecbar);
+    if (vis)
+    {
+        kRHSPointAdjVis<DIM, NDOF><<<(unsigned)((GF + 127) / 128), 128, 0, c.stream>>>(c.m, gp, u, uRec, rbar,
+                                                                                      c.adjPoint);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
+    if (c.m.nBnd > 0)
+    {
+        const size_t nb = (size_t)c.m.nBnd * c.m.G;
+        kRHSBndPointAdj<DIM, NDOF, RS, BATCH><<<(unsigned)((nb + 127) / 128), 128, 0, c.stream>>>(
+            c.m, gp, u, uRec, dlc, rbar, c.adjPoint, vis);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
+    kRHSGatherAdj<DIM, NDOF><<<gridFor(c.m.N, 128), 128, 0, c.stream>>>(c.m, c.adjPoint, ubar, urecbar,
+                                                                        vis && uRec != nullptr);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -351,6 +618,129 @@ __global__ void __launch_bounds__(128) kVRIterT(DevMesh m, const real *__restric
             ubar[(size_t)v * N + jc] = au[v];
 }
 
+// Transposed boundary term of the VR sweep (forward: uRecNew_i += scale A_i^-1 BCC_i,
+// BCC_i = sum_{boundary faces, g} phi(x_g) (w0^2 (u_B(x_g) - u_i)) w_g JDet_g, u_B the ghost
+// of the compressed point state u_i + phi . uRec_i, or the given point value).  One
+// thread per boundary cell; BCCbar = scale A^-T ybar, then the derivatives of BCC to the
+// cell's mean and coefficients from dual-number passes (point values given: to the mean
+// only), added to ubar / xbar.
+template <int DIM, int NDOF, int NV>
+__global__ void __launch_bounds__(64) kVRBoundaryT(DevMesh m, const real *__restrict__ uRec,
+                                                   const real *__restrict__ u, const real *__restrict__ ybar,
+                                                   real scale, real *__restrict__ xbar, real *__restrict__ ubar)
+{
+    constexpr int M = NDOF - 1;
+    const idx t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= m.nBndCell)
+        return;
+    const idx i = m.bndCell[t];
+    const size_t N = m.N, F = m.F;
+    real Bb[M][NV];
+#pragma unroll
+    for (int j = 0; j < M; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+        {
+            real a = 0;
+#pragma unroll
+            for (int r = 0; r < M; r++)
+                a += m.AinvB[((size_t)r * M + j) * N + i] * ybar[((size_t)v * M + r) * N + i];
+            Bb[j][v] = scale * a;
+        }
+    real mom[M], cen[3], um[NV];
+#pragma unroll
+    for (int j = 0; j < M; j++)
+        mom[j] = m.moment[(size_t)j * N + i];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        cen[r] = m.cent[(size_t)r * N + i];
+    const real isc = m.iscale[i];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        um[v] = u[(size_t)v * N + i];
+    const bool euler = NV == 5 && m.bcType != nullptr;
+    const int nDir = euler ? NV + NV * M : NV;
+#pragma unroll 1
+    for (int d = 0; d < nDir; d++)
+    {
+        real total = 0;
+        for (int k = 0; k < m.K; k++)
+        {
+            const idx f = m.c2f[(size_t)k * N + i];
+            if (f < 0 || m.c2c[(size_t)k * N + i] >= 0)
+                continue;
+            const int zone = m.fzone[f];
+            const real w0 = m.faceW[f];
+            const real w2 = w0 * w0;
+            for (int g = 0; g < m.G; g++)
+            {
+                real xs[3], phi[M];
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                {
+                    xs[r] = m.fpts[((size_t)g * 3 + r) * F + f] - cen[r];
+                    if (DIM == 3 || r < 2)
+                        xs[r] *= isc;
+                }
+                basisRow0<DIM, NDOF>(xs, mom, phi);
+                const real fac = m.fwj[(size_t)g * F + f];
+                real incd[NV]; // d inc / d (direction d)
+                if (euler)
+                {
+                    Dual umD[5], uBL[5], ULf[5], uB[5];
+                    real n[3];
+#pragma unroll
+                    for (int v = 0; v < 5; v++)
+                    {
+                        umD[v] = Dual(um[v], d == v ? 1.0 : 0.0);
+                        Dual r = 0;
+#pragma unroll
+                        for (int j = 0; j < M; j++)
+                            r += phi[j] * Dual(uRec[((size_t)v * M + j) * N + i], d == NV + v * M + j ? 1.0 : 0.0);
+                        uBL[v] = r;
+                    }
+#pragma unroll
+                    for (int r = 0; r < 3; r++)
+                        n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+                    compressRecPart(umD, uBL, ULf);
+                    bcGhost(m.bcType[zone], ULf, n, m.bcVal + (size_t)zone * 5, m.bcGamma, uB);
+#pragma unroll
+                    for (int v = 0; v < NV; v++)
+                        incd[v] = (uB[v] - umD[v]).d;
+                }
+                else
+#pragma unroll
+                    for (int v = 0; v < NV; v++)
+                        incd[v] = d == v ? -1.0 : 0.0; // given point values: inc = value - u_i
+#pragma unroll
+                for (int j = 0; j < M; j++)
+#pragma unroll
+                    for (int v = 0; v < NV; v++)
+                        total += Bb[j][v] * (((phi[j] * incd[v]) * w2) * fac);
+            }
+        }
+        if (d < NV)
+        {
+            if (ubar)
+                ubar[(size_t)d * N + i] += total;
+        }
+        else if (xbar)
+            xbar[(size_t)(d - NV) * N + i] += total;
+    }
+}
+
+template <int DIM, int NDOF>
+static void vrBndTLaunch(Ctx &c, int nVars, const real *uRec, const real *u, const real *ybar, real scale, real *xbar,
+                         real *ubar)
+{
+    const int g = gridFor(c.m.nBndCell, 64);
+    if (nVars == 5)
+        kVRBoundaryT<DIM, NDOF, 5><<<g, 64, 0, c.stream>>>(c.m, uRec, u, ybar, scale, xbar, ubar);
+    else
+        kVRBoundaryT<DIM, NDOF, 1><<<g, 64, 0, c.stream>>>(c.m, uRec, u, ybar, scale, xbar, ubar);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
 template <int M>
 static void vrTLaunch(Ctx &c, int nVars, const real *ybar, real *xbar, real *ubar, real c0, real c1, int useB,
                       int useb)
@@ -365,7 +755,8 @@ static void vrTLaunch(Ctx &c, int nVars, const real *ybar, real *xbar, real *uba
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
-void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *ybar, real *xbar, real *ubar)
+void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *uRec, const real *u, const real *ybar,
+                     real *xbar, real *ubar)
 {
     // mode 0 Jacobi (relax), 1 recordInc, 2 recordInc + uRecIsZero, 3 IterDiff
     real c0 = 1, c1 = -1;
@@ -385,6 +776,36 @@ void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *ybar, 
     case 9: vrTLaunch<9>(c, nVars, ybar, xbar, ubar, c0, c1, useB, useb); break;
     case 19: vrTLaunch<19>(c, nVars, ybar, xbar, ubar, c0, c1, useB, useb); break;
     default: throw std::runtime_error("unsupported number of reconstruction modes");
+    }
+    // boundary term (Jacobi: relax A^-1 BCC; recordInc: -A^-1 BCC; none with uRecIsZero)
+    const DevMesh &m = c.m;
+    if (m.nBnd == 0 || mode == 2)
+        return;
+    if (mode == 3)
+        throw std::runtime_error("the transposed VR Jacobian (IterDiff) covers meshes without boundary faces");
+    const bool euler = nVars == 5 && m.bcType != nullptr;
+    if (!euler && !(m.dirVal && m.dirNV == nVars))
+        throw std::runtime_error("the mesh has boundary faces: set the boundary conditions first");
+    if (!u || (euler && !uRec))
+        throw std::runtime_error("the transposed VR sweep on a mesh with boundary faces needs the base state uRec, u");
+    const real scale = mode == 0 ? relax : -1.0;
+    if (m.dim == 2)
+    {
+        if (m.P == 1)
+            vrBndTLaunch<2, 3>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
+        else if (m.P == 2)
+            vrBndTLaunch<2, 6>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
+        else
+            vrBndTLaunch<2, 10>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
+    }
+    else
+    {
+        if (m.P == 1)
+            vrBndTLaunch<3, 4>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
+        else if (m.P == 2)
+            vrBndTLaunch<3, 10>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
+        else
+            vrBndTLaunch<3, 20>(c, nVars, uRec, u, ybar, scale, xbar, ubar);
     }
 }
 

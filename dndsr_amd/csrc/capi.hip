@@ -1096,28 +1096,30 @@ int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dou
 {
     CAPI_BEGIN
     requireMesh(ctx);
-    need(ctx->c.m.nBnd == 0, "the discrete adjoint covers periodic / interior meshes only");
     need(es && u && rhsBar && uBar, "null field");
-    need(!es->viscous, "the discrete adjoint covers the inviscid residual only");
     need(!uRecBar || uRec, "uRecBar needs uRec");
     Ctx &c = ctx->c;
-    if (!c.adjPoint)
-        c.adjPoint = devAlloc<real>(c, (size_t)20 * c.m.G * c.m.F);
+    need(c.m.nBnd == 0 || c.m.bcType, "the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
+    const int rows = es->viscous ? 50 : 20;
+    if (c.adjPointRows < rows)
+    {
+        c.adjPoint = devAlloc<real>(c, (size_t)rows * c.m.G * c.m.F);
+        c.adjPointRows = rows;
+    }
     launchEvalRHSVJP(c, toGas(es), u, uRec, deltaLambdaCell, rhsBar, uBar, uRec ? uRecBar : nullptr);
     CAPI_END
 }
 
-int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRecNewBar, double *uRecBar,
-                      double *uBar)
+int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRec, const double *u,
+                      const double *uRecNewBar, double *uRecBar, double *uBar)
 {
     CAPI_BEGIN
     requireMesh(ctx);
-    need(ctx->c.m.nBnd == 0, "the discrete adjoint covers periodic / interior meshes only");
     need(uRecNewBar != nullptr, "null field");
     need(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
     need(mode == 3 || uBar != nullptr, "uBar is required for modes 0-2");
     Ctx &c = ctx->c;
-    launchVRIterVJP(c, nVars, mode, relax < 0 ? c.m.s.jacobiRelax : relax, uRecNewBar, uRecBar,
+    launchVRIterVJP(c, nVars, mode, relax < 0 ? c.m.s.jacobiRelax : relax, uRec, u, uRecNewBar, uRecBar,
                     mode == 3 ? nullptr : uBar);
     CAPI_END
 }

@@ -1057,7 +1057,7 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
         for (int r = 0; r < 3; r++)
             sh[r] = 0, dB[r] = m.fcent[(size_t)r * F + f] - m.bary[(size_t)r * N + cL];
     const real d0 = minVol / m.farea[f] * 2;
-    const real gamma = gp.gamma, Cp = cpGas(gp);
+    const real gamma = gp.gamma;
     real acc[5] = {0, 0, 0, 0, 0};
     for (int g = 0; g < m.G; g++)
     {
@@ -1081,23 +1081,8 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
             real xR[3] = {x[0] + sh[0], x[1] + sh[1], x[2] + sh[2]};
             R.eval(m, xR, uRec, UR, GR);
         }
-        const real distBaryPerp = fabs(dot3(dB, n)) * (bnd ? 2.0 : 1.0);
-        const real distGRP = fmax(fmin(distBaryPerp, d0 * 2), d0 * 0.25);
-        real Um[5], Gm[3][5], Gp[3][5];
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-            Um[v] = 0.5 * (UL[v] + UR[v]);
-#pragma unroll
-        for (int d = 0; d < 3; d++)
-#pragma unroll
-            for (int v = 0; v < 5; v++)
-                Gm[d][v] = (GR[d][v] + GL[d][v]) * 0.5 + (1.0 / distGRP) * (n[d] * (UR[v] - UL[v]));
-        gradCons2Prim(Um, Gm, Gp, gamma);
-        real pM, a2M, HM;
-        thermal(Um[4], Um[0], sq(Um[1] / Um[0]) + sq(Um[2] / Um[0]) + sq(Um[3] / Um[0]), gamma, pM, a2M, HM);
-        const real mu = muEff(gp, Um, gasTemperature(gp, pM, Um[0]));
         real Fv[5];
-        viscousFlux(Um, Gp, n, bnd && type == BC_WALL, gamma, mu, Cp * mu / gp.prGas, Cp, Fv);
+        viscousPointFlux(gp, UL, UR, GL, GR, n, dB, d0, bnd, bnd && type == BC_WALL, Fv);
         const real wj = m.fwj[(size_t)g * F + f];
 #pragma unroll
         for (int v = 0; v < 5; v++)

@@ -442,40 +442,44 @@ __host__ __device__ inline void fluxJacDu(const T *U, const T *n, const T *dU, T
 constexpr int BC_FAR = 1, BC_WALL = 2, BC_WALL_INVIS = 3, BC_OUT = 5, BC_OUTP = 6, BC_IN = 7, BC_SYM = 9;
 
 // IdealGasThermalConservative2Primitive / Primitive2Conservative (Gas.hpp:279-317)
-__host__ __device__ inline void cons2prim(const real *U, real *prim, real gamma)
+template <class T>
+__host__ __device__ inline void cons2prim(const T *U, T *prim, real gamma)
 {
     for (int k = 0; k < 5; k++)
         prim[k] = U[k] / U[0];
-    const real vSqr = sq(U[1] / U[0]) + sq(U[2] / U[0]) + sq(U[3] / U[0]);
+    const T vSqr = sq(U[1] / U[0]) + sq(U[2] / U[0]) + sq(U[3] / U[0]);
     prim[0] = U[0];
     prim[4] = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
 }
-__host__ __device__ inline void prim2cons(const real *prim, real *U, real gamma)
+template <class T>
+__host__ __device__ inline void prim2cons(const T *prim, T *U, real gamma)
 {
     for (int k = 0; k < 5; k++)
         U[k] = prim[k] * prim[0];
-    const real vSqr = sq(prim[1]) + sq(prim[2]) + sq(prim[3]);
+    const T vSqr = sq(prim[1]) + sq(prim[2]) + sq(prim[3]);
     U[0] = prim[0];
     U[4] = prim[4] / (gamma - 1) + (prim[0] * 0.5 * vSqr);
 }
 
-__host__ __device__ inline void bcGhost(int type, const real *UL, const real *n, const real *far, real gamma, real *UR)
+template <class T>
+__host__ __device__ inline void bcGhost(int type, const T *UL, const real *n, const real *far, real gamma, T *UR)
 {
     switch (type)
     {
     case BC_FAR:
     case BC_OUTP:
     {
-        const real un = (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) / UL[0];
-        real p, asqr, H;
+        const T un = (UL[1] * n[0] + UL[2] * n[1] + UL[3] * n[2]) / UL[0];
+        T p, asqr, H;
         thermal(UL[4], UL[0], sq(UL[1] / UL[0]) + sq(UL[2] / UL[0]) + sq(UL[3] / UL[0]), gamma, p, asqr, H);
-        const real a = sqrt(asqr);
+        const T a = sqrt(asqr);
         if (un - a > 0) // supersonic outflow
             for (int k = 0; k < 5; k++)
                 UR[k] = UL[k];
         else if (un > 0) // subsonic outflow: exterior pressure
         {
-            real fp[5], lp[5];
+            real fp[5];
+            T lp[5];
             cons2prim(far, fp, gamma);
             cons2prim(UL, lp, gamma);
             lp[4] = fp[4];
@@ -483,8 +487,11 @@ __host__ __device__ inline void bcGhost(int type, const real *UL, const real *n,
         }
         else if (un + a > 0) // subsonic inflow: interior pressure
         {
-            real fp[5], lp[5];
-            cons2prim(far, fp, gamma);
+            real fpr[5];
+            T fp[5], lp[5];
+            cons2prim(far, fpr, gamma);
+            for (int k = 0; k < 5; k++)
+                fp[k] = fpr[k];
             cons2prim(UL, lp, gamma);
             fp[4] = lp[4];
             prim2cons(fp, UR, gamma);
@@ -499,7 +506,7 @@ __host__ __device__ inline void bcGhost(int type, const real *UL, const real *n,
     {
         for (int k = 0; k < 5; k++)
             UR[k] = UL[k];
-        const real vn = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
+        const T vn = UR[1] * n[0] + UR[2] * n[1] + UR[3] * n[2];
         for (int d = 0; d < 3; d++)
             UR[1 + d] -= 2 * vn * n[d];
         return;
@@ -523,14 +530,15 @@ __host__ __device__ inline void bcGhost(int type, const real *UL, const real *n,
 
 // CompressRecPart (EulerEvaluator.hpp:1541-1631): the reconstructed point state, or the
 // cell mean where it would carry negative density or internal energy
-__host__ __device__ inline void compressRecPart(const real *umean, const real *inc, real *ret)
+template <class T>
+__host__ __device__ inline void compressRecPart(const T *umean, const T *inc, T *ret)
 {
     for (int k = 0; k < 5; k++)
         ret[k] = umean[k] + inc[k];
     if (ret[0] < 0)
         for (int k = 0; k < 5; k++)
             ret[k] = umean[k];
-    const real eK = (sq(ret[1]) + sq(ret[2]) + sq(ret[3])) * 0.5 / (kVerySmall + ret[0]);
+    const T eK = (sq(ret[1]) + sq(ret[2]) + sq(ret[3])) * 0.5 / (kVerySmall + ret[0]);
     if (ret[4] - eK < 0)
         for (int k = 0; k < 5; k++)
             ret[k] = umean[k];
@@ -542,20 +550,22 @@ __host__ __device__ inline real cpGas(const GasParams &gp) { return gp.Rgas * gp
 
 // muEff, src/Euler/EulerEvaluator.hpp:751-776 (laminar part): constant mu, Sutherland's
 // law at temperature T, or constant kinematic viscosity (mu = nu rho)
-__host__ __device__ inline real muEff(const GasParams &gp, const real *U, real T)
+template <class S>
+__host__ __device__ inline S muEff(const GasParams &gp, const S *U, S T)
 {
     if (gp.muModel == 1)
     {
-        const real TRel = T / gp.TRef;
+        const S TRel = T / gp.TRef;
         return gp.muGas * TRel * sqrt(TRel) * (gp.TRef + gp.CSutherland) / (T + gp.CSutherland);
     }
     if (gp.muModel == 2)
         return gp.muGas * U[0];
-    return gp.muGas;
+    return S(gp.muGas);
 }
 
 // T = p / ((gamma - 1) / gamma Cp rho), EulerEvaluator_EvaluateDt.hxx:969, :1178
-__host__ __device__ inline real gasTemperature(const GasParams &gp, real p, real rho)
+template <class S>
+__host__ __device__ inline S gasTemperature(const GasParams &gp, S p, S rho)
 {
     return p / ((gp.gamma - 1) / gp.gamma * cpGas(gp) * rho);
 }
@@ -572,16 +582,17 @@ __host__ __device__ inline real lamVisFace(const GasParams &gp, const real *uMea
 
 // GradientCons2Prim_IdealGas, Gas.hpp:1678-1698: G[d][k] = d U_k / d x_d (conservative)
 // -> Gp[d][k] = d (rho, u, v, w, p)_k / d x_d
-__host__ __device__ inline void gradCons2Prim(const real *U, const real (*G)[5], real (*Gp)[5], real gamma)
+template <class S>
+__host__ __device__ inline void gradCons2Prim(const S *U, const S (*G)[5], S (*Gp)[5], real gamma)
 {
-    const real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
-    const real r2 = 1.0 / sq(U[0]);
+    const S velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    const S r2 = 1.0 / sq(U[0]);
     for (int d = 0; d < 3; d++)
     {
         Gp[d][0] = G[d][0];
         for (int j = 0; j < 3; j++)
             Gp[d][1 + j] = r2 * (U[0] * G[d][1 + j] - G[d][0] * U[1 + j]);
-        real a = 0, b = 0;
+        S a = 0, b = 0;
         for (int j = 0; j < 3; j++)
             a += G[d][1 + j] * velo[j], b += Gp[d][1 + j] * U[1 + j];
         Gp[d][4] = (gamma - 1) * (G[d][4] - 0.5 * (a + b));
@@ -591,36 +602,62 @@ __host__ __device__ inline void gradCons2Prim(const real *U, const real (*G)[5],
 // ViscousFlux_IdealGas, Gas.hpp:1595-1655, laminar (no QCR): stress
 // tau = mu (grad u + grad u^T) + lambda mu div u I with lambda = -2/3, heat flux k grad T,
 // adiabatic walls drop the normal part of grad T.  Gp[d][1 + j] = d u_j / d x_d.
-__host__ __device__ inline void viscousFlux(const real *U, const real (*Gp)[5], const real *n, bool adiabatic,
-                                            real gamma, real mu, real k, real Cp, real *F)
+template <class Q>
+__host__ __device__ inline void viscousFlux(const Q *U, const Q (*Gp)[5], const real *n, bool adiabatic, real gamma,
+                                            Q mu, Q k, real Cp, Q *F)
 {
     const real lambda = -2. / 3.;
-    const real velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
-    const real vSqr = velo[0] * velo[0] + velo[1] * velo[1] + velo[2] * velo[2];
-    const real p = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
-    const real c = gamma / ((gamma - 1) * Cp * U[0] * U[0]);
-    real GradT[3];
+    const Q velo[3] = {U[1] / U[0], U[2] / U[0], U[3] / U[0]};
+    const Q vSqr = velo[0] * velo[0] + velo[1] * velo[1] + velo[2] * velo[2];
+    const Q p = (gamma - 1) * (U[4] - U[0] * 0.5 * vSqr);
+    const Q c = gamma / ((gamma - 1) * Cp * U[0] * U[0]);
+    Q GradT[3];
     for (int d = 0; d < 3; d++)
         GradT[d] = c * (U[0] * Gp[d][4] - p * Gp[d][0]);
     if (adiabatic)
     {
-        const real tn = dot3(GradT, n);
+        const Q tn = dot3(GradT, n);
         for (int d = 0; d < 3; d++)
             GradT[d] -= tn * n[d];
     }
-    const real tr = Gp[0][1] + Gp[1][2] + Gp[2][3];
-    real S[3][3];
+    const Q tr = Gp[0][1] + Gp[1][2] + Gp[2][3];
+    Q S[3][3];
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++)
-            S[i][j] = (Gp[i][1 + j] + Gp[j][1 + i]) * mu + (i == j ? lambda * mu * tr : 0.0);
+            S[i][j] = (Gp[i][1 + j] + Gp[j][1 + i]) * mu + (i == j ? lambda * mu * tr : Q(0.0));
     F[0] = 0;
-    real e = 0;
+    Q e = 0;
     for (int i = 0; i < 3; i++)
     {
         F[1 + i] = S[i][0] * n[0] + S[i][1] * n[1] + S[i][2] * n[2];
         e += (S[i][0] * velo[0] + S[i][1] * velo[1] + S[i][2] * velo[2] + k * GradT[i]) * n[i];
     }
     F[4] = e;
+}
+
+// Viscous flux at one face point from both sides' values and gradients (the GRP mean
+// gradient and distance, EvaluateRHS.hxx:346-424; fluxFace EvaluateDt.hxx:1163-1215):
+// dB = barycentre offset (the other cell's, or 2 x the face centre's on a boundary face,
+// which the bnd flag doubles), d0 = 2 minVol / area, adiabatic for no-slip walls.
+template <class T>
+__host__ __device__ inline void viscousPointFlux(const GasParams &gp, const T *UL, const T *UR, const T (*GL)[5],
+                                                 const T (*GR)[5], const real *n, const real *dB, real d0, bool bnd,
+                                                 bool adiabatic, T *Fv)
+{
+    const real distBaryPerp = fabs(dot3(dB, n)) * (bnd ? 2.0 : 1.0);
+    const real distGRP = fmax(fmin(distBaryPerp, d0 * 2), d0 * 0.25);
+    T Um[5], Gm[3][5], Gp[3][5];
+    for (int v = 0; v < 5; v++)
+        Um[v] = 0.5 * (UL[v] + UR[v]);
+    for (int d = 0; d < 3; d++)
+        for (int v = 0; v < 5; v++)
+            Gm[d][v] = (GR[d][v] + GL[d][v]) * 0.5 + (1.0 / distGRP) * (n[d] * (UR[v] - UL[v]));
+    gradCons2Prim(Um, Gm, Gp, gp.gamma);
+    T pM, a2M, HM;
+    thermal(Um[4], Um[0], sq(Um[1] / Um[0]) + sq(Um[2] / Um[0]) + sq(Um[3] / Um[0]), gp.gamma, pM, a2M, HM);
+    const T mu = muEff(gp, Um, gasTemperature(gp, pM, Um[0]));
+    const real Cp = cpGas(gp);
+    viscousFlux(Um, Gp, n, adiabatic, gp.gamma, mu, Cp * mu / gp.prGas, Cp, Fv);
 }
 
 } // namespace dndsr

@@ -116,7 +116,8 @@ struct Ctx
     size_t redCap = 0;
     real *hostRed = nullptr;    // pinned host mirror of reductions
     real *faceLamTmp = nullptr; // [2][F] (lambdaFace, deltaLambdaFace)
-    real *adjPoint = nullptr;   // [20][G F] point sensitivities of the RHS adjoint
+    real *adjPoint = nullptr;   // [20 or 50][G F] point sensitivities of the RHS adjoint
+    int adjPointRows = 0;
     size_t faceLamCap = 0;
     std::vector<void *> owned;  // device allocations freed at destroy
     // optional per-kernel timing with HIP events on the launch stream
@@ -254,7 +255,8 @@ int gmresSolve(Ctx &c, const GasParams &gp, const LinWork &w, const real *b, rea
 // ---- launchers (adjoint.hip)
 void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
                       const real *rbar, real *ubar, real *urecbar);
-void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *ybar, real *xbar, real *ubar);
+void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *uRec, const real *u, const real *ybar,
+                     real *xbar, real *ubar);
 
 template <typename T>
 T *devAlloc(Ctx &c, size_t n)

@@ -254,14 +254,18 @@ int dndsr_gmres_lusgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alp
  * Vector-Jacobian product of EvaluateRHS (dndsr_eval_rhs) with deltaLambdaCell held fixed:
  *   uBar = (d rhs / d u)^T rhsBar, uRecBar = (d rhs / d uRec)^T rhsBar (overwritten; uRecBar may
  *   be NULL, or must be when uRec is NULL).  Riemann solvers: all of dndsr_eval_rhs's;
- *   maxOrder 1 and 2. */
+ *   maxOrder 1 and 2; inviscid or viscous (es->viscous); periodic, interior and boundary
+ *   faces (the ghost states differentiated as evaluated). */
 int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                        const double *deltaLambdaCell, const double *rhsBar, double *uBar, double *uRecBar);
 /* Transpose of one VR sweep (dndsr_vr_iter modes 0-2, mode 3 = dndsr_vr_iter_diff):
  *   uRecBar = (d uRecNew / d uRec)^T uRecNewBar, uBar = (d uRecNew / d u)^T uRecNewBar
- * (uBar unused for mode 3; uRecBar may be NULL).  relax < 0 uses settings.jacobiRelax. */
-int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRecNewBar, double *uRecBar,
-                      double *uBar);
+ * (uBar unused for mode 3; uRecBar may be NULL).  relax < 0 uses settings.jacobiRelax.
+ * uRec, u: the state the forward sweep read; on a mesh with boundary faces the boundary
+ * term depends on it (Euler ghosts: both; given point values: u), elsewhere they may be
+ * NULL.  Mode 3 covers meshes without boundary faces. */
+int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRec, const double *u,
+                      const double *uRecNewBar, double *uRecBar, double *uBar);
 
 /* ---- instrumentation (no counterpart in the reference, whose timers are the host-side
  * PerformanceTimer slots of src/DNDS/Profiling.hpp:22-49) ------------------------------

@@ -3,7 +3,8 @@ viscous spectral radius, EvaluateRHS with the viscous face fluxes (its viscous p
 viscous minus inviscid residual, to 1e-10 of that part; the full residual to 1e-12), for
 the three viscosity laws, periodic and bounded meshes (no-slip adiabatic walls, symmetry,
 far field, outflow, inflow, back-pressure outflow), reconstruction orders 1-3, and the
-device time loops (SSP-RK3, implicit dual-time with LU-SGS) on a viscous run."""
+device time loops (SSP-RK3, implicit dual-time with LU-SGS) on a viscous run.  The viscous
+adjoint is checked in test_gpu_adjoint_bc.py."""
 from __future__ import annotations
 
 import numpy as np
@@ -154,15 +155,3 @@ def test_viscous_time_loops(require_gpu):
         assert n == ni == 3
         assert rel_l2(g.to_host(du), ui) <= 1e-11
         assert rel_l2(g.to_host(dr), ri) <= 1e-10
-
-
-def test_adjoint_rejects_viscous(require_gpu):
-    import torch
-    from dndsr_amd.adjoint import rhs_vjp
-    from dndsr_amd.cfv import VariationalReconstruction
-    m = small_mesh("2d")
-    g = VariationalReconstruction(m, vr_settings(1)).construct()
-    _, ev = _ev(g, None, None)
-    u = g.to_device(random_state(g.nCell, seed=1))
-    with pytest.raises(RuntimeError, match="inviscid"):
-        rhs_vjp(ev, u, None, None, torch.zeros_like(u))
