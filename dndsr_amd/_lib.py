@@ -84,6 +84,8 @@ SIGNATURES = {
                           C.c_int, C.c_int, dp],
     "dndsr_eval_rhs_vjp": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp, vp, vp],
     "dndsr_vr_iter_vjp": [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp, vp],
+    "dndsr_zone_force": [vp, C.POINTER(EulerSettings), vp, vp, vp, C.c_int, dp],
+    "dndsr_zone_force_vjp": [vp, C.POINTER(EulerSettings), vp, vp, vp, C.c_int, dp, vp, vp],
     "dndsr_set_timing": [vp, C.c_int],
     "dndsr_get_timing": [vp, dp, i64p, C.c_int],
 }

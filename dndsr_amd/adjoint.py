@@ -15,6 +15,7 @@ from __future__ import annotations
 
 import ctypes as C
 
+import numpy as np
 import torch
 
 from . import _lib
@@ -118,3 +119,24 @@ def residual(ev, u, uRec0, nRec=1, deltaLambdaCell=None):
     evaluator's value from its last EvaluateDt)."""
     dlc = ev.deltaLambdaCell if deltaLambdaCell is None else deltaLambdaCell
     return ResidualFunction.apply(u, uRec0, ev, nRec, dlc)
+
+
+def zone_force(ev, u, uRec, zone, deltaLambdaCell=None):
+    """Force of the flow on boundary zone `zone` (momentum flux out of the domain through
+    its faces, Riemann minus viscous flux) at (u, uRec); a length-3 numpy array."""
+    dlc = ev.deltaLambdaCell if deltaLambdaCell is None else deltaLambdaCell
+    out = np.zeros(3)
+    check(lib().dndsr_zone_force(ev.ctx.h, C.byref(ev.es), _ptr(u.contiguous()), _ptr(uRec), _ptr(dlc), int(zone),
+                                 out.ctypes.data_as(_lib.dp)))
+    return out
+
+
+def zone_force_vjp(ev, u, uRec, zone, forceBar, deltaLambdaCell=None, need_rec=True):
+    """(d force / d u)^T forceBar and (d force / d uRec)^T forceBar on the device."""
+    dlc = ev.deltaLambdaCell if deltaLambdaCell is None else deltaLambdaCell
+    fb = np.ascontiguousarray(forceBar, np.float64)
+    uBar = torch.empty_like(u)
+    recBar = torch.empty_like(uRec) if (uRec is not None and need_rec) else None
+    check(lib().dndsr_zone_force_vjp(ev.ctx.h, C.byref(ev.es), _ptr(u.contiguous()), _ptr(uRec), _ptr(dlc),
+                                     int(zone), fb.ctypes.data_as(_lib.dp), _ptr(uBar), _ptr(recBar)))
+    return uBar, recBar

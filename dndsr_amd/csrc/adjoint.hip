@@ -119,7 +119,7 @@ constexpr int kAdjRowsInv = 20, kAdjRowsVis = 50;
 template <int DIM, int NDOF, int RS, bool BATCH>
 __global__ void __launch_bounds__(128) kRHSPointAdj(DevMesh m, GasParams gp, const real *__restrict__ u,
                                                     const real *__restrict__ uRec, const real *__restrict__ dlc,
-                                                    const real *__restrict__ rbar, real *__restrict__ pbuf)
+                                                    const real *__restrict__ fbar, real *__restrict__ pbuf)
 {
     constexpr int M = NDOF - 1;
     const size_t N = m.N, F = m.F, GF = (size_t)m.G * F;
@@ -146,7 +146,7 @@ __global__ void __launch_bounds__(128) kRHSPointAdj(DevMesh m, GasParams gp, con
     {
         uLm[v] = UL[v] = u[(size_t)v * N + cL];
         uRm[v] = UR[v] = u[(size_t)v * N + cR];
-        Fbar[v] = -wj * (rbar[(size_t)v * N + cL] / m.vol[cL] - rbar[(size_t)v * N + cR] / m.vol[cR]);
+        Fbar[v] = -wj * fbar[(size_t)v * F + f];
     }
     if (uRec)
     {
@@ -210,7 +210,7 @@ __global__ void __launch_bounds__(128) kRHSPointAdj(DevMesh m, GasParams gp, con
 // writes the gradient rows 20-49.  Fvbar = -Fbar (the face flux gains +Fv w).
 template <int DIM, int NDOF>
 __global__ void __launch_bounds__(128) kRHSPointAdjVis(DevMesh m, GasParams gp, const real *__restrict__ u,
-                                                       const real *__restrict__ uRec, const real *__restrict__ rbar,
+                                                       const real *__restrict__ uRec, const real *__restrict__ fbar,
                                                        real *__restrict__ pbuf)
 {
     const size_t N = m.N, F = m.F, GF = (size_t)m.G * F;
@@ -236,7 +236,7 @@ __global__ void __launch_bounds__(128) kRHSPointAdjVis(DevMesh m, GasParams gp, 
     real Fvb[5];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        Fvb[v] = wj * (rbar[(size_t)v * N + cL] / m.vol[cL] - rbar[(size_t)v * N + cR] / m.vol[cR]);
+        Fvb[v] = wj * fbar[(size_t)v * F + f];
     real UL[5], UR[5], GL[3][5], GR[3][5];
     cellValGradAt<DIM, NDOF>(m, cL, x, nullptr, u, uRec, UL, GL);
     cellValGradAt<DIM, NDOF>(m, cR, x, sh, u, uRec, UR, GR);
@@ -275,7 +275,7 @@ __global__ void __launch_bounds__(128) kRHSPointAdjVis(DevMesh m, GasParams gp, 
 template <int DIM, int NDOF, int RS, bool BATCH>
 __global__ void __launch_bounds__(128) kRHSBndPointAdj(DevMesh m, GasParams gp, const real *__restrict__ u,
                                                        const real *__restrict__ uRec, const real *__restrict__ dlc,
-                                                       const real *__restrict__ rbar, real *__restrict__ pbuf, int vis)
+                                                       const real *__restrict__ fbar, real *__restrict__ pbuf, int vis)
 {
     const size_t N = m.N, F = m.F, GF = (size_t)m.G * F;
     const size_t tb = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -303,7 +303,7 @@ __global__ void __launch_bounds__(128) kRHSBndPointAdj(DevMesh m, GasParams gp, 
 #pragma unroll
     for (int v = 0; v < 5; v++)
     {
-        Fbar[v] = -wj * (rbar[(size_t)v * N + cL] / m.vol[cL]);
+        Fbar[v] = -wj * fbar[(size_t)v * F + f];
         uLm[v] = u[(size_t)v * N + cL];
     }
     real UL[5], GL[3][5];
@@ -445,6 +445,21 @@ __global__ void __launch_bounds__(128) kRHSGatherAdj(DevMesh m, const real *__re
                 urecbar[(size_t)(v * M + j) * N + i] = ar[v][j];
 }
 
+// adjoint of the face flux from the adjoint of the residual (the gather rhs_L += ff / V_L,
+// rhs_R -= ff / V_R): fbar_f = rbar_L / V_L - rbar_R / V_R
+__global__ void __launch_bounds__(256) kFaceBar(DevMesh m, const real *__restrict__ rbar, real *__restrict__ fbar)
+{
+    const idx f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F)
+        return;
+    const size_t N = m.N, F = m.F;
+    const idx cL = m.f2c[f], cR = m.f2c[F + f];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        fbar[(size_t)v * F + f] = cR >= 0 ? rbar[(size_t)v * N + cL] / m.vol[cL] - rbar[(size_t)v * N + cR] / m.vol[cR]
+                                          : rbar[(size_t)v * N + cL] / m.vol[cL];
+}
+
 template <int DIM, int NDOF, int RS, bool BATCH>
 static void rhsAdjLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
                          const real *rbar, real *ubar, real *urecbar)
@@ -453,6 +468,7 @@ static void rhsAdjLaunch(Ctx &c, const GasParams &gp, const real *u, const real 
     const int vis = gp.viscous ? 1 : 0;
     kRHSPointAdj<DIM, NDOF, RS, BATCH><<<(unsigned)((GF + 127) / 128), 128, 0, c.stream>>>(c.m, gp, u, uRec, dlc,
                                                                                           rbar, c.adjPoint);
+    // (rbar here is the face-flux adjoint [5][F], formed by launchEvalRHSVJP)
     DNDSR_HIP_CHECK(hipGetLastError());
     if (vis)
     {
@@ -516,10 +532,17 @@ static void rhsAdjRS(Ctx &c, const GasParams &gp, const real *u, const real *uRe
 }
 
 void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
-                      const real *rbar, real *ubar, real *urecbar)
+                      const real *rbar, real *ubar, real *urecbar, const real *faceBar)
 {
     DevMesh &m = c.m;
     KernelTimer kt(c, K_ADJOINT);
+    if (!faceBar)
+    {
+        kFaceBar<<<gridFor(m.F, 256), 256, 0, c.stream>>>(m, rbar, c.adjFaceBar);
+        DNDSR_HIP_CHECK(hipGetLastError());
+        faceBar = c.adjFaceBar;
+    }
+    rbar = faceBar; // the point kernels read the face-flux adjoint
     if (m.dim == 2)
     {
         if (!uRec || m.P == 1)

@@ -1106,7 +1106,59 @@ int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dou
         c.adjPoint = devAlloc<real>(c, (size_t)rows * c.m.G * c.m.F);
         c.adjPointRows = rows;
     }
+    if (!c.adjFaceBar)
+        c.adjFaceBar = devAlloc<real>(c, (size_t)5 * c.m.F);
     launchEvalRHSVJP(c, toGas(es), u, uRec, deltaLambdaCell, rhsBar, uBar, uRec ? uRecBar : nullptr);
+    CAPI_END
+}
+
+int dndsr_zone_force(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                     const double *deltaLambdaCell, int zone, double *force)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && u && force, "null argument");
+    Ctx &c = ctx->c;
+    need(zone >= 0 && zone < c.m.nBCZone, "zone out of range");
+    need(c.m.bcType, "the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
+    if (!c.adjRhs)
+        c.adjRhs = devAlloc<real>(c, (size_t)5 * c.m.N);
+    launchEvalRHS(c, toGas(es), u, uRec, deltaLambdaCell, c.adjRhs);
+    real *dev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 5); // three values
+    launchZoneForce(c, zone, dev);
+    DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, dev, 3 * sizeof(real), hipMemcpyDeviceToHost, c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    for (int d = 0; d < 3; d++)
+        force[d] = c.hostRed[d];
+    CAPI_END
+}
+
+int dndsr_zone_force_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                         const double *deltaLambdaCell, int zone, const double *forceBar, double *uBar,
+                         double *uRecBar)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(es && u && forceBar && uBar, "null argument");
+    need(!uRecBar || uRec, "uRecBar needs uRec");
+    Ctx &c = ctx->c;
+    need(zone >= 0 && zone < c.m.nBCZone, "zone out of range");
+    need(c.m.bcType, "the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
+    const int rows = es->viscous ? 50 : 20;
+    if (c.adjPointRows < rows)
+    {
+        c.adjPoint = devAlloc<real>(c, (size_t)rows * c.m.G * c.m.F);
+        c.adjPointRows = rows;
+    }
+    if (!c.adjFaceBar)
+        c.adjFaceBar = devAlloc<real>(c, (size_t)5 * c.m.F);
+    for (int d = 0; d < 3; d++)
+        c.hostRed[d] = forceBar[d];
+    real *dev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 5);
+    DNDSR_HIP_CHECK(hipMemcpyAsync(dev, c.hostRed, 3 * sizeof(real), hipMemcpyHostToDevice, c.stream));
+    launchZoneForceBar(c, zone, dev, c.adjFaceBar);
+    launchEvalRHSVJP(c, toGas(es), u, uRec, deltaLambdaCell, nullptr, uBar, uRec ? uRecBar : nullptr, c.adjFaceBar);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
     CAPI_END
 }
 

@@ -1702,6 +1702,66 @@ void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real
     launchSGSPackedT<float>(c, gp, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
 }
 
+// ---------------------------------------------------------------------------
+// Force on a boundary zone (the wall force of a drag functional): the momentum part of
+// the face fluxes of the zone's faces from the last EvaluateRHS, with the sign of the
+// flux out of the domain, summed in boundary-face order by one workgroup.
+// ---------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) kZoneForce(DevMesh m, int zone, const real *__restrict__ ff,
+                                                  real *__restrict__ out)
+{
+    __shared__ real sm[3][256];
+    real a[3] = {0, 0, 0};
+    for (idx t = threadIdx.x; t < m.nBnd; t += 256)
+    {
+        const idx f = m.bndFace[t];
+        if (m.fzone[f] != zone)
+            continue;
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            a[d] -= ff[(size_t)(1 + d) * m.F + f];
+    }
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        sm[d][threadIdx.x] = a[d];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                sm[d][threadIdx.x] += sm[d][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int d = 0; d < 3; d++)
+            out[d] = sm[d][0];
+}
+void launchZoneForce(Ctx &c, int zone, real *forceDev)
+{
+    kZoneForce<<<1, 256, 0, c.stream>>>(c.m, zone, c.faceFlux, forceDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+__global__ void __launch_bounds__(256) kZoneForceBar(DevMesh m, int zone, const real *__restrict__ fb,
+                                                     real *__restrict__ fbar)
+{
+    const idx f = blockIdx.x * blockDim.x + threadIdx.x;
+    if (f >= m.F)
+        return;
+    const bool on = m.f2c[(size_t)m.F + f] < 0 && m.fzone[f] == zone;
+    fbar[f] = 0;
+#pragma unroll
+    for (int d = 0; d < 3; d++)
+        fbar[(size_t)(1 + d) * m.F + f] = on ? -fb[d] : 0.0;
+    fbar[(size_t)4 * m.F + f] = 0;
+}
+void launchZoneForceBar(Ctx &c, int zone, const real *forceBarDev, real *fbar)
+{
+    kZoneForceBar<<<gridFor(c.m.F, 256), 256, 0, c.stream>>>(c.m, zone, forceBarDev, fbar);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
 // CompressInc (EulerEvaluator.hpp:1645-1830).  When neither density nor pressure
 // would fall below its relaxed bound, both reduce to the plain sum; a cell that
 // would need the limiter sets *flag instead (the caller reports it as an error).

@@ -118,6 +118,8 @@ struct Ctx
     real *faceLamTmp = nullptr; // [2][F] (lambdaFace, deltaLambdaFace)
     real *adjPoint = nullptr;   // [20 or 50][G F] point sensitivities of the RHS adjoint
     int adjPointRows = 0;
+    real *adjFaceBar = nullptr; // [5][F] face-flux adjoint
+    real *adjRhs = nullptr;     // [5][N] residual of a functional's forward evaluation
     size_t faceLamCap = 0;
     std::vector<void *> owned;  // device allocations freed at destroy
     // optional per-kernel timing with HIP events on the launch stream
@@ -253,8 +255,15 @@ void applyLUSGSPrecond(Ctx &c, const GasParams &gp, const LinWork &w, const real
 int gmresSolve(Ctx &c, const GasParams &gp, const LinWork &w, const real *b, real *x, int kSub, int nRestart,
                double *residualOut);
 // ---- launchers (adjoint.hip)
+// faceBar (optional, [5][F]): the adjoint of the face fluxes instead of the residual's
+// (rbar unused then); without it the face adjoint is formed from rbar
 void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc,
-                      const real *rbar, real *ubar, real *urecbar);
+                      const real *rbar, real *ubar, real *urecbar, const real *faceBar = nullptr);
+// force on the boundary faces of a zone from the last EvaluateRHS's face fluxes:
+// forceDev[d] = - sum_f ff[1 + d][f] (the momentum flux out of the domain)
+void launchZoneForce(Ctx &c, int zone, real *forceDev);
+// face-flux adjoint of that force: fbar[1 + d][f] = -forceBar[d] on the zone's faces, 0 elsewhere
+void launchZoneForceBar(Ctx &c, int zone, const real *forceBarDev, real *fbar);
 void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *uRec, const real *u, const real *ybar,
                      real *xbar, real *ubar);
 

@@ -258,6 +258,18 @@ int dndsr_gmres_lusgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alp
  *   faces (the ghost states differentiated as evaluated). */
 int dndsr_eval_rhs_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                        const double *deltaLambdaCell, const double *rhsBar, double *uBar, double *uRecBar);
+/* Force on the boundary faces of one zone (drag / lift of a wall): EvaluateRHS at (u, uRec),
+ * then force[d] = sum over the zone's faces of the momentum flux out of the domain
+ * (Riemann flux minus viscous flux, integrated over the face).  Synchronises; force is a
+ * host array of 3.  No counterpart in the reference's evaluator (its solver integrates
+ * wall forces in its output stage); the wall-force functional of an adjoint gradient. */
+int dndsr_zone_force(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                     const double *deltaLambdaCell, int zone, double *force);
+/* uBar = (d force / d u)^T forceBar, uRecBar = (d force / d uRec)^T forceBar (forceBar a host
+ * array of 3; deltaLambdaCell held fixed; overwritten outputs; uRecBar may be NULL). */
+int dndsr_zone_force_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                         const double *deltaLambdaCell, int zone, const double *forceBar, double *uBar,
+                         double *uRecBar);
 /* Transpose of one VR sweep (dndsr_vr_iter modes 0-2, mode 3 = dndsr_vr_iter_diff):
  *   uRecBar = (d uRecNew / d uRec)^T uRecNewBar, uBar = (d uRecNew / d u)^T uRecNewBar
  * (uBar unused for mode 3; uRecBar may be NULL).  relax < 0 uses settings.jacobiRelax.

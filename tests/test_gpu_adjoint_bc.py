@@ -9,7 +9,9 @@ adjoint gradients are held to):
 * the VR sweep VJP with the Euler boundary term (nonlinear in the state: ghost of the
   compressed point value), in (uRec, u) directions, Jacobi and recordInc;
 * the chained residual (nRec sweeps + RHS, reconstruction recomputed in backward) through
-  torch.autograd against FD of a weighted functional of the residual.
+  torch.autograd against FD of a weighted functional of the residual;
+* the wall-force functional (the drag of an adjoint shape gradient): its value against the
+  closed form p A n on a flat wall, its gradient against FD.
 """
 from __future__ import annotations
 
@@ -161,3 +163,51 @@ def test_vr_jacobian_transpose_refuses_boundaries(require_gpu):
     yb = torch.zeros((5, g.M, g.nCell), dtype=torch.float64, device="cuda")
     with pytest.raises(RuntimeError, match="without boundary faces"):
         adjoint.vr_vjp(g, yb, 3, -1.0)
+
+
+def test_zone_force_of_uniform_pressure(require_gpu):
+    """A uniform state moving parallel to a flat wall: the Riemann flux through the wall
+    is the pressure flux p n, so the force on the wall zone is p times its area along its
+    outward normal (the lower wall y = 0 of a unit-span box: (0, -p Lx, 0) in 2-D)."""
+    import torch
+    from dndsr_amd import adjoint, bc
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    Lx, p = 3.0, 0.9
+    m = mm.box_quad_grid(6, 4, Lx, 2.0, zones=dict(xmin=0, xmax=0, ymin=1, ymax=0), jitter=0.0)
+    types = [bc.BC_FAR, bc.BC_WALL_INVIS]
+    U = _cons(1.1, [0.4, 0.0, 0.0], p)
+    g = VariationalReconstruction(m, dict(vr_settings(1), bcTypes=types)).construct()
+    ev = EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType="Roe", bcTypes=types, bcValues=np.stack([U, U])))
+    u = g.to_device(np.tile(U, (g.nCell, 1)))
+    rec = g.BuildURec(5)
+    dt = torch.zeros(g.nCell, dtype=torch.float64, device=u.device)
+    ev.EvaluateDt(dt, u, None, 1.0)
+    f = adjoint.zone_force(ev, u, rec, 1)
+    np.testing.assert_allclose(f, [0.0, -p * Lx, 0.0], atol=1e-12)
+
+
+@pytest.mark.parametrize("kind,order,viscous", [("quad", 1, True), ("mixed", 2, True), ("tet", 1, True),
+                                                ("quad", 2, False)])
+def test_zone_force_vjp_matches_central_fd(require_gpu, kind, order, viscous):
+    """Gradient of the wall force (zone 2: the no-slip wall) to (u, uRec) against central FD."""
+    import torch
+    from dndsr_amd import adjoint
+    print(f"\n[adjoint bc] wall force vjp {kind} p{order} viscous={viscous}", flush=True)
+    g, ev = _case(kind, order, "Roe", viscous)
+    u, rec = _state(g)
+    dt = torch.zeros(g.nCell, dtype=torch.float64, device=u.device)
+    ev.EvaluateDt(dt, u, None, 1.0)
+    dlc = ev.deltaLambdaCell.clone()
+    fb = np.array([1.0, 0.3, -0.2])
+    uBar, recBar = adjoint.zone_force_vjp(ev, u, rec, 2, fb, dlc)
+    gen = torch.Generator(device="cpu").manual_seed(9)
+    vu = torch.randn(u.shape, generator=gen, dtype=torch.float64).to(u.device) * 1e-1
+    vr = torch.randn(rec.shape, generator=gen, dtype=torch.float64).to(u.device) * 1e-2
+    eps = 1e-6
+    for du, dr, lhs in ((vu, 0 * vr, _dot(uBar, vu)), (0 * vu, vr, _dot(recBar, vr))):
+        fp = adjoint.zone_force(ev, u + eps * du, rec + eps * dr, 2, dlc)
+        fm = adjoint.zone_force(ev, u - eps * du, rec - eps * dr, 2, dlc)
+        fd = float(fb @ ((fp - fm) / (2 * eps)))
+        assert abs(lhs - fd) <= 1e-6 * max(abs(fd), 1e-30), (lhs, fd)
