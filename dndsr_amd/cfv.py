@@ -65,7 +65,7 @@ class VariationalReconstruction:
     def __init__(self, mesh, settings: dict | None = None, device: int = 0, ctx: Context | None = None):
         s = dict(maxOrder=1, intOrder=1, dirWeightScheme="Factorial", geomWeightScheme="GWNone",
                  geomWeightBias=0.0, geomWeightPower=0.5, scaleMultiplier=1.0, jacobiRelax=1.0,
-                 nIterCellSmoothScale=15, bcWeight=1.0, zoneDirWeight=None)
+                 nIterCellSmoothScale=15, bcWeight=1.0, zoneDirWeight=None, SORInstead=False)
         if settings:
             s.update(settings)
             if "functionalSettings" in settings:
@@ -108,7 +108,7 @@ class VariationalReconstruction:
         vs = _lib.VRSettings(s["maxOrder"], s["intOrder"], s["dirWeightScheme"], s["geomWeightScheme"],
                              s["geomWeightBias"], s["geomWeightPower"], s["scaleMultiplier"], s["jacobiRelax"],
                              s["nIterCellSmoothScale"], s["bcWeight"],
-                             None if zw is None else zw.ctypes.data_as(_lib.dp))
+                             None if zw is None else zw.ctypes.data_as(_lib.dp), int(bool(s["SORInstead"])))
         check(lib().dndsr_vr_construct(self.ctx.h, C.byref(md), C.byref(vs)))
         nC, nF = C.c_int64(), C.c_int64()
         M, K, G, Gc, nCol = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()
@@ -217,14 +217,28 @@ class VariationalReconstruction:
     # ---- reconstruction (VariationalReconstruction_Reconstruction.hxx:485-670)
     def DoReconstructionIter(self, uRec, uRecNew, u, putIntoNew: bool = True, recordInc: bool = False,
                              uRecIsZero: bool = False, relax: float = -1.0):
-        """Jacobi form.  putIntoNew=False leaves the new iterate in uRec (as the reference does)."""
+        """putIntoNew=False leaves the new iterate in uRec (as the reference does).  With
+        settings SORInstead (and no recordInc) uRec is swept in place, colour by colour; for
+        putIntoNew the contents of uRec and uRecNew are then swapped, as the reference swaps
+        the arrays."""
         nVars = u.shape[0]
         mode = 0 if not recordInc else (2 if uRecIsZero else 1)
         if recordInc and not putIntoNew:
             raise ValueError("recordInc needs putIntoNew (the -RHS goes into uRecNew)")
+        if self.settings["SORInstead"] and not recordInc:
+            self.DoReconstructionSOR(uRec, u, relax)
+            if putIntoNew:
+                old = uRecNew.clone()
+                uRecNew.copy_(uRec)
+                uRec.copy_(old)
+            return
         check(lib().dndsr_vr_iter(self.ctx.h, nVars, _ptr(uRec), _ptr(uRecNew), _ptr(u), mode, float(relax)))
         if not putIntoNew:
             uRec.copy_(uRecNew)
+
+    def DoReconstructionSOR(self, uRec, u, relax: float = -1.0):
+        """One multicolour SOR sweep, uRec updated in place (cells colour by colour, colors())."""
+        check(lib().dndsr_vr_iter(self.ctx.h, u.shape[0], _ptr(uRec), None, _ptr(u), 4, float(relax)))
 
     def DoReconstructionIterDiff(self, uRec, uRecDiff, uRecNew, u=None):
         """uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j - A^-1 BCdiff; uRec and u feed the

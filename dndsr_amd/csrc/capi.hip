@@ -215,7 +215,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.nNode = (idx)md->nNode;
     m.s = VRSettingsDev{s->maxOrder, s->intOrder, s->dirWeightScheme, s->geomWeightScheme, s->geomWeightBias,
                         s->geomWeightPower, s->scaleMultiplier, s->jacobiRelax, s->nIterCellSmoothScale,
-                        s->bcWeight};
+                        s->bcWeight, s->SORInstead != 0};
     need(md->nBCZone >= 0 && md->nBCZone < 4096, "nBCZone out of range");
     m.nBCZone = md->nBCZone;
     const size_t N = m.N, F = m.F;
@@ -666,15 +666,25 @@ int dndsr_d2h_cellfield(dndsr_ctx *ctx, const double *dev, double *hostRows, int
     CAPI_END
 }
 
-int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew, const double *u, int mode,
+int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, double *uRec, double *uRecNew, const double *u, int mode,
                   double relax)
 {
     CAPI_BEGIN
     requireMesh(ctx);
-    need(uRec && uRecNew && u, "null field");
-    need(uRec != uRecNew, "uRec and uRecNew must be different arrays (Jacobi form)");
-    need(mode >= 0 && mode <= 2, "mode must be 0, 1 or 2");
-    launchVRIter(ctx->c, nVars, uRec, uRecNew, u, mode, relax < 0 ? ctx->c.m.s.jacobiRelax : relax);
+    need(mode >= 0 && mode <= 2 || mode == 4, "mode must be 0, 1, 2 or 4");
+    if (relax < 0)
+        relax = ctx->c.m.s.jacobiRelax;
+    if (mode == 4)
+    {
+        need(uRec && u, "null field");
+        launchVRSOR(ctx->c, nVars, uRec, u, relax);
+    }
+    else
+    {
+        need(uRec && uRecNew && u, "null field");
+        need(uRec != uRecNew, "uRec and uRecNew must be different arrays (Jacobi form)");
+        launchVRIter(ctx->c, nVars, uRec, uRecNew, u, mode, relax);
+    }
     CAPI_END
 }
 
@@ -866,6 +876,19 @@ int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int res
     CAPI_END
 }
 
+// One reconstruction sweep of the solver loops (frhs, EulerSolver.hxx:240-250, putIntoNew =
+// false): SOR in place when SORInstead, else Jacobi into recNew and the two swapped.
+static void solverVRSweep(Ctx &c, real *&rec, real *&recNew, const real *u)
+{
+    if (c.m.s.SORInstead)
+        launchVRSOR(c, 5, rec, u, c.m.s.jacobiRelax);
+    else
+    {
+        launchVRIter(c, 5, rec, recNew, u, 0, c.m.s.jacobiRelax);
+        std::swap(rec, recNew);
+    }
+}
+
 int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, double *uRec, int nStep, double CFL,
                  int nRec, double dtFixed)
 {
@@ -882,10 +905,7 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
     // one RK stage: VR sweeps at u, RHS(u), and u = a u + b xLast + cdt dt RHS in the gather pass
     auto stage = [&](real a, real b, real cdt, const real *dtp, real *save) {
         for (int r = 0; r < nRec; r++)
-        {
-            launchVRIter(c, 5, rec, recNew, u, 0, m.s.jacobiRelax);
-            std::swap(rec, recNew);
-        }
+            solverVRSweep(c, rec, recNew, u);
         StageUpdate st{a, b, cdt, b != 0.0 ? ctx->wXLast : nullptr, dtp, u, save};
         launchEvalRHS(c, gp, u, rec, ctx->wDlc, nullptr, &st);
     };
@@ -957,12 +977,9 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     {
         // fdtau: local pseudo time step (alphaDiag = 1)
         launchEvalDt(c, gp, u, is->CFL, 1e100, 1, ctx->iDTau, ctx->wLam, ctx->wDlc, ctx->wDtMin);
-        // frhs: nRec Jacobi VR sweeps + EvaluateRHS
+        // frhs: nRec VR sweeps + EvaluateRHS
         for (int r = 0; r < is->nRec; r++)
-        {
-            launchVRIter(c, 5, rec, recNew, u, 0, m.s.jacobiRelax);
-            std::swap(rec, recNew);
-        }
+            solverVRSweep(c, rec, recNew, u);
         launchEvalRHS(c, gp, u, rec, ctx->wDlc, ctx->iRhsF);
         launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
         // fsolve: LUSGSMatrixInit + (1 + sgsIter) symmetric sweeps from a zero increment

@@ -40,6 +40,7 @@ struct VRSettingsDev
     real jacobiRelax;
     int nIterCellSmoothScale;
     real bcWeight;
+    bool SORInstead; // solver loops sweep the reconstruction in place (multicolour SOR)
 };
 
 // Device-resident mesh, metrics and VR coefficients (all arrays SoA, see dev_common.hpp).
@@ -192,6 +193,7 @@ struct KernelTimer
 void setupMetricsAndVR(Ctx &c);
 // ---- launchers (vr.hip)
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax);
+void launchVRSOR(Ctx &c, int nVars, real *uRec, const real *u, real relax);
 void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *uRecDiff, real *uRecNew);
 // ---- launchers (euler.hip)
 void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,

@@ -1,7 +1,7 @@
 // vr.hip -- variational-reconstruction iteration kernels.
 //
-// DoReconstructionIter, Jacobi branch (SORInstead = false) and the recordInc
-// modes, src/CFV/VariationalReconstruction_Reconstruction.hxx:485-615, and
+// DoReconstructionIter, Jacobi branch (SORInstead = false), SOR branch and the
+// recordInc modes, src/CFV/VariationalReconstruction_Reconstruction.hxx:485-615, and
 // DoReconstructionIterDiff, :617-670.
 #include "gas.hpp"
 #include "internal.hpp"
@@ -224,6 +224,91 @@ static void vrIterDiffM(Ctx &c, int nVars, const real *x, real *y)
 // MODE 0: Jacobi (scale = relax); 1: recordInc (scale = -1); 2: the Jacobian action
 // (uB(uBL + D0phi x) - uB(uBL) in place of uB - u_i, scale = -1).
 // ---------------------------------------------------------------------------
+// BCC of boundary face f of cell i (mom, cen, isc: the cell's moments, centre and scale;
+// um its mean): the own block is read through rec(v, j) and, for LIN, the Jacobian
+// direction through dx(v, j).
+template <int DIM, int NDOF, int NV, bool LIN, class Rec, class Dx>
+__device__ inline void boundaryFaceBCC(const DevMesh &m, idx f, const real *mom, const real *cen, real isc,
+                                       const real *um, Rec rec, Dx dx, real (&BCC)[NDOF - 1][NV])
+{
+    constexpr int M = NDOF - 1;
+    const size_t F = m.F;
+    const bool euler = NV == 5 && m.bcType != nullptr;
+    const int zone = m.fzone[f];
+    const real w0 = m.faceW[f]; // order-0 weight; faceL^0 = 1
+    const real w2 = w0 * w0;
+#pragma unroll
+    for (int j = 0; j < M; j++)
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            BCC[j][v] = 0;
+    for (int g = 0; g < m.G; g++)
+    {
+        real xs[3], phi[M];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+        {
+            xs[r] = m.fpts[((size_t)g * 3 + r) * F + f] - cen[r];
+            if (DIM == 3 || r < 2)
+                xs[r] *= isc;
+        }
+        basisRow0<DIM, NDOF>(xs, mom, phi);
+        real inc[NV];
+        if (euler)
+        {
+            real uBL[5], n[3], ULf[5], uB[5];
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+            {
+                real r = 0;
+#pragma unroll
+                for (int j = 0; j < M; j++)
+                    r += phi[j] * rec(v, j);
+                uBL[v] = r;
+            }
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
+            const real *far = m.bcVal + (size_t)zone * 5;
+            const int type = m.bcType[zone];
+            compressRecPart(um, uBL, ULf);
+            bcGhost(type, ULf, n, far, m.bcGamma, uB);
+            if constexpr (LIN)
+            {
+                real in2[5], ULp[5], uBp[5];
+#pragma unroll
+                for (int v = 0; v < 5; v++)
+                {
+                    real r = 0;
+#pragma unroll
+                    for (int j = 0; j < M; j++)
+                        r += phi[j] * dx(v, j);
+                    in2[v] = uBL[v] + r;
+                }
+                compressRecPart(um, in2, ULp);
+                bcGhost(type, ULp, n, far, m.bcGamma, uBp);
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    inc[v] = uBp[v] - uB[v];
+            }
+            else
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    inc[v] = uB[v] - um[v];
+        }
+        else
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                inc[v] = LIN ? 0.0 : m.dirVal[((size_t)v * m.G + g) * F + f] - um[v];
+        const real fac = m.fwj[(size_t)g * F + f];
+#pragma unroll
+        for (int j = 0; j < M; j++)
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+                BCC[j][v] += ((phi[j] * inc[v]) * w2) * fac;
+    }
+}
+
 template <int DIM, int NDOF, int NV, int MODE>
 __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restrict__ uRec, const real *__restrict__ u,
                                                   const real *__restrict__ x, real *__restrict__ uRecNew, real scale)
@@ -233,7 +318,7 @@ __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restr
     if (t >= m.nBndCell)
         return;
     const idx i = m.bndCell[t];
-    const size_t N = m.N, F = m.F;
+    const size_t N = m.N;
     real mom[M], cen[3], um[NV];
 #pragma unroll
     for (int j = 0; j < M; j++)
@@ -245,86 +330,15 @@ __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restr
 #pragma unroll
     for (int v = 0; v < NV; v++)
         um[v] = u ? u[(size_t)v * N + i] : 0.0;
-    const bool euler = NV == 5 && m.bcType != nullptr;
+    auto rec = [&](int v, int j) { return uRec[((size_t)v * M + j) * N + i]; };
+    auto dx = [&](int v, int j) { return x[((size_t)v * M + j) * N + i]; };
     for (int k = 0; k < m.K; k++)
     {
         const idx f = m.c2f[(size_t)k * N + i];
         if (f < 0 || m.c2c[(size_t)k * N + i] >= 0)
             continue;
-        const int zone = m.fzone[f];
-        const real w0 = m.faceW[f]; // order-0 weight; faceL^0 = 1
-        const real w2 = w0 * w0;
         real BCC[M][NV];
-#pragma unroll
-        for (int j = 0; j < M; j++)
-#pragma unroll
-            for (int v = 0; v < NV; v++)
-                BCC[j][v] = 0;
-        for (int g = 0; g < m.G; g++)
-        {
-            real xs[3], phi[M];
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-            {
-                xs[r] = m.fpts[((size_t)g * 3 + r) * F + f] - cen[r];
-                if (DIM == 3 || r < 2)
-                    xs[r] *= isc;
-            }
-            basisRow0<DIM, NDOF>(xs, mom, phi);
-            real inc[NV];
-            if (euler)
-            {
-                real uBL[5], n[3], ULf[5], uB[5];
-#pragma unroll
-                for (int v = 0; v < 5; v++)
-                {
-                    real r = 0;
-#pragma unroll
-                    for (int j = 0; j < M; j++)
-                        r += phi[j] * uRec[((size_t)v * M + j) * N + i];
-                    uBL[v] = r;
-                }
-#pragma unroll
-                for (int r = 0; r < 3; r++)
-                    n[r] = m.fnorm[((size_t)g * 3 + r) * F + f];
-                const real *far = m.bcVal + (size_t)zone * 5;
-                const int type = m.bcType[zone];
-                compressRecPart(um, uBL, ULf);
-                bcGhost(type, ULf, n, far, m.bcGamma, uB);
-                if constexpr (MODE == 2)
-                {
-                    real in2[5], ULp[5], uBp[5];
-#pragma unroll
-                    for (int v = 0; v < 5; v++)
-                    {
-                        real r = 0;
-#pragma unroll
-                        for (int j = 0; j < M; j++)
-                            r += phi[j] * x[((size_t)v * M + j) * N + i];
-                        in2[v] = uBL[v] + r;
-                    }
-                    compressRecPart(um, in2, ULp);
-                    bcGhost(type, ULp, n, far, m.bcGamma, uBp);
-#pragma unroll
-                    for (int v = 0; v < NV; v++)
-                        inc[v] = uBp[v] - uB[v];
-                }
-                else
-#pragma unroll
-                    for (int v = 0; v < NV; v++)
-                        inc[v] = uB[v] - um[v];
-            }
-            else
-#pragma unroll
-                for (int v = 0; v < NV; v++)
-                    inc[v] = MODE == 2 ? 0.0 : m.dirVal[((size_t)v * m.G + g) * F + f] - um[v];
-            const real fac = m.fwj[(size_t)g * F + f];
-#pragma unroll
-            for (int j = 0; j < M; j++)
-#pragma unroll
-                for (int v = 0; v < NV; v++)
-                    BCC[j][v] += ((phi[j] * inc[v]) * w2) * fac;
-        }
+        boundaryFaceBCC<DIM, NDOF, NV, MODE == 2>(m, f, mom, cen, isc, um, rec, dx, BCC);
 #pragma unroll
         for (int v = 0; v < NV; v++)
 #pragma unroll
@@ -339,6 +353,102 @@ __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restr
     }
 }
 
+// ---------------------------------------------------------------------------
+// SOR form of the sweep (SORInstead = true, recordInc = false,
+// VariationalReconstruction_Reconstruction.hxx:534-589), multicolour: one launch per
+// colour, one thread per cell of the colour, the cell's block kept in registers and
+// updated in place.  Cells of one colour share no face, so a colour's neighbours are
+// not written during its launch and the sweep equals the serial SOR in colour order.
+// Per cell: uRec_i *= (1 - relax); per face in cell-face order
+//   interior: uRec_i += relax (A^-1 B_f uRec_j + A^-1 b_f (u_j - u_i))
+//   boundary: uRec_i += relax A^-1 BCC(uRec_i as accumulated so far), as the reference
+//             hands GetBoundaryRHS the array being updated (:584).
+// ---------------------------------------------------------------------------
+template <int DIM, int NDOF, int NV>
+__global__ void __launch_bounds__(64) kVRSOR(DevMesh m, const idx *__restrict__ cells, idx nCells,
+                                             real *__restrict__ uRec, const real *__restrict__ u, real relax)
+{
+    constexpr int M = NDOF - 1;
+    const idx t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nCells)
+        return;
+    const idx i = cells[t];
+    const size_t N = m.N;
+    real acc[NV][M], um[NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+    {
+        um[v] = u[(size_t)v * N + i];
+#pragma unroll
+        for (int r = 0; r < M; r++)
+            acc[v][r] = uRec[((size_t)v * M + r) * N + i] * (1 - relax);
+    }
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + i];
+        if (f < 0)
+            continue;
+        const idx o = m.c2c[(size_t)k * N + i];
+        if (o >= 0)
+        {
+            const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N;
+            const real *bv = m.Ainvb + ((size_t)k * M) * N;
+#pragma unroll
+            for (int v = 0; v < NV; v++)
+            {
+                const real du = u[(size_t)v * N + o] - um[v];
+                real nb[M];
+#pragma unroll
+                for (int q = 0; q < M; q++)
+                    nb[q] = uRec[((size_t)v * M + q) * N + o];
+#pragma unroll
+                for (int r = 0; r < M; r++)
+                {
+                    real s = 0;
+#pragma unroll
+                    for (int q = 0; q < M; q++)
+                        s += B[((size_t)r * M + q) * N + i] * nb[q];
+                    s += bv[(size_t)r * N + i] * du;
+                    acc[v][r] += relax * s;
+                }
+            }
+            continue;
+        }
+        real mom[M], cen[3];
+#pragma unroll
+        for (int j = 0; j < M; j++)
+            mom[j] = m.moment[(size_t)j * N + i];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cen[r] = m.cent[(size_t)r * N + i];
+        real BCC[M][NV];
+        auto rec = [&](int v, int j) { return acc[v][j]; };
+        boundaryFaceBCC<DIM, NDOF, NV, false>(m, f, mom, cen, m.iscale[i], um, rec, rec, BCC);
+        real d[NV][M];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int r = 0; r < M; r++)
+            {
+                real a = 0;
+#pragma unroll
+                for (int c2 = 0; c2 < M; c2++)
+                    a += m.AinvB[((size_t)r * M + c2) * N + i] * BCC[c2][v];
+                d[v][r] = a;
+            }
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int r = 0; r < M; r++)
+                acc[v][r] += relax * d[v][r];
+    }
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int r = 0; r < M; r++)
+            uRec[((size_t)v * M + r) * N + i] = acc[v][r];
+}
+
 template <int MODE, int DIM, int NDOF>
 static void vrBoundaryN(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
 {
@@ -349,16 +459,22 @@ static void vrBoundaryN(Ctx &c, int nVars, const real *uRec, const real *u, cons
         kVRBoundary<DIM, NDOF, 1, MODE><<<g, 64, 0, c.stream>>>(c.m, uRec, u, x, uRecNew, scale);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
+static void requireBoundaryValues(const DevMesh &m, int nVars)
+{
+    const bool euler = nVars == 5 && m.bcType != nullptr;
+    if (m.nBnd > 0 && !euler && !(m.dirVal && m.dirNV == nVars))
+        throw std::runtime_error("the mesh has boundary faces: set the boundary conditions (dndsr_set_bc, nVars 5) "
+                                 "or boundary point values (dndsr_set_vr_dirichlet) for this nVars");
+}
+
 template <int MODE>
 static void vrBoundary(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
 {
     const DevMesh &m = c.m;
     if (m.nBnd == 0)
         return;
+    requireBoundaryValues(m, nVars);
     const bool euler = nVars == 5 && m.bcType != nullptr;
-    if (!euler && !(m.dirVal && m.dirNV == nVars))
-        throw std::runtime_error("the mesh has boundary faces: set the boundary conditions (dndsr_set_bc, nVars 5) "
-                                 "or boundary point values (dndsr_set_vr_dirichlet) for this nVars");
     if (MODE == 2 && euler && (!uRec || !u))
         throw std::runtime_error("the boundary term of the VR Jacobian needs uRec and u");
     if (m.dim == 2)
@@ -398,6 +514,50 @@ void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real
         vrBoundary<0>(c, nVars, uRec, u, nullptr, uRecNew, relax);
     else if (mode == 1)
         vrBoundary<1>(c, nVars, uRec, u, nullptr, uRecNew, -1.0);
+}
+
+template <int DIM, int NDOF>
+static void vrSORN(Ctx &c, int nVars, real *uRec, const real *u, real relax)
+{
+    const DevMesh &m = c.m;
+    for (int col = 0; col < m.nColor; col++)
+    {
+        const idx b = m.colorPtrHost[col], n = m.colorPtrHost[col + 1] - b;
+        if (n == 0)
+            continue;
+        if (nVars == 5)
+            kVRSOR<DIM, NDOF, 5><<<gridFor(n, 64), 64, 0, c.stream>>>(m, m.colorCells + b, n, uRec, u, relax);
+        else
+            kVRSOR<DIM, NDOF, 1><<<gridFor(n, 64), 64, 0, c.stream>>>(m, m.colorCells + b, n, uRec, u, relax);
+        DNDSR_HIP_CHECK(hipGetLastError());
+    }
+}
+
+void launchVRSOR(Ctx &c, int nVars, real *uRec, const real *u, real relax)
+{
+    KernelTimer kt(c, K_VR_ITER);
+    const DevMesh &m = c.m;
+    if (nVars != 5 && nVars != 1)
+        throw std::runtime_error("nVars must be 1 or 5");
+    requireBoundaryValues(m, nVars);
+    if (m.dim == 2)
+    {
+        if (m.P == 1)
+            vrSORN<2, 3>(c, nVars, uRec, u, relax);
+        else if (m.P == 2)
+            vrSORN<2, 6>(c, nVars, uRec, u, relax);
+        else
+            vrSORN<2, 10>(c, nVars, uRec, u, relax);
+    }
+    else
+    {
+        if (m.P == 1)
+            vrSORN<3, 4>(c, nVars, uRec, u, relax);
+        else if (m.P == 2)
+            vrSORN<3, 10>(c, nVars, uRec, u, relax);
+        else
+            vrSORN<3, 20>(c, nVars, uRec, u, relax);
+    }
 }
 
 void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *y)

@@ -79,6 +79,10 @@ typedef struct
      * NULL = 1 for every zone and order */
     double bcWeight;
     const double *zoneDirWeight; /* host, [nBCZone][maxOrder+1] */
+    /* SORInstead (VRSettings.hpp:43): the solver loops (dndsr_ssprk3, dndsr_implicit_euler_step)
+     * sweep the reconstruction in place, multicolour SOR (dndsr_vr_iter mode 4), instead of
+     * by relaxed Jacobi.  0 = Jacobi. */
+    int SORInstead;
 } dndsr_vr_settings;
 
 /* EulerEvaluatorSettings subset (src/Euler/EulerEvaluatorSettings.hpp:53-80) */
@@ -145,10 +149,15 @@ int dndsr_d2h_cellfield(dndsr_ctx *ctx, const double *dev, double *hostRows, int
 /* DoReconstructionIter, Jacobi form (SORInstead = false):
  *   mode 0: uRecNew = (1-relax) uRec + relax sum_f (A^-1 B_f uRec_j + A^-1 b_f (u_j - u_i))
  *   mode 1: recordInc (uRecNew = uRec - sum_f ...), mode 2: recordInc with uRecIsZero.
+ * SOR form (SORInstead = true, putIntoNew = false; :534-589):
+ *   mode 4: uRec is updated in place, colour by colour (dndsr_get_colors), each cell as
+ *   uRec_i = (1-relax) uRec_i, then per face in cell-face order uRec_i += relax (...), the
+ *   boundary term evaluated on the uRec_i accumulated so far; uRecNew is not used (may be
+ *   NULL).  Equals the reference's serial sweep when cells are numbered colour by colour.
  * relax < 0 uses settings.jacobiRelax.
  * Replaces VariationalReconstruction<dim>::DoReconstructionIter,
  * src/CFV/VariationalReconstruction.hpp:984-992. */
-int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, const double *uRec, double *uRecNew, const double *u, int mode,
+int dndsr_vr_iter(dndsr_ctx *ctx, int nVars, double *uRec, double *uRecNew, const double *u, int mode,
                   double relax);
 /* DoReconstructionIterDiff: uRecNew = uRecDiff - sum_f A^-1 B_f uRecDiff_j - A^-1 BCdiff,
  * src/CFV/VariationalReconstruction.hpp:1012-1018.  The boundary term (GetBoundaryRHSDiff,

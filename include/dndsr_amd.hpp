@@ -189,11 +189,12 @@ struct VRSettings
     int nIterCellSmoothScale = 15;
     double bcWeight = 1;
     std::vector<double> zoneDirWeight; // [nBCZone][maxOrder+1]; empty = 1
+    bool SORInstead = false;           // multicolour in-place sweeps (the reference defaults to true)
     dndsr_vr_settings c() const
     {
         return {maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
                 scaleMultiplier, jacobiRelax, nIterCellSmoothScale, bcWeight,
-                zoneDirWeight.empty() ? nullptr : zoneDirWeight.data()};
+                zoneDirWeight.empty() ? nullptr : zoneDirWeight.data(), SORInstead ? 1 : 0};
     }
 };
 
@@ -238,13 +239,22 @@ class VariationalReconstruction
     void BuildUDof(ArrayDOF &u, int nVars) { construct(); u = ArrayDOF(ctx_, nCell_, nVars, 1); }
     void BuildURec(ArrayREC &uRec, int nVars) { construct(); uRec = ArrayREC(ctx_, nCell_, nVars, M_); }
 
-    // DoReconstructionIter (VariationalReconstruction.hpp:984-992), Jacobi form:
-    // putIntoNew = false leaves the new iterate in uRec; recordInc writes -residual to uRecNew.
+    // DoReconstructionIter (VariationalReconstruction.hpp:984-992): putIntoNew = false leaves
+    // the new iterate in uRec, true in uRecNew; recordInc writes -residual to uRecNew.  With
+    // SORInstead (and no recordInc) uRec is swept in place and, for putIntoNew, the two arrays
+    // are swapped as the reference does (:602-606).
     void DoReconstructionIter(ArrayREC &uRec, ArrayREC &uRecNew, ArrayDOF &u, bool putIntoNew = false,
                               bool recordInc = false, bool uRecIsZero = false, double relax = -1)
     {
         if (recordInc && !putIntoNew)
             throw std::runtime_error("dndsr_amd: recordInc needs putIntoNew");
+        if (s_.SORInstead && !recordInc)
+        {
+            check(dndsr_vr_iter(ctx_.handle(), u.nVars(), uRec.data(), nullptr, u.data(), 4, relax));
+            if (putIntoNew)
+                std::swap(uRec, uRecNew);
+            return;
+        }
         const int mode = !recordInc ? 0 : (uRecIsZero ? 2 : 1);
         check(dndsr_vr_iter(ctx_.handle(), u.nVars(), uRec.data(), uRecNew.data(), u.data(), mode, relax));
         if (!putIntoNew)

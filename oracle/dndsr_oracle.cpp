@@ -1820,6 +1820,79 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
     ORC_CATCH(-1)
 }
 
+// DoReconstructionIter, SOR branch (SORInstead = true, recordInc = false),
+// VariationalReconstruction_Reconstruction.hxx:534-589: cells in scan order, each
+// updated in place: uRec_i *= (1 - relax), then per face in cell2face order
+//   interior: uRec_i += relax (A^-1 B_f uRec_j + A^-1 b_f (u_j - u_i)), uRec_j as it stands
+//   boundary: uRec_i += relax A^-1 BCC, GetBoundaryRHS evaluated on the uRec_i accumulated so
+//             far (the reference passes the array being updated, :584)
+// scanOrder NULL = natural order.
+int orc_vr_sor(const orc_vr *h, int nVars, double *uRec, const double *u, const int64_t *scanOrder)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    const int M = v.M;
+    const size_t blkN = (size_t)M * nVars;
+    const real relax = v.s.jacobiRelax;
+    std::vector<real> BCC(blkN), nb(blkN);
+    for (idx_t s = 0; s < v.nCell; s++)
+    {
+        const idx_t i = scanOrder ? scanOrder[s] : s;
+        if (i < 0 || i >= v.nCell)
+            throw std::runtime_error("scanOrder entry out of range");
+        real *out = uRec + i * blkN;
+        for (size_t q = 0; q < blkN; q++)
+            out[q] *= 1 - relax;
+        const real *blk = &v.AinvB[(size_t)i * (v.maxFC + 1) * M * M];
+        for (size_t ic = 0; ic < v.c2f[i].size(); ic++)
+        {
+            const idx_t f = v.c2f[i][ic];
+            const idx_t o = v.cellFaceOther(i, f);
+            if (o < 0)
+            {
+                std::fill(BCC.begin(), BCC.end(), 0.0);
+                boundaryBCC(v, i, f, nVars, uRec, u, nullptr, BCC);
+                addAinvBCC(v, i, nVars, BCC, relax, out);
+                continue;
+            }
+            const real *B = blk + (1 + ic) * (size_t)M * M;
+            const real *bv = &v.Ainvb[((size_t)i * v.maxFC + ic) * M];
+            std::fill(nb.begin(), nb.end(), 0.0);
+            for (int var = 0; var < nVars; var++)
+            {
+                const real du = u[o * nVars + var] - u[i * nVars + var];
+                for (int r = 0; r < M; r++)
+                {
+                    real acc = 0;
+                    for (int k = 0; k < M; k++)
+                        acc += B[(size_t)r * M + k] * uRec[o * blkN + (size_t)var * M + k];
+                    acc += bv[r] * du;
+                    nb[(size_t)var * M + r] = acc;
+                }
+            }
+            for (size_t q = 0; q < blkN; q++)
+                out[q] += relax * nb[q];
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// One reconstruction sweep of the solver loops (frhs, EulerSolver.hxx:240-250, putIntoNew =
+// false): in place in scan order when SORInstead, else Jacobi into tmp and copied back.
+static void vrSweep(const orc_vr *h, double *uRec, const double *u, const int64_t *scanOrder, std::vector<real> &tmp)
+{
+    if (h->v.s.SORInstead)
+    {
+        if (orc_vr_sor(h, 5, uRec, u, scanOrder))
+            throw std::runtime_error(g_err);
+        return;
+    }
+    if (orc_vr_iter(h, 5, uRec, u, tmp.data()))
+        throw std::runtime_error(g_err);
+    std::copy(tmp.begin(), tmp.end(), uRec);
+}
+
 // DoReconstructionIterDiff, VariationalReconstruction_Reconstruction.hxx:617-670
 int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const double *u, const double *uRecDiff,
                      double *uRecNew)
@@ -2404,11 +2477,7 @@ int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double 
     std::vector<real> dtv(N), lam((size_t)v.nFace * 4), dlc(N), rhs(N * 5), xLast(N * 5), tmp(nR);
     auto frhs = [&](const real *x) {
         for (int r = 0; r < nRec; r++)
-        {
-            if (orc_vr_iter(h, 5, uRec, x, tmp.data()))
-                throw std::runtime_error(g_err);
-            std::copy(tmp.begin(), tmp.end(), uRec);
-        }
+            vrSweep(h, uRec, x, nullptr, tmp);
         if (orc_eval_rhs(h, es, x, uRec, dlc.data(), rhs.data()))
             throw std::runtime_error(g_err);
     };
@@ -2461,11 +2530,7 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
         if (orc_eval_dt(h, es, u, is->CFL, 1e100, 1, dTau.data(), lam.data(), dlc.data(), &dtMin))
             throw std::runtime_error(g_err);
         for (int r = 0; r < is->nRec; r++)
-        {
-            if (orc_vr_iter(h, 5, uRec, u, tmp.data()))
-                throw std::runtime_error(g_err);
-            std::copy(tmp.begin(), tmp.end(), uRec);
-        }
+            vrSweep(h, uRec, u, scanOrder, tmp);
         if (orc_eval_rhs(h, es, u, uRec, dlc.data(), rhsF.data()))
             throw std::runtime_error(g_err);
         // crhs = xLast / dt - x / dt + frhs

@@ -62,6 +62,9 @@ typedef struct
      * VariationalReconstruction.cpp:438-441); NULL = 1 for every zone and order */
     double bcWeight;
     const double *zoneDirWeight; /* [nBCZone][maxOrder+1] */
+    /* the solver loops (orc_ssprk3, orc_implicit_euler_step) sweep the reconstruction in
+     * place (orc_vr_sor) instead of by Jacobi */
+    int SORInstead;
 } orc_vr_settings;
 
 typedef struct
@@ -106,6 +109,8 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
 
 /* VR iteration: one Jacobi sweep uRecNew = (1-w) uRec + w * sum(...) */
 int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew);
+/* VR iteration, SOR form: uRec swept in place, cells in scanOrder (NULL = natural order) */
+int orc_vr_sor(const orc_vr *h, int nVars, double *uRec, const double *u, const int64_t *scanOrder);
 /* uRecNew = uRecDiff - sum_f AinvB_f * uRecDiff_j - A^-1 BCdiff (VR Jacobian action); the
  * boundary term needs the base state uRec, u (may be NULL on meshes without boundary faces) */
 int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const double *u, const double *uRecDiff,
@@ -143,7 +148,8 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
 /* LUSGSMatrixVec (scalar Jacobian): Ax = D x + sum_f (alpha/2) A_f / V fluxJacobian0(u_j) x_j */
 int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
                      const double *x, const double *jdiag, const double *lambdaFace, double *Ax);
-/* full explicit SSP-RK3 steps: nStep steps of (EvaluateDt once, 3 x (nRec VR + RHS)) */
+/* full explicit SSP-RK3 steps: nStep steps of (EvaluateDt once, 3 x (nRec VR + RHS)); the
+ * VR sweeps run in natural cell order when SORInstead */
 int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,
                double CFL, int nRec, double dtFixed);
 
