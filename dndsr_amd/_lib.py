@@ -30,7 +30,8 @@ class VRSettings(C.Structure):
                 ("geomWeightScheme", C.c_int), ("geomWeightBias", C.c_double),
                 ("geomWeightPower", C.c_double), ("scaleMultiplier", C.c_double),
                 ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int), ("bcWeight", C.c_double),
-                ("zoneDirWeight", dp), ("SORInstead", C.c_int)]
+                ("zoneDirWeight", dp), ("SORInstead", C.c_int), ("subs2ndOrder", C.c_int),
+                ("subs2ndOrderGGScheme", C.c_int)]
 
 
 class EulerSettings(C.Structure):

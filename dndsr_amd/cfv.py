@@ -65,7 +65,8 @@ class VariationalReconstruction:
     def __init__(self, mesh, settings: dict | None = None, device: int = 0, ctx: Context | None = None):
         s = dict(maxOrder=1, intOrder=1, dirWeightScheme="Factorial", geomWeightScheme="GWNone",
                  geomWeightBias=0.0, geomWeightPower=0.5, scaleMultiplier=1.0, jacobiRelax=1.0,
-                 nIterCellSmoothScale=15, bcWeight=1.0, zoneDirWeight=None, SORInstead=False)
+                 nIterCellSmoothScale=15, bcWeight=1.0, zoneDirWeight=None, SORInstead=False, subs2ndOrder=0,
+                 subs2ndOrderGGScheme=1)
         if settings:
             s.update(settings)
             if "functionalSettings" in settings:
@@ -108,7 +109,8 @@ class VariationalReconstruction:
         vs = _lib.VRSettings(s["maxOrder"], s["intOrder"], s["dirWeightScheme"], s["geomWeightScheme"],
                              s["geomWeightBias"], s["geomWeightPower"], s["scaleMultiplier"], s["jacobiRelax"],
                              s["nIterCellSmoothScale"], s["bcWeight"],
-                             None if zw is None else zw.ctypes.data_as(_lib.dp), int(bool(s["SORInstead"])))
+                             None if zw is None else zw.ctypes.data_as(_lib.dp), int(bool(s["SORInstead"])),
+                             int(s["subs2ndOrder"]), int(s["subs2ndOrderGGScheme"]))
         check(lib().dndsr_vr_construct(self.ctx.h, C.byref(md), C.byref(vs)))
         nC, nF = C.c_int64(), C.c_int64()
         M, K, G, Gc, nCol = C.c_int(), C.c_int(), C.c_int(), C.c_int(), C.c_int()

@@ -215,7 +215,10 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.nNode = (idx)md->nNode;
     m.s = VRSettingsDev{s->maxOrder, s->intOrder, s->dirWeightScheme, s->geomWeightScheme, s->geomWeightBias,
                         s->geomWeightPower, s->scaleMultiplier, s->jacobiRelax, s->nIterCellSmoothScale,
-                        s->bcWeight, s->SORInstead != 0};
+                        s->bcWeight, s->SORInstead != 0, s->subs2ndOrder, s->subs2ndOrderGGScheme};
+    need(s->subs2ndOrder == 0 || s->subs2ndOrder == 1 || s->subs2ndOrder == 2,
+         "subs2ndOrder must be 0, 1 (Gauss rule) or 2 (least squares); 11 (GGMP) is not provided");
+    need(s->subs2ndOrderGGScheme == 0 || s->subs2ndOrderGGScheme == 1, "subs2ndOrderGGScheme must be 0 or 1");
     need(md->nBCZone >= 0 && md->nBCZone < 4096, "nBCZone out of range");
     m.nBCZone = md->nBCZone;
     const size_t N = m.N, F = m.F;
@@ -880,7 +883,9 @@ int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int res
 // false): SOR in place when SORInstead, else Jacobi into recNew and the two swapped.
 static void solverVRSweep(Ctx &c, real *&rec, real *&recNew, const real *u)
 {
-    if (c.m.s.SORInstead)
+    if (usesSubs2nd(c.m)) // :503-512: the substitute reads only u
+        launchVR2nd(c, 5, u, rec);
+    else if (c.m.s.SORInstead)
         launchVRSOR(c, 5, rec, u, c.m.s.jacobiRelax);
     else
     {
@@ -1188,6 +1193,7 @@ int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const d
     need(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
     need(mode == 3 || uBar != nullptr, "uBar is required for modes 0-2");
     Ctx &c = ctx->c;
+    need(!usesSubs2nd(c.m), "the transpose of the second-order substitute (subs2ndOrder) is not provided");
     launchVRIterVJP(c, nVars, mode, relax < 0 ? c.m.s.jacobiRelax : relax, uRec, u, uRecNewBar, uRecBar,
                     mode == 3 ? nullptr : uBar);
     CAPI_END

@@ -41,6 +41,8 @@ struct VRSettingsDev
     int nIterCellSmoothScale;
     real bcWeight;
     bool SORInstead; // solver loops sweep the reconstruction in place (multicolour SOR)
+    int subs2ndOrder;         // maxOrder 1: 0 VR, 1 Gauss rule, 2 least squares
+    int subs2ndOrderGGScheme; // 1 distance-weighted face value, 0 plain average
 };
 
 // Device-resident mesh, metrics and VR coefficients (all arrays SoA, see dev_common.hpp).
@@ -194,6 +196,8 @@ void setupMetricsAndVR(Ctx &c);
 // ---- launchers (vr.hip)
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax);
 void launchVRSOR(Ctx &c, int nVars, real *uRec, const real *u, real relax);
+bool usesSubs2nd(const DevMesh &m);
+void launchVR2nd(Ctx &c, int nVars, const real *u, real *out);
 void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *uRecDiff, real *uRecNew);
 // ---- launchers (euler.hip)
 void launchEvalDt(Ctx &c, const GasParams &gp, const real *u, real CFL, real maxDt, int useLocalDt, real *dt,

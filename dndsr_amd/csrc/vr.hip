@@ -1,7 +1,8 @@
 // vr.hip -- variational-reconstruction iteration kernels.
 //
-// DoReconstructionIter, Jacobi branch (SORInstead = false), SOR branch and the
-// recordInc modes, src/CFV/VariationalReconstruction_Reconstruction.hxx:485-615, and
+// DoReconstructionIter, Jacobi branch (SORInstead = false), SOR branch, the recordInc
+// modes and the second-order substitute (subs2ndOrder),
+// src/CFV/VariationalReconstruction_Reconstruction.hxx:184-615, and
 // DoReconstructionIterDiff, :617-670.
 #include "gas.hpp"
 #include "internal.hpp"
@@ -449,6 +450,188 @@ __global__ void __launch_bounds__(64) kVRSOR(DevMesh m, const idx *__restrict__ 
             uRec[((size_t)v * M + r) * N + i] = acc[v][r];
 }
 
+// ---------------------------------------------------------------------------
+// Second-order substitute of the sweep (maxOrder = 1, subs2ndOrder != 0,
+// DoReconstruction2ndGrad + DoReconstruction2nd, VariationalReconstruction_Reconstruction.hxx:
+// 184-483): one thread per cell, a gradient from the means only.
+//   1 (Gauss rule): g = (1/V) sum_f (u_j - u_i) lThis / (lThis + lThat) A_f n_f, a boundary
+//     face adding (uB - u_i) A_f n_f / 2 (uB: ghost of u_i at the face centre, mean normal);
+//     lThis = |x_f - bary_i|, lThat = |bary_j - x_f|, both 1 for subs2ndOrderGGScheme 0
+//   2 (least squares): the normal equations of min sum_f |dx_f g^T - du_f|^2, dx_f = bary_j -
+//     bary_i (boundary: the mirrored barycentre, -2 ((bary_i - x_f).n) n), du_f = u_j - u_i
+//     (boundary: uB - u_i)
+// and uRec_i = g^T times the cell's length scale (the first derivatives of the linear modes
+// of the scaled Taylor basis are 1 / scale).
+// ---------------------------------------------------------------------------
+template <int DIM, int NV>
+__global__ void __launch_bounds__(64) kVR2nd(DevMesh m, const real *__restrict__ u, real *__restrict__ out)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N)
+        return;
+    const size_t N = m.N, F = m.F;
+    const int method = m.s.subs2ndOrder;
+    const bool plain = m.s.subs2ndOrderGGScheme == 0;
+    real ui[NV], bi[3], g[NV][DIM], S[DIM][DIM], T[DIM][NV];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+        ui[v] = u[(size_t)v * N + i];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+        bi[r] = m.bary[(size_t)r * N + i];
+#pragma unroll
+    for (int d = 0; d < DIM; d++)
+    {
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            g[v][d] = T[d][v] = 0;
+#pragma unroll
+        for (int e = 0; e < DIM; e++)
+            S[d][e] = 0;
+    }
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + i];
+        if (f < 0)
+            continue;
+        const idx o = m.c2c[(size_t)k * N + i];
+        const bool back = m.f2c[f] == i;
+        real xf[3], n[3], uB[NV];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+        {
+            xf[r] = m.fcent[(size_t)r * F + f] + (back ? 0.0 : m.fshift[(size_t)r * F + f]);
+            n[r] = m.fmeanN[(size_t)r * F + f];
+        }
+        const real A = m.farea[f];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+            uB[v] = ui[v];
+        if (o < 0)
+        {
+            if constexpr (NV == 5)
+            {
+                const int zone = m.fzone[f];
+                bcGhost(m.bcType[zone], ui, n, m.bcVal + (size_t)zone * 5, m.bcGamma, uB);
+            }
+        }
+        real ob[3] = {0, 0, 0};
+        if (o >= 0)
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                ob[r] = m.bary[(size_t)r * N + o] + (back ? -m.fshift[(size_t)r * F + f] : m.fshift[(size_t)r * F + f]);
+        if (method == 1)
+        {
+            real lThis = 0;
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                lThis += (xf[r] - bi[r]) * (xf[r] - bi[r]);
+            lThis = plain ? 1.0 : sqrt(lThis);
+            if (o >= 0)
+            {
+                real lThat = 0;
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    lThat += (ob[r] - xf[r]) * (ob[r] - xf[r]);
+                lThat = plain ? 1.0 : sqrt(lThat);
+                const real sgn = back ? 1.0 : -1.0;
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        g[v][d] += (u[(size_t)v * N + o] - ui[v]) * lThis / (lThis + lThat + kVerySmall) * A *
+                                   (n[d] * sgn);
+            }
+            else
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+#pragma unroll
+                    for (int d = 0; d < DIM; d++)
+                        g[v][d] += (uB[v] - ui[v]) * 0.5 * A * n[d];
+        }
+        else
+        {
+            real dc[DIM], du[NV];
+            if (o >= 0)
+            {
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    dc[d] = ob[d] - bi[d];
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    du[v] = u[(size_t)v * N + o] - ui[v];
+            }
+            else
+            {
+                real pr = 0;
+#pragma unroll
+                for (int r = 0; r < 3; r++)
+                    pr += (bi[r] - m.fcent[(size_t)r * F + f]) * n[r];
+#pragma unroll
+                for (int d = 0; d < DIM; d++)
+                    dc[d] = pr * -2.0 * n[d];
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    du[v] = uB[v] - ui[v];
+            }
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+            {
+#pragma unroll
+                for (int e = 0; e < DIM; e++)
+                    S[d][e] += dc[d] * dc[e];
+#pragma unroll
+                for (int v = 0; v < NV; v++)
+                    T[d][v] += dc[d] * du[v];
+            }
+        }
+    }
+    if (method == 1)
+    {
+        const real V = m.vol[i];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                g[v][d] /= V;
+    }
+    else if constexpr (DIM == 2)
+    {
+        const real det = S[0][0] * S[1][1] - S[0][1] * S[1][0];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+        {
+            g[v][0] = (S[1][1] * T[0][v] - S[0][1] * T[1][v]) / det;
+            g[v][1] = (S[0][0] * T[1][v] - S[1][0] * T[0][v]) / det;
+        }
+    }
+    else
+    {
+        real C[3][3]; // adjugate of S
+        C[0][0] = S[1][1] * S[2][2] - S[1][2] * S[2][1];
+        C[0][1] = S[0][2] * S[2][1] - S[0][1] * S[2][2];
+        C[0][2] = S[0][1] * S[1][2] - S[0][2] * S[1][1];
+        C[1][0] = S[1][2] * S[2][0] - S[1][0] * S[2][2];
+        C[1][1] = S[0][0] * S[2][2] - S[0][2] * S[2][0];
+        C[1][2] = S[0][2] * S[1][0] - S[0][0] * S[1][2];
+        C[2][0] = S[1][0] * S[2][1] - S[1][1] * S[2][0];
+        C[2][1] = S[0][1] * S[2][0] - S[0][0] * S[2][1];
+        C[2][2] = S[0][0] * S[1][1] - S[0][1] * S[1][0];
+        const real det = S[0][0] * C[0][0] + S[0][1] * C[1][0] + S[0][2] * C[2][0];
+#pragma unroll
+        for (int v = 0; v < NV; v++)
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                g[v][d] = (C[d][0] * T[0][v] + C[d][1] * T[1][v] + C[d][2] * T[2][v]) / det;
+    }
+    const real isc = m.iscale[i];
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int d = 0; d < DIM; d++)
+            out[((size_t)v * DIM + d) * N + i] = g[v][d] / isc;
+}
+
 template <int MODE, int DIM, int NDOF>
 static void vrBoundaryN(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
 {
@@ -499,6 +682,11 @@ static void vrBoundary(Ctx &c, int nVars, const real *uRec, const real *u, const
 
 void launchVRIter(Ctx &c, int nVars, const real *uRec, real *uRecNew, const real *u, int mode, real relax)
 {
+    if (usesSubs2nd(c.m)) // :503-512, every mode
+    {
+        launchVR2nd(c, nVars, u, uRecNew);
+        return;
+    }
     KernelTimer kt(c, K_VR_ITER);
     switch (c.m.M)
     {
@@ -535,6 +723,11 @@ static void vrSORN(Ctx &c, int nVars, real *uRec, const real *u, real relax)
 
 void launchVRSOR(Ctx &c, int nVars, real *uRec, const real *u, real relax)
 {
+    if (usesSubs2nd(c.m))
+    {
+        launchVR2nd(c, nVars, u, uRec);
+        return;
+    }
     KernelTimer kt(c, K_VR_ITER);
     const DevMesh &m = c.m;
     if (nVars != 5 && nVars != 1)
@@ -560,9 +753,45 @@ void launchVRSOR(Ctx &c, int nVars, real *uRec, const real *u, real relax)
     }
 }
 
+bool usesSubs2nd(const DevMesh &m) { return m.P == 1 && m.s.subs2ndOrder != 0; }
+
+void launchVR2nd(Ctx &c, int nVars, const real *u, real *out)
+{
+    KernelTimer kt(c, K_VR_ITER);
+    const DevMesh &m = c.m;
+    if (!usesSubs2nd(m))
+        throw std::runtime_error("the second-order substitute needs maxOrder 1 and subs2ndOrder 1 or 2");
+    if (nVars != 5 && nVars != 1)
+        throw std::runtime_error("nVars must be 1 or 5");
+    if (m.nBnd > 0 && !(nVars == 5 && m.bcType != nullptr))
+        throw std::runtime_error("the second-order substitute needs the Euler boundary table (dndsr_set_bc, "
+                                 "nVars 5) on meshes with boundary faces");
+    const int g = gridFor(m.N, 64);
+    if (m.dim == 2)
+    {
+        if (nVars == 5)
+            kVR2nd<2, 5><<<g, 64, 0, c.stream>>>(m, u, out);
+        else
+            kVR2nd<2, 1><<<g, 64, 0, c.stream>>>(m, u, out);
+    }
+    else
+    {
+        if (nVars == 5)
+            kVR2nd<3, 5><<<g, 64, 0, c.stream>>>(m, u, out);
+        else
+            kVR2nd<3, 1><<<g, 64, 0, c.stream>>>(m, u, out);
+    }
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
 void launchVRIterDiff(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *y)
 {
     KernelTimer kt(c, K_VR_ITER);
+    if (usesSubs2nd(c.m)) // :631-638: the substitute does not depend on uRec
+    {
+        DNDSR_HIP_CHECK(hipMemsetAsync(y, 0, sizeof(real) * (size_t)nVars * c.m.M * c.m.N, c.stream));
+        return;
+    }
     switch (c.m.M)
     {
     case 2: vrIterDiffM<2>(c, nVars, x, y); break;

@@ -83,6 +83,13 @@ typedef struct
      * sweep the reconstruction in place, multicolour SOR (dndsr_vr_iter mode 4), instead of
      * by relaxed Jacobi.  0 = Jacobi. */
     int SORInstead;
+    /* subs2ndOrder (VRSettings.hpp:49-50), maxOrder 1 only: 0 = VR; 1 = Gauss rule, 2 = least
+     * squares: every VR sweep (dndsr_vr_iter, any mode, and the solver loops) is replaced by a
+     * gradient of the means (DoReconstruction2nd, VariationalReconstruction_Reconstruction.hxx:
+     * 184-483; GGMP = 11 is rejected) and dndsr_vr_iter_diff returns zeros.
+     * subs2ndOrderGGScheme: 1 = distance-weighted face value, 0 = plain average. */
+    int subs2ndOrder;
+    int subs2ndOrderGGScheme;
 } dndsr_vr_settings;
 
 /* EulerEvaluatorSettings subset (src/Euler/EulerEvaluatorSettings.hpp:53-80) */

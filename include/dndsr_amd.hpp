@@ -190,11 +190,13 @@ struct VRSettings
     double bcWeight = 1;
     std::vector<double> zoneDirWeight; // [nBCZone][maxOrder+1]; empty = 1
     bool SORInstead = false;           // multicolour in-place sweeps (the reference defaults to true)
+    int subs2ndOrder = 0, subs2ndOrderGGScheme = 1;
     dndsr_vr_settings c() const
     {
         return {maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
                 scaleMultiplier, jacobiRelax, nIterCellSmoothScale, bcWeight,
-                zoneDirWeight.empty() ? nullptr : zoneDirWeight.data(), SORInstead ? 1 : 0};
+                zoneDirWeight.empty() ? nullptr : zoneDirWeight.data(), SORInstead ? 1 : 0, subs2ndOrder,
+                subs2ndOrderGGScheme};
     }
 };
 

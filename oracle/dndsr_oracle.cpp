@@ -1748,6 +1748,214 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
     return err / v.globalVol;
 }
 
+// Second-order substitute of the VR sweep (maxOrder = 1, subs2ndOrder != 0):
+// DoReconstruction2ndGrad + DoReconstruction2nd,
+// VariationalReconstruction_Reconstruction.hxx:184-483.  Per cell a gradient g (nVars x dim)
+//   subs2ndOrder 1 (Gauss rule): g = (1/V) sum_f (u_j - u_i) lThis / (lThis + lThat) A_f n_f
+//     with lThis = |x_f - bary_i|, lThat = |bary_j - x_f| (both 1 with subs2ndOrderGGScheme 0),
+//     n_f the mean normal out of the cell, x_f the face centre; a boundary face adds
+//     (uB - u_i) A_f n_f / 2 with uB the ghost of u_i at the face centre (mean normal)
+//   subs2ndOrder 2 (least squares): g = argmin sum_f |dx_f g^T - du_f|^2 with dx_f = bary_j -
+//     bary_i, du_f = u_j - u_i, and at a boundary face dx_f = -2 ((bary_i - x_f).n) n,
+//     du_f = uB - u_i (solved here by Householder QR, the minimum-norm solution for full rank)
+// then uRec_i (first dim modes) = D1phi^-1 g^T, D1phi the first derivatives of the linear
+// modes (partial-pivot elimination).
+static void vr2ndCell(const VR &v, idx_t i, int nVars, const real *u, real *out)
+{
+    const int dim = v.dim, M = v.M;
+    const int method = v.s.subs2ndOrder;
+    const bool euler = !v.bcType.empty() && nVars == 5;
+    const idx_t nf = (idx_t)v.c2f[i].size();
+    std::vector<real> grad((size_t)nVars * dim, 0.0);    // grad[var * dim + d]
+    std::vector<real> dcs((size_t)nf * dim), dus((size_t)nf * nVars);
+    auto ghost = [&](idx_t f, real *uB) {
+        if (!euler)
+            throw std::runtime_error("the second-order substitute needs the Euler boundary table (nVars = 5) on "
+                                     "meshes with boundary faces");
+        const int zone = v.faceZone[f];
+        bcGhost(v.bcType[zone], u + i * 5, v.faceMeanNorm[f].x, v.bcValue[zone].data(), v.bcGamma, uB);
+    };
+    for (idx_t ic = 0; ic < nf; ic++)
+    {
+        const idx_t f = v.c2f[i][ic];
+        const idx_t o = v.cellFaceOther(i, f);
+        const bool back = v.cellIsBack(i, f);
+        Pt xf = v.faceCent[f];
+        if (!back)
+            for (int d = 0; d < 3; d++)
+                xf.x[d] += v.fshift[f].x[d];
+        const real *n = v.faceMeanNorm[f].x;
+        const real area = v.area[f];
+        real uB[5];
+        if (o < 0)
+            ghost(f, uB);
+        if (method == 1)
+        {
+            real lThis = 0;
+            for (int d = 0; d < 3; d++)
+                lThis += sqr(xf.x[d] - v.bary[i].x[d]);
+            lThis = v.s.subs2ndOrderGGScheme == 0 ? 1.0 : std::sqrt(lThis);
+            if (o >= 0)
+            {
+                const Pt ob = v.otherBaryFromCell(i, o, f);
+                real lThat = 0;
+                for (int d = 0; d < 3; d++)
+                    lThat += sqr(ob.x[d] - xf.x[d]);
+                lThat = v.s.subs2ndOrderGGScheme == 0 ? 1.0 : std::sqrt(lThat);
+                const real sgn = back ? 1.0 : -1.0;
+                for (int var = 0; var < nVars; var++)
+                    for (int d = 0; d < dim; d++)
+                        grad[(size_t)var * dim + d] += (u[o * nVars + var] - u[i * nVars + var]) * lThis /
+                                                       (lThis + lThat + verySmallReal) * area * (n[d] * sgn);
+            }
+            else
+                for (int var = 0; var < nVars; var++)
+                    for (int d = 0; d < dim; d++)
+                        grad[(size_t)var * dim + d] += (uB[var] - u[i * nVars + var]) * 0.5 * area * n[d];
+        }
+        else
+        {
+            if (o >= 0)
+            {
+                const Pt ob = v.otherBaryFromCell(i, o, f);
+                for (int d = 0; d < dim; d++)
+                    dcs[(size_t)ic * dim + d] = ob.x[d] - v.bary[i].x[d];
+                for (int var = 0; var < nVars; var++)
+                    dus[(size_t)ic * nVars + var] = u[o * nVars + var] - u[i * nVars + var];
+            }
+            else
+            {
+                real pr = 0;
+                for (int d = 0; d < 3; d++)
+                    pr += (v.bary[i].x[d] - v.faceCent[f].x[d]) * n[d];
+                for (int d = 0; d < dim; d++)
+                    dcs[(size_t)ic * dim + d] = pr * -2.0 * n[d];
+                for (int var = 0; var < nVars; var++)
+                    dus[(size_t)ic * nVars + var] = uB[var] - u[i * nVars + var];
+            }
+        }
+    }
+    if (method == 1)
+        for (auto &g : grad)
+            g /= v.vol[i];
+    else
+    {
+        // Householder QR of dcs (nf x dim), applied to dus; back substitution
+        std::vector<real> A(dcs), B(dus);
+        for (int k = 0; k < dim; k++)
+        {
+            real nrm = 0;
+            for (idx_t r = k; r < nf; r++)
+                nrm += sqr(A[(size_t)r * dim + k]);
+            nrm = std::sqrt(nrm);
+            if (nrm == 0)
+                throw std::runtime_error("least-squares gradient: rank-deficient stencil");
+            const real alpha = A[(size_t)k * dim + k] > 0 ? -nrm : nrm;
+            std::vector<real> w(nf, 0.0);
+            for (idx_t r = k; r < nf; r++)
+                w[r] = A[(size_t)r * dim + k];
+            w[k] -= alpha;
+            real wn = 0;
+            for (idx_t r = k; r < nf; r++)
+                wn += w[r] * w[r];
+            if (wn == 0)
+                continue;
+            auto reflect = [&](real *col, int stride, int ncol) {
+                for (int c = 0; c < ncol; c++)
+                {
+                    real dot = 0;
+                    for (idx_t r = k; r < nf; r++)
+                        dot += w[r] * col[(size_t)r * stride + c];
+                    const real fct = 2 * dot / wn;
+                    for (idx_t r = k; r < nf; r++)
+                        col[(size_t)r * stride + c] -= fct * w[r];
+                }
+            };
+            reflect(A.data(), dim, dim);
+            reflect(B.data(), nVars, nVars);
+        }
+        for (int var = 0; var < nVars; var++)
+            for (int k = dim - 1; k >= 0; k--)
+            {
+                real acc = B[(size_t)k * nVars + var];
+                for (int c = k + 1; c < dim; c++)
+                    acc -= A[(size_t)k * dim + c] * grad[(size_t)var * dim + c];
+                grad[(size_t)var * dim + k] = acc / A[(size_t)k * dim + k];
+            }
+    }
+    // D1phi (dim x dim): rows d/dx_d, columns the linear modes; solve D1phi X = g^T
+    std::vector<real> D;
+    v.diffBase(i, v.bary[i], 1 + dim, 1, D);
+    real L[3][3], R[3][5];
+    for (int d = 0; d < dim; d++)
+    {
+        for (int e = 0; e < dim; e++)
+            L[d][e] = D[(size_t)(1 + d) * v.NDOF + 1 + e];
+        for (int var = 0; var < nVars; var++)
+            R[d][var] = grad[(size_t)var * dim + d];
+    }
+    for (int k = 0; k < dim; k++)
+    {
+        int piv = k;
+        for (int r = k + 1; r < dim; r++)
+            if (std::fabs(L[r][k]) > std::fabs(L[piv][k]))
+                piv = r;
+        for (int c = 0; c < dim; c++)
+            std::swap(L[k][c], L[piv][c]);
+        for (int var = 0; var < nVars; var++)
+            std::swap(R[k][var], R[piv][var]);
+        for (int r = k + 1; r < dim; r++)
+        {
+            const real fct = L[r][k] / L[k][k];
+            for (int c = k; c < dim; c++)
+                L[r][c] -= fct * L[k][c];
+            for (int var = 0; var < nVars; var++)
+                R[r][var] -= fct * R[k][var];
+        }
+    }
+    for (int q = 0; q < M * nVars; q++)
+        out[q] = 0;
+    for (int var = 0; var < nVars; var++)
+        for (int k = dim - 1; k >= 0; k--)
+        {
+            real acc = R[k][var];
+            for (int c = k + 1; c < dim; c++)
+                acc -= L[k][c] * out[(size_t)var * M + c];
+            out[(size_t)var * M + k] = acc / L[k][k];
+        }
+}
+
+int orc_vr_2nd(const orc_vr *h, int nVars, const double *u, double *uRec)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    if (v.s.maxOrder != 1 || (v.s.subs2ndOrder != 1 && v.s.subs2ndOrder != 2))
+        throw std::runtime_error("the second-order substitute needs maxOrder 1 and subs2ndOrder 1 or 2");
+    if (nVars > 5)
+        throw std::runtime_error("nVars must be at most 5");
+    const size_t blkN = (size_t)v.M * nVars;
+    std::string firstErr;
+#pragma omp parallel for schedule(static) num_threads(g_threads)
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        try
+        {
+            vr2ndCell(v, i, nVars, u, uRec + i * blkN);
+        }
+        catch (const std::exception &e)
+        {
+#pragma omp critical
+            firstErr = e.what();
+        }
+    }
+    if (!firstErr.empty())
+        throw std::runtime_error(firstErr);
+    return 0;
+    ORC_CATCH(-1)
+}
+
+static bool useSubs2nd(const VR &v) { return v.s.maxOrder == 1 && v.s.subs2ndOrder != 0; }
+
 // DoReconstructionIter (Jacobi branch, SORInstead = false),
 // VariationalReconstruction_Reconstruction.hxx:485-615; boundary faces add A^-1 BCC
 static void addAinvBCC(const VR &v, idx_t i, int nVars, const std::vector<real> &BCC, real scale, real *out)
@@ -1768,6 +1976,8 @@ int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u,
 {
     ORC_TRY
     const VR &v = h->v;
+    if (useSubs2nd(v)) // :503-512
+        return orc_vr_2nd(h, nVars, u, uRecNew);
     int M = v.M;
     size_t blkN = (size_t)M * nVars;
     std::string firstErr;
@@ -1831,6 +2041,8 @@ int orc_vr_sor(const orc_vr *h, int nVars, double *uRec, const double *u, const 
 {
     ORC_TRY
     const VR &v = h->v;
+    if (useSubs2nd(v)) // :503-512: the substitute reads only u
+        return orc_vr_2nd(h, nVars, u, uRec);
     const int M = v.M;
     const size_t blkN = (size_t)M * nVars;
     const real relax = v.s.jacobiRelax;
@@ -1901,6 +2113,11 @@ int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const doubl
     const VR &v = h->v;
     int M = v.M;
     size_t blkN = (size_t)M * nVars;
+    if (useSubs2nd(v)) // :631-638: the substitute does not depend on uRec
+    {
+        std::fill(uRecNew, uRecNew + (size_t)v.nCell * blkN, 0.0);
+        return 0;
+    }
     for (idx_t i = 0; i < v.nCell; i++)
     {
         real *out = uRecNew + i * blkN;

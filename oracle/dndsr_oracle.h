@@ -65,6 +65,10 @@ typedef struct
     /* the solver loops (orc_ssprk3, orc_implicit_euler_step) sweep the reconstruction in
      * place (orc_vr_sor) instead of by Jacobi */
     int SORInstead;
+    /* maxOrder 1 only: 0 = VR; 1 = Gauss rule, 2 = least squares (subs2ndOrder, VRSettings.hpp:49-50)
+     * replace every VR sweep by a gradient of the means */
+    int subs2ndOrder;
+    int subs2ndOrderGGScheme; /* 1: distance-weighted face value, 0: plain average */
 } orc_vr_settings;
 
 typedef struct
@@ -109,6 +113,9 @@ double orc_rec_error_l1(const orc_vr *h, int fieldId, const double *params, cons
 
 /* VR iteration: one Jacobi sweep uRecNew = (1-w) uRec + w * sum(...) */
 int orc_vr_iter(const orc_vr *h, int nVars, const double *uRec, const double *u, double *uRecNew);
+/* second-order substitute (settings.subs2ndOrder 1 or 2, maxOrder 1): uRec from the means only;
+ * orc_vr_iter / orc_vr_sor dispatch to it and orc_vr_iter_diff returns zeros, as the reference */
+int orc_vr_2nd(const orc_vr *h, int nVars, const double *u, double *uRec);
 /* VR iteration, SOR form: uRec swept in place, cells in scanOrder (NULL = natural order) */
 int orc_vr_sor(const orc_vr *h, int nVars, double *uRec, const double *u, const int64_t *scanOrder);
 /* uRecNew = uRecDiff - sum_f AinvB_f * uRecDiff_j - A^-1 BCdiff (VR Jacobian action); the

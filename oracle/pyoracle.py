@@ -33,7 +33,8 @@ class VRSettings(C.Structure):
                 ("geomWeightScheme", C.c_int), ("geomWeightBias", C.c_double),
                 ("geomWeightPower", C.c_double), ("scaleMultiplier", C.c_double),
                 ("jacobiRelax", C.c_double), ("nIterCellSmoothScale", C.c_int), ("bcWeight", C.c_double),
-                ("zoneDirWeight", dp), ("SORInstead", C.c_int)]
+                ("zoneDirWeight", dp), ("SORInstead", C.c_int), ("subs2ndOrder", C.c_int),
+                ("subs2ndOrderGGScheme", C.c_int)]
 
 
 class EulerSettings(C.Structure):
@@ -66,7 +67,7 @@ def lib():
         _lib.orc_last_error.restype = C.c_char_p
         _lib.orc_rec_error_l1.restype = C.c_double
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
-                     "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_iter_diff",
+                     "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
                      "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost"]:
             getattr(_lib, name).argtypes = None
@@ -116,7 +117,7 @@ class OracleVR:
 
     def __init__(self, mesh, maxOrder=1, intOrder=3, dirWeightScheme=0, geomWeightScheme=0,
                  geomWeightBias=0.0, geomWeightPower=0.5, jacobiRelax=1.0, nIterCellSmoothScale=15,
-                 bcWeight=1.0, zoneDirWeight=None, SORInstead=False):
+                 bcWeight=1.0, zoneDirWeight=None, SORInstead=False, subs2ndOrder=0, subs2ndOrderGGScheme=1):
         L = lib()
         self.mesh = mesh
         self._keep = [np.ascontiguousarray(x) for x in (
@@ -135,7 +136,8 @@ class OracleVR:
                       None if fz is None else fz.ctypes.data_as(i32p), nz)
         s = VRSettings(maxOrder, intOrder, dirWeightScheme, geomWeightScheme, geomWeightBias, geomWeightPower,
                        1.0, jacobiRelax, nIterCellSmoothScale, bcWeight,
-                       None if zw is None else zw.ctypes.data_as(dp), int(bool(SORInstead)))
+                       None if zw is None else zw.ctypes.data_as(dp), int(bool(SORInstead)), int(subs2ndOrder),
+                       int(subs2ndOrderGGScheme))
         self.settings = s
         h = L.orc_vr_create(C.byref(md), C.byref(s))
         if not h:
@@ -203,6 +205,13 @@ class OracleVR:
         out = np.zeros_like(uRec)
         _chk(lib().orc_vr_iter(self.h, C.c_int(nVars), P(np.ascontiguousarray(uRec)), P(np.ascontiguousarray(u)),
                                P(out)))
+        return out
+
+    def vr_2nd(self, u):
+        """Second-order substitute (settings subs2ndOrder): uRec from the means."""
+        nVars = u.shape[1]
+        out = np.zeros((self.nCell, nVars, self.M))
+        _chk(lib().orc_vr_2nd(self.h, C.c_int(nVars), P(np.ascontiguousarray(u)), P(out)))
         return out
 
     def vr_sor(self, uRec, u, scanOrder=None):
