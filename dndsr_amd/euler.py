@@ -114,6 +114,10 @@ class EulerEvaluator:
         tmp = uIncNew.clone()
         return self.UpdateSGS(alphaDiag, t, rhs, u, tmp, uIncNew, JDiag, False, True, False)
 
+    # FixUMaxFilter(u), EulerEvaluator.hxx:1318-1323: the reference returns at once
+    def FixUMaxFilter(self, u):
+        return None
+
     # EvaluateNorm(res, rhs, P, volWise, average), EulerEvaluator.hxx:1394-1440
     def EvaluateNorm(self, rhs, P=1, volWise=False, average=False):
         vol = torch.as_tensor(self.vfv.cell_metrics()["vol"], device=rhs.device)

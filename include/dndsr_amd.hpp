@@ -399,6 +399,8 @@ class EulerEvaluator
     {
         check(dndsr_ssprk3(vfv_.context().handle(), &es_, u.data(), uRec.data(), nStep, CFL, nRec, dtFixed));
     }
+    // FixUMaxFilter(u) (EulerEvaluator.hxx:1318-1323): the reference returns at once
+    void FixUMaxFilter(ArrayDOF &) const {}
     // EvaluateNorm(res, rhs, P, volWise = false) (EulerEvaluator.hpp:613), P = 1 or 2
     std::vector<double> EvaluateNorm(const ArrayDOF &rhs, int P = 1) const
     {
