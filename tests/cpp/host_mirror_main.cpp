@@ -18,6 +18,7 @@ static void run(const std::string &in, const std::string &out, const Mesh &mesh,
     VRSettings vs;
     vs.maxOrder = (int)par[1];
     vs.intOrder = (int)par[2];
+    vs.SORInstead = par.size() > 5 && par[5] != 0; // in-place colour-by-colour sweep
     VariationalReconstruction<dim> vfv(ctx, mesh, vs);
     vfv.ConstructMetrics();
     vfv.ConstructBaseAndWeight();
@@ -41,7 +42,7 @@ static void run(const std::string &in, const std::string &out, const Mesh &mesh,
     u.fromHostRows(npy::load<double>(in + "/u.npy"));
     rec.fromHostRows(npy::load<double>(in + "/uRec.npy"));
 
-    vfv.DoReconstructionIter(rec, recNew, u); // putIntoNew = false: rec holds the new iterate
+    vfv.DoReconstructionIter(rec, recNew, u); // putIntoNew = false: rec holds the new iterate (either form)
     double dtMin = 0;
     ev.EvaluateDt(dt, u, rec, par[4], dtMin, 1e100, true);
     ev.EvaluateRHS(rhs, u, rec);
@@ -74,7 +75,7 @@ int main(int argc, char **argv)
     try
     {
         Mesh mesh;
-        const std::vector<double> par = npy::load<double>(in + "/params.npy"); // dim, maxOrder, intOrder, rs, CFL
+        const std::vector<double> par = npy::load<double>(in + "/params.npy"); // dim, maxOrder, intOrder, rs, CFL[, SORInstead]
         mesh.dim = (int)par[0];
         mesh.coords = npy::load<double>(in + "/coords.npy");
         mesh.cellType = npy::load<int32_t>(in + "/cellType.npy");

@@ -1,5 +1,6 @@
 """The C++17 host mirror (include/dndsr_amd.hpp), driven by a compiled caller
-(tests/cpp/host_mirror_main.cpp), against the CPU oracle: one Jacobi VR sweep,
+(tests/cpp/host_mirror_main.cpp), against the CPU oracle: one VR sweep (Jacobi, or SOR
+in the device's colour order with SORInstead),
 EvaluateDt (local), EvaluateRHS, LUSGSMatrixInit and a forward Gauss-Seidel sweep in
 the device's colour order, at relative L2 <= 1e-12."""
 from __future__ import annotations
@@ -25,11 +26,12 @@ def _write_inputs(d, m, u, rec, par):
         np.save(os.path.join(d, k + ".npy"), np.ascontiguousarray(v))
 
 
-@pytest.mark.parametrize("kind,order,rs", [("2d", 1, "Roe"), ("2d", 2, "HLLEP"), ("3d", 2, "HLLEP")])
-def test_cpp_host_mirror_matches_oracle(require_gpu, tmp_path, kind, order, rs):
+@pytest.mark.parametrize("kind,order,rs,sor", [("2d", 1, "Roe", 0), ("2d", 2, "HLLEP", 0), ("3d", 2, "HLLEP", 0),
+                                                ("2d", 2, "Roe", 1)])
+def test_cpp_host_mirror_matches_oracle(require_gpu, tmp_path, kind, order, rs, sor):
     import pyoracle as po
     from dndsr_amd import build as b
-    print(f"\n[cpp host] {kind} p{order} {rs}", flush=True)
+    print(f"\n[cpp host] {kind} p{order} {rs} SOR={sor}", flush=True)
     m = small_mesh(kind, jitter=0.1)
     s = vr_settings(order, intOrder=5)
     o = po.OracleVR(m, **s)
@@ -37,14 +39,14 @@ def test_cpp_host_mirror_matches_oracle(require_gpu, tmp_path, kind, order, rs):
     u = random_state(N, 31)
     rec = random_rec(N, M, 5, seed=32)
     CFL = 2.0
-    _write_inputs(str(tmp_path / "in"), m, u, rec, [m.dim, order, 5, RS[rs], CFL])
+    _write_inputs(str(tmp_path / "in"), m, u, rec, [m.dim, order, 5, RS[rs], CFL, sor])
     os.makedirs(tmp_path / "out")
     r = subprocess.run([b.MIRROR_TEST_BIN, str(tmp_path / "in"), str(tmp_path / "out")], capture_output=True,
                        text=True, timeout=120)
     assert r.returncode == 0, r.stderr
     ld = lambda k: np.load(tmp_path / "out" / f"{k}.npy")  # noqa: E731
     es = o.euler_settings(1.4, RS[rs], 0, 1.0, 1.0)
-    recO = o.vr_iter(rec, u)
+    recO = o.vr_sor(rec, u, colour_scan_order(ld("colors"))) if sor else o.vr_iter(rec, u)
     assert rel_l2(ld("uRec"), recO) <= 1e-12
     dt, lam, dlc, dtMin = o.eval_dt(es, u, CFL, 1e100, True)
     assert rel_l2(ld("dt"), dt) <= 1e-12
