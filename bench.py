@@ -135,6 +135,43 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     return out
 
 
+def vr_sweep_forms(m, vfv, u, reps=20):
+    """Average time of one reconstruction sweep in each form on the benchmark mesh, from the
+    library's own per-sweep HIP-event timing: Jacobi and in-place multicolour SOR at p2, and
+    the second-order substitute (Gauss rule, least squares) at p1."""
+    import torch
+    from dndsr_amd import _lib
+    from dndsr_amd.cfv import VariationalReconstruction
+    L_ = _lib.lib()
+    nk = len(_lib.KERNEL_IDS)
+    k_vr = _lib.KERNEL_IDS.index("vr_iter")
+
+    def timed(g, fn):
+        fn()
+        torch.cuda.synchronize()
+        _lib.check(L_.dndsr_get_timing(g.ctx.h, None, None, 1))
+        _lib.check(L_.dndsr_set_timing(g.ctx.h, 1))
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        _lib.check(L_.dndsr_set_timing(g.ctx.h, 0))
+        ms, cnt = np.zeros(nk), np.zeros(nk, np.int64)
+        _lib.check(L_.dndsr_get_timing(g.ctx.h, ms.ctypes.data_as(_lib.dp), cnt.ctypes.data_as(_lib.i64p), 1))
+        return round(float(ms[k_vr] / max(cnt[k_vr], 1)), 5)
+
+    out = {"mesh": f"{vfv.nCell} periodic hexes (the benchmark mesh)", "timing": "HIP events per sweep, "
+           f"average of {reps}"}
+    rec, new = vfv.BuildURec(5), vfv.BuildURec(5)
+    out["p2_jacobi_ms"] = timed(vfv, lambda: vfv.DoReconstructionIter(rec, new, u))
+    out["p2_sor_ms"] = timed(vfv, lambda: vfv.DoReconstructionSOR(rec, u))
+    out["p2_sor_colours"] = int(vfv.colors().max()) + 1
+    for meth, name in ((1, "gauss"), (2, "least_squares")):
+        g1 = VariationalReconstruction(m, dict(maxOrder=1, intOrder=3, subs2ndOrder=meth)).construct()
+        r1, n1 = g1.BuildURec(5), g1.BuildURec(5)
+        out[f"p1_subs2nd_{name}_ms"] = timed(g1, lambda: g1.DoReconstructionIter(r1, n1, u))
+    return out
+
+
 def implicit_laminar_tet(n=69, Re=1e4, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=20000, fixed_iter=0):
     """Second half of the metric on the laminar configuration: steady laminar
     Navier-Stokes flow in the unit box with a sinusoidal bump of 0.1 of the height on
@@ -299,6 +336,8 @@ def main():
                 "frac": round(droof["frac"], 4), "traffic": droof["traffic"], "kernel": dom,
                 "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5)}
 
+    forms = vr_sweep_forms(m, vfv, u)
+
     implicit = implicit_tet = None
     if not args.no_implicit:
         implicit = implicit_bump_time_to_tol()
@@ -319,6 +358,7 @@ def main():
         "cpu_baseline": cpu,
         "implicit": implicit,
         "implicit_laminar_tet": implicit_tet,
+        "vr_sweep_forms": forms,
         "kernels": kern,
         "roofline_all": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                          for k, r in roof.items()},
