@@ -275,3 +275,29 @@ def test_sgs_symmetric_sweep_matches_dense_gauss_seidel():
     z = np.zeros((N, 5))
     x, _ = o.update_sgs(es, 1.0, rhs, u, z, z, jd, lam, True, True, True, order)
     assert rel_l2(x, ref) < 1e-13
+
+
+def test_hllep_stationary_contact_closed_form():
+    """HLLE+ (Gas.hpp:601-714) across a stationary contact (u = 0, equal p, rho_L != rho_R):
+    the physical fluxes agree and the density jump is the contact wave alone
+    (alpha_1 = d rho), so F = (0, p n, 0) + S+ S- / (S+ - S-) (1 - delta) d rho e_0 with
+    S- = min(-a_Roe, -a_L), S+ = max(a_Roe, a_R) and delta = a_Roe / (a_Roe +
+    max(|u_Roe|, dLambda fixScale)): pins the role of dLambda and fixScale."""
+    g = 1.4
+    rhoL, rhoR, p = 1.0, 0.6, 0.9
+    n = np.array([0.36, -0.48, 0.8])
+    UL = np.array([rhoL, 0, 0, 0, p / (g - 1)])
+    UR = np.array([rhoR, 0, 0, 0, p / (g - 1)])
+    HL, HR = g * p / ((g - 1) * rhoL), g * p / ((g - 1) * rhoR)
+    HRoe = (np.sqrt(rhoL) * HL + np.sqrt(rhoR) * HR) / (np.sqrt(rhoL) + np.sqrt(rhoR))
+    aRoe = np.sqrt((g - 1) * HRoe)
+    Sm, Sp = min(-aRoe, -np.sqrt(g * p / rhoL)), max(aRoe, np.sqrt(g * p / rhoR))
+    seen = set()
+    for dLambda, fixScale in ((0.0, 1.0), (0.3, 1.0), (2.0, 1.0), (2.0, 0.5)):
+        F, lam = po.riemann(RS["HLLEP"], UL, UR, n, g, dLambda=dLambda, fixScale=fixScale)
+        delta = aRoe / (aRoe + max(0.0, dLambda * fixScale))
+        expect = np.concatenate([[Sp * Sm / (Sp - Sm) * (1 - delta) * (rhoR - rhoL)], p * n, [0.0]])
+        assert np.abs(F - expect).max() <= 1e-14, (dLambda, fixScale, F, expect)
+        np.testing.assert_allclose(lam, [aRoe, 0.0, aRoe], rtol=1e-14, atol=1e-15)
+        seen.add(round(F[0], 12))
+    assert len(seen) == 4   # each (dLambda, fixScale) changes the contact dissipation
