@@ -1,6 +1,7 @@
 // euler.hip -- EulerEvaluator hot path on the device: EvaluateDt, EvaluateRHS,
 // LUSGSMatrixInit, the multicolour symmetric Gauss-Seidel sweeps and the
 // SSP-RK3 stage combination.
+#include <type_traits>
 #include "gas.hpp"
 #include "internal.hpp"
 
@@ -1022,6 +1023,70 @@ struct SideRec
     }
 };
 
+// Linear reconstruction (p1): the basis is x_s - moment with x_s = (x - cent) / scale, so
+// the side's value at x is U0 + G (x - cent) with U0 = u - sum_j moment_j uRec_j and the
+// gradient G_j = uRec_j / scale the same at every point: 23 values per side instead of the
+// moments, coefficients and a per-point basis derivative.
+template <int DIM>
+struct SideLin
+{
+    real U0[5], G[3][5], cen[3];
+
+    __device__ __forceinline__ void load(const DevMesh &m, idx c, const real *__restrict__ u,
+                                         const real *__restrict__ uRec)
+    {
+        const size_t N = m.N;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            U0[v] = u[(size_t)v * N + c];
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                G[d][v] = 0;
+        }
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cen[r] = 0;
+        if (!uRec)
+            return;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            cen[r] = m.cent[(size_t)r * N + c];
+        const real s = m.scale[c];
+        real mom[DIM];
+#pragma unroll
+        for (int j = 0; j < DIM; j++)
+            mom[j] = m.moment[(size_t)j * N + c];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+#pragma unroll
+            for (int j = 0; j < DIM; j++)
+            {
+                const real r = uRec[((size_t)v * DIM + j) * N + c];
+                U0[v] -= mom[j] * r;
+                G[j][v] = r / s;
+            }
+    }
+
+    __device__ __forceinline__ void eval(const DevMesh &, const real *p, const real *, real *U, real (*Gr)[5]) const
+    {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            real a = U0[v];
+#pragma unroll
+            for (int d = 0; d < DIM; d++)
+                a += G[d][v] * (p[d] - cen[d]);
+            U[v] = a;
+#pragma unroll
+            for (int d = 0; d < 3; d++)
+                Gr[d][v] = G[d][v];
+        }
+    }
+};
+template <int DIM, int NDOF>
+using ViscSide = std::conditional_t<NDOF == DIM + 1, SideLin<DIM>, SideRec<DIM, NDOF>>;
+
 template <int DIM, int NDOF>
 __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, const real *__restrict__ u,
                                                    const real *__restrict__ uRec, real *__restrict__ fflux)
@@ -1038,7 +1103,7 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
     const real *far = bnd ? m.bcVal + (size_t)m.fzone[f] * 5 : nullptr;
     real dB[3], sh[3];
     real minVol = m.vol[cL];
-    SideRec<DIM, NDOF> L, R;
+    ViscSide<DIM, NDOF> L, R;
     L.load(m, cL, u, uRec);
     if (!bnd)
         R.load(m, cR, u, uRec);
