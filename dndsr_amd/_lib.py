@@ -80,6 +80,7 @@ SIGNATURES = {
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
     "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
                                   C.POINTER(C.c_int), dp, dp, dp],
+    "dndsr_implicit_pp_stats": [vp, i64p, dp],
     "dndsr_lusgs_matvec": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp],
     "dndsr_gmres_lusgs": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
                           C.c_int, C.c_int, dp],

@@ -57,6 +57,17 @@ def _vr_sweep(vfv, uRec, u, relax):
     return out
 
 
+def _require_jacobi_frhs(vfv):
+    """The residual's transpose chains transposed Jacobi sweeps; with SORInstead the solver's
+    frhs sweeps in place (multicolour SOR) and with subs2ndOrder it uses the gradient
+    substitute, neither of which this chain differentiates."""
+    if vfv.settings.get("SORInstead"):
+        raise ValueError("the residual adjoint differentiates Jacobi VR sweeps; the settings select SORInstead "
+                         "(the transpose of the in-place SOR sweep is not provided)")
+    if vfv.settings.get("subs2ndOrder"):
+        raise ValueError("the residual adjoint does not cover the second-order substitute (subs2ndOrder)")
+
+
 class EvaluateRHSFunction(torch.autograd.Function):
     @staticmethod
     def forward(ctx, u, uRec, ev, deltaLambdaCell):
@@ -92,6 +103,7 @@ class ResidualFunction(torch.autograd.Function):
 
     @staticmethod
     def forward(ctx, u, uRec0, ev, nRec, deltaLambdaCell):
+        _require_jacobi_frhs(ev.vfv)
         u, uRec0 = u.contiguous(), uRec0.contiguous()
         ctx.ev, ctx.nRec = ev, int(nRec)
         ctx.save_for_backward(u, uRec0, deltaLambdaCell)

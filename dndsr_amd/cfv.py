@@ -28,9 +28,10 @@ def _ptr(t):
 
 
 class Context:
-    """One device context (C-ABI ``dndsr_ctx``) bound to torch's current stream (also when
-    that is the default stream, handle 0), so library kernels and torch operations on the
-    same tensors are ordered without explicit synchronisation."""
+    """One device context (C-ABI ``dndsr_ctx``) that follows torch's current stream: every
+    access to the handle re-binds the context when the caller has switched streams (e.g. inside
+    ``with torch.cuda.stream(s):``), so library kernels and torch operations on the same tensors
+    stay ordered without explicit synchronisation."""
 
     def __init__(self, device: int = 0):
         if not torch.cuda.is_available():
@@ -38,19 +39,28 @@ class Context:
         self.device = device
         h = C.c_void_p()
         check(lib().dndsr_create(C.byref(h), device))
-        self.h = h
+        self._h = h
+        self._stream = None
         self.torch_device = torch.device("cuda", device)
-        with torch.cuda.device(device):
-            stream = torch.cuda.current_stream(device)
-        check(lib().dndsr_set_stream(self.h, C.c_void_p(stream.cuda_stream)))
+
+    @property
+    def h(self):
+        """The C-ABI handle, bound to torch's current stream of this device."""
+        if self._h is None:
+            return None
+        s = torch.cuda.current_stream(self.device).cuda_stream
+        if s != self._stream:
+            check(lib().dndsr_set_stream(self._h, C.c_void_p(s)))
+            self._stream = s
+        return self._h
 
     def sync(self):
         check(lib().dndsr_sync(self.h))
 
     def close(self):
-        if getattr(self, "h", None) is not None and self.h.value:
-            lib().dndsr_destroy(self.h)
-            self.h = None
+        if getattr(self, "_h", None) is not None and self._h.value:
+            lib().dndsr_destroy(self._h)
+            self._h = None
 
     def __del__(self):
         try:

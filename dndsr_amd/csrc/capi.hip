@@ -29,7 +29,10 @@ struct dndsr_ctx
     // cell-major sweep state and per-cell-slot face data of the packed sweeps
     real *iSlot = nullptr, *iUA = nullptr;
     float *fSlot = nullptr, *fUA = nullptr;
-    int *iFlag = nullptr;
+    PPStat *iStat = nullptr;   // positivity statistics of the current iteration
+    real *iURecSave = nullptr; // uRec at entry, restored when a step fails
+    int64_t ppLimited = 0;     // limited cell increments of the last implicit step
+    double ppAlphaMin = 1.0;   // and their smallest alpha
     bool haveImplicit = false;
     // linear-solver workspace (GMRES Krylov basis for up to linK vectors + temporaries)
     LinWork lin;
@@ -156,6 +159,9 @@ int dndsr_create(dndsr_ctx **out, int device)
     DNDSR_HIP_CHECK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
     ctx->c.stream = ctx->own;
     DNDSR_HIP_CHECK(hipHostMalloc(reinterpret_cast<void **>(&ctx->c.hostRed), sizeof(real) * 16));
+    ctx->c.nonFinite = devAlloc<int>(ctx->c, 1);
+    DNDSR_HIP_CHECK(hipMemsetAsync(ctx->c.nonFinite, 0, sizeof(int), ctx->c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
     *out = ctx;
     CAPI_END
 }
@@ -178,11 +184,24 @@ int dndsr_destroy(dndsr_ctx *ctx)
     CAPI_END
 }
 
+// Reads and clears the residual's non-finite flag (the stream must be synchronised).
+static void checkFinite(Ctx &c, const char *where)
+{
+    int nf = 0;
+    DNDSR_HIP_CHECK(hipMemcpy(&nf, c.nonFinite, sizeof(int), hipMemcpyDeviceToHost));
+    if (nf)
+    {
+        DNDSR_HIP_CHECK(hipMemset(c.nonFinite, 0, sizeof(int)));
+        throw std::runtime_error(std::string("non-finite residual (EvaluateRHS) detected ") + where);
+    }
+}
+
 int dndsr_sync(dndsr_ctx *ctx)
 {
     CAPI_BEGIN
     need(ctx != nullptr, "null context");
     DNDSR_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    checkFinite(ctx->c, "since the last synchronisation");
     CAPI_END
 }
 
@@ -930,6 +949,8 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
     }
     if (rec != uRec) // odd number of VR sweeps in total: bring the result home
         launchCopy(c, nR, rec, uRec);
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    checkFinite(c, "in the SSP-RK3 steps");
     CAPI_END
 }
 
@@ -943,6 +964,9 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
          "bad implicit settings");
     need(is->gmresCode >= 0 && is->gmresCode <= 2 && (is->gmresCode == 0 || (is->nGmresSpace >= 1 && is->nGmresIter >= 0)),
          "bad GMRES settings");
+    need(is->gsUpdate == 0 || is->gsUpdate == 1,
+         "gsUpdate (jacobiCode) must be 0 (Jacobi) or 1 (Gauss-Seidel); jacobiCode 2 (ILU) is not provided");
+    need(is->incrementRelax >= 0 && is->incrementRelax <= 1, "incrementRelax must lie in [0, 1]");
     Ctx &c = ctx->c;
     DevMesh &m = c.m;
     GasParams gp = toGas(es);
@@ -968,18 +992,29 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             ctx->fLam = devAlloc<float>(c, 4 * F);
         ctx->fInc = devAlloc<float>(c, nU);
         ctx->fTmp = devAlloc<float>(c, nU);
-        ctx->iFlag = devAlloc<int>(c, 1);
+        ctx->iStat = devAlloc<PPStat>(c, 1);
+        ctx->iURecSave = devAlloc<real>(c, nR);
         ctx->haveImplicit = true;
     }
     const real gamma = es->gamma;
     const real rhoEps = kSmall * is->refRho * 1e-1, pEps = kSmall * is->refP * 1e-1;
-    DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iFlag, 0, sizeof(int), c.stream));
     launchCopy(c, nU, u, ctx->iXLast);
+    launchCopy(c, nR, uRec, ctx->iURecSave);
     real *rec = uRec, *recNew = ctx->wURecNew;
     real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
+    ctx->ppLimited = 0;
+    ctx->ppAlphaMin = 1.0;
+    // a failed step leaves u and uRec as they were at entry
+    auto fail = [&](const std::string &msg) {
+        launchCopy(c, nU, ctx->iXLast, u);
+        launchCopy(c, nR, ctx->iURecSave, uRec);
+        DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        throw std::runtime_error(msg);
+    };
     int it = 0;
     for (it = 1; it <= is->maxIter; it++)
     {
+        launchPPStatReset(c, ctx->iStat);
         // fdtau: local pseudo time step (alphaDiag = 1)
         launchEvalDt(c, gp, u, is->CFL, 1e100, 1, ctx->iDTau, ctx->wLam, ctx->wDlc, ctx->wDtMin);
         // frhs: nRec VR sweeps + EvaluateRHS
@@ -1008,7 +1043,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             else
                 DNDSR_HIP_CHECK(hipMemsetAsync(ctx->iInc, 0, nU * sizeof(real), c.stream));
             gmresSolve(c, gp, w, ctx->iRhs, ctx->iInc, is->nGmresSpace, is->nGmresIter, nullptr);
-            launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, ctx->iInc, ctx->iFlag);
+            launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, ctx->iInc, ctx->iStat);
         }
         else if (is->fp32Sweeps)
         {
@@ -1021,7 +1056,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             float *inc = nullptr;
             launchSGSPacked32(c, gp, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fInc, ctx->fTmp,
                               is->sgsIter, is->gsUpdate, &inc);
-            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iFlag);
+            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iStat);
         }
         else
         {
@@ -1030,15 +1065,30 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             real *inc = nullptr;
             launchSGSPacked64(c, gp, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iInc, ctx->iTmp,
                               is->sgsIter, is->gsUpdate, &inc);
-            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iFlag);
+            launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iStat);
         }
         // fstop: residual of this iteration relative to its running maximum (density component)
+        PPStat st;
         DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, ctx->iRes, 5 * sizeof(real), hipMemcpyDeviceToHost, c.stream));
+        DNDSR_HIP_CHECK(hipMemcpyAsync(&st, ctx->iStat, sizeof(PPStat), hipMemcpyDeviceToHost, c.stream));
         DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        for (int v = 0; v < 5; v++)
+            if (!std::isfinite(c.hostRed[v]))
+                fail("non-finite residual in the implicit step (iteration " + std::to_string(it) +
+                     "); u and uRec are left as they were at entry");
+        if (st.bad)
+            fail("the positivity-preserving increment met a state with non-positive density or internal energy "
+                 "(iteration " + std::to_string(it) + "); u and uRec are left as they were at entry");
+        ctx->ppLimited += st.nLim;
+        if (st.nLim)
+        {
+            double am;
+            std::memcpy(&am, &st.alphaMinBits, sizeof(am));
+            ctx->ppAlphaMin = std::min(ctx->ppAlphaMin, am);
+        }
         for (int v = 0; v < 5; v++)
         {
             rr[v] = c.hostRed[v];
-            need(std::isfinite(rr[v]), "non-finite residual in the implicit step");
             resBase[v] = std::max(resBase[v], rr[v]);
             rel[v] = rr[v] / (resBase[v] + kVerySmall);
         }
@@ -1051,11 +1101,6 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         it = is->maxIter;
     if (rec != uRec)
         launchCopy(c, nR, rec, uRec);
-    int flag = 0;
-    DNDSR_HIP_CHECK(hipMemcpyAsync(&flag, ctx->iFlag, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
-    need(flag == 0, "an increment needed the positivity limiter (EvaluateCellRHSAlpha / CompressInc), which this "
-                    "path does not apply; lower the pseudo-time CFL");
     if (nIter)
         *nIter = it;
     for (int v = 0; v < 5; v++)
@@ -1065,6 +1110,17 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         if (resRel)
             resRel[v] = rel[v];
     }
+    CAPI_END
+}
+
+int dndsr_implicit_pp_stats(dndsr_ctx *ctx, int64_t *nLimited, double *alphaMin)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    if (nLimited)
+        *nLimited = ctx->ppLimited;
+    if (alphaMin)
+        *alphaMin = ctx->ppAlphaMin;
     CAPI_END
 }
 
@@ -1087,6 +1143,7 @@ int dndsr_gmres_lusgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alp
     requireMesh(ctx);
     need(es && u && jdiag && lambdaFace && b && x, "null field");
     need(nGmresSpace >= 1 && nGmresIter >= 0 && sgsIter >= 0, "bad GMRES settings");
+    need(gsUpdate == 0 || gsUpdate == 1, "gsUpdate must be 0 (Jacobi) or 1 (Gauss-Seidel)");
     Ctx &c = ctx->c;
     const size_t N = c.m.N, F = c.m.F, nU = 5 * N;
     LinWork &w = linWork(ctx, nGmresSpace, fp32Precond != 0);

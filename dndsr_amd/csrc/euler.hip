@@ -458,7 +458,7 @@ kRHSFace(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__rest
 }
 
 __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restrict__ fflux, real *__restrict__ rhs,
-                                                  StageUpdate st)
+                                                  StageUpdate st, int *__restrict__ nonFinite)
 {
     idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= m.N)
@@ -479,6 +479,10 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
             acc[v] += (back ? fv : -fv) / vol;
         }
     }
+    // the reference asserts on non-finite fluxes (EulerEvaluator_EvaluateRHS.hxx:387-398, 782-788);
+    // here the first one raises a flag the host reads at its next synchronisation
+    if (!(isfinite(acc[0]) && isfinite(acc[1]) && isfinite(acc[2]) && isfinite(acc[3]) && isfinite(acc[4])))
+        atomicOr(nonFinite, 1);
     if (rhs)
     {
 #pragma unroll
@@ -1309,7 +1313,7 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
         DNDSR_HIP_CHECK(hipGetLastError());
     }
     KernelTimer kt2(c, K_RHS_GATHER);
-    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{});
+    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{}, c.nonFinite);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
@@ -1827,13 +1831,148 @@ void launchZoneForceBar(Ctx &c, int zone, const real *forceBarDev, real *fbar)
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
-// CompressInc (EulerEvaluator.hpp:1645-1830).  When neither density nor pressure
-// would fall below its relaxed bound, both reduce to the plain sum; a cell that
-// would need the limiter sets *flag instead (the caller reports it as an error).
+// Positivity of the increment, per cell (fincrement, EulerSolver.hxx:696-738):
+//   alpha_i = EvaluateCellRHSAlpha (EulerEvaluator.hxx:1938-2030; DEFAULT flag, compress 2,
+//             with IdealGasGetCompressionRatioPressure<dim, 0>, Gas.hpp:1750-1860),
+//   u_i    += CompressInc(u_i, alpha_i du_i) (AddFixedIncrement, EulerEvaluator.hpp:1645-1830).
+// Limited cells add to stat->nLim and lower stat->alphaMinBits (the bit pattern of a positive
+// double orders as the value); a cell that reaches one of the reference's assertions (non-positive
+// density or internal energy, no real root, an undecayed increment) sets stat->bad.
+__device__ inline real ppSq3(const real *a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+
+__device__ real ppCompressionRatioPressure(const real *u, const real *inc, real newRhoEi, bool &bad)
+{
+    const real safetyRatio = 1 - 1e-5;
+    real ret[5], un[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        ret[k] = inc[k], un[k] = u[k] + inc[k];
+    const real rhoEOld = u[4] - ppSq3(u + 1) / (u[0] + kVerySmall) * 0.5;
+    newRhoEi = fmax(kSmall * rhoEOld, newRhoEi);
+    const real c0 = 2 * u[4] * u[0] - ppSq3(u + 1) - 2 * u[0] * newRhoEi;
+    const real c1 = 2 * u[4] * ret[0] + 2 * u[0] * ret[4] -
+                    2 * (u[1] * ret[1] + u[2] * ret[2] + u[3] * ret[3]) - 2 * ret[0] * newRhoEi;
+    real c2 = 2 * ret[4] * ret[0] - ppSq3(ret + 1);
+    c2 += (c2 >= 0 ? 1.0 : -1.0) * kVerySmall;
+    real deltaC = c1 * c1 - 4 * c0 * c2;
+    if (deltaC <= -c0 * c0 * kSmall)
+        bad = true;
+    deltaC = fmax(0., deltaC);
+    const real alphaL = (-sqrt(deltaC) - c1) / (2 * c2);
+    real alpha;
+    if (fabs(c2) < 1e-10 * c0)
+        alpha = fabs(c1) < 1e-10 * c0 ? 0 : fmin(-c0 / c1, 1.);
+    else
+        alpha = fmin(alphaL, 1.);
+    alpha = fmax(0., alpha);
+    alpha *= safetyRatio;
+    if (alpha < kSmall)
+        alpha = 0;
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        ret[k] *= alpha;
+    const real decay = 1 - 1e-2;
+    int iter;
+    for (iter = 0; iter < 1000; iter++)
+    {
+        real w[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+        const real ek = 0.5 * ppSq3(w) / (u[0] + ret[0] + kVerySmall);
+        if (!(ret[4] + u[4] - ek < newRhoEi))
+            break;
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            ret[k] *= decay;
+        alpha *= decay;
+    }
+    if (iter >= 1000)
+        bad = true;
+    return alpha;
+}
+
+__device__ real ppCellAlpha(const real *u, const real *res, real gamma, real relax, real rhoEps, real pEps,
+                            bool &limited, bool &bad)
+{
+    const real safetyRatio = 1 - 1e-5;
+    real inc[5];
+    if (!(u[0] >= rhoEps))
+        bad = true;
+    real alphaRho = 1;
+    const real relaxedRho = rhoEps * relax + u[0] * (1 - relax);
+    if (res[0] < 0)
+        alphaRho = fmin(1.0, (u[0] - relaxedRho) / (-res[0] - kSmall * res[0]));
+    if (!(alphaRho >= 0 && alphaRho <= 1))
+        bad = true;
+    real un[5];
+#pragma unroll
+    for (int k = 0; k < 5; k++)
+        inc[k] = res[k] * alphaRho, un[k] = u[k] + inc[k];
+    const real pNew = (un[4] - 0.5 * ppSq3(un + 1) / un[0]) * (gamma - 1);
+    const real pOld = (u[4] - 0.5 * ppSq3(u + 1) / u[0]) * (gamma - 1);
+    real relaxedP = pEps;
+    if (pNew < pOld)
+        relaxedP = pEps + (pOld - pEps) * (1 - relax);
+    real alphaP = 1;
+    if (pNew < relaxedP)
+        alphaP = fmin(alphaP, ppCompressionRatioPressure(u, inc, relaxedP / (gamma - 1), bad));
+    real alpha = alphaRho * alphaP;
+    limited = alpha < 1;
+    if (limited)
+        alpha = alpha * alpha * safetyRatio; // uRecAlphaCompressPower 1, compress 2
+    return alpha;
+}
+
+__device__ void ppCompressInc(const real *u, real *ret, real gamma, real rhoEps, real pEps, bool &bad)
+{
+    if (!(u[0] > 0))
+        bad = true;
+    if (u[0] + ret[0] <= rhoEps)
+    {
+        const real f = (rhoEps - u[0]) / (ret[0] - kVerySmall);
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            ret[k] *= f;
+    }
+    const real rhoEi = u[4] - 0.5 * ppSq3(u + 1) / (u[0] + kVerySmall);
+    if (!(rhoEi > 0))
+        bad = true;
+    real w[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+    real ek = 0.5 * ppSq3(w) / (u[0] + ret[0] + kVerySmall);
+    if (u[4] + ret[4] - ek <= pEps)
+    {
+        const real newRhoEi = pEps / (gamma - 1);
+        const real c0 = 2 * u[4] * u[0] - ppSq3(u + 1) - 2 * u[0] * newRhoEi;
+        const real c1 = 2 * u[4] * ret[0] + 2 * u[0] * ret[4] -
+                        2 * (u[1] * ret[1] + u[2] * ret[2] + u[3] * ret[3]) - 2 * ret[0] * newRhoEi;
+        const real c2 = 2 * ret[4] * ret[0] - ppSq3(ret + 1);
+        const real deltaC = c1 * c1 - 4 * c0 * c2;
+        if (!(deltaC > 0))
+            bad = true;
+        const real alpha = fmax(0., fmin((-sqrt(deltaC) - c1) / (2 * c2), 1.));
+#pragma unroll
+        for (int k = 0; k < 5; k++)
+            ret[k] *= alpha * 0.99;
+        const real decay = 1 - 1e-1;
+        for (int iter = 0; iter < 1000; iter++)
+        {
+            real v[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+            ek = 0.5 * ppSq3(v) / (u[0] + ret[0] + kVerySmall);
+            if (!(ret[4] + u[4] - ek < newRhoEi))
+                break;
+#pragma unroll
+            for (int k = 0; k < 5; k++)
+                ret[k] *= decay;
+        }
+        real v[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+        ek = 0.5 * ppSq3(v) / (u[0] + ret[0] + kVerySmall);
+        if (ret[4] + u[4] - ek < newRhoEi * 0.5)
+            bad = true;
+    }
+}
+
 template <typename T, bool CELLMAJOR = false>
 __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real relax, real rhoEps, real pEps,
                                                      real *__restrict__ u, const T *__restrict__ du,
-                                                     int *__restrict__ flag)
+                                                     PPStat *__restrict__ stat)
 {
     const idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
@@ -1842,50 +1981,65 @@ __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real rel
 #pragma unroll
     for (int v = 0; v < 5; v++)
         U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? (size_t)i * 5 + v : (size_t)v * N + i];
-    bool limited = false;
-    const real relaxedRho = rhoEps * relax + U[0] * (1 - relax);
-    if (D[0] < 0 && (U[0] - relaxedRho) / (-D[0] - kSmall * D[0]) < 1.0)
-        limited = true;
-    real Un[5];
+    bool limited = false, bad = false;
+    const real a = ppCellAlpha(U, D, gamma, relax, rhoEps, pEps, limited, bad);
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        Un[v] = U[v] + D[v];
-    const real pNew = (Un[4] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) / Un[0]) * (gamma - 1);
-    const real pOld = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]) * (gamma - 1);
-    real relaxedP = pEps;
-    if (pNew < pOld)
-        relaxedP = pEps + (pOld - pEps) * (1 - relax);
-    if (pNew < relaxedP || !(Un[0] > 0))
-        limited = true;
+        D[v] *= a;
+    ppCompressInc(U, D, gamma, rhoEps, pEps, bad);
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        U[v] += D[v];
+    if (!(isfinite(U[0]) && isfinite(U[1]) && isfinite(U[2]) && isfinite(U[3]) && isfinite(U[4])))
+        bad = true;
     if (limited)
-        atomicOr(flag, 1);
+    {
+        atomicAdd(&stat->nLim, 1);
+        atomicMin(&stat->alphaMinBits, (unsigned long long)__double_as_longlong(a));
+    }
+    if (bad)
+        atomicOr(&stat->bad, 1);
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        u[(size_t)v * N + i] = U[v] + D[v];
+        u[(size_t)v * N + i] = U[v];
 }
-void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du, int *flagDev)
+__global__ void kPPStatReset(PPStat *stat)
+{
+    if (threadIdx.x == 0)
+    {
+        stat->alphaMinBits = (unsigned long long)__double_as_longlong(1.0);
+        stat->nLim = 0;
+        stat->bad = 0;
+    }
+}
+void launchPPStatReset(Ctx &c, PPStat *stat)
+{
+    kPPStatReset<<<1, 64, 0, c.stream>>>(stat);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du, PPStat *stat)
 {
     KernelTimer kt(c, K_INCREMENT);
-    kAddIncrement<real><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, flagDev);
+    kAddIncrement<real><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, stat);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
-                                 const float *du32, int *flagDev)
+                                 const float *du32, PPStat *stat)
 {
     KernelTimer kt(c, K_INCREMENT);
     if (du64)
         kAddIncrement<real, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
-                                                                             du64, flagDev);
+                                                                             du64, stat);
     else
         kAddIncrement<float, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
-                                                                              du32, flagDev);
+                                                                              du32, stat);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
-                          int *flagDev)
+                          PPStat *stat)
 {
     KernelTimer kt(c, K_INCREMENT);
-    kAddIncrement<float><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, flagDev);
+    kAddIncrement<float><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u, du, stat);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 

@@ -124,6 +124,7 @@ struct Ctx
     real *adjFaceBar = nullptr; // [5][F] face-flux adjoint
     real *adjRhs = nullptr;     // [5][N] residual of a functional's forward evaluation
     size_t faceLamCap = 0;
+    int *nonFinite = nullptr;   // set by the residual gather when a cell's residual is not finite
     std::vector<void *> owned;  // device allocations freed at destroy
     // optional per-kernel timing with HIP events on the launch stream
     bool timing = false;
@@ -226,10 +227,20 @@ void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, re
 void launchCopy(Ctx &c, size_t n, const real *src, real *dst);
 void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs,
                         real *resDev);
+// statistics of the positivity-preserving increment: limited cells, smallest alpha (as the bits
+// of a positive double), and a flag for states the reference asserts against
+struct PPStat
+{
+    unsigned long long alphaMinBits;
+    int nLim;
+    int bad;
+};
+void launchPPStatReset(Ctx &c, PPStat *stat);
+// fincrement: u += CompressInc(u, alpha du) with alpha from EvaluateCellRHSAlpha, per cell
 void launchAddIncrement64(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du,
-                          int *flagDev);
+                          PPStat *stat);
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
-                          int *flagDev);
+                          PPStat *stat);
 void launchToF32(Ctx &c, size_t n, const real *in, float *out);
 // (1 + sgsIter) forward/backward multicolour sweeps from a zero increment with the state
 // gathered from neighbours stored cell-major: uA <- u as [N][5], slot <- per-cell-slot face
@@ -241,7 +252,7 @@ void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real
                        float **result);
 // u += du with du cell-major ([N][5], FP64 or FP32: exactly one of du64 / du32 non-null)
 void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
-                                 const float *du32, int *flagDev);
+                                 const float *du32, PPStat *stat);
 // ---- linear solver (linsolve.hip)
 struct LinWork
 {

@@ -24,6 +24,14 @@ from ._lib import check, lib
 from .cfv import RIEMANN_TYPES, VariationalReconstruction, _ptr
 
 
+def _check_jacobi_code(code):
+    """linearSolverControl.jacobiCode (EulerSolver.hpp:703): 0 Jacobi, 1 Gauss-Seidel; 2 (ILU)
+    is not provided by this path."""
+    if code not in (0, 1):
+        raise ValueError(f"jacobiCode {code} is not supported: 0 (Jacobi) or 1 (Gauss-Seidel); "
+                         "2 (ILU) is not provided")
+
+
 class EulerEvaluator:
     nVars = 5
 
@@ -78,10 +86,15 @@ class EulerEvaluator:
 
     # EvaluateRHS(rhs, JSource, u, uRecUnlim, uRec, uRecBeta, cellRHSAlpha, onlyOnHalfAlpha, t, flags)
     def EvaluateRHS(self, rhs, JSource, u, uRecUnlim, uRec=None, uRecBeta=None, cellRHSAlpha=None,
-                    onlyOnHalfAlpha=False, t=0.0, flags=0):
+                    onlyOnHalfAlpha=False, t=0.0, flags=0, check_finite=True):
+        """check_finite synchronises and raises when a cell's residual is not finite (the
+        reference asserts, EvaluateRHS.hxx:387-398); without it the check waits for the next
+        Context.sync()."""
         rec = uRec if uRec is not None else uRecUnlim
         check(lib().dndsr_eval_rhs(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(rec), _ptr(self.deltaLambdaCell),
                                    _ptr(rhs)))
+        if check_finite:
+            self.ctx.sync()
 
     # LUSGSMatrixInit(JDiag, JSource, dTau, dt, alphaDiag, u, uRec, jacobianCode, t)
     def LUSGSMatrixInit(self, JDiag, JSource, dTau, dt, alphaDiag, u=None, uRec=None, jacobianCode=0, t=0.0):
@@ -142,6 +155,7 @@ class EulerEvaluator:
         LU-SGS preconditioner (FP32 sweeps when fp32Precond); x: initial guess -> solution.
         Returns (last preconditioned residual norm, |M b|)."""
         lam = self.lambdaFace if lambdaFace is None else lambdaFace
+        _check_jacobi_code(jacobiCode)
         r = np.zeros(2)
         check(lib().dndsr_gmres_lusgs(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(u), _ptr(JDiag),
                                       _ptr(lam), _ptr(b), _ptr(x), int(nGmresSpace), int(nGmresIter), int(sgsIter),
@@ -156,7 +170,10 @@ class EulerEvaluator:
         EulerSolver.hxx:645-830, 1206).  Defaults follow EulerSolver's config defaults
         (EulerSolver.hpp:171-760).  fp32Sweeps runs the LU-SGS sweeps (the solver, or the GMRES
         preconditioner when gmresCode is 1 or 2) in FP32 while the residual, reconstruction,
-        Krylov vectors and update stay FP64.  Returns (nIter, res, resRel, history)."""
+        Krylov vectors and update stay FP64.  The increment is positivity-limited and compressed
+        as the reference's fincrement (PPStats() gives the counts).  Returns (nIter, res, resRel,
+        history)."""
+        _check_jacobi_code(jacobiCode)
         s = _lib.ImplicitSettings(float(dt), float(CFL), int(nRec), int(sgsIter), int(jacobiCode), int(maxIter),
                                   int(minIter), float(tol), int(bool(fp32Sweeps)), float(incrementPPRelax),
                                   float(refRho), float(refP), int(gmresCode), int(nGmresSpace), int(nGmresIter))
@@ -166,6 +183,13 @@ class EulerEvaluator:
                                               C.byref(n), res.ctypes.data_as(_lib.dp), rel.ctypes.data_as(_lib.dp),
                                               hist.ctypes.data_as(_lib.dp)))
         return n.value, res, rel, hist[:n.value]
+
+    def PPStats(self):
+        """(cell increments limited by the positivity limiter, smallest alpha) of the last
+        ImplicitEulerStep (the solver's PPIncrementLimiter report, EulerSolver.hxx:708-718)."""
+        n, a = C.c_int64(), C.c_double()
+        check(lib().dndsr_implicit_pp_stats(self.ctx.h, C.byref(n), C.byref(a)))
+        return n.value, a.value
 
     def SSPRK3(self, u, uRec, nStep, CFL, nRec=1, dtFixed=0.0):
         """Device-resident explicit SSP-RK3 loop (ODE.hpp:1437-1535)."""

@@ -117,6 +117,9 @@ typedef struct
 int dndsr_create(dndsr_ctx **ctx, int device);
 int dndsr_destroy(dndsr_ctx *ctx);
 const char *dndsr_last_error(void);
+/* Waits for the context's stream.  Returns an error when an EvaluateRHS launched since the last
+ * synchronisation produced a non-finite residual in some cell (the reference asserts there,
+ * src/Euler/EulerEvaluator_EvaluateRHS.hxx:387-398,782-788); the flag is cleared by the call. */
 int dndsr_sync(dndsr_ctx *ctx);
 /* enqueue on the caller's stream (e.g. the framework's current stream) instead of the
  * context's own one, so that the library's work is ordered with the caller's; NULL selects
@@ -197,7 +200,8 @@ int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *
                   double *dtMinHost);
 /* EvaluateRHS, src/Euler/EulerEvaluator.hpp:399-409 (inviscid, plus the laminar viscous
  * fluxes when es->viscous; uRec may be NULL for the first-order residual; deltaLambdaCell
- * feeds HLLE+ and may be NULL = 0). */
+ * feeds HLLE+ and may be NULL = 0).  Asynchronous: a non-finite residual is reported by the
+ * next dndsr_sync (a NaN or infinite input is a value, not a fault: every kernel completes). */
 int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                    const double *deltaLambdaCell, double *rhs);
 /* LUSGSMatrixInit (scalar Jacobian), src/Euler/EulerEvaluator.hpp:427-434. */
@@ -248,10 +252,21 @@ typedef struct
  * increment.  Stops when the density residual relative to its running maximum is below
  * tol (after minIter iterations).  u and uRec are updated in place.  Outputs (host, may
  * be NULL): nIter, res[5] (L1 residual of the last iteration), resRel[5], history[maxIter]
- * (relative density residual per iteration).  Returns an error if a cell would need the
- * positivity limiter of the increment. */
+ * (relative density residual per iteration).  The increment is applied as the reference's
+ * fincrement (EulerSolver.hxx:696-738): per cell alpha = EvaluateCellRHSAlpha (relaxation
+ * incrementRelax, compress 2; src/Euler/EulerEvaluator.hxx:1938-2030), then AddFixedIncrement
+ * u += CompressInc(u, alpha inc) (src/Euler/EulerEvaluator.hpp:1645-1830); the counts are read
+ * with dndsr_implicit_pp_stats.  Returns an error, with u and uRec restored to their values at
+ * entry, when the residual is not finite or a cell meets a state the reference asserts against
+ * (non-positive density or internal energy).  gsUpdate must be 0 or 1 (jacobiCode 2, ILU, is
+ * not provided). */
 int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
                               double *u, double *uRec, int *nIter, double *res, double *resRel, double *history);
+
+/* Positivity statistics of the last dndsr_implicit_euler_step: the number of cell increments
+ * the positivity limiter scaled (summed over its iterations; the PPIncrementLimiter count the
+ * reference's solver logs, EulerSolver.hxx:708-718) and the smallest alpha (1 when none). */
+int dndsr_implicit_pp_stats(dndsr_ctx *ctx, int64_t *nLimited, double *alphaMin);
 
 /* LUSGSMatrixVec (scalar Jacobian), src/Euler/EulerEvaluator.hpp:449-455:
  *   Ax = D x + sum_f (alphaDiag/2) A_f / V fluxJacobian0(u_j) x_j, the matrix UpdateSGS sweeps. */

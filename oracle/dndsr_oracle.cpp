@@ -1580,6 +1580,184 @@ static void evalField(int id, const double *prm, const Pt &p, int nVars, real *o
     }
 }
 
+// ---------------------------------------------------------------------------
+// Positivity of the increment (ideal gas, nVars = 5).
+// ---------------------------------------------------------------------------
+static real sq3(const real *a) { return a[0] * a[0] + a[1] * a[1] + a[2] * a[2]; }
+static real signPlus(real a) { return a >= 0 ? 1 : -1; } // signP, src/DNDS/Defines.hpp:549-552
+
+// IdealGasGetCompressionRatioPressure<dim, 0> (src/Euler/Gas.hpp:1750-1860): the largest alpha in
+// [0, 1] (times the safety ratio, then decayed) with rho E_internal(u + alpha inc) >= newRhoEi.
+// bad is set where the reference asserts.
+static real compressionRatioPressure(const real *u, const real *inc, real newRhoEi, bool &bad)
+{
+    static const real safetyRatio = 1 - 1e-5;
+    real ret[5], un[5];
+    for (int k = 0; k < 5; k++)
+        ret[k] = inc[k], un[k] = u[k] + inc[k];
+    real rhoEOld = u[4] - sq3(u + 1) / (u[0] + verySmallReal) * 0.5;
+    newRhoEi = std::max(smallReal * rhoEOld, newRhoEi);
+    real rhoENew = un[4] - sq3(un + 1) / (un[0] + verySmallReal) * 0.5;
+    real alpha = (rhoEOld - newRhoEi) / std::max(-rhoENew + rhoEOld, verySmallReal);
+    if (rhoENew > rhoEOld)
+        alpha = 1;
+    alpha = std::max(std::min(alpha, 1.), 0.); // the convex estimate, superseded below
+    real c0 = 2 * u[4] * u[0] - sq3(u + 1) - 2 * u[0] * newRhoEi;
+    real c1 = 2 * u[4] * ret[0] + 2 * u[0] * ret[4] - 2 * dot3(u + 1, ret + 1) - 2 * ret[0] * newRhoEi;
+    real c2 = 2 * ret[4] * ret[0] - sq3(ret + 1);
+    c2 += signPlus(c2) * verySmallReal;
+    real deltaC = c1 * c1 - 4 * c0 * c2;
+    if (deltaC <= -c0 * c0 * smallReal)
+        bad = true;
+    deltaC = std::max(0., deltaC);
+    real alphaL = (-std::sqrt(deltaC) - c1) / (2 * c2);
+    if (std::abs(c2) < 1e-10 * c0)
+        alpha = std::abs(c1) < 1e-10 * c0 ? 0 : std::min(-c0 / c1, 1.);
+    else
+        alpha = std::min(alphaL, 1.);
+    alpha = std::max(0., alpha);
+    alpha *= safetyRatio;
+    if (alpha < smallReal)
+        alpha = 0;
+    for (int k = 0; k < 5; k++)
+        ret[k] *= alpha;
+    const real decay = 1 - 1e-2;
+    int iter;
+    for (iter = 0; iter < 1000; iter++)
+    {
+        real uv[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+        real ek = 0.5 * sq3(uv) / (u[0] + ret[0] + verySmallReal);
+        if (ret[4] + u[4] - ek < newRhoEi)
+        {
+            for (int k = 0; k < 5; k++)
+                ret[k] *= decay;
+            alpha *= decay;
+        }
+        else
+            break;
+    }
+    if (iter >= 1000)
+        bad = true;
+    return alpha;
+}
+
+// EvaluateCellRHSAlpha for one cell (src/Euler/EulerEvaluator.hxx:1938-2030; flag DEFAULT,
+// uRecAlphaCompressPower 1): alpha = (alphaRho alphaP)^compress * safetyRatio when below 1.
+// Returns alpha; limited tells whether the cell was limited; bad where the reference asserts.
+static real cellRHSAlpha(const real *u, const real *res, real gamma, real relax, real rhoEps, real pEps, int compress,
+                         bool &limited, bool &bad)
+{
+    static const real safetyRatio = 1 - 1e-5;
+    real inc[5];
+    for (int k = 0; k < 5; k++)
+        inc[k] = res[k];
+    if (!(u[0] >= rhoEps))
+        bad = true;
+    real alphaRho = 1;
+    real relaxedRho = rhoEps * relax + u[0] * (1 - relax);
+    if (inc[0] < 0)
+        alphaRho = std::min(1.0, (u[0] - relaxedRho) / (-inc[0] - smallReal * inc[0]));
+    if (!(alphaRho >= 0 && alphaRho <= 1))
+        bad = true;
+    for (int k = 0; k < 5; k++)
+        inc[k] *= alphaRho;
+    real un[5];
+    for (int k = 0; k < 5; k++)
+        un[k] = u[k] + inc[k];
+    real pNew = (un[4] - 0.5 * sq3(un + 1) / un[0]) * (gamma - 1);
+    real pOld = (u[4] - 0.5 * sq3(u + 1) / u[0]) * (gamma - 1);
+    real relaxedP = pEps;
+    if (pNew < pOld)
+        relaxedP = pEps + (pOld - pEps) * (1 - relax);
+    real alphaP = 1;
+    if (pNew < relaxedP)
+        alphaP = std::min(alphaP, compressionRatioPressure(u, inc, relaxedP / (gamma - 1), bad));
+    real alpha = alphaRho * alphaP;
+    limited = alpha < 1;
+    if (limited)
+        alpha = std::pow(alpha, compress) * safetyRatio;
+    return alpha;
+}
+
+// CompressInc (src/Euler/EulerEvaluator.hpp:1645-1760, ideal-gas part): scales the increment so
+// that density stays above rhoEps and rho E_internal above pEps / (gamma - 1).
+static void compressInc(const real *u, const real *inc, real gamma, real rhoEps, real pEps, real *ret, bool &bad)
+{
+    for (int k = 0; k < 5; k++)
+        ret[k] = inc[k];
+    if (!(u[0] > 0))
+        bad = true;
+    if (u[0] + ret[0] <= rhoEps)
+    {
+        real newrho = rhoEps;
+        real f = (newrho - u[0]) / (ret[0] - verySmallReal);
+        for (int k = 0; k < 5; k++)
+            ret[k] *= f;
+    }
+    real ekOld = 0.5 * sq3(u + 1) / (u[0] + verySmallReal);
+    real rhoEi = u[4] - ekOld;
+    if (!(rhoEi > 0))
+        bad = true;
+    real uv[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+    real ek = 0.5 * sq3(uv) / (u[0] + ret[0] + verySmallReal);
+    real rhoEiNew = u[4] + ret[4] - ek;
+    if (rhoEiNew <= pEps)
+    {
+        real newRhoEi = pEps / (gamma - 1);
+        real c0 = 2 * u[4] * u[0] - sq3(u + 1) - 2 * u[0] * newRhoEi;
+        real c1 = 2 * u[4] * ret[0] + 2 * u[0] * ret[4] - 2 * dot3(u + 1, ret + 1) - 2 * ret[0] * newRhoEi;
+        real c2 = 2 * ret[4] * ret[0] - sq3(ret + 1);
+        real deltaC = c1 * c1 - 4 * c0 * c2;
+        if (!(deltaC > 0))
+            bad = true;
+        real alphaL = (-std::sqrt(deltaC) - c1) / (2 * c2);
+        real alpha = std::max(0., std::min(alphaL, 1.));
+        for (int k = 0; k < 5; k++)
+            ret[k] *= alpha * 0.99;
+        const real decay = 1 - 1e-1;
+        for (int iter = 0; iter < 1000; iter++)
+        {
+            real w[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+            ek = 0.5 * sq3(w) / (u[0] + ret[0] + verySmallReal);
+            if (ret[4] + u[4] - ek < newRhoEi)
+                for (int k = 0; k < 5; k++)
+                    ret[k] *= decay;
+            else
+                break;
+        }
+        real w[3] = {u[1] + ret[1], u[2] + ret[2], u[3] + ret[3]};
+        ek = 0.5 * sq3(w) / (u[0] + ret[0] + verySmallReal);
+        if (ret[4] + u[4] - ek < newRhoEi * 0.5)
+            bad = true;
+    }
+}
+
+// fincrement (src/Euler/EulerSolver.hxx:696-738): alpha_i = EvaluateCellRHSAlpha(u, inc, relax,
+// compress 2, DEFAULT) per cell, then AddFixedIncrement (EulerEvaluator.hpp:1795-1830):
+// u_i += CompressInc(u_i, alpha_i inc_i).  Returns the number of limited cells; alphaMin gets
+// the smallest alpha (1 when none); bad when a cell hits one of the reference's assertions.
+static idx_t addIncrementPP(idx_t N, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du,
+                            real &alphaMin, bool &bad)
+{
+    idx_t nLim = 0;
+    alphaMin = 1;
+    for (idx_t i = 0; i < N; i++)
+    {
+        real *U = u + i * 5;
+        bool lim = false;
+        real a = cellRHSAlpha(U, du + i * 5, gamma, relax, rhoEps, pEps, 2, lim, bad);
+        if (lim)
+            nLim++, alphaMin = std::min(alphaMin, a);
+        real d[5], r[5];
+        for (int k = 0; k < 5; k++)
+            d[k] = du[i * 5 + k] * a;
+        compressInc(U, d, gamma, rhoEps, pEps, r, bad);
+        for (int k = 0; k < 5; k++)
+            U[k] += r[k];
+    }
+    return nLim;
+}
+
 } // namespace orc
 
 using namespace orc;
@@ -2729,7 +2907,7 @@ int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double 
 // L1 norm, running-maximum base, density component).
 int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
                             double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
-                            double *resRel, double *history)
+                            double *resRel, double *history, double *ppStats)
 {
     ORC_TRY
     const VR &v = h->v;
@@ -2740,6 +2918,8 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
     const real gamma = es->gamma;
     const real rhoEps = smallReal * is->refRho * 1e-1, pEps = smallReal * is->refP * 1e-1;
     real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
+    idx_t nLimTot = 0;
+    real alphaMinAll = 1;
     int it;
     for (it = 1; it <= is->maxIter; it++)
     {
@@ -2770,26 +2950,17 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
                     throw std::runtime_error(g_err);
                 std::swap(inc, incNew);
             }
-        // EvaluateCellRHSAlpha (EulerEvaluator.hxx:1964-2020): a limited cell is an error here
-        for (idx_t i = 0; i < N; i++)
+        // fincrement: positivity-limited, compressed increment
         {
-            const real *U = u + i * 5, *D = inc.data() + i * 5;
-            real relaxedRho = rhoEps * is->incrementRelax + U[0] * (1 - is->incrementRelax);
-            if (D[0] < 0 && std::min(1.0, (U[0] - relaxedRho) / (-D[0] - smallReal * D[0])) < 1.0)
-                throw std::runtime_error("positivity limiter needed (density)");
-            real Un[5];
-            for (int k = 0; k < 5; k++)
-                Un[k] = U[k] + D[k];
-            real pNew = (Un[4] - 0.5 * (Un[1] * Un[1] + Un[2] * Un[2] + Un[3] * Un[3]) / Un[0]) * (gamma - 1);
-            real pOld = (U[4] - 0.5 * (U[1] * U[1] + U[2] * U[2] + U[3] * U[3]) / U[0]) * (gamma - 1);
-            real relaxedP = pEps;
-            if (pNew < pOld)
-                relaxedP = pEps + (pOld - pEps) * (1 - is->incrementRelax);
-            if (pNew < relaxedP)
-                throw std::runtime_error("positivity limiter needed (pressure)");
+            real aMin = 1;
+            bool bad = false;
+            idx_t nl = addIncrementPP(N, gamma, is->incrementRelax, rhoEps, pEps, u, inc.data(), aMin, bad);
+            if (bad)
+                throw std::runtime_error("the positivity-preserving increment failed (a state with non-positive "
+                                         "density or internal energy)");
+            nLimTot += nl;
+            alphaMinAll = std::min(alphaMinAll, aMin);
         }
-        for (size_t q = 0; q < nU; q++)
-            u[q] += inc[q];
         for (int k = 0; k < 5; k++)
             rr[k] = 0;
         for (idx_t i = 0; i < N; i++)
@@ -2814,8 +2985,39 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
         if (resRel)
             resRel[k] = rel[k];
     }
+    if (ppStats)
+        ppStats[0] = (double)nLimTot, ppStats[1] = alphaMinAll;
     return 0;
     ORC_CATCH(-1)
+}
+
+int orc_cell_rhs_alpha(const double *u, const double *res, double gamma, double relax, double rhoEps, double pEps,
+                       int compress, double *alpha, int *limited)
+{
+    bool lim = false, bad = false;
+    *alpha = cellRHSAlpha(u, res, gamma, relax, rhoEps, pEps, compress, lim, bad);
+    *limited = lim ? 1 : 0;
+    return bad ? 1 : 0;
+}
+
+int orc_compress_inc(const double *u, const double *inc, double gamma, double rhoEps, double pEps, double *ret)
+{
+    bool bad = false;
+    compressInc(u, inc, gamma, rhoEps, pEps, ret, bad);
+    return bad ? 1 : 0;
+}
+
+int64_t orc_add_increment_pp(int64_t N, double gamma, double relax, double rhoEps, double pEps, double *u,
+                             const double *du, double *alphaMin, int *bad)
+{
+    bool b = false;
+    real am = 1;
+    idx_t n = addIncrementPP(N, gamma, relax, rhoEps, pEps, u, du, am, b);
+    if (alphaMin)
+        *alphaMin = am;
+    if (bad)
+        *bad = b ? 1 : 0;
+    return n;
 }
 
 } // extern "C"

@@ -171,7 +171,18 @@ typedef struct
 } orc_implicit_settings;
 int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
                             double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
-                            double *resRel, double *history);
+                            double *resRel, double *history, double *ppStats /* [2]: limited cells, min alpha */);
+
+/* positivity of the increment (ideal gas, 5 conservative variables per cell):
+ * EvaluateCellRHSAlpha for one cell (returns nonzero where the reference asserts) */
+int orc_cell_rhs_alpha(const double *u, const double *res, double gamma, double relax, double rhoEps, double pEps,
+                       int compress, double *alpha, int *limited);
+/* CompressInc of one cell's increment */
+int orc_compress_inc(const double *u, const double *inc, double gamma, double rhoEps, double pEps, double *ret);
+/* fincrement over N cells (row layout [N][5]): u += CompressInc(u, alpha_i du); returns the
+ * number of limited cells */
+int64_t orc_add_increment_pp(int64_t N, double gamma, double relax, double rhoEps, double pEps, double *u,
+                             const double *du, double *alphaMin, int *bad);
 
 #ifdef __cplusplus
 }

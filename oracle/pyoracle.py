@@ -69,7 +69,8 @@ def lib():
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
                      "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
-                     "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost"]:
+                     "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost",
+                     "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp"]:
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -100,6 +101,41 @@ def riemann(rsType, UL, UR, n, gamma=1.4, ULm=None, URm=None, dLambda=0.0, fixSc
     _chk(lib().orc_riemann(C.c_int(rsType), C.c_int(3), P(UL), P(UR), P(ULm), P(URm), P(n), C.c_double(gamma),
                            C.c_double(dLambda), C.c_double(fixScale), C.c_double(incFScale), P(F), P(lam)))
     return F, lam
+
+
+def cell_rhs_alpha(u, res, gamma=1.4, relax=0.9, rhoEps=1e-11, pEps=1e-11, compress=2):
+    """EvaluateCellRHSAlpha of one cell: (alpha, limited); raises where the reference asserts."""
+    a, lim = C.c_double(), C.c_int()
+    rc = lib().orc_cell_rhs_alpha(P(np.ascontiguousarray(u, np.float64)), P(np.ascontiguousarray(res, np.float64)),
+                                  C.c_double(gamma), C.c_double(relax), C.c_double(rhoEps), C.c_double(pEps),
+                                  C.c_int(compress), C.byref(a), C.byref(lim))
+    if rc:
+        raise RuntimeError("oracle: EvaluateCellRHSAlpha assertion")
+    return a.value, bool(lim.value)
+
+
+def compress_inc(u, inc, gamma=1.4, rhoEps=1e-11, pEps=1e-11):
+    """CompressInc of one cell's increment; raises where the reference asserts."""
+    out = np.zeros(5)
+    rc = lib().orc_compress_inc(P(np.ascontiguousarray(u, np.float64)), P(np.ascontiguousarray(inc, np.float64)),
+                                C.c_double(gamma), C.c_double(rhoEps), C.c_double(pEps), P(out))
+    if rc:
+        raise RuntimeError("oracle: CompressInc assertion")
+    return out
+
+
+def add_increment_pp(u, du, gamma=1.4, relax=0.9, rhoEps=1e-11, pEps=1e-11):
+    """fincrement over all cells (rows [N, 5]): returns (u_new, nLimited, alphaMin)."""
+    u = np.array(u, dtype=np.float64, copy=True)
+    du = np.ascontiguousarray(du, np.float64)
+    am, bad = C.c_double(), C.c_int()
+    f = lib().orc_add_increment_pp
+    f.restype = C.c_int64
+    n = f(C.c_int64(u.shape[0]), C.c_double(gamma), C.c_double(relax), C.c_double(rhoEps), C.c_double(pEps), P(u),
+          P(du), C.byref(am), C.byref(bad))
+    if bad.value:
+        raise RuntimeError("oracle: positivity-preserving increment failed")
+    return u, int(n), am.value
 
 
 def bc_ghost(bcType, UL, n, zoneValue=None, gamma=1.4):
@@ -290,16 +326,20 @@ class OracleVR:
         return u, uRec
 
     def implicit_euler_step(self, es, u, uRec, dt, CFL=10.0, nRec=1, sgsIter=0, gsUpdate=1, maxIter=20, minIter=5,
-                            tol=1e-10, incrementRelax=0.9, refRho=1.0, refP=1.0, scanOrder=None):
-        """Implicit-Euler dual-time step; returns (u, uRec, nIter, res, resRel, history)."""
+                            tol=1e-10, incrementRelax=0.9, refRho=1.0, refP=1.0, scanOrder=None, stats=False):
+        """Implicit-Euler dual-time step; returns (u, uRec, nIter, res, resRel, history), plus
+        (number of positivity-limited cell increments, smallest alpha) when stats."""
         u = np.array(u, dtype=np.float64, copy=True)
         uRec = np.array(uRec, dtype=np.float64, copy=True)
         st = ImplicitSettings(dt, CFL, nRec, sgsIter, gsUpdate, maxIter, minIter, tol, 0, incrementRelax, refRho, refP)
         so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
         n = C.c_int()
         res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(maxIter, 1))
+        pps = np.zeros(2)
         _chk(lib().orc_implicit_euler_step(self.h, C.byref(es), C.byref(st), P(u), P(uRec), P(so), C.byref(n), P(res),
-                                           P(rel), P(hist)))
+                                           P(rel), P(hist), P(pps)))
+        if stats:
+            return u, uRec, n.value, res, rel, hist[:n.value], (int(pps[0]), pps[1])
         return u, uRec, n.value, res, rel, hist[:n.value]
 
     def lusgs_matvec(self, es, alphaDiag, u, x, jdiag, lam):

@@ -236,3 +236,86 @@ def test_iv_pipeline_golden_on_gpu(require_gpu):
     inc = np.linalg.norm(ev.EvaluateNorm(x1, 2, False))
     assert abs(res[0] - 1.0524823780e+01) <= 1e-6 * 1.0524823780e+01
     assert abs(inc - 3.1557730986e+00) <= 1e-6 * 3.1557730986e+00
+
+
+def test_box_pipeline_golden_on_gpu(require_gpu):
+    """Reference golden (test/cpp/Euler/test_EulerEvaluator.cpp:297-330) through the HIP path:
+    3-D box on the 32^3 periodic unit cube, p1 / intOrder 3, HLLC, EvaluateDt(CFL 10, local)
+    -> first-order EvaluateRHS -> LUSGSMatrixInit(dt 1e100) -> block-Jacobi increment."""
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    from test_oracle_goldens import box_state
+    m = mm.periodic_hex_grid(32, 32, 32, 1.0, 1.0, 1.0)
+    g = VariationalReconstruction(m, dict(maxOrder=1, intOrder=3)).construct()
+    ev = EulerEvaluator(g, dict(riemannSolverType="HLLC"))
+    du = g.to_device(box_state(g.cell_metrics()["bary"]))
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, du, None, CFL=10.0, MaxDt=1e100, UseLocaldt=True)
+    rhs = g.BuildUDof(5)
+    ev.EvaluateRHS(rhs, None, du, None, None)
+    res = ev.EvaluateNorm(rhs, 1, True)
+    jd = g.BuildUDof(1)[0]
+    ev.LUSGSMatrixInit(jd, None, dTau, 1e100, 1.0, du, None)
+    x0, x1 = g.BuildUDof(5), g.BuildUDof(5)
+    ev.UpdateSGS(1.0, 0.0, rhs, du, x0, x1, jd, True, False, False)
+    inc = np.linalg.norm(ev.EvaluateNorm(x1, 2, False))
+    assert abs(res[0] - 5.8855408029e-01) <= 1e-6 * 5.8855408029e-01
+    assert abs(inc - 3.4532977412e+01) <= 1e-6 * 3.4532977412e+01
+
+
+GOLDEN_REC = [
+    # (grid n, dim, maxOrder, hqm, field, subs2ndOrder, golden, reference file:line)
+    (10, 2, 1, False, 2, 1, 1.5599270188e-02, "test_Reconstruction.cpp:533"),
+    (40, 2, 1, False, 2, 1, 7.8943623278e-04, "test_Reconstruction.cpp:533"),
+    (32, 3, 1, False, 2, 1, 1.5098818118e-03, "test_Reconstruction3D.cpp:395-397"),
+    (32, 3, 1, False, 3, 1, 1.4931684237e-03, "test_Reconstruction3D.cpp:401-403"),
+    (32, 3, 1, True, 3, 0, 2.4261881197e-03, "test_Reconstruction3D.cpp:401-403"),
+    (32, 3, 2, True, 2, 0, 7.4823241394e-05, "test_Reconstruction3D.cpp:395-397"),
+    (40, 2, 3, True, 2, 0, 1.4890868814e-06, "test_Reconstruction.cpp:536"),
+    (40, 2, 3, False, 2, 0, 8.3670061853e-07, "test_Reconstruction.cpp:538"),
+]
+
+
+@pytest.mark.parametrize("n,dim,order,hqm,fid,subs,golden,src", GOLDEN_REC,
+                         ids=[f"{c[1]}d-n{c[0]}-p{c[2]}-f{c[4]}-{'GG' if c[5] else 'VR'}" for c in GOLDEN_REC])
+def test_reconstruction_goldens_on_gpu(require_gpu, n, dim, order, hqm, fid, subs, golden, src):
+    """Reference reconstruction goldens through the HIP path: the device's Jacobi VR sweeps
+    (200 at most, stopping at an RMS increment below 1e-14, as the reference test does) or its
+    Gauss-Green substitute, then the reference's L1 error metric (evaluated by the oracle on the
+    device's result) against the golden, and the device's uRec against the oracle's."""
+    import pyoracle as po
+    import torch
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    print(f"\n[golden] {src} n={n} dim={dim} p={order} field={fid} subs2nd={subs}", flush=True)
+    L = 10.0 if dim == 2 else 1.0
+    m = mm.periodic_quad_grid(n, n, L, L) if dim == 2 else mm.periodic_hex_grid(n, n, n, L, L, L)
+    s = dict(maxOrder=order, intOrder=max(order + 2, 5), subs2ndOrder=subs, jacobiRelax=1.0)
+    if hqm:
+        s.update(dirWeightScheme=1, geomWeightScheme=1, geomWeightBias=1.0, geomWeightPower=0.5)
+    o = po.OracleVR(m, **s)
+    g = VariationalReconstruction(m, s).construct()
+    u = o.cell_average(fid, [L, dim], 1)
+    du = g.to_device(u)
+    rec, new = g.BuildURec(1), g.BuildURec(1)
+    for _ in range(1 if subs else 200):
+        g.DoReconstructionIter(rec, new, du, putIntoNew=True)
+        inc = torch.sqrt(((new - rec) ** 2).sum() / g.nCell).item()
+        rec, new = new, rec
+        if inc < 1e-14:
+            break
+    rg = g.to_host(rec)
+    err = o.rec_error_l1(fid, [L, dim], u, rg)
+    assert abs(err - golden) <= 1e-6 * golden, (err, golden)
+    if subs:
+        ro = o.vr_2nd(u)
+    else:
+        ro = np.zeros((o.nCell, 1, o.M))
+        for _ in range(200):
+            nw = o.vr_iter(ro, u)
+            inc = np.sqrt(((nw - ro) ** 2).sum() / o.nCell)
+            ro = nw
+            if inc < 1e-14:
+                break
+    assert rel_l2(rg, ro) <= 1e-10

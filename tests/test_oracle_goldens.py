@@ -118,9 +118,9 @@ HQM = dict(dirWeightScheme=1, geomWeightScheme=1, geomWeightBias=1.0, geomWeight
     (10, 1, False, 2, 4.6604402914e-02), (20, 1, False, 2, 9.2961629825e-03), (40, 1, False, 2, 1.5347312301e-03),
     (10, 1, True, 2, 4.6604402914e-02), (20, 1, True, 2, 9.2961629825e-03),
     (10, 2, True, 2, 3.0528143687e-03), (20, 2, True, 2, 2.3057099673e-04), (40, 2, True, 2, 2.4367733525e-05),
-    (10, 3, True, 2, 1.9105219870e-03), (20, 3, True, 2, 4.6701352192e-05),
-    (10, 3, False, 2, 1.8840503911e-03), (20, 3, False, 2, 2.4995731251e-05),
-    (10, 3, True, 3, 5.5612138851e-04), (20, 3, True, 3, 1.6082305163e-05),
+    (10, 3, True, 2, 1.9105219870e-03), (20, 3, True, 2, 4.6701352192e-05), (40, 3, True, 2, 1.4890868814e-06),
+    (10, 3, False, 2, 1.8840503911e-03), (20, 3, False, 2, 2.4995731251e-05), (40, 3, False, 2, 8.3670061853e-07),
+    (10, 3, True, 3, 5.5612138851e-04), (20, 3, True, 3, 1.6082305163e-05), (40, 3, True, 3, 6.6188225747e-07),
 ])
 def test_vr_golden_2d(n, order, hqm, fid, golden):
     m = mm.periodic_quad_grid(n, n, 10.0, 10.0)
@@ -131,12 +131,37 @@ def test_vr_golden_2d(n, order, hqm, fid, golden):
 
 @pytest.mark.parametrize("order,fid,golden", [
     # test/cpp/CFV/test_Reconstruction3D.cpp:393-409, Uniform32_3D_Periodic on [0,1]^3
-    (1, 2, 3.2475021825e-03), (2, 2, 7.4823241394e-05), (2, 3, 3.7014576777e-05),
+    (1, 2, 3.2475021825e-03), (2, 2, 7.4823241394e-05), (1, 3, 2.4261881197e-03), (2, 3, 3.7014576777e-05),
 ])
 def test_vr_golden_3d(order, fid, golden):
     m = mm.periodic_hex_grid(32, 32, 32, 1.0, 1.0, 1.0)
     o = po.OracleVR(m, maxOrder=order, intOrder=5, **HQM)
     err = _converge(o, fid, [1.0, 3])
+    assert abs(err - golden) <= 1e-6 * golden
+
+
+@pytest.mark.parametrize("n,golden", [
+    # test/cpp/CFV/test_Reconstruction.cpp:533: Gauss-Green (subs2ndOrder 1, distance-weighted
+    # face values), IV10 quads and two bisections, sin(kx)cos(ky)
+    (10, 1.5599270188e-02), (20, 3.4233789068e-03), (40, 7.8943623278e-04)])
+def test_gauss_green_golden_2d(n, golden):
+    """The reference measures u_i + grad_i.(x - bary_i); the substitute's linear modes are
+    grad * scale on the zero-mean basis (x - bary) / scale, the same polynomial."""
+    m = mm.periodic_quad_grid(n, n, 10.0, 10.0)
+    o = po.OracleVR(m, maxOrder=1, intOrder=5, subs2ndOrder=1)
+    u = o.cell_average(2, [10.0, 2], 1)
+    err = o.rec_error_l1(2, [10.0, 2], u, o.vr_2nd(u))
+    assert abs(err - golden) <= 1e-6 * golden
+
+
+@pytest.mark.parametrize("fid,golden", [
+    # test/cpp/CFV/test_Reconstruction3D.cpp:395-397,401-403: Gauss-Green on Uniform32_3D_Periodic
+    (2, 1.5098818118e-03), (3, 1.4931684237e-03)])
+def test_gauss_green_golden_3d(fid, golden):
+    m = mm.periodic_hex_grid(32, 32, 32, 1.0, 1.0, 1.0)
+    o = po.OracleVR(m, maxOrder=1, intOrder=5, subs2ndOrder=1)
+    u = o.cell_average(fid, [1.0, 3], 1)
+    err = o.rec_error_l1(fid, [1.0, 3], u, o.vr_2nd(u))
     assert abs(err - golden) <= 1e-6 * golden
 
 
@@ -195,6 +220,36 @@ def test_iv_pipeline_golden():
     x, _ = o.update_sgs(es, 1.0, rhs, u, z, z, jd, lam, True, False, False)
     inc = np.linalg.norm(np.sqrt((x ** 2).sum(0)))
     assert abs(inc - 3.1557730986e+00) <= 1e-6 * 3.1557730986e+00
+
+
+def box_state(o):
+    """cases/euler3D/euler3D_config_Box.json: far field (1, 0, 0, 0, 2.5), and
+    (0.5, 1, 0, 0, 4) in cells whose barycentre lies strictly inside (0.25, 0.75)^3
+    (boxInitializers, src/Euler/EulerEvaluator.hxx:1204-1218)."""
+    bary = o.cell_metrics()["bary"] if hasattr(o, "nCell") else o
+    u = np.tile([1.0, 0.0, 0.0, 0.0, 2.5], (len(bary), 1))
+    u[np.all((bary > 0.25) & (bary < 0.75), 1)] = [0.5, 1.0, 0.0, 0.0, 4.0]
+    return u
+
+
+def test_box_pipeline_golden():
+    """test/cpp/Euler/test_EulerEvaluator.cpp:297-330: 3-D box (NS_3D) on the 32^3 periodic
+    unit cube, p1 / intOrder 3, HLLC, local dt at CFL 10 (euler3D_config_Box.json), first-order
+    EvaluateRHS, LUSGSMatrixInit (dt 1e100) and the block-Jacobi increment."""
+    m = mm.periodic_hex_grid(32, 32, 32, 1.0, 1.0, 1.0)
+    o = po.OracleVR(m, maxOrder=1, intOrder=3)
+    u = box_state(o)
+    es = o.euler_settings(1.4, RS["HLLC"])
+    dt, lam, dlc, _ = o.eval_dt(es, u, 10.0, 1e100, True)
+    rhs = o.eval_rhs(es, u, None, dlc)
+    vol = o.cell_metrics()["vol"]
+    res = (np.abs(rhs) * vol[:, None]).sum(0)
+    assert abs(res[0] - 5.8855408029e-01) <= 1e-6 * 5.8855408029e-01
+    jd = o.lusgs_init(dt, 1e100, 1.0, lam)
+    z = np.zeros_like(u)
+    x, _ = o.update_sgs(es, 1.0, rhs, u, z, z, jd, lam, True, False, False)
+    inc = np.linalg.norm(np.sqrt((x ** 2).sum(0)))
+    assert abs(inc - 3.4532977412e+01) <= 1e-6 * 3.4532977412e+01
 
 
 # ---------------------------------------------------------------- RHS / SGS properties
