@@ -9,8 +9,8 @@ EvaluateRHS + stage combination), all on the device (dndsr_ssprk3).
 value = cell-updates per second per RHS evaluation = nCell * 3 * steps / time.
 The roofline entry is the dominant kernel: algorithmic bytes per launch divided
 by its average launch time, measured with HIP events on its stream during the
-timed region.  cpu_baseline times the CPU oracle (OpenMP on the host cores) on a
-bounded 32^3 sample of the same discretisation.
+timed region.  cpu_baseline times the CPU oracle on the same 64^3 workload with all
+physical host cores and with one core (median of five repetitions).
 """
 from __future__ import annotations
 
@@ -54,33 +54,74 @@ def read_traffic(kernel_tag):
     return None
 
 
-def cpu_baseline(n=32, evals_min=3, budget_s=12.0):
-    """Oracle frhs (Jacobi VR sweep + EvaluateRHS, OpenMP face buffer + cell gather)
-    on the TGV sample mesh n^3, p2, HLLE+."""
+def host_cpu_info():
+    """nproc (the CPUs this process may run on), and lscpu's model, sockets, cores per socket and
+    threads per core; physical cores available = min(affinity, sockets x cores per socket)."""
+    import subprocess
+    aff = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
+    info = {"nproc": aff}
+    try:
+        txt = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        kv = {ln.split(":", 1)[0].strip(): ln.split(":", 1)[1].strip() for ln in txt.splitlines() if ":" in ln}
+        info.update({"model": kv.get("Model name"), "sockets": int(kv.get("Socket(s)", "1")),
+                     "cores_per_socket": int(kv.get("Core(s) per socket", "0") or 0),
+                     "threads_per_core": int(kv.get("Thread(s) per core", "1") or 1)})
+    except (OSError, ValueError, subprocess.SubprocessError):
+        pass
+    phys = info.get("sockets", 1) * info.get("cores_per_socket", 0)
+    tpc = info.get("threads_per_core", 1)
+    info["physical_cores_available"] = min(aff, phys) if phys else max(1, aff // tpc)
+    return info
+
+
+def cpu_baseline(n=64, reps=5, evals_all=4, evals_one=1):
+    """The reference-path CPU baseline: the oracle's frhs (one Jacobi VR sweep + EvaluateRHS, the
+    OpenMP face-buffer + cell-gather form of EvaluateRHS.hxx:153-159,640-662) on the benchmark
+    configuration itself (TGV n^3 hex, p2, HLLE+, FP64), timed with all physical cores available
+    and with one core: after one warm-up evaluation, the median of `reps` repetitions of
+    `evals_all` (all cores) or `evals_one` (one core) evaluations; value = cells / median seconds
+    per evaluation.  The oracle's setup (metrics and VR coefficients) is not timed."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as po
     from dndsr_amd import mesh as mm
-    threads = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else os.cpu_count()
-    threads = min(threads, 16)
-    po.set_threads(threads, rhsGather=True)
+    info = host_cpu_info()
+    cores = int(info["physical_cores_available"])
+    t_setup = time.perf_counter()
+    po.set_threads(cores, rhsGather=True)
     m = mm.periodic_hex_grid(n, n, n, 2 * np.pi, 2 * np.pi, 2 * np.pi)
     o = po.OracleVR(m, maxOrder=2, intOrder=5)
     u = o.cell_average(11, [0.1, 1.4], 5)
     es = o.euler_settings(1.4, 3, 0, 1.0, 1.0)
-    rec = np.zeros((o.nCell, 5, o.M))
     _, _, dlc, _ = o.eval_dt(es, u, 0.5, 1e100, False)
-    rec = o.vr_iter(rec, u)
-    o.eval_rhs(es, u, rec, dlc)  # warm-up
-    t0 = time.perf_counter()
-    k = 0
-    while k < evals_min or (time.perf_counter() - t0 < budget_s and k < 5000):
+    rec = o.vr_iter(np.zeros((o.nCell, 5, o.M)), u)
+    t_setup = time.perf_counter() - t_setup
+
+    def frhs():
+        nonlocal rec
         rec = o.vr_iter(rec, u)
         o.eval_rhs(es, u, rec, dlc)
-        k += 1
-    dt = time.perf_counter() - t0
-    return {"value": o.nCell * k / dt, "unit": "cell-updates/s", "cores": threads, "kind": "port",
-            "sample": f"TGV {n}^3 hex p2 HLLE+ FP64: {k} x (Jacobi VR sweep + EvaluateRHS), oracle C++ "
-                      f"with OpenMP face-buffer/cell-gather on {threads} threads, {dt:.1f} s"}
+
+    res = {}
+    for label, threads, k in (("all", cores, evals_all), ("one", 1, evals_one)):
+        po.set_threads(threads, rhsGather=True)
+        frhs()  # warm-up
+        times = []
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            for _ in range(k):
+                frhs()
+            times.append((time.perf_counter() - t0) / k)
+        res[label] = (float(np.median(times)), times, k)
+    v_all = o.nCell / res["all"][0]
+    v_one = o.nCell / res["one"][0]
+    return {"value": v_all, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+            "value_1core": v_one, "cores_used_1core": 1,
+            "sample": (f"the benchmark workload itself (TGV {n}^3 = {o.nCell} hex, p2, HLLE+, FP64): oracle frhs = "
+                       f"Jacobi VR sweep + EvaluateRHS (OpenMP face buffer + cell gather); median of {reps} "
+                       f"repetitions of {evals_all} evaluations on {cores} cores "
+                       f"({res['all'][0] * 1e3:.1f} ms each) and of {evals_one} on 1 core "
+                       f"({res['one'][0] * 1e3:.1f} ms); oracle setup {t_setup:.1f} s untimed"),
+            "host": info}
 
 
 def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, shape=(160, 48, 32)):

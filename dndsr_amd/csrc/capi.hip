@@ -808,6 +808,17 @@ int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double 
     CAPI_END
 }
 
+int dndsr_model_rhs(dndsr_ctx *ctx, int nVars, double ax, double ay, double sigma, const double *u,
+                    const double *uRec, double *rhs)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && rhs, "null field");
+    need(sigma >= 0, "sigma must be non-negative");
+    launchModelRHS(ctx->c, nVars, ax, ay, sigma, u, uRec, rhs);
+    CAPI_END
+}
+
 int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alphaDiag, const double *lambdaFace,
                      double *jdiag)
 {

@@ -253,6 +253,8 @@ void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real
 // u += du with du cell-major ([N][5], FP64 or FP32: exactly one of du64 / du32 non-null)
 void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
                                  const float *du32, PPStat *stat);
+// ---- launchers (model.hip): ModelEvaluator::EvaluateRHS, scalar advection-diffusion (2-D)
+void launchModelRHS(Ctx &c, int nVars, real ax, real ay, real sigma, const real *u, const real *uRec, real *rhs);
 // ---- linear solver (linsolve.hip)
 struct LinWork
 {

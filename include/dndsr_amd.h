@@ -204,6 +204,14 @@ int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *
  * next dndsr_sync (a NaN or infinite input is a value, not a fault: every kernel completes). */
 int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                    const double *deltaLambdaCell, double *rhs);
+/* ModelEvaluator::EvaluateRHS (src/CFV/ModelEvaluator.hpp:118-119; ModelEvaluator.cpp:6-115,
+ * default options): the reference's scalar advection-diffusion test physics on a 2-D mesh, for
+ * nVars (1..5) independent variables u [nVars][nCell] with reconstruction uRec [nVars][M][nCell]
+ * (NULL = first order); advection velocity (ax, ay) with upwinding, diffusion sigma >= 0 through
+ * the mean face gradient plus the jump over 2 min(V_L, V_R) / A_f.  Boundary faces see a zero
+ * exterior value (the model's generateBoundaryValue).  rhs [nVars][nCell]. */
+int dndsr_model_rhs(dndsr_ctx *ctx, int nVars, double ax, double ay, double sigma, const double *u,
+                    const double *uRec, double *rhs);
 /* LUSGSMatrixInit (scalar Jacobian), src/Euler/EulerEvaluator.hpp:427-434. */
 int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alphaDiag, const double *lambdaFace,
                      double *jdiag);

@@ -2859,6 +2859,83 @@ int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alpha
     return 0;
 }
 
+// ModelEvaluator::EvaluateRHS (src/CFV/ModelEvaluator.cpp:6-115, default options; model
+// src/CFV/ModelEvaluator.hpp:18-123): scalar advection-diffusion of nVars independent variables
+// in 2-D, rhs_i = (1/V_i) sum_f +- sum_g w JDet [ -a.n (U_L + U_R)/2 + |a.n| (U_R - U_L)/2
+// + sigma (GradU_mean . n) ], with GradU_mean = (GradU_L + GradU_R)/2 + n (U_R - U_L) / distGRP,
+// distGRP = 2 min(V_L, V_R) / A_f.  Boundary faces take U_R = 0 (generateBoundaryValue,
+// ModelEvaluator.hpp:96-112) and GradU_R = GradU_L.  u: [N][nVars], uRec: [N][nVars][M]
+// (may be NULL: first order), rhs: [N][nVars].
+int orc_model_rhs(const orc_vr *h, int nVars, double ax, double ay, double sigma, const double *u, const double *uRec,
+                  double *rhs)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    if (v.dim != 2)
+        throw std::runtime_error("ModelEvaluator is two-dimensional");
+    const int M = v.M;
+    const size_t G = v.faceQw.size();
+    std::fill(rhs, rhs + (size_t)v.nCell * nVars, 0.0);
+    std::vector<real> D, UL(nVars), UR(nVars), fluxEs(nVars);
+    std::vector<std::array<real, 2>> GL(nVars), GR(nVars);
+    for (idx_t f = 0; f < v.nFace; f++)
+    {
+        const idx_t cL = v.f2c[f][0], cR = v.f2c[f][1];
+        const bool bnd = cR < 0;
+        std::fill(fluxEs.begin(), fluxEs.end(), 0.0);
+        for (size_t g = 0; g < G; g++)
+        {
+            const real *n = v.faceN[f][g].x;
+            auto side = [&](idx_t c, int s, std::vector<real> &U, std::vector<std::array<real, 2>> &Gr) {
+                v.diffBase(c, v.facePointFromSide(f, s, v.faceQP[f][g]), 3, 0, D);
+                for (int var = 0; var < nVars; var++)
+                {
+                    real r = 0, g0 = 0, g1 = 0;
+                    if (uRec)
+                        for (int k = 0; k < M; k++)
+                        {
+                            const real x = uRec[((size_t)c * nVars + var) * M + k];
+                            r += D[1 + k] * x;
+                            g0 += D[(size_t)v.NDOF + 1 + k] * x;
+                            g1 += D[(size_t)2 * v.NDOF + 1 + k] * x;
+                        }
+                    U[var] = u[(size_t)c * nVars + var] + r;
+                    Gr[var] = {g0, g1};
+                }
+            };
+            side(cL, 0, UL, GL);
+            real minVol = v.vol[cL];
+            if (!bnd)
+            {
+                side(cR, 1, UR, GR);
+                minVol = std::min(minVol, v.vol[cR]);
+            }
+            else
+                for (int var = 0; var < nVars; var++)
+                    UR[var] = 0.0, GR[var] = GL[var];
+            const real distGRP = minVol / v.area[f] * 2;
+            const real an = n[0] * ax + n[1] * ay;
+            for (int var = 0; var < nVars; var++)
+            {
+                real gm0 = (GR[var][0] + GL[var][0]) * 0.5 + (1.0 / distGRP) * (n[0] * (UR[var] - UL[var]));
+                real gm1 = (GR[var][1] + GL[var][1]) * 0.5 + (1.0 / distGRP) * (n[1] * (UR[var] - UL[var]));
+                real finc = -(an * UR[var] + an * UL[var]) * 0.5 + 0.5 * std::abs(an) * (UR[var] - UL[var]);
+                finc += (gm0 * n[0] + gm1 * n[1]) * sigma;
+                finc *= v.faceJD[f][g];
+                fluxEs[var] += finc * v.faceQw[g];
+            }
+        }
+        for (int var = 0; var < nVars; var++)
+        {
+            rhs[(size_t)cL * nVars + var] += fluxEs[var] / v.vol[cL];
+            if (!bnd)
+                rhs[(size_t)cR * nVars + var] += -fluxEs[var] / v.vol[cR];
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
 // Explicit SSP-RK3 (src/Solver/ODE.hpp:1477-1520) around frhs = nRec x Jacobi VR + EvaluateRHS,
 // fdt = EvaluateDt (global dt from CFL unless dtFixed > 0); fincrement = plain add (the
 // positivity compression of EulerEvaluator.hpp:1795 is the identity for these smooth states).

@@ -155,6 +155,10 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
 /* LUSGSMatrixVec (scalar Jacobian): Ax = D x + sum_f (alpha/2) A_f / V fluxJacobian0(u_j) x_j */
 int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
                      const double *x, const double *jdiag, const double *lambdaFace, double *Ax);
+/* ModelEvaluator::EvaluateRHS (scalar advection-diffusion, 2-D): u [N][nVars], uRec [N][nVars][M]
+ * (NULL = first order), rhs [N][nVars] */
+int orc_model_rhs(const orc_vr *h, int nVars, double ax, double ay, double sigma, const double *u, const double *uRec,
+                  double *rhs);
 /* full explicit SSP-RK3 steps: nStep steps of (EvaluateDt once, 3 x (nRec VR + RHS)); the
  * VR sweeps run in natural cell order when SORInstead */
 int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double *uRec, int nStep,

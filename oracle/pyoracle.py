@@ -70,7 +70,7 @@ def lib():
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
                      "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost",
-                     "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp"]:
+                     "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp", "orc_model_rhs"]:
             getattr(_lib, name).argtypes = None
     return _lib
 
@@ -341,6 +341,15 @@ class OracleVR:
         if stats:
             return u, uRec, n.value, res, rel, hist[:n.value], (int(pps[0]), pps[1])
         return u, uRec, n.value, res, rel, hist[:n.value]
+
+    def model_rhs(self, u, uRec=None, ax=1.0, ay=0.0, sigma=0.0):
+        """ModelEvaluator::EvaluateRHS (2-D scalar advection-diffusion of each column of u)."""
+        nVars = u.shape[1]
+        out = np.zeros((self.nCell, nVars))
+        _chk(lib().orc_model_rhs(self.h, C.c_int(nVars), C.c_double(ax), C.c_double(ay), C.c_double(sigma),
+                                 P(np.ascontiguousarray(u, np.float64)),
+                                 None if uRec is None else P(np.ascontiguousarray(uRec, np.float64)), P(out)))
+        return out
 
     def lusgs_matvec(self, es, alphaDiag, u, x, jdiag, lam):
         """LUSGSMatrixVec: the scalar-Jacobian LU-SGS matrix applied to x."""
