@@ -72,6 +72,7 @@ SIGNATURES = {
     "dndsr_set_vr_dirichlet": [vp, C.c_int, vp],
     "dndsr_eval_dt": [vp, C.POINTER(EulerSettings), vp, C.c_double, C.c_double, C.c_int, vp, vp, vp, vp, vp],
     "dndsr_eval_rhs": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],
+    "dndsr_eval_rhs_noalloc": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],
     "dndsr_model_rhs": [vp, C.c_int, C.c_double, C.c_double, C.c_double, vp, vp, vp],
     "dndsr_lusgs_init": [vp, vp, C.c_double, C.c_double, vp, vp],
     "dndsr_update_sgs": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,

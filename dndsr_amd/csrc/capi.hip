@@ -15,9 +15,19 @@
 
 using namespace dndsr;
 
+// Scratch of dndsr_eval_rhs_noalloc: the face-flux buffer [5][F] of EvaluateRHS for meshes of up
+// to kNoAllocMaxFaces faces (40 MiB), a file-scope device array that every GPU the library runs on
+// holds until the process ends.  In host code `scratch` only names the symbol; its device address
+// (one per GPU) comes from hipGetSymbolAddress at dndsr_create and is the only form passed on.
+static constexpr size_t kNoAllocMaxFaces = size_t(1) << 20;
+static __device__ __align__(256) double scratch[5 * kNoAllocMaxFaces];
+static constexpr int kMaxDevices = 64;
+static double *g_scratchDev[kMaxDevices] = {nullptr};
+
 struct dndsr_ctx
 {
     Ctx c;
+    double *scratchDev = nullptr; // this device's address of `scratch`
     hipStream_t own = nullptr;
     // device-resident time-loop workspace
     real *wXLast = nullptr, *wURecNew = nullptr, *wLam = nullptr, *wDlc = nullptr, *wDtMin = nullptr;
@@ -154,7 +164,16 @@ int dndsr_create(dndsr_ctx **out, int device)
     DNDSR_HIP_CHECK(hipGetDeviceCount(&n));
     need(device >= 0 && device < n, "invalid device index");
     DNDSR_HIP_CHECK(hipSetDevice(device));
+    need(device < kMaxDevices, "device index beyond the library's table");
+    if (!g_scratchDev[device])
+    {
+        void *p = nullptr;
+        DNDSR_HIP_CHECK(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        need(p != nullptr, "no device address for the no-allocation scratch");
+        g_scratchDev[device] = static_cast<double *>(p);
+    }
     auto *ctx = new dndsr_ctx;
+    ctx->scratchDev = g_scratchDev[device];
     ctx->c.device = device;
     DNDSR_HIP_CHECK(hipStreamCreateWithFlags(&ctx->own, hipStreamNonBlocking));
     ctx->c.stream = ctx->own;
@@ -805,6 +824,33 @@ int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double 
     requireMesh(ctx);
     need(u && rhs, "null field");
     launchEvalRHS(ctx->c, toGas(es), u, uRec, deltaLambdaCell, rhs);
+    CAPI_END
+}
+
+int dndsr_eval_rhs_noalloc(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                           const double *deltaLambdaCell, double *rhs)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && rhs, "null field");
+    Ctx &c = ctx->c;
+    need((size_t)c.m.F <= kNoAllocMaxFaces,
+         "dndsr_eval_rhs_noalloc: the mesh has more faces than its scratch holds (1048576); use dndsr_eval_rhs");
+    need(ctx->scratchDev != nullptr, "no-allocation scratch not resolved");
+    // every face's flux is written by the face passes before the gather reads it, so the
+    // scratch needs no clearing; timing events are not taken (they would be created here)
+    const bool timing = c.timing;
+    c.timing = false;
+    try
+    {
+        launchEvalRHS(c, toGas(es), u, uRec, deltaLambdaCell, rhs, nullptr, ctx->scratchDev, false);
+    }
+    catch (...)
+    {
+        c.timing = timing;
+        throw;
+    }
+    c.timing = timing;
     CAPI_END
 }
 

@@ -481,7 +481,7 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
     }
     // the reference asserts on non-finite fluxes (EulerEvaluator_EvaluateRHS.hxx:387-398, 782-788);
     // here the first one raises a flag the host reads at its next synchronisation
-    if (!(isfinite(acc[0]) && isfinite(acc[1]) && isfinite(acc[2]) && isfinite(acc[3]) && isfinite(acc[4])))
+    if (nonFinite && !(isfinite(acc[0]) && isfinite(acc[1]) && isfinite(acc[2]) && isfinite(acc[3]) && isfinite(acc[4])))
         atomicOr(nonFinite, 1);
     if (rhs)
     {
@@ -1261,12 +1261,12 @@ static void rhsDispatch(Ctx &c, const GasParams &gp, const real *u, const real *
 }
 
 void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *rhs,
-                   const StageUpdate *stage)
+                   const StageUpdate *stage, real *faceFluxBuf, bool flagNonFinite)
 {
     DevMesh &m = c.m;
     if (m.nBnd > 0 && !m.bcType)
         throw std::runtime_error("the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
-    real *ff = c.faceFlux;
+    real *ff = faceFluxBuf ? faceFluxBuf : c.faceFlux;
     KernelTimer kt(c, K_RHS_FACE);
     if (m.dim == 2)
     {
@@ -1313,7 +1313,8 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
         DNDSR_HIP_CHECK(hipGetLastError());
     }
     KernelTimer kt2(c, K_RHS_GATHER);
-    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{}, c.nonFinite);
+    kRHSGather<<<gridFor(m.N, 256), 256, 0, c.stream>>>(m, ff, rhs, stage ? *stage : StageUpdate{},
+                                                         flagNonFinite ? c.nonFinite : nullptr);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 

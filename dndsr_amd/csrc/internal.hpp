@@ -211,8 +211,11 @@ struct StageUpdate
     const real *y = nullptr, *dtDev = nullptr;
     real *u = nullptr, *save = nullptr;
 };
+// faceFluxBuf: the [5][F] face-flux scratch (NULL = the context's); flagNonFinite: raise the
+// context's non-finite flag from the gather
 void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *deltaLambdaCell,
-                   real *rhs, const StageUpdate *stage = nullptr);
+                   real *rhs, const StageUpdate *stage = nullptr, real *faceFluxBuf = nullptr,
+                   bool flagNonFinite = true);
 void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag);
 void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,

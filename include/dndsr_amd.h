@@ -204,6 +204,15 @@ int dndsr_eval_dt(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *
  * next dndsr_sync (a NaN or infinite input is a value, not a fault: every kernel completes). */
 int dndsr_eval_rhs(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
                    const double *deltaLambdaCell, double *rhs);
+/* EvaluateRHS that allocates nothing: no counterpart in the reference.  For callers that cannot
+ * allocate device memory (code under HIP graph capture, tight serving loops).  Same arguments and
+ * result as dndsr_eval_rhs; its face-flux scratch is a static device array of the library holding
+ * meshes of up to 1048576 faces (40 MiB per GPU, held until the process ends; larger meshes are
+ * rejected with an error).  Every call on one GPU shares that array, so calls that overlap on one
+ * GPU (two contexts or streams) race.  It does not raise the non-finite flag read by dndsr_sync
+ * and records no timing events. */
+int dndsr_eval_rhs_noalloc(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                           const double *deltaLambdaCell, double *rhs);
 /* ModelEvaluator::EvaluateRHS (src/CFV/ModelEvaluator.hpp:118-119; ModelEvaluator.cpp:6-115,
  * default options): the reference's scalar advection-diffusion test physics on a 2-D mesh, for
  * nVars (1..5) independent variables u [nVars][nCell] with reconstruction uRec [nVars][M][nCell]
