@@ -2537,6 +2537,12 @@ int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, 
 int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
                  const double *deltaLambdaCell, double *rhs)
 {
+    return orc_eval_rhs_lam(h, es, u, uRec, deltaLambdaCell, rhs, nullptr);
+}
+
+int orc_eval_rhs_lam(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                     const double *deltaLambdaCell, double *rhs, double *lamRoe)
+{
     ORC_TRY
     const VR &v = h->v;
     int M = v.M;
@@ -2551,6 +2557,7 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
     {
     std::vector<real> D;
     std::vector<std::array<real, 5>> ULv(G), URv(G), fincC(G);
+    std::vector<std::array<real, 3>> lamv(G); // per-point lam0 / lam123 / lam4 of the Riemann solver
     std::vector<std::array<real, 3>> nv(G);
     std::vector<std::array<std::array<real, 5>, 3>> GLv(G), GRv(G); // point gradients d U / d x_d
     const bool vis = es->viscous != 0;
@@ -2638,12 +2645,26 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
                          reinterpret_cast<real(*)[5]>(Fb.data()), dLambda, es->rsFixScale, es->rsIncFScale,
                          lam0, lam123, lam4);
             for (size_t g = 0; g < G; g++)
-                fincC[g] = Fb[g];
+                fincC[g] = Fb[g], lamv[g] = {lam0, lam123, lam4};
         }
         else
             for (size_t g = 0; g < G; g++)
+            {
                 dispatchFlux(rs, ULv[g].data(), URv[g].data(), ULm, URm, nv[g].data(), es->gamma, fincC[g].data(),
                              dLambda, es->rsFixScale, es->rsIncFScale, lam0, lam123, lam4);
+                lamv[g] = {lam0, lam123, lam4};
+            }
+        if (lamRoe)
+        {
+            // useRoeJacobian: the face's eigenvalues for the block Jacobian, integrated over the
+            // face and divided by its area (EvaluateRHS.hxx:527-544)
+            real le[3] = {0, 0, 0};
+            for (size_t g = 0; g < G; g++)
+                for (int k = 0; k < 3; k++)
+                    le[k] += (lamv[g][k] * v.faceJD[f][g]) * v.faceQw[g];
+            for (int k = 0; k < 3; k++)
+                lamRoe[(size_t)f * 3 + k] = le[k] / v.area[f];
+        }
         if (vis)
         {
             // viscous flux, EvaluateRHS.hxx:346-424 and fluxFace EvaluateDt.hxx:1163-1215, finc -= visFlux (:1460)
@@ -2829,6 +2850,252 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
     return 0;
 }
 
+// fluxJacobian0_Right_Times_du with the Roe term (EulerEvaluator.hpp:1100-1158, useRoeTerm = 1):
+// dF = [A(U) dU unless omitF] + s RoeFluxIncFDiff(dU; Roe average of U and UOther, lam0, lam123,
+// lam4) + s lambdaVis dU, with s = incFsign (RoeFluxIncFDiff Gas.hpp:1209-1243, GetRoeAverage
+// Gas.hpp:1149-1180; vg = 0).
+static void fluxJacobianRoeTimesDu(const real *U, const real *UOther, const real *n, const real *dU, real lambdaVis,
+                                   real lam0, real lam123, real lam4, real gamma, int incFsign, bool omitF, real *dF)
+{
+    if (!omitF)
+        fluxJacobianTimesDu(U, n, dU, 0.0, gamma, dF);
+    else
+        for (int k = 0; k < 5; k++)
+            dF[k] = 0;
+    RoePre r = roePreamble(U, UOther, gamma);
+    real inc[5];
+    roeIncF(dU, n, r.veloRoe, r.vsqrRoe, r.aRoe, r.asqrRoe, r.HRoe, incFsign * lam0, incFsign * lam123,
+            incFsign * lam4, gamma, inc);
+    for (int k = 0; k < 5; k++)
+        dF[k] += inc[k];
+    for (int k = 0; k < 5; k++)
+        dF[k] += incFsign * lambdaVis * dU[k];
+}
+
+// JacobianDiagBlock::GetInvert in block mode (EulerJacobian.hpp:153-190): the block scaled by
+// the inverse of its diagonal, inverted by full-pivot LU (Gauss-Jordan with complete pivoting,
+// the textbook form of Eigen's FullPivLU::inverse), then the diagonal scaling undone:
+// J^-1 = (D^-1 J)^-1 D^-1.  J and Jinv are 5x5 row-major.
+static void invertBlock5(const real *J, real *Jinv)
+{
+    real P[5][5], X[5][5], dinv[5];
+    for (int i = 0; i < 5; i++)
+    {
+        if (J[i * 5 + i] == 0)
+            throw std::runtime_error("block JDiag has a zero diagonal entry");
+        dinv[i] = 1.0 / J[i * 5 + i];
+    }
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++)
+            P[i][j] = dinv[i] * J[i * 5 + j], X[i][j] = i == j ? 1.0 : 0.0;
+    int colPerm[5] = {0, 1, 2, 3, 4};
+    for (int k = 0; k < 5; k++)
+    {
+        int pr = k, pc = k;
+        real best = -1;
+        for (int i = k; i < 5; i++)
+            for (int j = k; j < 5; j++)
+                if (std::abs(P[i][j]) > best)
+                    best = std::abs(P[i][j]), pr = i, pc = j;
+        if (!(best > 0))
+            throw std::runtime_error("block JDiag is singular");
+        for (int j = 0; j < 5; j++)
+            std::swap(P[k][j], P[pr][j]), std::swap(X[k][j], X[pr][j]);
+        for (int i = 0; i < 5; i++)
+            std::swap(P[i][k], P[i][pc]);
+        std::swap(colPerm[k], colPerm[pc]);
+        real piv = 1.0 / P[k][k];
+        for (int j = 0; j < 5; j++)
+            P[k][j] *= piv, X[k][j] *= piv;
+        for (int i = 0; i < 5; i++)
+        {
+            if (i == k)
+                continue;
+            real fct = P[i][k];
+            if (fct == 0)
+                continue;
+            for (int j = 0; j < 5; j++)
+                P[i][j] -= fct * P[k][j], X[i][j] -= fct * X[k][j];
+        }
+    }
+    // P Q = I after the eliminations with column permutation Q: P^-1 = Q X
+    real Pinv[5][5];
+    for (int k = 0; k < 5; k++)
+        for (int j = 0; j < 5; j++)
+            Pinv[colPerm[k]][j] = X[k][j];
+    for (int i = 0; i < 5; i++)
+        for (int j = 0; j < 5; j++)
+            Jinv[i * 5 + j] = Pinv[i][j] * dinv[j];
+}
+
+// the face's Roe eigenvalues seen from cell i: lambdaFace0 and lambdaFace4 swap when i is the
+// front cell (EulerEvaluator.hxx:96, 617)
+static void roeLambdas(const real *lambdaFace, const real *lamRoe, idx_t f, int atFace, real &l0, real &l123,
+                       real &l4)
+{
+    if (lamRoe)
+        l0 = lamRoe[(size_t)f * 3], l123 = lamRoe[(size_t)f * 3 + 1], l4 = lamRoe[(size_t)f * 3 + 2];
+    else
+        l0 = l123 = l4 = lambdaFace[(size_t)f * 4 + 3];
+    if (atFace)
+        std::swap(l0, l4);
+}
+
+// LUSGSMatrixInit with useRoeJacobian (EulerEvaluator.hxx:37-159, ignoreSourceTerm): per face
+// 0.5 alpha A_f / V_i (|A|_Roe(u_i, u_j) + lambdaVis I), for a boundary face also
+// 0.5 alpha A_f / V_i (A(u_g) - |A|_Roe(u_g, u_i) - lambdaVis I) dU_g/dU_i with the ghost u_g at the
+// face centre and its linearisation (generateBoundaryValue linMode 1, EvaluateDt.hxx:1762-1835),
+// then 1/dTau + 1/dt on the diagonal.
+int orc_lusgs_init_block(const orc_vr *h, const orc_euler_settings *es, const double *dTau, double dt,
+                         double alphaDiag, const double *u, const double *lambdaFace, const double *lamRoe,
+                         double *jdiag, double *jinv)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    const real gamma = es->gamma;
+    for (idx_t i = 0; i < v.nCell; i++)
+    {
+        real J[25] = {0};
+        const real *ui = u + (size_t)i * 5;
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            int atFace = v.cellIsBack(i, f) ? 0 : 1;
+            real n[3];
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFace ? -1 : 1);
+            real l0, l123, l4;
+            roeLambdas(lambdaFace, lamRoe, f, atFace, l0, l123, l4);
+            const real lvis = lambdaFace[(size_t)f * 4 + 2];
+            const real coef = (0.5 * alphaDiag) * v.area[f] / v.vol[i];
+            real uj[5];
+            int bt = -1;
+            if (o < 0)
+            {
+                int zone = v.faceZone[f];
+                bt = v.bcType.empty() ? -1 : v.bcType[zone];
+                if (bt < 0)
+                    throw std::runtime_error("boundary faces need the boundary-condition table (orc_set_bc)");
+                bcGhost(bt, ui, n, v.bcValue[zone].data(), gamma, uj);
+            }
+            else
+                for (int k = 0; k < 5; k++)
+                    uj[k] = u[(size_t)o * 5 + k];
+            real jacII[25];
+            for (int c = 0; c < 5; c++)
+            {
+                real e[5] = {0, 0, 0, 0, 0}, col[5];
+                e[c] = 1;
+                fluxJacobianRoeTimesDu(ui, uj, n, e, lvis, l0, l123, l4, gamma, +1, true, col);
+                for (int r = 0; r < 5; r++)
+                    jacII[r * 5 + c] = col[r];
+            }
+            if (o < 0)
+            {
+                // dU_g / dU_i column by column; zero for far field, in- and outflow (linMode 1)
+                real JBC[25] = {0};
+                if (bt == ORC_BC_WALL_INVIS || bt == ORC_BC_SYM || bt == ORC_BC_WALL)
+                    for (int c = 0; c < 5; c++)
+                    {
+                        real e[5] = {0, 0, 0, 0, 0}, g[5];
+                        e[c] = 1;
+                        bcGhost(bt, e, n, v.bcValue[v.faceZone[f]].data(), gamma, g);
+                        for (int r = 0; r < 5; r++)
+                            JBC[r * 5 + c] = g[r];
+                    }
+                real jacIJ[25];
+                for (int c = 0; c < 5; c++)
+                {
+                    real e[5] = {0, 0, 0, 0, 0}, col[5];
+                    e[c] = 1;
+                    fluxJacobianRoeTimesDu(uj, ui, n, e, lvis, l0, l123, l4, gamma, -1, false, col);
+                    for (int r = 0; r < 5; r++)
+                        jacIJ[r * 5 + c] = col[r];
+                }
+                for (int r = 0; r < 5; r++)
+                    for (int c = 0; c < 5; c++)
+                    {
+                        real acc = 0;
+                        for (int k = 0; k < 5; k++)
+                            acc += jacIJ[r * 5 + k] * JBC[k * 5 + c];
+                        J[r * 5 + c] += coef * acc;
+                    }
+            }
+            for (int q = 0; q < 25; q++)
+                J[q] += coef * jacII[q];
+        }
+        for (int k = 0; k < 5; k++)
+            J[k * 5 + k] += 1.0 / dTau[i] + 1.0 / dt;
+        for (int q = 0; q < 25; q++)
+            jdiag[(size_t)i * 25 + q] = J[q];
+        invertBlock5(J, jinv + (size_t)i * 25);
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
+// UpdateSGS with useRoeJacobian (EulerEvaluator.hxx:550-693): the off-diagonal term of a
+// neighbour j is fluxJacobian0(u_j, u_i, n, dU_j) with the Roe term and incFsign -1, the
+// update x_i = JDiag_i^-1 buf (MatVecLeftInvert, EulerJacobian.hpp:215).
+int orc_update_sgs_block(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                         const double *u, const double *uInc, double *uIncNew, const double *jinv,
+                         const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate,
+                         int uIncIsZero, const int64_t *scanOrder, double *sumInc)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    std::vector<idx_t> pos(N), ord(N);
+    for (idx_t s = 0; s < N; s++)
+    {
+        ord[s] = scanOrder ? scanOrder[s] : s;
+        pos[ord[s]] = s;
+    }
+    for (int k = 0; k < 5; k++)
+        sumInc[k] = 0;
+    for (idx_t sIt = 0; sIt < N; sIt++)
+    {
+        idx_t iScan = forward ? sIt : N - 1 - sIt;
+        idx_t i = ord[iScan];
+        real buf[5];
+        for (int k = 0; k < 5; k++)
+            buf[k] = rhs[i * 5 + k];
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            int atFace = v.cellIsBack(i, f) ? 0 : 1;
+            if (o < 0 || o == i)
+                continue;
+            real n[3];
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFace ? -1 : 1);
+            bool gsUseNew = gsUpdate && (forward ? (pos[o] < iScan) : (pos[o] > iScan));
+            if (uIncIsZero && !gsUseNew)
+                continue;
+            const real *duj = gsUseNew ? uIncNew + o * 5 : uInc + o * 5;
+            real l0, l123, l4;
+            roeLambdas(lambdaFace, lamRoe, f, atFace, l0, l123, l4);
+            real dF[5];
+            fluxJacobianRoeTimesDu(u + o * 5, u + i * 5, n, duj, lambdaFace[(size_t)f * 4 + 2], l0, l123, l4,
+                                   es->gamma, -1, false, dF);
+            for (int k = 0; k < 5; k++)
+                buf[k] -= (0.5 * alphaDiag) * v.area[f] / v.vol[i] * dF[k];
+        }
+        const real *Ji = jinv + (size_t)i * 25;
+        for (int k = 0; k < 5; k++)
+        {
+            real nw = 0;
+            for (int c = 0; c < 5; c++)
+                nw += Ji[k * 5 + c] * buf[c];
+            real old = uIncNew[i * 5 + k];
+            uIncNew[i * 5 + k] = nw;
+            sumInc[k] += std::abs(nw - old);
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
 // LUSGSMatrixVec (EulerEvaluator.hxx:175-275), scalar Jacobian: the matrix whose
 // Gauss-Seidel sweeps UpdateSGS performs, Ax_i = D_i x_i - buf_i with
 // buf_i = -sum_f (alpha/2) A_f / V_i * fluxJacobian0(u_j) x_j (lambda term included).
@@ -2992,6 +3259,9 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
     size_t nU = (size_t)N * 5, nR = nU * v.M;
     std::vector<real> dTau(N), lam((size_t)v.nFace * 4), dlc(N), rhsF(nU), rhs(nU), xLast(u, u + nU), tmp(nR),
         jd(N), inc(nU), incNew(nU);
+    const bool roeJ = es->useRoeJacobian != 0;
+    std::vector<real> lamRoe(roeJ ? (size_t)v.nFace * 3 : 0), jdB(roeJ ? (size_t)N * 25 : 0),
+        jinvB(roeJ ? (size_t)N * 25 : 0);
     const real gamma = es->gamma;
     const real rhoEps = smallReal * is->refRho * 1e-1, pEps = smallReal * is->refP * 1e-1;
     real resBase[5] = {0, 0, 0, 0, 0}, rr[5] = {0, 0, 0, 0, 0}, rel[5] = {0, 0, 0, 0, 0};
@@ -3005,7 +3275,7 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
             throw std::runtime_error(g_err);
         for (int r = 0; r < is->nRec; r++)
             vrSweep(h, uRec, u, scanOrder, tmp);
-        if (orc_eval_rhs(h, es, u, uRec, dlc.data(), rhsF.data()))
+        if (orc_eval_rhs_lam(h, es, u, uRec, dlc.data(), rhsF.data(), roeJ ? lamRoe.data() : nullptr))
             throw std::runtime_error(g_err);
         // crhs = xLast / dt - x / dt + frhs
         for (size_t q = 0; q < nU; q++)
@@ -3015,15 +3285,20 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
             r += rhsF[q];
             rhs[q] = r;
         }
-        if (orc_lusgs_init(h, dTau.data(), is->dt, 1.0, lam.data(), jd.data()))
+        if (roeJ ? orc_lusgs_init_block(h, es, dTau.data(), is->dt, 1.0, u, lam.data(), lamRoe.data(), jdB.data(),
+                                        jinvB.data())
+                 : orc_lusgs_init(h, dTau.data(), is->dt, 1.0, lam.data(), jd.data()))
             throw std::runtime_error(g_err);
         std::fill(inc.begin(), inc.end(), 0.0);
         real sumInc[5];
         for (int s = 0; s <= is->sgsIter; s++)
             for (int dir = 1; dir >= 0; dir--)
             {
-                if (orc_update_sgs(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jd.data(), lam.data(), dir,
-                                   is->gsUpdate, 0, scanOrder, sumInc))
+                int rc = roeJ ? orc_update_sgs_block(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jinvB.data(),
+                                                     lam.data(), lamRoe.data(), dir, is->gsUpdate, 0, scanOrder, sumInc)
+                              : orc_update_sgs(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jd.data(),
+                                               lam.data(), dir, is->gsUpdate, 0, scanOrder, sumInc);
+                if (rc)
                     throw std::runtime_error(g_err);
                 std::swap(inc, incNew);
             }

@@ -83,6 +83,7 @@ typedef struct
     int viscous;
     int muModel;
     double Rgas, muGas, prGas, TRef, CSutherland;
+    int useRoeJacobian; /* EulerEvaluatorSettings::useRoeJacobian: block JDiag with the Roe |A| */
 } orc_euler_settings;
 
 typedef struct orc_vr orc_vr;
@@ -146,6 +147,21 @@ int orc_eval_dt(const orc_vr *h, const orc_euler_settings *es, const double *u, 
                 double *deltaLambdaCell, double *dtMin);
 int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
                  const double *uRec, const double *deltaLambdaCell, double *rhs);
+/* EvaluateRHS that also writes the face-averaged Riemann eigenvalues lambdaFace0/123/4
+ * ([nFace][3], EvaluateRHS.hxx:527-544) when lamRoe is not NULL */
+int orc_eval_rhs_lam(const orc_vr *h, const orc_euler_settings *es, const double *u,
+                     const double *uRec, const double *deltaLambdaCell, double *rhs, double *lamRoe);
+/* LUSGSMatrixInit with useRoeJacobian (EulerEvaluator.hxx:37-159): 5x5 blocks [N][5][5]
+ * (row-major) and their inverses (JacobianDiagBlock::GetInvert, EulerJacobian.hpp:153-190);
+ * lamRoe [nFace][3] = lambdaFace0/123/4, or NULL for EvaluateDt's (lambdaFace[.][3]) */
+int orc_lusgs_init_block(const orc_vr *h, const orc_euler_settings *es, const double *dTau, double dt,
+                         double alphaDiag, const double *u, const double *lambdaFace, const double *lamRoe,
+                         double *jdiag, double *jinv);
+/* UpdateSGS with the block JDiag and the Roe off-diagonal terms */
+int orc_update_sgs_block(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                         const double *u, const double *uInc, double *uIncNew, const double *jinv,
+                         const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate,
+                         int uIncIsZero, const int64_t *scanOrder, double *sumInc);
 int orc_lusgs_init(const orc_vr *h, const double *dTau, double dt, double alphaDiag,
                    const double *lambdaFace, double *jdiag);
 int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDiag,

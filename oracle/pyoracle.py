@@ -41,7 +41,7 @@ class EulerSettings(C.Structure):
     _fields_ = [("gamma", C.c_double), ("rsType", C.c_int), ("rsMeanValueEig", C.c_int),
                 ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double), ("viscous", C.c_int),
                 ("muModel", C.c_int), ("Rgas", C.c_double), ("muGas", C.c_double), ("prGas", C.c_double),
-                ("TRef", C.c_double), ("CSutherland", C.c_double)]
+                ("TRef", C.c_double), ("CSutherland", C.c_double), ("useRoeJacobian", C.c_int)]
 
 
 class ImplicitSettings(C.Structure):
@@ -68,7 +68,8 @@ def lib():
         _lib.orc_rec_error_l1.restype = C.c_double
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
-                     "orc_eval_dt", "orc_eval_rhs", "orc_lusgs_init", "orc_update_sgs", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
+                     "orc_eval_dt", "orc_eval_rhs", "orc_eval_rhs_lam", "orc_lusgs_init", "orc_update_sgs",
+                     "orc_lusgs_init_block", "orc_update_sgs_block", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost",
                      "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp", "orc_model_rhs"]:
             getattr(_lib, name).argtypes = None
@@ -282,9 +283,9 @@ class OracleVR:
     # ---- Euler
     @staticmethod
     def euler_settings(gamma=1.4, rsType=1, rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0, viscous=0,
-                       muModel=1, Rgas=1.0, muGas=1.0, prGas=0.72, TRef=273.15, CSutherland=110.4):
+                       muModel=1, Rgas=1.0, muGas=1.0, prGas=0.72, TRef=273.15, CSutherland=110.4, useRoeJacobian=0):
         return EulerSettings(gamma, rsType, rsMeanValueEig, rsFixScale, rsIncFScale, viscous, muModel, Rgas, muGas,
-                             prGas, TRef, CSutherland)
+                             prGas, TRef, CSutherland, int(useRoeJacobian))
 
     def eval_dt(self, es, u, CFL, maxDt=1e100, useLocalDt=True):
         N, F = self.nCell, self.nFace
@@ -294,12 +295,37 @@ class OracleVR:
                                C.c_int(int(useLocalDt)), P(dt), P(lam), P(dlc), C.byref(dmin)))
         return dt, lam, dlc, dmin.value
 
-    def eval_rhs(self, es, u, uRec=None, deltaLambdaCell=None):
+    def eval_rhs(self, es, u, uRec=None, deltaLambdaCell=None, roeLambdas=False):
+        """EvaluateRHS; with roeLambdas also the face-averaged Riemann eigenvalues
+        lambdaFace0/123/4 [nFace, 3] that useRoeJacobian keeps (EvaluateRHS.hxx:527-544)."""
         rhs = np.zeros((self.nCell, 5))
-        _chk(lib().orc_eval_rhs(self.h, C.byref(es), P(np.ascontiguousarray(u)),
-                                None if uRec is None else P(np.ascontiguousarray(uRec)),
-                                None if deltaLambdaCell is None else P(np.ascontiguousarray(deltaLambdaCell)), P(rhs)))
-        return rhs
+        lr = np.zeros((self.nFace, 3)) if roeLambdas else None
+        _chk(lib().orc_eval_rhs_lam(self.h, C.byref(es), P(np.ascontiguousarray(u)),
+                                    None if uRec is None else P(np.ascontiguousarray(uRec)),
+                                    None if deltaLambdaCell is None else P(np.ascontiguousarray(deltaLambdaCell)),
+                                    P(rhs), P(lr)))
+        return (rhs, lr) if roeLambdas else rhs
+
+    def lusgs_init_block(self, es, dTau, dt, alphaDiag, u, lam, lamRoe=None):
+        """LUSGSMatrixInit with useRoeJacobian: (JDiag [N, 5, 5], its inverse [N, 5, 5])."""
+        jd, ji = np.zeros((self.nCell, 5, 5)), np.zeros((self.nCell, 5, 5))
+        _chk(lib().orc_lusgs_init_block(self.h, C.byref(es), P(np.ascontiguousarray(dTau)), C.c_double(dt),
+                                        C.c_double(alphaDiag), P(np.ascontiguousarray(u)), P(np.ascontiguousarray(lam)),
+                                        None if lamRoe is None else P(np.ascontiguousarray(lamRoe)), P(jd), P(ji)))
+        return jd, ji
+
+    def update_sgs_block(self, es, alphaDiag, rhs, u, uInc, uIncNew, jinv, lam, lamRoe, forward, gsUpdate,
+                         uIncIsZero, scanOrder=None):
+        out = np.array(uIncNew, dtype=np.float64, copy=True)
+        s = np.zeros(5)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        _chk(lib().orc_update_sgs_block(self.h, C.byref(es), C.c_double(alphaDiag), P(np.ascontiguousarray(rhs)),
+                                        P(np.ascontiguousarray(u)), P(np.ascontiguousarray(uInc)), P(out),
+                                        P(np.ascontiguousarray(jinv)), P(np.ascontiguousarray(lam)),
+                                        None if lamRoe is None else P(np.ascontiguousarray(lamRoe)),
+                                        C.c_int(int(forward)), C.c_int(int(gsUpdate)), C.c_int(int(uIncIsZero)), P(so),
+                                        P(s)))
+        return out, s
 
     def lusgs_init(self, dTau, dt, alphaDiag, lam):
         jd = np.zeros(self.nCell)
