@@ -38,7 +38,7 @@ class EulerSettings(C.Structure):
     _fields_ = [("gamma", C.c_double), ("rsType", C.c_int), ("rsMeanValueEig", C.c_int),
                 ("rsFixScale", C.c_double), ("rsIncFScale", C.c_double), ("viscous", C.c_int),
                 ("muModel", C.c_int), ("Rgas", C.c_double), ("muGas", C.c_double), ("prGas", C.c_double),
-                ("TRef", C.c_double), ("CSutherland", C.c_double)]
+                ("TRef", C.c_double), ("CSutherland", C.c_double), ("useRoeJacobian", C.c_int)]
 
 
 class ImplicitSettings(C.Structure):
@@ -79,6 +79,12 @@ SIGNATURES = {
                          C.c_int, vp],
     "dndsr_update_sgs_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                              C.c_int, vp],
+    "dndsr_eval_rhs_roe_lambda": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],
+    "dndsr_lusgs_init_block": [vp, C.POINTER(EulerSettings), vp, C.c_double, C.c_double, vp, vp, vp, vp, vp],
+    "dndsr_update_sgs_block": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, vp, C.c_int,
+                               C.c_int, C.c_int, vp],
+    "dndsr_update_sgs_block_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, vp, C.c_int,
+                                   C.c_int, C.c_int, vp],
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
     "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
                                   C.POINTER(C.c_int), dp, dp, dp],

@@ -44,6 +44,11 @@ struct dndsr_ctx
     int64_t ppLimited = 0;     // limited cell increments of the last implicit step
     double ppAlphaMin = 1.0;   // and their smallest alpha
     bool haveImplicit = false;
+    // Roe-Jacobian block LU-SGS workspace: face eigenvalues [3][F], inverse blocks [25][N]
+    // (FP64 and FP32), the singular-block flag
+    real *bLamRoe = nullptr, *bJinv = nullptr;
+    float *bJinvF = nullptr, *bLamRoeF = nullptr;
+    int *bBad = nullptr;
     // linear-solver workspace (GMRES Krylov basis for up to linK vectors + temporaries)
     LinWork lin;
     int linK = -1;
@@ -875,6 +880,53 @@ int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alpha
     CAPI_END
 }
 
+static int *blockFlag(dndsr_ctx *ctx)
+{
+    if (!ctx->bBad)
+    {
+        ctx->bBad = devAlloc<int>(ctx->c, 1);
+        DNDSR_HIP_CHECK(hipMemsetAsync(ctx->bBad, 0, sizeof(int), ctx->c.stream));
+    }
+    return ctx->bBad;
+}
+
+// a singular JDiag block raised the flag: clear it and report
+static void checkBlocks(dndsr_ctx *ctx, int flagHost)
+{
+    if (!flagHost)
+        return;
+    DNDSR_HIP_CHECK(hipMemsetAsync(ctx->bBad, 0, sizeof(int), ctx->c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(ctx->c.stream));
+    throw std::runtime_error("a block of JDiag is singular or has a zero diagonal entry (GetInvert)");
+}
+
+int dndsr_eval_rhs_roe_lambda(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                              const double *deltaLambdaCell, double *lamRoe)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(u && lamRoe, "null field");
+    launchFaceRoeLambda(ctx->c, toGas(es), u, uRec, deltaLambdaCell, lamRoe);
+    CAPI_END
+}
+
+int dndsr_lusgs_init_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *dTau, double dt,
+                           double alphaDiag, const double *u, const double *lambdaFace, const double *lamRoe,
+                           double *jdiag, double *jdiagInv)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(dTau && u && lambdaFace && jdiagInv, "null field");
+    Ctx &c = ctx->c;
+    int *bad = blockFlag(ctx);
+    launchLUSGSInitBlock(c, toGas(es), dTau, dt, alphaDiag, u, lambdaFace, lamRoe, jdiag, jdiagInv, bad);
+    int h = 0;
+    DNDSR_HIP_CHECK(hipMemcpyAsync(&h, bad, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+    checkBlocks(ctx, h);
+    CAPI_END
+}
+
 static void fetchSum(Ctx &c, real *sumIncHost)
 {
     if (!sumIncHost)
@@ -915,6 +967,40 @@ int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float a
     real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
     launchSGSSweep32(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
                      sumDev);
+    fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
+int dndsr_update_sgs_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
+                           const double *u, const double *uInc, double *uIncNew, const double *jdiagInv,
+                           const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate, int uIncIsZero,
+                           double *sumIncHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && uInc && uIncNew && jdiagInv && lambdaFace, "null field");
+    need(uInc != uIncNew, "uInc and uIncNew must be different arrays");
+    Ctx &c = ctx->c;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    launchSGSSweepBlock64(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiagInv, lambdaFace, lamRoe, forward,
+                          gsUpdate, uIncIsZero, sumDev);
+    fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
+int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
+                               const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
+                               const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,
+                               int uIncIsZero, double *sumIncHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && uInc && uIncNew && jdiagInv && lambdaFace, "null field");
+    need(uInc != uIncNew, "uInc and uIncNew must be different arrays");
+    Ctx &c = ctx->c;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    launchSGSSweepBlock32(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiagInv, lambdaFace, lamRoe, forward,
+                          gsUpdate, uIncIsZero, sumDev);
     fetchSum(c, sumIncHost);
     CAPI_END
 }
@@ -1024,12 +1110,25 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
     need(is->gsUpdate == 0 || is->gsUpdate == 1,
          "gsUpdate (jacobiCode) must be 0 (Jacobi) or 1 (Gauss-Seidel); jacobiCode 2 (ILU) is not provided");
     need(is->incrementRelax >= 0 && is->incrementRelax <= 1, "incrementRelax must lie in [0, 1]");
+    const bool roeJ = es->useRoeJacobian != 0;
+    need(!roeJ || is->gmresCode == 0, "useRoeJacobian runs with the LU-SGS solver (gmresCode 0)");
     Ctx &c = ctx->c;
     DevMesh &m = c.m;
     GasParams gp = toGas(es);
     const size_t N = m.N, F = m.F;
     const size_t nU = 5 * N, nR = 5 * (size_t)m.M * N;
     timeLoopWork(ctx);
+    if (roeJ && !ctx->bJinv)
+    {
+        ctx->bLamRoe = devAlloc<real>(c, 3 * F);
+        ctx->bJinv = devAlloc<real>(c, 25 * N);
+    }
+    if (roeJ && is->fp32Sweeps && !ctx->bJinvF)
+    {
+        ctx->bJinvF = devAlloc<float>(c, 25 * N);
+        ctx->bLamRoeF = devAlloc<float>(c, 3 * F);
+    }
+    int *bBad = roeJ ? blockFlag(ctx) : nullptr;
     if (!ctx->haveImplicit)
     {
         ctx->iXLast = devAlloc<real>(c, nU);
@@ -1078,10 +1177,54 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         for (int r = 0; r < is->nRec; r++)
             solverVRSweep(c, rec, recNew, u);
         launchEvalRHS(c, gp, u, rec, ctx->wDlc, ctx->iRhsF);
+        if (roeJ) // the face eigenvalues EvaluateRHS keeps for the Roe Jacobian
+            launchFaceRoeLambda(c, gp, u, rec, ctx->wDlc, ctx->bLamRoe);
         launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
         // fsolve: LUSGSMatrixInit + (1 + sgsIter) symmetric sweeps from a zero increment
-        launchLUSGSInit(c, ctx->iDTau, is->dt, 1.0, ctx->wLam, ctx->iJd);
-        if (is->gmresCode != 0)
+        if (roeJ)
+            launchLUSGSInitBlock(c, gp, ctx->iDTau, is->dt, 1.0, u, ctx->wLam, ctx->bLamRoe, nullptr, ctx->bJinv,
+                                 bBad);
+        else
+            launchLUSGSInit(c, ctx->iDTau, is->dt, 1.0, ctx->wLam, ctx->iJd);
+        if (roeJ)
+        {
+            // block sweeps in colour order, ping-ponging the increment from zero
+            real *sumDev = c.red + 5 * ((size_t)gridFor(m.N, 256) + m.nColor + 6);
+            if (is->fp32Sweeps)
+            {
+                launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
+                launchToF32(c, nU, u, ctx->fU);
+                launchToF32(c, 25 * N, ctx->bJinv, ctx->bJinvF);
+                launchToF32(c, 4 * F, ctx->wLam, ctx->fLam);
+                launchToF32(c, 3 * F, ctx->bLamRoe, ctx->bLamRoeF);
+                float *a = ctx->fInc, *b = ctx->fTmp;
+                DNDSR_HIP_CHECK(hipMemsetAsync(a, 0, nU * sizeof(float), c.stream));
+                DNDSR_HIP_CHECK(hipMemsetAsync(b, 0, nU * sizeof(float), c.stream));
+                for (int s = 0; s <= is->sgsIter; s++)
+                    for (int dir = 1; dir >= 0; dir--)
+                    {
+                        launchSGSSweepBlock32(c, gp, 1.0f, ctx->fRhs, ctx->fU, a, b, ctx->bJinvF, ctx->fLam,
+                                              ctx->bLamRoeF, dir, is->gsUpdate, 0, sumDev);
+                        std::swap(a, b);
+                    }
+                launchAddIncrement32(c, gamma, is->incrementRelax, rhoEps, pEps, u, a, ctx->iStat);
+            }
+            else
+            {
+                real *a = ctx->iInc, *b = ctx->iTmp;
+                DNDSR_HIP_CHECK(hipMemsetAsync(a, 0, nU * sizeof(real), c.stream));
+                DNDSR_HIP_CHECK(hipMemsetAsync(b, 0, nU * sizeof(real), c.stream));
+                for (int s = 0; s <= is->sgsIter; s++)
+                    for (int dir = 1; dir >= 0; dir--)
+                    {
+                        launchSGSSweepBlock64(c, gp, 1.0, ctx->iRhs, u, a, b, ctx->bJinv, ctx->wLam, ctx->bLamRoe,
+                                              dir, is->gsUpdate, 0, sumDev);
+                        std::swap(a, b);
+                    }
+                launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, a, ctx->iStat);
+            }
+        }
+        else if (is->gmresCode != 0)
         {
             // solveLinear with gmresCode 1 (GMRES from zero) or 2 (LU-SGS result as the start),
             // EulerSolver.hxx:1407-1578; the preconditioner sweeps in FP32 when fp32Sweeps
@@ -1126,9 +1269,18 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         }
         // fstop: residual of this iteration relative to its running maximum (density component)
         PPStat st;
+        int blockBad = 0;
         DNDSR_HIP_CHECK(hipMemcpyAsync(c.hostRed, ctx->iRes, 5 * sizeof(real), hipMemcpyDeviceToHost, c.stream));
         DNDSR_HIP_CHECK(hipMemcpyAsync(&st, ctx->iStat, sizeof(PPStat), hipMemcpyDeviceToHost, c.stream));
+        if (bBad)
+            DNDSR_HIP_CHECK(hipMemcpyAsync(&blockBad, bBad, sizeof(int), hipMemcpyDeviceToHost, c.stream));
         DNDSR_HIP_CHECK(hipStreamSynchronize(c.stream));
+        if (blockBad)
+        {
+            DNDSR_HIP_CHECK(hipMemsetAsync(bBad, 0, sizeof(int), c.stream));
+            fail("a block of JDiag is singular or has a zero diagonal entry (iteration " + std::to_string(it) +
+                 "); u and uRec are left as they were at entry");
+        }
         for (int v = 0; v < 5; v++)
             if (!std::isfinite(c.hostRed[v]))
                 fail("non-finite residual in the implicit step (iteration " + std::to_string(it) +

@@ -1,6 +1,7 @@
 // euler.hip -- EulerEvaluator hot path on the device: EvaluateDt, EvaluateRHS,
 // LUSGSMatrixInit, the multicolour symmetric Gauss-Seidel sweeps and the
 // SSP-RK3 stage combination.
+#include <cstdlib>
 #include <type_traits>
 #include "gas.hpp"
 #include "internal.hpp"
@@ -814,6 +815,272 @@ kRHSFaceW(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
     }
 }
 
+// ---------------------------------------------------------------------------
+// Side-sequential face tile kernel: as kRHSFaceW (NW waves per 64-face tile, PPW
+// points per lane, direct-to-LDS row staging), but the two sides of the tile are
+// staged one after the other into the same LDS rows: the left cells' mean,
+// coefficients and metrics land, every wave reconstructs its points' left states,
+// then the right cells' rows (and the periodic shift) replace them and the right
+// states follow.  The LDS of a tile is one side's rows (5 + 5 M + 4 + M + 3), half
+// of the two-sided kernel's, so twice as many tiles share a compute unit and hide
+// each other's staging latency.  Every wave forms the face constants (Roe average of
+// the means, batched-Roe eigenvalues, dLambda) itself from the means it read, so
+// they take no LDS.  The point fluxes are parked in the staging rows once both
+// sides are reconstructed and summed in point order.
+// ---------------------------------------------------------------------------
+template <int NDOF>
+struct RHSSideRows
+{
+    static constexpr int M = NDOF - 1;
+    static constexpr int MEAN = 0;            // [5]
+    static constexpr int REC = 5;             // [5][M]
+    static constexpr int MET = REC + 5 * M;   // centre 3, 1/scale, moment M
+    static constexpr int SHIFT = MET + 4 + M; // [3], right side only
+    static constexpr int STAGED = SHIFT + 3;
+};
+
+template <int NDOF, int Q>
+__device__ __forceinline__ void rhsSideRow(const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                           const uint32_t *oc, const uint32_t *of)
+{
+    using R = RHSSideRows<NDOF>;
+    const size_t N = m.N, F = m.F;
+    const uint64_t rowN = (uint64_t)N * 8;
+    real *dst = lds + Q * kTileF;
+    const real *base;
+    uint64_t bytes = rowN;
+    const uint32_t *o = oc;
+    if constexpr (Q < R::REC)
+        base = u + (size_t)Q * N;
+    else if constexpr (Q < R::MET)
+    {
+        if (!uRec) // without coefficients these rows are never read
+            return;
+        base = uRec + (size_t)(Q - R::REC) * N;
+    }
+    else if constexpr (Q < R::SHIFT)
+    {
+        constexpr int k = Q - R::MET;
+        base = k < 3 ? m.cent + (size_t)k * N : (k == 3 ? m.iscale : m.moment + (size_t)(k - 4) * N);
+    }
+    else
+        base = m.fshift + (size_t)(Q - R::SHIFT) * F, bytes = (uint64_t)F * 8, o = of;
+    const Rsrc rs = rowRsrc(base, bytes);
+#pragma unroll
+    for (int h = 0; h < 2; h++)
+        __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, o[h], 0, 0, 0);
+}
+template <int NDOF, int W, int NS, int T, int NR>
+__device__ __forceinline__ void rhsSideSeq(const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                           const uint32_t *oc, const uint32_t *of)
+{
+    constexpr int Q = W + T * NS;
+    if constexpr (Q < NR)
+    {
+        rhsSideRow<NDOF, Q>(m, u, uRec, lds, oc, of);
+        rhsSideSeq<NDOF, W, NS, T + 1, NR>(m, u, uRec, lds, oc, of);
+    }
+}
+template <int NDOF, int NS, int W, int NR>
+__device__ __forceinline__ void rhsSideDispatch(int w, const DevMesh &m, const real *u, const real *uRec, real *lds,
+                                                const uint32_t *oc, const uint32_t *of)
+{
+    if constexpr (W < NS)
+    {
+        if (w == W)
+            rhsSideSeq<NDOF, W, NS, 0, NR>(m, u, uRec, lds, oc, of);
+        else
+            rhsSideDispatch<NDOF, NS, W + 1, NR>(w, m, u, uRec, lds, oc, of);
+    }
+}
+
+// one side's reconstructed states at the lane's PPW points from the staged rows
+template <int DIM, int NDOF, int PPW, bool SH>
+__device__ __forceinline__ void rhsSideStates(const real *lds, int lane, const real (&x)[PPW][3], bool haveRec,
+                                              real (&Um)[5], real (&U)[PPW][5])
+{
+    using R = RHSSideRows<NDOF>;
+    constexpr int M = R::M;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        Um[v] = lds[(R::MEAN + v) * kTileF + lane];
+#pragma unroll
+        for (int p = 0; p < PPW; p++)
+            U[p][v] = Um[v];
+    }
+    if (!haveRec)
+        return;
+    const real *met = lds + R::MET * kTileF + lane;
+    const real isc = met[3 * kTileF];
+    real cen[3], mom[M > 0 ? M : 1], phi[PPW][M > 0 ? M : 1];
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+    {
+        cen[r] = met[r * kTileF];
+        if constexpr (SH)
+            cen[r] -= lds[(R::SHIFT + r) * kTileF + lane]; // x + shift - cen
+    }
+#pragma unroll
+    for (int j = 0; j < M; j++)
+        mom[j] = met[(4 + j) * kTileF];
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        real xs[3];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+        {
+            xs[r] = x[p][r] - cen[r];
+            if (DIM == 3 || r < 2)
+                xs[r] *= isc;
+        }
+        basisRow0<DIM, NDOF>(xs, mom, phi[p]);
+    }
+    const real *rec = lds + R::REC * kTileF + lane;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+    {
+        real sacc[PPW];
+#pragma unroll
+        for (int p = 0; p < PPW; p++)
+            sacc[p] = 0;
+#pragma unroll
+        for (int j = 0; j < M; j++)
+        {
+            const real c = rec[(v * M + j) * kTileF];
+#pragma unroll
+            for (int p = 0; p < PPW; p++)
+                sacc[p] += phi[p][j] * c;
+        }
+#pragma unroll
+        for (int p = 0; p < PPW; p++)
+            U[p][v] += sacc[p];
+    }
+}
+
+template <int DIM, int NDOF, int G, int RS, bool BATCH, int PPW, int WPE>
+__global__ void __launch_bounds__(kTileF *((G + PPW - 1) / PPW)) __attribute__((amdgpu_waves_per_eu(WPE)))
+kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__restrict__ uRec,
+          const real *__restrict__ dlc, real *__restrict__ fflux)
+{
+    using R = RHSSideRows<NDOF>;
+    constexpr int NW = (G + PPW - 1) / PPW;
+    extern __shared__ real lds[];
+    const int lane = threadIdx.x % kTileF, g = waveIndex(kTileF);
+    const size_t N = m.N, F = m.F;
+    const uint64_t rowN = (uint64_t)N * 8, rowF = (uint64_t)F * 8;
+    const int tile = xcdTile(blockIdx.x, gridDim.x);
+    const idx f = tile * kTileF + lane;
+    const bool valid = f < m.F;
+    const uint32_t fo = (uint32_t)(valid ? f : 0) * 8u;
+    const idx cL = bldi(rowRsrc(m.f2c, (uint64_t)F * 4), fo / 2);
+    const idx cR = bldi(rowRsrc(m.f2c + F, (uint64_t)F * 4), fo / 2);
+    const uint32_t oL = (uint32_t)cL * 8u, oR = (uint32_t)cR * 8u;
+    // byte offsets of the direct-to-LDS loads: lane l carries dword (l & 1) of rows (l >> 1)
+    // and 32 + (l >> 1) of the tile, so a row of 64 doubles is two dword loads
+    uint32_t ocL[2], ocR[2], of[2];
+    {
+        const int hl = lane >> 1;
+        const uint32_t par = (uint32_t)(lane & 1) * 4u;
+#pragma unroll
+        for (int h = 0; h < 2; h++)
+        {
+            ocL[h] = (uint32_t)__shfl((int)oL, 32 * h + hl) + par;
+            ocR[h] = (uint32_t)__shfl((int)oR, 32 * h + hl) + par;
+            const idx f2 = tile * kTileF + 32 * h + hl;
+            of[h] = (uint32_t)(f2 < m.F ? f2 : 0) * 8u + par;
+        }
+    }
+    rhsSideDispatch<NDOF, NW, 0, R::SHIFT>(g, m, u, uRec, lds, ocL, of);
+    // the lane's point coordinates (streamed once)
+    real x[PPW][3];
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        const int q = g * PPW + p < G ? g * PPW + p : G - 1; // the last wave may hold fewer points
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            x[p][r] = bldnt(rowRsrc(m.fpts + ((size_t)q * 3 + r) * F, rowF), fo);
+    }
+    real dLambda = 0;
+    if (dlc)
+    {
+        const Rsrc rr = rowRsrc(dlc, rowN);
+        dLambda = fmax(bld(rr, oL), bld(rr, oR));
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    real uLm[5], uRm[5], UL[PPW][5], UR[PPW][5];
+    rhsSideStates<DIM, NDOF, PPW, false>(lds, lane, x, uRec != nullptr, uLm, UL);
+    __syncthreads(); // the left rows are consumed
+    rhsSideDispatch<NDOF, NW, 0, R::STAGED>(g, m, u, uRec, lds, ocR, of);
+    // the lane's point normals and weights, in flight with the right side's rows
+    real n[PPW][3], wj[PPW];
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        const int q = g * PPW + p < G ? g * PPW + p : G - 1;
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            n[p][r] = bldnt(rowRsrc(m.fnorm + ((size_t)q * 3 + r) * F, rowF), fo);
+        wj[p] = bldnt(rowRsrc(m.fwj + (size_t)q * F, rowF), fo);
+    }
+    real nm[3];
+    if constexpr (BATCH)
+    {
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            nm[r] = bld(rowRsrc(m.fmeanN + (size_t)r * F, rowF), fo);
+    }
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    rhsSideStates<DIM, NDOF, PPW, true>(lds, lane, x, uRec != nullptr, uRm, UR);
+    __syncthreads(); // the right rows are consumed: the staging rows take the point fluxes
+    if (!valid)
+    {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            uLm[v] = uRm[v] = v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0);
+    }
+    RoeBatchFace bf;
+    if constexpr (BATCH)
+        roeBatchSetup<RS>(uLm, uRm, nm, gp, dLambda, bf);
+    else
+        roeAverage(uLm, uRm, gp.gamma, bf.r);
+#pragma unroll
+    for (int p = 0; p < PPW; p++)
+    {
+        const int q = g * PPW + p;
+        if (q >= G)
+            continue;
+        real Fl[5];
+        if constexpr (BATCH)
+            roeBatchPoint<RS>(UL[p], UR[p], n[p], gp, bf, Fl);
+        else
+        {
+            real l0, l1, l4;
+            riemannFluxR<RS>(UL[p], UR[p], bf.r, n[p], gp, dLambda, Fl, l0, l1, l4);
+        }
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            lds[(q * 5 + v) * kTileF + lane] = valid ? (-Fl[v]) * wj[p] : 0.0;
+    }
+    __syncthreads();
+    for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
+    {
+        const int v = t / kTileF, l = t % kTileF;
+        const idx fout = tile * kTileF + l;
+        if (fout >= m.F)
+            continue;
+        real acc = 0;
+#pragma unroll
+        for (int q = 0; q < G; q++)
+            acc += lds[(q * 5 + v) * kTileF + l];
+        fflux[(size_t)v * F + fout] = acc;
+    }
+}
+
 // points per lane of the face tile kernel for a face rule of G points (1: one
 // wavefront per point)
 template <int DIM, int G>
@@ -1170,6 +1437,27 @@ static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uR
         rows = 5 * G;
     size_t smem = (size_t)rows * kTileF * sizeof(real);
     constexpr int PPW = rhsPointsPerLane<DIM, G>();
+    static const int variant = getenv("DNDSR_RHS_VARIANT") ? atoi(getenv("DNDSR_RHS_VARIANT")) : 0;
+    if constexpr (PPW > 1)
+        if (variant > 0)
+        {
+            constexpr int NW = (G + PPW - 1) / PPW;
+            int srows = RHSSideRows<NDOF>::STAGED;
+            if (srows < 5 * G)
+                srows = 5 * G;
+            const size_t ss = (size_t)srows * kTileF * sizeof(real);
+            auto k3 = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, 3>;
+            auto k4 = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, 4>;
+            auto kern = variant == 1 ? k3 : k4;
+            kern<<<gridFor(c.m.F, kTileF), kTileF * NW, ss, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+            DNDSR_HIP_CHECK(hipGetLastError());
+            if (c.m.nBnd > 0)
+            {
+                kRHSBoundary<DIM, NDOF, G, RS, BATCH><<<gridFor(c.m.nBnd, 64), 64, 0, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+                DNDSR_HIP_CHECK(hipGetLastError());
+            }
+            return;
+        }
     if constexpr (PPW > 1)
     {
         constexpr int NW = (G + PPW - 1) / PPW;
@@ -1473,6 +1761,12 @@ __global__ void __launch_bounds__(256) kReduceSum5(const real *part, int nPart, 
 #pragma unroll
         for (int v = 0; v < 5; v++)
             out[v] = sm[v][0];
+}
+
+void launchReduceSum5(Ctx &c, int nPart, real *out)
+{
+    kReduceSum5<<<1, 256, 0, c.stream>>>(c.red, nPart, out);
+    DNDSR_HIP_CHECK(hipGetLastError());
 }
 
 template <typename T>

@@ -38,6 +38,26 @@ __host__ __device__ inline real signP(T a) { return a >= 0 ? 1 : -1; }
 template <typename T>
 __host__ __device__ inline T max3(T a, T b, T c) { return fmax(a, fmax(b, c)); }
 
+// Reciprocal for the flux arithmetic.  FP64 on the device: the hardware estimate
+// (v_rcp_f64) refined by two Newton steps r <- r + r (1 - x r), each squaring the relative
+// error, so the result is the correctly rounded 1/x to within an ulp -- six VALU operations
+// against the ten of the IEEE division sequence.  FP32 and the host take the quotient.
+template <typename T>
+__host__ __device__ inline T rcpN(T x)
+{
+#if defined(__HIP_DEVICE_COMPILE__)
+    if constexpr (sizeof(T) == 8)
+    {
+        double r = __builtin_amdgcn_rcp(x);
+        double e = fma(-x, r, 1.0);
+        r = fma(r, e, r);
+        e = fma(-x, r, 1.0);
+        return fma(r, e, r);
+    }
+#endif
+    return T(1.0) / x;
+}
+
 // IdealGasThermal (src/DNDS/IdealGasPhysics.hpp:37-44).  Divisions by rho are
 // taken as products with one reciprocal (FP64 division costs ~10 VALU ops on
 // CDNA; the product differs from the quotient by at most an ulp).
@@ -81,7 +101,7 @@ template <typename T>
 __host__ __device__ inline void roeAverage(const T *ULm, const T *URm, real gamma, RoeAveT<T> &r)
 {
     T HLm, HRm, aLm2, aRm2;
-    const T rinvL = T(1.0) / ULm[0], rinvR = T(1.0) / URm[0];
+    const T rinvL = rcpN(ULm[0]), rinvR = rcpN(URm[0]);
 #pragma unroll
     for (int d = 0; d < 3; d++)
         r.veloLm[d] = ULm[1 + d] * rinvL, r.veloRm[d] = URm[1 + d] * rinvR;
@@ -90,7 +110,7 @@ __host__ __device__ inline void roeAverage(const T *ULm, const T *URm, real gamm
     r.aLm = sqrt(aLm2);
     r.aRm = sqrt(aRm2);
     T sL = sqrt(ULm[0]), sR = sqrt(URm[0]);
-    const T sInv = T(1.0) / (sL + sR);
+    const T sInv = rcpN(sL + sR);
 #pragma unroll
     for (int d = 0; d < 3; d++)
         r.veloRoe[d] = (sL * r.veloLm[d] + sR * r.veloRm[d]) * sInv;
@@ -99,8 +119,8 @@ __host__ __device__ inline void roeAverage(const T *ULm, const T *URm, real gamm
     r.asqrRoe = (gamma - 1) * (r.HRoe - 0.5 * r.vsqrRoe);
     r.rhoRoe = sL * sR;
     r.aRoe = sqrt(r.asqrRoe); // NaN when asqrRoe <= 0; the NaN flag of the caller catches it
-    r.g1a2 = (gamma - 1) / r.asqrRoe;
-    r.inv2a = 0.5 / r.aRoe;
+    r.g1a2 = (gamma - 1) * rcpN(r.asqrRoe);
+    r.inv2a = 0.5 * rcpN(r.aRoe);
 }
 
 // Entropy fixes (Roe_EntropyFixer, Gas.hpp:878-1007)
@@ -206,7 +226,7 @@ __host__ __device__ inline void roeDissipation(const T *dU, const real *n, const
 template <typename T>
 __host__ __device__ inline void sideFlux(const T *U, const real *n, real gamma, T *F, T &p, T &a)
 {
-    const T rinv = T(1.0) / U[0];
+    const T rinv = rcpN(U[0]);
     T v[3] = {U[1] * rinv, U[2] * rinv, U[3] * rinv};
     T a2, H;
     thermalR(U[4], U[0], rinv, dot3(v, v), gamma, p, a2, H);
@@ -269,21 +289,21 @@ __host__ __device__ inline void riemannFluxR(const T *UL, const T *UR, const Roe
         T al1 = r.g1a2 * (dU[0] * (r.HRoe - vRN * vRN) + vRN * dUn - dU4b);
         T SL = fmin(vRN - r.aRoe, vLm0 - aL);
         T SR = fmax(vRN + r.aRoe, vRm0 + aR);
-        T dfix = r.aRoe / (r.aRoe + fmax(fabs(vRN), dLambda * gp.fixScale));
+        T dfix = r.aRoe * rcpN(r.aRoe + fmax(fabs(vRN), dLambda * gp.fixScale));
         T SP = fmax(SR, T(0.0)), SM = fmin(SL, T(0.0));
         if constexpr (RS == RS_HLLEP)
         {
             T rev[5] = {T(1), r.veloRoe[0], r.veloRoe[1], r.veloRoe[2], r.vsqrRoe * 0.5};
             T div = SP - SM;
             div += signP(div) * kVerySmall;
-            const T dinv = T(1.0) / div, spsm = SP * SM * dinv, dal = dfix * al1;
+            const T dinv = rcpN(div), spsm = SP * SM * dinv, dal = dfix * al1;
 #pragma unroll
             for (int i = 0; i < 5; i++)
                 F[i] = (SP * FL[i] - SM * FR[i]) * dinv + spsm * (dU[i] - rev[i] * dal);
         }
         else
         {
-            const T dinv = T(1.0) / (SP - SM);
+            const T dinv = rcpN(SP - SM);
             lam123 = fabs(((SP + SM) * vRN - 2.0 * (1.0 - dfix) * (SP * SM)) * dinv);
             lam4 = fabs(((SP + SM) * (vRN + r.aRoe) - 2.0 * (SP * SM)) * dinv);
             lam0 = fabs(((SP + SM) * (vRN - r.aRoe) - 2.0 * (SP * SM)) * dinv);

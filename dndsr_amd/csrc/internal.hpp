@@ -224,6 +224,19 @@ void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float 
                       const float *uInc, float *uIncNew, const float *jdiag, const float *lambdaFace, int forward,
                       int gsUpdate, int uIncIsZero, real *sumIncDev);
 // out = a x + b y + cdt dt z (y, z, dtDev optional); save, when given, receives x first
+// Roe-Jacobian block LU-SGS (useRoeJacobian): face eigenvalues lamRoe [3][F], block JDiag and its
+// inverse [25][N] (bad set when a block cannot be inverted), block sweeps
+void launchFaceRoeLambda(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *lamRoe);
+void launchLUSGSInitBlock(Ctx &c, const GasParams &gp, const real *dTau, real dt, real alphaDiag, const real *u,
+                          const real *lam, const real *lamRoe, real *jd, real *jinv, int *bad);
+void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
+                           const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
+                           int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
+void launchSGSSweepBlock32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
+                           const float *uInc, float *uIncNew, const float *jinv, const float *lam,
+                           const float *lamRoe, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
+// sum of nPart 5-wide partials in c.red into out[5]
+void launchReduceSum5(Ctx &c, int nPart, real *out);
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
                  const real *dtDev, real *out, real *save = nullptr);
 // dst[0:n) = src[0:n) on the library's stream

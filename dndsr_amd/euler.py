@@ -32,12 +32,31 @@ def _check_jacobi_code(code):
                          "2 (ILU) is not provided")
 
 
+class JacobianDiagBlock:
+    """5x5 diagonal blocks of the LU-SGS matrix and their inverses, [25, nCell] each (row-major
+    block per cell), the block mode of the reference's JacobianDiagBlock
+    (src/Euler/EulerJacobian.hpp:42-260) that useRoeJacobian needs.  ``value`` is the block,
+    ``invert`` its inverse (GetInvert, :153-190); dtype float64, or float32 for the FP32 sweep."""
+
+    def __init__(self, nCell, device, dtype=torch.float64):
+        self.value = torch.zeros((25, nCell), dtype=dtype, device=device)
+        self.invert = torch.zeros((25, nCell), dtype=dtype, device=device)
+
+    def to(self, dtype):
+        out = JacobianDiagBlock.__new__(JacobianDiagBlock)
+        out.value, out.invert = self.value.to(dtype), self.invert.to(dtype)
+        return out
+
+    def getBlock(self, iCell):
+        return self.value[:, iCell].reshape(5, 5)
+
+
 class EulerEvaluator:
     nVars = 5
 
     def __init__(self, vfv: VariationalReconstruction, settings: dict | None = None):
         s = dict(gamma=1.4, riemannSolverType="Roe", rsMeanValueEig=0, rsFixScale=1.0, rsIncFScale=1.0,
-                 viscous=False)
+                 viscous=False, useRoeJacobian=False)
         gas = dict(Rgas=1.0, muGas=1.0, prGas=0.72, TRef=273.15, CSutherland=110.4, muModel=1)
         if settings:
             s.update(settings)
@@ -52,7 +71,7 @@ class EulerEvaluator:
                                      int(s["rsMeanValueEig"]), float(s["rsFixScale"]), float(s["rsIncFScale"]),
                                      int(bool(s["viscous"])), int(gas["muModel"]), float(gas["Rgas"]),
                                      float(gas["muGas"]), float(gas["prGas"]), float(gas["TRef"]),
-                                     float(gas["CSutherland"]))
+                                     float(gas["CSutherland"]), int(bool(s["useRoeJacobian"])))
         bcTypes = s.get("bcTypes")
         if bcTypes is not None:
             # boundary weights of the VR functional follow the boundary types, as the
@@ -73,6 +92,8 @@ class EulerEvaluator:
                                      None if v is None else v.ctypes.data_as(_lib.dp), float(s["gamma"])))
         dev = self.ctx.torch_device
         self.lambdaFace = torch.zeros((4, vfv.nFace), dtype=torch.float64, device=dev)
+        # lambdaFace0/123/4 of the Roe Jacobian, written by EvaluateRHS when useRoeJacobian
+        self.lambdaFaceRoe = torch.zeros((3, vfv.nFace), dtype=torch.float64, device=dev)
         self.deltaLambdaCell = torch.zeros(vfv.nCell, dtype=torch.float64, device=dev)
         self.dtMinDev = torch.zeros(1, dtype=torch.float64, device=dev)
 
@@ -93,13 +114,30 @@ class EulerEvaluator:
         rec = uRec if uRec is not None else uRecUnlim
         check(lib().dndsr_eval_rhs(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(rec), _ptr(self.deltaLambdaCell),
                                    _ptr(rhs)))
+        if self.settings["useRoeJacobian"]:  # EvaluateRHS.hxx:527-544
+            check(lib().dndsr_eval_rhs_roe_lambda(self.ctx.h, C.byref(self.es), _ptr(u), _ptr(rec),
+                                                  _ptr(self.deltaLambdaCell), _ptr(self.lambdaFaceRoe)))
         if check_finite:
             self.ctx.sync()
 
     # LUSGSMatrixInit(JDiag, JSource, dTau, dt, alphaDiag, u, uRec, jacobianCode, t)
+    def BuildJacobianDiagBlock(self):
+        """The JDiag container of useRoeJacobian (block mode)."""
+        return JacobianDiagBlock(self.vfv.nCell, self.lambdaFace.device)
+
     def LUSGSMatrixInit(self, JDiag, JSource, dTau, dt, alphaDiag, u=None, uRec=None, jacobianCode=0, t=0.0):
+        """Scalar JDiag (a [nCell] tensor), or with useRoeJacobian a JacobianDiagBlock filled with the
+        Roe-Jacobian blocks and their inverses (EulerEvaluator.hxx:37-159); the block mode uses the
+        face eigenvalues of the last EvaluateRHS."""
         if jacobianCode != 0:
             raise ValueError("jacobianCode must be 0")
+        if self.settings["useRoeJacobian"]:
+            if not isinstance(JDiag, JacobianDiagBlock) or u is None:
+                raise ValueError("useRoeJacobian needs a JacobianDiagBlock (BuildJacobianDiagBlock) and u")
+            check(lib().dndsr_lusgs_init_block(self.ctx.h, C.byref(self.es), _ptr(dTau), float(dt),
+                                               float(alphaDiag), _ptr(u), _ptr(self.lambdaFace),
+                                               _ptr(self.lambdaFaceRoe), _ptr(JDiag.value), _ptr(JDiag.invert)))
+            return
         check(lib().dndsr_lusgs_init(self.ctx.h, _ptr(dTau), float(dt), float(alphaDiag), _ptr(self.lambdaFace),
                                      _ptr(JDiag)))
 
@@ -108,6 +146,19 @@ class EulerEvaluator:
                   lambdaFace=None):
         s = np.zeros(5)
         lam = self.lambdaFace if lambdaFace is None else lambdaFace
+        if isinstance(JDiag, JacobianDiagBlock):
+            if rhs.dtype == torch.float32:
+                lr = self.lambdaFaceRoe.float()
+                check(lib().dndsr_update_sgs_block_f32(
+                    self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u), _ptr(uInc), _ptr(uIncNew),
+                    _ptr(JDiag.invert), _ptr(lam), _ptr(lr), int(forward), int(gsUpdate), int(uIncIsZero),
+                    s.ctypes.data_as(_lib.dp)))
+            else:
+                check(lib().dndsr_update_sgs_block(
+                    self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u), _ptr(uInc), _ptr(uIncNew),
+                    _ptr(JDiag.invert), _ptr(lam), _ptr(self.lambdaFaceRoe), int(forward), int(gsUpdate),
+                    int(uIncIsZero), s.ctypes.data_as(_lib.dp)))
+            return s
         if rhs.dtype == torch.float32:
             check(lib().dndsr_update_sgs_f32(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u),
                                              _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(lam), int(forward),
@@ -145,6 +196,8 @@ class EulerEvaluator:
 
     # LUSGSMatrixVec(alphaDiag, t, u, uInc, JDiag, AuInc), EulerEvaluator.hpp:449-455
     def LUSGSMatrixVec(self, alphaDiag, t, u, uInc, JDiag, AuInc, lambdaFace=None):
+        if isinstance(JDiag, JacobianDiagBlock):
+            raise ValueError("LUSGSMatrixVec is provided for the scalar Jacobian only")
         lam = self.lambdaFace if lambdaFace is None else lambdaFace
         check(lib().dndsr_lusgs_matvec(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(u), _ptr(uInc),
                                        _ptr(JDiag), _ptr(lam), _ptr(AuInc)))
@@ -154,6 +207,8 @@ class EulerEvaluator:
         """GMRES_LeftPreconditioned::solve (Linear.hpp:47-131) on the LU-SGS matrix with the
         LU-SGS preconditioner (FP32 sweeps when fp32Precond); x: initial guess -> solution.
         Returns (last preconditioned residual norm, |M b|)."""
+        if isinstance(JDiag, JacobianDiagBlock):
+            raise ValueError("GMRES is provided for the scalar Jacobian only")
         lam = self.lambdaFace if lambdaFace is None else lambdaFace
         _check_jacobi_code(jacobiCode)
         r = np.zeros(2)

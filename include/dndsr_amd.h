@@ -111,6 +111,10 @@ typedef struct
     double prGas;
     double TRef;
     double CSutherland;
+    /* useRoeJacobian (EulerEvaluatorSettings.hpp:54): the implicit step builds 5x5 JDiag blocks with
+     * the Roe |A| and sweeps with the Roe off-diagonal terms (dndsr_lusgs_init_block,
+     * dndsr_update_sgs_block); 0 = the scalar Jacobian */
+    int useRoeJacobian;
 } dndsr_euler_settings;
 
 /* ---- context ---------------------------------------------------------- */
@@ -233,6 +237,33 @@ int dndsr_update_sgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alph
 int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                          const float *u, const float *uInc, float *uIncNew, const float *jdiag,
                          const float *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
+/* The Roe-Jacobian block LU-SGS (useRoeJacobian).
+ * dndsr_eval_rhs_roe_lambda: the face eigenvalues EvaluateRHS keeps with useRoeJacobian
+ * (lambdaFace0/123/4 = the Riemann solver's entropy-fixed eigenvalues integrated over the face and
+ * divided by its area, src/Euler/EulerEvaluator_EvaluateRHS.hxx:527-544), lamRoe [3][nFace].
+ * dndsr_lusgs_init_block: LUSGSMatrixInit with useRoeJacobian (src/Euler/EulerEvaluator.hxx:37-159;
+ * EulerEvaluator.hpp:427-434): per face (alphaDiag/2) A_f / V (|A|_Roe + lambdaVis I), boundary
+ * faces with the ghost state and its linearisation, 1/dTau + 1/dt on the diagonal; jdiag (may be
+ * NULL) and jdiagInv are [25][nCell] (row-major 5x5 per cell), the inverse as
+ * JacobianDiagBlock::GetInvert (src/Euler/EulerJacobian.hpp:153-190).  lamRoe NULL takes
+ * EvaluateDt's eigenvalue (lambdaFace row 3) for all three.  Synchronises; a singular block is an
+ * error.
+ * dndsr_update_sgs_block(_f32): UpdateSGS with useRoeJacobian (EulerEvaluator.hxx:550-693), the
+ * neighbour term A(u_j) dU_j - |A|_Roe dU_j - lambdaVis dU_j, x_i = JDiag_i^-1 buf, in multicolour
+ * order as dndsr_update_sgs. */
+int dndsr_eval_rhs_roe_lambda(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *u, const double *uRec,
+                              const double *deltaLambdaCell, double *lamRoe);
+int dndsr_lusgs_init_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, const double *dTau, double dt,
+                           double alphaDiag, const double *u, const double *lambdaFace, const double *lamRoe,
+                           double *jdiag, double *jdiagInv);
+int dndsr_update_sgs_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
+                           const double *u, const double *uInc, double *uIncNew, const double *jdiagInv,
+                           const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate, int uIncIsZero,
+                           double *sumIncHost);
+int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
+                               const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
+                               const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,
+                               int uIncIsZero, double *sumIncHost);
 /* Explicit SSP-RK3 time loop kept on the device (ExplicitSSPRK3TimeStepAsImplicitDualTimeStep::Step,
  * src/Solver/ODE.hpp:1477-1520): per step EvaluateDt once (global dt = CFL-limited minimum, or
  * dtFixed > 0), then three stages of nRec Jacobi VR iterations + EvaluateRHS + combination.
@@ -266,7 +297,7 @@ typedef struct
  * 1206; solveLinear / doPrecondition :1407-1658).  Per inner iteration: local dTau from
  * EvaluateDt, nRec Jacobi VR sweeps, EvaluateRHS, the unsteady residual and its L1 norm,
  * LUSGSMatrixInit, (1 + sgsIter) symmetric multicolour sweeps (FP64 or FP32) and the
- * increment.  Stops when the density residual relative to its running maximum is below
+ * increment (with es->useRoeJacobian: the face eigenvalues, block init and block sweeps).  Stops when the density residual relative to its running maximum is below
  * tol (after minIter iterations).  u and uRec are updated in place.  Outputs (host, may
  * be NULL): nIter, res[5] (L1 residual of the last iteration), resRel[5], history[maxIter]
  * (relative density residual per iteration).  The increment is applied as the reference's

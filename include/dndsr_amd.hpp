@@ -317,6 +317,7 @@ struct EulerEvaluatorSettings
     // laminar viscous terms (false: the RHS_Ignore_Viscosity residual, EvaluateRHS.hxx:78)
     // and IdealGasProperty (src/Euler/EulerEvaluatorSettings.hpp:291-300)
     bool viscous = false;
+    bool useRoeJacobian = false; // EulerEvaluatorSettings.hpp:54 (block LU-SGS in the implicit step)
     struct IdealGasProperty
     {
         double Rgas = 1, muGas = 1, prGas = 0.72, TRef = 273.15, CSutherland = 110.4;
@@ -326,7 +327,8 @@ struct EulerEvaluatorSettings
     {
         const IdealGasProperty &g = idealGasProperty;
         return {gamma,   riemannSolverType, rsMeanValueEig, rsFixScale, rsIncFScale, viscous ? 1 : 0,
-                g.muModel, g.Rgas,          g.muGas,        g.prGas,    g.TRef,      g.CSutherland};
+                g.muModel, g.Rgas,          g.muGas,        g.prGas,    g.TRef,      g.CSutherland,
+                useRoeJacobian ? 1 : 0};
     }
 };
 
