@@ -7,10 +7,15 @@ explicit SSP-RK3 at CFL 0.5.  One step = EvaluateDt + 3 x (one Jacobi VR sweep +
 EvaluateRHS + stage combination), all on the device (dndsr_ssprk3).
 
 value = cell-updates per second per RHS evaluation = nCell * 3 * steps / time.
-The roofline entry is the dominant kernel: algorithmic bytes per launch divided
-by its average launch time, measured with HIP events on its stream during the
-timed region.  cpu_baseline times the CPU oracle on the same 64^3 workload with all
-physical host cores and with one core (median of five repetitions).
+The roofline entry is the dominant kernel: algorithmic bytes per launch divided by
+its average launch time, measured with HIP events on its stream during the timed
+region; the same kernels are also timed as back-to-back loops under one event pair,
+and the device's own read and copy streams give the measured bandwidth the fractions
+are also quoted against.  The second half of the metric, time to a 1e-8 residual with
+FP64 and with FP32 sweeps, runs on an inviscid hex bump and on the laminar bump box
+(2M Delaunay tetrahedra).  cpu_baseline times the CPU oracle on the 64^3 workload
+with all physical host cores and with one core (median of five repetitions), and
+checks the device results against it (error_vs_oracle).
 """
 from __future__ import annotations
 
@@ -74,13 +79,17 @@ def host_cpu_info():
     return info
 
 
-def cpu_baseline(n=64, reps=5, evals_all=4, evals_one=1):
+def cpu_baseline(n=64, reps=5, evals_all=4, evals_one=1, checks=None):
     """The reference-path CPU baseline: the oracle's frhs (one Jacobi VR sweep + EvaluateRHS, the
     OpenMP face-buffer + cell-gather form of EvaluateRHS.hxx:153-159,640-662) on the benchmark
     configuration itself (TGV n^3 hex, p2, HLLE+, FP64), timed with all physical cores available
     and with one core: after one warm-up evaluation, the median of `reps` repetitions of
     `evals_all` (all cores) or `evals_one` (one core) evaluations; value = cells / median seconds
-    per evaluation.  The oracle's setup (metrics and VR coefficients) is not timed."""
+    per evaluation.  The oracle's setup (metrics and VR coefficients) is not timed.
+    checks: device results to check against the oracle on the same inputs (relative L2 of the
+    difference, returned as error_vs_oracle): "tgv" the frhs of the 64^3 state (VR sweep from zero
+    and EvaluateRHS), "laminar" the state after two implicit iterations on the tetrahedral box,
+    "bump" the same on the hex bump."""
     sys.path.insert(0, os.path.join(HERE, "oracle"))
     import pyoracle as po
     from dndsr_amd import mesh as mm
@@ -114,14 +123,43 @@ def cpu_baseline(n=64, reps=5, evals_all=4, evals_one=1):
         res[label] = (float(np.median(times)), times, k)
     v_all = o.nCell / res["all"][0]
     v_one = o.nCell / res["one"][0]
-    return {"value": v_all, "unit": "cell-updates/s", "cores": cores, "kind": "port",
-            "value_1core": v_one, "cores_used_1core": 1,
-            "sample": (f"the benchmark workload itself (TGV {n}^3 = {o.nCell} hex, p2, HLLE+, FP64): oracle frhs = "
-                       f"Jacobi VR sweep + EvaluateRHS (OpenMP face buffer + cell gather); median of {reps} "
-                       f"repetitions of {evals_all} evaluations on {cores} cores "
-                       f"({res['all'][0] * 1e3:.1f} ms each) and of {evals_one} on 1 core "
-                       f"({res['one'][0] * 1e3:.1f} ms); oracle setup {t_setup:.1f} s untimed"),
-            "host": info}
+    out = {"value": v_all, "unit": "cell-updates/s", "cores": cores, "kind": "port",
+           "value_1core": v_one, "cores_used_1core": 1,
+           "sample": (f"the benchmark workload itself (TGV {n}^3 = {o.nCell} hex, p2, HLLE+, FP64): oracle frhs = "
+                      f"Jacobi VR sweep + EvaluateRHS (OpenMP face buffer + cell gather); median of {reps} "
+                      f"repetitions of {evals_all} evaluations on {cores} cores "
+                      f"({res['all'][0] * 1e3:.1f} ms each) and of {evals_one} on 1 core "
+                      f"({res['one'][0] * 1e3:.1f} ms); oracle setup {t_setup:.1f} s untimed"),
+           "host": info}
+    errors = {}
+    if checks:
+        po.set_threads(cores)
+        rl2 = lambda a, b: float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))  # noqa: E731
+        c = checks.get("tgv")
+        if c is not None:
+            u0 = c["u0"]
+            _, _, dlc0, _ = o.eval_dt(es, u0, 0.5, 1e100, False)
+            rec0 = o.vr_iter(np.zeros((o.nCell, 5, o.M)), u0)
+            errors["tgv64_frhs"] = {"rel_l2": rl2(c["rhs"], o.eval_rhs(es, u0, rec0, dlc0)),
+                                    "what": "one Jacobi VR sweep from zero + EvaluateRHS on the initial state"}
+        del o
+        for key in ("laminar", "bump"):
+            c = checks.get(key)
+            if c is None:
+                continue
+            t0 = time.perf_counter()
+            ol = po.OracleVR(c["mesh"], maxOrder=1, intOrder=3, zoneDirWeight=c["zoneDirWeight"])
+            ol.set_bc(c["types"], c["values"], 1.4)
+            el = po.OracleVR.euler_settings(1.4, 1, 0, 1.0, 1.0, viscous=int(c["viscous"]), **c["gas"])
+            uo, _, nIt, _, _, _ = ol.implicit_euler_step(el, c["u0"], np.zeros((ol.nCell, 5, ol.M)), 1e100,
+                                                          CFL=c["CFL"], sgsIter=c["sgsIter"], maxIter=c["iters"],
+                                                          minIter=c["iters"], tol=0.0, scanOrder=c["scanOrder"])
+            errors[key] = {"rel_l2": rl2(c["u"], uo), "what": f"state after {nIt} implicit iterations (FP64 sweeps) "
+                           f"from the free stream, oracle in the device's colour order",
+                           "oracle_s": round(time.perf_counter() - t0, 1)}
+            del ol
+    out["error_vs_oracle"] = errors
+    return out
 
 
 def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, shape=(160, 48, 32)):
@@ -169,6 +207,7 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     a, b = states["mixed"], states["fp64"]
     out["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out["speedup_mixed"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    out["_check"] = oracle_check_data(m, g, ev, u0, types, np.stack([far] * len(types)), False, {}, CFL, sgsIter)
     out["case"] = (f"steady inviscid Mach 0.5 flow over a 0.1H sinusoidal bump, {shape[0]}x{shape[1]}x{shape[2]} "
                    f"= {g.nCell} jittered hexes, p1 VR, Roe, slip wall + free-stream (BCIn) boundaries")
     out["settings"] = {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
@@ -213,35 +252,46 @@ def vr_sweep_forms(m, vfv, u, reps=20):
     return out
 
 
-def implicit_laminar_tet(n=69, Re=1e4, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=20000, fixed_iter=0):
-    """Second half of the metric on the laminar configuration: steady laminar
-    Navier-Stokes flow in the unit box with a sinusoidal bump of 0.1 of the height on
-    the lower wall (y_w = 0.1 sin^2(pi x)), an n^3 lattice (n = 69: 70^3 vertices,
-    jittered uniformly by +-10 % of the edge, PCG64 seed 20261202) split into six
-    tetrahedra per cube (1 971 054 cells), Re = 1e4 on the height and the free-stream
-    speed, Mach 0.5, p1 VR (intOrder 3: 6-point face and 8-point cell rules), Roe,
-    laminar viscous fluxes (constant mu, Pr 0.72), no-slip adiabatic lower wall, free
-    stream through the Riemann flux at inlet, outlet and top, spanwise periodic; the
-    implicit LU-SGS pseudo-time loop (infinite physical step) to a density residual
-    below tol of its running maximum, with FP64 sweeps and with FP32 sweeps (residual,
-    reconstruction and update in FP64).  Cells are grouped by Gauss-Seidel colour
-    (mesh.colour_sorted) first.  fixed_iter > 0 runs that many iterations per precision
-    instead (short runs for kernel profiles)."""
-    import torch
+def laminar_box(n=66, reorder="rcm"):
+    """The laminar configuration's mesh and evaluator: the unit box with a 0.1H sin^2 bump on
+    the lower wall, the (n+1)^3 jittered lattice points Delaunay-tetrahedralised
+    (mesh.bump_box_delaunay, ~6.95 n^3 cells), cells renumbered for locality (reverse
+    Cuthill-McKee, as the reference's ReorderLocalCells) and then grouped by Gauss-Seidel
+    colour (coalesced colour passes).  Re 1e4 on the height and free-stream speed, Mach 0.5,
+    p1 VR (intOrder 3), Roe, laminar viscous fluxes (constant mu, Pr 0.72); no-slip adiabatic
+    lower wall, free stream through the Riemann flux at inlet, outlet and top, symmetry on
+    the spanwise sides."""
     from dndsr_amd import bc
     from dndsr_amd import mesh as mm
     from dndsr_amd.cfv import VariationalReconstruction
     from dndsr_amd.euler import EulerEvaluator
-    gamma = 1.4
+    gamma, Re = 1.4, 1e4
     far = np.array([1.0, 0.5, 0.0, 0.0, 1 / gamma / (gamma - 1) + 0.5 * 0.25])  # Mach 0.5, a = 1, H = 1
-    types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL, bc.BC_IN]
-    m = mm.bump_box_tet(n)
+    types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL, bc.BC_IN, bc.BC_SYM]
+    m = mm.bump_box_delaunay(n)
+    if reorder:
+        m = mm.reorder_cells(m, reorder)
     vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
-    m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
+    m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())
     g = VariationalReconstruction(m, vs).construct()
+    gas = dict(Rgas=1.0, muGas=0.5 / Re, prGas=0.72, muModel=0)
     ev = EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
-                                bcValues=np.stack([far] * len(types)), viscous=True,
-                                idealGasProperty=dict(Rgas=1.0, muGas=0.5 / Re, prGas=0.72, muModel=0)))
+                                bcValues=np.stack([far] * len(types)), viscous=True, idealGasProperty=gas))
+    return m, g, ev, far, types, gas
+
+
+def implicit_laminar_tet(n=66, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=20000, fixed_iter=0, profile_iter=20):
+    """Second half of the metric on the laminar configuration (laminar_box): the implicit
+    LU-SGS pseudo-time loop (infinite physical step) to a density residual below tol of its
+    running maximum, with FP64 sweeps and with FP32 sweeps (residual, reconstruction and update
+    in FP64).  Then profile_iter iterations of each with per-launch kernel timing give the
+    rooflines of the loop's kernels.  fixed_iter > 0 runs that many iterations per precision
+    instead (short runs for kernel profiles)."""
+    import torch
+    from dndsr_amd import _lib
+    t_mesh = time.perf_counter()
+    m, g, ev, far, types, gas = laminar_box(n)
+    t_mesh = time.perf_counter() - t_mesh
     u0 = g.to_device(np.tile(far, (g.nCell, 1)))
 
     def run(fp32, maxIter, tol_, minIter=5):
@@ -254,9 +304,12 @@ def implicit_laminar_tet(n=69, Re=1e4, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter
         torch.cuda.synchronize()
         return (time.perf_counter() - t0) * 1e3, n_, float(rel[0]), u
 
-    out = {"case": (f"steady laminar NS, unit box with a 0.1H sin^2 bump, {n}^3 cubes x 6 = {g.nCell} jittered "
-                    f"tetrahedra, Re {Re:g}, Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far "
-                    f"boundaries, cells grouped by colour"),
+    out = {"case": (f"steady laminar NS, unit box with a 0.1H sin^2 bump, {n + 1}^3 jittered lattice points (10% "
+                    f"of the edge, PCG64 seed 20261202) Delaunay-tetrahedralised: {g.nCell} tetrahedra, Re 1e4, "
+                    f"Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far boundaries, symmetric "
+                    f"sides; cells in reverse Cuthill-McKee order grouped by colour ({int(g.colors().max()) + 1} "
+                    f"colours)"),
+           "cells": g.nCell, "mesh_s": round(t_mesh, 1),
            "settings": {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
                         "physical_dt": "infinite (steady)"}}
     states = {}
@@ -273,7 +326,104 @@ def implicit_laminar_tet(n=69, Re=1e4, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter
     a, b = states["mixed"], states["fp64"]
     out["state_rel_l2_mixed_vs_fp64"] = float(np.linalg.norm(a - b) / np.linalg.norm(b))
     out["speedup_mixed"] = round(out["fp64"]["ms"] / out["mixed"]["ms"], 3)
+    if profile_iter and not fixed_iter:
+        out["kernels"] = laminar_kernel_rooflines(g, run, profile_iter, sgsIter)
+        out["_check"] = oracle_check_data(m, g, ev, u0, types, np.stack([far] * len(types)), True, gas, CFL, sgsIter)
     return out
+
+
+def oracle_check_data(m, g, ev, u0, types, values, viscous, gas, CFL, sgsIter, iters=2):
+    """Two FP64 implicit iterations from the free stream on the device, and what the CPU
+    baseline leg needs to repeat them in the oracle (the mesh, boundary table, gas, the
+    device's colour order as the serial scan order)."""
+    import torch
+    from dndsr_amd import bc
+    u = u0.clone()
+    rec = g.BuildURec(5)
+    ev.ImplicitEulerStep(u, rec, 1e100, CFL=CFL, sgsIter=sgsIter, maxIter=iters, minIter=iters, tol=0.0)
+    torch.cuda.synchronize()
+    col = g.colors()
+    return {"mesh": m, "types": types, "values": values, "viscous": viscous, "gas": gas, "CFL": CFL,
+            "sgsIter": sgsIter, "iters": iters, "u0": g.to_host(u0), "u": g.to_host(u),
+            "zoneDirWeight": bc.euler_dir_weights(types, 1),
+            "scanOrder": np.lexsort((np.arange(col.size), col)).astype(np.int64)}
+
+
+def laminar_kernel_rooflines(g, run, iters, sgsIter):
+    """Per-kernel averages of `iters` implicit iterations (per-launch HIP events) with FP64 and
+    with FP32 sweeps, against the algorithmic bytes per cell of the tetrahedral p1 case:
+    EvaluateRHS (face + viscous + gather passes) B_RHS, the VR sweep B_VR, one SGS sweep over
+    all colours 8 (5 nV + 1) + f_c (4 + s (2 + d)) + s (5 nV + 1) with s the sweep's precision
+    (rhs, JDiag, the cell's increment in and out; per face the neighbour, its state and
+    increment, the face geometry)."""
+    from dndsr_amd import _lib
+    L_ = _lib.lib()
+    nk = len(_lib.KERNEL_IDS)
+    N = g.nCell
+    f_c = g.nFace / N
+    b_vr, b_rhs = alg_bytes(3, 3, 5, 4, f_c, g.nGFace)
+    res = {"bytes_per_cell": {"eval_rhs": round(b_rhs, 1), "vr_iter": b_vr}}
+    for name, fp32 in (("fp64", False), ("mixed", True)):
+        _lib.check(L_.dndsr_get_timing(g.ctx.h, None, None, 1))
+        _lib.check(L_.dndsr_set_timing(g.ctx.h, 1))
+        run(fp32, iters, 0.0, iters)
+        _lib.check(L_.dndsr_set_timing(g.ctx.h, 0))
+        ms, cnt = np.zeros(nk), np.zeros(nk, np.int64)
+        _lib.check(L_.dndsr_get_timing(g.ctx.h, ms.ctypes.data_as(_lib.dp), cnt.ctypes.data_as(_lib.i64p), 1))
+        k = {nm: (float(ms[i]), int(cnt[i])) for i, nm in enumerate(_lib.KERNEL_IDS) if cnt[i]}
+        per_it = lambda nm: k.get(nm, (0.0, 0))[0] / iters  # noqa: E731
+        rhs_ms = per_it("rhs_face") + per_it("rhs_viscous") + per_it("rhs_gather")
+        vr_ms = per_it("vr_iter")
+        sweeps = 2 * (1 + sgsIter)
+        sgs_ms = per_it("sgs_colour") / sweeps
+        s_ = 4 if fp32 else 8
+        b_sgs = 8 * (5 * 5 + 1) + f_c * (4 + s_ * (2 + 3)) + s_ * (5 * 5 + 1)
+        ent = {}
+        for key, t, b in (("eval_rhs", rhs_ms, b_rhs), ("vr_iter", vr_ms, b_vr), ("sgs_sweep", sgs_ms, b_sgs)):
+            if t > 0:
+                gbs = N * b / (t * 1e-3) / 1e9
+                ent[key] = {"avg_ms": round(t, 4), "bytes_per_cell": round(b, 1), "achieved": round(gbs, 1),
+                            "frac": round(gbs / HBM_PEAK_GBS, 4)}
+        ent["passes_ms_per_iteration"] = {nm: round(v[0] / iters, 4) for nm, v in k.items()}
+        res[name] = ent
+    return res
+
+
+def loop_ms(fn, reps=20):
+    """Average milliseconds of one call over `reps` back-to-back calls bracketed by one event
+    pair on torch's current stream, the stream the library launches on (no per-launch events)."""
+    import torch
+    fn()
+    torch.cuda.synchronize()
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record()
+    for _ in range(reps):
+        fn()
+    e1.record()
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / reps
+
+
+def stream_peaks(ctx, gib=1.0, reps=10):
+    """The device's own read stream (every element read once) and copy over gib GiB
+    (dndsr_stream_probe), in GB/s."""
+    import ctypes
+    import torch
+    from dndsr_amd import _lib
+    n = int(gib * (1 << 30) // 8) // 2 * 2
+    x = torch.ones(n, dtype=torch.float64, device="cuda")
+    y = torch.empty_like(x)
+    part = torch.empty(4096, dtype=torch.float64, device="cuda")
+    L_ = _lib.lib()
+    ms = ctypes.c_double()
+    _lib.check(L_.dndsr_stream_probe(ctx.h, 0, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(part.data_ptr()), n,
+                                     reps, ctypes.byref(ms)))
+    read = n * 8 / (ms.value * 1e-3) / 1e9
+    _lib.check(L_.dndsr_stream_probe(ctx.h, 1, ctypes.c_void_p(x.data_ptr()), ctypes.c_void_p(y.data_ptr()), n,
+                                     reps, ctypes.byref(ms)))
+    copy = 2 * n * 8 / (ms.value * 1e-3) / 1e9
+    del x, y, part
+    return {"read_GBs": round(read, 1), "copy_GBs": round(copy, 1), "bytes": n * 8, "passes": reps}
 
 
 def main():
@@ -290,6 +440,10 @@ def main():
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
+              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
+        sys.exit(2)
     if args.copy_calibration:
         import torch
         n = args.copy_calibration * (1 << 20) // 8
@@ -304,10 +458,6 @@ def main():
         r = implicit_laminar_tet(fixed_iter=args.tet_iterations)
         print(json.dumps(r), flush=True)
         return
-    if args.gpus != 1:
-        print("bench.py: this build runs on one GPU; --gpus above 1 is not supported "
-              "(the sharded design is described in DESIGN.md)", file=sys.stderr)
-        sys.exit(2)
 
     import torch
     from dndsr_amd import _lib, fields
@@ -328,6 +478,15 @@ def main():
     t_setup = time.perf_counter() - t_setup
     N = vfv.nCell
     CFL = 0.5
+
+    # the frhs of the initial state, for the check against the oracle
+    rec1, scratch = vfv.BuildURec(5), vfv.BuildURec(5)
+    vfv.DoReconstructionIter(scratch, rec1, u)
+    dt0 = vfv.BuildUDof(1)[0]
+    ev.EvaluateDt(dt0, u, None, CFL=CFL, UseLocaldt=False)
+    rhs0 = vfv.BuildUDof(5)
+    ev.EvaluateRHS(rhs0, None, u, rec1, rec1)
+    checks = {"tgv": {"u0": u0, "rhs": vfv.to_host(rhs0)}}
 
     ev.SSPRK3(u, uRec, args.warmup, CFL, 1)
     torch.cuda.synchronize()
@@ -360,33 +519,47 @@ def main():
             kern[nm] = {"ms_total": float(ms[k]), "launches": int(cnt[k]), "avg_ms": float(ms[k] / cnt[k])}
     vr_avg = kern["vr_iter"]["avg_ms"]
     rhs_avg = kern["rhs_face"]["avg_ms"] + kern["rhs_gather"]["avg_ms"]
-    roof = {
-        "vr_iter": {"bound": "hbm", "achieved": N * b_vr / (vr_avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                    "unit": "GB/s", "bytes_per_cell": b_vr, "avg_ms": vr_avg,
-                    "traffic": read_traffic("vr_iter_tgv64_p2")},
-        "eval_rhs": {"bound": "hbm", "achieved": N * b_rhs / (rhs_avg * 1e-3) / 1e9, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "bytes_per_cell": b_rhs, "avg_ms": rhs_avg,
-                     "traffic": read_traffic("rhs_tgv64_p2")},
-    }
-    for r in roof.values():
-        r["frac"] = r["achieved"] / r["peak"]
+
+    # the same kernels as back-to-back loops under one event pair (no per-launch events)
+    rr, rn = vfv.BuildURec(5), vfv.BuildURec(5)
+    vfv.DoReconstructionIter(rn, rr, u)
+    rhs_l = vfv.BuildUDof(5)
+    loop_rhs = loop_ms(lambda: ev.EvaluateRHS(rhs_l, None, u, rr, rr, check_finite=False))
+    loop_vr = loop_ms(lambda: vfv.DoReconstructionIter(rr, rn, u))
+    peaks = stream_peaks(vfv.ctx)
+
+    def entry(b, t, t_loop, tag):
+        a = N * b / (t * 1e-3) / 1e9
+        a_l = N * b / (t_loop * 1e-3) / 1e9
+        return {"bound": "hbm", "achieved": a, "peak": HBM_PEAK_GBS, "unit": "GB/s", "bytes_per_cell": b,
+                "avg_ms": t, "frac": a / HBM_PEAK_GBS, "traffic": read_traffic(tag),
+                "loop_avg_ms": t_loop, "loop_achieved": a_l, "loop_frac": a_l / HBM_PEAK_GBS,
+                "frac_of_measured_read": a / peaks["read_GBs"]}
+
+    roof = {"vr_iter": entry(b_vr, vr_avg, loop_vr, "vr_iter_tgv64_p2"),
+            "eval_rhs": entry(b_rhs, rhs_avg, loop_rhs, "rhs_tgv64_p2")}
     dom = "vr_iter" if kern["vr_iter"]["ms_total"] >= kern["rhs_face"]["ms_total"] + kern["rhs_gather"]["ms_total"] \
         else "eval_rhs"
     droof = roof[dom]
     roofline = {"bound": "hbm", "achieved": round(droof["achieved"], 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                 "frac": round(droof["frac"], 4), "traffic": droof["traffic"], "kernel": dom,
-                "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5)}
+                "bytes_per_launch": N * droof["bytes_per_cell"], "avg_launch_ms": round(droof["avg_ms"], 5),
+                "measured_peaks": peaks}
 
     forms = vr_sweep_forms(m, vfv, u)
 
     implicit = implicit_tet = None
     if not args.no_implicit:
         implicit = implicit_bump_time_to_tol()
+        checks["bump"] = implicit.pop("_check")
         implicit_tet = implicit_laminar_tet()
+        if "_check" in implicit_tet:
+            checks["laminar"] = implicit_tet.pop("_check")
 
-    cpu = None
+    cpu = errors = None
     if not args.no_cpu_baseline:
-        cpu = cpu_baseline()
+        cpu = cpu_baseline(checks=checks)
+        errors = cpu.pop("error_vs_oracle")
 
     out = {
         "metric": METRIC, "value": round(value, 1), "unit": "cell-updates/s", "n_gpus": 1, "steps": args.steps,
@@ -397,10 +570,13 @@ def main():
                                f"1 Jacobi VR sweep per RHS", "cells": N, "rhs_evals_per_step": 3},
         "roofline": roofline,
         "cpu_baseline": cpu,
+        "error_vs_oracle": errors,
         "implicit": implicit,
         "implicit_laminar_tet": implicit_tet,
         "vr_sweep_forms": forms,
         "kernels": kern,
+        "kernel_timing": "kernels: per-launch HIP event pairs during the timed steps; loop_*: back-to-back calls "
+                         "under one event pair",
         "roofline_all": {k: {kk: (round(vv, 4) if isinstance(vv, float) else vv) for kk, vv in r.items()}
                          for k, r in roof.items()},
         "setup_s": round(t_setup, 3),

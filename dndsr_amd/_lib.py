@@ -88,6 +88,8 @@ SIGNATURES = {
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],
     "dndsr_implicit_euler_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), vp, vp,
                                   C.POINTER(C.c_int), dp, dp, dp],
+    "dndsr_implicit_bdf_step": [vp, C.POINTER(EulerSettings), C.POINTER(ImplicitSettings), C.c_int,
+                                C.POINTER(vp), vp, vp, C.POINTER(C.c_int), dp, dp, dp],
     "dndsr_implicit_pp_stats": [vp, i64p, dp],
     "dndsr_lusgs_matvec": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp],
     "dndsr_gmres_lusgs": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, C.c_int, C.c_int, C.c_int,
@@ -96,12 +98,13 @@ SIGNATURES = {
     "dndsr_vr_iter_vjp": [vp, C.c_int, C.c_int, C.c_double, vp, vp, vp, vp, vp],
     "dndsr_zone_force": [vp, C.POINTER(EulerSettings), vp, vp, vp, C.c_int, dp],
     "dndsr_zone_force_vjp": [vp, C.POINTER(EulerSettings), vp, vp, vp, C.c_int, dp, vp, vp],
+    "dndsr_stream_probe": [vp, C.c_int, vp, vp, C.c_int64, C.c_int, dp],
     "dndsr_set_timing": [vp, C.c_int],
     "dndsr_get_timing": [vp, dp, i64p, C.c_int],
 }
 
 KERNEL_IDS = ["vr_iter", "rhs_face", "rhs_gather", "dt_face", "dt_cell", "rk_axpby", "sgs_colour", "lusgs_init",
-              "dual_residual", "increment", "to_f32", "adjoint", "linsolve"]
+              "dual_residual", "increment", "to_f32", "adjoint", "linsolve", "rhs_viscous"]
 
 _lib = None
 

@@ -9,7 +9,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libdndsr_amd.so")
-SOURCES = ["setup.hip", "vr.hip", "euler.hip", "adjoint.hip", "linsolve.hip", "model.hip", "roejac.hip", "capi.hip"]
+SOURCES = ["setup.hip", "vr.hip", "euler.hip", "adjoint.hip", "linsolve.hip", "model.hip", "roejac.hip", "probe.hip", "capi.hip"]
 ARCH = os.environ.get("DNDSR_OFFLOAD_ARCH", "gfx950")
 
 

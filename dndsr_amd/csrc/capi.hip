@@ -1005,6 +1005,20 @@ int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, f
     CAPI_END
 }
 
+int dndsr_stream_probe(dndsr_ctx *ctx, int mode, const double *src, double *dst, int64_t n, int reps,
+                       double *msPerPass)
+{
+    CAPI_BEGIN
+    need(ctx != nullptr, "null context");
+    need(src && dst && msPerPass, "null argument");
+    need(mode == 0 || mode == 1, "mode must be 0 (read) or 1 (copy)");
+    need(n >= 2 && n % 2 == 0 && reps >= 1, "n must be even and positive, reps at least 1");
+    need(((uintptr_t)src % 16) == 0 && ((uintptr_t)dst % 16) == 0, "src and dst must be 16-byte aligned");
+    need(mode == 1 || streamProbeBlocks(ctx->c) <= 4096, "more than 4096 partial sums");
+    *msPerPass = streamProbe(ctx->c, mode, src, dst, (size_t)n, reps);
+    CAPI_END
+}
+
 int dndsr_set_timing(dndsr_ctx *ctx, int enable)
 {
     CAPI_BEGIN
@@ -1097,10 +1111,43 @@ int dndsr_ssprk3(dndsr_ctx *ctx, const dndsr_euler_settings *es, double *u, doub
     CAPI_END
 }
 
+// BDF coefficients of ImplicitBDFDualTimeStep (src/Solver/ODE.hpp:590-595): row k - 1 holds
+// (alpha, c_last, c_prev1, ...) of the k-step formula
+static const real kBDFCoef[4][5] = {{1. / 1., 1. / 1., 0, 0, 0},
+                                    {2. / 3., 4. / 3., -1. / 3., 0, 0},
+                                    {6. / 11., 18. / 11., -9. / 11., 2. / 11., 0},
+                                    {12. / 25., 48. / 25., -36. / 25., 16. / 25., -3. / 25.}};
+
+static void implicitStep(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is, int kCur,
+                         const double *const *xPrevs, double *u, double *uRec, int *nIter, double *res,
+                         double *resRel, double *history);
+
 int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
                               double *u, double *uRec, int *nIter, double *res, double *resRel, double *history)
 {
     CAPI_BEGIN
+    implicitStep(ctx, es, is, 0, nullptr, u, uRec, nIter, res, resRel, history);
+    CAPI_END
+}
+
+int dndsr_implicit_bdf_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
+                            int nPrev, const double *const *xPrevs, double *u, double *uRec, int *nIter, double *res,
+                            double *resRel, double *history)
+{
+    CAPI_BEGIN
+    need(nPrev >= 0 && nPrev <= 3, "nPrev must be 0..3 (BDF1..BDF4)");
+    need(nPrev == 0 || xPrevs != nullptr, "null earlier states");
+    for (int j = 0; j < nPrev; j++)
+        need(xPrevs[j] != nullptr, "null earlier state");
+    implicitStep(ctx, es, is, nPrev + 1, xPrevs, u, uRec, nIter, res, resRel, history);
+    CAPI_END
+}
+
+// kCur 0: ImplicitEulerDualTimeStep::Step; 1..4: the BDF step with kCur - 1 earlier states.
+static void implicitStep(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is, int kCur,
+                         const double *const *xPrevs, double *u, double *uRec, int *nIter, double *res,
+                         double *resRel, double *history)
+{
     requireMesh(ctx);
     need(es && is && u && uRec, "null argument");
     need(is->dt > 0 && is->CFL > 0 && is->nRec >= 0 && is->sgsIter >= 0 && is->maxIter >= 1 && is->minIter >= 0,
@@ -1153,6 +1200,9 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         ctx->haveImplicit = true;
     }
     const real gamma = es->gamma;
+    const real alphaDiag = kCur ? kBDFCoef[kCur - 1][0] : 1.0;
+    const real *xs[4] = {nullptr, nullptr, nullptr, nullptr};
+    real coefs[4] = {0, 0, 0, 0};
     const real rhoEps = kSmall * is->refRho * 1e-1, pEps = kSmall * is->refP * 1e-1;
     launchCopy(c, nU, u, ctx->iXLast);
     launchCopy(c, nR, uRec, ctx->iURecSave);
@@ -1173,19 +1223,29 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         launchPPStatReset(c, ctx->iStat);
         // fdtau: local pseudo time step (alphaDiag = 1)
         launchEvalDt(c, gp, u, is->CFL, 1e100, 1, ctx->iDTau, ctx->wLam, ctx->wDlc, ctx->wDtMin);
+        if (kCur) // dTau *= 1 / alphaDiag (EulerSolver.hxx:693)
+            launchScale(c, (idx)N, 1. / alphaDiag, ctx->iDTau);
         // frhs: nRec VR sweeps + EvaluateRHS
         for (int r = 0; r < is->nRec; r++)
             solverVRSweep(c, rec, recNew, u);
         launchEvalRHS(c, gp, u, rec, ctx->wDlc, ctx->iRhsF);
         if (roeJ) // the face eigenvalues EvaluateRHS keeps for the Roe Jacobian
             launchFaceRoeLambda(c, gp, u, rec, ctx->wDlc, ctx->bLamRoe);
-        launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
+        if (kCur)
+        {
+            xs[0] = ctx->iXLast, coefs[0] = kBDFCoef[kCur - 1][1];
+            for (int j = 0; j < kCur - 1; j++)
+                xs[1 + j] = xPrevs[j], coefs[1 + j] = kBDFCoef[kCur - 1][2 + j];
+            launchDualResidualBDF(c, is->dt, xs, coefs, kCur, alphaDiag, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
+        }
+        else
+            launchDualResidual(c, is->dt, ctx->iXLast, u, ctx->iRhsF, ctx->iRhs, ctx->iRes);
         // fsolve: LUSGSMatrixInit + (1 + sgsIter) symmetric sweeps from a zero increment
         if (roeJ)
-            launchLUSGSInitBlock(c, gp, ctx->iDTau, is->dt, 1.0, u, ctx->wLam, ctx->bLamRoe, nullptr, ctx->bJinv,
-                                 bBad);
+            launchLUSGSInitBlock(c, gp, ctx->iDTau, is->dt, alphaDiag, u, ctx->wLam, ctx->bLamRoe, nullptr,
+                                 ctx->bJinv, bBad);
         else
-            launchLUSGSInit(c, ctx->iDTau, is->dt, 1.0, ctx->wLam, ctx->iJd);
+            launchLUSGSInit(c, ctx->iDTau, is->dt, alphaDiag, ctx->wLam, ctx->iJd);
         if (roeJ)
         {
             // block sweeps in colour order, ping-ponging the increment from zero
@@ -1203,7 +1263,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
                 for (int s = 0; s <= is->sgsIter; s++)
                     for (int dir = 1; dir >= 0; dir--)
                     {
-                        launchSGSSweepBlock32(c, gp, 1.0f, ctx->fRhs, ctx->fU, a, b, ctx->bJinvF, ctx->fLam,
+                        launchSGSSweepBlock32(c, gp, (float)alphaDiag, ctx->fRhs, ctx->fU, a, b, ctx->bJinvF, ctx->fLam,
                                               ctx->bLamRoeF, dir, is->gsUpdate, 0, sumDev);
                         std::swap(a, b);
                     }
@@ -1217,7 +1277,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
                 for (int s = 0; s <= is->sgsIter; s++)
                     for (int dir = 1; dir >= 0; dir--)
                     {
-                        launchSGSSweepBlock64(c, gp, 1.0, ctx->iRhs, u, a, b, ctx->bJinv, ctx->wLam, ctx->bLamRoe,
+                        launchSGSSweepBlock64(c, gp, alphaDiag, ctx->iRhs, u, a, b, ctx->bJinv, ctx->wLam, ctx->bLamRoe,
                                               dir, is->gsUpdate, 0, sumDev);
                         std::swap(a, b);
                     }
@@ -1229,7 +1289,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             // solveLinear with gmresCode 1 (GMRES from zero) or 2 (LU-SGS result as the start),
             // EulerSolver.hxx:1407-1578; the preconditioner sweeps in FP32 when fp32Sweeps
             LinWork &w = linWork(ctx, is->nGmresSpace, is->fp32Sweeps != 0);
-            w.u = u, w.jd = ctx->iJd, w.lam = ctx->wLam, w.alphaDiag = 1.0;
+            w.u = u, w.jd = ctx->iJd, w.lam = ctx->wLam, w.alphaDiag = alphaDiag;
             w.sgsIter = is->sgsIter, w.gsUpdate = is->gsUpdate, w.fp32 = is->fp32Sweeps != 0;
             if (w.fp32)
             {
@@ -1254,7 +1314,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             launchToF32(c, N, ctx->iJd, ctx->fJd);
             launchToF32(c, F, ctx->wLam, ctx->fLam); // lambdaFace row only
             float *inc = nullptr;
-            launchSGSPacked32(c, gp, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fInc, ctx->fTmp,
+            launchSGSPacked32(c, gp, (float)alphaDiag, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fInc, ctx->fTmp,
                               is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iStat);
         }
@@ -1263,7 +1323,7 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
             if (!ctx->iSlot)
                 ctx->iSlot = devAlloc<real>(c, (size_t)m.K * 5 * N), ctx->iUA = devAlloc<real>(c, nU);
             real *inc = nullptr;
-            launchSGSPacked64(c, gp, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iInc, ctx->iTmp,
+            launchSGSPacked64(c, gp, alphaDiag, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iInc, ctx->iTmp,
                               is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iStat);
         }
@@ -1319,7 +1379,6 @@ int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, co
         if (resRel)
             resRel[v] = rel[v];
     }
-    CAPI_END
 }
 
 int dndsr_implicit_pp_stats(dndsr_ctx *ctx, int64_t *nLimited, double *alphaMin)

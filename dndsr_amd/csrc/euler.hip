@@ -1,7 +1,6 @@
 // euler.hip -- EulerEvaluator hot path on the device: EvaluateDt, EvaluateRHS,
 // LUSGSMatrixInit, the multicolour symmetric Gauss-Seidel sweeps and the
 // SSP-RK3 stage combination.
-#include <cstdlib>
 #include <type_traits>
 #include "gas.hpp"
 #include "internal.hpp"
@@ -508,317 +507,18 @@ __global__ void __launch_bounds__(256) kRHSGather(DevMesh m, const real *__restr
     }
 }
 
-// ---------------------------------------------------------------------------
-// Multi-point face tile kernel: NW = ceil(G / PPW) wavefronts per 64-face tile,
-// each lane evaluating PPW quadrature points of its face.  One LDS read of a
-// staged row (reconstruction coefficient, centre, moment, mean, face constant)
-// feeds PPW point evaluations, and the PPW independent point computations give
-// each wave instruction-level parallelism in place of wave count.  Every wave
-// stages rows straight from global memory into LDS (buffer_load ... lds: no
-// VGPRs, no LDS store instructions, all rows of a wave in flight at once); wave 0
-// also forms the face constants.  Point fluxes go through LDS and are summed in
-// point order, as in the one-point-per-wave kernel above.
-// ---------------------------------------------------------------------------
 typedef __attribute__((address_space(3))) void *LdsPtr;
 
-// Byte offsets of the direct-to-LDS row loads: lane l carries dword (l & 1) of
-// the faces (l >> 1) and 32 + (l >> 1) of the tile, so a row of 64 doubles is
-// two dword loads.
-struct LdsRowOffsets
-{
-    uint32_t L[2], R[2], F[2];
-};
-
-template <int NDOF, int Q>
-__device__ __forceinline__ void rhsLdsRow(const DevMesh &m, const real *u, const real *uRec, real *lds,
-                                          const LdsRowOffsets &o)
-{
-    using R = RHSRows<NDOF>;
-    constexpr int M = R::M;
-    const size_t N = m.N, F = m.F;
-    const uint64_t rowN = (uint64_t)N * 8;
-    real *dst = lds + Q * kTileF;
-    if constexpr (Q < R::REC)
-    {
-        const Rsrc rs = rowRsrc(u + (size_t)(Q % 5) * N, rowN);
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, Q < 5 ? o.L[h] : o.R[h], 0, 0, 0);
-    }
-    else if constexpr (Q < R::MET)
-    {
-        constexpr int r = Q - R::REC, side = r / (5 * M), vj = r % (5 * M);
-        if (uRec) // without coefficients these rows are never read
-        {
-            const Rsrc rs = rowRsrc(uRec + (size_t)vj * N, rowN);
-#pragma unroll
-            for (int h = 0; h < 2; h++)
-                __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, side ? o.R[h] : o.L[h], 0, 0,
-                                                         0);
-        }
-    }
-    else if constexpr (Q < R::SHIFT)
-    {
-        constexpr int r = Q - R::MET, side = r / R::METW, k = r % R::METW;
-        const real *base = k < 3 ? m.cent + (size_t)k * N : (k == 3 ? m.iscale : m.moment + (size_t)(k - 4) * N);
-        const Rsrc rs = rowRsrc(base, rowN);
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, side ? o.R[h] : o.L[h], 0, 0, 0);
-    }
-    else
-    {
-        const Rsrc rs = rowRsrc(m.fshift + (size_t)(Q - R::SHIFT) * F, (uint64_t)F * 8);
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-            __builtin_amdgcn_raw_ptr_buffer_load_lds(rs, (LdsPtr)(dst + 32 * h), 4, o.F[h], 0, 0, 0);
-    }
-}
-template <int NDOF, int W, int NS, int T>
-__device__ __forceinline__ void rhsLdsSeq(const DevMesh &m, const real *u, const real *uRec, real *lds,
-                                          const LdsRowOffsets &o)
-{
-    constexpr int PER = (RHSRows<NDOF>::STAGED + NS - 1) / NS;
-    if constexpr (T < PER)
-    {
-        constexpr int Q = W + T * NS;
-        if constexpr (Q < RHSRows<NDOF>::STAGED)
-            rhsLdsRow<NDOF, Q>(m, u, uRec, lds, o);
-        rhsLdsSeq<NDOF, W, NS, T + 1>(m, u, uRec, lds, o);
-    }
-}
-template <int NDOF, int NS, int W>
-__device__ __forceinline__ void rhsLdsDispatch(int w, const DevMesh &m, const real *u, const real *uRec, real *lds,
-                                               const LdsRowOffsets &o)
-{
-    if constexpr (W < NS)
-    {
-        if (w == W)
-            rhsLdsSeq<NDOF, W, NS, 0>(m, u, uRec, lds, o);
-        else
-            rhsLdsDispatch<NDOF, NS, W + 1>(w, m, u, uRec, lds, o);
-    }
-}
-
-template <int DIM, int NDOF, int G, int RS, bool BATCH, int PPW>
-__global__ void __launch_bounds__(kTileF *((G + PPW - 1) / PPW)) __attribute__((amdgpu_waves_per_eu(2)))
-kRHSFaceW(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__restrict__ uRec,
-          const real *__restrict__ dlc, real *__restrict__ fflux)
-{
-    using R = RHSRows<NDOF>;
-    constexpr int M = R::M;
-    constexpr int NW = (G + PPW - 1) / PPW;
-    extern __shared__ real lds[];
-    const int lane = threadIdx.x % kTileF, g = waveIndex(kTileF);
-    const size_t N = m.N, F = m.F;
-    const uint64_t rowN = (uint64_t)N * 8, rowF = (uint64_t)F * 8;
-    const int tile = xcdTile(blockIdx.x, gridDim.x);
-    const idx f = tile * kTileF + lane;
-    const bool valid = f < m.F;
-    const uint32_t fo = (uint32_t)(valid ? f : 0) * 8u;
-    const idx cL = bldi(rowRsrc(m.f2c, (uint64_t)F * 4), fo / 2);
-    const idx cR = bldi(rowRsrc(m.f2c + F, (uint64_t)F * 4), fo / 2);
-    const uint32_t oL = (uint32_t)cL * 8u, oR = (uint32_t)cR * 8u;
-
-    {
-        LdsRowOffsets o;
-        const int hl = lane >> 1;
-        const uint32_t par = (uint32_t)(lane & 1) * 4u;
-#pragma unroll
-        for (int h = 0; h < 2; h++)
-        {
-            o.L[h] = (uint32_t)__shfl((int)oL, 32 * h + hl) + par;
-            o.R[h] = (uint32_t)__shfl((int)oR, 32 * h + hl) + par;
-            const idx f2 = tile * kTileF + 32 * h + hl;
-            o.F[h] = (uint32_t)(f2 < m.F ? f2 : 0) * 8u + par;
-        }
-        rhsLdsDispatch<NDOF, NW, 0>(g, m, u, uRec, lds, o);
-    }
-    // this lane's point geometry (registers, used after the barrier; read once, streamed)
-    real x[PPW][3], n[PPW][3], wj[PPW];
-#pragma unroll
-    for (int p = 0; p < PPW; p++)
-    {
-        const int q = g * PPW + p < G ? g * PPW + p : G - 1; // the last wave may hold fewer points
-#pragma unroll
-        for (int r = 0; r < 3; r++)
-        {
-            x[p][r] = bldnt(rowRsrc(m.fpts + ((size_t)q * 3 + r) * F, rowF), fo);
-            n[p][r] = bldnt(rowRsrc(m.fnorm + ((size_t)q * 3 + r) * F, rowF), fo);
-        }
-        wj[p] = bldnt(rowRsrc(m.fwj + (size_t)q * F, rowF), fo);
-    }
-    if (g == 0)
-    {
-        // face constants from the cell means
-        real uLm[5], uRm[5];
-        real dLambda = 0;
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-        {
-            const Rsrc rr = rowRsrc(u + (size_t)v * N, rowN);
-            uLm[v] = bld(rr, oL), uRm[v] = bld(rr, oR);
-        }
-        if (dlc)
-        {
-            const Rsrc rr = rowRsrc(dlc, rowN);
-            dLambda = fmax(bld(rr, oL), bld(rr, oR));
-        }
-        if (!valid)
-        {
-#pragma unroll
-            for (int v = 0; v < 5; v++)
-                uLm[v] = uRm[v] = v == 0 ? 1.0 : (v == 4 ? 2.5 : 0.0);
-        }
-        if constexpr (BATCH)
-        {
-            RoeBatchFace bf;
-            real nm[3];
-#pragma unroll
-            for (int r = 0; r < 3; r++)
-                nm[r] = bld(rowRsrc(m.fmeanN + (size_t)r * F, rowF), fo);
-            roeBatchSetup<RS>(uLm, uRm, nm, gp, dLambda, bf);
-            roeStore(lds, R::FC, lane, bf.r);
-            lds[(R::FC + kRoeRows + 0) * kTileF + lane] = bf.vRN;
-            lds[(R::FC + kRoeRows + 1) * kTileF + lane] = bf.lam0;
-            lds[(R::FC + kRoeRows + 2) * kTileF + lane] = bf.lam123;
-            lds[(R::FC + kRoeRows + 3) * kTileF + lane] = bf.lam4;
-        }
-        else
-        {
-            RoeAve ra;
-            roeAverage(uLm, uRm, gp.gamma, ra);
-            roeStore(lds, R::FC, lane, ra);
-        }
-        lds[(R::FC + kRoeRows + 4) * kTileF + lane] = dLambda;
-    }
-    __builtin_amdgcn_s_waitcnt(0); // this wave's direct-to-LDS rows have landed
-    __syncthreads();
-    real Fp[PPW][5];
-#pragma unroll
-    for (int p = 0; p < PPW; p++)
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-            Fp[p][v] = 0;
-    if (valid)
-    {
-        real U[PPW][2][5];
-#pragma unroll
-        for (int v = 0; v < 5; v++)
-        {
-            const real ul = lds[(R::MEAN + v) * kTileF + lane], ur = lds[(R::MEAN + 5 + v) * kTileF + lane];
-#pragma unroll
-            for (int p = 0; p < PPW; p++)
-                U[p][0][v] = ul, U[p][1][v] = ur;
-        }
-        if (uRec)
-        {
-#pragma unroll
-            for (int side = 0; side < 2; side++)
-            {
-                const real *met = lds + (size_t)(R::MET + side * R::METW) * kTileF + lane;
-                const real isc = met[3 * kTileF];
-                real cen[3], sh[3], mom[M > 0 ? M : 1], phi[PPW][M > 0 ? M : 1];
-#pragma unroll
-                for (int r = 0; r < 3; r++)
-                    cen[r] = met[r * kTileF], sh[r] = side ? lds[(R::SHIFT + r) * kTileF + lane] : 0.0;
-#pragma unroll
-                for (int j = 0; j < M; j++)
-                    mom[j] = met[(4 + j) * kTileF];
-#pragma unroll
-                for (int p = 0; p < PPW; p++)
-                {
-                    real xs[3];
-#pragma unroll
-                    for (int r = 0; r < 3; r++)
-                    {
-                        real pt = x[p][r] + sh[r];
-                        xs[r] = pt - cen[r];
-                        if (DIM == 3 || r < 2)
-                            xs[r] *= isc;
-                    }
-                    basisRow0<DIM, NDOF>(xs, mom, phi[p]);
-                }
-                const real *rec = lds + (size_t)(R::REC + side * 5 * M) * kTileF + lane;
-#pragma unroll
-                for (int v = 0; v < 5; v++)
-                {
-                    real sacc[PPW];
-#pragma unroll
-                    for (int p = 0; p < PPW; p++)
-                        sacc[p] = 0;
-#pragma unroll
-                    for (int j = 0; j < M; j++)
-                    {
-                        const real c = rec[(v * M + j) * kTileF];
-#pragma unroll
-                        for (int p = 0; p < PPW; p++)
-                            sacc[p] += phi[p][j] * c;
-                    }
-#pragma unroll
-                    for (int p = 0; p < PPW; p++)
-                        U[p][side][v] += sacc[p];
-                }
-            }
-        }
-        const real dLambda = lds[(R::FC + kRoeRows + 4) * kTileF + lane];
-        RoeAve ra;
-        roeLoad(lds, R::FC, lane, ra);
-#pragma unroll
-        for (int p = 0; p < PPW; p++)
-        {
-            if (g * PPW + p >= G)
-                continue;
-            real Fl[5];
-            if constexpr (BATCH)
-            {
-                RoeBatchFace bf;
-                bf.r = ra;
-                bf.vRN = lds[(R::FC + kRoeRows + 0) * kTileF + lane];
-                bf.lam0 = lds[(R::FC + kRoeRows + 1) * kTileF + lane];
-                bf.lam123 = lds[(R::FC + kRoeRows + 2) * kTileF + lane];
-                bf.lam4 = lds[(R::FC + kRoeRows + 3) * kTileF + lane];
-                roeBatchPoint<RS>(U[p][0], U[p][1], n[p], gp, bf, Fl);
-            }
-            else
-            {
-                real l0, l1, l4;
-                riemannFluxR<RS>(U[p][0], U[p][1], ra, n[p], gp, dLambda, Fl, l0, l1, l4);
-            }
-#pragma unroll
-            for (int v = 0; v < 5; v++)
-                Fp[p][v] = (-Fl[v]) * wj[p];
-        }
-    }
-    __syncthreads(); // all lanes are done with the staged data
-#pragma unroll
-    for (int p = 0; p < PPW; p++)
-    {
-        const int q = g * PPW + p;
-        if (q < G)
-#pragma unroll
-            for (int v = 0; v < 5; v++)
-                lds[(q * 5 + v) * kTileF + lane] = Fp[p][v];
-    }
-    __syncthreads();
-    for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
-    {
-        const int v = t / kTileF, l = t % kTileF;
-        const idx fout = tile * kTileF + l;
-        if (fout >= m.F)
-            continue;
-        real acc = 0;
-#pragma unroll
-        for (int q = 0; q < G; q++)
-            acc += lds[(q * 5 + v) * kTileF + l];
-        fflux[(size_t)v * F + fout] = acc;
-    }
-}
-
 // ---------------------------------------------------------------------------
-// Side-sequential face tile kernel: as kRHSFaceW (NW waves per 64-face tile, PPW
-// points per lane, direct-to-LDS row staging), but the two sides of the tile are
-// staged one after the other into the same LDS rows: the left cells' mean,
+// Multi-point, side-sequential face tile kernel: NW = ceil(G / PPW) wavefronts per
+// 64-face tile, each lane evaluating PPW quadrature points of its face (lane = face),
+// so one LDS read of a staged row (reconstruction coefficient, centre, moment, mean)
+// feeds PPW point evaluations and the independent point computations give each wave
+// instruction-level parallelism.  Rows are staged straight from global memory into
+// LDS (buffer_load ... lds: a row of 64 doubles is two dword loads, lane l carrying
+// half of face l/2; no VGPRs, no LDS store instructions, all rows of a wave in
+// flight at once).  The two sides of the tile are staged one after the other into
+// the same LDS rows: the left cells' mean,
 // coefficients and metrics land, every wave reconstructs its points' left states,
 // then the right cells' rows (and the periodic shift) replace them and the right
 // states follow.  The LDS of a tile is one side's rows (5 + 5 M + 4 + M + 3), half
@@ -1437,35 +1137,19 @@ static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uR
         rows = 5 * G;
     size_t smem = (size_t)rows * kTileF * sizeof(real);
     constexpr int PPW = rhsPointsPerLane<DIM, G>();
-    static const int variant = getenv("DNDSR_RHS_VARIANT") ? atoi(getenv("DNDSR_RHS_VARIANT")) : 0;
-    if constexpr (PPW > 1)
-        if (variant > 0)
-        {
-            constexpr int NW = (G + PPW - 1) / PPW;
-            int srows = RHSSideRows<NDOF>::STAGED;
-            if (srows < 5 * G)
-                srows = 5 * G;
-            const size_t ss = (size_t)srows * kTileF * sizeof(real);
-            auto k3 = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, 3>;
-            auto k4 = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, 4>;
-            auto kern = variant == 1 ? k3 : k4;
-            kern<<<gridFor(c.m.F, kTileF), kTileF * NW, ss, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
-            DNDSR_HIP_CHECK(hipGetLastError());
-            if (c.m.nBnd > 0)
-            {
-                kRHSBoundary<DIM, NDOF, G, RS, BATCH><<<gridFor(c.m.nBnd, 64), 64, 0, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
-                DNDSR_HIP_CHECK(hipGetLastError());
-            }
-            return;
-        }
     if constexpr (PPW > 1)
     {
         constexpr int NW = (G + PPW - 1) / PPW;
-        auto kern = kRHSFaceW<DIM, NDOF, G, RS, BATCH, PPW>;
-        if (smem > 65536)
+        int srows = RHSSideRows<NDOF>::STAGED;
+        if (srows < 5 * G)
+            srows = 5 * G;
+        const size_t ss = (size_t)srows * kTileF * sizeof(real);
+        // three waves per SIMD where the registers allow it; the cubic basis needs the room of two
+        auto kern = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, (NDOF >= 20 ? 2 : 3)>;
+        if (ss > 65536)
             DNDSR_HIP_CHECK(hipFuncSetAttribute(reinterpret_cast<const void *>(kern),
-                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)smem));
-        kern<<<gridFor(c.m.F, kTileF), kTileF * NW, smem, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
+                                                hipFuncAttributeMaxDynamicSharedMemorySize, (int)ss));
+        kern<<<gridFor(c.m.F, kTileF), kTileF * NW, ss, c.stream>>>(c.m, gp, u, uRec, dlc, ff);
     }
     else
     {
@@ -1555,27 +1239,30 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
     if (m.nBnd > 0 && !m.bcType)
         throw std::runtime_error("the mesh has boundary faces: set the boundary conditions first (dndsr_set_bc)");
     real *ff = faceFluxBuf ? faceFluxBuf : c.faceFlux;
-    KernelTimer kt(c, K_RHS_FACE);
-    if (m.dim == 2)
     {
-        if (!uRec || m.P == 1)
-            rhsDispatch<2, 3>(c, gp, u, uRec, dlc, ff);
-        else if (m.P == 2)
-            rhsDispatch<2, 6>(c, gp, u, uRec, dlc, ff);
+        KernelTimer kt(c, K_RHS_FACE); // inviscid interior, periodic and boundary faces
+        if (m.dim == 2)
+        {
+            if (!uRec || m.P == 1)
+                rhsDispatch<2, 3>(c, gp, u, uRec, dlc, ff);
+            else if (m.P == 2)
+                rhsDispatch<2, 6>(c, gp, u, uRec, dlc, ff);
+            else
+                rhsDispatch<2, 10>(c, gp, u, uRec, dlc, ff);
+        }
         else
-            rhsDispatch<2, 10>(c, gp, u, uRec, dlc, ff);
-    }
-    else
-    {
-        if (!uRec || m.P == 1)
-            rhsDispatch<3, 4>(c, gp, u, uRec, dlc, ff);
-        else if (m.P == 2)
-            rhsDispatch<3, 10>(c, gp, u, uRec, dlc, ff);
-        else
-            rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
+        {
+            if (!uRec || m.P == 1)
+                rhsDispatch<3, 4>(c, gp, u, uRec, dlc, ff);
+            else if (m.P == 2)
+                rhsDispatch<3, 10>(c, gp, u, uRec, dlc, ff);
+            else
+                rhsDispatch<3, 20>(c, gp, u, uRec, dlc, ff);
+        }
     }
     if (gp.viscous)
     {
+        KernelTimer kt(c, K_RHS_VISCOUS);
         if (!(gp.prGas > 0 && gp.Rgas > 0 && gp.muGas >= 0))
             throw std::runtime_error("viscous residual needs prGas > 0, Rgas > 0 and muGas >= 0");
         const int nb = gridFor(m.F, 128);
@@ -1919,6 +1606,83 @@ __global__ void __launch_bounds__(256) kDualResidual(idx N, real rdt, const real
         for (int v = 0; v < 5; v++)
             part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
 }
+// The BDF unsteady residual (ImplicitBDFDualTimeStep::Step, src/Solver/ODE.hpp:440-451):
+// rhs = sum_j a_j x_j - x / dt + alpha rhsF with a_j = c_j / dt over xLast and up to three
+// earlier states, summed in that order; L1 partial sums as kDualResidual.
+struct BDFTerms
+{
+    const real *x[4];
+    real a[4];
+    int n;
+};
+__global__ void __launch_bounds__(256) kDualResidualBDF(idx N, BDFTerms bt, real rdt, real alpha,
+                                                        const real *__restrict__ x, const real *__restrict__ rhsF,
+                                                        real *__restrict__ rhs, real *__restrict__ part)
+{
+    __shared__ real sm[5][256];
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    real loc[5] = {0, 0, 0, 0, 0};
+    if (i < N)
+    {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            const size_t k = (size_t)v * N + i;
+            real r = 0;
+            for (int j = 0; j < bt.n; j++)
+                r += bt.x[j][k] * bt.a[j];
+            r += x[k] * (-rdt);
+            r += rhsF[k] * alpha;
+            rhs[k] = r;
+            loc[v] = fabs(r);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = loc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
+}
+void launchDualResidualBDF(Ctx &c, real dt, const real *const *xs, const real *coef, int nx, real alpha,
+                           const real *x, const real *rhsF, real *rhs, real *resDev)
+{
+    KernelTimer kt(c, K_DUAL_RES);
+    if (nx < 1 || nx > 4)
+        throw std::runtime_error("the BDF residual takes 1..4 earlier states");
+    BDFTerms bt{};
+    bt.n = nx;
+    for (int j = 0; j < nx; j++)
+        bt.x[j] = xs[j], bt.a[j] = coef[j] / dt;
+    const int nb = gridFor(c.m.N, 256);
+    kDualResidualBDF<<<nb, 256, 0, c.stream>>>(c.m.N, bt, 1. / dt, alpha, x, rhsF, rhs, c.red);
+    DNDSR_HIP_CHECK(hipGetLastError());
+    kReduceSum5<<<1, 256, 0, c.stream>>>(c.red, nb, resDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+__global__ void __launch_bounds__(256) kScale(idx n, real s, real *__restrict__ x)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n)
+        x[i] *= s;
+}
+void launchScale(Ctx &c, idx n, real s, real *x)
+{
+    kScale<<<gridFor(n, 256), 256, 0, c.stream>>>(n, s, x);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
 void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs, real *resDev)
 {
     KernelTimer kt(c, K_DUAL_RES);
@@ -2025,8 +1789,8 @@ __global__ void __launch_bounds__(256) kToCellMajor(idx N, const real *__restric
 }
 
 template <typename T>
-void launchSGSPackedT(Ctx &c, const GasParams &gp, const T *rhs, const real *u, const T *lam, const T *jd, T *uA,
-                      T *slot, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
+void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const real *u, const T *lam,
+                      const T *jd, T *uA, T *slot, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
 {
     DevMesh &m = c.m;
     const int nb = gridFor(m.N, 256);
@@ -2046,24 +1810,25 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, const T *rhs, const real *u, 
                 if (e <= b)
                     continue;
                 KernelTimer kt(c, K_SGS);
-                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, T(1), rhs, uA, inc, tmp,
-                                                                                jd, slot, dir, gsUpdate, b, e);
+                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, uA, inc,
+                                                                                tmp, jd, slot, dir, gsUpdate, b, e);
                 DNDSR_HIP_CHECK(hipGetLastError());
             }
             std::swap(inc, tmp);
         }
     *result = inc;
 }
-void launchSGSPacked64(Ctx &c, const GasParams &gp, const real *rhs, const real *u, const real *lam, const real *jd,
-                       real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate, real **result)
+void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
+                       const real *jd, real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate,
+                       real **result)
 {
-    launchSGSPackedT<real>(c, gp, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
 }
-void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real *u, const float *lam,
-                       const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter, int gsUpdate,
-                       float **result)
+void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
+                       const float *lam, const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter,
+                       int gsUpdate, float **result)
 {
-    launchSGSPackedT<float>(c, gp, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
 }
 
 // ---------------------------------------------------------------------------

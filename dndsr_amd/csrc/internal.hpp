@@ -155,7 +155,8 @@ enum KernelId : int
     K_CONVERT = 10,
     K_ADJOINT = 11,
     K_LINSOLVE = 12,
-    K_NUM = 13,
+    K_RHS_VISCOUS = 13,
+    K_NUM = 14,
 };
 
 // Records a HIP event pair around the launches in its scope when timing is on.
@@ -235,12 +236,19 @@ void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const re
 void launchSGSSweepBlock32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
                            const float *uInc, float *uIncNew, const float *jinv, const float *lam,
                            const float *lamRoe, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
+// device bandwidth self-test (probe.hip): mode 0 read stream, 1 copy; average ms per pass
+double streamProbe(Ctx &c, int mode, const double *src, double *dst, size_t n, int reps);
+int streamProbeBlocks(Ctx &c);
 // sum of nPart 5-wide partials in c.red into out[5]
 void launchReduceSum5(Ctx &c, int nPart, real *out);
 void launchAxpby(Ctx &c, idx n, real a, const real *x, real b, const real *y, real cdt, const real *z,
                  const real *dtDev, real *out, real *save = nullptr);
 // dst[0:n) = src[0:n) on the library's stream
 void launchCopy(Ctx &c, size_t n, const real *src, real *dst);
+// the BDF unsteady residual: sum_j coef_j / dt xs_j - x / dt + alpha rhsF, nx in 1..4 (xs[0] = xLast)
+void launchDualResidualBDF(Ctx &c, real dt, const real *const *xs, const real *coef, int nx, real alpha,
+                           const real *x, const real *rhsF, real *rhs, real *resDev);
+void launchScale(Ctx &c, idx n, real s, real *x);
 void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const real *rhsF, real *rhs,
                         real *resDev);
 // statistics of the positivity-preserving increment: limited cells, smallest alpha (as the bits
@@ -261,11 +269,12 @@ void launchToF32(Ctx &c, size_t n, const real *in, float *out);
 // (1 + sgsIter) forward/backward multicolour sweeps from a zero increment with the state
 // gathered from neighbours stored cell-major: uA <- u as [N][5], slot <- per-cell-slot face
 // data; inc / tmp are [N][5] scratch; *result points at the buffer holding the increment.
-void launchSGSPacked64(Ctx &c, const GasParams &gp, const real *rhs, const real *u, const real *lam, const real *jd,
-                       real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate, real **result);
-void launchSGSPacked32(Ctx &c, const GasParams &gp, const float *rhs, const real *u, const float *lam,
-                       const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter, int gsUpdate,
-                       float **result);
+void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
+                       const real *jd, real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate,
+                       real **result);
+void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
+                       const float *lam, const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter,
+                       int gsUpdate, float **result);
 // u += du with du cell-major ([N][5], FP64 or FP32: exactly one of du64 / du32 non-null)
 void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
                                  const float *du32, PPStat *stat);

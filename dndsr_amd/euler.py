@@ -51,6 +51,29 @@ class JacobianDiagBlock:
         return self.value[:, iCell].reshape(5, 5)
 
 
+class ImplicitBDFDualTimeStep:
+    """The reference's k-step BDF stepper (src/Solver/ODE.hpp:362-596) over an EulerEvaluator:
+    it keeps the k - 1 earlier states on the device and starts at first order, raising the
+    order by one per step up to k (kCurrent = number of stored states + 1)."""
+
+    def __init__(self, ev, k: int = 2):
+        if not 1 <= k <= 4:
+            raise ValueError("k must be 1..4")
+        self.ev, self.k = ev, k
+        self.xPrevs = []
+        self.dtPrev = None
+
+    def Step(self, u, uRec, dt, **settings):
+        if self.xPrevs and abs(self.dtPrev - dt) > 1e-8 * dt:
+            raise ValueError("the BDF stepper takes a constant dt (ODE.hpp:432)")
+        xLast = u.clone()
+        out = self.ev.ImplicitBDFStep(u, uRec, dt, self.xPrevs, **settings)
+        if self.k > 1:
+            self.xPrevs = ([xLast] + self.xPrevs)[: self.k - 1]
+        self.dtPrev = dt
+        return out
+
+
 class EulerEvaluator:
     nVars = 5
 
@@ -237,6 +260,27 @@ class EulerEvaluator:
         check(lib().dndsr_implicit_euler_step(self.ctx.h, C.byref(self.es), C.byref(s), _ptr(u), _ptr(uRec),
                                               C.byref(n), res.ctypes.data_as(_lib.dp), rel.ctypes.data_as(_lib.dp),
                                               hist.ctypes.data_as(_lib.dp)))
+        return n.value, res, rel, hist[:n.value]
+
+    def ImplicitBDFStep(self, u, uRec, dt, xPrevs=(), CFL=10.0, nRec=1, sgsIter=0, jacobiCode=1, maxIter=20,
+                        minIter=5, tol=1e-10, fp32Sweeps=False, incrementPPRelax=0.9, refRho=1.0, refP=1.0,
+                        gmresCode=0, nGmresSpace=10, nGmresIter=2):
+        """One BDF dual-time step (ImplicitBDFDualTimeStep::Step, src/Solver/ODE.hpp:427-477) with
+        the earlier states xPrevs (device [5, nCell] tensors, newest first, up to three): the
+        (len(xPrevs) + 1)-step formula.  Settings and return value as ImplicitEulerStep."""
+        _check_jacobi_code(jacobiCode)
+        if len(xPrevs) > 3:
+            raise ValueError("at most three earlier states (BDF4)")
+        s = _lib.ImplicitSettings(float(dt), float(CFL), int(nRec), int(sgsIter), int(jacobiCode), int(maxIter),
+                                  int(minIter), float(tol), int(bool(fp32Sweeps)), float(incrementPPRelax),
+                                  float(refRho), float(refP), int(gmresCode), int(nGmresSpace), int(nGmresIter))
+        ptrs = (C.c_void_p * max(len(xPrevs), 1))(*[x.data_ptr() for x in xPrevs])
+        n = C.c_int()
+        res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(int(maxIter), 1))
+        check(lib().dndsr_implicit_bdf_step(self.ctx.h, C.byref(self.es), C.byref(s), len(xPrevs),
+                                            C.cast(ptrs, C.POINTER(C.c_void_p)), _ptr(u), _ptr(uRec), C.byref(n),
+                                            res.ctypes.data_as(_lib.dp), rel.ctypes.data_as(_lib.dp),
+                                            hist.ctypes.data_as(_lib.dp)))
         return n.value, res, rel, hist[:n.value]
 
     def PPStats(self):

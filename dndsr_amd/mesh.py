@@ -632,6 +632,68 @@ def bump_box_tet(n: int = 69, amp: float = 0.1, jitter: float = 0.1, seed: int =
                     zones=dict(xmin=0, xmax=1, ymin=2, ymax=3), jitter=jitter, seed=seed, node_map=bump)
 
 
+def bump_box_delaunay(n: int = 66, amp: float = 0.1, jitter: float = 0.1, seed: int = 20261202) -> Mesh:
+    """Unit box [0,1]^3: the n^3 lattice's (n + 1)^3 vertices jittered uniformly by +-jitter of
+    the edge (numpy PCG64 seeded by seed; boundary vertices move only within their side) and
+    Delaunay-tetrahedralised (scipy.spatial.Delaunay, Qhull), then the lower wall raised by the
+    bump y_w = amp sin^2(pi x) with the grid following it (y' = y + y_w (1 - y)).  About
+    6.95 n^3 tetrahedra (n = 66: 1.998M), with the slivers a Delaunay mesh of jittered points
+    carries.  Cells come in Qhull's order (see reorder_cells).  Zones: 0 inlet (x = 0), 1 outlet
+    (x = 1), 2 lower wall, 3 upper boundary, 4 the spanwise sides (z = 0, 1)."""
+    from scipy.spatial import Delaunay
+    h = 1.0 / n
+    I, J, K = np.meshgrid(np.arange(n + 1), np.arange(n + 1), np.arange(n + 1), indexing="ij")
+    X = np.stack([I * h, J * h, K * h], axis=-1).astype(np.float64)
+    X = _jitter_nodes_box(X, np.array([h, h, h]), jitter, seed, (False, False, False))
+    P = X.reshape(-1, 3)
+    cells = Delaunay(P).simplices.astype(np.int64)
+    Q = P.copy()
+    Q[:, 1] = P[:, 1] + amp * np.sin(np.pi * P[:, 0]) ** 2 * (1 - P[:, 1])
+    lat = np.stack([I.ravel(), J.ravel(), K.ravel()], 1)
+    zones = dict(xmin=0, xmax=1, ymin=2, ymax=3, zmin=4, zmax=4)
+    return simplex_mesh(3, Q, cells, None, _lattice_zone(lat, (n, n, n), (False, False, False), zones), (n, n, n))
+
+
+def reorder_cells(m: Mesh, method: str = "rcm") -> Mesh:
+    """Locality renumbering of the cells of an unstructured mesh (the role of
+    UnstructuredMesh::ReorderLocalCells, src/Geom/Mesh/Mesh_Reorder.cpp:662-700, whose
+    permutation is a Metis partition with a reverse Cuthill-McKee order inside each part,
+    Mesh_CellPermutation.hpp:42-130; one GPU holds one part).  "rcm": reverse Cuthill-McKee on
+    the face-adjacency graph (scipy.sparse.csgraph), which bounds the index distance between
+    face neighbours by the graph bandwidth; "morton": cells sorted along the Z-order curve of
+    their node-mean points.  Faces follow their back cell (permute_cells)."""
+    N = m.nCell
+    if method == "rcm":
+        import scipy.sparse as sp
+        from scipy.sparse.csgraph import reverse_cuthill_mckee
+        f2c = m.face2cell
+        inner = (f2c[:, 1] >= 0) & (f2c[:, 0] != f2c[:, 1])
+        a, b = f2c[inner, 0], f2c[inner, 1]
+        g = sp.coo_matrix((np.ones(2 * len(a), np.int8), (np.r_[a, b], np.r_[b, a])), shape=(N, N)).tocsr()
+        order = np.asarray(reverse_cuthill_mckee(g, symmetric_mode=True), np.int64)
+    elif method == "morton":
+        ptr = m.cell2nodePtr
+        cnt = np.diff(ptr)
+        c = np.add.reduceat(m.coords[m.cell2node], ptr[:-1], axis=0) / cnt[:, None]
+        lo, hi = c.min(0), c.max(0)
+        q = np.clip(((c - lo) / np.maximum(hi - lo, 1e-300) * (1 << 20)).astype(np.int64), 0, (1 << 20) - 1)
+        key = np.zeros(N, np.int64)
+        for bit in range(20):
+            for ax in range(3):
+                key |= ((q[:, ax] >> bit) & 1) << (3 * bit + ax)
+        order = np.argsort(key, kind="stable")
+    else:
+        raise ValueError(f"unknown reorder method {method!r}")
+    return permute_cells(m, order)
+
+
+def bandwidth(m: Mesh) -> int:
+    """Largest index distance between the two cells of an interior face."""
+    f2c = m.face2cell
+    inner = f2c[:, 1] >= 0
+    return int(np.abs(f2c[inner, 0] - f2c[inner, 1]).max()) if inner.any() else 0
+
+
 QUAD4_FACES = np.array([[0, 1], [1, 2], [2, 3], [3, 0]])
 
 

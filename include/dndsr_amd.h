@@ -311,6 +311,17 @@ typedef struct
 int dndsr_implicit_euler_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
                               double *u, double *uRec, int *nIter, double *res, double *resRel, double *history);
 
+/* One BDF dual-time step (ImplicitBDFDualTimeStep::Step, src/Solver/ODE.hpp:427-477; coefficients
+ * :590-595): as dndsr_implicit_euler_step, with the unsteady residual of the (nPrev + 1)-step
+ * formula, sum_j c_j x_j / dt - x / dt + alpha frhs(x) over the entry state and the nPrev (0..3)
+ * earlier states xPrevs[j] (device arrays [5][nCell], newest first), dTau scaled by 1 / alpha
+ * (EulerSolver.hxx:693) and alphaDiag = alpha in LUSGSMatrixInit and the sweeps.  The caller
+ * keeps the history between steps (the reference's xPrevs ring) and starts with nPrev = 0 (BDF1,
+ * the implicit-Euler step), as the reference does. */
+int dndsr_implicit_bdf_step(dndsr_ctx *ctx, const dndsr_euler_settings *es, const dndsr_implicit_settings *is,
+                            int nPrev, const double *const *xPrevs, double *u, double *uRec, int *nIter, double *res,
+                            double *resRel, double *history);
+
 /* Positivity statistics of the last dndsr_implicit_euler_step: the number of cell increments
  * the positivity limiter scaled (summed over its iterations; the PPIncrementLimiter count the
  * reference's solver logs, EulerSolver.hxx:708-718) and the smallest alpha (1 when none). */
@@ -366,9 +377,17 @@ int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const d
  *   0 VR iteration, 1 RHS face pass, 2 RHS cell gather, 3 EvaluateDt face pass,
  *   4 EvaluateDt cell pass, 5 RK stage combination, 6 SGS colour sweep, 7 LU-SGS init,
  *   8 dual-time residual + norm, 9 increment, 10 FP64 -> FP32 conversion, 11 adjoint kernels,
- *   12 linear-solver kernels (LUSGSMatrixVec, GMRES vector operations).
+ *   12 linear-solver kernels (LUSGSMatrixVec, GMRES vector operations), 13 RHS viscous face
+ *   pass.  The face pass (1) covers the inviscid interior, periodic and boundary faces only.
  * reset != 0 clears the totals after reading them. */
-#define DNDSR_NUM_KERNEL_IDS 13
+#define DNDSR_NUM_KERNEL_IDS 14
+/* Device bandwidth self-test (no counterpart in the reference; it calibrates the roofline the
+ * kernels are quoted against): mode 0 streams n doubles of src once, reading 16 bytes per lane
+ * (dst receives one partial sum per workgroup and must hold 4096 doubles); mode 1 copies src
+ * to dst.  Both 16-byte aligned, n even.  Returns the average milliseconds of one pass over
+ * reps passes, timed with HIP events on the context's stream (synchronises). */
+int dndsr_stream_probe(dndsr_ctx *ctx, int mode, const double *src, double *dst, int64_t n, int reps,
+                       double *msPerPass);
 int dndsr_set_timing(dndsr_ctx *ctx, int enable);
 int dndsr_get_timing(dndsr_ctx *ctx, double *msTotal, int64_t *launches, int reset);
 

@@ -2540,8 +2540,38 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
     return orc_eval_rhs_lam(h, es, u, uRec, deltaLambdaCell, rhs, nullptr);
 }
 
+static int evalRHSCore(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                       const double *deltaLambdaCell, double *rhs, double *lamRoe, double *faceFlux);
+
 int orc_eval_rhs_lam(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
                      const double *deltaLambdaCell, double *rhs, double *lamRoe)
+{
+    return evalRHSCore(h, es, u, uRec, deltaLambdaCell, rhs, lamRoe, nullptr);
+}
+
+// Force on the boundary faces of one zone: the momentum part of the integrated face fluxes
+// (Riemann minus viscous flux, sum_g -finc JDet w) with the sign of the flux out of the domain.
+// The wall force of a drag functional; the reference's solver integrates it in its output stage.
+int orc_zone_force(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                   const double *deltaLambdaCell, int zone, double *force)
+{
+    const VR &v = h->v;
+    std::vector<real> rhs((size_t)v.nCell * 5), ff((size_t)v.nFace * 5);
+    int rc = evalRHSCore(h, es, u, uRec, deltaLambdaCell, rhs.data(), nullptr, ff.data());
+    if (rc)
+        return rc;
+    real a[3] = {0, 0, 0};
+    for (idx_t f = 0; f < v.nFace; f++)
+        if (v.f2c[f][1] < 0 && v.faceZone[f] == zone)
+            for (int d = 0; d < 3; d++)
+                a[d] -= ff[(size_t)f * 5 + 1 + d];
+    for (int d = 0; d < 3; d++)
+        force[d] = a[d];
+    return 0;
+}
+
+static int evalRHSCore(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                       const double *deltaLambdaCell, double *rhs, double *lamRoe, double *faceFlux)
 {
     ORC_TRY
     const VR &v = h->v;
@@ -2730,6 +2760,10 @@ int orc_eval_rhs_lam(const orc_vr *h, const orc_euler_settings *es, const double
     }
     if (!firstErr.empty())
         throw std::runtime_error(firstErr);
+    if (faceFlux)
+        for (idx_t f = 0; f < v.nFace; f++)
+            for (int var = 0; var < 5; var++)
+                faceFlux[(size_t)f * 5 + var] = fbuf[f][var];
     if (!g_rhsGather)
     {
         // serial build: direct scatter in face order, EvaluateRHS.hxx:550-562
@@ -3249,11 +3283,48 @@ int orc_ssprk3(const orc_vr *h, const orc_euler_settings *es, double *u, double 
 // 1 + sgsIter times), fincrement :696-738 (increment alpha = 1 when no positivity limit
 // is needed, then AddFixedIncrement), fstop -> functor_fstop (EulerSolver_Init.hxx:490-530:
 // L1 norm, running-maximum base, density component).
+// BDF coefficients of ImplicitBDFDualTimeStep (src/Solver/ODE.hpp:590-595): row k - 1 holds
+// (alpha, c_last, c_prev1, ...) of the k-step formula
+static const real kBDF[4][5] = {{1. / 1., 1. / 1., 0, 0, 0},
+                                {2. / 3., 4. / 3., -1. / 3., 0, 0},
+                                {6. / 11., 18. / 11., -9. / 11., 2. / 11., 0},
+                                {12. / 25., 48. / 25., -36. / 25., 16. / 25., -3. / 25.}};
+
+static int implicitStep(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is, int kCur,
+                        const double *xPrevs, double *u, double *uRec, const int64_t *scanOrder, int *nIter,
+                        double *res, double *resRel, double *history, double *ppStats);
+
 int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
                             double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
                             double *resRel, double *history, double *ppStats)
 {
+    return implicitStep(h, es, is, 0, nullptr, u, uRec, scanOrder, nIter, res, resRel, history, ppStats);
+}
+
+// ImplicitBDFDualTimeStep::Step (src/Solver/ODE.hpp:427-477): the k-step BDF with nPrev earlier
+// states (kCurrent = nPrev + 1), xPrevs [nPrev][N][5] newest first
+int orc_implicit_bdf_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is, int nPrev,
+                          const double *xPrevs, double *u, double *uRec, const int64_t *scanOrder, int *nIter,
+                          double *res, double *resRel, double *history, double *ppStats)
+{
+    if (nPrev < 0 || nPrev > 3)
+    {
+        g_err = "nPrev must be 0..3 (BDF1..BDF4)";
+        return -1;
+    }
+    return implicitStep(h, es, is, nPrev + 1, xPrevs, u, uRec, scanOrder, nIter, res, resRel, history, ppStats);
+}
+
+// kCur 0: ImplicitEulerDualTimeStep::Step (ODE.hpp:80-98); 1..4: the BDF step with kCur - 1
+// earlier states.  Per iteration: fdtau (EvaluateDt, dTau / alpha), frhs, the unsteady residual
+// sum_j c_j x_j / dt - x / dt + alpha frhs, fsolve (LUSGSMatrixInit and the sweeps with alphaDiag =
+// alpha), fincrement, fstop.
+static int implicitStep(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is, int kCur,
+                        const double *xPrevs, double *u, double *uRec, const int64_t *scanOrder, int *nIter,
+                        double *res, double *resRel, double *history, double *ppStats)
+{
     ORC_TRY
+    const real alphaDiag = kCur ? kBDF[kCur - 1][0] : 1.0;
     const VR &v = h->v;
     idx_t N = v.nCell;
     size_t nU = (size_t)N * 5, nR = nU * v.M;
@@ -3273,30 +3344,45 @@ int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const
         real dtMin;
         if (orc_eval_dt(h, es, u, is->CFL, 1e100, 1, dTau.data(), lam.data(), dlc.data(), &dtMin))
             throw std::runtime_error(g_err);
+        if (kCur) // dTau *= 1 / alphaDiag (EulerSolver.hxx:693)
+            for (idx_t i = 0; i < N; i++)
+                dTau[i] *= 1. / alphaDiag;
         for (int r = 0; r < is->nRec; r++)
             vrSweep(h, uRec, u, scanOrder, tmp);
         if (orc_eval_rhs_lam(h, es, u, uRec, dlc.data(), rhsF.data(), roeJ ? lamRoe.data() : nullptr))
             throw std::runtime_error(g_err);
-        // crhs = xLast / dt - x / dt + frhs
-        for (size_t q = 0; q < nU; q++)
-        {
-            real r = xLast[q] * (1.0 / is->dt);
-            r += u[q] * (-1.0 / is->dt);
-            r += rhsF[q];
-            rhs[q] = r;
-        }
-        if (roeJ ? orc_lusgs_init_block(h, es, dTau.data(), is->dt, 1.0, u, lam.data(), lamRoe.data(), jdB.data(),
-                                        jinvB.data())
-                 : orc_lusgs_init(h, dTau.data(), is->dt, 1.0, lam.data(), jd.data()))
+        if (!kCur) // crhs = xLast / dt - x / dt + frhs
+            for (size_t q = 0; q < nU; q++)
+            {
+                real r = xLast[q] * (1.0 / is->dt);
+                r += u[q] * (-1.0 / is->dt);
+                r += rhsF[q];
+                rhs[q] = r;
+            }
+        else // BDF: sum_j c_j x_j / dt - x / dt + alpha frhs (ODE.hpp:440-451)
+            for (size_t q = 0; q < nU; q++)
+            {
+                real r = 0;
+                r += xLast[q] * (kBDF[kCur - 1][1] / is->dt);
+                for (int j = 0; j < kCur - 1; j++)
+                    r += xPrevs[(size_t)j * nU + q] * (kBDF[kCur - 1][2 + j] / is->dt);
+                r += u[q] * (-1. / is->dt);
+                r += rhsF[q] * kBDF[kCur - 1][0];
+                rhs[q] = r;
+            }
+        if (roeJ ? orc_lusgs_init_block(h, es, dTau.data(), is->dt, alphaDiag, u, lam.data(), lamRoe.data(),
+                                        jdB.data(), jinvB.data())
+                 : orc_lusgs_init(h, dTau.data(), is->dt, alphaDiag, lam.data(), jd.data()))
             throw std::runtime_error(g_err);
         std::fill(inc.begin(), inc.end(), 0.0);
         real sumInc[5];
         for (int s = 0; s <= is->sgsIter; s++)
             for (int dir = 1; dir >= 0; dir--)
             {
-                int rc = roeJ ? orc_update_sgs_block(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jinvB.data(),
-                                                     lam.data(), lamRoe.data(), dir, is->gsUpdate, 0, scanOrder, sumInc)
-                              : orc_update_sgs(h, es, 1.0, rhs.data(), u, inc.data(), incNew.data(), jd.data(),
+                int rc = roeJ ? orc_update_sgs_block(h, es, alphaDiag, rhs.data(), u, inc.data(), incNew.data(),
+                                                     jinvB.data(), lam.data(), lamRoe.data(), dir, is->gsUpdate, 0,
+                                                     scanOrder, sumInc)
+                              : orc_update_sgs(h, es, alphaDiag, rhs.data(), u, inc.data(), incNew.data(), jd.data(),
                                                lam.data(), dir, is->gsUpdate, 0, scanOrder, sumInc);
                 if (rc)
                     throw std::runtime_error(g_err);

@@ -151,6 +151,10 @@ int orc_eval_rhs(const orc_vr *h, const orc_euler_settings *es, const double *u,
  * ([nFace][3], EvaluateRHS.hxx:527-544) when lamRoe is not NULL */
 int orc_eval_rhs_lam(const orc_vr *h, const orc_euler_settings *es, const double *u,
                      const double *uRec, const double *deltaLambdaCell, double *rhs, double *lamRoe);
+/* force on the boundary faces of zone `zone` (momentum flux out of the domain, Riemann minus
+ * viscous flux integrated over the faces), force[3] */
+int orc_zone_force(const orc_vr *h, const orc_euler_settings *es, const double *u, const double *uRec,
+                   const double *deltaLambdaCell, int zone, double *force);
 /* LUSGSMatrixInit with useRoeJacobian (EulerEvaluator.hxx:37-159): 5x5 blocks [N][5][5]
  * (row-major) and their inverses (JacobianDiagBlock::GetInvert, EulerJacobian.hpp:153-190);
  * lamRoe [nFace][3] = lambdaFace0/123/4, or NULL for EvaluateDt's (lambdaFace[.][3]) */
@@ -192,6 +196,12 @@ typedef struct
 int orc_implicit_euler_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is,
                             double *u, double *uRec, const int64_t *scanOrder, int *nIter, double *res,
                             double *resRel, double *history, double *ppStats /* [2]: limited cells, min alpha */);
+
+/* ImplicitBDFDualTimeStep::Step (src/Solver/ODE.hpp:427-477) with nPrev (0..3) earlier states
+ * xPrevs [nPrev][N][5], newest first: the (nPrev + 1)-step BDF formula */
+int orc_implicit_bdf_step(const orc_vr *h, const orc_euler_settings *es, const orc_implicit_settings *is, int nPrev,
+                          const double *xPrevs, double *u, double *uRec, const int64_t *scanOrder, int *nIter,
+                          double *res, double *resRel, double *history, double *ppStats);
 
 /* positivity of the increment (ideal gas, 5 conservative variables per cell):
  * EvaluateCellRHSAlpha for one cell (returns nonzero where the reference asserts) */

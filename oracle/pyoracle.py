@@ -68,8 +68,8 @@ def lib():
         _lib.orc_rec_error_l1.restype = C.c_double
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
-                     "orc_eval_dt", "orc_eval_rhs", "orc_eval_rhs_lam", "orc_lusgs_init", "orc_update_sgs",
-                     "orc_lusgs_init_block", "orc_update_sgs_block", "orc_ssprk3", "orc_implicit_euler_step", "orc_lusgs_matvec",
+                     "orc_eval_dt", "orc_eval_rhs", "orc_eval_rhs_lam", "orc_zone_force", "orc_lusgs_init", "orc_update_sgs",
+                     "orc_lusgs_init_block", "orc_update_sgs_block", "orc_ssprk3", "orc_implicit_euler_step", "orc_implicit_bdf_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost",
                      "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp", "orc_model_rhs"]:
             getattr(_lib, name).argtypes = None
@@ -306,6 +306,15 @@ class OracleVR:
                                     P(rhs), P(lr)))
         return (rhs, lr) if roeLambdas else rhs
 
+    def zone_force(self, es, u, uRec, deltaLambdaCell, zone):
+        """Force on the boundary faces of a zone (the wall force of a drag functional), [3]."""
+        f = np.zeros(3)
+        _chk(lib().orc_zone_force(self.h, C.byref(es), P(np.ascontiguousarray(u)),
+                                  None if uRec is None else P(np.ascontiguousarray(uRec)),
+                                  None if deltaLambdaCell is None else P(np.ascontiguousarray(deltaLambdaCell)),
+                                  C.c_int(zone), P(f)))
+        return f
+
     def lusgs_init_block(self, es, dTau, dt, alphaDiag, u, lam, lamRoe=None):
         """LUSGSMatrixInit with useRoeJacobian: (JDiag [N, 5, 5], its inverse [N, 5, 5])."""
         jd, ji = np.zeros((self.nCell, 5, 5)), np.zeros((self.nCell, 5, 5))
@@ -366,6 +375,21 @@ class OracleVR:
                                            P(rel), P(hist), P(pps)))
         if stats:
             return u, uRec, n.value, res, rel, hist[:n.value], (int(pps[0]), pps[1])
+        return u, uRec, n.value, res, rel, hist[:n.value]
+
+    def implicit_bdf_step(self, es, u, uRec, dt, xPrevs=(), CFL=10.0, nRec=1, sgsIter=0, gsUpdate=1, maxIter=20,
+                          minIter=5, tol=1e-10, incrementRelax=0.9, refRho=1.0, refP=1.0, scanOrder=None):
+        """ImplicitBDFDualTimeStep::Step with the earlier states xPrevs (newest first, up to 3):
+        the (len(xPrevs) + 1)-step BDF formula.  Returns (u, uRec, nIter, res, resRel, history)."""
+        u = np.array(u, dtype=np.float64, copy=True)
+        uRec = np.array(uRec, dtype=np.float64, copy=True)
+        st = ImplicitSettings(dt, CFL, nRec, sgsIter, gsUpdate, maxIter, minIter, tol, 0, incrementRelax, refRho, refP)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        xp = np.ascontiguousarray(np.stack(xPrevs), np.float64) if len(xPrevs) else None
+        n = C.c_int()
+        res, rel, hist = np.zeros(5), np.zeros(5), np.zeros(max(maxIter, 1))
+        _chk(lib().orc_implicit_bdf_step(self.h, C.byref(es), C.byref(st), C.c_int(len(xPrevs)), P(xp), P(u),
+                                         P(uRec), P(so), C.byref(n), P(res), P(rel), P(hist), None))
         return u, uRec, n.value, res, rel, hist[:n.value]
 
     def model_rhs(self, u, uRec=None, ax=1.0, ay=0.0, sigma=0.0):

@@ -18,7 +18,7 @@ from __future__ import annotations
 import numpy as np
 import pytest
 
-from helpers import random_rec, random_state, small_mesh, vr_settings
+from helpers import RS, random_rec, random_state, small_mesh, vr_settings
 
 pytestmark = pytest.mark.gpu
 
@@ -210,4 +210,41 @@ def test_zone_force_vjp_matches_central_fd(require_gpu, kind, order, viscous):
         fp = adjoint.zone_force(ev, u + eps * du, rec + eps * dr, 2, dlc)
         fm = adjoint.zone_force(ev, u - eps * du, rec - eps * dr, 2, dlc)
         fd = float(fb @ ((fp - fm) / (2 * eps)))
+        assert abs(lhs - fd) <= 1e-6 * max(abs(fd), 1e-30), (lhs, fd)
+
+
+@pytest.mark.parametrize("kind,order,viscous", [("quad", 1, True), ("tet", 1, True), ("quad", 2, False)])
+def test_zone_force_vjp_matches_oracle_fd(require_gpu, kind, order, viscous):
+    """The wall-force gradient of the device (zone 2, the no-slip wall) against central finite
+    differences of the same functional evaluated by the oracle (the reference-path FD gradient)."""
+    import pyoracle as po
+    import torch
+    from dndsr_amd import adjoint, bc
+    print(f"\n[adjoint bc] wall force vjp vs oracle FD {kind} p{order} viscous={viscous}", flush=True)
+    g, ev = _case(kind, order, "Roe", viscous)
+    u, rec = _state(g)
+    dt = torch.zeros(g.nCell, dtype=torch.float64, device=u.device)
+    ev.EvaluateDt(dt, u, None, 1.0)
+    dlc = ev.deltaLambdaCell.clone()
+    fb = np.array([1.0, 0.3, -0.2])
+    uBar, recBar = adjoint.zone_force_vjp(ev, u, rec, 2, fb, dlc)
+    types = ev.settings["bcTypes"]
+    vals = np.asarray(ev.settings["bcValues"], np.float64)
+    s = vr_settings(order)
+    o = po.OracleVR(g.mesh, **s, zoneDirWeight=bc.euler_dir_weights(types, order))
+    o.set_bc([bc.bc_code(t) for t in types], vals, GAMMA)
+    es = po.OracleVR.euler_settings(GAMMA, RS["Roe"], 0, 1.0, 1.0, viscous=int(viscous), **GAS)
+    uh, rh, dlh = g.to_host(u), g.to_host(rec), dlc.cpu().numpy()
+    # the functional itself agrees
+    np.testing.assert_allclose(o.zone_force(es, uh, rh, dlh, 2), adjoint.zone_force(ev, u, rec, 2, dlc),
+                               rtol=1e-12, atol=1e-14)
+    rng = np.random.default_rng(9)
+    vu = rng.normal(size=uh.shape) * 1e-1
+    vr = rng.normal(size=rh.shape) * 1e-2
+    eps = 1e-6
+    for du, dr, bar in ((vu, 0 * vr, uBar), (0 * vu, vr, recBar)):
+        fp = o.zone_force(es, uh + eps * du, rh + eps * dr, dlh, 2)
+        fm = o.zone_force(es, uh - eps * du, rh - eps * dr, dlh, 2)
+        fd = float(fb @ ((fp - fm) / (2 * eps)))
+        lhs = float(np.sum(g.to_host(bar) * (du if bar is uBar else dr)))
         assert abs(lhs - fd) <= 1e-6 * max(abs(fd), 1e-30), (lhs, fd)

@@ -231,3 +231,19 @@ def test_symmetry_plane_equals_mirrored_domain(rs, order):
     assert np.abs(rh - rf[upper]).max() <= 1e-12 * scale
     # the full-domain residual is itself mirror symmetric (a check on the construction)
     assert np.abs(rf[lower] - rf[upper] * mirror).max() <= 1e-12 * scale
+
+
+def test_zone_force_of_uniform_pressure():
+    """A uniform state moving parallel to a flat slip wall: the Riemann flux through the wall
+    is p n, so the oracle's force on the wall zone is p times the wall length along its outward
+    normal, (0, -p Lx, 0)."""
+    Lx, p = 3.0, 0.9
+    m = mm.box_quad_grid(6, 4, Lx, 2.0, zones=dict(xmin=0, xmax=0, ymin=1, ymax=0), jitter=0.0)
+    types = [bc.BC_FAR, bc.BC_WALL_INVIS]
+    U = _cons(1.1, [0.4, 0.0, 0.0], p)
+    o = po.OracleVR(m, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+    o.set_bc(types, np.stack([U, U]), 1.4)
+    es = o.euler_settings(1.4, RS["Roe"])
+    u = np.tile(U, (o.nCell, 1))
+    f = o.zone_force(es, u, np.zeros((o.nCell, 5, o.M)), None, 1)
+    np.testing.assert_allclose(f, [0.0, -p * Lx, 0.0], atol=1e-12)
