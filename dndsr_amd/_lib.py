@@ -77,6 +77,8 @@ SIGNATURES = {
     "dndsr_lusgs_init": [vp, vp, C.c_double, C.c_double, vp, vp],
     "dndsr_update_sgs": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                          C.c_int, vp],
+    "dndsr_update_sgs_transpose": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, C.c_int,
+                                   C.c_int, C.c_int, vp],
     "dndsr_update_sgs_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, C.c_int, C.c_int,
                              C.c_int, vp],
     "dndsr_eval_rhs_roe_lambda": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],

@@ -152,3 +152,120 @@ def zone_force_vjp(ev, u, uRec, zone, forceBar, deltaLambdaCell=None, need_rec=T
     check(lib().dndsr_zone_force_vjp(ev.ctx.h, C.byref(ev.es), _ptr(u.contiguous()), _ptr(uRec), _ptr(dlc),
                                      int(zone), fb.ctypes.data_as(_lib.dp), _ptr(uBar), _ptr(recBar)))
     return uBar, recBar
+
+
+# flux forms whose dissipation reads deltaLambdaCell: the entropy fixes of Roe_M6 / Roe_M8
+# (Gas.hpp:878-1007, eig 6 and 8) and the HLLE+ dissipation factor (Gas.hpp:1150-1280)
+_READS_DELTA_LAMBDA = (3, 16, 18, 21)
+
+
+class SteadyAdjoint:
+    """Discrete adjoint of a converged steady state on the device: the reference has none.
+
+    At a steady state of the implicit solver (EulerSolver.hxx:645-830 marched with an
+    infinite physical step) the state u and the reconstruction uRec satisfy
+
+        R(u, uRec; p) = 0          (EvaluateRHS)
+        uRec = S(uRec, u; p)       (a converged Jacobi VR sweep),
+
+    where p are boundary values.  For a functional J(u, uRec) the multipliers lam (per cell,
+    5 values) and mu (per reconstruction coefficient) satisfy
+
+        mu  = dJ/duRec + (dR/duRec)^T lam + (dS/duRec)^T mu
+        0   = dJ/du    + (dR/du)^T lam    + (dS/du)^T mu,
+
+    and dJ/dp = dJ/dp|explicit + lam . dR/dp + mu . dS/dp at fixed (u, uRec).  ``solve``
+    iterates both with the transposed operators (dndsr_eval_rhs_vjp, dndsr_vr_iter_vjp): mu by
+    the transposed VR fixed point, lam by the transposed pseudo-time iteration
+    lam += P^-T r, where P^-T is (1 + sgsIter) symmetric sweeps of the transposed scalar
+    LU-SGS matrix (dndsr_update_sgs_transpose) built at u with the local pseudo time step of
+    CFL.  The fixed point does not depend on P; P only sets the rate, which is the forward
+    solver's.  deltaLambdaCell is held at its value at u: flux forms whose dissipation reads it
+    (Roe_M6, Roe_M8, HLLEP, HLLEP_V1) are refused unless rsFixScale is 0."""
+
+    def __init__(self, ev, CFL=10.0, sgsIter=1, gsUpdate=True):
+        if ev.vfv.settings.get("subs2ndOrder"):
+            raise ValueError("the steady adjoint needs the VR fixed point; subs2ndOrder replaces it")
+        if ev.es.rsType in _READS_DELTA_LAMBDA and ev.es.rsFixScale != 0.0:
+            raise ValueError("the flux form reads deltaLambdaCell, which the steady adjoint holds fixed; "
+                             "use rsFixScale = 0 or a flux form that does not read it")
+        self.ev, self.CFL, self.sgsIter, self.gsUpdate = ev, float(CFL), int(sgsIter), bool(gsUpdate)
+        self.history = []
+
+    def _prepare(self, u):
+        ev = self.ev
+        dTau = torch.empty(ev.vfv.nCell, dtype=torch.float64, device=u.device)
+        ev.EvaluateDt(dTau, u, None, CFL=self.CFL)
+        self.dlc = ev.deltaLambdaCell.clone()
+        jd = torch.empty_like(dTau)
+        check(lib().dndsr_lusgs_init(ev.ctx.h, _ptr(dTau), 1e100, 1.0, _ptr(ev.lambdaFace), _ptr(jd)))
+        return jd
+
+    def _precond(self, r, u, jd):
+        ev = self.ev
+        a, b = torch.zeros_like(r), torch.zeros_like(r)
+        for _ in range(self.sgsIter + 1):
+            for forward in (True, False):
+                ev.UpdateSGSTranspose(1.0, r, u, a, b, jd, forward, self.gsUpdate)
+                a, b = b, a
+        return a
+
+    def solve(self, u, uRec, uBarJ, recBarJ=None, tol=1e-11, maxIter=2000, lam0=None, mu0=None):
+        """Multipliers (lam, mu) for the functional gradients uBarJ = dJ/du ([5, nCell]) and
+        recBarJ = dJ/duRec (uRec's shape, or None).  Stops when the u-stationarity residual
+        falls below tol times its first value and the last change of mu below tol times its
+        norm; history keeps the residual norm per iteration.  Returns (lam, mu, nIter)."""
+        u, uRec = u.contiguous(), uRec.contiguous()
+        jd = self._prepare(u)
+        vfv = self.ev.vfv
+        lam = torch.zeros_like(u) if lam0 is None else lam0.clone()
+        mu = torch.zeros_like(uRec) if mu0 is None else mu0.clone()
+        self.history = []
+        r0 = None
+        for it in range(1, int(maxIter) + 1):
+            uB, rB = rhs_vjp(self.ev, u, uRec, self.dlc, lam)
+            if recBarJ is not None:
+                rB = rB + recBarJ
+            rS, uS = vr_vjp(vfv, mu, 0, -1.0, uRec, u)
+            muNew = rB + rS
+            dmu = float(torch.linalg.vector_norm(muNew - mu))
+            mu = muNew
+            r = uB + uS + uBarJ
+            rn = float(torch.linalg.vector_norm(r))
+            if not np.isfinite(rn + dmu):
+                raise FloatingPointError(f"steady adjoint diverged at iteration {it}")
+            self.history.append(rn)
+            r0 = rn if r0 is None else r0
+            if rn <= tol * r0 and dmu <= tol * float(torch.linalg.vector_norm(mu)):
+                return lam, mu, it
+            lam = lam + self._precond(r, u, jd)
+        return lam, mu, int(maxIter)
+
+    def lagrangian(self, u, uRec, lam, mu, J):
+        """J(u, uRec) + lam . R(u, uRec) + mu . S(uRec, u) with the evaluator's current boundary
+        values; J is a callable (ev, u, uRec, deltaLambdaCell) -> float.  Its derivative in a
+        boundary value at fixed (u, uRec, lam, mu) is dJ/dp of the steady state."""
+        R = _eval_rhs(self.ev, u, uRec, self.dlc)
+        S = _vr_sweep(self.ev.vfv, uRec, u, -1.0)
+        return float(J(self.ev, u, uRec, self.dlc)) + float((lam * R).sum()) + float((mu * S).sum())
+
+    def bc_value_gradient(self, u, uRec, lam, mu, J, zones, h=1e-6):
+        """dJ/d(bcValues[z, k]) for z in zones, k = 0..4, by central differences of the
+        Lagrangian in the boundary value at fixed (u, uRec, lam, mu); [len(zones), 5]."""
+        ev = self.ev
+        base = np.array(ev._bc[1], copy=True)
+        g = np.zeros((len(zones), 5))
+        try:
+            for iz, z in enumerate(zones):
+                for k in range(5):
+                    step = h * max(1.0, abs(base[z, k]))
+                    vals = []
+                    for sgn in (1.0, -1.0):
+                        p = base.copy()
+                        p[z, k] += sgn * step
+                        ev.SetBCValues(p)
+                        vals.append(self.lagrangian(u, uRec, lam, mu, J))
+                    g[iz, k] = (vals[0] - vals[1]) / (2 * step)
+        finally:
+            ev.SetBCValues(base)
+        return g

@@ -1424,6 +1424,97 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
             part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
 }
 
+// ---------------------------------------------------------------------------
+// The transpose of the scalar LU-SGS matrix, swept in the same multicolour order: the
+// preconditioner of the discrete-adjoint fixed point (no counterpart in the reference).
+// (P^T y)_i = D_i y_i + sum_j c_ji (A(u_i) n_ji - lambda_f I)^T y_j, c_ji = (alpha/2) A_f / V_j,
+// n_ji out of the neighbour j: per face slot the matrix A(u_i) n_ji - lambda_f I is built column
+// by column with fluxJacDu and applied transposed to the neighbour's y_j.
+// ---------------------------------------------------------------------------
+template <typename T>
+__global__ void __launch_bounds__(256) kSGSColourT(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ rhs,
+                                                   const T *__restrict__ u, const T *__restrict__ uInc,
+                                                   T *__restrict__ uIncNew, const T *__restrict__ jd,
+                                                   const T *__restrict__ lam, int forward, int gsUpdate,
+                                                   int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+{
+    __shared__ real sm[5][256];
+    const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    real loc[5] = {0, 0, 0, 0, 0};
+    if (t < end)
+    {
+        const size_t N = m.N, F = m.F;
+        const idx i = m.colorCells[t];
+        const T gamma = (T)gammaD;
+        const T *__restrict__ farea = SGSGeom<T>::area(m);
+        const T *__restrict__ fnorm = SGSGeom<T>::norm(m);
+        const T *__restrict__ vol = SGSGeom<T>::vol(m);
+        T buf[5], ui[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            buf[v] = rhs[v * N + i], ui[v] = u[v * N + i];
+        for (int k = 0; k < m.K; k++)
+        {
+            const idx f = m.c2f[(size_t)k * N + i];
+            const idx o = m.c2c[(size_t)k * N + i];
+            const uint8_t fl = m.c2flag[(size_t)k * N + i];
+            if (f < 0 || o < 0 || o == i)
+                continue;
+            const bool earlier = forward ? (fl & 2) != 0 : (fl & 2) == 0;
+            const bool useNew = gsUpdate && earlier;
+            if (uIncIsZero && !useNew)
+                continue;
+            const T *y = useNew ? uIncNew : uInc;
+            T yj[5], n[3];
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                yj[v] = y[v * N + o];
+            const T sgn = (fl & 1) ? T(1) : T(-1); // out of the neighbour
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                n[r] = fnorm[r * F + f] * sgn;
+            const T cji = (T(0.5) * alphaDiag) * farea[f] / vol[o];
+#pragma unroll
+            for (int c = 0; c < 5; c++)
+            {
+                T e[5] = {0, 0, 0, 0, 0}, col[5];
+                e[c] = 1;
+                fluxJacDu<T>(ui, n, e, lam[f], gamma, col);
+                T acc = 0;
+#pragma unroll
+                for (int r = 0; r < 5; r++)
+                    acc += col[r] * yj[r];
+                buf[c] -= cji * acc;
+            }
+        }
+        const T d = jd[i];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+        {
+            const T old = uIncNew[v * N + i];
+            const T nw = buf[v] / d;
+            uIncNew[v * N + i] = nw;
+            loc[v] = (real)fabs(nw - old);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = loc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
+}
+
 __global__ void __launch_bounds__(256) kReduceSum5(const real *part, int nPart, real *out)
 {
     __shared__ real sm[5][256];
@@ -1473,6 +1564,29 @@ static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, con
         KernelTimer kt(c, K_SGS);
         kSGSColour<T><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jd, lam, col, forward,
                                                gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+        DNDSR_HIP_CHECK(hipGetLastError());
+        off += nb;
+    }
+    kReduceSum5<<<1, 256, 0, c.stream>>>(c.red, (int)off, sumIncDev);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+
+void launchSGSSweepT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
+                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
+                       int uIncIsZero, real *sumIncDev)
+{
+    DevMesh &m = c.m;
+    size_t off = 0;
+    for (int s = 0; s < m.nColor; s++)
+    {
+        const int col = forward ? s : m.nColor - 1 - s;
+        const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
+        if (e <= b)
+            continue;
+        const int nb = gridFor(e - b, 256);
+        KernelTimer kt(c, K_ADJOINT);
+        kSGSColourT<real><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace,
+                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5);
         DNDSR_HIP_CHECK(hipGetLastError());
         off += nb;
     }

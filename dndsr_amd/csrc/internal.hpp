@@ -218,6 +218,10 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
                    real *rhs, const StageUpdate *stage = nullptr, real *faceFluxBuf = nullptr,
                    bool flagNonFinite = true);
 void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const real *lambdaFace, real *jdiag);
+// a sweep of the transposed scalar LU-SGS matrix (the adjoint preconditioner), FP64
+void launchSGSSweepT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
+                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
+                       int uIncIsZero, real *sumIncDev);
 void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
                       int uIncIsZero, real *sumIncDev);

@@ -96,6 +96,7 @@ class EulerEvaluator:
                                      float(gas["muGas"]), float(gas["prGas"]), float(gas["TRef"]),
                                      float(gas["CSutherland"]), int(bool(s["useRoeJacobian"])))
         bcTypes = s.get("bcTypes")
+        self._bc = None
         if bcTypes is not None:
             # boundary weights of the VR functional follow the boundary types, as the
             # reference's constructor sets them (EulerEvaluator.hpp:138-152)
@@ -191,6 +192,29 @@ class EulerEvaluator:
                                          _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(lam), int(forward),
                                          int(gsUpdate), int(uIncIsZero), s.ctypes.data_as(_lib.dp)))
         return s
+
+    def UpdateSGSTranspose(self, alphaDiag, rhs, u, uInc, uIncNew, JDiag, forward, gsUpdate, uIncIsZero=False):
+        """A sweep of the transpose of the scalar LU-SGS matrix that UpdateSGS sweeps, in the same
+        colour order (FP64, scalar JDiag only): the preconditioner of the steady adjoint."""
+        if isinstance(JDiag, JacobianDiagBlock):
+            raise ValueError("UpdateSGSTranspose: the transposed sweep takes the scalar JDiag")
+        s = np.zeros(5)
+        check(lib().dndsr_update_sgs_transpose(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u),
+                                               _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(self.lambdaFace),
+                                               int(forward), int(gsUpdate), int(uIncIsZero),
+                                               s.ctypes.data_as(_lib.dp)))
+        return s
+
+    def SetBCValues(self, bcValues):
+        """Replaces the per-zone boundary states (5 conservative values per zone) of the
+        evaluator's boundary set; the zone types stay."""
+        if self._bc is None:
+            raise ValueError("SetBCValues: the evaluator was built without bcTypes")
+        t = self._bc[0]
+        v = np.ascontiguousarray(bcValues, np.float64).reshape(len(t), 5)
+        self._bc = (t, v)
+        check(lib().dndsr_set_bc(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p), v.ctypes.data_as(_lib.dp),
+                                 float(self.settings["gamma"])))
 
     # deprecated LU-SGS pair (EulerEvaluator.hxx:376-529): Forward writes D^-1 (r - L x_new),
     # Backward adds D^-1 (-U x); with separate uInc they reduce to block Jacobi.

@@ -233,6 +233,14 @@ int dndsr_lusgs_init(dndsr_ctx *ctx, const double *dTau, double dt, double alpha
 int dndsr_update_sgs(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
                      const double *u, const double *uInc, double *uIncNew, const double *jdiag,
                      const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
+/* A sweep of the transpose of the scalar LU-SGS matrix that dndsr_update_sgs sweeps, in the same
+ * multicolour order: y_i = (rhs_i - sum_j c_ji (A(u_i) n_ji - lambda_f I)^T y_j) / D_i with
+ * c_ji = (alphaDiag/2) A_f / V_j.  No counterpart in the reference: the preconditioner of the
+ * discrete-adjoint fixed point (dndsr_amd.adjoint.SteadyAdjoint). */
+int dndsr_update_sgs_transpose(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag, const double *rhs,
+                               const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                               const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero,
+                               double *sumIncHost);
 /* FP32 sweep for the mixed-precision path: every array is float (same layouts). */
 int dndsr_update_sgs_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                          const float *u, const float *uInc, float *uIncNew, const float *jdiag,

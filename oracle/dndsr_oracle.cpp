@@ -3130,6 +3130,79 @@ int orc_update_sgs_block(const orc_vr *h, const orc_euler_settings *es, double a
     ORC_CATCH(-1)
 }
 
+// The transpose of the scalar LU-SGS matrix P of UpdateSGS (the matrix LUSGSMatrixVec applies),
+// swept in the same order: (P^T y)_i = D_i y_i + sum_j c_ji (A(u_i) n_ji - lambda_f I)^T y_j with
+// c_ji = (alpha/2) A_f / V_j and n_ji the normal out of j, so a sweep solves
+// y_i = (b_i - sum_j c_ji (A(u_i) n_ji - lambda_f)^T y_j) / D_i.  No counterpart in the reference:
+// the preconditioner of the discrete-adjoint fixed point, whose iteration it makes converge at the
+// rate of the primal LU-SGS iteration.  A(u_i) n_ji is assembled column by column from
+// fluxJacobianTimesDu.
+int orc_update_sgs_transpose(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                             const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                             const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero,
+                             const int64_t *scanOrder, double *sumInc)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    std::vector<idx_t> pos(N), ord(N);
+    for (idx_t s = 0; s < N; s++)
+    {
+        ord[s] = scanOrder ? scanOrder[s] : s;
+        pos[ord[s]] = s;
+    }
+    for (int k = 0; k < 5; k++)
+        sumInc[k] = 0;
+    for (idx_t sIt = 0; sIt < N; sIt++)
+    {
+        idx_t iScan = forward ? sIt : N - 1 - sIt;
+        idx_t i = ord[iScan];
+        real buf[5];
+        for (int k = 0; k < 5; k++)
+            buf[k] = rhs[i * 5 + k];
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            int atFace = v.cellIsBack(i, f) ? 0 : 1;
+            if (o < 0 || o == i)
+                continue;
+            bool gsUseNew = gsUpdate && (forward ? (pos[o] < iScan) : (pos[o] > iScan));
+            if (uIncIsZero && !gsUseNew)
+                continue;
+            const real *yj = gsUseNew ? uIncNew + o * 5 : uInc + o * 5;
+            real n[3]; // out of the neighbour j
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFace ? 1 : -1);
+            real B[5][5]; // B[r][c] = (A(u_i) n_ji - lambda I)[r][c]
+            for (int c = 0; c < 5; c++)
+            {
+                real e[5] = {0, 0, 0, 0, 0}, col[5];
+                e[c] = 1;
+                fluxJacobianTimesDu(u + i * 5, n, e, lambdaFace[4 * f], es->gamma, col);
+                for (int r = 0; r < 5; r++)
+                    B[r][c] = col[r];
+            }
+            const real cji = (0.5 * alphaDiag) * v.area[f] / v.vol[o];
+            for (int c = 0; c < 5; c++)
+            {
+                real acc = 0;
+                for (int r = 0; r < 5; r++)
+                    acc += B[r][c] * yj[r];
+                buf[c] -= cji * acc;
+            }
+        }
+        for (int k = 0; k < 5; k++)
+        {
+            real old = uIncNew[i * 5 + k];
+            real nw = buf[k] / jdiag[i];
+            uIncNew[i * 5 + k] = nw;
+            sumInc[k] += std::abs(nw - old);
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
 // LUSGSMatrixVec (EulerEvaluator.hxx:175-275), scalar Jacobian: the matrix whose
 // Gauss-Seidel sweeps UpdateSGS performs, Ax_i = D_i x_i - buf_i with
 // buf_i = -sum_f (alpha/2) A_f / V_i * fluxJacobian0(u_j) x_j (lambda term included).

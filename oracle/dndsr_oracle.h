@@ -172,6 +172,11 @@ int orc_update_sgs(const orc_vr *h, const orc_euler_settings *es, double alphaDi
                    const double *rhs, const double *u, const double *uInc, double *uIncNew,
                    const double *jdiag, const double *lambdaFace, int forward, int gsUpdate,
                    int uIncIsZero, const int64_t *scanOrder, double *sumInc);
+/* the transpose of the scalar LU-SGS matrix, swept as UpdateSGS (the adjoint preconditioner) */
+int orc_update_sgs_transpose(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                             const double *u, const double *uInc, double *uIncNew, const double *jdiag,
+                             const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero,
+                             const int64_t *scanOrder, double *sumInc);
 /* LUSGSMatrixVec (scalar Jacobian): Ax = D x + sum_f (alpha/2) A_f / V fluxJacobian0(u_j) x_j */
 int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
                      const double *x, const double *jdiag, const double *lambdaFace, double *Ax);

@@ -69,7 +69,7 @@ def lib():
         for name in ["orc_vr_info", "orc_get_cell_metrics", "orc_get_face_metrics", "orc_get_cell_quad",
                      "orc_get_vr_coeffs", "orc_cell_average", "orc_vr_iter", "orc_vr_sor", "orc_vr_2nd", "orc_vr_iter_diff",
                      "orc_eval_dt", "orc_eval_rhs", "orc_eval_rhs_lam", "orc_zone_force", "orc_lusgs_init", "orc_update_sgs",
-                     "orc_lusgs_init_block", "orc_update_sgs_block", "orc_ssprk3", "orc_implicit_euler_step", "orc_implicit_bdf_step", "orc_lusgs_matvec",
+                     "orc_lusgs_init_block", "orc_update_sgs_block", "orc_update_sgs_transpose", "orc_ssprk3", "orc_implicit_euler_step", "orc_implicit_bdf_step", "orc_lusgs_matvec",
                      "orc_rec_error_l1", "orc_riemann", "orc_set_bc", "orc_set_vr_dirichlet", "orc_bc_ghost",
                      "orc_cell_rhs_alpha", "orc_compress_inc", "orc_add_increment_pp", "orc_model_rhs"]:
             getattr(_lib, name).argtypes = None
@@ -314,6 +314,19 @@ class OracleVR:
                                   None if deltaLambdaCell is None else P(np.ascontiguousarray(deltaLambdaCell)),
                                   C.c_int(zone), P(f)))
         return f
+
+    def update_sgs_transpose(self, es, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lam, forward, gsUpdate, uIncIsZero,
+                             scanOrder=None):
+        """A sweep of the transposed scalar LU-SGS matrix (the adjoint preconditioner)."""
+        out = np.array(uIncNew, dtype=np.float64, copy=True)
+        s = np.zeros(5)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        _chk(lib().orc_update_sgs_transpose(self.h, C.byref(es), C.c_double(alphaDiag), P(np.ascontiguousarray(rhs)),
+                                            P(np.ascontiguousarray(u)), P(np.ascontiguousarray(uInc)), P(out),
+                                            P(np.ascontiguousarray(jdiag)), P(np.ascontiguousarray(lam)),
+                                            C.c_int(int(forward)), C.c_int(int(gsUpdate)), C.c_int(int(uIncIsZero)),
+                                            P(so), P(s)))
+        return out, s
 
     def lusgs_init_block(self, es, dTau, dt, alphaDiag, u, lam, lamRoe=None):
         """LUSGSMatrixInit with useRoeJacobian: (JDiag [N, 5, 5], its inverse [N, 5, 5])."""
