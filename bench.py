@@ -427,6 +427,114 @@ def stream_peaks(ctx, gib=1.0, reps=10):
     return {"read_GBs": round(read, 1), "copy_GBs": round(copy, 1), "bytes": n * 8, "passes": reps}
 
 
+def _converge(ev, u, uRec, absTol, chunk=1000, maxChunks=60, log=None, **settings):
+    """Steady iterations (infinite physical step) in chunks until the density residual is at
+    most absTol, or has stopped falling (rounding level) within 100 absTol; returns
+    (iterations, last residual norms, seconds)."""
+    import torch
+    t0, nTot, res, hist = time.perf_counter(), 0, None, []
+    for _ in range(maxChunks):
+        n, res, _, _ = ev.ImplicitEulerStep(u, uRec, 1e100, maxIter=chunk, minIter=chunk, tol=0.0, **settings)
+        nTot += n
+        hist.append(float(res[0]))
+        if log:
+            log(f"  {nTot} iterations, density residual {res[0]:.3e}")
+        if res[0] <= absTol or (len(hist) > 2 and hist[-1] > 0.7 * hist[-3] and hist[-1] < 100 * absTol):
+            break
+    torch.cuda.synchronize()
+    return nTot, res, time.perf_counter() - t0
+
+
+def adjoint_bump(nx=516, ny=258, Re=5e3, M=0.7, fd=True, h=1e-6, seed=20261203):
+    """The 2-D discrete-adjoint workload: Mach 0.7, Re 5e3 laminar flow over a lower-wall bump
+    of eight Hicks-Henne functions (coefficients uniform in [-0.01, 0.01], seeded) in the
+    channel [-1, 2] x [0, 1.5] of 1.5 nx ny mixed triangles and quadrilaterals (199692 cells
+    at 516 x 258); drag = x-force on the wall.  The steady state is converged with the
+    block Roe-Jacobian LU-SGS (CFL 1000, 5 extra symmetric sweeps) to a density residual of
+    its rounding level (about 6e-13 of the free-stream start); the adjoint is solved by
+    GMRES(300) on the transposed system, preconditioned by transposed block LU-SGS sweeps, to
+    1e-11; the shape gradient comes from the Lagrangian on moved meshes (step h).  With fd,
+    each of the 8 gradient entries is checked against central differences (step h) of the
+    drag of steady states converged on the moved meshes to the same absolute residual,
+    started from the unmoved steady state."""
+    import torch
+    from dndsr_amd import bc
+    from dndsr_amd.adjoint import SteadyAdjoint, zone_force, zone_force_vjp
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    from dndsr_amd.shape import HicksHenneBumpChannel, shape_gradient
+
+    def log(msg):
+        print(msg, file=sys.stderr, flush=True)
+
+    ch = HicksHenneBumpChannel(nx, ny)
+    alpha = np.random.default_rng(seed).uniform(-0.01, 0.01, 8)
+    vals, gas = ch.flow(M=M, Re=Re)
+    types = [bc.bc_code(t) for t in ch.BC_TYPES]
+    solver = dict(CFL=1000.0, sgsIter=5)
+    adjSolver = dict(CFL=1000.0, sgsIter=5, restart=300, maxIter=6000, tol=1e-11)
+
+    def build(a, roeJ=False):
+        g = VariationalReconstruction(ch.mesh(a), dict(maxOrder=1, intOrder=3, bcTypes=types)).construct()
+        return EulerEvaluator(g, dict(gamma=1.4, riemannSolverType="Roe", bcTypes=types, bcValues=vals,
+                                      viscous=True, idealGasProperty=gas, useRoeJacobian=roeJ))
+
+    def drag(ev_, u, rec, dlc=None):
+        return zone_force(ev_, u, rec, 0, dlc)[0]
+
+    out = {"workload": f"Hicks-Henne bump channel, {ch.base.nCell} mixed tri/quad cells, Mach {M}, Re {Re:g}, "
+                       "p1 VR, Roe, laminar", "nCell": ch.base.nCell, "alpha": [round(float(a), 6) for a in alpha]}
+    ev = build(alpha, roeJ=True)
+    g = ev.vfv
+    u = g.to_device(np.tile(vals[1], (g.nCell, 1)))
+    rec = g.BuildURec(5)
+    n0, res0, _, _ = ev.ImplicitEulerStep(u, rec, 1e100, maxIter=1, minIter=1, tol=0.0, **solver)
+    absTol = 1e-13 * res0[0]  # below the rounding level: the solves stop where the residual stops falling
+    log(f"[adjoint bump] forward solve, target density residual {absTol:.3e}")
+    n, res, t = _converge(ev, u, rec, absTol, log=log, **solver)
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, u, None, CFL=1.0)
+    J0 = drag(ev, u, rec)
+    out["forward"] = {"iterations": n + n0, "seconds": round(t, 2), "density_residual": float(res[0]),
+                      "density_residual_start": float(res0[0]), "drag": J0}
+    adj = SteadyAdjoint(ev, CFL=adjSolver["CFL"], sgsIter=adjSolver["sgsIter"], block=True)
+    uB, rB = zone_force_vjp(ev, u, rec, 0, [1.0, 0.0, 0.0])
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lam, mu, nIt = adj.solve(u, rec, uB, rB, tol=adjSolver["tol"], maxIter=adjSolver["maxIter"], method="gmres",
+                             restart=adjSolver["restart"])
+    torch.cuda.synchronize()
+    tAdj = time.perf_counter() - t0
+    log(f"[adjoint bump] adjoint: {nIt} GMRES iterations, {tAdj:.2f} s")
+    t0 = time.perf_counter()
+    gS = shape_gradient(adj, u, rec, lam, mu, drag, build, alpha, h=h)
+    tShape = time.perf_counter() - t0
+    out["adjoint"] = {"method": "GMRES(300), preconditioner 6 symmetric transposed Roe-Jacobian block LU-SGS sweeps "
+                                "(CFL 1000)", "iterations": nIt,
+                      "seconds": round(tAdj, 2), "residual_rel": adj.history[-1] / max(adj.history[0], 1e-300),
+                      "shape_gradient_seconds": round(tShape, 2), "gradient": [float(x) for x in gS]}
+    if fd:
+        gF = np.zeros(8)
+        tF = time.perf_counter()
+        for k in range(8):
+            Js = []
+            for sgn in (1.0, -1.0):
+                a = alpha.copy()
+                a[k] += sgn * h
+                evk = build(a, roeJ=True)
+                uk, rk = u.clone(), rec.clone()
+                nk, resk, tk = _converge(evk, uk, rk, absTol, **solver)
+                evk.EvaluateDt(dTau, uk, None, CFL=1.0)
+                Js.append(drag(evk, uk, rk))
+                log(f"[adjoint bump] fd {k} {sgn:+.0f}: {nk} iterations, residual {resk[0]:.3e}, {tk:.1f} s")
+                evk.ctx.close()
+            gF[k] = (Js[0] - Js[1]) / (2 * h)
+        err = np.abs(gS - gF) / np.abs(gF).max()
+        out["fd"] = {"step": h, "gradient": [float(x) for x in gF], "max_rel_error": float(err.max()),
+                     "rel_error": [float(x) for x in err], "seconds": round(time.perf_counter() - tF, 1)}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -438,6 +546,11 @@ def main():
     ap.add_argument("--tet-iterations", type=int, default=0, metavar="K",
                     help="only run K implicit iterations of the laminar tetrahedral case with FP64 and with FP32 "
                          "sweeps (a short run for kernel profiles)")
+    ap.add_argument("--adjoint-bump", action="store_true",
+                    help="only run the 2-D adjoint workload (Hicks-Henne bump drag gradient); with --adjoint-fd also "
+                         "the central-difference check of its 8 entries")
+    ap.add_argument("--adjoint-fd", action="store_true")
+    ap.add_argument("--adjoint-step", type=float, default=1e-6, help="parameter step of the shape derivatives")
     ap.add_argument("--copy-calibration", type=int, default=0, metavar="MIB",
                     help="only run a device-to-device copy of MIB MiB (reference for rocprofv3 FETCH/WRITE_SIZE)")
     args = ap.parse_args()
@@ -454,6 +567,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_bytes": n * 8, "launches": 3}))
+        return
+    if args.adjoint_bump:
+        print(json.dumps(adjoint_bump(fd=args.adjoint_fd, h=args.adjoint_step)), flush=True)
         return
     if args.tet_iterations:
         r = implicit_laminar_tet(fixed_iter=args.tet_iterations)

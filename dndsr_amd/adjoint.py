@@ -175,28 +175,41 @@ class SteadyAdjoint:
         0   = dJ/du    + (dR/du)^T lam    + (dS/du)^T mu,
 
     and dJ/dp = dJ/dp|explicit + lam . dR/dp + mu . dS/dp at fixed (u, uRec).  ``solve``
-    iterates both with the transposed operators (dndsr_eval_rhs_vjp, dndsr_vr_iter_vjp): mu by
-    the transposed VR fixed point, lam by the transposed pseudo-time iteration
-    lam += P^-T r, where P^-T is (1 + sgsIter) symmetric sweeps of the transposed scalar
-    LU-SGS matrix (dndsr_update_sgs_transpose) built at u with the local pseudo time step of
-    CFL.  The fixed point does not depend on P; P only sets the rate, which is the forward
-    solver's.  deltaLambdaCell is held at its value at u: flux forms whose dissipation reads it
+    solves them with the transposed operators (dndsr_eval_rhs_vjp, dndsr_vr_iter_vjp), either
+    (method "fixed-point") by the transposed VR fixed point for mu and the transposed
+    pseudo-time iteration lam += P^-T r, whose rate is the forward solver's, or (method
+    "gmres", the default) by restarted right-preconditioned GMRES on the linear system in
+    (lam, mu) with -P^-T on the lam block and -I on the mu block.  P^-T is (1 + sgsIter)
+    symmetric sweeps of the transposed LU-SGS matrix built at u with the local pseudo time
+    step of CFL: the scalar one (dndsr_update_sgs_transpose), or with block (the default
+    when the evaluator has useRoeJacobian) the Roe-Jacobian block one
+    (dndsr_update_sgs_block_transpose); the solution does not depend on P.  deltaLambdaCell is held at its value at u: flux forms whose dissipation reads it
     (Roe_M6, Roe_M8, HLLEP, HLLEP_V1) are refused unless rsFixScale is 0."""
 
-    def __init__(self, ev, CFL=10.0, sgsIter=1, gsUpdate=True):
+    def __init__(self, ev, CFL=10.0, sgsIter=1, gsUpdate=True, block=None):
+        if block is None:
+            block = bool(ev.settings["useRoeJacobian"])
+        if block and not ev.settings["useRoeJacobian"]:
+            raise ValueError("the block preconditioner needs an evaluator with useRoeJacobian")
         if ev.vfv.settings.get("subs2ndOrder"):
             raise ValueError("the steady adjoint needs the VR fixed point; subs2ndOrder replaces it")
         if ev.es.rsType in _READS_DELTA_LAMBDA and ev.es.rsFixScale != 0.0:
             raise ValueError("the flux form reads deltaLambdaCell, which the steady adjoint holds fixed; "
                              "use rsFixScale = 0 or a flux form that does not read it")
         self.ev, self.CFL, self.sgsIter, self.gsUpdate = ev, float(CFL), int(sgsIter), bool(gsUpdate)
+        self.block = bool(block)
         self.history = []
 
-    def _prepare(self, u):
+    def _prepare(self, u, uRec):
         ev = self.ev
         dTau = torch.empty(ev.vfv.nCell, dtype=torch.float64, device=u.device)
         ev.EvaluateDt(dTau, u, None, CFL=self.CFL)
         self.dlc = ev.deltaLambdaCell.clone()
+        if self.block:  # the Roe-Jacobian blocks need the face eigenvalues of EvaluateRHS
+            ev.EvaluateRHS(torch.empty_like(u), None, u, uRec)
+            jd = ev.BuildJacobianDiagBlock()
+            ev.LUSGSMatrixInit(jd, None, dTau, 1e100, 1.0, u, uRec)
+            return jd
         jd = torch.empty_like(dTau)
         check(lib().dndsr_lusgs_init(ev.ctx.h, _ptr(dTau), 1e100, 1.0, _ptr(ev.lambdaFace), _ptr(jd)))
         return jd
@@ -210,13 +223,29 @@ class SteadyAdjoint:
                 a, b = b, a
         return a
 
-    def solve(self, u, uRec, uBarJ, recBarJ=None, tol=1e-11, maxIter=2000, lam0=None, mu0=None):
+    def _apply(self, u, uRec, lam, mu):
+        """The transposed steady system: ((dR/du)^T lam + (dS/du)^T mu,
+        (dR/duRec)^T lam + (dS/duRec)^T mu - mu)."""
+        uB, rB = rhs_vjp(self.ev, u, uRec, self.dlc, lam)
+        rS, uS = vr_vjp(self.ev.vfv, mu, 0, -1.0, uRec, u)
+        return uB + uS, rB + rS - mu
+
+    def solve(self, u, uRec, uBarJ, recBarJ=None, tol=1e-11, maxIter=2000, lam0=None, mu0=None, method="gmres",
+              restart=40):
         """Multipliers (lam, mu) for the functional gradients uBarJ = dJ/du ([5, nCell]) and
-        recBarJ = dJ/duRec (uRec's shape, or None).  Stops when the u-stationarity residual
-        falls below tol times its first value and the last change of mu below tol times its
-        norm; history keeps the residual norm per iteration.  Returns (lam, mu, nIter)."""
+        recBarJ = dJ/duRec (uRec's shape, or None).  GMRES stops when the residual of the
+        transposed system falls below tol times the norm of (uBarJ, recBarJ); the fixed-point
+        iteration when the u-stationarity residual falls below tol times its first value and
+        the last change of mu below tol times its norm.  history keeps the residual norm per
+        iteration (for GMRES, its estimate).  Returns (lam, mu, nIter)."""
         u, uRec = u.contiguous(), uRec.contiguous()
-        jd = self._prepare(u)
+        jd = self._prepare(u, uRec)
+        if recBarJ is None:
+            recBarJ = torch.zeros_like(uRec)
+        if method == "gmres":
+            return self._gmres(u, uRec, uBarJ, recBarJ, jd, tol, int(maxIter), int(restart), lam0, mu0)
+        if method != "fixed-point":
+            raise ValueError("method is 'gmres' or 'fixed-point'")
         vfv = self.ev.vfv
         lam = torch.zeros_like(u) if lam0 is None else lam0.clone()
         mu = torch.zeros_like(uRec) if mu0 is None else mu0.clone()
@@ -224,8 +253,7 @@ class SteadyAdjoint:
         r0 = None
         for it in range(1, int(maxIter) + 1):
             uB, rB = rhs_vjp(self.ev, u, uRec, self.dlc, lam)
-            if recBarJ is not None:
-                rB = rB + recBarJ
+            rB = rB + recBarJ
             rS, uS = vr_vjp(vfv, mu, 0, -1.0, uRec, u)
             muNew = rB + rS
             dmu = float(torch.linalg.vector_norm(muNew - mu))
@@ -241,13 +269,87 @@ class SteadyAdjoint:
             lam = lam + self._precond(r, u, jd)
         return lam, mu, int(maxIter)
 
-    def lagrangian(self, u, uRec, lam, mu, J):
-        """J(u, uRec) + lam . R(u, uRec) + mu . S(uRec, u) with the evaluator's current boundary
-        values; J is a callable (ev, u, uRec, deltaLambdaCell) -> float.  Its derivative in a
-        boundary value at fixed (u, uRec, lam, mu) is dJ/dp of the steady state."""
-        R = _eval_rhs(self.ev, u, uRec, self.dlc)
-        S = _vr_sweep(self.ev.vfv, uRec, u, -1.0)
-        return float(J(self.ev, u, uRec, self.dlc)) + float((lam * R).sum()) + float((mu * S).sum())
+    def _gmres(self, u, uRec, bu, brec, jd, tol, maxIter, m, lam0, mu0):
+        """Restarted right-preconditioned GMRES(m) on A^T x = -b; the Krylov basis lives in two
+        [m + 1, n] device arrays and is orthogonalised by classical Gram-Schmidt applied twice
+        (two matrix-vector products per step)."""
+        nU = u.numel()
+
+        def pack(a, b):
+            return torch.cat([a.reshape(-1), b.reshape(-1)])
+
+        def unpack(x):
+            return x[:nU].view_as(u), x[nU:].view_as(uRec)
+
+        def A(x):
+            a, b = self._apply(u, uRec, *unpack(x))
+            return pack(a, b)
+
+        def Minv(y):
+            ru, rr = unpack(y)
+            return pack(-self._precond(ru.contiguous(), u, jd), -rr)
+
+        b = -pack(bu, brec)
+        bnorm = float(torch.linalg.vector_norm(b))
+        x = torch.zeros_like(b) if lam0 is None else pack(lam0, mu0 if mu0 is not None else torch.zeros_like(uRec))
+        self.history = []
+        it = 0
+        if bnorm == 0.0:
+            return (*[t.clone() for t in unpack(x)], 0)
+        V = torch.empty((m + 1, b.numel()), dtype=b.dtype, device=b.device)
+        Z = torch.empty((m, b.numel()), dtype=b.dtype, device=b.device)
+        while it < maxIter:
+            r = b - A(x)
+            beta = float(torch.linalg.vector_norm(r))
+            self.history.append(beta)
+            if not np.isfinite(beta):
+                raise FloatingPointError("steady adjoint GMRES diverged")
+            if beta <= tol * bnorm:
+                break
+            V[0] = r / beta
+            H = np.zeros((m + 1, m))
+            cs, sn = np.zeros(m), np.zeros(m)
+            g = np.zeros(m + 1)
+            g[0] = beta
+            k = 0
+            for j in range(m):
+                Z[j] = Minv(V[j])
+                w = A(Z[j])
+                h1 = V[: j + 1] @ w
+                w = w - V[: j + 1].T @ h1
+                h2 = V[: j + 1] @ w
+                w = w - V[: j + 1].T @ h2
+                H[: j + 1, j] = (h1 + h2).cpu().numpy()
+                H[j + 1, j] = float(torch.linalg.vector_norm(w))
+                for i in range(j):  # earlier Givens rotations
+                    t = cs[i] * H[i, j] + sn[i] * H[i + 1, j]
+                    H[i + 1, j] = -sn[i] * H[i, j] + cs[i] * H[i + 1, j]
+                    H[i, j] = t
+                d = np.hypot(H[j, j], H[j + 1, j])
+                cs[j], sn[j] = (1.0, 0.0) if d == 0 else (H[j, j] / d, H[j + 1, j] / d)
+                hj1 = H[j + 1, j]
+                H[j, j], H[j + 1, j] = cs[j] * H[j, j] + sn[j] * hj1, 0.0
+                g[j + 1] = -sn[j] * g[j]
+                g[j] = cs[j] * g[j]
+                k, it = j + 1, it + 1
+                self.history.append(abs(g[j + 1]))
+                if abs(g[j + 1]) <= tol * bnorm or it >= maxIter or hj1 == 0.0:
+                    break
+                V[j + 1] = w / hj1
+            y = np.linalg.solve(np.triu(H[:k, :k]), g[:k])
+            x = x + torch.as_tensor(y, dtype=b.dtype, device=b.device) @ Z[:k]
+        lam, mu = unpack(x)
+        return lam.clone(), mu.clone(), it
+
+    def lagrangian(self, u, uRec, lam, mu, J, ev=None):
+        """J(u, uRec) + lam . R(u, uRec) + mu . S(uRec, u) on the evaluator ev (default: the
+        adjoint's, with its current boundary values); J is a callable
+        (ev, u, uRec, deltaLambdaCell) -> float.  Its derivative in a boundary value or a mesh
+        parameter at fixed (u, uRec, lam, mu) is dJ/dp of the steady state."""
+        ev = self.ev if ev is None else ev
+        R = _eval_rhs(ev, u, uRec, self.dlc)
+        S = _vr_sweep(ev.vfv, uRec, u, -1.0)
+        return float(J(ev, u, uRec, self.dlc)) + float((lam * R).sum()) + float((mu * S).sum())
 
     def bc_value_gradient(self, u, uRec, lam, mu, J, zones, h=1e-6):
         """dJ/d(bcValues[z, k]) for z in zones, k = 0..4, by central differences of the

@@ -1005,6 +1005,23 @@ int dndsr_update_sgs_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, doubl
     CAPI_END
 }
 
+int dndsr_update_sgs_block_transpose(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag,
+                                     const double *rhs, const double *u, const double *uInc, double *uIncNew,
+                                     const double *jdiagInv, const double *lambdaFace, const double *lamRoe,
+                                     int forward, int gsUpdate, int uIncIsZero, double *sumIncHost)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && uInc && uIncNew && jdiagInv && lambdaFace, "null field");
+    need(uInc != uIncNew, "uInc and uIncNew must be different arrays");
+    Ctx &c = ctx->c;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(c.m.N, 256) + c.m.nColor + 6);
+    launchSGSSweepBlockT64(c, toGas(es), alphaDiag, rhs, u, uInc, uIncNew, jdiagInv, lambdaFace, lamRoe, forward,
+                           gsUpdate, uIncIsZero, sumDev);
+    fetchSum(c, sumIncHost);
+    CAPI_END
+}
+
 int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                                const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
                                const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,

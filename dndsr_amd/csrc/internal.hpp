@@ -237,6 +237,9 @@ void launchLUSGSInitBlock(Ctx &c, const GasParams &gp, const real *dTau, real dt
 void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
                            const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
+void launchSGSSweepBlockT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
+                            const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
+                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
 void launchSGSSweepBlock32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
                            const float *uInc, float *uIncNew, const float *jinv, const float *lam,
                            const float *lamRoe, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);

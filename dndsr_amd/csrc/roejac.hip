@@ -561,6 +561,119 @@ static void sgsSweepBlock(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs
     launchReduceSum5(c, (int)off, sumIncDev);
 }
 
+// A sweep of the transpose of the block matrix kSGSColourBlock sweeps: for cell i,
+// y_i = JDiag_i^-T (rhs_i - sum_j O_ji^T y_j) with O_ji = (alpha/2) A_f / V_j
+// (A(u_i) n_ji - |A|_Roe(u_i, u_j; n_ji) - lambdaVis I) and the face eigenvalues seen from j,
+// the 5x5 block assembled column by column from the same increment routines.
+__global__ void __launch_bounds__(256) kSGSColourBlockT(DevMesh m, real gamma, real alphaDiag,
+                                                        const real *__restrict__ rhs, const real *__restrict__ u,
+                                                        const real *__restrict__ uInc, real *__restrict__ uIncNew,
+                                                        const real *__restrict__ jinv, const real *__restrict__ lam,
+                                                        const real *__restrict__ lamRoe, int forward, int gsUpdate,
+                                                        int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+{
+    __shared__ real sm[5][256];
+    const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    real loc[5] = {0, 0, 0, 0, 0};
+    if (t < end)
+    {
+        const size_t N = m.N, F = m.F;
+        const idx i = m.colorCells[t];
+        real buf[5], ui[5];
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            buf[v] = rhs[v * N + i], ui[v] = u[v * N + i];
+        for (int k = 0; k < m.K; k++)
+        {
+            const idx f = m.c2f[(size_t)k * N + i];
+            const idx o = m.c2c[(size_t)k * N + i];
+            const uint8_t fl = m.c2flag[(size_t)k * N + i];
+            if (f < 0 || o < 0 || o == i)
+                continue;
+            const bool earlier = forward ? (fl & 2) != 0 : (fl & 2) == 0;
+            const bool useNew = gsUpdate && earlier;
+            if (uIncIsZero && !useNew)
+                continue;
+            const real *y = useNew ? uIncNew : uInc;
+            const bool frontJ = (fl & 1) == 0; // the neighbour is the front cell when i is the back
+            real U[5], yj[5], n[3], l0, l123, l4;
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                U[v] = u[v * N + o], yj[v] = y[v * N + o];
+#pragma unroll
+            for (int r = 0; r < 3; r++)
+                n[r] = m.fmeanN[r * F + f] * (frontJ ? real(-1) : real(1));
+            faceLambdas(lam, lamRoe, F, f, frontJ, l0, l123, l4);
+            RoeAveT<real> ra;
+            roeAverage(ui, U, gamma, ra);
+            const real cji = (real(0.5) * alphaDiag) * m.farea[f] / m.vol[o];
+#pragma unroll
+            for (int c = 0; c < 5; c++)
+            {
+                real e[5] = {0, 0, 0, 0, 0}, dF[5], inc[5];
+                e[c] = 1;
+                fluxJacDu<real>(ui, n, e, lam[2 * F + f], gamma, dF);
+                roeIncT<real>(e, n, ra, l0, l123, l4, gamma, inc);
+                real acc = 0;
+#pragma unroll
+                for (int r = 0; r < 5; r++)
+                    acc += (dF[r] - inc[r]) * yj[r];
+                buf[c] -= cji * acc;
+            }
+        }
+#pragma unroll
+        for (int r = 0; r < 5; r++)
+        {
+            real nw = 0;
+#pragma unroll
+            for (int q = 0; q < 5; q++)
+                nw += jinv[(size_t)(q * 5 + r) * N + i] * buf[q];
+            const real old = uIncNew[r * N + i];
+            uIncNew[r * N + i] = nw;
+            loc[r] = fabs(nw - old);
+        }
+    }
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        sm[v][threadIdx.x] = loc[v];
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1)
+    {
+        if (threadIdx.x < s)
+#pragma unroll
+            for (int v = 0; v < 5; v++)
+                sm[v][threadIdx.x] += sm[v][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0)
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            part[(size_t)blockIdx.x * 5 + v] = sm[v][0];
+}
+
+void launchSGSSweepBlockT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
+                            const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
+                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev)
+{
+    DevMesh &m = c.m;
+    const int nc = m.nColor;
+    size_t off = 0;
+    for (int s = 0; s < nc; s++)
+    {
+        const int col = forward ? s : nc - 1 - s;
+        const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
+        if (e <= b)
+            continue;
+        const int nb = gridFor(e - b, 256);
+        KernelTimer kt(c, K_ADJOINT);
+        kSGSColourBlockT<<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jinv, lam, lamRoe,
+                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+        DNDSR_HIP_CHECK(hipGetLastError());
+        off += nb;
+    }
+    launchReduceSum5(c, (int)off, sumIncDev);
+}
+
 void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
                            const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev)

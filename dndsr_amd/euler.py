@@ -194,11 +194,16 @@ class EulerEvaluator:
         return s
 
     def UpdateSGSTranspose(self, alphaDiag, rhs, u, uInc, uIncNew, JDiag, forward, gsUpdate, uIncIsZero=False):
-        """A sweep of the transpose of the scalar LU-SGS matrix that UpdateSGS sweeps, in the same
-        colour order (FP64, scalar JDiag only): the preconditioner of the steady adjoint."""
-        if isinstance(JDiag, JacobianDiagBlock):
-            raise ValueError("UpdateSGSTranspose: the transposed sweep takes the scalar JDiag")
+        """A sweep of the transpose of the LU-SGS matrix that UpdateSGS sweeps with the same JDiag
+        (scalar, or a JacobianDiagBlock with the face eigenvalues of the last EvaluateRHS), in the
+        same colour order, FP64: the preconditioner of the steady adjoint."""
         s = np.zeros(5)
+        if isinstance(JDiag, JacobianDiagBlock):
+            check(lib().dndsr_update_sgs_block_transpose(
+                self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u), _ptr(uInc), _ptr(uIncNew),
+                _ptr(JDiag.invert), _ptr(self.lambdaFace), _ptr(self.lambdaFaceRoe), int(forward), int(gsUpdate),
+                int(uIncIsZero), s.ctypes.data_as(_lib.dp)))
+            return s
         check(lib().dndsr_update_sgs_transpose(self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u),
                                                _ptr(uInc), _ptr(uIncNew), _ptr(JDiag), _ptr(self.lambdaFace),
                                                int(forward), int(gsUpdate), int(uIncIsZero),

@@ -268,6 +268,15 @@ int dndsr_update_sgs_block(dndsr_ctx *ctx, const dndsr_euler_settings *es, doubl
                            const double *u, const double *uInc, double *uIncNew, const double *jdiagInv,
                            const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate, int uIncIsZero,
                            double *sumIncHost);
+/* A sweep of the transpose of the block matrix dndsr_update_sgs_block sweeps, in the same
+ * multicolour order: y_i = JDiag_i^-T (rhs_i - sum_j O_ji^T y_j) with
+ * O_ji = (alphaDiag/2) A_f / V_j (A(u_i) n_ji - |A|_Roe - lambdaVis I) and the face eigenvalues
+ * seen from j.  No counterpart in the reference: the block preconditioner of the discrete-adjoint
+ * solve (dndsr_amd.adjoint.SteadyAdjoint). */
+int dndsr_update_sgs_block_transpose(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag,
+                                     const double *rhs, const double *u, const double *uInc, double *uIncNew,
+                                     const double *jdiagInv, const double *lambdaFace, const double *lamRoe,
+                                     int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
 int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                                const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
                                const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,

@@ -3130,6 +3130,85 @@ int orc_update_sgs_block(const orc_vr *h, const orc_euler_settings *es, double a
     ORC_CATCH(-1)
 }
 
+// The transpose of the block LU-SGS matrix of orc_update_sgs_block (useRoeJacobian), swept in the
+// same order.  Row i of that matrix holds JDiag_i and, per neighbour j, the block
+// O_ij = (alpha/2) A_f / V_i (A(u_j) n_ij - |A|_Roe(u_j, u_i; n_ij) - lambdaVis I) with the face
+// eigenvalues seen from i; the transposed sweep solves
+// y_i = JDiag_i^-T (b_i - sum_j O_ji^T y_j), O_ji assembled column by column from
+// fluxJacobianRoeTimesDu with the roles of i and j exchanged.  No counterpart in the reference:
+// the block preconditioner of the discrete adjoint.
+int orc_update_sgs_block_transpose(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                                   const double *u, const double *uInc, double *uIncNew, const double *jinv,
+                                   const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate,
+                                   int uIncIsZero, const int64_t *scanOrder, double *sumInc)
+{
+    ORC_TRY
+    const VR &v = h->v;
+    idx_t N = v.nCell;
+    std::vector<idx_t> pos(N), ord(N);
+    for (idx_t s = 0; s < N; s++)
+    {
+        ord[s] = scanOrder ? scanOrder[s] : s;
+        pos[ord[s]] = s;
+    }
+    for (int k = 0; k < 5; k++)
+        sumInc[k] = 0;
+    for (idx_t sIt = 0; sIt < N; sIt++)
+    {
+        idx_t iScan = forward ? sIt : N - 1 - sIt;
+        idx_t i = ord[iScan];
+        real buf[5];
+        for (int k = 0; k < 5; k++)
+            buf[k] = rhs[i * 5 + k];
+        for (auto f : v.c2f[i])
+        {
+            idx_t o = v.cellFaceOther(i, f);
+            if (o < 0 || o == i)
+                continue;
+            bool gsUseNew = gsUpdate && (forward ? (pos[o] < iScan) : (pos[o] > iScan));
+            if (uIncIsZero && !gsUseNew)
+                continue;
+            const real *yj = gsUseNew ? uIncNew + o * 5 : uInc + o * 5;
+            int atFaceJ = v.cellIsBack(i, f) ? 1 : 0; // the neighbour is the front cell when i is the back
+            real n[3];                                  // out of the neighbour
+            for (int d = 0; d < 3; d++)
+                n[d] = v.faceMeanNorm[f].x[d] * (atFaceJ ? -1 : 1);
+            real l0, l123, l4;
+            roeLambdas(lambdaFace, lamRoe, f, atFaceJ, l0, l123, l4);
+            real B[5][5]; // O_ji / coef, B[r][c]
+            for (int c = 0; c < 5; c++)
+            {
+                real e[5] = {0, 0, 0, 0, 0}, col[5];
+                e[c] = 1;
+                fluxJacobianRoeTimesDu(u + i * 5, u + o * 5, n, e, lambdaFace[(size_t)f * 4 + 2], l0, l123, l4,
+                                       es->gamma, -1, false, col);
+                for (int r = 0; r < 5; r++)
+                    B[r][c] = col[r];
+            }
+            const real cji = (0.5 * alphaDiag) * v.area[f] / v.vol[o];
+            for (int c = 0; c < 5; c++)
+            {
+                real acc = 0;
+                for (int r = 0; r < 5; r++)
+                    acc += B[r][c] * yj[r];
+                buf[c] -= cji * acc;
+            }
+        }
+        const real *Ji = jinv + (size_t)i * 25;
+        for (int k = 0; k < 5; k++)
+        {
+            real nw = 0;
+            for (int c = 0; c < 5; c++)
+                nw += Ji[c * 5 + k] * buf[c];
+            real old = uIncNew[i * 5 + k];
+            uIncNew[i * 5 + k] = nw;
+            sumInc[k] += std::abs(nw - old);
+        }
+    }
+    return 0;
+    ORC_CATCH(-1)
+}
+
 // The transpose of the scalar LU-SGS matrix P of UpdateSGS (the matrix LUSGSMatrixVec applies),
 // swept in the same order: (P^T y)_i = D_i y_i + sum_j c_ji (A(u_i) n_ji - lambda_f I)^T y_j with
 // c_ji = (alpha/2) A_f / V_j and n_ji the normal out of j, so a sweep solves

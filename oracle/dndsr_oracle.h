@@ -177,6 +177,11 @@ int orc_update_sgs_transpose(const orc_vr *h, const orc_euler_settings *es, doub
                              const double *u, const double *uInc, double *uIncNew, const double *jdiag,
                              const double *lambdaFace, int forward, int gsUpdate, int uIncIsZero,
                              const int64_t *scanOrder, double *sumInc);
+/* the transpose of the block LU-SGS matrix of orc_update_sgs_block, swept in the same order */
+int orc_update_sgs_block_transpose(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *rhs,
+                                   const double *u, const double *uInc, double *uIncNew, const double *jinv,
+                                   const double *lambdaFace, const double *lamRoe, int forward, int gsUpdate,
+                                   int uIncIsZero, const int64_t *scanOrder, double *sumInc);
 /* LUSGSMatrixVec (scalar Jacobian): Ax = D x + sum_f (alpha/2) A_f / V fluxJacobian0(u_j) x_j */
 int orc_lusgs_matvec(const orc_vr *h, const orc_euler_settings *es, double alphaDiag, const double *u,
                      const double *x, const double *jdiag, const double *lambdaFace, double *Ax);

@@ -328,6 +328,19 @@ class OracleVR:
                                             P(so), P(s)))
         return out, s
 
+    def update_sgs_block_transpose(self, es, alphaDiag, rhs, u, uInc, uIncNew, jinv, lam, lamRoe, forward,
+                                   gsUpdate, uIncIsZero, scanOrder=None):
+        """A sweep of the transposed block LU-SGS matrix (the block adjoint preconditioner)."""
+        out = np.array(uIncNew, dtype=np.float64, copy=True)
+        s = np.zeros(5)
+        so = None if scanOrder is None else np.ascontiguousarray(scanOrder, np.int64)
+        _chk(lib().orc_update_sgs_block_transpose(
+            self.h, C.byref(es), C.c_double(alphaDiag), P(np.ascontiguousarray(rhs)), P(np.ascontiguousarray(u)),
+            P(np.ascontiguousarray(uInc)), P(out), P(np.ascontiguousarray(jinv)), P(np.ascontiguousarray(lam)),
+            None if lamRoe is None else P(np.ascontiguousarray(lamRoe)), C.c_int(int(forward)),
+            C.c_int(int(gsUpdate)), C.c_int(int(uIncIsZero)), P(so), P(s)))
+        return out, s
+
     def lusgs_init_block(self, es, dTau, dt, alphaDiag, u, lam, lamRoe=None):
         """LUSGSMatrixInit with useRoeJacobian: (JDiag [N, 5, 5], its inverse [N, 5, 5])."""
         jd, ji = np.zeros((self.nCell, 5, 5)), np.zeros((self.nCell, 5, 5))

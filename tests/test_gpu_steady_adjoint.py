@@ -71,6 +71,62 @@ def test_transposed_sweep_matches_oracle(require_gpu, kind, gsUpdate):
             np.testing.assert_allclose(s, s_o, rtol=1e-11)
 
 
+@pytest.mark.parametrize("kind", ["periodic", "bounded"])
+@pytest.mark.parametrize("gsUpdate", [0, 1])
+def test_transposed_block_sweep_matches_oracle(require_gpu, kind, gsUpdate):
+    """The transposed Roe-Jacobian block sweep (dndsr_update_sgs_block_transpose) against the
+    oracle's in the device's colour order, viscous, with the face eigenvalues of EvaluateRHS."""
+    import pyoracle as po
+    import torch
+    from dndsr_amd import bc
+    from dndsr_amd import mesh as mm
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    gas = dict(Rgas=1.0, muGas=0.05, prGas=0.72, muModel=0)
+    if kind == "periodic":
+        m = mm.periodic_hex_grid(5, 4, 4, 1.0, 0.8, 0.8, jitter=0.1, seed=3)
+        types = vals = None
+        o = po.OracleVR(m, maxOrder=1, intOrder=3)
+        vs = dict(maxOrder=1, intOrder=3)
+    else:
+        types = [bc.BC_WALL, bc.BC_FAR, bc.BC_SYM]
+        m = mm.box_quad_grid(7, 6, 7.0, 6.0, zones=dict(xmin=1, xmax=1, ymin=0, ymax=2), jitter=0.15, seed=2)
+        vals = np.tile(random_state(1, seed=9)[0], (3, 1))
+        o = po.OracleVR(m, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+        o.set_bc(types, vals, GAMMA)
+        vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
+    g = VariationalReconstruction(m, vs).construct()
+    st = dict(gamma=GAMMA, riemannSolverType="Roe", viscous=True, idealGasProperty=gas, useRoeJacobian=True)
+    if types is not None:
+        st.update(bcTypes=types, bcValues=vals)
+    ev = EulerEvaluator(g, st)
+    es = po.OracleVR.euler_settings(GAMMA, RS["Roe"], viscous=1, useRoeJacobian=1, **gas)
+    N = o.nCell
+    u = random_state(N, seed=4)
+    rec = 0.05 * np.random.default_rng(5).normal(size=(N, 5, o.M))
+    du, dr = g.to_device(u), g.to_device(rec)
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, du, None, CFL=2.0)
+    ev.EvaluateRHS(g.BuildUDof(5), None, du, dr)
+    jd = ev.BuildJacobianDiagBlock()
+    ev.LUSGSMatrixInit(jd, None, dTau, 0.7, 1.0, du, dr)
+    dTau_o, lam_o, dlc, _ = o.eval_dt(es, u, 2.0, 1e100, True)
+    _, lr_o = o.eval_rhs(es, u, rec, dlc, roeLambdas=True)
+    _, ji_o = o.lusgs_init_block(es, dTau_o, 0.7, 1.0, u, lam_o, lr_o)
+    rng = np.random.default_rng(8)
+    rhs, y0 = rng.normal(size=(N, 5)), rng.normal(size=(N, 5))
+    order = colour_scan_order(g.colors())
+    for forward in (True, False):
+        for zero in (False, True):
+            out = torch.zeros_like(du)
+            s = ev.UpdateSGSTranspose(1.0, g.to_device(rhs), du, g.to_device(y0), out, jd, forward, gsUpdate,
+                                      uIncIsZero=zero)
+            ref, s_o = o.update_sgs_block_transpose(es, 1.0, rhs, u, y0, np.zeros((N, 5)), ji_o, lam_o, lr_o,
+                                                    forward, gsUpdate, zero, scanOrder=order)
+            assert rel_l2(g.to_host(out), ref) <= 1e-12, (forward, zero)
+            np.testing.assert_allclose(s, s_o, rtol=1e-11)
+
+
 def _bump_channel():
     from dndsr_amd import bc
     from dndsr_amd import mesh as mm
@@ -134,14 +190,19 @@ def test_steady_adjoint_drag_gradient_in_far_field_values(require_gpu):
 
     adj = SteadyAdjoint(ev, CFL=20.0, sgsIter=1)
     uB, rB = zone_force_vjp(ev, du, dr, 0, [1.0, 0.0, 0.0])
-    lam, mu, nIt = adj.solve(du, dr, uB, rB, tol=1e-11, maxIter=4000)
-    assert nIt < 4000 and adj.history[-1] <= 1e-11 * adj.history[0]
 
     def drag(ev_, u, rec, dlc):
         return zone_force(ev_, u, rec, 0, dlc)[0]
 
-    grad = adj.bc_value_gradient(du, dr, lam, mu, drag, zones=[1])[0]
-    assert np.abs(grad - g_fd).max() <= 1e-6 * np.abs(g_fd).max(), (grad, g_fd)
+    lams = []
+    for method in ("fixed-point", "gmres"):
+        lam, mu, nIt = adj.solve(du, dr, uB, rB, tol=1e-11, maxIter=4000, method=method)
+        assert nIt < 4000, method
+        lams.append(lam)
+        grad = adj.bc_value_gradient(du, dr, lam, mu, drag, zones=[1])[0]
+        assert np.abs(grad - g_fd).max() <= 1e-6 * np.abs(g_fd).max(), (method, grad, g_fd)
+    # both solvers land on the same multipliers
+    assert float(torch.linalg.vector_norm(lams[0] - lams[1])) <= 1e-8 * float(torch.linalg.vector_norm(lams[1]))
     # the boundary values are restored
     assert abs(zone_force(ev, du, dr, 0)[0] - J0) <= 1e-10 * abs(J0)
     assert torch.isfinite(lam).all()
@@ -157,3 +218,79 @@ def test_steady_adjoint_refuses_fixed_delta_lambda_forms(require_gpu):
     with pytest.raises(ValueError, match="deltaLambdaCell"):
         SteadyAdjoint(EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType="HLLEP")))
     SteadyAdjoint(EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType="HLLEP", rsFixScale=0.0)))
+
+
+def _c4_reduced():
+    """The 2-D adjoint workload (Hicks-Henne bump, Mach 0.7, mixed cells) at 192 cells and
+    Re 200, where the oracle converges a steady state in about 600 block LU-SGS iterations."""
+    from dndsr_amd import bc
+    from dndsr_amd.shape import HicksHenneBumpChannel
+    ch = HicksHenneBumpChannel(16, 8)
+    alpha = np.random.default_rng(20261203).uniform(-0.01, 0.01, 8)
+    vals, gas = ch.flow(M=0.7, Re=200.0)
+    types = [bc.bc_code(t) for t in ch.BC_TYPES]
+    return ch, alpha, vals, gas, types
+
+
+C4_STEADY = dict(CFL=500.0, sgsIter=4, maxIter=20000, minIter=5, tol=1e-13)
+
+
+def _oracle_c4_drag(ch, a, vals, gas, types):
+    import pyoracle as po
+    from dndsr_amd import bc
+    o = po.OracleVR(ch.mesh(a), maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+    o.set_bc(types, vals, GAMMA)
+    es = po.OracleVR.euler_settings(GAMMA, RS["Roe"], viscous=1, useRoeJacobian=1,
+                                    **{k: gas[k] for k in ("Rgas", "muGas", "prGas", "muModel")})
+    uo, ro, _, _, relo, _ = o.implicit_euler_step(es, np.tile(vals[1], (o.nCell, 1)), np.zeros((o.nCell, 5, o.M)),
+                                                  1e100, **C4_STEADY)
+    assert relo[0] < 1e-12
+    _, _, dlc, _ = o.eval_dt(es, uo, 1.0, 1e100, True)
+    return o.zone_force(es, uo, ro, dlc, 0)[0], uo
+
+
+@pytest.mark.parametrize("method,block", [("gmres", True), ("fixed-point", True), ("gmres", False)])
+def test_shape_gradient_of_drag_matches_oracle_fd(require_gpu, method, block):
+    """Hicks-Henne shape gradient of the wall drag: adjoint multipliers on the device, the
+    Lagrangian differenced in each parameter on moved meshes (step 1e-6), against central
+    differences (step 1e-6) of the drag of oracle steady states on the moved meshes."""
+    from dndsr_amd.adjoint import SteadyAdjoint, zone_force, zone_force_vjp
+    from dndsr_amd.cfv import VariationalReconstruction
+    from dndsr_amd.euler import EulerEvaluator
+    from dndsr_amd.shape import shape_gradient
+    ch, alpha, vals, gas, types = _c4_reduced()
+    J0, uo = _oracle_c4_drag(ch, alpha, vals, gas, types)
+    h = 1e-6
+    g_fd = np.zeros(8)
+    for k in range(8):
+        e = np.zeros(8)
+        e[k] = h
+        g_fd[k] = (_oracle_c4_drag(ch, alpha + e, vals, gas, types)[0]
+                   - _oracle_c4_drag(ch, alpha - e, vals, gas, types)[0]) / (2 * h)
+
+    def build(a, roeJ=False):
+        g = VariationalReconstruction(ch.mesh(a), dict(maxOrder=1, intOrder=3, bcTypes=types)).construct()
+        return EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType="Roe", bcTypes=types, bcValues=vals,
+                                      viscous=True, idealGasProperty=gas, useRoeJacobian=roeJ))
+
+    ev = build(alpha, roeJ=True)
+    g = ev.vfv
+    du, dr = g.to_device(np.tile(vals[1], (g.nCell, 1))), g.BuildURec(5)
+    n, _, rel, _ = ev.ImplicitEulerStep(du, dr, 1e100, **C4_STEADY)
+    assert rel[0] < 1e-12, n
+    assert rel_l2(g.to_host(du), uo) <= 1e-10
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, du, None, CFL=1.0)
+    assert abs(zone_force(ev, du, dr, 0)[0] - J0) <= 1e-10 * abs(J0)
+    adj = SteadyAdjoint(ev, CFL=50.0, sgsIter=2, block=block)
+    uB, rB = zone_force_vjp(ev, du, dr, 0, [1.0, 0.0, 0.0])
+    lam, mu, nIt = adj.solve(du, dr, uB, rB, tol=1e-12, maxIter=6000, method=method)
+    assert nIt < 6000, adj.history[-3:]
+
+    def drag(ev_, u, rec, dlc):
+        return zone_force(ev_, u, rec, 0, dlc)[0]
+
+    gS = shape_gradient(adj, du, dr, lam, mu, drag, build, alpha, h=1e-6)
+    err = np.abs(gS - g_fd).max() / np.abs(g_fd).max()
+    print(f"shape gradient ({method}, block {block}, {nIt} iterations): max rel error vs oracle FD {err:.3e}")
+    assert err <= 1e-6, (gS, g_fd)
