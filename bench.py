@@ -182,7 +182,8 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL_INVIS, bc.BC_IN]
     m = mm.bump_channel_hex(*shape, jitter=0.1, seed=1)
     vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
-    m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
+    if colour_sort:
+        m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
     g = VariationalReconstruction(m, vs).construct()
     ev = EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
                                 bcValues=np.stack([far] * len(types))))
@@ -252,16 +253,18 @@ def vr_sweep_forms(m, vfv, u, reps=20):
     return out
 
 
-def laminar_box(n=66, reorder="morton"):
+def laminar_box(n=66, reorder="morton", colour_sort=False):
     """The laminar configuration's mesh and evaluator: the unit box with a 0.1H sin^2 bump on
     the lower wall, the (n+1)^3 jittered lattice points Delaunay-tetrahedralised
-    (mesh.bump_box_delaunay, ~6.95 n^3 cells), cells renumbered for locality (Morton order of
-    the cell centres: 8.04 ms per implicit iteration against 8.95 ms with reverse
-    Cuthill-McKee and 9.33 ms in Qhull's order on the 1.97M-cell mesh) and then grouped by
-    Gauss-Seidel colour (coalesced colour passes).  Re 1e4 on the height and free-stream speed, Mach 0.5,
-    p1 VR (intOrder 3), Roe, laminar viscous fluxes (constant mu, Pr 0.72); no-slip adiabatic
-    lower wall, free stream through the Riemann flux at inlet, outlet and top, symmetry on
-    the spanwise sides."""
+    (mesh.bump_box_delaunay, ~6.95 n^3 cells), cells renumbered for locality by the Morton
+    order of their centres.  The implicit sweep keeps its own colour-major copy of what it
+    reads, so the cells need not be grouped by colour (colour_sort groups them anyway).
+    Measured per implicit iteration on the 2.0M-cell mesh (FP64 / FP32 sweeps): Morton
+    6.84 / 5.24 ms; Morton grouped by colour 7.97 / 6.45 ms; reverse Cuthill-McKee grouped
+    by colour 8.95 / 7.07 ms; Qhull's order grouped by colour 9.33 / 7.61 ms.  Re 1e4 on the
+    height and free-stream speed, Mach 0.5, p1 VR (intOrder 3), Roe, laminar viscous fluxes
+    (constant mu, Pr 0.72); no-slip adiabatic lower wall, free stream through the Riemann
+    flux at inlet, outlet and top, symmetry on the spanwise sides."""
     from dndsr_amd import bc
     from dndsr_amd import mesh as mm
     from dndsr_amd.cfv import VariationalReconstruction
@@ -273,7 +276,8 @@ def laminar_box(n=66, reorder="morton"):
     if reorder:
         m = mm.reorder_cells(m, reorder)
     vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
-    m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())
+    if colour_sort:
+        m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())
     g = VariationalReconstruction(m, vs).construct()
     gas = dict(Rgas=1.0, muGas=0.5 / Re, prGas=0.72, muModel=0)
     ev = EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
@@ -308,8 +312,7 @@ def implicit_laminar_tet(n=66, tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=20000, 
     out = {"case": (f"steady laminar NS, unit box with a 0.1H sin^2 bump, {n + 1}^3 jittered lattice points (10% "
                     f"of the edge, PCG64 seed 20261202) Delaunay-tetrahedralised: {g.nCell} tetrahedra, Re 1e4, "
                     f"Mach 0.5, p1 VR intOrder 3, Roe, no-slip adiabatic wall, BCIn far boundaries, symmetric "
-                    f"sides; cells in reverse Cuthill-McKee order grouped by colour ({int(g.colors().max()) + 1} "
-                    f"colours)"),
+                    f"sides; cells in Morton order ({int(g.colors().max()) + 1} Gauss-Seidel colours)"),
            "cells": g.nCell, "mesh_s": round(t_mesh, 1),
            "settings": {"pseudo_CFL": CFL, "tol_rel_rho": tol, "nRec": 1, "sgsIter": sgsIter, "jacobiCode": 1,
                         "physical_dt": "infinite (steady)"}}

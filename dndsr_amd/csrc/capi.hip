@@ -39,6 +39,7 @@ struct dndsr_ctx
     // cell-major sweep state and per-cell-slot face data of the packed sweeps
     real *iSlot = nullptr, *iUA = nullptr;
     float *fSlot = nullptr, *fUA = nullptr;
+    idx *pNbr = nullptr; // packed sweep: the neighbours' colour-major positions, [K][N]
     PPStat *iStat = nullptr;   // positivity statistics of the current iteration
     real *iURecSave = nullptr; // uRec at entry, restored when a step fails
     int64_t ppLimited = 0;     // limited cell increments of the last implicit step
@@ -423,6 +424,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.fshift = devAlloc<real>(c, hsh.size());
     m.color = devAlloc<int32_t>(c, N);
     m.colorCells = devAlloc<idx>(c, N);
+    m.cellPos = devAlloc<idx>(c, N);
     {
         std::vector<uint8_t> hfl((size_t)K * N, 0);
         for (size_t i = 0; i < N; i++)
@@ -455,6 +457,12 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     upload(c, m.fshift, hsh);
     upload(c, m.color, col);
     upload(c, m.colorCells, cells);
+    {
+        std::vector<idx> pos(N);
+        for (size_t q = 0; q < N; q++)
+            pos[cells[q]] = (idx)q;
+        upload(c, m.cellPos, pos);
+    }
     // boundary faces, the cells that own them, per-zone VR direction weights
     m.nBnd = (idx)hbnd.size();
     m.fzone = devAlloc<int32_t>(c, F);
@@ -1343,22 +1351,26 @@ static void implicitStep(dndsr_ctx *ctx, const dndsr_euler_settings *es, const d
         {
             // sweeps with the neighbour state cell-major (the sweep's gathers), FP32
             if (!ctx->fSlot)
-                ctx->fSlot = devAlloc<float>(c, (size_t)m.K * 5 * N), ctx->fUA = devAlloc<float>(c, nU);
+                ctx->fSlot = devAlloc<float>(c, ((size_t)m.K * 5 + 7) * N), ctx->fUA = devAlloc<float>(c, nU);
+            if (!ctx->pNbr)
+                ctx->pNbr = devAlloc<idx>(c, (size_t)m.K * N);
             launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
             launchToF32(c, N, ctx->iJd, ctx->fJd);
             launchToF32(c, F, ctx->wLam, ctx->fLam); // lambdaFace row only
             float *inc = nullptr;
-            launchSGSPacked32(c, gp, (float)alphaDiag, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fInc, ctx->fTmp,
-                              is->sgsIter, is->gsUpdate, &inc);
+            launchSGSPacked32(c, gp, (float)alphaDiag, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->pNbr,
+                              ctx->fInc, ctx->fTmp, is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iStat);
         }
         else
         {
             if (!ctx->iSlot)
-                ctx->iSlot = devAlloc<real>(c, (size_t)m.K * 5 * N), ctx->iUA = devAlloc<real>(c, nU);
+                ctx->iSlot = devAlloc<real>(c, ((size_t)m.K * 5 + 7) * N), ctx->iUA = devAlloc<real>(c, nU);
+            if (!ctx->pNbr)
+                ctx->pNbr = devAlloc<idx>(c, (size_t)m.K * N);
             real *inc = nullptr;
-            launchSGSPacked64(c, gp, alphaDiag, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iInc, ctx->iTmp,
-                              is->sgsIter, is->gsUpdate, &inc);
+            launchSGSPacked64(c, gp, alphaDiag, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->pNbr,
+                              ctx->iInc, ctx->iTmp, is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iStat);
         }
         // fstop: residual of this iteration relative to its running maximum (density component)

@@ -713,10 +713,23 @@ kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
     __syncthreads();
     real uLm[5], uRm[5], UL[PPW][5], UR[PPW][5];
     rhsSideStates<DIM, NDOF, PPW, false>(lds, lane, x, uRec != nullptr, uLm, UL);
+    // the left mean waits in LDS rows past the staging and flux rows (the same for every wave),
+    // so that it holds no registers through the right side
+    constexpr int KEEP = R::STAGED > 5 * G ? R::STAGED : 5 * G;
+    if (g == 0)
+    {
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            lds[(KEEP + v) * kTileF + lane] = uLm[v];
+    }
     __syncthreads(); // the left rows are consumed
     rhsSideDispatch<NDOF, NW, 0, R::STAGED>(g, m, u, uRec, lds, ocR, of);
-    // the lane's point normals and weights, in flight with the right side's rows
-    real n[PPW][3], wj[PPW];
+    __builtin_amdgcn_s_waitcnt(0);
+    __syncthreads();
+    rhsSideStates<DIM, NDOF, PPW, true>(lds, lane, x, uRec != nullptr, uRm, UR);
+    // the lane's point normals, loaded after the right states so that they do not hold registers
+    // through them; their latency overlaps the barrier (the weights are applied in the final sum)
+    real n[PPW][3];
 #pragma unroll
     for (int p = 0; p < PPW; p++)
     {
@@ -724,7 +737,6 @@ kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
 #pragma unroll
         for (int r = 0; r < 3; r++)
             n[p][r] = bldnt(rowRsrc(m.fnorm + ((size_t)q * 3 + r) * F, rowF), fo);
-        wj[p] = bldnt(rowRsrc(m.fwj + (size_t)q * F, rowF), fo);
     }
     real nm[3];
     if constexpr (BATCH)
@@ -733,10 +745,10 @@ kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
         for (int r = 0; r < 3; r++)
             nm[r] = bld(rowRsrc(m.fmeanN + (size_t)r * F, rowF), fo);
     }
-    __builtin_amdgcn_s_waitcnt(0);
-    __syncthreads();
-    rhsSideStates<DIM, NDOF, PPW, true>(lds, lane, x, uRec != nullptr, uRm, UR);
     __syncthreads(); // the right rows are consumed: the staging rows take the point fluxes
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        uLm[v] = lds[(KEEP + v) * kTileF + lane];
     if (!valid)
     {
 #pragma unroll
@@ -764,7 +776,7 @@ kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
         }
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            lds[(q * 5 + v) * kTileF + lane] = valid ? (-Fl[v]) * wj[p] : 0.0;
+            lds[(q * 5 + v) * kTileF + lane] = valid ? -Fl[v] : 0.0;
     }
     __syncthreads();
     for (int t = threadIdx.x; t < 5 * kTileF; t += blockDim.x)
@@ -776,7 +788,7 @@ kRHSFaceS(DevMesh m, GasParams gp, const real *__restrict__ u, const real *__res
         real acc = 0;
 #pragma unroll
         for (int q = 0; q < G; q++)
-            acc += lds[(q * 5 + v) * kTileF + l];
+            acc += lds[(q * 5 + v) * kTileF + l] * m.fwj[(size_t)q * F + fout];
         fflux[(size_t)v * F + fout] = acc;
     }
 }
@@ -1143,6 +1155,7 @@ static void rhsLaunch(Ctx &c, const GasParams &gp, const real *u, const real *uR
         int srows = RHSSideRows<NDOF>::STAGED;
         if (srows < 5 * G)
             srows = 5 * G;
+        srows += 5; // the left mean
         const size_t ss = (size_t)srows * kTileF * sizeof(real);
         // three waves per SIMD where the registers allow it; the cubic basis needs the room of two
         auto kern = kRHSFaceS<DIM, NDOF, G, RS, BATCH, PPW, (NDOF >= 20 ? 2 : 3)>;
@@ -1811,24 +1824,33 @@ void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const
 // EulerEvaluator.hxx:1964-2020, with relax = incrementPPRelax) and AddFixedIncrement /
 // ---------------------------------------------------------------------------
 // Packed Gauss-Seidel sweep of the device implicit loop.  Same arithmetic as
-// kSGSColour, other layout: the state the sweep gathers from neighbours (u, uInc,
-// uIncNew) is stored cell-major, [N][5], so one neighbour costs one or two cache
-// lines instead of ten, and the face data a cell needs per slot (signed mean normal,
-// area, lambdaFace) is laid out per cell and slot, [K][5][N], by kSGSSlots once per
-// iteration, so the cell reads it coalesced instead of through the face index.
+// kSGSColour, other layout: everything the sweep touches is stored in colour-major
+// position order (position p holds cell colorCells[p]), so a colour pass reads and writes
+// one contiguous range whatever the caller's cell order (a locality order for the face
+// and cell kernels need not group cells by colour).  The state the sweep gathers from
+// neighbours (u, uInc, uIncNew) is cell-major, [N][5], so one neighbour costs one or two
+// cache lines instead of ten; the face data a cell needs per slot (signed mean normal,
+// area, lambdaFace) and the neighbour's position are laid out per position and slot,
+// [K][5][N] and [K][N], and the cell's rhs, JDiag and volume per position, all by
+// kSGSSlots once per iteration.  A neighbour precedes in the sweep exactly when its
+// position is lower (the colouring has no neighbours of one colour).
 // ---------------------------------------------------------------------------
 template <typename T>
-__global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict__ lam, T *__restrict__ slot)
+__global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict__ lam, const T *__restrict__ rhs,
+                                                 const T *__restrict__ jd, T *__restrict__ slot,
+                                                 idx *__restrict__ nbr)
 {
-    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= m.N)
+    const idx p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.N)
         return;
     const size_t N = m.N, F = m.F;
+    const idx i = m.colorCells[p];
     const T *__restrict__ farea = SGSGeom<T>::area(m);
     const T *__restrict__ fnorm = SGSGeom<T>::norm(m);
     for (int k = 0; k < m.K; k++)
     {
         const idx f = m.c2f[(size_t)k * N + i];
+        const idx o = m.c2c[(size_t)k * N + i];
         T q[5] = {0, 0, 0, 0, 0};
         if (f >= 0)
         {
@@ -1841,75 +1863,83 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
         }
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            slot[((size_t)k * 5 + v) * N + i] = q[v];
+            slot[((size_t)k * 5 + v) * N + p] = q[v];
+        nbr[(size_t)k * N + p] = (f < 0 || o < 0 || o == i) ? idx(-1) : m.cellPos[o];
     }
+    T *cellRows = slot + (size_t)m.K * 5 * N;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        cellRows[(size_t)v * N + p] = rhs[(size_t)v * N + i];
+    cellRows[5 * N + p] = jd[i];
+    cellRows[6 * N + p] = SGSGeom<T>::vol(m)[i];
 }
 
 template <typename T>
-__global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag,
-                                                        const T *__restrict__ rhs, const T *__restrict__ uA,
+__global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ uA,
                                                         const T *__restrict__ incA, T *__restrict__ incNewA,
-                                                        const T *__restrict__ jd, const T *__restrict__ slot,
+                                                        const T *__restrict__ slot, const idx *__restrict__ nbr,
                                                         int forward, int gsUpdate, idx begin, idx end)
 {
-    const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= end)
+    const idx p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end)
         return;
     const size_t N = m.N;
-    const idx i = m.colorCells[t];
     const T gamma = (T)gammaD;
+    const T *__restrict__ cellRows = slot + (size_t)m.K * 5 * N;
     T buf[5];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        buf[v] = rhs[v * N + i];
-    const T voli = SGSGeom<T>::vol(m)[i];
+        buf[v] = cellRows[(size_t)v * N + p];
+    const T voli = cellRows[6 * N + p];
     for (int k = 0; k < m.K; k++)
     {
-        const idx o = m.c2c[(size_t)k * N + i];
-        const uint8_t fl = m.c2flag[(size_t)k * N + i];
-        if (o < 0 || o == i)
+        const idx po = nbr[(size_t)k * N + p];
+        if (po < 0)
             continue;
-        const bool earlier = forward ? (fl & 2) != 0 : (fl & 2) == 0;
+        const bool earlier = forward ? po < p : po > p;
         const T *du = (gsUpdate && earlier) ? incNewA : incA;
         T U[5], dU[5], n[3], dF[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            U[v] = uA[(size_t)o * 5 + v], dU[v] = du[(size_t)o * 5 + v];
+            U[v] = uA[(size_t)po * 5 + v], dU[v] = du[(size_t)po * 5 + v];
 #pragma unroll
         for (int r = 0; r < 3; r++)
-            n[r] = slot[((size_t)k * 5 + r) * N + i];
-        const T area = slot[((size_t)k * 5 + 3) * N + i], lamf = slot[((size_t)k * 5 + 4) * N + i];
+            n[r] = slot[((size_t)k * 5 + r) * N + p];
+        const T area = slot[((size_t)k * 5 + 3) * N + p], lamf = slot[((size_t)k * 5 + 4) * N + p];
         fluxJacDu<T>(U, n, dU, lamf, gamma, dF);
         T coef = (T(0.5) * alphaDiag) * area / voli;
 #pragma unroll
         for (int v = 0; v < 5; v++)
             buf[v] -= coef * dF[v];
     }
-    const T d = jd[i];
+    const T d = cellRows[5 * N + p];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        incNewA[(size_t)i * 5 + v] = buf[v] / d;
+        incNewA[(size_t)p * 5 + v] = buf[v] / d;
 }
 
+// the state cell-major in colour-major position order
 template <typename T>
-__global__ void __launch_bounds__(256) kToCellMajor(idx N, const real *__restrict__ soa, T *__restrict__ aos)
+__global__ void __launch_bounds__(256) kToCellMajor(idx N, const idx *__restrict__ cellAt,
+                                                    const real *__restrict__ soa, T *__restrict__ aos)
 {
-    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= N)
+    const idx p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= N)
         return;
+    const idx i = cellAt[p];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        aos[(size_t)i * 5 + v] = (T)soa[(size_t)v * N + i];
+        aos[(size_t)p * 5 + v] = (T)soa[(size_t)v * N + i];
 }
 
 template <typename T>
 void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const real *u, const T *lam,
-                      const T *jd, T *uA, T *slot, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
+                      const T *jd, T *uA, T *slot, idx *nbr, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
 {
     DevMesh &m = c.m;
     const int nb = gridFor(m.N, 256);
-    kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, u, uA);
-    kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, slot);
+    kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, m.colorCells, u, uA);
+    kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, rhs, jd, slot, nbr);
     DNDSR_HIP_CHECK(hipGetLastError());
     const size_t nU = 5 * (size_t)m.N;
     DNDSR_HIP_CHECK(hipMemsetAsync(inc, 0, nU * sizeof(T), c.stream));
@@ -1924,8 +1954,8 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
                 if (e <= b)
                     continue;
                 KernelTimer kt(c, K_SGS);
-                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, uA, inc,
-                                                                                tmp, jd, slot, dir, gsUpdate, b, e);
+                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, uA, inc, tmp,
+                                                                                slot, nbr, dir, gsUpdate, b, e);
                 DNDSR_HIP_CHECK(hipGetLastError());
             }
             std::swap(inc, tmp);
@@ -1933,16 +1963,16 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
     *result = inc;
 }
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
-                       const real *jd, real *uA, real *slot, real *inc, real *tmp, int sgsIter, int gsUpdate,
+                       const real *jd, real *uA, real *slot, idx *nbr, real *inc, real *tmp, int sgsIter, int gsUpdate,
                        real **result)
 {
-    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, inc, tmp, sgsIter, gsUpdate, result);
 }
 void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
-                       const float *lam, const float *jd, float *uA, float *slot, float *inc, float *tmp, int sgsIter,
-                       int gsUpdate, float **result)
+                       const float *lam, const float *jd, float *uA, float *slot, idx *nbr, float *inc, float *tmp,
+                       int sgsIter, int gsUpdate, float **result)
 {
-    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, inc, tmp, sgsIter, gsUpdate, result);
 }
 
 // ---------------------------------------------------------------------------
@@ -2143,18 +2173,20 @@ __device__ void ppCompressInc(const real *u, real *ret, real gamma, real rhoEps,
     }
 }
 
+// CELLMAJOR: du is [N][5] in colour-major position order (pos = the cell's position)
 template <typename T, bool CELLMAJOR = false>
 __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real relax, real rhoEps, real pEps,
                                                      real *__restrict__ u, const T *__restrict__ du,
-                                                     PPStat *__restrict__ stat)
+                                                     PPStat *__restrict__ stat, const idx *__restrict__ pos = nullptr)
 {
     const idx i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= N)
         return;
     real U[5], D[5];
+    const size_t pi = CELLMAJOR ? (size_t)pos[i] : (size_t)i;
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? (size_t)i * 5 + v : (size_t)v * N + i];
+        U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? pi * 5 + v : (size_t)v * N + i];
     bool limited = false, bad = false;
     const real a = ppCellAlpha(U, D, gamma, relax, rhoEps, pEps, limited, bad);
 #pragma unroll
@@ -2203,10 +2235,10 @@ void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, re
     KernelTimer kt(c, K_INCREMENT);
     if (du64)
         kAddIncrement<real, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
-                                                                             du64, stat);
+                                                                             du64, stat, c.m.cellPos);
     else
         kAddIncrement<float, true><<<gridFor(c.m.N, 256), 256, 0, c.stream>>>(c.m.N, gamma, relax, rhoEps, pEps, u,
-                                                                              du32, stat);
+                                                                              du32, stat, c.m.cellPos);
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 void launchAddIncrement32(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const float *du,
