@@ -425,6 +425,7 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.color = devAlloc<int32_t>(c, N);
     m.colorCells = devAlloc<idx>(c, N);
     m.cellPos = devAlloc<idx>(c, N);
+    m.sorCells = devAlloc<idx>(c, N);
     {
         std::vector<uint8_t> hfl((size_t)K * N, 0);
         for (size_t i = 0; i < N; i++)
@@ -462,6 +463,26 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
         for (size_t q = 0; q < N; q++)
             pos[cells[q]] = (idx)q;
         upload(c, m.cellPos, pos);
+        // per colour: cells without a boundary face, then cells with one
+        std::vector<char> hasBnd(N, 0);
+        for (size_t i = 0; i < N; i++)
+            for (int k = 0; k < K; k++)
+                if (hc2f[(size_t)k * N + i] >= 0 && hc2c[(size_t)k * N + i] < 0)
+                    hasBnd[i] = 1;
+        std::vector<idx> sc(N);
+        m.sorSplitHost.assign(nColor, 0);
+        for (int q = 0; q < nColor; q++)
+        {
+            size_t w = m.colorPtrHost[q];
+            for (idx t = m.colorPtrHost[q]; t < m.colorPtrHost[q + 1]; t++)
+                if (!hasBnd[cells[t]])
+                    sc[w++] = cells[t];
+            m.sorSplitHost[q] = (idx)w;
+            for (idx t = m.colorPtrHost[q]; t < m.colorPtrHost[q + 1]; t++)
+                if (hasBnd[cells[t]])
+                    sc[w++] = cells[t];
+        }
+        upload(c, m.sorCells, sc);
     }
     // boundary faces, the cells that own them, per-zone VR direction weights
     m.nBnd = (idx)hbnd.size();

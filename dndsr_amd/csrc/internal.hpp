@@ -90,6 +90,10 @@ struct DevMesh
     std::vector<idx> colorPtrHost;
     idx *colorCells = nullptr; // [N] cells grouped by colour (the cell at each colour-major position)
     idx *cellPos = nullptr;    // [N] the colour-major position of each cell (inverse of colorCells)
+    // the SOR sweep's cell lists: per colour (the ranges of colorPtrHost) the cells without a
+    // boundary face first, then those with one, from sorSplitHost[colour] on
+    idx *sorCells = nullptr;
+    std::vector<idx> sorSplitHost;
     int32_t *color = nullptr;  // [N]
     real globalVol = 0;
     // boundary faces (face2cell[f][1] = -1) and their zones

@@ -365,7 +365,7 @@ __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restr
 //   boundary: uRec_i += relax A^-1 BCC(uRec_i as accumulated so far), as the reference
 //             hands GetBoundaryRHS the array being updated (:584).
 // ---------------------------------------------------------------------------
-template <int DIM, int NDOF, int NV>
+template <int DIM, int NDOF, int NV, bool BND>
 __global__ void __launch_bounds__(64) kVRSOR(DevMesh m, const idx *__restrict__ cells, idx nCells,
                                              real *__restrict__ uRec, const real *__restrict__ u, real relax)
 {
@@ -392,29 +392,41 @@ __global__ void __launch_bounds__(64) kVRSOR(DevMesh m, const idx *__restrict__ 
         const idx o = m.c2c[(size_t)k * N + i];
         if (o >= 0)
         {
+            // the neighbour's block once, then the matrix row by row (each matrix entry read once)
             const real *B = m.AinvB + ((size_t)(1 + k) * M * M) * N;
             const real *bv = m.Ainvb + ((size_t)k * M) * N;
+            real nb[NV][M], du[NV];
 #pragma unroll
             for (int v = 0; v < NV; v++)
             {
-                const real du = u[(size_t)v * N + o] - um[v];
-                real nb[M];
+                du[v] = u[(size_t)v * N + o] - um[v];
 #pragma unroll
                 for (int q = 0; q < M; q++)
-                    nb[q] = uRec[((size_t)v * M + q) * N + o];
+                    nb[v][q] = uRec[((size_t)v * M + q) * N + o];
+            }
 #pragma unroll
-                for (int r = 0; r < M; r++)
+            for (int r = 0; r < M; r++)
+            {
+                real br[M];
+#pragma unroll
+                for (int q = 0; q < M; q++)
+                    br[q] = B[((size_t)r * M + q) * N + i];
+                const real b = bv[(size_t)r * N + i];
+#pragma unroll
+                for (int v = 0; v < NV; v++)
                 {
                     real s = 0;
 #pragma unroll
                     for (int q = 0; q < M; q++)
-                        s += B[((size_t)r * M + q) * N + i] * nb[q];
-                    s += bv[(size_t)r * N + i] * du;
+                        s += br[q] * nb[v][q];
+                    s += b * du[v];
                     acc[v][r] += relax * s;
                 }
             }
             continue;
         }
+        if constexpr (!BND) // the cell lists without a boundary face
+            continue;
         real mom[M], cen[3];
 #pragma unroll
         for (int j = 0; j < M; j++)
@@ -708,16 +720,31 @@ template <int DIM, int NDOF>
 static void vrSORN(Ctx &c, int nVars, real *uRec, const real *u, real relax)
 {
     const DevMesh &m = c.m;
+    // per colour two launches: the cells without a boundary face (no boundary branch, fewer
+    // registers), then those with one; cells of one colour are independent
     for (int col = 0; col < m.nColor; col++)
     {
-        const idx b = m.colorPtrHost[col], n = m.colorPtrHost[col + 1] - b;
-        if (n == 0)
-            continue;
-        if (nVars == 5)
-            kVRSOR<DIM, NDOF, 5><<<gridFor(n, 64), 64, 0, c.stream>>>(m, m.colorCells + b, n, uRec, u, relax);
-        else
-            kVRSOR<DIM, NDOF, 1><<<gridFor(n, 64), 64, 0, c.stream>>>(m, m.colorCells + b, n, uRec, u, relax);
-        DNDSR_HIP_CHECK(hipGetLastError());
+        const idx b = m.colorPtrHost[col], s = m.sorSplitHost[col], e = m.colorPtrHost[col + 1];
+        if (s > b)
+        {
+            if (nVars == 5)
+                kVRSOR<DIM, NDOF, 5, false><<<gridFor(s - b, 64), 64, 0, c.stream>>>(m, m.sorCells + b, s - b, uRec, u,
+                                                                                     relax);
+            else
+                kVRSOR<DIM, NDOF, 1, false><<<gridFor(s - b, 64), 64, 0, c.stream>>>(m, m.sorCells + b, s - b, uRec, u,
+                                                                                     relax);
+            DNDSR_HIP_CHECK(hipGetLastError());
+        }
+        if (e > s)
+        {
+            if (nVars == 5)
+                kVRSOR<DIM, NDOF, 5, true><<<gridFor(e - s, 64), 64, 0, c.stream>>>(m, m.sorCells + s, e - s, uRec, u,
+                                                                                    relax);
+            else
+                kVRSOR<DIM, NDOF, 1, true><<<gridFor(e - s, 64), 64, 0, c.stream>>>(m, m.sorCells + s, e - s, uRec, u,
+                                                                                    relax);
+            DNDSR_HIP_CHECK(hipGetLastError());
+        }
     }
 }
 
