@@ -559,7 +559,7 @@ void launchEvalRHSVJP(Ctx &c, const GasParams &gp, const real *u, const real *uR
         else if (m.P == 2)
             rhsAdjRS<3, 10>(c, gp, u, uRec, dlc, rbar, ubar, urecbar);
         else
-            throw std::runtime_error("the adjoint of EvaluateRHS supports maxOrder 1 and 2");
+            rhsAdjRS<3, 20>(c, gp, u, uRec, dlc, rbar, ubar, urecbar);
     }
 }
 

@@ -36,7 +36,7 @@ def _dot(a, b):
     return float((a.double() * b.double()).sum())
 
 
-@pytest.mark.parametrize("kind,order", [("2d", 1), ("2d", 2), ("3d", 1), ("3d", 2)])
+@pytest.mark.parametrize("kind,order", [("2d", 1), ("2d", 2), ("2d", 3), ("3d", 1), ("3d", 2), ("3d", 3)])
 @pytest.mark.parametrize("rs", ["Roe", "HLLEP", "HLLC", "Roe_M1b"])
 def test_rhs_vjp_matches_central_fd(require_gpu, kind, order, rs):
     import torch
