@@ -182,8 +182,7 @@ def implicit_bump_time_to_tol(tol=1e-8, CFL=1000.0, sgsIter=3, max_iter=6000, sh
     types = [bc.BC_IN, bc.BC_IN, bc.BC_WALL_INVIS, bc.BC_IN]
     m = mm.bump_channel_hex(*shape, jitter=0.1, seed=1)
     vs = dict(maxOrder=1, intOrder=3, bcTypes=types)
-    if colour_sort:
-        m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
+    m = mm.colour_sorted(m, VariationalReconstruction(m, vs).construct().colors())  # coalesced colour passes
     g = VariationalReconstruction(m, vs).construct()
     ev = EulerEvaluator(g, dict(gamma=gamma, riemannSolverType="Roe", bcTypes=types,
                                 bcValues=np.stack([far] * len(types))))

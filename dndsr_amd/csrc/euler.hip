@@ -1835,10 +1835,10 @@ void launchDualResidual(Ctx &c, real dt, const real *xLast, const real *x, const
 // kSGSSlots once per iteration.  A neighbour precedes in the sweep exactly when its
 // position is lower (the colouring has no neighbours of one colour).
 // ---------------------------------------------------------------------------
+// the mesh-only part of the packed layout (slot normals and areas, neighbour positions, volume):
+// written once per context and precision
 template <typename T>
-__global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict__ lam, const T *__restrict__ rhs,
-                                                 const T *__restrict__ jd, T *__restrict__ slot,
-                                                 idx *__restrict__ nbr)
+__global__ void __launch_bounds__(256) kSGSSlotsStatic(DevMesh m, T *__restrict__ slot, idx *__restrict__ nbr)
 {
     const idx p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= m.N)
@@ -1851,7 +1851,7 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
     {
         const idx f = m.c2f[(size_t)k * N + i];
         const idx o = m.c2c[(size_t)k * N + i];
-        T q[5] = {0, 0, 0, 0, 0};
+        T q[4] = {0, 0, 0, 0};
         if (f >= 0)
         {
             const T sgn = (m.c2flag[(size_t)k * N + i] & 1) ? T(-1) : T(1);
@@ -1859,19 +1859,35 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
             for (int r = 0; r < 3; r++)
                 q[r] = fnorm[r * F + f] * sgn;
             q[3] = farea[f];
-            q[4] = lam[f];
         }
 #pragma unroll
-        for (int v = 0; v < 5; v++)
+        for (int v = 0; v < 4; v++)
             slot[((size_t)k * 5 + v) * N + p] = q[v];
         nbr[(size_t)k * N + p] = (f < 0 || o < 0 || o == i) ? idx(-1) : m.cellPos[o];
+    }
+    slot[((size_t)m.K * 5 + 6) * N + p] = SGSGeom<T>::vol(m)[i];
+}
+
+// the per-iteration part: lambdaFace per slot, the cell's rhs and JDiag
+template <typename T>
+__global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict__ lam, const T *__restrict__ rhs,
+                                                 const T *__restrict__ jd, T *__restrict__ slot)
+{
+    const idx p = blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= m.N)
+        return;
+    const size_t N = m.N;
+    const idx i = m.colorCells[p];
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + i];
+        slot[((size_t)k * 5 + 4) * N + p] = f >= 0 ? lam[f] : T(0);
     }
     T *cellRows = slot + (size_t)m.K * 5 * N;
 #pragma unroll
     for (int v = 0; v < 5; v++)
         cellRows[(size_t)v * N + p] = rhs[(size_t)v * N + i];
     cellRows[5 * N + p] = jd[i];
-    cellRows[6 * N + p] = SGSGeom<T>::vol(m)[i];
 }
 
 template <typename T>
@@ -1934,12 +1950,15 @@ __global__ void __launch_bounds__(256) kToCellMajor(idx N, const idx *__restrict
 
 template <typename T>
 void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const real *u, const T *lam,
-                      const T *jd, T *uA, T *slot, idx *nbr, T *inc, T *tmp, int sgsIter, int gsUpdate, T **result)
+                      const T *jd, T *uA, T *slot, idx *nbr, bool fresh, T *inc, T *tmp, int sgsIter, int gsUpdate,
+                      T **result)
 {
     DevMesh &m = c.m;
     const int nb = gridFor(m.N, 256);
+    if (fresh)
+        kSGSSlotsStatic<T><<<nb, 256, 0, c.stream>>>(m, slot, nbr);
     kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, m.colorCells, u, uA);
-    kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, rhs, jd, slot, nbr);
+    kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, rhs, jd, slot);
     DNDSR_HIP_CHECK(hipGetLastError());
     const size_t nU = 5 * (size_t)m.N;
     DNDSR_HIP_CHECK(hipMemsetAsync(inc, 0, nU * sizeof(T), c.stream));
@@ -1963,16 +1982,18 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
     *result = inc;
 }
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
-                       const real *jd, real *uA, real *slot, idx *nbr, real *inc, real *tmp, int sgsIter, int gsUpdate,
-                       real **result)
+                       const real *jd, real *uA, real *slot, idx *nbr, bool fresh, real *inc, real *tmp, int sgsIter,
+                       int gsUpdate, real **result)
 {
-    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, fresh, inc, tmp, sgsIter, gsUpdate,
+                           result);
 }
 void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
-                       const float *lam, const float *jd, float *uA, float *slot, idx *nbr, float *inc, float *tmp,
-                       int sgsIter, int gsUpdate, float **result)
+                       const float *lam, const float *jd, float *uA, float *slot, idx *nbr, bool fresh, float *inc,
+                       float *tmp, int sgsIter, int gsUpdate, float **result)
 {
-    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, inc, tmp, sgsIter, gsUpdate, result);
+    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, fresh, inc, tmp, sgsIter, gsUpdate,
+                            result);
 }
 
 // ---------------------------------------------------------------------------
