@@ -69,6 +69,7 @@ SIGNATURES = {
     "dndsr_vr_iter": [vp, C.c_int, vp, vp, vp, C.c_int, C.c_double],
     "dndsr_vr_iter_diff": [vp, C.c_int, vp, vp, vp, vp],
     "dndsr_set_bc": [vp, C.c_int, i32p, dp, C.c_double],
+    "dndsr_set_bc_gas": [vp, C.c_int, i32p, dp, C.c_double, C.c_double],
     "dndsr_set_vr_dirichlet": [vp, C.c_int, vp],
     "dndsr_eval_dt": [vp, C.POINTER(EulerSettings), vp, C.c_double, C.c_double, C.c_int, vp, vp, vp, vp, vp],
     "dndsr_eval_rhs": [vp, C.POINTER(EulerSettings), vp, vp, vp, vp],

@@ -12,9 +12,9 @@ import numpy as np
 BC_FAR, BC_WALL, BC_WALL_INVIS, BC_WALL_ISOTHERMAL = 1, 2, 3, 4
 BC_OUT, BC_OUTP, BC_IN, BC_IN_PSTS, BC_SYM, BC_SPECIAL = 5, 6, 7, 8, 9, 10
 
-NAMES = {"BCFar": BC_FAR, "BCWall": BC_WALL, "BCWallInvis": BC_WALL_INVIS, "BCOut": BC_OUT, "BCOutP": BC_OUTP,
-         "BCIn": BC_IN, "BCSym": BC_SYM}
-SUPPORTED = (BC_FAR, BC_WALL, BC_WALL_INVIS, BC_OUT, BC_OUTP, BC_IN, BC_SYM)
+NAMES = {"BCFar": BC_FAR, "BCWall": BC_WALL, "BCWallInvis": BC_WALL_INVIS, "BCWallIsothermal": BC_WALL_ISOTHERMAL,
+         "BCOut": BC_OUT, "BCOutP": BC_OUTP, "BCIn": BC_IN, "BCInPsTs": BC_IN_PSTS, "BCSym": BC_SYM}
+SUPPORTED = (BC_FAR, BC_WALL, BC_WALL_INVIS, BC_WALL_ISOTHERMAL, BC_OUT, BC_OUTP, BC_IN, BC_IN_PSTS, BC_SYM)
 
 
 def bc_code(t) -> int:

@@ -773,9 +773,8 @@ int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRec, const doub
     CAPI_END
 }
 
-int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma)
+static void setBC(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma, double Rgas)
 {
-    CAPI_BEGIN
     requireMesh(ctx);
     Ctx &c = ctx->c;
     DevMesh &m = c.m;
@@ -786,15 +785,36 @@ int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *v
     std::vector<real> v((size_t)nZone * 5, 0.0);
     for (int z = 0; z < nZone; z++)
     {
-        need(t[z] == BC_FAR || t[z] == BC_WALL || t[z] == BC_WALL_INVIS || t[z] == BC_OUT || t[z] == BC_OUTP ||
-                 t[z] == BC_IN || t[z] == BC_SYM,
-             "boundary type not supported on this path (BCFar, BCWall, BCWallInvis, BCOut, BCOutP, BCIn, BCSym)");
+        need(t[z] == BC_FAR || t[z] == BC_WALL || t[z] == BC_WALL_INVIS || t[z] == BC_WALL_ISO || t[z] == BC_OUT ||
+                 t[z] == BC_OUTP || t[z] == BC_IN || t[z] == BC_IN_PSTS || t[z] == BC_SYM,
+             "boundary type not supported on this path (BCFar, BCWall, BCWallInvis, BCWallIsothermal, BCOut, BCOutP, "
+             "BCIn, BCInPsTs, BCSym)");
+        real *vz = v.data() + (size_t)z * 5;
         if (t[z] == BC_FAR || t[z] == BC_OUTP || t[z] == BC_IN)
         {
             need(value != nullptr, "far-field / inflow zones need a value");
             for (int k = 0; k < 5; k++)
-                v[(size_t)z * 5 + k] = value[(size_t)z * 5 + k];
-            need(v[(size_t)z * 5] > 0, "boundary state with nonpositive density");
+                vz[k] = value[(size_t)z * 5 + k];
+            need(vz[0] > 0, "boundary state with nonpositive density");
+        }
+        else if (t[z] == BC_WALL_ISO || t[z] == BC_IN_PSTS)
+        {
+            need(Rgas > 0, "BCWallIsothermal and BCInPsTs need the gas constant (dndsr_set_bc_gas with Rgas > 0)");
+            need(value != nullptr, "isothermal-wall / total-condition zones need a value");
+            const double *w = value + (size_t)z * 5;
+            if (t[z] == BC_WALL_ISO) // (T_wall): stored Rgas T_wall
+            {
+                need(w[0] > 0, "isothermal wall with nonpositive temperature");
+                vz[0] = Rgas * w[0];
+            }
+            else // (p0, T0, direction): stored (p0, Rgas T0, unit direction)
+            {
+                const double dn = std::sqrt(w[2] * w[2] + w[3] * w[3] + w[4] * w[4]);
+                need(w[0] > 0 && w[1] > 0 && dn > 0, "total-condition inflow needs p0 > 0, T0 > 0 and a direction");
+                vz[0] = w[0], vz[1] = Rgas * w[1];
+                for (int d = 0; d < 3; d++)
+                    vz[2 + d] = w[2 + d] / dn;
+            }
         }
     }
     if (!m.bcType)
@@ -805,6 +825,20 @@ int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *v
     upload(c, m.bcType, t);
     upload(c, m.bcVal, v);
     m.bcGamma = gamma;
+}
+
+int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma)
+{
+    CAPI_BEGIN
+    setBC(ctx, nZone, type, value, gamma, 0.0);
+    CAPI_END
+}
+
+int dndsr_set_bc_gas(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma, double Rgas)
+{
+    CAPI_BEGIN
+    need(Rgas > 0, "Rgas must be positive");
+    setBC(ctx, nZone, type, value, gamma, Rgas);
     CAPI_END
 }
 

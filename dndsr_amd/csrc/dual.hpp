@@ -68,6 +68,12 @@ __host__ __device__ inline Dual sqrt(const Dual &a)
     const real s = ::sqrt(a.v);
     return {s, a.d * (0.5 / s)};
 }
+// a^k for a > 0
+__host__ __device__ inline Dual pow(const Dual &a, real k)
+{
+    const real p = ::pow(a.v, k);
+    return {p, a.d * (k * p / a.v)};
+}
 __host__ __device__ inline Dual fabs(const Dual &a) { return a.v < 0 ? -a : a; }
 __host__ __device__ inline Dual fmax(const Dual &a, const Dual &b) { return a.v >= b.v ? a : b; }
 __host__ __device__ inline Dual fmin(const Dual &a, const Dual &b) { return a.v <= b.v ? a : b; }

@@ -456,10 +456,15 @@ __host__ __device__ inline void fluxJacDu(const T *U, const T *n, const T *dU, T
 // Boundary ghost states (generateBoundaryValue, EulerEvaluator_EvaluateDt.hxx:1743-1838),
 // static frame: generateBV_FarField :1870-1950 (BCFar, BCOutP without anchors),
 // generateBV_InviscidWall :2199-2215 (BCWallInvis, BCSym), generateBV_ViscousWall
-// :2241-2270 (BCWall: velocity reversed), generateBV_Outflow :2346-2353, generateBV_Inflow
-// :2377-2390.  Codes are EulerBCType (src/Euler/EulerBC.hpp:35-48).
+// :2241-2270 (BCWall: velocity reversed; BCWallIsothermal: velocity reversed, density
+// p / (Rgas T_wall)), generateBV_Outflow :2346-2353, generateBV_Inflow :2377-2390,
+// generateBV_TotalConditionInflow :2417-2455 (BCInPsTs).  Codes are EulerBCType
+// (src/Euler/EulerBC.hpp:35-48).  The zone values of the two gas-dependent types are stored
+// prepared (dndsr_set_bc_gas): BCWallIsothermal (Rgas T_wall, ...), BCInPsTs (p0, Rgas T0,
+// unit flow direction).
 // ---------------------------------------------------------------------------
-constexpr int BC_FAR = 1, BC_WALL = 2, BC_WALL_INVIS = 3, BC_OUT = 5, BC_OUTP = 6, BC_IN = 7, BC_SYM = 9;
+constexpr int BC_FAR = 1, BC_WALL = 2, BC_WALL_INVIS = 3, BC_WALL_ISO = 4, BC_OUT = 5, BC_OUTP = 6, BC_IN = 7,
+              BC_IN_PSTS = 8, BC_SYM = 9;
 
 // IdealGasThermalConservative2Primitive / Primitive2Conservative (Gas.hpp:279-317)
 template <class T>
@@ -537,6 +542,34 @@ __host__ __device__ inline void bcGhost(int type, const T *UL, const real *n, co
         for (int d = 0; d < 3; d++)
             UR[1 + d] *= -1;
         return;
+    case BC_WALL_ISO: // far[0] = Rgas T_wall: the mirrored state at the wall temperature
+    {
+        T prim[5];
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] *= -1;
+        cons2prim(UR, prim, gamma);
+        prim[0] = prim[4] / far[0];
+        prim2cons(prim, UR, gamma);
+        return;
+    }
+    case BC_IN_PSTS: // far = (p0, Rgas T0, unit direction): static state of the total condition
+    {
+        const real g1 = gamma / (gamma - 1), A = 2 * g1 * far[1]; // A = 2 Cp T0
+        T vSqr = (sq(UL[1]) + sq(UL[2]) + sq(UL[3])) / sq(UL[0]);
+        vSqr = fmin(vSqr, 0.95 * A);
+        const T r = 1.0 - vSqr / A; // T / T0
+        const T ps = far[0] * pow(r, g1);
+        const T vm = sqrt(vSqr);
+        T prim[5];
+        prim[0] = ps / (far[1] * r);
+        for (int d = 0; d < 3; d++)
+            prim[1 + d] = far[2 + d] * vm;
+        prim[4] = ps;
+        prim2cons(prim, UR, gamma);
+        return;
+    }
     case BC_IN:
         for (int k = 0; k < 5; k++)
             UR[k] = far[k];

@@ -360,7 +360,10 @@ __global__ void __launch_bounds__(128) kLUSGSInitBlock(DevMesh m, real gamma, co
                 uj[v] = u[(size_t)v * N + o];
         RoeAve ra;
         roeAverage(ui, uj, gamma, ra);
-        if (bnd && (type == BC_WALL || type == BC_WALL_INVIS || type == BC_SYM))
+        // the isothermal wall linearises as the adiabatic one (the linearised ghost of
+        // generateBV_ViscousWall returns before the density reset, EvaluateDt.hxx:2265-2266)
+        const int typeLin = type == BC_WALL_ISO ? BC_WALL : type;
+        if (bnd && (typeLin == BC_WALL || typeLin == BC_WALL_INVIS || typeLin == BC_SYM))
         {
             // (A(u_g) - |A| - lambdaVis) dU_g/dU_i, column c of dU_g/dU_i being the ghost map of e_c
             real JBC[25];
@@ -368,7 +371,7 @@ __global__ void __launch_bounds__(128) kLUSGSInitBlock(DevMesh m, real gamma, co
             {
                 real e[5] = {0, 0, 0, 0, 0}, g[5];
                 e[c] = 1;
-                bcGhost(type, e, n, m.bcVal, gamma, g);
+                bcGhost(typeLin, e, n, m.bcVal, gamma, g);
 #pragma unroll
                 for (int r = 0; r < 5; r++)
                     JBC[r * 5 + c] = g[r];

@@ -112,8 +112,9 @@ class EulerEvaluator:
             vals = s.get("bcValues")
             v = None if vals is None else np.ascontiguousarray(vals, np.float64).reshape(len(t), 5)
             self._bc = (t, v)
-            check(lib().dndsr_set_bc(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p),
-                                     None if v is None else v.ctypes.data_as(_lib.dp), float(s["gamma"])))
+            check(lib().dndsr_set_bc_gas(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p),
+                                         None if v is None else v.ctypes.data_as(_lib.dp), float(s["gamma"]),
+                                         float(gas["Rgas"])))
         dev = self.ctx.torch_device
         self.lambdaFace = torch.zeros((4, vfv.nFace), dtype=torch.float64, device=dev)
         # lambdaFace0/123/4 of the Roe Jacobian, written by EvaluateRHS when useRoeJacobian
@@ -218,8 +219,8 @@ class EulerEvaluator:
         t = self._bc[0]
         v = np.ascontiguousarray(bcValues, np.float64).reshape(len(t), 5)
         self._bc = (t, v)
-        check(lib().dndsr_set_bc(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p), v.ctypes.data_as(_lib.dp),
-                                 float(self.settings["gamma"])))
+        check(lib().dndsr_set_bc_gas(self.ctx.h, len(t), t.ctypes.data_as(_lib.i32p), v.ctypes.data_as(_lib.dp),
+                                     float(self.settings["gamma"]), float(self.settings["idealGasProperty"]["Rgas"])))
 
     # deprecated LU-SGS pair (EulerEvaluator.hxx:376-529): Forward writes D^-1 (r - L x_new),
     # Backward adds D^-1 (-U x); with separate uInc they reduce to block Jacobi.

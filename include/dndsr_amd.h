@@ -183,13 +183,18 @@ int dndsr_vr_iter_diff(dndsr_ctx *ctx, int nVars, const double *uRec, const doub
 
 /* ---- boundary conditions ----------------------------------------------------------
  * Boundary zones of the Euler path (BoundaryHandler / EulerBCType, src/Euler/EulerBC.hpp:35-48):
- * type[z] is an EulerBCType code (BCFar 1, BCWall 2, BCWallInvis 3, BCOut 5, BCOutP 6, BCIn 7,
- * BCSym 9; the others are rejected) and value[z][5] the conservative far-field / inflow state
- * (host; may be NULL when no zone needs one).  The ghost states follow generateBoundaryValue
- * (EulerEvaluator_EvaluateDt.hxx:1743-2390): EvaluateRHS takes them at every face point and,
- * for the Roe average, at the face centre; the VR sweep with nVars = 5 takes them at the
- * compressed point state (EulerSolver.hxx:163-175) with the given gamma. */
+ * type[z] is an EulerBCType code (BCFar 1, BCWall 2, BCWallInvis 3, BCWallIsothermal 4, BCOut 5,
+ * BCOutP 6, BCIn 7, BCInPsTs 8, BCSym 9; BCSpecial is rejected) and value[z][5] the zone's
+ * values (host; may be NULL when no zone needs one): the conservative far-field / inflow state
+ * (BCFar, BCOutP, BCIn), the wall temperature in value[z][0] (BCWallIsothermal), or total
+ * pressure, total temperature and flow direction (p0, T0, dx, dy, dz) (BCInPsTs).  The ghost
+ * states follow generateBoundaryValue (EulerEvaluator_EvaluateDt.hxx:1743-2455): EvaluateRHS
+ * takes them at every face point and, for the Roe average, at the face centre; the VR sweep
+ * with nVars = 5 takes them at the compressed point state (EulerSolver.hxx:163-175) with the
+ * given gamma.  The two gas-dependent types need the gas constant: dndsr_set_bc_gas takes it
+ * (idealGasProperty.Rgas, EulerEvaluatorSettings.hpp:291-300); dndsr_set_bc rejects them. */
 int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma);
+int dndsr_set_bc_gas(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma, double Rgas);
 /* Boundary values of the VR sweep given as data, for any nVars (the TFBoundary callback of
  * DoReconstructionIter, VariationalReconstruction.hpp:893-901, evaluated by the caller):
  * value is a device array [nVars][nGFace][nFace] of which the boundary faces are read; the

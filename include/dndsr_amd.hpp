@@ -357,8 +357,8 @@ class EulerEvaluator
         }
         vfv.ConstructRecCoeff();
         if (!s.bcType.empty())
-            check(dndsr_set_bc(vfv.context().handle(), (int)s.bcType.size(), s.bcType.data(),
-                               s.bcValue.empty() ? nullptr : s.bcValue.data(), s.gamma));
+            check(dndsr_set_bc_gas(vfv.context().handle(), (int)s.bcType.size(), s.bcType.data(),
+                                   s.bcValue.empty() ? nullptr : s.bcValue.data(), s.gamma, s.idealGasProperty.Rgas));
         lambdaFace = DeviceBuffer<double>(4 * (size_t)vfv.NumFace());
         lambdaFace.zero();
         deltaLambdaCell = DeviceBuffer<double>((size_t)vfv.NumCell());

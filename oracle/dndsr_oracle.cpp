@@ -840,6 +840,7 @@ struct VR
     std::vector<int32_t> bcType;              // EulerBCType per zone (empty: no Euler BC table set)
     std::vector<std::array<real, 5>> bcValue; // per zone
     real bcGamma{1.4};
+    real bcRgas{1.0};
     int dirNVars{0};
     std::vector<real> dirValue;               // [nFace][G][dirNVars] boundary point values
 
@@ -1362,11 +1363,46 @@ static void prim2cons(const real *prim, real *U, real gamma)
 // generateBoundaryValue (EulerEvaluator_EvaluateDt.hxx:1743-1838) for the inviscid types,
 // static frame, no CL driver / anchors: generateBV_FarField :1870-1950 (BCFar, BCOutP),
 // generateBV_InviscidWall :2199-2215 (BCWallInvis, BCSym), generateBV_ViscousWall :2241-2270
-// (BCWall, velocity reversed), generateBV_Outflow :2346-2353, generateBV_Inflow :2377-2390.
-static void bcGhost(int type, const real *UL, const real *n, const real *far, real gamma, real *UR)
+// (BCWall, velocity reversed; BCWallIsothermal, velocity reversed and the density set to
+// p / (Rgas T_wall), T_wall = far[0]), generateBV_Outflow :2346-2353, generateBV_Inflow
+// :2377-2390, generateBV_TotalConditionInflow :2417-2455 (BCInPsTs: far = (p0, T0, direction);
+// the interior speed, clamped to 0.95 of 2 Cp T0 in square, with the static state of the
+// given total pressure and temperature).
+static void bcGhost(int type, const real *UL, const real *n, const real *far, real gamma, real Rgas, real *UR)
 {
     switch (type)
     {
+    case ORC_BC_WALL_ISO:
+    {
+        real prim[5];
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] *= -1;
+        cons2prim(UR, prim, gamma);
+        const real temp = far[0];
+        if (!(prim[0] > 0 && prim[4] > 0 && temp > 0))
+            throw std::runtime_error("isothermal wall: non-positive density, pressure or wall temperature");
+        prim[0] = prim[4] / temp / Rgas;
+        prim2cons(prim, UR, gamma);
+        break;
+    }
+    case ORC_BC_IN_PSTS:
+    {
+        const real Cp = gamma / (gamma - 1) * Rgas;
+        real vSqr = (sqr(UL[1]) + sqr(UL[2]) + sqr(UL[3])) / sqr(UL[0]);
+        const real pStag = far[0], tStag = far[1];
+        vSqr = std::min(vSqr, tStag * 2 * Cp * 0.95);
+        const real tStatic = tStag - 0.5 * vSqr / Cp;
+        const real pStatic = pStag * std::pow(tStatic / tStag, gamma / (gamma - 1));
+        const real rStatic = pStatic / (Rgas * tStatic);
+        const real dn = std::sqrt(sqr(far[2]) + sqr(far[3]) + sqr(far[4]));
+        real prim[5] = {rStatic, 0, 0, 0, pStatic};
+        for (int d = 0; d < 3; d++)
+            prim[1 + d] = far[2 + d] / dn * std::sqrt(vSqr);
+        prim2cons(prim, UR, gamma);
+        break;
+    }
     case ORC_BC_FAR:
     case ORC_BC_OUTP:
     {
@@ -1487,14 +1523,14 @@ static void boundaryBCC(const VR &v, idx_t i, idx_t f, int nVars, const real *uR
             const real *far = v.bcValue[zone].data();
             real ULf[5];
             compressRecPart(u + i * 5, uBL, ULf);
-            bcGhost(v.bcType[zone], ULf, n, far, v.bcGamma, uB);
+            bcGhost(v.bcType[zone], ULf, n, far, v.bcGamma, v.bcRgas, uB);
             if (uRecDiff)
             {
                 real inc2[5], ULp[5], uBp[5];
                 for (int k = 0; k < 5; k++)
                     inc2[k] = uBL[k] + dU[k];
                 compressRecPart(u + i * 5, inc2, ULp);
-                bcGhost(v.bcType[zone], ULp, n, far, v.bcGamma, uBp);
+                bcGhost(v.bcType[zone], ULp, n, far, v.bcGamma, v.bcRgas, uBp);
                 for (int k = 0; k < 5; k++)
                     inc[k] = uBp[k] - uB[k];
             }
@@ -1951,7 +1987,7 @@ static void vr2ndCell(const VR &v, idx_t i, int nVars, const real *u, real *out)
             throw std::runtime_error("the second-order substitute needs the Euler boundary table (nVars = 5) on "
                                      "meshes with boundary faces");
         const int zone = v.faceZone[f];
-        bcGhost(v.bcType[zone], u + i * 5, v.faceMeanNorm[f].x, v.bcValue[zone].data(), v.bcGamma, uB);
+        bcGhost(v.bcType[zone], u + i * 5, v.faceMeanNorm[f].x, v.bcValue[zone].data(), v.bcGamma, v.bcRgas, uB);
     };
     for (idx_t ic = 0; ic < nf; ic++)
     {
@@ -2330,7 +2366,7 @@ int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const doubl
     ORC_CATCH(-1)
 }
 
-int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma)
+int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma, double Rgas)
 {
     ORC_TRY
     VR &v = h->v;
@@ -2342,6 +2378,7 @@ int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, d
         for (int k = 0; k < 5; k++)
             v.bcValue[z][k] = value ? value[z * 5 + k] : 0.0;
     v.bcGamma = gamma;
+    v.bcRgas = Rgas;
     return 0;
     ORC_CATCH(-1)
 }
@@ -2356,10 +2393,11 @@ int orc_set_vr_dirichlet(orc_vr *h, int nVars, const double *value)
     ORC_CATCH(-1)
 }
 
-int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double *UR)
+int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double Rgas,
+                 double *UR)
 {
     ORC_TRY
-    bcGhost(type, UL, n, zoneValue, gamma, UR);
+    bcGhost(type, UL, n, zoneValue, gamma, Rgas, UR);
     return 0;
     ORC_CATCH(-1)
 }
@@ -2604,7 +2642,7 @@ static int evalRHSCore(const orc_vr *h, const orc_euler_settings *es, const doub
         real URmB[5];
         if (bnd) // URMeanXy = ghost of the mean at the face centre, EvaluateRHS.hxx:194-207
         {
-            bcGhost(v.bcType[zone], ULm, v.faceMeanNorm[f].x, v.bcValue[zone].data(), es->gamma, URmB);
+            bcGhost(v.bcType[zone], ULm, v.faceMeanNorm[f].x, v.bcValue[zone].data(), es->gamma, v.bcRgas, URmB);
             URm = URmB;
         }
         for (size_t g = 0; g < G; g++)
@@ -2632,7 +2670,7 @@ static int evalRHSCore(const orc_vr *h, const orc_euler_settings *es, const doub
                 }
             if (bnd) // URxy = generateBoundaryValue(ULxy, ...), GradURxy = GradULxy, EvaluateRHS.hxx:336-355
             {
-                bcGhost(v.bcType[zone], UL, v.faceN[f][g].x, v.bcValue[zone].data(), es->gamma, UR);
+                bcGhost(v.bcType[zone], UL, v.faceN[f][g].x, v.bcValue[zone].data(), es->gamma, v.bcRgas, UR);
                 GRv[g] = GLv[g];
             }
             else
@@ -3010,7 +3048,7 @@ int orc_lusgs_init_block(const orc_vr *h, const orc_euler_settings *es, const do
                 bt = v.bcType.empty() ? -1 : v.bcType[zone];
                 if (bt < 0)
                     throw std::runtime_error("boundary faces need the boundary-condition table (orc_set_bc)");
-                bcGhost(bt, ui, n, v.bcValue[zone].data(), gamma, uj);
+                bcGhost(bt, ui, n, v.bcValue[zone].data(), gamma, v.bcRgas, uj);
             }
             else
                 for (int k = 0; k < 5; k++)
@@ -3028,12 +3066,15 @@ int orc_lusgs_init_block(const orc_vr *h, const orc_euler_settings *es, const do
             {
                 // dU_g / dU_i column by column; zero for far field, in- and outflow (linMode 1)
                 real JBC[25] = {0};
-                if (bt == ORC_BC_WALL_INVIS || bt == ORC_BC_SYM || bt == ORC_BC_WALL)
+                // the isothermal wall linearises as the adiabatic one: linMode 1 returns before the
+                // density reset (generateBV_ViscousWall, EvaluateDt.hxx:2265-2266)
+                const int btLin = bt == ORC_BC_WALL_ISO ? ORC_BC_WALL : bt;
+                if (btLin == ORC_BC_WALL_INVIS || btLin == ORC_BC_SYM || btLin == ORC_BC_WALL)
                     for (int c = 0; c < 5; c++)
                     {
                         real e[5] = {0, 0, 0, 0, 0}, g[5];
                         e[c] = 1;
-                        bcGhost(bt, e, n, v.bcValue[v.faceZone[f]].data(), gamma, g);
+                        bcGhost(btLin, e, n, v.bcValue[v.faceZone[f]].data(), gamma, v.bcRgas, g);
                         for (int r = 0; r < 5; r++)
                             JBC[r * 5 + c] = g[r];
                     }

@@ -127,14 +127,15 @@ int orc_vr_iter_diff(const orc_vr *h, int nVars, const double *uRec, const doubl
 /* Boundary conditions (EulerBCType codes, src/Euler/EulerBC.hpp:35-48): per zone a type and a
  * value vector of 5 conservative variables (far-field / inflow state; unused otherwise).
  * Used by the VR sweep for nVars = 5 (with gamma), by EvaluateRHS and by EvaluateDt. */
-enum { ORC_BC_FAR = 1, ORC_BC_WALL = 2, ORC_BC_WALL_INVIS = 3, ORC_BC_OUT = 5, ORC_BC_OUTP = 6, ORC_BC_IN = 7,
-       ORC_BC_SYM = 9 };
-int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma);
+enum { ORC_BC_FAR = 1, ORC_BC_WALL = 2, ORC_BC_WALL_INVIS = 3, ORC_BC_WALL_ISO = 4, ORC_BC_OUT = 5, ORC_BC_OUTP = 6,
+       ORC_BC_IN = 7, ORC_BC_IN_PSTS = 8, ORC_BC_SYM = 9 };
+int orc_set_bc(orc_vr *h, int nZone, const int32_t *type, const double *value, double gamma, double Rgas);
 /* VR sweep boundary values given as data (the TFBoundary callback of the reference tests):
  * value[f][g][var] at boundary face f, quadrature point g; used when no Euler BC applies. */
 int orc_set_vr_dirichlet(orc_vr *h, int nVars, const double *value);
 /* ghost state of one boundary point (generateBoundaryValue, EulerEvaluator_EvaluateDt.hxx:1743-1838) */
-int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double *UR);
+int orc_bc_ghost(int type, const double *UL, const double *n, const double *zoneValue, double gamma, double Rgas,
+                 double *UR);
 
 /* Riemann flux at one point; F gets the flux, lam[3] = lam0, lam123, lam4 */
 int orc_riemann(int rsType, int dim, const double *UL, const double *UR, const double *ULm,

@@ -139,13 +139,13 @@ def add_increment_pp(u, du, gamma=1.4, relax=0.9, rhoEps=1e-11, pEps=1e-11):
     return u, int(n), am.value
 
 
-def bc_ghost(bcType, UL, n, zoneValue=None, gamma=1.4):
+def bc_ghost(bcType, UL, n, zoneValue=None, gamma=1.4, Rgas=1.0):
     """Ghost state of one boundary point (generateBoundaryValue)."""
     UL = np.ascontiguousarray(UL, np.float64)
     n = np.ascontiguousarray(n, np.float64)
     zv = np.zeros(5) if zoneValue is None else np.ascontiguousarray(zoneValue, np.float64)
     UR = np.zeros(5)
-    _chk(lib().orc_bc_ghost(C.c_int(bcType), P(UL), P(n), P(zv), C.c_double(gamma), P(UR)))
+    _chk(lib().orc_bc_ghost(C.c_int(bcType), P(UL), P(n), P(zv), C.c_double(gamma), C.c_double(Rgas), P(UR)))
     return UR
 
 
@@ -268,12 +268,13 @@ class OracleVR:
         return out
 
     # ---- boundary conditions
-    def set_bc(self, types, values=None, gamma=1.4):
-        """Per-zone EulerBCType codes and 5-vectors (far-field / inflow states)."""
+    def set_bc(self, types, values=None, gamma=1.4, Rgas=1.0):
+        """Per-zone EulerBCType codes and 5-vectors (far-field / inflow states; the wall
+        temperature of BCWallIsothermal; p0, T0 and the flow direction of BCInPsTs)."""
         t = np.ascontiguousarray(types, np.int32)
         v = None if values is None else np.ascontiguousarray(values, np.float64).reshape(len(t), 5)
         self._bc = (t, v)
-        _chk(lib().orc_set_bc(self.h, C.c_int(len(t)), P(t), P(v), C.c_double(gamma)))
+        _chk(lib().orc_set_bc(self.h, C.c_int(len(t)), P(t), P(v), C.c_double(gamma), C.c_double(Rgas)))
 
     def set_vr_dirichlet(self, values):
         """Boundary point values [nFace, nGFace, nVars] for the VR sweep (boundary faces read)."""

@@ -247,3 +247,72 @@ def test_zone_force_of_uniform_pressure():
     u = np.tile(U, (o.nCell, 1))
     f = o.zone_force(es, u, np.zeros((o.nCell, 5, o.M)), None, 1)
     np.testing.assert_allclose(f, [0.0, -p * Lx, 0.0], atol=1e-12)
+
+
+# ------------------------------------------------------ isothermal wall and total-condition inflow
+R_GAS = 0.8
+
+
+def _prim(U, gamma=GAMMA):
+    v = U[1:4] / U[0]
+    return U[0], v, (gamma - 1) * (U[4] - 0.5 * U[0] * v @ v)
+
+
+def test_isothermal_wall_ghost():
+    """generateBV_ViscousWall with BCWallIsothermal (EvaluateDt.hxx:2241-2280): velocity
+    reversed, the interior pressure, and the density that puts the ghost at the wall
+    temperature (p = rho Rgas T_wall); at the interior temperature it is the adiabatic wall's."""
+    UL = _cons(1.1, [0.3, -0.2, 0.1], 0.8)
+    Tw = 1.3
+    UR = po.bc_ghost(bc.BC_WALL_ISOTHERMAL, UL, N_, [Tw, 0, 0, 0, 0], GAMMA, R_GAS)
+    rL, vL, pL = _prim(UL)
+    rR, vR, pR = _prim(UR)
+    assert abs(pR - pL) <= 1e-14 * pL
+    np.testing.assert_allclose(vR, -vL, rtol=1e-14)
+    assert abs(pR / (rR * R_GAS) - Tw) <= 1e-14 * Tw
+    Tin = pL / (rL * R_GAS)
+    np.testing.assert_allclose(po.bc_ghost(bc.BC_WALL_ISOTHERMAL, UL, N_, [Tin, 0, 0, 0, 0], GAMMA, R_GAS),
+                               po.bc_ghost(bc.BC_WALL, UL, N_, None), rtol=1e-14, atol=1e-15)
+
+
+def test_total_condition_inflow_ghost():
+    """generateBV_TotalConditionInflow (EvaluateDt.hxx:2417-2455): the interior speed along the
+    given direction, and the static state whose total temperature and total pressure are the
+    given ones (isentropic: T0 = T + v^2/(2 Cp), p0 = p (T0/T)^(gamma/(gamma-1))); a speed above
+    the limit sqrt(0.95 * 2 Cp T0) is clamped to it."""
+    Cp = GAMMA / (GAMMA - 1) * R_GAS
+    p0, T0, d = 1.2, 0.9, np.array([2.0, 1.0, -0.5])
+    for speed in (0.3, 1.5, 5.0):
+        UL = _cons(1.1, speed * np.array([0.6, -0.8, 0.0]), 0.8)
+        UR = po.bc_ghost(bc.BC_IN_PSTS, UL, N_, [p0, T0, *d], GAMMA, R_GAS)
+        r, v, p = _prim(UR)
+        vmax = np.sqrt(0.95 * 2 * Cp * T0)
+        s = min(speed, vmax)
+        np.testing.assert_allclose(v, s * d / np.linalg.norm(d), rtol=1e-13)
+        T = p / (r * R_GAS)
+        assert abs(T + s * s / (2 * Cp) - T0) <= 1e-13 * T0
+        assert abs(p * (T0 / T) ** (GAMMA / (GAMMA - 1)) - p0) <= 1e-12 * p0
+
+
+def test_isothermal_wall_conducts_heat():
+    """A gas at rest next to an isothermal wall: no residual when the wall is at the gas
+    temperature; a hotter wall heats the cells along it (positive energy residual there,
+    momentum along the wall zero), a colder one cools them, the adiabatic wall does neither."""
+    m = mm.box_quad_grid(6, 5, 3.0, 2.5, zones=dict(xmin=0, xmax=1, ymin=1, ymax=1), jitter=0.0, seed=1)
+    types = [bc.BC_WALL_ISOTHERMAL, bc.BC_SYM]
+    o = po.OracleVR(m, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+    rho, p = 1.0, 1.0 / GAMMA
+    Tgas = p / (rho * R_GAS)
+    u = np.tile(_cons(rho, [0.0, 0.0, 0.0], p), (o.nCell, 1))
+    es = o.euler_settings(GAMMA, RS["Roe"], viscous=1, muModel=0, Rgas=R_GAS, muGas=0.05, prGas=0.72)
+    wallCells = np.unique(m.face2cell[(m.face2cell[:, 1] < 0) & (m.faceZone == 0), 0])
+    res = {}
+    for Tw in (Tgas, 1.5 * Tgas, 0.5 * Tgas):
+        o.set_bc(types, np.tile([Tw, 0, 0, 0, 0], (2, 1)), GAMMA, R_GAS)
+        _, _, dlc, _ = o.eval_dt(es, u, 1.0, 1e100, True)
+        res[Tw] = o.eval_rhs(es, u, None, dlc)
+    assert np.abs(res[Tgas]).max() <= 1e-13
+    hot, cold = res[1.5 * Tgas], res[0.5 * Tgas]
+    assert np.all(hot[wallCells, 4] > 0) and np.all(cold[wallCells, 4] < 0)
+    o.set_bc([bc.BC_WALL, bc.BC_SYM], None, GAMMA, R_GAS)
+    assert np.abs(o.eval_rhs(es, u, None, dlc)).max() <= 1e-13
