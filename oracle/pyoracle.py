@@ -12,7 +12,9 @@ import subprocess
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = os.path.join(_HERE, "liboracle.so")
+# DNDSR_ORACLE_LIB selects another build of the same sources (the sanitizer build of
+# oracle/Makefile's `sanitize` target)
+_LIB = os.environ.get("DNDSR_ORACLE_LIB") or os.path.join(_HERE, "liboracle.so")
 _lib = None
 
 dp = C.POINTER(C.c_double)
