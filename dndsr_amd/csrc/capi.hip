@@ -825,6 +825,9 @@ static void setBC(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *
     upload(c, m.bcType, t);
     upload(c, m.bcVal, v);
     m.bcGamma = gamma;
+    m.gasBC = false;
+    for (int z = 0; z < nZone; z++)
+        m.gasBC = m.gasBC || t[z] == BC_WALL_ISO || t[z] == BC_IN_PSTS;
 }
 
 int dndsr_set_bc(dndsr_ctx *ctx, int nZone, const int32_t *type, const double *value, double gamma)

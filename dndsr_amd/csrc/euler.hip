@@ -1070,7 +1070,7 @@ struct SideLin
 template <int DIM, int NDOF>
 using ViscSide = std::conditional_t<NDOF == DIM + 1, SideLin<DIM>, SideRec<DIM, NDOF>>;
 
-template <int DIM, int NDOF>
+template <int DIM, int NDOF, bool GAS>
 __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, const real *__restrict__ u,
                                                    const real *__restrict__ uRec, real *__restrict__ fflux)
 {
@@ -1117,7 +1117,7 @@ __global__ void __launch_bounds__(128) kRHSViscous(DevMesh m, GasParams gp, cons
         L.eval(m, x, uRec, UL, GL);
         if (bnd)
         {
-            bcGhost(type, UL, n, far, gamma, UR);
+            bcGhost<real, GAS>(type, UL, n, far, gamma, UR);
 #pragma unroll
             for (int d = 0; d < 3; d++)
 #pragma unroll
@@ -1280,23 +1280,34 @@ void launchEvalRHS(Ctx &c, const GasParams &gp, const real *u, const real *uRec,
             throw std::runtime_error("viscous residual needs prGas > 0, Rgas > 0 and muGas >= 0");
         const int nb = gridFor(m.F, 128);
         const int P = uRec ? m.P : 1;
+        // the gas-dependent ghosts only where the table has such a zone (their branch costs
+        // the kernel registers)
+        auto launchVis = [&](auto dimC, auto ndofC) {
+            constexpr int D = decltype(dimC)::value, ND = decltype(ndofC)::value;
+            if (m.gasBC)
+                kRHSViscous<D, ND, true><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+            else
+                kRHSViscous<D, ND, false><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+        };
+        using I2 = std::integral_constant<int, 2>;
+        using I3 = std::integral_constant<int, 3>;
         if (m.dim == 2)
         {
             if (P == 1)
-                kRHSViscous<2, 3><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I2{}, I3{});
             else if (P == 2)
-                kRHSViscous<2, 6><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I2{}, std::integral_constant<int, 6>{});
             else
-                kRHSViscous<2, 10><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I2{}, std::integral_constant<int, 10>{});
         }
         else
         {
             if (P == 1)
-                kRHSViscous<3, 4><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I3{}, std::integral_constant<int, 4>{});
             else if (P == 2)
-                kRHSViscous<3, 10><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I3{}, std::integral_constant<int, 10>{});
             else
-                kRHSViscous<3, 20><<<nb, 128, 0, c.stream>>>(m, gp, u, uRec, ff);
+                launchVis(I3{}, std::integral_constant<int, 20>{});
         }
         DNDSR_HIP_CHECK(hipGetLastError());
     }

@@ -9,6 +9,8 @@
 #pragma once
 #include "dev_common.hpp"
 
+#include <type_traits>
+
 namespace dndsr
 {
 enum RSKind : int
@@ -486,9 +488,59 @@ __host__ __device__ inline void prim2cons(const T *prim, T *U, real gamma)
     U[4] = prim[4] / (gamma - 1) + (prim[0] * 0.5 * vSqr);
 }
 
+// the two gas-dependent ghosts (zone values prepared by dndsr_set_bc_gas):
+// BCWallIsothermal far = (Rgas T_wall, ...), BCInPsTs far = (p0, Rgas T0, unit direction)
 template <class T>
+__host__ __device__ inline void bcGhostGasT(int type, const T *UL, const real *far, real gamma, T *UR)
+{
+    if (type == BC_WALL_ISO) // the mirrored state at the wall temperature
+    {
+        T prim[5];
+        for (int k = 0; k < 5; k++)
+            UR[k] = UL[k];
+        for (int d = 0; d < 3; d++)
+            UR[1 + d] *= -1;
+        cons2prim(UR, prim, gamma);
+        prim[0] = prim[4] / far[0];
+        prim2cons(prim, UR, gamma);
+        return;
+    }
+    // BC_IN_PSTS: static state of the total condition at the interior speed
+    const real g1 = gamma / (gamma - 1), A = 2 * g1 * far[1]; // A = 2 Cp T0
+    T vSqr = (sq(UL[1]) + sq(UL[2]) + sq(UL[3])) / sq(UL[0]);
+    vSqr = fmin(vSqr, 0.95 * A);
+    const T r = 1.0 - vSqr / A; // T / T0
+    using ::pow;                 // the library pow for doubles; a dual number finds its own by ADL
+    const T ps = far[0] * pow(r, g1);
+    const T vm = sqrt(vSqr);
+    T prim[5];
+    prim[0] = ps / (far[1] * r);
+    for (int d = 0; d < 3; d++)
+        prim[1 + d] = far[2 + d] * vm;
+    prim[4] = ps;
+    prim2cons(prim, UR, gamma);
+}
+// out of line in the FP64 kernels, so that the rarely taken branch does not add its
+// registers to the face and reconstruction kernels that inline bcGhost
+__host__ __device__ __noinline__ inline void bcGhostGas(int type, const real *UL, const real *far, real gamma,
+                                                        real *UR)
+{
+    bcGhostGasT<real>(type, UL, far, gamma, UR);
+}
+
+// GAS = false: a caller that knows the table has no gas-dependent zone drops that branch
+template <class T, bool GAS = true>
 __host__ __device__ inline void bcGhost(int type, const T *UL, const real *n, const real *far, real gamma, T *UR)
 {
+    if constexpr (GAS)
+        if (type == BC_WALL_ISO || type == BC_IN_PSTS)
+        {
+            if constexpr (std::is_same<T, real>::value)
+                bcGhostGas(type, UL, far, gamma, UR);
+            else
+                bcGhostGasT<T>(type, UL, far, gamma, UR);
+            return;
+        }
     switch (type)
     {
     case BC_FAR:
@@ -542,34 +594,6 @@ __host__ __device__ inline void bcGhost(int type, const T *UL, const real *n, co
         for (int d = 0; d < 3; d++)
             UR[1 + d] *= -1;
         return;
-    case BC_WALL_ISO: // far[0] = Rgas T_wall: the mirrored state at the wall temperature
-    {
-        T prim[5];
-        for (int k = 0; k < 5; k++)
-            UR[k] = UL[k];
-        for (int d = 0; d < 3; d++)
-            UR[1 + d] *= -1;
-        cons2prim(UR, prim, gamma);
-        prim[0] = prim[4] / far[0];
-        prim2cons(prim, UR, gamma);
-        return;
-    }
-    case BC_IN_PSTS: // far = (p0, Rgas T0, unit direction): static state of the total condition
-    {
-        const real g1 = gamma / (gamma - 1), A = 2 * g1 * far[1]; // A = 2 Cp T0
-        T vSqr = (sq(UL[1]) + sq(UL[2]) + sq(UL[3])) / sq(UL[0]);
-        vSqr = fmin(vSqr, 0.95 * A);
-        const T r = 1.0 - vSqr / A; // T / T0
-        const T ps = far[0] * pow(r, g1);
-        const T vm = sqrt(vSqr);
-        T prim[5];
-        prim[0] = ps / (far[1] * r);
-        for (int d = 0; d < 3; d++)
-            prim[1 + d] = far[2 + d] * vm;
-        prim[4] = ps;
-        prim2cons(prim, UR, gamma);
-        return;
-    }
     case BC_IN:
         for (int k = 0; k < 5; k++)
             UR[k] = far[k];

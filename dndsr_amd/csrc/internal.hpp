@@ -98,6 +98,7 @@ struct DevMesh
     real globalVol = 0;
     // boundary faces (face2cell[f][1] = -1) and their zones
     int nBCZone = 0;
+    bool gasBC = false; // a zone of the boundary table is BCWallIsothermal or BCInPsTs
     idx nBnd = 0, nBndCell = 0;
     int32_t *fzone = nullptr;   // [F] zone of a boundary face, -1 elsewhere
     idx *bndFace = nullptr;     // [nBnd]
