@@ -37,7 +37,7 @@ struct dndsr_ctx
          *iTmp = nullptr, *iRes = nullptr;
     float *fRhs = nullptr, *fU = nullptr, *fJd = nullptr, *fLam = nullptr, *fInc = nullptr, *fTmp = nullptr;
     // cell-major sweep state and per-cell-slot face data of the packed sweeps
-    real *iSlot = nullptr, *iUA = nullptr;
+    real *iSlot = nullptr, *iUA = nullptr; // packed sweep: slot data; two [N][16] state + increment arrays
     float *fSlot = nullptr, *fUA = nullptr;
     idx *iNbr = nullptr, *fNbr = nullptr; // packed sweep: the neighbours' colour-major positions, [K][N]
     PPStat *iStat = nullptr;   // positivity statistics of the current iteration
@@ -1410,25 +1410,25 @@ static void implicitStep(dndsr_ctx *ctx, const dndsr_euler_settings *es, const d
             // sweeps with the neighbour state cell-major (the sweep's gathers), FP32
             const bool fresh = !ctx->fSlot;
             if (!ctx->fSlot)
-                ctx->fSlot = devAlloc<float>(c, ((size_t)m.K * 5 + 7) * N), ctx->fUA = devAlloc<float>(c, nU),
+                ctx->fSlot = devAlloc<float>(c, ((size_t)m.K * 5 + 7) * N), ctx->fUA = devAlloc<float>(c, 32 * N),
                 ctx->fNbr = devAlloc<idx>(c, (size_t)m.K * N);
             launchToF32(c, nU, ctx->iRhs, ctx->fRhs);
             launchToF32(c, N, ctx->iJd, ctx->fJd);
             launchToF32(c, F, ctx->wLam, ctx->fLam); // lambdaFace row only
             float *inc = nullptr;
             launchSGSPacked32(c, gp, (float)alphaDiag, ctx->fRhs, u, ctx->fLam, ctx->fJd, ctx->fUA, ctx->fSlot, ctx->fNbr,
-                              fresh, ctx->fInc, ctx->fTmp, is->sgsIter, is->gsUpdate, &inc);
+                              fresh, is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, nullptr, inc, ctx->iStat);
         }
         else
         {
             const bool fresh = !ctx->iSlot;
             if (!ctx->iSlot)
-                ctx->iSlot = devAlloc<real>(c, ((size_t)m.K * 5 + 7) * N), ctx->iUA = devAlloc<real>(c, nU),
+                ctx->iSlot = devAlloc<real>(c, ((size_t)m.K * 5 + 7) * N), ctx->iUA = devAlloc<real>(c, 32 * N),
                 ctx->iNbr = devAlloc<idx>(c, (size_t)m.K * N);
             real *inc = nullptr;
             launchSGSPacked64(c, gp, alphaDiag, ctx->iRhs, u, ctx->wLam, ctx->iJd, ctx->iUA, ctx->iSlot, ctx->iNbr,
-                              fresh, ctx->iInc, ctx->iTmp, is->sgsIter, is->gsUpdate, &inc);
+                              fresh, is->sgsIter, is->gsUpdate, &inc);
             launchAddIncrementCellMajor(c, gamma, is->incrementRelax, rhoEps, pEps, u, inc, nullptr, ctx->iStat);
         }
         // fstop: residual of this iteration relative to its running maximum (density component)

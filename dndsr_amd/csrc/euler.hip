@@ -1901,12 +1901,18 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
     cellRows[5 * N + p] = jd[i];
 }
 
+// the packed state: per position a 128-byte row (FP64; 64 bytes FP32) holding the state U
+// (5) and the increment (5), so that a neighbour costs one cache line; two such arrays,
+// read and written alternately by the sweeps
+constexpr int kPackStride = 16;
+
 template <typename T>
-__global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ uA,
-                                                        const T *__restrict__ incA, T *__restrict__ incNewA,
-                                                        const T *__restrict__ slot, const idx *__restrict__ nbr,
-                                                        int forward, int gsUpdate, idx begin, idx end)
+__global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ xr,
+                                                        T *__restrict__ xw, const T *__restrict__ slot,
+                                                        const idx *__restrict__ nbr, int forward, int gsUpdate,
+                                                        int incZero, idx begin, idx end)
 {
+    constexpr int S = kPackStride;
     const idx p = begin + blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= end)
         return;
@@ -1924,11 +1930,14 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
         if (po < 0)
             continue;
         const bool earlier = forward ? po < p : po > p;
-        const T *du = (gsUpdate && earlier) ? incNewA : incA;
+        const bool useNew = gsUpdate && earlier;
+        if (incZero && !useNew) // the first sweep starts from a zero increment
+            continue;
+        const T *x = (useNew ? xw : xr) + (size_t)po * S;
         T U[5], dU[5], n[3], dF[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            U[v] = uA[(size_t)po * 5 + v], dU[v] = du[(size_t)po * 5 + v];
+            U[v] = x[v], dU[v] = x[5 + v];
 #pragma unroll
         for (int r = 0; r < 3; r++)
             n[r] = slot[((size_t)k * 5 + r) * N + p];
@@ -1942,38 +1951,41 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
     const T d = cellRows[5 * N + p];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        incNewA[(size_t)p * 5 + v] = buf[v] / d;
+        xw[(size_t)p * S + 5 + v] = buf[v] / d;
 }
 
-// the state cell-major in colour-major position order
+// the state into both packed arrays in colour-major position order (the increments need no
+// clearing: the first sweep reads none it has not written, see incZero)
 template <typename T>
 __global__ void __launch_bounds__(256) kToCellMajor(idx N, const idx *__restrict__ cellAt,
-                                                    const real *__restrict__ soa, T *__restrict__ aos)
+                                                    const real *__restrict__ soa, T *__restrict__ x0,
+                                                    T *__restrict__ x1)
 {
+    constexpr int S = kPackStride;
     const idx p = blockIdx.x * blockDim.x + threadIdx.x;
     if (p >= N)
         return;
     const idx i = cellAt[p];
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        aos[(size_t)p * 5 + v] = (T)soa[(size_t)v * N + i];
+    {
+        const T a = (T)soa[(size_t)v * N + i];
+        x0[(size_t)p * S + v] = a, x1[(size_t)p * S + v] = a;
+    }
 }
 
 template <typename T>
 void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const real *u, const T *lam,
-                      const T *jd, T *uA, T *slot, idx *nbr, bool fresh, T *inc, T *tmp, int sgsIter, int gsUpdate,
-                      T **result)
+                      const T *jd, T *xbuf, T *slot, idx *nbr, bool fresh, int sgsIter, int gsUpdate, T **result)
 {
     DevMesh &m = c.m;
     const int nb = gridFor(m.N, 256);
     if (fresh)
         kSGSSlotsStatic<T><<<nb, 256, 0, c.stream>>>(m, slot, nbr);
-    kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, m.colorCells, u, uA);
+    T *x0 = xbuf, *x1 = xbuf + (size_t)kPackStride * m.N;
+    kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, m.colorCells, u, x0, x1);
     kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, rhs, jd, slot);
     DNDSR_HIP_CHECK(hipGetLastError());
-    const size_t nU = 5 * (size_t)m.N;
-    DNDSR_HIP_CHECK(hipMemsetAsync(inc, 0, nU * sizeof(T), c.stream));
-    DNDSR_HIP_CHECK(hipMemsetAsync(tmp, 0, nU * sizeof(T), c.stream));
     for (int s = 0; s <= sgsIter; s++)
         for (int dir = 1; dir >= 0; dir--)
         {
@@ -1984,27 +1996,25 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
                 if (e <= b)
                     continue;
                 KernelTimer kt(c, K_SGS);
-                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, uA, inc, tmp,
-                                                                                slot, nbr, dir, gsUpdate, b, e);
+                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(
+                    m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir, gsUpdate, (s == 0 && dir == 1) ? 1 : 0, b, e);
                 DNDSR_HIP_CHECK(hipGetLastError());
             }
-            std::swap(inc, tmp);
+            std::swap(x0, x1);
         }
-    *result = inc;
+    *result = x0 + 5; // the last increment, stride kPackStride
 }
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
-                       const real *jd, real *uA, real *slot, idx *nbr, bool fresh, real *inc, real *tmp, int sgsIter,
-                       int gsUpdate, real **result)
+                       const real *jd, real *xbuf, real *slot, idx *nbr, bool fresh, int sgsIter, int gsUpdate,
+                       real **result)
 {
-    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, fresh, inc, tmp, sgsIter, gsUpdate,
-                           result);
+    launchSGSPackedT<real>(c, gp, alphaDiag, rhs, u, lam, jd, xbuf, slot, nbr, fresh, sgsIter, gsUpdate, result);
 }
 void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
-                       const float *lam, const float *jd, float *uA, float *slot, idx *nbr, bool fresh, float *inc,
-                       float *tmp, int sgsIter, int gsUpdate, float **result)
+                       const float *lam, const float *jd, float *xbuf, float *slot, idx *nbr, bool fresh, int sgsIter,
+                       int gsUpdate, float **result)
 {
-    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, uA, slot, nbr, fresh, inc, tmp, sgsIter, gsUpdate,
-                            result);
+    launchSGSPackedT<float>(c, gp, alphaDiag, rhs, u, lam, jd, xbuf, slot, nbr, fresh, sgsIter, gsUpdate, result);
 }
 
 // ---------------------------------------------------------------------------
@@ -2205,7 +2215,8 @@ __device__ void ppCompressInc(const real *u, real *ret, real gamma, real rhoEps,
     }
 }
 
-// CELLMAJOR: du is [N][5] in colour-major position order (pos = the cell's position)
+// CELLMAJOR: du is the packed sweep's increment, rows of kPackStride in colour-major position
+// order (pos = the cell's position)
 template <typename T, bool CELLMAJOR = false>
 __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real relax, real rhoEps, real pEps,
                                                      real *__restrict__ u, const T *__restrict__ du,
@@ -2218,7 +2229,7 @@ __global__ void __launch_bounds__(256) kAddIncrement(idx N, real gamma, real rel
     const size_t pi = CELLMAJOR ? (size_t)pos[i] : (size_t)i;
 #pragma unroll
     for (int v = 0; v < 5; v++)
-        U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? pi * 5 + v : (size_t)v * N + i];
+        U[v] = u[(size_t)v * N + i], D[v] = (real)du[CELLMAJOR ? pi * kPackStride + v : (size_t)v * N + i];
     bool limited = false, bad = false;
     const real a = ppCellAlpha(U, D, gamma, relax, rhoEps, pEps, limited, bad);
 #pragma unroll

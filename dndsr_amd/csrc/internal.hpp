@@ -282,17 +282,18 @@ void launchToF32(Ctx &c, size_t n, const real *in, float *out);
 // (1 + sgsIter) forward/backward multicolour sweeps from a zero increment with the state
 // gathered from neighbours stored cell-major: uA <- u as [N][5], slot <- per-cell-slot face
 // data; inc / tmp are [N][5] scratch; *result points at the buffer holding the increment.
-// slot: [(K*5 + 7) N] (per-slot face data, then rhs, JDiag and volume in colour-major position
-// order); nbr: [K N]; fresh: slot and nbr are new (their mesh-only part is written); the result
-// (*result) is cell-major in colour-major position order
+// xbuf: [2][N][16] (two packed arrays of state + increment rows); slot: [(K*5 + 7) N] (per-slot
+// face data, then rhs, JDiag and volume in colour-major position order); nbr: [K N]; fresh: slot
+// and nbr are new (their mesh-only part is written); *result: the increment, rows of 16 in
+// colour-major position order
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
-                       const real *jd, real *uA, real *slot, idx *nbr, bool fresh, real *inc, real *tmp, int sgsIter,
-                       int gsUpdate, real **result);
+                       const real *jd, real *xbuf, real *slot, idx *nbr, bool fresh, int sgsIter, int gsUpdate,
+                       real **result);
 void launchSGSPacked32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const real *u,
-                       const float *lam, const float *jd, float *uA, float *slot, idx *nbr, bool fresh, float *inc,
-                       float *tmp, int sgsIter, int gsUpdate, float **result);
+                       const float *lam, const float *jd, float *xbuf, float *slot, idx *nbr, bool fresh, int sgsIter,
+                       int gsUpdate, float **result);
 // u += du with du cell-major ([N][5], FP64 or FP32: exactly one of du64 / du32 non-null)
-// du cell-major in colour-major position order (the packed sweep's increment)
+// du: the packed sweep's increment (rows of 16 in colour-major position order)
 void launchAddIncrementCellMajor(Ctx &c, real gamma, real relax, real rhoEps, real pEps, real *u, const real *du64,
                                  const float *du32, PPStat *stat);
 // ---- launchers (model.hip): ModelEvaluator::EvaluateRHS, scalar advection-diffusion (2-D)
