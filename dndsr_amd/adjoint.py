@@ -58,14 +58,12 @@ def _vr_sweep(vfv, uRec, u, relax):
 
 
 def _require_jacobi_frhs(vfv):
-    """The residual's transpose chains transposed Jacobi sweeps; with SORInstead the solver's
-    frhs sweeps in place (multicolour SOR) and with subs2ndOrder it uses the gradient
-    substitute, neither of which this chain differentiates."""
-    if vfv.settings.get("SORInstead"):
+    """The residual's transpose chains transposed Jacobi sweeps (or, with subs2ndOrder, the
+    transposed second-order substitute, which every sweep then is); with SORInstead the
+    solver's frhs sweeps in place (multicolour SOR), which this chain does not differentiate."""
+    if vfv.settings.get("SORInstead") and not vfv.settings.get("subs2ndOrder"):
         raise ValueError("the residual adjoint differentiates Jacobi VR sweeps; the settings select SORInstead "
                          "(the transpose of the in-place SOR sweep is not provided)")
-    if vfv.settings.get("subs2ndOrder"):
-        raise ValueError("the residual adjoint does not cover the second-order substitute (subs2ndOrder)")
 
 
 class EvaluateRHSFunction(torch.autograd.Function):
@@ -168,8 +166,9 @@ class SteadyAdjoint:
         R(u, uRec; p) = 0          (EvaluateRHS)
         uRec = S(uRec, u; p)       (a converged Jacobi VR sweep),
 
-    where p are boundary values.  For a functional J(u, uRec) the multipliers lam (per cell,
-    5 values) and mu (per reconstruction coefficient) satisfy
+    where p are boundary values (with subs2ndOrder the reconstruction is the substitute
+    uRec = G(u), which is S with dS/duRec = 0).  For a functional J(u, uRec) the multipliers
+    lam (per cell, 5 values) and mu (per reconstruction coefficient) satisfy
 
         mu  = dJ/duRec + (dR/duRec)^T lam + (dS/duRec)^T mu
         0   = dJ/du    + (dR/du)^T lam    + (dS/du)^T mu,
@@ -191,8 +190,6 @@ class SteadyAdjoint:
             block = bool(ev.settings["useRoeJacobian"])
         if block and not ev.settings["useRoeJacobian"]:
             raise ValueError("the block preconditioner needs an evaluator with useRoeJacobian")
-        if ev.vfv.settings.get("subs2ndOrder"):
-            raise ValueError("the steady adjoint needs the VR fixed point; subs2ndOrder replaces it")
         if ev.es.rsType in _READS_DELTA_LAMBDA and ev.es.rsFixScale != 0.0:
             raise ValueError("the flux form reads deltaLambdaCell, which the steady adjoint holds fixed; "
                              "use rsFixScale = 0 or a flux form that does not read it")

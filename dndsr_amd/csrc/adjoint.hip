@@ -778,9 +778,242 @@ static void vrTLaunch(Ctx &c, int nVars, const real *ybar, real *xbar, real *uba
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 
+// ---------------------------------------------------------------------------
+// Transpose of the second-order substitute (kVR2nd): uRec_i = G_i(u) with
+//   Gauss rule:     uRec_i[v][d] = 1/(V_i isc_i) sum_k c_ik[d] (u_k[v] - u_i[v]),
+//                   c_ik = lThis/(lThis + lThat) A n sgn (boundary: A n / 2, u_k the ghost)
+//   least squares:  uRec_i[v][d] = 1/isc_i (S_i^-1 sum_k dc_ik du_ik[v])[d], S_i = sum_k dc_ik dc_ik^T
+// which is linear in u but for the boundary ghosts.  Pass 1 forms per cell the weight of its
+// differences, q_i[v] = recBar_i[v] / (V_i isc_i) or S_i^-T recBar_i[v] / isc_i; pass 2 gathers
+// per cell j: sum over its faces of c_ik . q_i from the neighbour i's side, minus its own
+// c_jk . q_j, plus the ghosts' Jacobian (dual numbers) transposed.
+// ---------------------------------------------------------------------------
+template <int DIM>
+__device__ inline void vr2ndFaceWeight(const DevMesh &m, idx i, int k, int method, bool plain, real *w)
+{
+    const size_t N = m.N, F = m.F;
+    const idx f = m.c2f[(size_t)k * N + i];
+    const idx o = m.c2c[(size_t)k * N + i];
+    const bool back = m.f2c[f] == i;
+    real bi[3], xf[3], n[3], ob[3] = {0, 0, 0};
+    for (int r = 0; r < 3; r++)
+    {
+        bi[r] = m.bary[(size_t)r * N + i];
+        xf[r] = m.fcent[(size_t)r * F + f] + (back ? 0.0 : m.fshift[(size_t)r * F + f]);
+        n[r] = m.fmeanN[(size_t)r * F + f];
+        if (o >= 0)
+            ob[r] = m.bary[(size_t)r * N + o] + (back ? -m.fshift[(size_t)r * F + f] : m.fshift[(size_t)r * F + f]);
+    }
+    const real A = m.farea[f];
+    if (method == 1)
+    {
+        if (o < 0)
+        {
+            for (int d = 0; d < DIM; d++)
+                w[d] = 0.5 * A * n[d];
+            return;
+        }
+        real lThis = 0, lThat = 0;
+        for (int r = 0; r < 3; r++)
+            lThis += (xf[r] - bi[r]) * (xf[r] - bi[r]), lThat += (ob[r] - xf[r]) * (ob[r] - xf[r]);
+        lThis = plain ? 1.0 : sqrt(lThis);
+        lThat = plain ? 1.0 : sqrt(lThat);
+        const real sgn = back ? 1.0 : -1.0;
+        for (int d = 0; d < DIM; d++)
+            w[d] = lThis / (lThis + lThat + kVerySmall) * A * (n[d] * sgn);
+        return;
+    }
+    if (o >= 0)
+        for (int d = 0; d < DIM; d++)
+            w[d] = ob[d] - bi[d];
+    else
+    {
+        real pr = 0;
+        for (int r = 0; r < 3; r++)
+            pr += (bi[r] - m.fcent[(size_t)r * F + f]) * n[r];
+        for (int d = 0; d < DIM; d++)
+            w[d] = pr * -2.0 * n[d];
+    }
+}
+
+template <int DIM, int NV>
+__global__ void __launch_bounds__(64) kVR2ndTq(DevMesh m, const real *__restrict__ ybar, real *__restrict__ q)
+{
+    const idx i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= m.N)
+        return;
+    const size_t N = m.N;
+    const int method = m.s.subs2ndOrder;
+    const bool plain = m.s.subs2ndOrderGGScheme == 0;
+    const real isc = m.iscale[i];
+    real gb[NV][DIM];
+    for (int v = 0; v < NV; v++)
+        for (int d = 0; d < DIM; d++)
+            gb[v][d] = ybar[((size_t)v * DIM + d) * N + i] / isc;
+    if (method == 1)
+    {
+        const real V = m.vol[i];
+        for (int v = 0; v < NV; v++)
+            for (int d = 0; d < DIM; d++)
+                q[((size_t)v * DIM + d) * N + i] = gb[v][d] / V;
+        return;
+    }
+    real S[DIM][DIM];
+    for (int d = 0; d < DIM; d++)
+        for (int e = 0; e < DIM; e++)
+            S[d][e] = 0;
+    for (int k = 0; k < m.K; k++)
+    {
+        if (m.c2f[(size_t)k * N + i] < 0)
+            continue;
+        real w[3];
+        vr2ndFaceWeight<DIM>(m, i, k, method, plain, w);
+        for (int d = 0; d < DIM; d++)
+            for (int e = 0; e < DIM; e++)
+                S[d][e] += w[d] * w[e];
+    }
+    real Si[DIM][DIM];
+    if constexpr (DIM == 2)
+    {
+        const real det = S[0][0] * S[1][1] - S[0][1] * S[1][0];
+        Si[0][0] = S[1][1] / det, Si[0][1] = -S[0][1] / det, Si[1][0] = -S[1][0] / det, Si[1][1] = S[0][0] / det;
+    }
+    else
+    {
+        real C[3][3];
+        C[0][0] = S[1][1] * S[2][2] - S[1][2] * S[2][1];
+        C[0][1] = S[0][2] * S[2][1] - S[0][1] * S[2][2];
+        C[0][2] = S[0][1] * S[1][2] - S[0][2] * S[1][1];
+        C[1][0] = S[1][2] * S[2][0] - S[1][0] * S[2][2];
+        C[1][1] = S[0][0] * S[2][2] - S[0][2] * S[2][0];
+        C[1][2] = S[0][2] * S[1][0] - S[0][0] * S[1][2];
+        C[2][0] = S[1][0] * S[2][1] - S[1][1] * S[2][0];
+        C[2][1] = S[0][1] * S[2][0] - S[0][0] * S[2][1];
+        C[2][2] = S[0][0] * S[1][1] - S[0][1] * S[1][0];
+        const real det = S[0][0] * C[0][0] + S[0][1] * C[1][0] + S[0][2] * C[2][0];
+        for (int d = 0; d < 3; d++)
+            for (int e = 0; e < 3; e++)
+                Si[d][e] = C[d][e] / det;
+    }
+    for (int v = 0; v < NV; v++)
+        for (int e = 0; e < DIM; e++)
+        {
+            real a = 0;
+            for (int d = 0; d < DIM; d++)
+                a += Si[d][e] * gb[v][d];
+            q[((size_t)v * DIM + e) * N + i] = a;
+        }
+}
+
+template <int DIM, int NV>
+__global__ void __launch_bounds__(64) kVR2ndTGather(DevMesh m, const real *__restrict__ u,
+                                                    const real *__restrict__ q, real *__restrict__ ubar)
+{
+    const idx j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.N)
+        return;
+    const size_t N = m.N, F = m.F;
+    const int method = m.s.subs2ndOrder;
+    const bool plain = m.s.subs2ndOrderGGScheme == 0;
+    real qj[NV][DIM], acc[NV];
+    for (int v = 0; v < NV; v++)
+    {
+        acc[v] = 0;
+        for (int d = 0; d < DIM; d++)
+            qj[v][d] = q[((size_t)v * DIM + d) * N + j];
+    }
+    for (int k = 0; k < m.K; k++)
+    {
+        const idx f = m.c2f[(size_t)k * N + j];
+        if (f < 0)
+            continue;
+        const idx o = m.c2c[(size_t)k * N + j];
+        real w[3];
+        vr2ndFaceWeight<DIM>(m, j, k, method, plain, w);
+        real s[NV]; // this face's own term: -c_jk . q_j
+        for (int v = 0; v < NV; v++)
+        {
+            s[v] = 0;
+            for (int d = 0; d < DIM; d++)
+                s[v] += w[d] * qj[v][d];
+            acc[v] -= s[v];
+        }
+        if (o >= 0)
+        {
+            // u_j as the neighbour of o, across o's slot of this face
+            const int ko = m.c2s[(size_t)k * N + j];
+            real wo[3];
+            vr2ndFaceWeight<DIM>(m, o, ko, method, plain, wo);
+            for (int v = 0; v < NV; v++)
+                for (int d = 0; d < DIM; d++)
+                    acc[v] += wo[d] * q[((size_t)v * DIM + d) * N + o];
+        }
+        else if constexpr (NV == 5)
+        {
+            // the ghost of u_j at the face centre (mean normal): its Jacobian transposed
+            const int zone = m.fzone[f];
+            real n[3];
+            for (int r = 0; r < 3; r++)
+                n[r] = m.fmeanN[(size_t)r * F + f];
+            for (int cc = 0; cc < 5; cc++)
+            {
+                Dual U[5], G[5];
+                for (int v = 0; v < 5; v++)
+                    U[v] = Dual(u[(size_t)v * N + j], v == cc ? 1.0 : 0.0);
+                bcGhost(m.bcType[zone], U, n, m.bcVal + (size_t)zone * 5, m.bcGamma, G);
+                real a = 0;
+                for (int v = 0; v < 5; v++)
+                    a += G[v].d * s[v];
+                acc[cc] += a;
+            }
+        }
+    }
+    for (int v = 0; v < NV; v++)
+        ubar[(size_t)v * N + j] = acc[v];
+}
+
 void launchVRIterVJP(Ctx &c, int nVars, int mode, real relax, const real *uRec, const real *u, const real *ybar,
                      real *xbar, real *ubar)
 {
+    if (usesSubs2nd(c.m))
+    {
+        // every mode writes G(u): nothing flows back to uRec; IterDiff (mode 3) is zero
+        KernelTimer kt(c, K_ADJOINT);
+        const DevMesh &m = c.m;
+        const size_t nR = (size_t)nVars * m.M * m.N;
+        if (xbar)
+            DNDSR_HIP_CHECK(hipMemsetAsync(xbar, 0, nR * sizeof(real), c.stream));
+        if (mode == 3 || !ubar)
+            return;
+        if (nVars != 5 && nVars != 1)
+            throw std::runtime_error("nVars must be 1 or 5");
+        if (m.nBnd > 0 && !(nVars == 5 && m.bcType != nullptr && u != nullptr))
+            throw std::runtime_error("the transposed second-order substitute on a mesh with boundary faces needs the "
+                                     "Euler boundary table (nVars 5) and the base state u");
+        if (!c.adj2ndQ)
+            c.adj2ndQ = devAlloc<real>(c, (size_t)5 * 3 * m.N);
+        const int g = gridFor(m.N, 64);
+        if (m.dim == 2)
+        {
+            if (nVars == 5)
+                kVR2ndTq<2, 5><<<g, 64, 0, c.stream>>>(m, ybar, c.adj2ndQ),
+                    kVR2ndTGather<2, 5><<<g, 64, 0, c.stream>>>(m, u, c.adj2ndQ, ubar);
+            else
+                kVR2ndTq<2, 1><<<g, 64, 0, c.stream>>>(m, ybar, c.adj2ndQ),
+                    kVR2ndTGather<2, 1><<<g, 64, 0, c.stream>>>(m, u, c.adj2ndQ, ubar);
+        }
+        else
+        {
+            if (nVars == 5)
+                kVR2ndTq<3, 5><<<g, 64, 0, c.stream>>>(m, ybar, c.adj2ndQ),
+                    kVR2ndTGather<3, 5><<<g, 64, 0, c.stream>>>(m, u, c.adj2ndQ, ubar);
+            else
+                kVR2ndTq<3, 1><<<g, 64, 0, c.stream>>>(m, ybar, c.adj2ndQ),
+                    kVR2ndTGather<3, 1><<<g, 64, 0, c.stream>>>(m, u, c.adj2ndQ, ubar);
+        }
+        DNDSR_HIP_CHECK(hipGetLastError());
+        return;
+    }
     // mode 0 Jacobi (relax), 1 recordInc, 2 recordInc + uRecIsZero, 3 IterDiff
     real c0 = 1, c1 = -1;
     int useB = 1, useb = 1;

@@ -1622,7 +1622,6 @@ int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const d
     need(mode >= 0 && mode <= 3, "mode must be 0, 1, 2 or 3");
     need(mode == 3 || uBar != nullptr, "uBar is required for modes 0-2");
     Ctx &c = ctx->c;
-    need(!usesSubs2nd(c.m), "the transpose of the second-order substitute (subs2ndOrder) is not provided");
     launchVRIterVJP(c, nVars, mode, relax < 0 ? c.m.s.jacobiRelax : relax, uRec, u, uRecNewBar, uRecBar,
                     mode == 3 ? nullptr : uBar);
     CAPI_END

@@ -128,6 +128,7 @@ struct Ctx
     real *adjPoint = nullptr;   // [20 or 50][G F] point sensitivities of the RHS adjoint
     int adjPointRows = 0;
     real *adjFaceBar = nullptr; // [5][F] face-flux adjoint
+    real *adj2ndQ = nullptr;    // [5][3][N] per-cell weights of the transposed second-order substitute
     real *adjRhs = nullptr;     // [5][N] residual of a functional's forward evaluation
     size_t faceLamCap = 0;
     int *nonFinite = nullptr;   // set by the residual gather when a cell's residual is not finite

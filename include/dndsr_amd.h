@@ -387,7 +387,10 @@ int dndsr_zone_force_vjp(dndsr_ctx *ctx, const dndsr_euler_settings *es, const d
  * (uBar unused for mode 3; uRecBar may be NULL).  relax < 0 uses settings.jacobiRelax.
  * uRec, u: the state the forward sweep read; on a mesh with boundary faces the boundary
  * term depends on it (Euler ghosts: both; given point values: u), elsewhere they may be
- * NULL.  Mode 3 covers meshes without boundary faces. */
+ * NULL.  Mode 3 covers meshes without boundary faces.  With the second-order substitute
+ * (subs2ndOrder 1 or 2) every mode writes a zero uRecBar (mode 3 writes nothing else) and
+ * the substitute's transpose into uBar; a bounded mesh then needs nVars 5, the boundary
+ * table and u. */
 int dndsr_vr_iter_vjp(dndsr_ctx *ctx, int nVars, int mode, double relax, const double *uRec, const double *u,
                       const double *uRecNewBar, double *uRecBar, double *uBar);
 

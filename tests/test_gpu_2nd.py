@@ -4,8 +4,8 @@ rule with both face interpolations, 2 = least squares) vs the CPU oracle (FP64; 
 equations and the oracle by Householder QR): bounded meshes with Euler ghost states,
 periodic faces (face centre and barycentre seen from the front cell), nVars 1 on a
 periodic mesh, triangles, mixed cells and tetrahedra; every dndsr_vr_iter mode dispatches
-to it, the Jacobian action is zero, the transpose is refused; SSP-RK3 and implicit steps
-with the substitute."""
+to it, the Jacobian action is zero, the transpose against central differences; SSP-RK3 and
+implicit steps with the substitute."""
 from __future__ import annotations
 
 import numpy as np
@@ -92,13 +92,50 @@ def test_substitute_periodic_one_variable(require_gpu, method):
     assert rel_l2(g.to_host(out), o.vr_2nd(u)) <= TOL[method]
 
 
-def test_substitute_transpose_is_refused(require_gpu):
-    from dndsr_amd.adjoint import vr_vjp
-    m, types, vals = _mesh("2d")
-    o, g, ev = _pair(m, 1, 1, types, vals)
-    du = g.to_device(random_state(o.nCell, seed=2))
-    with pytest.raises(RuntimeError, match="second-order substitute"):
-        vr_vjp(g, g.BuildURec(5), 0, 1.0, g.BuildURec(5), du)
+@pytest.mark.parametrize("kind,method,scheme", CASES, ids=[f"{k}-m{m}-s{s}" for k, m, s in CASES])
+def test_substitute_transpose_matches_central_fd(require_gpu, kind, method, scheme):
+    """The transposed substitute (dndsr_vr_iter_vjp with subs2ndOrder): nothing flows back to
+    uRec, and <ybar, dG/du v> from central differences equals <G^T ybar, v> (relative 1e-6, the
+    bound of the other transposes; the substitute is linear but for the boundary ghosts)."""
+    import torch
+    from dndsr_amd.adjoint import _vr_sweep, vr_vjp
+    m, types, vals = _mesh(kind)
+    o, g, ev = _pair(m, method, scheme, types, vals)
+    N = o.nCell
+    du = g.to_device(random_state(N, seed=2))
+    gen = torch.Generator(device="cpu").manual_seed(5)
+    yb = torch.randn((5, g.M, N), generator=gen, dtype=torch.float64).to(du.device)
+    v = torch.randn(du.shape, generator=gen, dtype=torch.float64).to(du.device) * 0.05
+    rec = g.BuildURec(5)
+    xb, ub = vr_vjp(g, yb, 0, 1.0, rec, du)
+    assert float(xb.abs().max()) == 0.0
+    eps = 1e-6
+    fd = float((yb * (_vr_sweep(g, rec, du + eps * v, 1.0) - _vr_sweep(g, rec, du - eps * v, 1.0))).sum()) / (2 * eps)
+    lhs = float((ub * v).sum())
+    assert abs(lhs - fd) <= 1e-6 * abs(fd), (lhs, fd)
+
+
+@pytest.mark.parametrize("method", [1, 2])
+def test_substitute_transpose_periodic_dot_product(require_gpu, method):
+    """On a periodic mesh with nVars 1 the substitute is linear in u: <ybar, G v> equals
+    <G^T ybar, v> to rounding (relative 1e-12), and the transpose is linear in ybar."""
+    import torch
+    from dndsr_amd.adjoint import _vr_sweep, vr_vjp
+    m = small_mesh("2d")
+    o, g, _ = _pair(m, method, 1)
+    N = o.nCell
+    gen = torch.Generator(device="cpu").manual_seed(6)
+    rec = g.BuildURec(1)
+    v = torch.randn((1, N), generator=gen, dtype=torch.float64).to(rec.device)
+    y1 = torch.randn((1, g.M, N), generator=gen, dtype=torch.float64).to(rec.device)
+    y2 = torch.randn((1, g.M, N), generator=gen, dtype=torch.float64).to(rec.device)
+    Gv = _vr_sweep(g, rec, v, 1.0)
+    _, b1 = vr_vjp(g, y1, 0, 1.0, rec, v)
+    _, b2 = vr_vjp(g, y2, 0, 1.0, rec, v)
+    _, b12 = vr_vjp(g, 2.0 * y1 - y2, 0, 1.0, rec, v)
+    lhs, rhs = float((y1 * Gv).sum()), float((b1 * v).sum())
+    assert abs(lhs - rhs) <= 1e-12 * abs(lhs), (lhs, rhs)
+    assert float((b12 - (2.0 * b1 - b2)).abs().max()) <= 1e-12 * float(b1.abs().max())
 
 
 @pytest.mark.parametrize("method", [1, 2])

@@ -157,7 +157,8 @@ def _oracle_drag(o, es, types, far, u0, farValue):
     return o.zone_force(es, uo, ro, dlc, 0)[0], uo
 
 
-def test_steady_adjoint_drag_gradient_in_far_field_values(require_gpu):
+@pytest.mark.parametrize("subs2nd", [0, 1])
+def test_steady_adjoint_drag_gradient_in_far_field_values(require_gpu, subs2nd):
     import pyoracle as po
     import torch
     from dndsr_amd import bc
@@ -165,7 +166,8 @@ def test_steady_adjoint_drag_gradient_in_far_field_values(require_gpu):
     from dndsr_amd.cfv import VariationalReconstruction
     from dndsr_amd.euler import EulerEvaluator
     m, types, far, gas = _bump_channel()
-    o = po.OracleVR(m, maxOrder=1, intOrder=3, zoneDirWeight=bc.euler_dir_weights(types, 1))
+    vs = dict(maxOrder=1, intOrder=3, subs2ndOrder=subs2nd)
+    o = po.OracleVR(m, **vs, zoneDirWeight=bc.euler_dir_weights(types, 1))
     es = po.OracleVR.euler_settings(GAMMA, RS["Roe"], viscous=1, **gas)
     u0 = np.tile(far, (o.nCell, 1))
     J0, uo = _oracle_drag(o, es, types, far, u0, far)
@@ -177,7 +179,7 @@ def test_steady_adjoint_drag_gradient_in_far_field_values(require_gpu):
         q[k] -= h
         g_fd[k] = (_oracle_drag(o, es, types, far, u0, p)[0] - _oracle_drag(o, es, types, far, u0, q)[0]) / (2 * h)
 
-    g = VariationalReconstruction(m, dict(maxOrder=1, intOrder=3, bcTypes=types)).construct()
+    g = VariationalReconstruction(m, dict(vs, bcTypes=types)).construct()
     ev = EulerEvaluator(g, dict(gamma=GAMMA, riemannSolverType="Roe", bcTypes=types, bcValues=np.stack([far] * 3),
                                 viscous=True, idealGasProperty=gas))
     du, dr = g.to_device(u0), g.BuildURec(5)
