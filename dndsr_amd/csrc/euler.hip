@@ -1954,6 +1954,76 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
         xw[(size_t)p * S + 5 + v] = buf[v] / d;
 }
 
+// The same pass with the slot count K fixed at compile time: all K neighbour positions are
+// loaded first and then all K neighbour rows, so a thread waits for two memory round trips
+// instead of two per slot (a colour pass is about one round of waves on the device, so the
+// chain of dependent loads per thread is its run time).  Same arithmetic in the same slot
+// order as kSGSColourPacked: bit-identical results.
+template <typename T, int K>
+constexpr int sgsPackedWaves()
+{
+    return sizeof(T) == 8 ? (K <= 4 ? 4 : 3) : 1; // FP64: the rows of 4 (6) slots in 128 (168) VGPRs
+}
+template <typename T, int K>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sgsPackedWaves<T, K>())))
+kSGSColourPackedK(DevMesh m, real gammaD, T alphaDiag,
+                                                         const T *__restrict__ xr, T *__restrict__ xw,
+                                                         const T *__restrict__ slot, const idx *__restrict__ nbr,
+                                                         int forward, int gsUpdate, int incZero, idx begin, idx end)
+{
+    constexpr int S = kPackStride;
+    const idx p = begin + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= end)
+        return;
+    const size_t N = m.N;
+    const T gamma = (T)gammaD;
+    const T *__restrict__ cellRows = slot + (size_t)K * 5 * N;
+    idx po[K];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+        po[k] = nbr[(size_t)k * N + p];
+    // the gathers are unconditional (a slot that is not used reads the cell's own row, a valid
+    // address): a load under a branch makes the compiler wait for every earlier load at the join
+    bool use[K];
+    T U[K][5], dU[K][5];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        const bool earlier = forward ? po[k] < p : po[k] > p;
+        const bool useNew = gsUpdate && earlier;
+        use[k] = po[k] >= 0 && !(incZero && !useNew); // the first sweep starts from a zero increment
+        const T *x = use[k] ? (useNew ? xw : xr) + (size_t)po[k] * S : xr + (size_t)p * S;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            U[k][v] = x[v], dU[k][v] = x[5 + v];
+    }
+    T buf[5];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        buf[v] = cellRows[(size_t)v * N + p];
+    const T voli = cellRows[6 * N + p];
+#pragma unroll
+    for (int k = 0; k < K; k++)
+    {
+        if (!use[k])
+            continue;
+        T n[3], dF[5];
+#pragma unroll
+        for (int r = 0; r < 3; r++)
+            n[r] = slot[((size_t)k * 5 + r) * N + p];
+        const T area = slot[((size_t)k * 5 + 3) * N + p], lamf = slot[((size_t)k * 5 + 4) * N + p];
+        fluxJacDu<T>(U[k], n, dU[k], lamf, gamma, dF);
+        T coef = (T(0.5) * alphaDiag) * area / voli;
+#pragma unroll
+        for (int v = 0; v < 5; v++)
+            buf[v] -= coef * dF[v];
+    }
+    const T d = cellRows[5 * N + p];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        xw[(size_t)p * S + 5 + v] = buf[v] / d;
+}
+
 // the state into both packed arrays in colour-major position order (the increments need no
 // clearing: the first sweep reads none it has not written, see incZero)
 template <typename T>
@@ -1996,8 +2066,19 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
                 if (e <= b)
                     continue;
                 KernelTimer kt(c, K_SGS);
-                kSGSColourPacked<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(
-                    m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir, gsUpdate, (s == 0 && dir == 1) ? 1 : 0, b, e);
+                const int g = gridFor(e - b, 256), z = (s == 0 && dir == 1) ? 1 : 0;
+                if (m.K == 3)
+                    kSGSColourPackedK<T, 3><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
+                                                                     gsUpdate, z, b, e);
+                else if (m.K == 4)
+                    kSGSColourPackedK<T, 4><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
+                                                                     gsUpdate, z, b, e);
+                else if (m.K == 6)
+                    kSGSColourPackedK<T, 6><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
+                                                                     gsUpdate, z, b, e);
+                else
+                    kSGSColourPacked<T><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
+                                                                 gsUpdate, z, b, e);
                 DNDSR_HIP_CHECK(hipGetLastError());
             }
             std::swap(x0, x1);
