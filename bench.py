@@ -377,8 +377,12 @@ def laminar_kernel_rooflines(g, run, iters, sgsIter):
         per_it = lambda nm: k.get(nm, (0.0, 0))[0] / iters  # noqa: E731
         rhs_ms = per_it("rhs_face") + per_it("rhs_viscous") + per_it("rhs_gather")
         vr_ms = per_it("vr_iter")
+        # a symmetric sweep after the first skips its first colour (the previous sweep's
+        # last, whose increments it would repeat): time per full pass over the mesh
         sweeps = 2 * (1 + sgsIter)
-        sgs_ms = per_it("sgs_colour") / sweeps
+        cs = np.bincount(np.asarray(g.colors()))
+        processed = sweeps * N - (1 + sgsIter) * int(cs[-1]) - sgsIter * int(cs[0])
+        sgs_ms = per_it("sgs_colour") * N / processed
         s_ = 4 if fp32 else 8
         b_sgs = 8 * (5 * 5 + 1) + f_c * (4 + s_ * (2 + 3)) + s_ * (5 * 5 + 1)
         ent = {}
@@ -388,6 +392,7 @@ def laminar_kernel_rooflines(g, run, iters, sgsIter):
                 ent[key] = {"avg_ms": round(t, 4), "bytes_per_cell": round(b, 1), "achieved": round(gbs, 1),
                             "frac": round(gbs / HBM_PEAK_GBS, 4)}
         ent["passes_ms_per_iteration"] = {nm: round(v[0] / iters, 4) for nm, v in k.items()}
+        ent["sgs_cells_per_iteration"] = {"processed": processed, "full_sweeps": sweeps * N}
         res[name] = ent
     return res
 

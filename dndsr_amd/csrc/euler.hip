@@ -1906,11 +1906,32 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
 // read and written alternately by the sweeps
 constexpr int kPackStride = 16;
 
+// Skipped colour passes (Gauss-Seidel updates only).  A symmetric sweep after the first
+// begins with the colour the previous sweep ended with; that pass would read the same
+// neighbour increments as the previous one and write the same values, so it is skipped,
+// and the positions [fLo, fHi) of that colour keep their current increments in the array
+// the sweep reads (xr) instead of the one it writes (xw).  The colour the previous sweep
+// skipped is this sweep's last: until its own pass, its current increments are in xw,
+// positions [lLo, lHi).  A neighbour in the first range is read from xr, in the second
+// from xw, any other as usual (xw when already updated in this sweep, else xr).
+struct SweepSkip
+{
+    idx fLo = 0, fHi = 0, lLo = 0, lHi = 0;
+};
+__device__ __forceinline__ bool sgsReadNew(const SweepSkip &sk, idx po, bool useNew)
+{
+    if (po >= sk.fLo && po < sk.fHi)
+        return false;
+    if (po >= sk.lLo && po < sk.lHi)
+        return true;
+    return useNew;
+}
+
 template <typename T>
 __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, T alphaDiag, const T *__restrict__ xr,
                                                         T *__restrict__ xw, const T *__restrict__ slot,
                                                         const idx *__restrict__ nbr, int forward, int gsUpdate,
-                                                        int incZero, idx begin, idx end)
+                                                        int incZero, idx begin, idx end, SweepSkip sk)
 {
     constexpr int S = kPackStride;
     const idx p = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1933,7 +1954,7 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
         const bool useNew = gsUpdate && earlier;
         if (incZero && !useNew) // the first sweep starts from a zero increment
             continue;
-        const T *x = (useNew ? xw : xr) + (size_t)po * S;
+        const T *x = (sgsReadNew(sk, po, useNew) ? xw : xr) + (size_t)po * S;
         T U[5], dU[5], n[3], dF[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
@@ -1969,7 +1990,8 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(sgsPac
 kSGSColourPackedK(DevMesh m, real gammaD, T alphaDiag,
                                                          const T *__restrict__ xr, T *__restrict__ xw,
                                                          const T *__restrict__ slot, const idx *__restrict__ nbr,
-                                                         int forward, int gsUpdate, int incZero, idx begin, idx end)
+                                                         int forward, int gsUpdate, int incZero, idx begin, idx end,
+                                                         SweepSkip sk)
 {
     constexpr int S = kPackStride;
     const idx p = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1992,7 +2014,7 @@ kSGSColourPackedK(DevMesh m, real gammaD, T alphaDiag,
         const bool earlier = forward ? po[k] < p : po[k] > p;
         const bool useNew = gsUpdate && earlier;
         use[k] = po[k] >= 0 && !(incZero && !useNew); // the first sweep starts from a zero increment
-        const T *x = use[k] ? (useNew ? xw : xr) + (size_t)po[k] * S : xr + (size_t)p * S;
+        const T *x = use[k] ? (sgsReadNew(sk, po[k], useNew) ? xw : xr) + (size_t)po[k] * S : xr + (size_t)p * S;
 #pragma unroll
         for (int v = 0; v < 5; v++)
             U[k][v] = x[v], dU[k][v] = x[5 + v];
@@ -2044,6 +2066,19 @@ __global__ void __launch_bounds__(256) kToCellMajor(idx N, const idx *__restrict
     }
 }
 
+// the increments of positions [lo, hi) from one packed array into the other
+template <typename T>
+__global__ void __launch_bounds__(256) kCopyIncRows(idx lo, idx hi, const T *__restrict__ from, T *__restrict__ to)
+{
+    constexpr int S = kPackStride;
+    const idx p = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= hi)
+        return;
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        to[(size_t)p * S + 5 + v] = from[(size_t)p * S + 5 + v];
+}
+
 template <typename T>
 void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const real *u, const T *lam,
                       const T *jd, T *xbuf, T *slot, idx *nbr, bool fresh, int sgsIter, int gsUpdate, T **result)
@@ -2056,12 +2091,22 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
     kToCellMajor<T><<<nb, 256, 0, c.stream>>>(m.N, m.colorCells, u, x0, x1);
     kSGSSlots<T><<<nb, 256, 0, c.stream>>>(m, lam, rhs, jd, slot);
     DNDSR_HIP_CHECK(hipGetLastError());
+    const int nC = m.nColor;
+    const bool canSkip = gsUpdate && nC >= 2;
+    int prevSkip = -1; // the colour whose pass the previous sweep skipped
     for (int s = 0; s <= sgsIter; s++)
         for (int dir = 1; dir >= 0; dir--)
         {
-            for (int q = 0; q < m.nColor; q++)
+            const int first = dir ? 0 : nC - 1, last = dir ? nC - 1 : 0;
+            const bool skipFirst = canSkip && !(s == 0 && dir == 1);
+            SweepSkip sk;
+            if (skipFirst)
+                sk.fLo = m.colorPtrHost[first], sk.fHi = m.colorPtrHost[first + 1];
+            if (prevSkip == last)
+                sk.lLo = m.colorPtrHost[last], sk.lHi = m.colorPtrHost[last + 1];
+            for (int q = skipFirst ? 1 : 0; q < nC; q++)
             {
-                const int col = dir ? q : m.nColor - 1 - q;
+                const int col = dir ? q : nC - 1 - q;
                 const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
                 if (e <= b)
                     continue;
@@ -2069,20 +2114,33 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
                 const int g = gridFor(e - b, 256), z = (s == 0 && dir == 1) ? 1 : 0;
                 if (m.K == 3)
                     kSGSColourPackedK<T, 3><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
-                                                                     gsUpdate, z, b, e);
+                                                                     gsUpdate, z, b, e, sk);
                 else if (m.K == 4)
                     kSGSColourPackedK<T, 4><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
-                                                                     gsUpdate, z, b, e);
+                                                                     gsUpdate, z, b, e, sk);
                 else if (m.K == 6)
                     kSGSColourPackedK<T, 6><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
-                                                                     gsUpdate, z, b, e);
+                                                                     gsUpdate, z, b, e, sk);
                 else
                     kSGSColourPacked<T><<<g, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, x0, x1, slot, nbr, dir,
-                                                                 gsUpdate, z, b, e);
+                                                                 gsUpdate, z, b, e, sk);
                 DNDSR_HIP_CHECK(hipGetLastError());
             }
+            prevSkip = skipFirst ? first : -1;
             std::swap(x0, x1);
         }
+    if (prevSkip >= 0)
+    {
+        // the last sweep skipped its first colour: its increments are still in the array that
+        // sweep read (now x1)
+        const idx b = m.colorPtrHost[prevSkip], e = m.colorPtrHost[prevSkip + 1];
+        if (e > b)
+        {
+            KernelTimer kt(c, K_SGS);
+            kCopyIncRows<T><<<gridFor(e - b, 256), 256, 0, c.stream>>>(b, e, x1, x0);
+            DNDSR_HIP_CHECK(hipGetLastError());
+        }
+    }
     *result = x0 + 5; // the last increment, stride kPackStride
 }
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
