@@ -88,6 +88,8 @@ SIGNATURES = {
                                C.c_int, C.c_int, vp],
     "dndsr_update_sgs_block_transpose": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, vp,
                                          C.c_int, C.c_int, C.c_int, vp],
+    "dndsr_sgs_transpose_symmetric": [vp, C.POINTER(EulerSettings), C.c_double, vp, vp, vp, vp, vp, vp, C.c_int,
+                                      C.c_int, vp, vp],
     "dndsr_update_sgs_block_f32": [vp, C.POINTER(EulerSettings), C.c_float, vp, vp, vp, vp, vp, vp, vp, C.c_int,
                                    C.c_int, C.c_int, vp],
     "dndsr_ssprk3": [vp, C.POINTER(EulerSettings), vp, vp, C.c_int, C.c_double, C.c_int, C.c_double],

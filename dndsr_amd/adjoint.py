@@ -212,13 +212,8 @@ class SteadyAdjoint:
         return jd
 
     def _precond(self, r, u, jd):
-        ev = self.ev
-        a, b = torch.zeros_like(r), torch.zeros_like(r)
-        for _ in range(self.sgsIter + 1):
-            for forward in (True, False):
-                ev.UpdateSGSTranspose(1.0, r, u, a, b, jd, forward, self.gsUpdate)
-                a, b = b, a
-        return a
+        # (1 + sgsIter) symmetric transposed sweeps from zero, in one call
+        return self.ev.SGSTransposeSymmetric(1.0, r.contiguous(), u, jd, self.sgsIter + 1, self.gsUpdate)
 
     def _apply(self, u, uRec, lam, mu):
         """The transposed steady system: ((dR/du)^T lam + (dS/du)^T mu,

@@ -1088,6 +1088,64 @@ int dndsr_update_sgs_block_transpose(dndsr_ctx *ctx, const dndsr_euler_settings 
     CAPI_END
 }
 
+// nSweep symmetric sweeps of the transposed LU-SGS matrix (scalar with jd, block with jinv)
+// from zero increments, ping-ponging between a and b; with Gauss-Seidel updates a sweep after
+// the first skips its first colour, whose pass would repeat the previous sweep's last one (same
+// inputs, same values): that sweep reads the colour from the array it reads, the next sweep
+// reads it from the array it writes until the colour's own pass, and one copy after the last
+// sweep puts the skipped colour's values into the result.
+static real *sgsTransposeSymmetric(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
+                                   const real *jd, const real *jinv, const real *lam, const real *lamRoe, int nSweep,
+                                   int gsUpdate, real *a, real *b)
+{
+    DevMesh &m = c.m;
+    const size_t nU = (size_t)5 * m.N;
+    real *sumDev = c.red + 5 * ((size_t)gridFor(m.N, 256) + m.nColor + 6);
+    DNDSR_HIP_CHECK(hipMemsetAsync(a, 0, nU * sizeof(real), c.stream));
+    DNDSR_HIP_CHECK(hipMemsetAsync(b, 0, nU * sizeof(real), c.stream));
+    const int nC = m.nColor;
+    const bool canSkip = gsUpdate && nC >= 2;
+    int prevSkip = -1;
+    for (int s = 0; s < nSweep; s++)
+        for (int dir = 1; dir >= 0; dir--)
+        {
+            const int first = dir ? 0 : nC - 1, last = dir ? nC - 1 : 0;
+            const int fCol = canSkip && !(s == 0 && dir == 1) ? first : -1;
+            const int lCol = prevSkip == last ? last : -1;
+            if (jinv)
+                launchSGSSweepBlockT64(c, gp, alphaDiag, rhs, u, a, b, jinv, lam, lamRoe, dir, gsUpdate, 0, sumDev,
+                                       fCol, lCol);
+            else
+                launchSGSSweepT64(c, gp, alphaDiag, rhs, u, a, b, jd, lam, dir, gsUpdate, 0, sumDev, fCol, lCol);
+            prevSkip = fCol;
+            std::swap(a, b);
+        }
+    if (prevSkip >= 0)
+        launchCopyColourInc64(c, prevSkip, b, a);
+    return a;
+}
+
+int dndsr_sgs_transpose_symmetric(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag,
+                                  const double *rhs, const double *u, const double *jdiag, const double *jdiagInv,
+                                  const double *lambdaFace, const double *lamRoe, int nSweep, int gsUpdate,
+                                  double *work, double *out)
+{
+    CAPI_BEGIN
+    requireMesh(ctx);
+    need(rhs && u && lambdaFace && work && out, "null field");
+    need((jdiag != nullptr) != (jdiagInv != nullptr), "pass exactly one of jdiag (scalar) and jdiagInv (block)");
+    need(!jdiagInv || lamRoe, "the block sweeps need the Roe face eigenvalues (lamRoe)");
+    need(work != out && work != rhs && out != rhs, "work, out and rhs must be different arrays");
+    need(nSweep >= 1, "nSweep must be at least 1");
+    Ctx &c = ctx->c;
+    real *res = sgsTransposeSymmetric(c, toGas(es), alphaDiag, rhs, u, jdiag, jdiagInv, lambdaFace, lamRoe, nSweep,
+                                      gsUpdate, out, work);
+    if (res != out)
+        DNDSR_HIP_CHECK(hipMemcpyAsync(out, res, (size_t)5 * c.m.N * sizeof(real), hipMemcpyDeviceToDevice,
+                                       c.stream));
+    CAPI_END
+}
+
 int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                                const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
                                const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,
@@ -1360,28 +1418,20 @@ static void implicitStep(dndsr_ctx *ctx, const dndsr_euler_settings *es, const d
                 float *a = ctx->fInc, *b = ctx->fTmp;
                 DNDSR_HIP_CHECK(hipMemsetAsync(a, 0, nU * sizeof(float), c.stream));
                 DNDSR_HIP_CHECK(hipMemsetAsync(b, 0, nU * sizeof(float), c.stream));
-                for (int s = 0; s <= is->sgsIter; s++)
-                    for (int dir = 1; dir >= 0; dir--)
-                    {
-                        launchSGSSweepBlock32(c, gp, (float)alphaDiag, ctx->fRhs, ctx->fU, a, b, ctx->bJinvF, ctx->fLam,
-                                              ctx->bLamRoeF, dir, is->gsUpdate, 0, sumDev);
-                        std::swap(a, b);
-                    }
-                launchAddIncrement32(c, gamma, is->incrementRelax, rhoEps, pEps, u, a, ctx->iStat);
+                float *res = nullptr;
+                launchSGSBlockSymmetric32(c, gp, (float)alphaDiag, ctx->fRhs, ctx->fU, a, b, ctx->bJinvF, ctx->fLam,
+                                          ctx->bLamRoeF, is->sgsIter, is->gsUpdate, sumDev, &res);
+                launchAddIncrement32(c, gamma, is->incrementRelax, rhoEps, pEps, u, res, ctx->iStat);
             }
             else
             {
                 real *a = ctx->iInc, *b = ctx->iTmp;
                 DNDSR_HIP_CHECK(hipMemsetAsync(a, 0, nU * sizeof(real), c.stream));
                 DNDSR_HIP_CHECK(hipMemsetAsync(b, 0, nU * sizeof(real), c.stream));
-                for (int s = 0; s <= is->sgsIter; s++)
-                    for (int dir = 1; dir >= 0; dir--)
-                    {
-                        launchSGSSweepBlock64(c, gp, alphaDiag, ctx->iRhs, u, a, b, ctx->bJinv, ctx->wLam, ctx->bLamRoe,
-                                              dir, is->gsUpdate, 0, sumDev);
-                        std::swap(a, b);
-                    }
-                launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, a, ctx->iStat);
+                real *res = nullptr;
+                launchSGSBlockSymmetric64(c, gp, alphaDiag, ctx->iRhs, u, a, b, ctx->bJinv, ctx->wLam, ctx->bLamRoe,
+                                          is->sgsIter, is->gsUpdate, sumDev, &res);
+                launchAddIncrement64(c, gamma, is->incrementRelax, rhoEps, pEps, u, res, ctx->iStat);
             }
         }
         else if (is->gmresCode != 0)

@@ -1460,7 +1460,8 @@ __global__ void __launch_bounds__(256) kSGSColourT(DevMesh m, real gammaD, T alp
                                                    const T *__restrict__ u, const T *__restrict__ uInc,
                                                    T *__restrict__ uIncNew, const T *__restrict__ jd,
                                                    const T *__restrict__ lam, int forward, int gsUpdate,
-                                                   int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+                                                   int uIncIsZero, idx begin, idx end, real *__restrict__ part,
+                                                   int fCol, int lCol)
 {
     __shared__ real sm[5][256];
     const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1488,7 +1489,13 @@ __global__ void __launch_bounds__(256) kSGSColourT(DevMesh m, real gammaD, T alp
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            const T *y = useNew ? uIncNew : uInc;
+            bool readNew = useNew; // skipped colour passes, see sgsTransposeSymmetric
+            if (fCol >= 0 || lCol >= 0)
+            {
+                const int co = m.color[o];
+                readNew = co == fCol ? false : (co == lCol ? true : useNew);
+            }
+            const T *y = readNew ? uIncNew : uInc;
             T yj[5], n[3];
 #pragma unroll
             for (int v = 0; v < 5; v++)
@@ -1597,20 +1604,22 @@ static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, con
 
 void launchSGSSweepT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                        real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
-                       int uIncIsZero, real *sumIncDev)
+                       int uIncIsZero, real *sumIncDev, int fCol, int lCol)
 {
     DevMesh &m = c.m;
     size_t off = 0;
     for (int s = 0; s < m.nColor; s++)
     {
         const int col = forward ? s : m.nColor - 1 - s;
+        if (col == fCol)
+            continue;
         const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
         if (e <= b)
             continue;
         const int nb = gridFor(e - b, 256);
         KernelTimer kt(c, K_ADJOINT);
         kSGSColourT<real><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace,
-                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5, fCol, lCol);
         DNDSR_HIP_CHECK(hipGetLastError());
         off += nb;
     }

@@ -228,7 +228,7 @@ void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const re
 // a sweep of the transposed scalar LU-SGS matrix (the adjoint preconditioner), FP64
 void launchSGSSweepT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                        real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
-                       int uIncIsZero, real *sumIncDev);
+                       int uIncIsZero, real *sumIncDev, int fCol = -1, int lCol = -1);
 void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
                       int uIncIsZero, real *sumIncDev);
@@ -241,12 +241,20 @@ void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float 
 void launchFaceRoeLambda(Ctx &c, const GasParams &gp, const real *u, const real *uRec, const real *dlc, real *lamRoe);
 void launchLUSGSInitBlock(Ctx &c, const GasParams &gp, const real *dTau, real dt, real alphaDiag, const real *u,
                           const real *lam, const real *lamRoe, real *jd, real *jinv, int *bad);
+void launchSGSBlockSymmetric64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, real *a,
+                               real *b, const real *jinv, const real *lam, const real *lamRoe, int sgsIter,
+                               int gsUpdate, real *sumIncDev, real **result);
+void launchSGSBlockSymmetric32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
+                               float *a, float *b, const float *jinv, const float *lam, const float *lamRoe,
+                               int sgsIter, int gsUpdate, real *sumIncDev, float **result);
 void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
                            const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
 void launchSGSSweepBlockT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
                             const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
-                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);
+                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev, int fCol = -1,
+                            int lCol = -1);
+void launchCopyColourInc64(Ctx &c, int col, const real *from, real *to);
 void launchSGSSweepBlock32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
                            const float *uInc, float *uIncNew, const float *jinv, const float *lam,
                            const float *lamRoe, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev);

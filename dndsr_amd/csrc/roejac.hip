@@ -463,7 +463,7 @@ __global__ void __launch_bounds__(256) kSGSColourBlock(DevMesh m, real gammaD, T
                                                        T *__restrict__ uIncNew, const T *__restrict__ jinv,
                                                        const T *__restrict__ lam, const T *__restrict__ lamRoe,
                                                        int forward, int gsUpdate, int uIncIsZero, idx begin, idx end,
-                                                       real *__restrict__ part)
+                                                       real *__restrict__ part, int fCol, int lCol)
 {
     __shared__ real sm[5][256];
     const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -491,7 +491,15 @@ __global__ void __launch_bounds__(256) kSGSColourBlock(DevMesh m, real gammaD, T
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            const T *du = useNew ? uIncNew : uInc;
+            // a skipped colour pass (fCol) keeps its increments in uInc, the colour the previous
+            // sweep skipped (lCol) in uIncNew until its own pass
+            bool readNew = useNew;
+            if (fCol >= 0 || lCol >= 0)
+            {
+                const int co = m.color[o];
+                readNew = co == fCol ? false : (co == lCol ? true : useNew);
+            }
+            const T *du = readNew ? uIncNew : uInc;
             const bool front = (fl & 1) != 0;
             T U[5], dU[5], n[3], dF[5], inc[5], l0, l123, l4;
 #pragma unroll
@@ -543,7 +551,7 @@ __global__ void __launch_bounds__(256) kSGSColourBlock(DevMesh m, real gammaD, T
 template <typename T>
 static void sgsSweepBlock(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const T *u, const T *uInc,
                           T *uIncNew, const T *jinv, const T *lam, const T *lamRoe, int forward, int gsUpdate,
-                          int uIncIsZero, real *sumIncDev)
+                          int uIncIsZero, real *sumIncDev, int fCol = -1, int lCol = -1)
 {
     DevMesh &m = c.m;
     const int nc = m.nColor;
@@ -551,13 +559,15 @@ static void sgsSweepBlock(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs
     for (int s = 0; s < nc; s++)
     {
         const int col = forward ? s : nc - 1 - s;
+        if (col == fCol)
+            continue;
         const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
         if (e <= b)
             continue;
         const int nb = gridFor(e - b, 256);
         KernelTimer kt(c, K_SGS);
         kSGSColourBlock<T><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jinv, lam, lamRoe,
-                                                    forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+                                                    forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5, fCol, lCol);
         DNDSR_HIP_CHECK(hipGetLastError());
         off += nb;
     }
@@ -573,7 +583,8 @@ __global__ void __launch_bounds__(256) kSGSColourBlockT(DevMesh m, real gamma, r
                                                         const real *__restrict__ uInc, real *__restrict__ uIncNew,
                                                         const real *__restrict__ jinv, const real *__restrict__ lam,
                                                         const real *__restrict__ lamRoe, int forward, int gsUpdate,
-                                                        int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+                                                        int uIncIsZero, idx begin, idx end, real *__restrict__ part,
+                                                        int fCol, int lCol)
 {
     __shared__ real sm[5][256];
     const idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -597,7 +608,13 @@ __global__ void __launch_bounds__(256) kSGSColourBlockT(DevMesh m, real gamma, r
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            const real *y = useNew ? uIncNew : uInc;
+            bool readNew = useNew; // skipped colour passes, see sgsTransposeSymmetric
+            if (fCol >= 0 || lCol >= 0)
+            {
+                const int co = m.color[o];
+                readNew = co == fCol ? false : (co == lCol ? true : useNew);
+            }
+            const real *y = readNew ? uIncNew : uInc;
             const bool frontJ = (fl & 1) == 0; // the neighbour is the front cell when i is the back
             real U[5], yj[5], n[3], l0, l123, l4;
 #pragma unroll
@@ -656,7 +673,7 @@ __global__ void __launch_bounds__(256) kSGSColourBlockT(DevMesh m, real gamma, r
 
 void launchSGSSweepBlockT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,
                             const real *uInc, real *uIncNew, const real *jinv, const real *lam, const real *lamRoe,
-                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev)
+                            int forward, int gsUpdate, int uIncIsZero, real *sumIncDev, int fCol, int lCol)
 {
     DevMesh &m = c.m;
     const int nc = m.nColor;
@@ -664,17 +681,93 @@ void launchSGSSweepBlockT64(Ctx &c, const GasParams &gp, real alphaDiag, const r
     for (int s = 0; s < nc; s++)
     {
         const int col = forward ? s : nc - 1 - s;
+        if (col == fCol)
+            continue;
         const idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
         if (e <= b)
             continue;
         const int nb = gridFor(e - b, 256);
         KernelTimer kt(c, K_ADJOINT);
         kSGSColourBlockT<<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jinv, lam, lamRoe,
-                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+                                                   forward, gsUpdate, uIncIsZero, b, e, c.red + off * 5, fCol, lCol);
         DNDSR_HIP_CHECK(hipGetLastError());
         off += nb;
     }
     launchReduceSum5(c, (int)off, sumIncDev);
+}
+
+// the increments of the cells of colour col from one array into the other
+template <typename T>
+__global__ void __launch_bounds__(256) kCopyColourInc(DevMesh m, idx lo, idx hi, const T *__restrict__ from,
+                                                      T *__restrict__ to)
+{
+    const idx p = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= hi)
+        return;
+    const size_t N = m.N;
+    const idx i = m.colorCells[p];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        to[(size_t)v * N + i] = from[(size_t)v * N + i];
+}
+
+// (1 + sgsIter) symmetric block sweeps from the increments in a (b: the other array), the
+// result in *result.  With Gauss-Seidel updates a sweep after the first skips its first
+// colour, whose pass would repeat the previous sweep's last one (same inputs, same values);
+// the sweep reads that colour's increments from the array it reads, and the next sweep the
+// colour from the array it writes until that colour's own pass; one copy after the last
+// sweep puts the skipped colour's increments into the result.
+template <typename T>
+static void sgsBlockSymmetric(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const T *u, T *a, T *b,
+                              const T *jinv, const T *lam, const T *lamRoe, int sgsIter, int gsUpdate,
+                              real *sumIncDev, T **result)
+{
+    DevMesh &m = c.m;
+    const int nC = m.nColor;
+    const bool canSkip = gsUpdate && nC >= 2;
+    int prevSkip = -1;
+    for (int s = 0; s <= sgsIter; s++)
+        for (int dir = 1; dir >= 0; dir--)
+        {
+            const int first = dir ? 0 : nC - 1, last = dir ? nC - 1 : 0;
+            const int fCol = canSkip && !(s == 0 && dir == 1) ? first : -1;
+            const int lCol = prevSkip == last ? last : -1;
+            sgsSweepBlock<T>(c, gp, alphaDiag, rhs, u, a, b, jinv, lam, lamRoe, dir, gsUpdate, 0, sumIncDev, fCol,
+                             lCol);
+            prevSkip = fCol;
+            std::swap(a, b);
+        }
+    if (prevSkip >= 0)
+    {
+        const idx lo = m.colorPtrHost[prevSkip], hi = m.colorPtrHost[prevSkip + 1];
+        if (hi > lo)
+        {
+            KernelTimer kt(c, K_SGS);
+            kCopyColourInc<T><<<gridFor(hi - lo, 256), 256, 0, c.stream>>>(m, lo, hi, b, a);
+            DNDSR_HIP_CHECK(hipGetLastError());
+        }
+    }
+    *result = a;
+}
+void launchCopyColourInc64(Ctx &c, int col, const real *from, real *to)
+{
+    const idx lo = c.m.colorPtrHost[col], hi = c.m.colorPtrHost[col + 1];
+    if (hi <= lo)
+        return;
+    kCopyColourInc<real><<<gridFor(hi - lo, 256), 256, 0, c.stream>>>(c.m, lo, hi, from, to);
+    DNDSR_HIP_CHECK(hipGetLastError());
+}
+void launchSGSBlockSymmetric64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, real *a,
+                               real *b, const real *jinv, const real *lam, const real *lamRoe, int sgsIter,
+                               int gsUpdate, real *sumIncDev, real **result)
+{
+    sgsBlockSymmetric<real>(c, gp, alphaDiag, rhs, u, a, b, jinv, lam, lamRoe, sgsIter, gsUpdate, sumIncDev, result);
+}
+void launchSGSBlockSymmetric32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u,
+                               float *a, float *b, const float *jinv, const float *lam, const float *lamRoe,
+                               int sgsIter, int gsUpdate, real *sumIncDev, float **result)
+{
+    sgsBlockSymmetric<float>(c, gp, alphaDiag, rhs, u, a, b, jinv, lam, lamRoe, sgsIter, gsUpdate, sumIncDev, result);
 }
 
 void launchSGSSweepBlock64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u,

@@ -211,6 +211,19 @@ class EulerEvaluator:
                                                s.ctypes.data_as(_lib.dp)))
         return s
 
+    def SGSTransposeSymmetric(self, alphaDiag, rhs, u, JDiag, nSweep, gsUpdate, out=None, work=None):
+        """nSweep symmetric sweeps of the transposed LU-SGS matrix from zero increments in one call
+        (dndsr_sgs_transpose_symmetric): the result of nSweep forward/backward pairs of
+        UpdateSGSTranspose, without a host synchronisation per sweep.  Returns out."""
+        out = torch.empty_like(rhs) if out is None else out
+        work = torch.empty_like(rhs) if work is None else work
+        blk = isinstance(JDiag, JacobianDiagBlock)
+        check(lib().dndsr_sgs_transpose_symmetric(
+            self.ctx.h, C.byref(self.es), float(alphaDiag), _ptr(rhs), _ptr(u), None if blk else _ptr(JDiag),
+            _ptr(JDiag.invert) if blk else None, _ptr(self.lambdaFace), _ptr(self.lambdaFaceRoe) if blk else None,
+            int(nSweep), int(gsUpdate), _ptr(work), _ptr(out)))
+        return out
+
     def SetBCValues(self, bcValues):
         """Replaces the per-zone boundary states (5 conservative values per zone) of the
         evaluator's boundary set; the zone types stay."""

@@ -282,6 +282,17 @@ int dndsr_update_sgs_block_transpose(dndsr_ctx *ctx, const dndsr_euler_settings 
                                      const double *rhs, const double *u, const double *uInc, double *uIncNew,
                                      const double *jdiagInv, const double *lambdaFace, const double *lamRoe,
                                      int forward, int gsUpdate, int uIncIsZero, double *sumIncHost);
+/* nSweep symmetric (forward then backward) sweeps of the transposed LU-SGS matrix from zero
+ * increments into out: scalar with jdiag (dndsr_update_sgs_transpose), block with jdiagInv and
+ * lamRoe (dndsr_update_sgs_block_transpose); pass exactly one of jdiag and jdiagInv.  work: a
+ * second [5][N] array.  With gsUpdate a sweep after the first skips the colour pass that would
+ * repeat the previous sweep's last one: the result equals that of the single sweeps.  No
+ * counterpart in the reference: the preconditioner of the discrete-adjoint solve in one call,
+ * without a host synchronisation per sweep. */
+int dndsr_sgs_transpose_symmetric(dndsr_ctx *ctx, const dndsr_euler_settings *es, double alphaDiag,
+                                  const double *rhs, const double *u, const double *jdiag, const double *jdiagInv,
+                                  const double *lambdaFace, const double *lamRoe, int nSweep, int gsUpdate,
+                                  double *work, double *out);
 int dndsr_update_sgs_block_f32(dndsr_ctx *ctx, const dndsr_euler_settings *es, float alphaDiag, const float *rhs,
                                const float *u, const float *uInc, float *uIncNew, const float *jdiagInv,
                                const float *lambdaFace, const float *lamRoe, int forward, int gsUpdate,

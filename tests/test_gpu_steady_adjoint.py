@@ -3,6 +3,8 @@
 * The transposed scalar LU-SGS sweep (dndsr_update_sgs_transpose), the adjoint's
   preconditioner, against the oracle's sweep in the device's colour order (1e-12); the
   oracle's sweep is pinned as a sweep of the assembled transposed LU-SGS matrix on the CPU.
+  The symmetric multi-sweep call (dndsr_sgs_transpose_symmetric, scalar and block) equals
+  the chain of single sweeps bitwise and the oracle's chain to 1e-12.
 * The gradient of the wall drag of a converged viscous bump-channel flow with respect to the
   five far-field values: SteadyAdjoint's multipliers against central differences of the drag
   of oracle steady states converged to 1e-13 at perturbed far-field values (1e-6 relative).
@@ -40,6 +42,26 @@ def _sweep_case(kind):
     return o, g, ev
 
 
+def _check_symmetric(ev, g, du, jd, rhs, gsUpdate, oracle_sweep):
+    """SGSTransposeSymmetric (one call, colour passes that repeat the previous sweep's last
+    one skipped under gsUpdate) equals the chain of single transposed sweeps from zero to the
+    last bit, and the oracle's chain in the device's colour order to 1e-12."""
+    import torch
+    N = rhs.shape[0]
+    r = g.to_device(rhs)
+    for nS in (1, 3):
+        a, b = torch.zeros_like(r), torch.zeros_like(r)
+        y = np.zeros((N, 5))
+        for _ in range(nS):
+            for fw in (True, False):
+                ev.UpdateSGSTranspose(1.0, r, du, a, b, jd, fw, gsUpdate)
+                a, b = b, a
+                y = oracle_sweep(y, fw)
+        out = ev.SGSTransposeSymmetric(1.0, r, du, jd, nS, gsUpdate)
+        assert torch.equal(out, a), (nS, float((out - a).abs().max()))
+        assert rel_l2(g.to_host(out), y) <= 1e-12, nS
+
+
 @pytest.mark.parametrize("kind", ["periodic", "bounded"])
 @pytest.mark.parametrize("gsUpdate", [0, 1])
 def test_transposed_sweep_matches_oracle(require_gpu, kind, gsUpdate):
@@ -69,6 +91,8 @@ def test_transposed_sweep_matches_oracle(require_gpu, kind, gsUpdate):
                                               zero, scanOrder=order)
             assert rel_l2(g.to_host(out), ref) <= 1e-12, (forward, zero)
             np.testing.assert_allclose(s, s_o, rtol=1e-11)
+    _check_symmetric(ev, g, du, jd, rhs, gsUpdate, lambda y, fw: o.update_sgs_transpose(
+        es, 1.0, rhs, u, y, np.zeros((N, 5)), jd_o, lam_o, fw, gsUpdate, False, scanOrder=order)[0])
 
 
 @pytest.mark.parametrize("kind", ["periodic", "bounded"])
@@ -125,6 +149,8 @@ def test_transposed_block_sweep_matches_oracle(require_gpu, kind, gsUpdate):
                                                     forward, gsUpdate, zero, scanOrder=order)
             assert rel_l2(g.to_host(out), ref) <= 1e-12, (forward, zero)
             np.testing.assert_allclose(s, s_o, rtol=1e-11)
+    _check_symmetric(ev, g, du, jd, rhs, gsUpdate, lambda y, fw: o.update_sgs_block_transpose(
+        es, 1.0, rhs, u, y, np.zeros((N, 5)), ji_o, lam_o, lr_o, fw, gsUpdate, False, scanOrder=order)[0])
 
 
 def _bump_channel():
