@@ -1376,7 +1376,8 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
                                                   const T *__restrict__ u, const T *__restrict__ uInc,
                                                   T *__restrict__ uIncNew, const T *__restrict__ jd,
                                                   const T *__restrict__ lam, int col, int forward, int gsUpdate,
-                                                  int uIncIsZero, idx begin, idx end, real *__restrict__ part)
+                                                  int uIncIsZero, idx begin, idx end, real *__restrict__ part,
+                                                  int fCol, int lCol)
 {
     __shared__ real sm[5][256];
     idx t = begin + blockIdx.x * blockDim.x + threadIdx.x;
@@ -1405,7 +1406,13 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            const T *du = useNew ? uIncNew : uInc;
+            bool readNew = useNew; // skipped colour passes, see sgsSymmetric
+            if (fCol >= 0 || lCol >= 0)
+            {
+                const int co = m.color[o];
+                readNew = co == fCol ? false : (co == lCol ? true : useNew);
+            }
+            const T *du = readNew ? uIncNew : uInc;
             T U[5], dU[5], n[3], dF[5];
 #pragma unroll
             for (int v = 0; v < 5; v++)
@@ -1580,7 +1587,8 @@ void launchReduceSum5(Ctx &c, int nPart, real *out)
 
 template <typename T>
 static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const T *u, const T *uInc, T *uIncNew,
-                     const T *jd, const T *lam, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev)
+                     const T *jd, const T *lam, int forward, int gsUpdate, int uIncIsZero, real *sumIncDev,
+                     int fCol = -1, int lCol = -1)
 {
     DevMesh &m = c.m;
     int nc = m.nColor;
@@ -1588,13 +1596,15 @@ static void sgsSweep(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, con
     for (int s = 0; s < nc; s++)
     {
         int col = forward ? s : nc - 1 - s;
+        if (col == fCol)
+            continue;
         idx b = m.colorPtrHost[col], e = m.colorPtrHost[col + 1];
         if (e <= b)
             continue;
         int nb = gridFor(e - b, 256);
         KernelTimer kt(c, K_SGS);
         kSGSColour<T><<<nb, 256, 0, c.stream>>>(m, gp.gamma, alphaDiag, rhs, u, uInc, uIncNew, jd, lam, col, forward,
-                                               gsUpdate, uIncIsZero, b, e, c.red + off * 5);
+                                               gsUpdate, uIncIsZero, b, e, c.red + off * 5, fCol, lCol);
         DNDSR_HIP_CHECK(hipGetLastError());
         off += nb;
     }
@@ -1640,6 +1650,67 @@ void launchSGSSweep32(Ctx &c, const GasParams &gp, float alphaDiag, const float 
 {
     sgsSweep<float>(c, gp, alphaDiag, rhs, u, uInc, uIncNew, jdiag, lambdaFace, forward, gsUpdate, uIncIsZero,
                     sumIncDev);
+}
+
+// the increments of the cells of one colour from one array into the other ([5][N])
+template <typename T>
+__global__ void __launch_bounds__(256) kCopyColourIncS(DevMesh m, idx lo, idx hi, const T *__restrict__ from,
+                                                       T *__restrict__ to)
+{
+    const idx p = lo + blockIdx.x * blockDim.x + threadIdx.x;
+    if (p >= hi)
+        return;
+    const size_t N = m.N;
+    const idx i = m.colorCells[p];
+#pragma unroll
+    for (int v = 0; v < 5; v++)
+        to[(size_t)v * N + i] = from[(size_t)v * N + i];
+}
+
+// nSweep symmetric scalar sweeps from the increments in a (b: the other array); returns the
+// array holding the result.  With Gauss-Seidel updates a sweep after the first skips its first
+// colour, whose pass would repeat the previous sweep's last one (same inputs, same values): the
+// sweep reads that colour from the array it reads, the next sweep reads it from the array it
+// writes until the colour's own pass, and one copy after the last sweep completes the result.
+template <typename T>
+static T *sgsSymmetric(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, const T *u, T *a, T *b, const T *jd,
+                       const T *lam, int nSweep, int gsUpdate, real *sumIncDev)
+{
+    DevMesh &m = c.m;
+    const int nC = m.nColor;
+    const bool canSkip = gsUpdate && nC >= 2;
+    int prevSkip = -1;
+    for (int s = 0; s < nSweep; s++)
+        for (int dir = 1; dir >= 0; dir--)
+        {
+            const int first = dir ? 0 : nC - 1, last = dir ? nC - 1 : 0;
+            const int fCol = canSkip && !(s == 0 && dir == 1) ? first : -1;
+            const int lCol = prevSkip == last ? last : -1;
+            sgsSweep<T>(c, gp, alphaDiag, rhs, u, a, b, jd, lam, dir, gsUpdate, 0, sumIncDev, fCol, lCol);
+            prevSkip = fCol;
+            std::swap(a, b);
+        }
+    if (prevSkip >= 0)
+    {
+        const idx lo = m.colorPtrHost[prevSkip], hi = m.colorPtrHost[prevSkip + 1];
+        if (hi > lo)
+        {
+            KernelTimer kt(c, K_SGS);
+            kCopyColourIncS<T><<<gridFor(hi - lo, 256), 256, 0, c.stream>>>(m, lo, hi, b, a);
+            DNDSR_HIP_CHECK(hipGetLastError());
+        }
+    }
+    return a;
+}
+real *launchSGSSymmetric64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, real *a,
+                           real *b, const real *jd, const real *lam, int nSweep, int gsUpdate, real *sumIncDev)
+{
+    return sgsSymmetric<real>(c, gp, alphaDiag, rhs, u, a, b, jd, lam, nSweep, gsUpdate, sumIncDev);
+}
+float *launchSGSSymmetric32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u, float *a,
+                            float *b, const float *jd, const float *lam, int nSweep, int gsUpdate, real *sumIncDev)
+{
+    return sgsSymmetric<float>(c, gp, alphaDiag, rhs, u, a, b, jd, lam, nSweep, gsUpdate, sumIncDev);
 }
 
 // out = a x + b y + c (z * dt); dt read from device memory (1 when null) so that a

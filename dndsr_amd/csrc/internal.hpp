@@ -229,6 +229,10 @@ void launchLUSGSInit(Ctx &c, const real *dTau, real dt, real alphaDiag, const re
 void launchSGSSweepT64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                        real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
                        int uIncIsZero, real *sumIncDev, int fCol = -1, int lCol = -1);
+real *launchSGSSymmetric64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, real *a,
+                           real *b, const real *jd, const real *lam, int nSweep, int gsUpdate, real *sumIncDev);
+float *launchSGSSymmetric32(Ctx &c, const GasParams &gp, float alphaDiag, const float *rhs, const float *u, float *a,
+                            float *b, const float *jd, const float *lam, int nSweep, int gsUpdate, real *sumIncDev);
 void launchSGSSweep64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *uInc,
                       real *uIncNew, const real *jdiag, const real *lambdaFace, int forward, int gsUpdate,
                       int uIncIsZero, real *sumIncDev);

@@ -214,14 +214,8 @@ void applyLUSGSPrecond(Ctx &c, const GasParams &gp, const LinWork &w, const real
         launchToF32(c, nU, r, w.fR);
         DNDSR_HIP_CHECK(hipMemsetAsync(w.fX, 0, nU * sizeof(float), c.stream));
         DNDSR_HIP_CHECK(hipMemsetAsync(w.fT, 0, nU * sizeof(float), c.stream));
-        float *x = w.fX, *t = w.fT;
-        for (int s = 0; s <= w.sgsIter; s++)
-            for (int dir = 1; dir >= 0; dir--)
-            {
-                launchSGSSweep32(c, gp, (float)w.alphaDiag, w.fR, w.fU, x, t, w.fJd, w.fLam, dir, w.gsUpdate, 0,
-                                 sumDev);
-                std::swap(x, t);
-            }
+        const float *x = launchSGSSymmetric32(c, gp, (float)w.alphaDiag, w.fR, w.fU, w.fX, w.fT, w.fJd, w.fLam,
+                                              w.sgsIter + 1, w.gsUpdate, sumDev);
         kToF64<<<blocksFor(nU), 256, 0, c.stream>>>(nU, x, out);
         DNDSR_HIP_CHECK(hipGetLastError());
     }
@@ -229,14 +223,9 @@ void applyLUSGSPrecond(Ctx &c, const GasParams &gp, const LinWork &w, const real
     {
         DNDSR_HIP_CHECK(hipMemsetAsync(out, 0, nU * sizeof(real), c.stream));
         DNDSR_HIP_CHECK(hipMemsetAsync(w.dT, 0, nU * sizeof(real), c.stream));
-        real *x = out, *t = w.dT;
-        for (int s = 0; s <= w.sgsIter; s++)
-            for (int dir = 1; dir >= 0; dir--)
-            {
-                launchSGSSweep64(c, gp, w.alphaDiag, r, w.u, x, t, w.jd, w.lam, dir, w.gsUpdate, 0, sumDev);
-                std::swap(x, t);
-            }
-        if (x != out) // an odd number of sweeps leaves the result in the scratch array
+        const real *x = launchSGSSymmetric64(c, gp, w.alphaDiag, r, w.u, out, w.dT, w.jd, w.lam, w.sgsIter + 1,
+                                             w.gsUpdate, sumDev);
+        if (x != out) // the result may be left in the scratch array
             launchCopy(c, nU, x, out);
     }
 }

@@ -4,6 +4,8 @@
 * GMRES against a dense solve of the same operator, assembled column by column from the
   oracle's matvec, with FP64 and FP32 preconditioner sweeps (the Krylov space and the
   operator stay FP64, so both reach the FP64 solution);
+* one Krylov vector from zero is alpha M b with M the chain of single symmetric sweeps
+  (1e-12): the preconditioner's skipped colour passes change nothing;
 * the implicit dual-time step with gmresCode 1 / 2 reaches the same converged state as
   the LU-SGS path (<= 1e-10).
 """
@@ -72,6 +74,34 @@ def test_gmres_solves_the_lusgs_system(require_gpu, fp32):
     assert rel_l2(xs, xTrue) <= (1e-5 if fp32 else 1e-8)
     assert np.linalg.norm(A @ xs.ravel() - b.ravel()) <= (1e-5 if fp32 else 1e-8) * np.linalg.norm(b)
 
+
+@pytest.mark.parametrize("sgsIter", [0, 2])
+@pytest.mark.parametrize("kind", ["2d", "3d"])
+def test_gmres_one_krylov_vector_is_the_preconditioned_step(require_gpu, kind, sgsIter):
+    """The preconditioner M of GMRES, (1 + sgsIter) symmetric Gauss-Seidel sweeps from zero (its
+    colour passes that repeat the previous sweep's last one skipped), equals the chain of
+    single UpdateSGS sweeps: one Krylov vector from x = 0 gives x = alpha M b with
+    alpha = <M A M b, M b> / |M A M b|^2 (1e-12)."""
+    import torch
+    o, g, ev, es, u, lam, jd, du, jdd = _system(kind)
+    b = g.to_device(np.random.default_rng(9).uniform(-1, 1, (o.nCell, 5)))
+
+    def precond(r):
+        a, c = torch.zeros_like(r), torch.zeros_like(r)
+        for _ in range(sgsIter + 1):
+            for fw in (True, False):
+                ev.UpdateSGS(1.0, 0.0, r, du, a, c, jdd, fw, True)
+                a, c = c, a
+        return a
+
+    Mb = precond(b)
+    AMb = torch.empty_like(Mb)
+    ev.LUSGSMatrixVec(1.0, 0.0, du, Mb, jdd, AMb)
+    MAMb = precond(AMb)
+    alpha = float((MAMb * Mb).sum()) / float((MAMb * MAMb).sum())
+    x = torch.zeros_like(du)
+    ev.GMRES(1.0, du, jdd, b, x, nGmresSpace=1, nGmresIter=1, sgsIter=sgsIter)
+    assert rel_l2(g.to_host(x), g.to_host(alpha * Mb)) <= 1e-12
 
 @pytest.mark.parametrize("gmresCode,fp32", [(1, False), (2, True)])
 def test_implicit_step_with_gmres_converges_to_lusgs_state(require_gpu, gmresCode, fp32):
