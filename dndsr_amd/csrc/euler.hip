@@ -1981,10 +1981,12 @@ __global__ void __launch_bounds__(256) kSGSSlots(DevMesh m, const T *__restrict_
     cellRows[5 * N + p] = jd[i];
 }
 
-// the packed state: per position a 128-byte row (FP64; 64 bytes FP32) holding the state U
-// (5) and the increment (5), so that a neighbour costs one cache line; two such arrays,
-// read and written alternately by the sweeps
-constexpr int kPackStride = 16;
+// the packed state: per position a 128-byte row (FP64; 64 bytes FP32) holding the increment
+// (5, at kPackInc) and the state U (5, at kPackU), so that a neighbour costs one cache line;
+// two such arrays, read and written alternately by the sweeps.  Each half of a row is written
+// whole (the increment or the state and three zeros), so no write covers part of a 32-byte
+// sector.
+constexpr int kPackStride = 16, kPackInc = 0, kPackU = 8;
 
 // Skipped colour passes (Gauss-Seidel updates only).  A symmetric sweep after the first
 // begins with the colour the previous sweep ended with; that pass would read the same
@@ -2038,7 +2040,7 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
         T U[5], dU[5], n[3], dF[5];
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            U[v] = x[v], dU[v] = x[5 + v];
+            U[v] = x[kPackU + v], dU[v] = x[kPackInc + v];
 #pragma unroll
         for (int r = 0; r < 3; r++)
             n[r] = slot[((size_t)k * 5 + r) * N + p];
@@ -2051,8 +2053,8 @@ __global__ void __launch_bounds__(256) kSGSColourPacked(DevMesh m, real gammaD, 
     }
     const T d = cellRows[5 * N + p];
 #pragma unroll
-    for (int v = 0; v < 5; v++)
-        xw[(size_t)p * S + 5 + v] = buf[v] / d;
+    for (int v = 0; v < 8; v++)
+        xw[(size_t)p * S + kPackInc + v] = v < 5 ? buf[v] / d : T(0);
 }
 
 // The same pass with the slot count K fixed at compile time: all K neighbour positions are
@@ -2097,7 +2099,7 @@ kSGSColourPackedK(DevMesh m, real gammaD, T alphaDiag,
         const T *x = use[k] ? (sgsReadNew(sk, po[k], useNew) ? xw : xr) + (size_t)po[k] * S : xr + (size_t)p * S;
 #pragma unroll
         for (int v = 0; v < 5; v++)
-            U[k][v] = x[v], dU[k][v] = x[5 + v];
+            U[k][v] = x[kPackU + v], dU[k][v] = x[kPackInc + v];
     }
     T buf[5];
 #pragma unroll
@@ -2122,8 +2124,8 @@ kSGSColourPackedK(DevMesh m, real gammaD, T alphaDiag,
     }
     const T d = cellRows[5 * N + p];
 #pragma unroll
-    for (int v = 0; v < 5; v++)
-        xw[(size_t)p * S + 5 + v] = buf[v] / d;
+    for (int v = 0; v < 8; v++)
+        xw[(size_t)p * S + kPackInc + v] = v < 5 ? buf[v] / d : T(0);
 }
 
 // the state into both packed arrays in colour-major position order (the increments need no
@@ -2139,10 +2141,10 @@ __global__ void __launch_bounds__(256) kToCellMajor(idx N, const idx *__restrict
         return;
     const idx i = cellAt[p];
 #pragma unroll
-    for (int v = 0; v < 5; v++)
+    for (int v = 0; v < 8; v++)
     {
-        const T a = (T)soa[(size_t)v * N + i];
-        x0[(size_t)p * S + v] = a, x1[(size_t)p * S + v] = a;
+        const T a = v < 5 ? (T)soa[(size_t)v * N + i] : T(0);
+        x0[(size_t)p * S + kPackU + v] = a, x1[(size_t)p * S + kPackU + v] = a;
     }
 }
 
@@ -2155,8 +2157,8 @@ __global__ void __launch_bounds__(256) kCopyIncRows(idx lo, idx hi, const T *__r
     if (p >= hi)
         return;
 #pragma unroll
-    for (int v = 0; v < 5; v++)
-        to[(size_t)p * S + 5 + v] = from[(size_t)p * S + 5 + v];
+    for (int v = 0; v < 8; v++)
+        to[(size_t)p * S + kPackInc + v] = from[(size_t)p * S + kPackInc + v];
 }
 
 template <typename T>
@@ -2221,7 +2223,7 @@ void launchSGSPackedT(Ctx &c, const GasParams &gp, T alphaDiag, const T *rhs, co
             DNDSR_HIP_CHECK(hipGetLastError());
         }
     }
-    *result = x0 + 5; // the last increment, stride kPackStride
+    *result = x0 + kPackInc; // the last increment, stride kPackStride
 }
 void launchSGSPacked64(Ctx &c, const GasParams &gp, real alphaDiag, const real *rhs, const real *u, const real *lam,
                        const real *jd, real *xbuf, real *slot, idx *nbr, bool fresh, int sgsIter, int gsUpdate,
