@@ -322,3 +322,39 @@ def test_shape_gradient_of_drag_matches_oracle_fd(require_gpu, method, block):
     err = np.abs(gS - g_fd).max() / np.abs(g_fd).max()
     print(f"shape gradient ({method}, block {block}, {nIt} iterations): max rel error vs oracle FD {err:.3e}")
     assert err <= 1e-6, (gS, g_fd)
+
+
+def test_symmetric_transposed_sweeps_reject_bad_arguments(require_gpu):
+    """dndsr_sgs_transpose_symmetric: exactly one of the scalar and block Jacobians, nSweep >= 1,
+    distinct work / out / rhs arrays, lamRoe with the block Jacobian; each refusal names its
+    cause and leaves out untouched."""
+    import ctypes as C
+
+    import torch
+    from dndsr_amd import _lib
+    o, g, ev = _sweep_case("bounded")
+    N = o.nCell
+    du = g.to_device(random_state(N, seed=4))
+    dTau = g.BuildUDof(1)[0]
+    ev.EvaluateDt(dTau, du, None, CFL=2.0)
+    jd = torch.empty_like(dTau)
+    ev.LUSGSMatrixInit(jd, None, dTau, 0.7, 1.0)
+    rhs = g.to_device(np.random.default_rng(3).normal(size=(N, 5)))
+    out, work = torch.full_like(rhs, 7.0), torch.zeros_like(rhs)
+    L = _lib.lib()
+    p = lambda t: C.c_void_p(t.data_ptr()) if t is not None else None  # noqa: E731
+
+    def call(jdiag, jinv, lamRoe, nSweep, w, o_):
+        return L.dndsr_sgs_transpose_symmetric(ev.ctx.h, C.byref(ev.es), 1.0, p(rhs), p(du), p(jdiag), p(jinv),
+                                               p(ev.lambdaFace), p(lamRoe), nSweep, 1, p(w), p(o_))
+
+    cases = [((jd, jd, ev.lambdaFaceRoe, 2, work, out), "exactly one of jdiag"),
+             ((None, None, None, 2, work, out), "exactly one of jdiag"),
+             ((jd, None, None, 0, work, out), "nSweep must be at least 1"),
+             ((jd, None, None, 2, out, out), "different arrays"),
+             ((None, jd, None, 2, work, out), "lamRoe")]
+    for args, msg in cases:
+        assert call(*args) != 0
+        assert msg in L.dndsr_last_error().decode(), (msg, L.dndsr_last_error())
+    assert bool((out == 7.0).all())
+    assert call(jd, None, None, 2, work, out) == 0
