@@ -438,6 +438,10 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
                 uint8_t fl = hf2c[f] == (idx)i ? 0 : 1;
                 if (o >= 0 && col[o] < col[i])
                     fl |= 2;
+                if (o >= 0 && col[o] == 0)
+                    fl |= 4;
+                if (o >= 0 && col[o] == nColor - 1)
+                    fl |= 8;
                 hfl[(size_t)k * N + i] = fl;
             }
         m.c2flag = devAlloc<uint8_t>(c, hfl.size());

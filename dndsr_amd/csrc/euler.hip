@@ -1406,12 +1406,7 @@ __global__ void __launch_bounds__(256) kSGSColour(DevMesh m, real gammaD, T alph
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            bool readNew = useNew; // skipped colour passes, see sgsSymmetric
-            if (fCol >= 0 || lCol >= 0)
-            {
-                const int co = m.color[o];
-                readNew = co == fCol ? false : (co == lCol ? true : useNew);
-            }
+            const bool readNew = sweepReadNew(fl, fCol, lCol, m.nColor, useNew);
             const T *du = readNew ? uIncNew : uInc;
             T U[5], dU[5], n[3], dF[5];
 #pragma unroll
@@ -1496,12 +1491,7 @@ __global__ void __launch_bounds__(256) kSGSColourT(DevMesh m, real gammaD, T alp
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            bool readNew = useNew; // skipped colour passes, see sgsTransposeSymmetric
-            if (fCol >= 0 || lCol >= 0)
-            {
-                const int co = m.color[o];
-                readNew = co == fCol ? false : (co == lCol ? true : useNew);
-            }
+            const bool readNew = sweepReadNew(fl, fCol, lCol, m.nColor, useNew);
             const T *y = readNew ? uIncNew : uInc;
             T yj[5], n[3];
 #pragma unroll

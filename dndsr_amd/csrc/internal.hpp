@@ -62,7 +62,8 @@ struct DevMesh
     idx *c2c = nullptr;     // [K][N] cell across face slot k, -1 = absent
     int32_t *c2s = nullptr; // [K][N] slot of the same face in that neighbour cell
     uint8_t *c2flag = nullptr; // [K][N] bit 0: the cell is the face's front (R) side;
-                               //        bit 1: the neighbour's colour precedes the cell's
+                               //        bit 1: the neighbour's colour precedes the cell's;
+                               //        bits 2, 3: the neighbour is in the first, last colour
     float *fareaF = nullptr, *fmeanNF = nullptr, *volF = nullptr; // FP32 copies for the FP32 sweeps
     idx *cell2node = nullptr; // [nnc][N] (nnc = the most nodes of a cell type present)
     idx *face2node = nullptr; // [nnf][F]
@@ -354,5 +355,22 @@ T *devAlloc(Ctx &c, size_t n)
 }
 
 inline int gridFor(idx n, int block) { return (int)((n + block - 1) / block); }
+
+// Skipped colour passes of the symmetric Gauss-Seidel sweeps: a sweep may skip its first colour
+// (fCol; its current values stay in the array the sweep reads) and read the colour the previous
+// sweep skipped (lCol) from the array it writes.  Both are the first or the last colour, which
+// the slot flags mark (bits 2, 3); -1 names none.
+__device__ __forceinline__ bool nbrInColour(uint8_t fl, int col, int nColor)
+{
+    return (col == 0 && (fl & 4)) || (col >= 0 && col == nColor - 1 && (fl & 8));
+}
+__device__ __forceinline__ bool sweepReadNew(uint8_t fl, int fCol, int lCol, int nColor, bool useNew)
+{
+    if (nbrInColour(fl, fCol, nColor))
+        return false;
+    if (nbrInColour(fl, lCol, nColor))
+        return true;
+    return useNew;
+}
 
 } // namespace dndsr

@@ -493,12 +493,7 @@ __global__ void __launch_bounds__(256) kSGSColourBlock(DevMesh m, real gammaD, T
                 continue;
             // a skipped colour pass (fCol) keeps its increments in uInc, the colour the previous
             // sweep skipped (lCol) in uIncNew until its own pass
-            bool readNew = useNew;
-            if (fCol >= 0 || lCol >= 0)
-            {
-                const int co = m.color[o];
-                readNew = co == fCol ? false : (co == lCol ? true : useNew);
-            }
+            const bool readNew = sweepReadNew(fl, fCol, lCol, m.nColor, useNew);
             const T *du = readNew ? uIncNew : uInc;
             const bool front = (fl & 1) != 0;
             T U[5], dU[5], n[3], dF[5], inc[5], l0, l123, l4;
@@ -608,12 +603,7 @@ __global__ void __launch_bounds__(256) kSGSColourBlockT(DevMesh m, real gamma, r
             const bool useNew = gsUpdate && earlier;
             if (uIncIsZero && !useNew)
                 continue;
-            bool readNew = useNew; // skipped colour passes, see sgsTransposeSymmetric
-            if (fCol >= 0 || lCol >= 0)
-            {
-                const int co = m.color[o];
-                readNew = co == fCol ? false : (co == lCol ? true : useNew);
-            }
+            const bool readNew = sweepReadNew(fl, fCol, lCol, m.nColor, useNew);
             const real *y = readNew ? uIncNew : uInc;
             const bool frontJ = (fl & 1) == 0; // the neighbour is the front cell when i is the back
             real U[5], yj[5], n[3], l0, l123, l4;
