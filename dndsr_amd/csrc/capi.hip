@@ -504,6 +504,24 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
         m.nBndCell = (idx)bcells.size();
         m.bndCell = devAlloc<idx>(c, bcells.size());
         upload(c, m.bndCell, bcells);
+        // per boundary cell and face slot: the slot's face in the boundary-face list, -1 elsewhere
+        std::vector<int32_t> tOf(F, -1);
+        for (size_t t = 0; t < hbnd.size(); t++)
+            tOf[hbnd[t]] = (int32_t)t;
+        const size_t nbc = bcells.size();
+        std::vector<int32_t> bs((size_t)K * nbc, -1);
+        for (size_t j = 0; j < nbc; j++)
+            for (int k = 0; k < K; k++)
+            {
+                const idx f = hc2f[(size_t)k * N + bcells[j]];
+                if (f >= 0 && hc2c[(size_t)k * N + bcells[j]] < 0)
+                {
+                    need(tOf[f] >= 0, "a face without a neighbour cell is missing from the boundary faces");
+                    bs[(size_t)k * nbc + j] = tOf[f];
+                }
+            }
+        m.bndSlot = devAlloc<int32_t>(c, bs.size());
+        upload(c, m.bndSlot, bs);
     }
     if (m.nBCZone > 0)
     {
@@ -533,6 +551,8 @@ int dndsr_vr_construct(dndsr_ctx *ctx, const dndsr_mesh_desc *md, const dndsr_vr
     m.AinvB = devAlloc<real>(c, (size_t)(K + 1) * m.M * m.M * N);
     m.Ainvb = devAlloc<real>(c, (size_t)K * m.M * N);
     c.faceFlux = devAlloc<real>(c, 5 * F);
+    if (m.nBnd > 0)
+        c.vrBnd = devAlloc<real>(c, (size_t)5 * m.M * m.nBnd);
     c.faceLamTmp = devAlloc<real>(c, F);
     size_t nPart = (size_t)gridFor(m.N, 256) + (size_t)nColor + 8;
     c.red = devAlloc<real>(c, nPart * 5);

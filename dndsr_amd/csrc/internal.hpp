@@ -104,6 +104,7 @@ struct DevMesh
     int32_t *fzone = nullptr;   // [F] zone of a boundary face, -1 elsewhere
     idx *bndFace = nullptr;     // [nBnd]
     idx *bndCell = nullptr;     // [nBndCell] cells with at least one boundary face
+    int32_t *bndSlot = nullptr; // [K][nBndCell] index in bndFace of the face at slot k of bndCell[j], -1 elsewhere
     real *zoneW = nullptr;      // [nBCZone][P+1] boundary direction weights (setup only)
     // boundary-condition table (dndsr_set_bc) and VR boundary point values (dndsr_set_vr_dirichlet)
     int32_t *bcType = nullptr;  // [nBCZone]
@@ -121,6 +122,7 @@ struct Ctx
     bool haveMesh = false;
     // scratch owned by the context (grown on demand, freed at destroy)
     real *faceFlux = nullptr;   // [5][F]
+    real *vrBnd = nullptr;      // [nVars][M][nBnd] A^-1 BCC of each boundary face (VR sweep boundary term)
     size_t faceFluxCap = 0;
     real *red = nullptr;        // reduction partials
     size_t redCap = 0;

@@ -310,16 +310,22 @@ __device__ inline void boundaryFaceBCC(const DevMesh &m, idx f, const real *mom,
     }
 }
 
+// The boundary term in two passes.  kVRBndFace: one thread per boundary face, A^-1 BCC of the
+// face (BCC from the owner cell's block, A^-1 = the cell's slot-0 block) into vrBnd; kVRBndCell:
+// one thread per boundary cell adds scale times its faces' terms in slot order.  (One thread per
+// boundary cell doing both would run, in every wave, each slot that any lane has on the
+// boundary.)
 template <int DIM, int NDOF, int NV, int MODE>
-__global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restrict__ uRec, const real *__restrict__ u,
-                                                  const real *__restrict__ x, real *__restrict__ uRecNew, real scale)
+__global__ void __launch_bounds__(64) kVRBndFace(DevMesh m, const real *__restrict__ uRec, const real *__restrict__ u,
+                                                 const real *__restrict__ x, real *__restrict__ out)
 {
     constexpr int M = NDOF - 1;
     const idx t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= m.nBndCell)
+    if (t >= m.nBnd)
         return;
-    const idx i = m.bndCell[t];
-    const size_t N = m.N;
+    const idx f = m.bndFace[t];
+    const idx i = m.f2c[f];
+    const size_t N = m.N, NB = m.nBnd;
     real mom[M], cen[3], um[NV];
 #pragma unroll
     for (int j = 0; j < M; j++)
@@ -333,24 +339,40 @@ __global__ void __launch_bounds__(64) kVRBoundary(DevMesh m, const real *__restr
         um[v] = u ? u[(size_t)v * N + i] : 0.0;
     auto rec = [&](int v, int j) { return uRec[((size_t)v * M + j) * N + i]; };
     auto dx = [&](int v, int j) { return x[((size_t)v * M + j) * N + i]; };
+    real BCC[M][NV];
+    boundaryFaceBCC<DIM, NDOF, NV, MODE == 2>(m, f, mom, cen, isc, um, rec, dx, BCC);
+#pragma unroll
+    for (int v = 0; v < NV; v++)
+#pragma unroll
+        for (int r = 0; r < M; r++)
+        {
+            real acc = 0;
+#pragma unroll
+            for (int c2 = 0; c2 < M; c2++)
+                acc += m.AinvB[((size_t)r * M + c2) * N + i] * BCC[c2][v];
+            out[((size_t)v * M + r) * NB + t] = acc;
+        }
+}
+
+template <int M, int NV>
+__global__ void __launch_bounds__(64) kVRBndCell(DevMesh m, const real *__restrict__ bt, real *__restrict__ uRecNew,
+                                                 real scale)
+{
+    const idx j = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= m.nBndCell)
+        return;
+    const idx i = m.bndCell[j];
+    const size_t N = m.N, NB = m.nBnd, NC = m.nBndCell;
     for (int k = 0; k < m.K; k++)
     {
-        const idx f = m.c2f[(size_t)k * N + i];
-        if (f < 0 || m.c2c[(size_t)k * N + i] >= 0)
+        const int t = m.bndSlot[(size_t)k * NC + j];
+        if (t < 0)
             continue;
-        real BCC[M][NV];
-        boundaryFaceBCC<DIM, NDOF, NV, MODE == 2>(m, f, mom, cen, isc, um, rec, dx, BCC);
 #pragma unroll
         for (int v = 0; v < NV; v++)
 #pragma unroll
             for (int r = 0; r < M; r++)
-            {
-                real acc = 0;
-#pragma unroll
-                for (int c2 = 0; c2 < M; c2++)
-                    acc += m.AinvB[((size_t)r * M + c2) * N + i] * BCC[c2][v];
-                uRecNew[((size_t)v * M + r) * N + i] += scale * acc;
-            }
+                uRecNew[((size_t)v * M + r) * N + i] += scale * bt[((size_t)v * M + r) * NB + t];
     }
 }
 
@@ -647,11 +669,20 @@ __global__ void __launch_bounds__(64) kVR2nd(DevMesh m, const real *__restrict__
 template <int MODE, int DIM, int NDOF>
 static void vrBoundaryN(Ctx &c, int nVars, const real *uRec, const real *u, const real *x, real *uRecNew, real scale)
 {
-    const int g = gridFor(c.m.nBndCell, 64);
+    constexpr int M = NDOF - 1;
+    if (c.m.nBnd == 0)
+        return;
+    const int gf = gridFor(c.m.nBnd, 64), gc = gridFor(c.m.nBndCell, 64);
     if (nVars == 5)
-        kVRBoundary<DIM, NDOF, 5, MODE><<<g, 64, 0, c.stream>>>(c.m, uRec, u, x, uRecNew, scale);
+    {
+        kVRBndFace<DIM, NDOF, 5, MODE><<<gf, 64, 0, c.stream>>>(c.m, uRec, u, x, c.vrBnd);
+        kVRBndCell<M, 5><<<gc, 64, 0, c.stream>>>(c.m, c.vrBnd, uRecNew, scale);
+    }
     else
-        kVRBoundary<DIM, NDOF, 1, MODE><<<g, 64, 0, c.stream>>>(c.m, uRec, u, x, uRecNew, scale);
+    {
+        kVRBndFace<DIM, NDOF, 1, MODE><<<gf, 64, 0, c.stream>>>(c.m, uRec, u, x, c.vrBnd);
+        kVRBndCell<M, 1><<<gc, 64, 0, c.stream>>>(c.m, c.vrBnd, uRecNew, scale);
+    }
     DNDSR_HIP_CHECK(hipGetLastError());
 }
 static void requireBoundaryValues(const DevMesh &m, int nVars)
